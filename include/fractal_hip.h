@@ -1,0 +1,243 @@
+/*
+ * fractal_hip.h -- C ABI of the MI355X (gfx950) deep-zoom field backend.
+ *
+ * The library built from fractal-audio-viz_amd/csrc (libfractal_hip.so)
+ * exports the hot-path subset of the reference's renderer.h ABI, version 10,
+ * with identical names, argument meaning, struct layouts, return codes and
+ * thread-local error text.  Every render runs on the HIP device; the
+ * reference's CPU backends (0 = scalar, 1 = AVX2) are not part of this
+ * library and are rejected with an error instead of being emulated.
+ *
+ * Struct layouts below must stay byte-identical to renderer.h: they are the
+ * wire format that visualizer.py's ctypes mirrors fill in
+ * (visualizer.py:824-944, 986-1112, 1117-1215, 1237-1256).
+ *
+ * Conventions shared by every entry point (renderer.cpp:4596-4599,
+ * 15922-15924, 19961-19969):
+ *   - int results: 0 = success, 1 = failure; creators return NULL on failure;
+ *   - the failure message is read with fractal_last_error() on the same
+ *     thread;
+ *   - the caller owns every buffer: float32 fields are row-major W*H, planes
+ *     are W*H int64/double arrays, RGB is interleaved W*H*3 bytes;
+ *   - reference handles are opaque increasing tokens (never pointers).
+ */
+#ifndef FRACTAL_HIP_H
+#define FRACTAL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FRACTAL_ABI_VERSION 10
+#define FRACTAL_RENDER_OPTIONS_VERSION 2
+#define FRACTAL_KFP_OPTIONS_VERSION 3
+#define FRACTAL_KFP_PLANES_VERSION 1
+#define FRACTAL_RENDER_PLANES_VERSION 1
+#define FRACTAL_KFP_MAX_MULTI_COLORS 256
+
+#define FRACTAL_FORMULA_MANDELBROT 0
+#define FRACTAL_FORMULA_JULIA 1
+#define FRACTAL_FORMULA_BURNING_SHIP 2
+#define FRACTAL_FORMULA_TRICORN 3
+
+#define FRACTAL_COORDINATE_MODE_PROJECT 0
+#define FRACTAL_COORDINATE_MODE_KALLES 1
+
+#define FRACTAL_RENDER_HINT_SKIP_PHASE (1u << 0)
+#define FRACTAL_RENDER_HINT_SKIP_ANALYTIC_DE (1u << 1)
+
+/* Same layout as renderer.h:43-67. */
+typedef struct FractalRenderOptions {
+    uint32_t struct_size;
+    uint32_t version;
+    int32_t strict;
+    int32_t allow_recovery;
+    int32_t time_budget_ms;
+    int32_t disable_bla;
+    int32_t disable_cycle;
+    int32_t strict_cycle;
+    int32_t series_min_terms;
+    int32_t series_max_terms;
+    int32_t max_bla_length;
+    int32_t max_linear_bla_length;
+    int32_t backend;
+    int32_t escape_radius_mode;
+    int32_t coordinate_mode;
+    int32_t reserved[1];
+    double output_bias;
+} FractalRenderOptions;
+
+/* Same layout as renderer.h:75-119. */
+typedef struct FractalKfpOptions {
+    uint32_t struct_size;
+    uint32_t version;
+    double iter_div;
+    double color_offset;
+    double ratio;
+    int32_t color_method;
+    int32_t smooth_method;
+    int32_t smooth;
+    int32_t flat;
+    int32_t inverse_transition;
+    double phase_color_strength;
+    int32_t multi_color;
+    int32_t blend_multi_color;
+    uint32_t multi_color_count;
+    double multi_color_period[FRACTAL_KFP_MAX_MULTI_COLORS];
+    int32_t multi_color_start[FRACTAL_KFP_MAX_MULTI_COLORS];
+    int32_t multi_color_type[FRACTAL_KFP_MAX_MULTI_COLORS];
+    double power;
+    int32_t slopes;
+    double slope_power;
+    double slope_ratio;
+    double slope_angle;
+    int32_t differences;
+    int32_t interior_color[3];
+    double field_bias;
+    int32_t bailout_radius_preset;
+    double bailout_radius_custom;
+    int32_t bailout_norm_preset;
+    double bailout_norm_custom;
+    int32_t texture_enabled;
+    double texture_merge;
+    double texture_power;
+    double texture_ratio;
+    int32_t texture_resize;
+    int32_t use_opengl;
+    int32_t use_srgb;
+    int32_t show_glitches;
+} FractalKfpOptions;
+
+/* Same layout as renderer.h:135-153. */
+typedef struct FractalKfpPlanes {
+    uint32_t struct_size;
+    uint32_t version;
+    const int64_t *orbit_iteration;
+    const double *bailout;
+    const int64_t *iteration;
+    const double *transition;
+    const double *phase;
+    const double *de_x;
+    const double *de_y;
+    const double *test1;
+    const double *test2;
+    const uint8_t *texture_rgb;
+    int32_t texture_width;
+    int32_t texture_height;
+    int32_t texture_stride;
+} FractalKfpPlanes;
+
+/* Same layout as renderer.h:161-170. */
+typedef struct FractalRenderPlanes {
+    uint32_t struct_size;
+    uint32_t version;
+    int64_t *orbit_iteration;
+    double *phase;
+    double *de_x;
+    double *de_y;
+    double *test1;
+    double *test2;
+} FractalRenderPlanes;
+
+/* ---- Library identity and diagnostics --------------------------------- */
+
+/* renderer.h:172 / renderer.cpp:15872.  Returns 10. */
+int fractal_abi_version(void);
+/* renderer.h:173 / renderer.cpp:15886.  Returns 2. */
+int fractal_render_options_version(void);
+/* renderer.h:174 / renderer.cpp:15888-15890 (defaults: renderer.cpp:4708-4727),
+ * except that backend defaults to 2, the only backend this library runs. */
+void fractal_render_options_default(FractalRenderOptions *options);
+/* renderer.h:177 / renderer.cpp:15892-15920.  Bit 4 (double-precision GPU
+ * compute) and bit 8 (physical GPU) when a gfx950 device is visible; 0
+ * otherwise.  The CPU bits 1/2 are never set by this library. */
+int fractal_backend_capabilities(void);
+/* renderer.h:178 / renderer.cpp:15922-15924. */
+const char *fractal_last_error(void);
+
+/* renderer.h:180-182 / renderer.cpp:15874-15884.  Counter order follows
+ * RenderStats (renderer.cpp:4615-4706): 16 basic or 39 extended fields. */
+void fractal_set_stats_enabled(int enabled);
+int fractal_get_last_stats(uint64_t *values, int capacity);
+int fractal_get_last_stats_ex(uint64_t *values, int capacity);
+
+/* ---- Reference orbits (host MPFR + BLA build, uploaded to HBM) -------- */
+
+/* renderer.h:184-191 / renderer.cpp:19568-19594. */
+void *fractal_create_reference(
+    const char *x_center, const char *y_center, const char *viewport_zoom,
+    int max_iter, int precision_bits, int series_order);
+/* renderer.h:193-200 / renderer.cpp:19596-19622. */
+void *fractal_create_reference_reusable(
+    const char *x_center, const char *y_center, const char *viewport_zoom,
+    int max_iter, int precision_bits, int series_order);
+/* renderer.h:203-211 / renderer.cpp:19624-19653. */
+void *fractal_create_reference_reusable_options(
+    const char *x_center, const char *y_center, const char *viewport_zoom,
+    int max_iter, int precision_bits, int series_order,
+    const FractalRenderOptions *options);
+/* renderer.h:215-225 / renderer.cpp:19655-19685.  Mandelbrot only in this
+ * library; other formula ids fail with an error. */
+void *fractal_create_reference_ex(
+    const char *x_center, const char *y_center, const char *viewport_zoom,
+    int max_iter, int precision_bits, int series_order, int formula,
+    const char *julia_real, const char *julia_imag);
+/* renderer.h:226-237 / renderer.cpp:19687-19722. */
+void *fractal_create_reference_ex_options(
+    const char *x_center, const char *y_center, const char *viewport_zoom,
+    int max_iter, int precision_bits, int series_order, int formula,
+    const char *julia_real, const char *julia_imag,
+    const FractalRenderOptions *options);
+/* renderer.h:239 / renderer.cpp:19724-19750 (retargeted BLA radii,
+ * renderer.cpp:11776-11868). */
+void *fractal_clone_reference(void *source_handle, const char *viewport_zoom);
+/* renderer.h:240 / renderer.cpp:19752-19772.  Idempotent on NULL; a stale
+ * handle sets an error. */
+void fractal_destroy_reference(void *handle);
+/* renderer.h:241 / renderer.cpp:19779-19812: build ns (orbit, series, BLA),
+ * series iteration, series order. */
+int fractal_get_reference_stats(void *handle, uint64_t *values, int capacity);
+
+/* ---- Field renders (HIP) ---------------------------------------------- */
+
+/* renderer.h:243-254. */
+int fractal_render_mandelbrot_reference_ex(
+    float *output, int width, int height, const char *zoom_text, void *handle,
+    int max_iter, int threads, int series_order, int series_block,
+    const FractalRenderOptions *options);
+/* renderer.h:257-268. */
+int fractal_render_reference_ex(
+    float *output, int width, int height, const char *zoom_text, void *handle,
+    int max_iter, int threads, int series_order, int series_block,
+    const FractalRenderOptions *options);
+/* renderer.h:273-285 / renderer.cpp:19814-19970. */
+int fractal_render_reference_ex_planes(
+    float *output, int width, int height, const char *zoom_text, void *handle,
+    int max_iter, int threads, int series_order, int series_block,
+    const FractalRenderOptions *options, FractalRenderPlanes *planes);
+/* renderer.h:376-386 / renderer.cpp:20250-20273. */
+int render_mandelbrot_reference(
+    float *output, int width, int height, const char *zoom_text, void *handle,
+    int max_iter, int threads, int series_order, int series_block);
+/* renderer.h:338-353 / renderer.cpp:20279-20358.  use_perturbation=1 only. */
+int render_fractal_ex(
+    float *output, int width, int height, const char *zoom_text,
+    const char *x_center, const char *y_center, int max_iter,
+    int precision_bits, int use_perturbation, int threads, int formula,
+    double julia_real, double julia_imag,
+    const FractalRenderOptions *options);
+/* renderer.h:357-373 / renderer.cpp:20360-20474. */
+int render_fractal_ex_planes(
+    float *output, int width, int height, const char *zoom_text,
+    const char *x_center, const char *y_center, int max_iter,
+    int precision_bits, int use_perturbation, int threads, int formula,
+    double julia_real, double julia_imag,
+    const FractalRenderOptions *options, FractalRenderPlanes *planes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FRACTAL_HIP_H */

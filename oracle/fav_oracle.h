@@ -1,0 +1,90 @@
+/*
+ * fav_oracle.h -- CPU oracle for the deep-zoom field path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg compare the HIP
+ * product against.  The product (libfractal_hip.so) never links, loads or
+ * calls it.
+ *
+ * It is a plain-C restatement of the reference's scalar FP64 path in
+ * renderer.cpp (citations on each function in fav_oracle.c), compiled with
+ * -ffp-contract=off so every product and sum is rounded separately, exactly
+ * as the reference build used for the golden fixtures.
+ *
+ * Parity of this restatement is pinned by tests/test_oracle_*.py:
+ *   - bit-exact against golden vectors written by the reference's own
+ *     renderer.cpp compiled from source (oracle/_ref, tests/golden/);
+ *   - against closed forms and brute force (direct MPFR iteration of single
+ *     pixels, the analytic viewport mapping, the radius-2 escape set).
+ */
+#ifndef FAV_ORACLE_H
+#define FAV_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/fractal_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fo_reference fo_reference;
+
+/* Number of RenderStats counters (renderer.cpp:4615-4643, 4651). */
+#define FO_STATS_FIELDS 39
+
+fo_reference *fo_reference_create(
+    const char *x_center, const char *y_center, const char *viewport_zoom,
+    int max_iter, int precision_bits, int series_order,
+    int escape_radius_mode, int coordinate_mode,
+    char *error, int error_capacity);
+
+fo_reference *fo_reference_clone(
+    const fo_reference *source, const char *viewport_zoom,
+    char *error, int error_capacity);
+
+void fo_reference_destroy(fo_reference *reference);
+
+/* values[0..] = orbit length, map_end, series enabled, series iteration,
+ * series order, cubic level count, linear level count, deep linear level
+ * count, cubic level-0 size.  Returns the number written. */
+int fo_reference_info(const fo_reference *reference, int64_t *values, int capacity);
+
+/* Copies the compact orbit (mantissa pairs and shared exponents). */
+int fo_reference_orbit(const fo_reference *reference, double *real, double *imag,
+                       int32_t *exponent, int capacity);
+
+/* Deep Mandelbrot field render for log10(zoom) >= 6 (renderer.cpp:12672-13583).
+ * stats may be NULL or hold FO_STATS_FIELDS counters.  threads <= 0 uses
+ * the OpenMP default.  Returns 0 on success, 1 on failure. */
+int fo_render(
+    float *output, int width, int height, const char *zoom_text,
+    const fo_reference *reference, int max_iter, int series_order,
+    int series_block, const FractalRenderOptions *options,
+    FractalRenderPlanes *planes, uint64_t *stats, int threads,
+    char *error, int error_capacity);
+
+/* Shallow direct render (renderer.cpp:10054-10136, 10138-10278) for the
+ * Mandelbrot formula with double coordinates. */
+int fo_render_direct(
+    float *output, int width, int height, double zoom, double x_center,
+    double y_center, int max_iter, double output_bias, int escape_radius_mode,
+    int coordinate_mode, FractalRenderPlanes *planes, int threads);
+
+/* KFP SetColor transfer of a scalar field (renderer.cpp:16314-16545). */
+int fo_colourise_kfp(
+    const float *field, uint8_t *output, int width, int height, int max_iter,
+    const FractalKfpOptions *options, const FractalKfpPlanes *planes,
+    const uint8_t *lut, int lut_size, int threads,
+    char *error, int error_capacity);
+
+/* 1024-entry cyclic sine-interpolated palette (visualizer.py:13492-13521). */
+int fo_kfp_palette_lut(const uint8_t *stops, int stop_count, uint8_t *lut,
+                       int lut_size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,295 @@
+"""ctypes mirrors of the renderer.h ABI and loaders for the three libraries.
+
+The struct mirrors follow renderer.h:43-170 (the same layout visualizer.py
+declares at visualizer.py:824-944 and 986-1112).  Three shared objects speak
+this ABI (or a subset):
+
+* the HIP product, ``fractal-audio-viz_amd/libfractal_hip.so``;
+* the C oracle, ``oracle/_build/libfav_oracle.so`` (test checker only);
+* the reference's own renderer.cpp compiled from source into
+  ``oracle/_ref/mandelbrot.so`` when the reference tree was available.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+PRODUCT_SO = ROOT / "fractal-audio-viz_amd" / "libfractal_hip.so"
+ORACLE_SO = ROOT / "oracle" / "_build" / "libfav_oracle.so"
+REFERENCE_SO = ROOT / "oracle" / "_ref" / "mandelbrot.so"
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+c_uint64_p = ctypes.POINTER(ctypes.c_uint64)
+c_uint8_p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class RenderOptions(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("version", ctypes.c_uint32),
+        ("strict", ctypes.c_int32),
+        ("allow_recovery", ctypes.c_int32),
+        ("time_budget_ms", ctypes.c_int32),
+        ("disable_bla", ctypes.c_int32),
+        ("disable_cycle", ctypes.c_int32),
+        ("strict_cycle", ctypes.c_int32),
+        ("series_min_terms", ctypes.c_int32),
+        ("series_max_terms", ctypes.c_int32),
+        ("max_bla_length", ctypes.c_int32),
+        ("max_linear_bla_length", ctypes.c_int32),
+        ("backend", ctypes.c_int32),
+        ("escape_radius_mode", ctypes.c_int32),
+        ("coordinate_mode", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
+        ("output_bias", ctypes.c_double),
+    ]
+
+    @classmethod
+    def defaults(cls, **overrides) -> "RenderOptions":
+        """renderer.cpp:4708-4727 defaults, then keyword overrides."""
+        o = cls()
+        o.struct_size = ctypes.sizeof(cls)
+        o.version = 2
+        o.strict = 1
+        o.series_min_terms = 8
+        o.series_max_terms = 32
+        o.max_bla_length = 64
+        o.max_linear_bla_length = 4096
+        for key, value in overrides.items():
+            setattr(o, key, value)
+        return o
+
+
+class RenderPlanes(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("version", ctypes.c_uint32),
+        ("orbit_iteration", c_int64_p),
+        ("phase", c_double_p),
+        ("de_x", c_double_p),
+        ("de_y", c_double_p),
+        ("test1", c_double_p),
+        ("test2", c_double_p),
+    ]
+
+
+class PlaneArrays:
+    """Owns numpy storage for a FractalRenderPlanes struct."""
+
+    NAMES = ("orbit_iteration", "phase", "de_x", "de_y", "test1", "test2")
+
+    def __init__(self, width: int, height: int):
+        n = width * height
+        self.orbit_iteration = np.full(n, -7, dtype=np.int64)
+        self.phase = np.full(n, -7.0)
+        self.de_x = np.full(n, -7.0)
+        self.de_y = np.full(n, -7.0)
+        self.test1 = np.full(n, -7.0)
+        self.test2 = np.full(n, -7.0)
+        self.struct = RenderPlanes()
+        self.struct.struct_size = ctypes.sizeof(RenderPlanes)
+        self.struct.version = 1
+        self.struct.orbit_iteration = self.orbit_iteration.ctypes.data_as(c_int64_p)
+        for name in self.NAMES[1:]:
+            setattr(self.struct, name, getattr(self, name).ctypes.data_as(c_double_p))
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name in self.NAMES}
+
+
+def _bind_render_abi(lib: ctypes.CDLL) -> ctypes.CDLL:
+    """Argument types for the renderer.h entry points (renderer.h:172-373)."""
+    lib.fractal_abi_version.restype = ctypes.c_int
+    lib.fractal_last_error.restype = ctypes.c_char_p
+    lib.fractal_backend_capabilities.restype = ctypes.c_int
+    lib.fractal_render_options_default.argtypes = [ctypes.POINTER(RenderOptions)]
+    lib.fractal_set_stats_enabled.argtypes = [ctypes.c_int]
+    for name in ("fractal_get_last_stats", "fractal_get_last_stats_ex"):
+        fn = getattr(lib, name)
+        fn.argtypes = [c_uint64_p, ctypes.c_int]
+        fn.restype = ctypes.c_int
+    lib.fractal_create_reference_ex_options.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
+        ctypes.POINTER(RenderOptions)]
+    lib.fractal_create_reference_ex_options.restype = ctypes.c_void_p
+    for name in ("fractal_create_reference", "fractal_create_reference_reusable"):
+        fn = getattr(lib, name)
+        fn.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                       ctypes.c_int, ctypes.c_int]
+        fn.restype = ctypes.c_void_p
+    lib.fractal_create_reference_reusable_options.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.POINTER(RenderOptions)]
+    lib.fractal_create_reference_reusable_options.restype = ctypes.c_void_p
+    lib.fractal_create_reference_ex.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p]
+    lib.fractal_create_reference_ex.restype = ctypes.c_void_p
+    lib.fractal_clone_reference.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    lib.fractal_clone_reference.restype = ctypes.c_void_p
+    lib.fractal_destroy_reference.argtypes = [ctypes.c_void_p]
+    lib.fractal_destroy_reference.restype = None
+    lib.fractal_get_reference_stats.argtypes = [ctypes.c_void_p, c_uint64_p, ctypes.c_int]
+    lib.fractal_get_reference_stats.restype = ctypes.c_int
+    lib.fractal_render_reference_ex_planes.argtypes = [
+        c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.POINTER(RenderOptions), ctypes.POINTER(RenderPlanes)]
+    lib.fractal_render_reference_ex_planes.restype = ctypes.c_int
+    for name in ("fractal_render_reference_ex", "fractal_render_mandelbrot_reference_ex"):
+        fn = getattr(lib, name)
+        fn.argtypes = [c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p,
+                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                       ctypes.POINTER(RenderOptions)]
+        fn.restype = ctypes.c_int
+    lib.render_fractal_ex.argtypes = [
+        c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
+        ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_double, ctypes.c_double, ctypes.POINTER(RenderOptions)]
+    lib.render_fractal_ex.restype = ctypes.c_int
+    return lib
+
+
+def load_reference():
+    """The reference renderer.cpp built from source, or None if absent."""
+    if not REFERENCE_SO.exists():
+        return None
+    return _bind_render_abi(ctypes.CDLL(str(REFERENCE_SO)))
+
+
+def load_oracle():
+    lib = ctypes.CDLL(str(ORACLE_SO))
+    lib.fo_reference_create.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.fo_reference_create.restype = ctypes.c_void_p
+    lib.fo_reference_clone.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+    lib.fo_reference_clone.restype = ctypes.c_void_p
+    lib.fo_reference_destroy.argtypes = [ctypes.c_void_p]
+    lib.fo_reference_info.argtypes = [ctypes.c_void_p, c_int64_p, ctypes.c_int]
+    lib.fo_render.argtypes = [
+        c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int, ctypes.POINTER(RenderOptions), ctypes.POINTER(RenderPlanes),
+        c_uint64_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.fo_render.restype = ctypes.c_int
+    return lib
+
+
+def load_product():
+    """The HIP library; raises if it has not been built."""
+    if not PRODUCT_SO.exists():
+        raise FileNotFoundError(f"{PRODUCT_SO} is not built; run __graft_entry__.build()")
+    return _bind_render_abi(ctypes.CDLL(str(PRODUCT_SO)))
+
+
+def _b(text) -> bytes:
+    return text.encode() if isinstance(text, str) else text
+
+
+class OracleReference:
+    """A fo_reference owned by Python."""
+
+    def __init__(self, lib, x, y, zoom, max_iter, precision_bits, series_order=8,
+                 escape_radius_mode=0, coordinate_mode=0, handle=None):
+        self.lib = lib
+        if handle is None:
+            err = ctypes.create_string_buffer(512)
+            handle = lib.fo_reference_create(_b(x), _b(y), _b(zoom), max_iter, precision_bits,
+                                              series_order, escape_radius_mode, coordinate_mode,
+                                              err, 512)
+            if not handle:
+                raise RuntimeError(err.value.decode())
+        self.handle = handle
+
+    def clone(self, zoom):
+        err = ctypes.create_string_buffer(512)
+        handle = self.lib.fo_reference_clone(self.handle, _b(zoom), err, 512)
+        if not handle:
+            raise RuntimeError(err.value.decode())
+        return OracleReference(self.lib, None, None, None, 0, 0, handle=handle)
+
+    def info(self):
+        values = np.zeros(9, dtype=np.int64)
+        self.lib.fo_reference_info(self.handle, values.ctypes.data_as(c_int64_p), 9)
+        keys = ("orbit_size", "map_end", "series_enabled", "series_iteration", "series_order",
+                "cubic_levels", "linear_levels", "deep_linear_levels", "cubic_base")
+        return dict(zip(keys, values.tolist()))
+
+    def render(self, width, height, zoom, max_iter, series_order=8, series_block=32,
+               options=None, planes=False, stats=False, threads=0):
+        out = np.empty(width * height, dtype=np.float32)
+        opts = options if options is not None else RenderOptions.defaults()
+        plane_arrays = PlaneArrays(width, height) if planes else None
+        stat_values = np.zeros(39, dtype=np.uint64) if stats else None
+        err = ctypes.create_string_buffer(512)
+        status = self.lib.fo_render(
+            out.ctypes.data_as(c_float_p), width, height, _b(zoom), self.handle, max_iter,
+            series_order, series_block, ctypes.byref(opts),
+            ctypes.byref(plane_arrays.struct) if plane_arrays else None,
+            stat_values.ctypes.data_as(c_uint64_p) if stats else None, threads, err, 512)
+        if status != 0:
+            raise RuntimeError(err.value.decode())
+        return out.reshape(height, width), plane_arrays, stat_values
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.lib.fo_reference_destroy(self.handle)
+            self.handle = None
+
+
+class AbiReference:
+    """A reference handle created through a renderer.h-speaking library."""
+
+    def __init__(self, lib, x, y, zoom, max_iter, precision_bits, series_order=8,
+                 options=None, formula=0):
+        self.lib = lib
+        opts = options if options is not None else RenderOptions.defaults()
+        handle = lib.fractal_create_reference_ex_options(
+            _b(x), _b(y), _b(zoom), max_iter, precision_bits, series_order, formula,
+            b"0", b"0", ctypes.byref(opts))
+        if not handle:
+            raise RuntimeError(lib.fractal_last_error().decode())
+        self.handle = handle
+
+    def render(self, width, height, zoom, max_iter, series_order=8, series_block=32,
+               options=None, planes=False, stats=False, threads=0):
+        out = np.empty(width * height, dtype=np.float32)
+        opts = options if options is not None else RenderOptions.defaults()
+        plane_arrays = PlaneArrays(width, height) if planes else None
+        if stats:
+            self.lib.fractal_set_stats_enabled(1)
+        status = self.lib.fractal_render_reference_ex_planes(
+            out.ctypes.data_as(c_float_p), width, height, _b(zoom), ctypes.c_void_p(self.handle),
+            max_iter, threads, series_order, series_block, ctypes.byref(opts),
+            ctypes.byref(plane_arrays.struct) if plane_arrays else None)
+        stat_values = None
+        if stats:
+            stat_values = np.zeros(39, dtype=np.uint64)
+            self.lib.fractal_get_last_stats_ex(stat_values.ctypes.data_as(c_uint64_p), 39)
+            self.lib.fractal_set_stats_enabled(0)
+        if status != 0:
+            raise RuntimeError(self.lib.fractal_last_error().decode())
+        return out.reshape(height, width), plane_arrays, stat_values
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.lib.fractal_destroy_reference(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+
+def escape_counts(field: np.ndarray, planes: PlaneArrays, max_iter: int) -> np.ndarray:
+    """n_p = orbit_iteration + 1 for escaped pixels, max_iter for interior
+    and unresolved (NaN) pixels; orbit_iteration is Kalles' antal = n - 1
+    (renderer.cpp:8098-8106)."""
+    counts = planes.orbit_iteration.astype(np.int64) + 1
+    interior = planes.orbit_iteration >= max_iter
+    counts[interior] = max_iter
+    counts[np.isnan(field.reshape(-1))] = max_iter
+    return counts.reshape(field.shape)
