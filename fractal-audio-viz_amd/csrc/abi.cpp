@@ -1,0 +1,468 @@
+// abi.cpp -- extern "C" entry points of libfractal_hip.so.
+//
+// Names, argument meaning, validation order, return codes and the
+// thread-local error text follow the reference's C ABI (renderer.h,
+// renderer.cpp:15870-15924, 19568-20474).  No exception crosses the ABI.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <locale>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+
+#include "../../include/fractal_hip.h"
+#include "fav_host.h"
+#include "fav_render.h"
+
+namespace {
+
+using fav::ReferenceContext;
+
+constexpr int kMaxIterations = 10'000'000;   // renderer.cpp:88-93
+constexpr int kMaxThreads = 4096;
+constexpr int kMaxPrecisionBits = 131'072;
+constexpr int kMaxPixels = 100'000'000;
+constexpr size_t kMaxTextLength = 50'000;
+
+thread_local std::string last_error;  // renderer.cpp:4596-4599
+
+void set_error(const std::string& message) { last_error = message; }
+
+bool valid_text(const char* text) {  // renderer.cpp:207-213
+    if (!text) return false;
+    for (size_t n = 0; n <= kMaxTextLength; ++n)
+        if (text[n] == '\0') return true;
+    return false;
+}
+
+bool valid_dimensions(int w, int h) {  // renderer.cpp:170-175
+    if (w <= 0 || h <= 0) return false;
+    return static_cast<uint64_t>(w) <= static_cast<uint64_t>(kMaxPixels) / static_cast<uint64_t>(h);
+}
+bool valid_iterations(int n) { return n > 0 && n <= kMaxIterations; }
+bool valid_threads(int n) { return n >= 0 && n <= kMaxThreads; }
+bool valid_precision(int bits) { return bits >= 128 && bits <= kMaxPrecisionBits; }
+bool valid_series(int order, int block) { return order >= 1 && order <= 32 && block >= 2 && block <= 4096; }
+
+void parse_coordinate(const char* text, const char* label) {  // renderer.cpp:214-241
+    if (!valid_text(text)) throw std::runtime_error("native coordinate text is too long or null");
+    std::istringstream stream(text);
+    stream.imbue(std::locale::classic());
+    long double value = 0.0L;
+    stream >> value;
+    bool ok = !stream.fail();
+    if (ok) {
+        stream >> std::ws;
+        ok = stream.eof();
+    }
+    if (!ok || !std::isfinite(value)) throw std::runtime_error(std::string("invalid ") + label + " coordinate");
+}
+
+FractalRenderOptions default_options() {  // renderer.cpp:4708-4727
+    FractalRenderOptions o{};
+    o.struct_size = sizeof(FractalRenderOptions);
+    o.version = FRACTAL_RENDER_OPTIONS_VERSION;
+    o.strict = 1;
+    o.series_min_terms = 8;
+    o.series_max_terms = 32;
+    o.max_bla_length = 64;
+    o.max_linear_bla_length = 4096;
+    o.backend = 2;
+    return o;
+}
+
+FractalRenderOptions checked_options(const FractalRenderOptions* supplied) {  // renderer.cpp:4729-4765
+    FractalRenderOptions o = default_options();
+    if (!supplied) return o;
+    if (supplied->struct_size < sizeof(FractalRenderOptions) || supplied->version != FRACTAL_RENDER_OPTIONS_VERSION)
+        throw std::runtime_error("unsupported FractalRenderOptions version or size");
+    o = *supplied;
+    auto flag = [](int32_t v, const char* label) {
+        if (v != 0 && v != 1) throw std::runtime_error(std::string(label) + " must be 0 or 1");
+    };
+    flag(o.strict, "strict render flag");
+    flag(o.allow_recovery, "recovery flag");
+    flag(o.disable_bla, "disable-BLA flag");
+    flag(o.disable_cycle, "disable-cycle flag");
+    flag(o.strict_cycle, "strict-cycle flag");
+    if (o.escape_radius_mode != 0 && o.escape_radius_mode != 1) throw std::runtime_error("unknown escape-radius mode");
+    if (o.coordinate_mode != 0 && o.coordinate_mode != 1) throw std::runtime_error("unknown coordinate mode");
+    o.series_min_terms = std::clamp(o.series_min_terms, 1, 32);
+    o.series_max_terms = std::clamp(o.series_max_terms, o.series_min_terms, 32);
+    o.max_bla_length = std::clamp(o.max_bla_length, 1, 64);
+    o.max_linear_bla_length = std::clamp(o.max_linear_bla_length, 1, 4096);
+    if (o.time_budget_ms < 0) throw std::runtime_error("render time budget cannot be negative");
+    if (!std::isfinite(o.output_bias) || o.output_bias < 0.0)
+        throw std::runtime_error("render output bias must be finite and non-negative");
+    return o;
+}
+
+void require_hip_backend(const FractalRenderOptions& o) {  // renderer.cpp:19835-19837
+    if (o.backend != 0 && o.backend != 1 && o.backend != 2) throw std::runtime_error("unknown native render backend");
+    if (o.backend != 2)
+        throw std::runtime_error(
+            "this library is the HIP backend (backend=2); the scalar/AVX2 CPU backends are not part of it");
+}
+
+bool valid_planes(const FractalRenderPlanes* p, int w, int h) {  // renderer.cpp:7493-7507
+    if (!p || !valid_dimensions(w, h)) return false;
+    return p->struct_size >= sizeof(FractalRenderPlanes) && p->version == FRACTAL_RENDER_PLANES_VERSION
+        && p->orbit_iteration && p->phase && p->de_x && p->de_y && p->test1 && p->test2;
+}
+
+// ---- opaque handle registry, renderer.cpp:9791-9836 -----------------------
+std::mutex registry_mutex;
+std::unordered_map<uintptr_t, std::shared_ptr<ReferenceContext>> registry;
+uintptr_t next_handle = 0x1000U;
+
+void* register_reference(std::unique_ptr<ReferenceContext> context) {
+    auto shared = std::shared_ptr<ReferenceContext>(std::move(context));
+    std::lock_guard<std::mutex> lock(registry_mutex);
+    for (;;) {
+        const uintptr_t token = next_handle++;
+        if (token == 0 || registry.count(token)) continue;
+        registry.emplace(token, std::move(shared));
+        return reinterpret_cast<void*>(token);
+    }
+}
+
+std::shared_ptr<ReferenceContext> acquire(void* handle) {
+    if (!handle) return {};
+    std::lock_guard<std::mutex> lock(registry_mutex);
+    auto it = registry.find(reinterpret_cast<uintptr_t>(handle));
+    return it == registry.end() ? nullptr : it->second;
+}
+
+std::shared_ptr<ReferenceContext> remove(void* handle) {
+    if (!handle) return {};
+    std::lock_guard<std::mutex> lock(registry_mutex);
+    auto it = registry.find(reinterpret_cast<uintptr_t>(handle));
+    if (it == registry.end()) return {};
+    auto context = std::move(it->second);
+    registry.erase(it);
+    return context;
+}
+
+// ---- opt-in statistics, renderer.cpp:4645-4706 ----------------------------
+std::atomic<bool> stats_enabled{false};
+std::mutex stats_mutex;
+uint64_t last_stats[fav::kStatsFields] = {};
+
+void publish_stats(const uint64_t* values) {
+    std::lock_guard<std::mutex> lock(stats_mutex);
+    std::memcpy(last_stats, values, sizeof(last_stats));
+}
+
+int copy_stats(uint64_t* values, int capacity, int fields) {
+    if (!values || capacity < fields) return -1;
+    std::lock_guard<std::mutex> lock(stats_mutex);
+    std::memcpy(values, last_stats, sizeof(uint64_t) * static_cast<size_t>(fields));
+    return fields;
+}
+
+void* create(const char* x, const char* y, const char* zoom, int max_iter, int bits, int series_order,
+             bool reusable, int formula, const char* julia_re, const char* julia_im,
+             const FractalRenderOptions* supplied, bool use_options, const char* what) {
+    try {
+        if (!x || !y || !zoom || !julia_re || !julia_im || !valid_iterations(max_iter) || !valid_precision(bits)
+            || !valid_series(series_order, 2) || formula < 0 || formula > 3)
+            throw std::runtime_error(what);
+        const FractalRenderOptions o = use_options ? checked_options(supplied) : default_options();
+        if (!valid_text(x) || !valid_text(y) || !valid_text(zoom))
+            throw std::runtime_error("native reference text is too long or null");
+        parse_coordinate(x, "real");
+        parse_coordinate(y, "imaginary");
+        auto context = fav::build_reference(x, y, zoom, max_iter, bits, series_order, reusable, formula,
+                                            use_options ? o.escape_radius_mode : 0,
+                                            use_options ? o.coordinate_mode : 0);
+        void* handle = register_reference(std::move(context));
+        set_error("");
+        return handle;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return nullptr;
+    } catch (...) {
+        set_error("native reference creation failed with an unknown exception");
+        return nullptr;
+    }
+}
+
+int render_impl(float* output, int width, int height, const char* zoom, void* handle, int max_iter,
+                int threads, int series_order, int series_block, const FractalRenderOptions* supplied,
+                FractalRenderPlanes* planes, bool device_pointers) {
+    try {
+        if (!output || !zoom || !handle || !valid_dimensions(width, height) || !valid_iterations(max_iter)
+            || !valid_threads(threads) || !valid_series(series_order, series_block)
+            || (planes != nullptr && !valid_planes(planes, width, height)))
+            throw std::runtime_error("invalid native render dimensions or handle");
+        const FractalRenderOptions o = checked_options(supplied);
+        require_hip_backend(o);
+        const auto context = acquire(handle);
+        if (!context) throw std::runtime_error("invalid or already-destroyed reference handle");
+        if (context->escape_radius_mode != o.escape_radius_mode)
+            throw std::runtime_error("reference escape-radius mode does not match render options");
+        if (max_iter > context->requested_max_iter)
+            throw std::runtime_error("render iteration count exceeds prepared reference");
+        if (!valid_text(zoom)) throw std::runtime_error("native zoom text is too long or null");
+        const fav::FE z = fav::parse_zoom(zoom, context->precision_bits);
+        const long double log10_zoom =
+            (std::log(static_cast<long double>(z.m)) + static_cast<long double>(z.e) * 0.693147180559945309417232121458176568L)
+            / std::log(10.0L);
+        if (log10_zoom < 6.0L)
+            throw std::runtime_error(
+                "zoom below 1e6 is the reference's direct renderer, which the HIP backend does not provide");
+        uint64_t stats[fav::kStatsFields] = {};
+        const bool collect = stats_enabled.load(std::memory_order_relaxed);
+        fav::FieldRequest request{output, width, height, zoom, max_iter, series_order, series_block, o,
+                                  planes, collect ? stats : nullptr, device_pointers};
+        fav::render_field(*context, request);
+        if (collect) publish_stats(stats);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native reference render failed with an unknown exception");
+        return 1;
+    }
+}
+
+int one_shot(float* output, int width, int height, const char* zoom, const char* x, const char* y, int max_iter,
+             int bits, int use_perturbation, int threads, int formula, double julia_re, double julia_im,
+             const FractalRenderOptions* supplied, FractalRenderPlanes* planes) {
+    // render_fractal_ex[_planes], renderer.cpp:20279-20474
+    try {
+        if (!output || !zoom || !x || !y || !valid_dimensions(width, height)
+            || (planes && !valid_planes(planes, width, height)) || !valid_iterations(max_iter)
+            || !valid_precision(bits) || use_perturbation < 0 || use_perturbation > 1 || !valid_threads(threads)
+            || formula < 0 || formula > 3 || !std::isfinite(julia_re) || !std::isfinite(julia_im))
+            throw std::runtime_error("invalid native render dimensions, formula, or argument");
+        const FractalRenderOptions o = checked_options(supplied);
+        require_hip_backend(o);
+        if (!use_perturbation)
+            throw std::runtime_error("the direct (non-perturbation) renderer is not part of the HIP backend");
+        char jr[64], ji[64];
+        std::snprintf(jr, sizeof(jr), "%.17g", julia_re);
+        std::snprintf(ji, sizeof(ji), "%.17g", julia_im);
+        void* handle = fractal_create_reference_ex_options(x, y, zoom, max_iter, bits, 8, formula, jr, ji, &o);
+        if (!handle) throw std::runtime_error(last_error);
+        const int status = render_impl(output, width, height, zoom, handle, max_iter, threads, 8, 32, &o, planes, false);
+        const std::string render_error = last_error;
+        fractal_destroy_reference(handle);
+        if (status != 0) throw std::runtime_error(render_error);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native render failed with an unknown exception");
+        return 1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fractal_abi_version(void) { return FRACTAL_ABI_VERSION; }
+
+int fractal_render_options_version(void) { return FRACTAL_RENDER_OPTIONS_VERSION; }
+
+void fractal_render_options_default(FractalRenderOptions* options) {
+    if (options) *options = default_options();
+}
+
+int fractal_backend_capabilities(void) {
+    try {
+        const int caps = fav::device_available() ? (4 | 8) : 0;
+        set_error(caps ? "" : "no gfx950 HIP device is visible");
+        return caps;
+    } catch (...) {
+        set_error("HIP capability query failed");
+        return 0;
+    }
+}
+
+const char* fractal_last_error(void) { return last_error.c_str(); }
+
+void fractal_set_stats_enabled(int enabled) { stats_enabled.store(enabled != 0, std::memory_order_relaxed); }
+int fractal_get_last_stats(uint64_t* values, int capacity) { return copy_stats(values, capacity, 16); }
+int fractal_get_last_stats_ex(uint64_t* values, int capacity) { return copy_stats(values, capacity, fav::kStatsFields); }
+
+void* fractal_create_reference(const char* x, const char* y, const char* zoom, int max_iter, int bits,
+                               int series_order) {
+    return create(x, y, zoom, max_iter, bits, series_order, false, 0, "0", "0", nullptr, false,
+                  "invalid reference configuration");
+}
+
+void* fractal_create_reference_reusable(const char* x, const char* y, const char* zoom, int max_iter, int bits,
+                                        int series_order) {
+    return create(x, y, zoom, max_iter, bits, series_order, true, 0, "0", "0", nullptr, false,
+                  "invalid reusable reference configuration");
+}
+
+void* fractal_create_reference_reusable_options(const char* x, const char* y, const char* zoom, int max_iter,
+                                                int bits, int series_order, const FractalRenderOptions* options) {
+    return create(x, y, zoom, max_iter, bits, series_order, true, 0, "0", "0", options, true,
+                  "invalid reusable reference configuration");
+}
+
+void* fractal_create_reference_ex(const char* x, const char* y, const char* zoom, int max_iter, int bits,
+                                  int series_order, int formula, const char* julia_re, const char* julia_im) {
+    return create(x, y, zoom, max_iter, bits, series_order, formula != FRACTAL_FORMULA_MANDELBROT, formula,
+                  julia_re, julia_im, nullptr, false, "invalid formula-aware reference configuration");
+}
+
+void* fractal_create_reference_ex_options(const char* x, const char* y, const char* zoom, int max_iter, int bits,
+                                          int series_order, int formula, const char* julia_re,
+                                          const char* julia_im, const FractalRenderOptions* options) {
+    return create(x, y, zoom, max_iter, bits, series_order, formula != FRACTAL_FORMULA_MANDELBROT, formula,
+                  julia_re, julia_im, options, true, "invalid formula-aware reference configuration");
+}
+
+void* fractal_clone_reference(void* source_handle, const char* zoom) {
+    try {
+        if (!source_handle || !zoom) throw std::runtime_error("invalid reference tier clone configuration");
+        const auto source = acquire(source_handle);
+        if (!source) throw std::runtime_error("invalid or already-destroyed reference handle");
+        auto context = fav::clone_reference(*source, zoom);
+        void* handle = register_reference(std::move(context));
+        set_error("");
+        return handle;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return nullptr;
+    } catch (...) {
+        set_error("native reference clone failed with an unknown exception");
+        return nullptr;
+    }
+}
+
+void fractal_destroy_reference(void* handle) {
+    try {
+        if (!handle) {
+            set_error("");
+            return;
+        }
+        if (!remove(handle)) {
+            set_error("invalid or already-destroyed reference handle");
+            return;
+        }
+        set_error("");
+    } catch (...) {
+        set_error("native reference destruction failed with an unknown exception");
+    }
+}
+
+int fractal_get_reference_stats(void* handle, uint64_t* values, int capacity) {
+    try {
+        if (!values || capacity < 5) {
+            set_error("reference statistics buffer is too small or null");
+            return -1;
+        }
+        const auto c = acquire(handle);
+        if (!c) {
+            set_error("invalid or already-destroyed reference handle");
+            return -1;
+        }
+        values[0] = c->reference_build_ns;
+        values[1] = c->series_build_ns;
+        values[2] = c->bla_build_ns;
+        values[3] = c->series_enabled ? static_cast<uint64_t>(c->series_iteration) : 0;
+        values[4] = c->series_enabled ? static_cast<uint64_t>(c->series_order) : 0;
+        set_error("");
+        return 5;
+    } catch (...) {
+        set_error("reference statistics lookup failed with an unknown exception");
+        return -1;
+    }
+}
+
+int fractal_render_reference_ex_planes(float* output, int width, int height, const char* zoom, void* handle,
+                                       int max_iter, int threads, int series_order, int series_block,
+                                       const FractalRenderOptions* options, FractalRenderPlanes* planes) {
+    return render_impl(output, width, height, zoom, handle, max_iter, threads, series_order, series_block, options,
+                       planes, false);
+}
+
+int fractal_render_reference_ex(float* output, int width, int height, const char* zoom, void* handle, int max_iter,
+                                int threads, int series_order, int series_block, const FractalRenderOptions* options) {
+    return render_impl(output, width, height, zoom, handle, max_iter, threads, series_order, series_block, options,
+                       nullptr, false);
+}
+
+int fractal_render_mandelbrot_reference_ex(float* output, int width, int height, const char* zoom, void* handle,
+                                           int max_iter, int threads, int series_order, int series_block,
+                                           const FractalRenderOptions* options) {
+    return render_impl(output, width, height, zoom, handle, max_iter, threads, series_order, series_block, options,
+                       nullptr, false);
+}
+
+int render_mandelbrot_reference(float* output, int width, int height, const char* zoom, void* handle, int max_iter,
+                                int threads, int series_order, int series_block) {
+    return render_impl(output, width, height, zoom, handle, max_iter, threads, series_order, series_block, nullptr,
+                       nullptr, false);
+}
+
+int render_fractal_ex(float* output, int width, int height, const char* zoom, const char* x, const char* y,
+                      int max_iter, int bits, int use_perturbation, int threads, int formula, double julia_re,
+                      double julia_im, const FractalRenderOptions* options) {
+    return one_shot(output, width, height, zoom, x, y, max_iter, bits, use_perturbation, threads, formula, julia_re,
+                    julia_im, options, nullptr);
+}
+
+int render_fractal_ex_planes(float* output, int width, int height, const char* zoom, const char* x, const char* y,
+                             int max_iter, int bits, int use_perturbation, int threads, int formula,
+                             double julia_re, double julia_im, const FractalRenderOptions* options,
+                             FractalRenderPlanes* planes) {
+    if (!planes) {
+        set_error("render metadata requires valid native render arguments");
+        return 1;
+    }
+    return one_shot(output, width, height, zoom, x, y, max_iter, bits, use_perturbation, threads, formula, julia_re,
+                    julia_im, options, planes);
+}
+
+// ---- extensions (fav_extensions.h) ----------------------------------------
+
+int fractal_hip_render_reference_device(float* device_output, int width, int height, const char* zoom, void* handle,
+                                        int max_iter, int series_order, int series_block,
+                                        const FractalRenderOptions* options, FractalRenderPlanes* device_planes) {
+    return render_impl(device_output, width, height, zoom, handle, max_iter, 0, series_order, series_block, options,
+                       device_planes, true);
+}
+
+int fractal_hip_last_timing(double* values, int capacity) {
+    if (!values || capacity < 3) {
+        set_error("timing buffer is too small or null");
+        return -1;
+    }
+    const fav::LastTiming t = fav::last_timing();
+    values[0] = t.kernel_ms;
+    values[1] = t.total_ms;
+    values[2] = t.pixels;
+    set_error("");
+    return 3;
+}
+
+int fractal_hip_reference_upload(void* handle) {
+    try {
+        const auto c = acquire(handle);
+        if (!c) throw std::runtime_error("invalid or already-destroyed reference handle");
+        fav::ensure_device_tables(*c);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+
+}  // extern "C"

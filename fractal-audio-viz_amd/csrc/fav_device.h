@@ -1,0 +1,117 @@
+// fav_device.h -- device-side table layouts and the launch interface shared
+// by the host library (abi.cpp / reference_build.cpp, g++) and the gfx950
+// kernels (render_kernel.hip, hipcc).  Plain structs and pointers only.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace fav {
+
+constexpr int kMaxLevels = 16;
+constexpr int kStatsFields = 39;  // RenderStats, renderer.cpp:4615-4643
+
+// One compact linear BLA step (LinearBlaStep, renderer.cpp:8525-8530):
+// delta' = A*delta + B*dc, valid while |delta|^2 < r2.  64 bytes, one
+// cache line, so a wave-uniform lookup is one broadcast load.
+struct alignas(64) LinearStepDev {
+    double a_re, a_im, b_re, b_im, r2_m;
+    int32_t a_e, b_e, r2_e, length;
+    int32_t pad[2];
+};
+static_assert(sizeof(LinearStepDev) == 64, "linear step is one cache line");
+
+// One compact cubic BLA step (FastBlaStep, renderer.cpp:8519-8523): nine
+// ScaledComplex coefficients A..I of the bivariate map in (delta, dc).
+struct alignas(64) CubicStepDev {
+    double re[9];
+    double im[9];
+    double r2_m;
+    int32_t e[9];
+    int32_t r2_e, length;
+    int32_t pad[5];
+};
+static_assert(sizeof(CubicStepDev) == 256, "cubic step is four cache lines");
+
+// Level directory of one BLA hierarchy: level l holds size[l] steps starting
+// at offset[l] in the flat step array.
+struct LevelDir {
+    int32_t count;
+    int32_t size[kMaxLevels];
+    int64_t offset[kMaxLevels];
+};
+
+// Device copy of one reference (orbit + tables), structure of arrays.  The
+// per-index FloatExp parts of Z, the reference escape margin |Z|^2-R^2 and
+// the glitch threshold 1e-7*|Z|^2 are pure functions of the orbit; they are
+// evaluated once on the host with the reference's exact operations, so the
+// kernel reads them instead of re-deriving them at every step.
+struct OrbitDev {
+    const double* re;        // ScaledComplex mantissas of Z_n
+    const double* im;
+    const int32_t* e;        // shared exponent of Z_n
+    const double* zr_m;      // FloatExp parts of Re Z_n, Im Z_n
+    const double* zi_m;
+    const int32_t* zr_e;
+    const int32_t* zi_e;
+    const double* margin_m;  // |Z_n|^2 - R^2 (FloatExp)
+    const int32_t* margin_e;
+    const double* glitch_m;  // 1e-7*|Z_n|^2 (FloatExp; 0 disables the test)
+    const int32_t* glitch_e;
+    const double* real_double;  // ldexp(re, e): the plain-double tail orbit
+    const double* imag_double;
+    int32_t size;
+    int32_t map_end;
+};
+
+struct SeriesDev {
+    const double* re;  // coefficients, index 0..count-1
+    const double* im;
+    const int32_t* e;
+    int32_t count;
+    int32_t iteration;
+    int32_t available;
+    double radius2_m;
+    int32_t radius2_e;
+};
+
+// Everything one field launch needs, passed by value as the kernel argument.
+struct RenderParams {
+    OrbitDev orbit;
+    SeriesDev series;
+    const LinearStepDev* linear;  // tier selected on the host (ordinary/deep)
+    LevelDir linear_dir;
+    const CubicStepDev* cubic;
+    LevelDir cubic_dir;
+    const double* x_off_m;  // per-column FloatExp offsets (width entries)
+    const int32_t* x_off_e;
+    const double* y_off_m;  // per-row FloatExp offsets (height entries)
+    const int32_t* y_off_e;
+    float* output;
+    int64_t* plane_iteration;  // planes (all null when not requested)
+    double* plane_phase;
+    double* plane_de_x;
+    double* plane_de_y;
+    double* plane_test1;
+    double* plane_test2;
+    unsigned long long* stats;   // kStatsFields counters or null
+    int* budget_flag;            // set once the time budget has expired
+    const unsigned long long* budget_start;  // wall_clock64 at launch start
+    unsigned long long budget_ticks;         // 0 = no time budget
+    double output_bias;
+    double pixel_spacing_m;
+    int32_t pixel_spacing_e;
+    int32_t width, height;
+    int32_t max_iter;
+    int32_t escape_radius_mode;
+    int32_t strict;
+    int32_t allow_recovery;
+    int32_t disable_bla;
+    int32_t cycle_enabled;
+    int32_t strict_cycle;
+    int32_t max_bla_length;
+    int32_t max_linear_length;
+    int32_t approximation_order;
+};
+
+}  // namespace fav

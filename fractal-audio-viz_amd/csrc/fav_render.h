@@ -1,0 +1,35 @@
+// fav_render.h -- host interface between the C ABI layer (abi.cpp) and the
+// device launch code (device_host.cpp).
+#pragma once
+
+#include <cstdint>
+
+#include "../../include/fractal_hip.h"
+#include "fav_host.h"
+
+namespace fav {
+
+struct FieldRequest {
+    float* output;                 // host or device pointer (see device_pointers)
+    int width, height;
+    const char* zoom_text;
+    int max_iter;
+    int series_order;
+    int series_block;
+    FractalRenderOptions options;  // validated copy
+    FractalRenderPlanes* planes;   // null when no planes are requested
+    uint64_t* stats;               // kStatsFields host counters or null
+    bool device_pointers;          // output/planes already live in HBM
+};
+
+struct LastTiming {
+    double kernel_ms;   // HIP-event time of the field kernel on the library stream
+    double total_ms;    // wall time from launch to results on the host
+    double pixels;
+};
+
+void render_field(ReferenceContext& context, const FieldRequest& request);
+LastTiming last_timing();
+bool device_available();
+
+}  // namespace fav

@@ -236,6 +236,27 @@ int render_fractal_ex_planes(
     double julia_real, double julia_imag,
     const FractalRenderOptions *options, FractalRenderPlanes *planes);
 
+/* ---- Extensions of the HIP backend (no counterpart in renderer.h) ------ */
+
+/* Same contract as fractal_render_reference_ex_planes, but `device_output`
+ * (width*height floats) and the plane arrays of `device_planes` (may be
+ * NULL) are device pointers already allocated in HBM.  Used to time the
+ * field kernel with its inputs and outputs resident on the GPU. */
+int fractal_hip_render_reference_device(
+    float *device_output, int width, int height, const char *zoom_text,
+    void *handle, int max_iter, int series_order, int series_block,
+    const FractalRenderOptions *options, FractalRenderPlanes *device_planes);
+
+/* Timing of the most recent field launch: values[0] = field kernel time in
+ * ms from HIP events on the library stream, values[1] = wall time of the
+ * call in ms (launch to results on the host), values[2] = pixels.
+ * Returns 3, or -1 if capacity < 3. */
+int fractal_hip_last_timing(double *values, int capacity);
+
+/* Copies a reference's orbit and BLA tables into HBM now instead of at its
+ * first render.  Returns 0 on success, 1 on failure. */
+int fractal_hip_reference_upload(void *handle);
+
 #ifdef __cplusplus
 }
 #endif
