@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Deep-zoom field throughput on one MI355X.
+
+Workload (one step): the 1920x1080 field at zoom 1e100 with a 360-bit MPFR
+reference orbit, cap 100000, BLA on, rebase on, Kalles bailout/grid, render
+planes on, FP64 perturbation.  The reference orbit and BLA tables are built
+on the host once and copied to HBM before timing; each timed step is one
+field render into device-resident output buffers.
+
+value = Gpixel-iterations/s = sum over pixels of n_p (escape index, or the
+cap for interior / unresolved pixels) per second of wall time per step.
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "fractal-audio-viz_amd"))
+
+import fav_hip  # noqa: E402
+import fav_workloads  # noqa: E402
+
+MI355X_FP64_VECTOR_PEAK_TFLOPS = 78.6
+
+
+class Hip:
+    def __init__(self):
+        self.lib = ctypes.CDLL("libamdhip64.so")
+
+    def malloc(self, nbytes):
+        ptr = ctypes.c_void_p()
+        status = self.lib.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes))
+        if status != 0:
+            raise RuntimeError(f"hipMalloc({nbytes}) failed with {status}")
+        return ptr
+
+    def free(self, ptr):
+        self.lib.hipFree(ptr)
+
+    def to_host(self, host_array, ptr):
+        status = self.lib.hipMemcpy(host_array.ctypes.data_as(ctypes.c_void_p), ptr,
+                                    ctypes.c_size_t(host_array.nbytes), 2)
+        if status != 0:
+            raise RuntimeError(f"hipMemcpy D2H failed with {status}")
+
+    def synchronize(self):
+        self.lib.hipDeviceSynchronize()
+
+
+def escape_count_sum(orbit_iteration, field, cap):
+    counts = orbit_iteration.astype(np.int64) + 1
+    counts[orbit_iteration >= cap] = cap
+    counts[np.isnan(field)] = cap
+    return int(counts.sum())
+
+
+def cpu_baseline(work, threads):
+    """The reference's renderer.cpp (oracle/_ref, built from source) on the
+    host cores, over a 1/64-area sample of the same workload."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import native  # checker-side ctypes mirrors
+
+    lib = native.load_reference()
+    kind = "reference"
+    sample_w, sample_h = work["width"] // 8, work["height"] // 8
+    opts = native.RenderOptions.defaults(backend=0, **work["options"])
+    if lib is not None:
+        ref = native.AbiReference(lib, work["x"], work["y"], work["zoom"], work["max_iter"],
+                                  work["bits"], work["image_series_order"], opts)
+    else:
+        kind = "port"
+        lib = native.load_oracle()
+        ref = native.OracleReference(lib, work["x"], work["y"], work["zoom"], work["max_iter"],
+                                     work["bits"], work["image_series_order"],
+                                     work["options"].get("escape_radius_mode", 0),
+                                     work["options"].get("coordinate_mode", 0))
+    started = time.perf_counter()
+    field, planes, _ = ref.render(sample_w, sample_h, work["zoom"], work["max_iter"],
+                                  work["series_order"], work["series_block"], opts,
+                                  planes=work["planes"], threads=threads)
+    seconds = time.perf_counter() - started
+    total = escape_count_sum(planes.orbit_iteration, field.reshape(-1), work["max_iter"])
+    return {"value": total / seconds / 1e9, "unit": "Gpixel-iter/s", "cores": threads, "kind": kind,
+            "sample": f"{sample_w}x{sample_h} of the same field (1/64 of the pixels), "
+                      f"{seconds:.1f} s on {threads} threads ({platform.processor() or platform.machine()})"}
+
+
+def main():
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=10)
+    parser.add_argument("--warmup", type=int, default=2)
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the field path has no cross-GPU exchange in this build "
+              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    work = fav_workloads.field_1e100_1080p()
+    lib = fav_hip.load_library()
+    if not lib.fractal_backend_capabilities() & 8:
+        raise SystemExit(f"no gfx950 device: {lib.fractal_last_error()!r}")
+    hip = Hip()
+    opts = fav_hip.NativeRenderOptions(**work["options"])
+    t0 = time.perf_counter()
+    ref = fav_hip.NativeReference(work["x"], work["y"], work["zoom"], work["max_iter"], work["bits"],
+                                  work["image_series_order"], opts)
+    build_s = time.perf_counter() - t0
+    ref.upload()
+    w, h = work["width"], work["height"]
+    n = w * h
+    d_field = hip.malloc(n * 4)
+    d_planes_block = hip.malloc(n * 48)
+    planes = fav_hip.NativeRenderPlanes()
+    planes.struct_size = ctypes.sizeof(fav_hip.NativeRenderPlanes)
+    planes.version = 1
+    base = d_planes_block.value
+    for i, name in enumerate(("orbit_iteration", "phase", "de_x", "de_y", "test1", "test2")):
+        setattr(planes, name, ctypes.cast(ctypes.c_void_p(base + i * n * 8), planes._fields_[2 + i][1]))
+
+    def step():
+        status = lib.fractal_hip_render_reference_device(
+            d_field, w, h, work["zoom"].encode(), ref.handle, work["max_iter"], work["series_order"],
+            work["series_block"], ctypes.byref(opts), ctypes.byref(planes) if work["planes"] else None)
+        if status != 0:
+            raise RuntimeError(lib.fractal_last_error().decode())
+        timing = fav_hip.last_timing()
+        return timing["kernel_ms"]
+
+    # event counters for the roofline numerator (deterministic, untimed)
+    lib.fractal_set_stats_enabled(1)
+    step()
+    stats = np.zeros(39, dtype=np.uint64)
+    lib.fractal_get_last_stats_ex(stats.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 39)
+    lib.fractal_set_stats_enabled(0)
+    field = np.empty(n, dtype=np.float32)
+    orbit_iteration = np.empty(n, dtype=np.int64)
+    hip.to_host(field, d_field)
+    hip.to_host(orbit_iteration, ctypes.c_void_p(base))
+    pixel_iterations = escape_count_sum(orbit_iteration, field, work["max_iter"])
+    flops = fav_workloads.flops_from_stats(stats, work["planes"])
+
+    for _ in range(args.warmup):
+        step()
+    hip.synchronize()
+    kernel_ms = []
+    started = time.perf_counter()
+    for _ in range(args.steps):
+        kernel_ms.append(step())
+    hip.synchronize()
+    elapsed = time.perf_counter() - started
+    ms_per_step = elapsed / args.steps * 1e3
+    value = pixel_iterations / (elapsed / args.steps) / 1e9
+    mean_kernel_ms = float(np.mean(kernel_ms))
+    achieved = flops / (mean_kernel_ms * 1e-3) / 1e12
+    hip.free(d_field)
+    hip.free(d_planes_block)
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    baseline = None if args.no_cpu_baseline else cpu_baseline(work, threads)
+    result = {
+        "metric": "Gpixel-iterations/s at 1e100 zoom (FP64 vs floatexp/batched), % pixels ≠ FP64",
+        "value": round(value, 3),
+        "unit": "Gpixel-iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded 100-digit extension of the reference's bundled deep point",
+        "config": {"workload": work["name"], "width": w, "height": h, "zoom": work["zoom"],
+                   "max_iter": work["max_iter"], "precision_bits": work["bits"],
+                   "path": "plain FP64 HIP kernel", "planes": work["planes"],
+                   "pixel_iterations": pixel_iterations, "mean_kernel_ms": round(mean_kernel_ms, 3),
+                   "reference_build_s": round(build_s, 3),
+                   "linear_blocks": int(stats[3]), "cubic_blocks": int(stats[4]),
+                   "exact_steps": int(stats[5]), "tail_steps": int(stats[11]),
+                   "nan_pixels": int(np.isnan(field).sum())},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "peak": MI355X_FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / MI355X_FP64_VECTOR_PEAK_TFLOPS, 4), "traffic": None},
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

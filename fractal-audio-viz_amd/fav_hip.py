@@ -1,0 +1,185 @@
+"""Python host binding of the HIP deep-zoom backend.
+
+Mirrors the parts of visualizer.py's ctypes layer that drive the field path
+(visualizer.py:824-944 NativeRenderOptions, 986-1112 planes, 1512-1560 the
+loader's ABI check, 6250-6370 reference creation, 6529-6639 reference render)
+so a caller written against the reference's binding finds the same names,
+argument meaning and errors.  The library must be built (make -C
+fractal-audio-viz_amd); there is no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "libfractal_hip.so"
+ABI_VERSION = 10
+
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int64_p = ctypes.POINTER(ctypes.c_int64)
+_c_uint64_p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class NativeRenderOptions(ctypes.Structure):
+    """FractalRenderOptions (renderer.h:43-67), same defaults as
+    visualizer.py:840-944 except backend=2 (the HIP device)."""
+
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("version", ctypes.c_uint32),
+        ("strict", ctypes.c_int32), ("allow_recovery", ctypes.c_int32),
+        ("time_budget_ms", ctypes.c_int32), ("disable_bla", ctypes.c_int32),
+        ("disable_cycle", ctypes.c_int32), ("strict_cycle", ctypes.c_int32),
+        ("series_min_terms", ctypes.c_int32), ("series_max_terms", ctypes.c_int32),
+        ("max_bla_length", ctypes.c_int32), ("max_linear_bla_length", ctypes.c_int32),
+        ("backend", ctypes.c_int32), ("escape_radius_mode", ctypes.c_int32),
+        ("coordinate_mode", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1),
+        ("output_bias", ctypes.c_double),
+    ]
+
+    def __init__(self, *, strict=True, allow_recovery=False, time_budget_ms=0, disable_bla=False,
+                 disable_cycle=False, strict_cycle=False, series_min_terms=8, series_max_terms=32,
+                 max_bla_length=64, max_linear_bla_length=4096, backend=2, escape_radius_mode=0,
+                 coordinate_mode=0, output_bias=0.0):
+        super().__init__()
+        self.struct_size = ctypes.sizeof(NativeRenderOptions)
+        self.version = 2
+        self.strict = int(bool(strict))
+        self.allow_recovery = int(bool(allow_recovery))
+        self.time_budget_ms = int(time_budget_ms)
+        self.disable_bla = int(bool(disable_bla))
+        self.disable_cycle = int(bool(disable_cycle))
+        self.strict_cycle = int(bool(strict_cycle))
+        self.series_min_terms = int(series_min_terms)
+        self.series_max_terms = int(series_max_terms)
+        self.max_bla_length = int(max_bla_length)
+        self.max_linear_bla_length = int(max_linear_bla_length)
+        if backend not in (0, 1, 2):
+            raise ValueError("backend must be 0, 1 or 2")
+        self.backend = int(backend)
+        self.escape_radius_mode = int(escape_radius_mode)
+        self.coordinate_mode = int(coordinate_mode)
+        self.output_bias = float(output_bias)
+
+
+class NativeRenderPlanes(ctypes.Structure):
+    """FractalRenderPlanes (renderer.h:161-170)."""
+
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("version", ctypes.c_uint32),
+        ("orbit_iteration", _c_int64_p), ("phase", _c_double_p), ("de_x", _c_double_p),
+        ("de_y", _c_double_p), ("test1", _c_double_p), ("test2", _c_double_p),
+    ]
+
+
+_library = None
+
+
+def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
+    """Loads libfractal_hip.so and checks its ABI version (visualizer.py:1550)."""
+    global _library
+    if _library is not None and path is None:
+        return _library
+    target = Path(path or os.environ.get("FRACTAL_HIP_LIBRARY", LIBRARY_PATH))
+    if not target.exists():
+        raise FileNotFoundError(f"{target} is not built; run make -C {LIBRARY_PATH.parent}")
+    lib = ctypes.CDLL(str(target))
+    lib.fractal_abi_version.restype = ctypes.c_int
+    if lib.fractal_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{target} speaks ABI {lib.fractal_abi_version()}, expected {ABI_VERSION}")
+    lib.fractal_last_error.restype = ctypes.c_char_p
+    lib.fractal_backend_capabilities.restype = ctypes.c_int
+    lib.fractal_create_reference_ex_options.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(NativeRenderOptions)]
+    lib.fractal_create_reference_ex_options.restype = ctypes.c_void_p
+    lib.fractal_destroy_reference.argtypes = [ctypes.c_void_p]
+    lib.fractal_destroy_reference.restype = None
+    lib.fractal_render_reference_ex_planes.argtypes = [
+        _c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(NativeRenderOptions),
+        ctypes.POINTER(NativeRenderPlanes)]
+    lib.fractal_render_reference_ex_planes.restype = ctypes.c_int
+    lib.fractal_hip_render_reference_device.argtypes = [
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int, ctypes.POINTER(NativeRenderOptions), ctypes.c_void_p]
+    lib.fractal_hip_render_reference_device.restype = ctypes.c_int
+    lib.fractal_hip_last_timing.argtypes = [_c_double_p, ctypes.c_int]
+    lib.fractal_hip_last_timing.restype = ctypes.c_int
+    lib.fractal_hip_reference_upload.argtypes = [ctypes.c_void_p]
+    lib.fractal_hip_reference_upload.restype = ctypes.c_int
+    lib.fractal_set_stats_enabled.argtypes = [ctypes.c_int]
+    lib.fractal_get_last_stats_ex.argtypes = [_c_uint64_p, ctypes.c_int]
+    lib.fractal_get_last_stats_ex.restype = ctypes.c_int
+    if path is None:
+        _library = lib
+    return lib
+
+
+class NativeReference:
+    """An owned reference handle (visualizer.py:6250-6370, without the Python
+    fallback tiers)."""
+
+    def __init__(self, x_center: str, y_center: str, viewport_zoom: str, max_iter: int,
+                 precision_bits: int, image_series_order: int = 8,
+                 options: NativeRenderOptions | None = None, library=None):
+        self.library = library or load_library()
+        self.options = options or NativeRenderOptions()
+        handle = self.library.fractal_create_reference_ex_options(
+            x_center.encode(), y_center.encode(), viewport_zoom.encode(), int(max_iter),
+            int(precision_bits), int(image_series_order), 0, b"0", b"0", ctypes.byref(self.options))
+        if not handle:
+            raise RuntimeError(self.library.fractal_last_error().decode())
+        self.handle = handle
+
+    def upload(self) -> None:
+        if self.library.fractal_hip_reference_upload(self.handle) != 0:
+            raise RuntimeError(self.library.fractal_last_error().decode())
+
+    def render(self, width: int, height: int, zoom: str, max_iter: int, series_order: int = 3,
+               series_block: int = 256, planes: bool = False,
+               options: NativeRenderOptions | None = None):
+        """Field (H, W) float32 and, when requested, a dict of planes."""
+        opts = options or self.options
+        n = int(width) * int(height)
+        out = np.empty(n, dtype=np.float32)
+        plane_arrays = None
+        plane_struct = None
+        if planes:
+            plane_arrays = {"orbit_iteration": np.empty(n, dtype=np.int64)}
+            for name in ("phase", "de_x", "de_y", "test1", "test2"):
+                plane_arrays[name] = np.empty(n, dtype=np.float64)
+            plane_struct = NativeRenderPlanes()
+            plane_struct.struct_size = ctypes.sizeof(NativeRenderPlanes)
+            plane_struct.version = 1
+            plane_struct.orbit_iteration = plane_arrays["orbit_iteration"].ctypes.data_as(_c_int64_p)
+            for name in ("phase", "de_x", "de_y", "test1", "test2"):
+                setattr(plane_struct, name, plane_arrays[name].ctypes.data_as(_c_double_p))
+        status = self.library.fractal_render_reference_ex_planes(
+            out.ctypes.data_as(_c_float_p), int(width), int(height), zoom.encode(), self.handle,
+            int(max_iter), 0, int(series_order), int(series_block), ctypes.byref(opts),
+            ctypes.byref(plane_struct) if plane_struct is not None else None)
+        if status != 0:
+            raise RuntimeError(self.library.fractal_last_error().decode())
+        field = out.reshape(height, width)
+        if planes:
+            return field, {k: v.reshape(height, width) for k, v in plane_arrays.items()}
+        return field
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.library.fractal_destroy_reference(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+def last_timing() -> dict:
+    """HIP-event kernel time and wall time of the most recent render."""
+    values = (ctypes.c_double * 3)()
+    load_library().fractal_hip_last_timing(values, 3)
+    return {"kernel_ms": values[0], "total_ms": values[1], "pixels": values[2]}

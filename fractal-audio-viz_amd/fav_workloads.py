@@ -1,0 +1,67 @@
+"""Synthetic inputs of the measured deep-zoom workloads.
+
+The centres are the reference's bundled deep point
+(-0.743643887037158704752191506114774 + 0.131825904205311970493132056385139i)
+optionally extended by seeded decimal digits, so the 1e100 field has a
+well-defined 360-bit reference orbit.
+"""
+import random
+
+BASE_X = "-0.743643887037158704752191506114774"
+BASE_Y = "0.131825904205311970493132056385139"
+
+
+def extended_centre(seed: int = 12345, digits: int = 100):
+    rng = random.Random(seed)
+    x = BASE_X + "".join(rng.choice("0123456789") for _ in range(digits))
+    y = BASE_Y + "".join(rng.choice("0123456789") for _ in range(digits))
+    return x, y
+
+
+def field_1e100_1080p():
+    """1920x1080 at zoom 1e100, 360-bit orbit, cap 100000, BLA on, rebase
+    on, Kalles bailout (R=1e4) and pixel grid, image series off
+    (series_min_terms=32 with an order-8 series, as the reference's
+    benchmark.py:48-55 does), BLA order 3 with block 256, planes on (the
+    distance/slope shading inputs)."""
+    x, y = extended_centre()
+    return dict(name="field_1920x1080_zoom1e100_cap100000", x=x, y=y, zoom="1e100", width=1920,
+                height=1080, max_iter=100000, bits=360, image_series_order=8, series_order=3,
+                series_block=256, options=dict(series_min_terms=32, escape_radius_mode=1,
+                                               coordinate_mode=1), planes=True)
+
+
+def field_1e12_640x360():
+    """640x360 at zoom 1e12, 128-bit orbit, cap 20000, project grid,
+    R=2, image series on (render_fractal_ex's order 8 / block 32)."""
+    return dict(name="field_640x360_zoom1e12_cap20000", x=BASE_X, y=BASE_Y, zoom="1e12", width=640,
+                height=360, max_iter=20000, bits=128, image_series_order=8, series_order=8,
+                series_block=32, options={}, planes=True)
+
+
+# Algorithmic FP64 operations per event (the roofline numerator).  Complex
+# multiply 6, complex add 2, squared norm 3; normalisation, exponent and
+# comparison work is not counted.
+FLOPS = {
+    "exact_step": 16 + 16,    # d' = 2Zd + d^2 + dc, plus |Z+d|^2 margin and |d|^2
+    "replay_step": 16 + 16,
+    "tail_step": 19,          # plain-double tail step (renderer.cpp:12527-12554)
+    "linear_block": 14 + 16,  # A*d + B*dc, plus endpoint margin and |d|^2
+    "cubic_block": 70 + 16,   # order-3 Horner (renderer.cpp:8710-8730)
+    "derivative": 10,         # dz' = 2 z dz + 1, per step when planes are written
+    "derivative_linear": 8,   # dz' = A dz + B, per linear block
+}
+
+
+def flops_from_stats(stats, planes: bool) -> float:
+    """FP64 operations of one field from its RenderStats counters
+    (renderer.cpp:4615-4643)."""
+    linear, cubic, exact, replay, tail = (float(stats[3]), float(stats[4]), float(stats[5]),
+                                          float(stats[6]), float(stats[11]))
+    scaled_exact = exact - tail  # exact_steps includes the plain-double tail steps
+    total = (scaled_exact * FLOPS["exact_step"] + replay * FLOPS["replay_step"]
+             + tail * FLOPS["tail_step"] + linear * FLOPS["linear_block"]
+             + cubic * FLOPS["cubic_block"])
+    if planes:
+        total += (scaled_exact + replay + tail) * FLOPS["derivative"] + linear * FLOPS["derivative_linear"]
+    return total

@@ -1,0 +1,99 @@
+"""Loading and comparing golden fixtures (tests/golden/*.npz)."""
+import ctypes
+import json
+from pathlib import Path
+
+import numpy as np
+
+import native
+from cases import CASES
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def load(name):
+    with np.load(GOLDEN / f"{name}.npz", allow_pickle=False) as data:
+        out = {k: data[k] for k in data.files}
+    out["meta"] = json.loads(str(out["meta"]))
+    return out
+
+
+def case_names():
+    return [c["name"] for c in CASES]
+
+
+def render_oracle(lib, case, stats=True):
+    """Replays a golden case through the C oracle."""
+    m = case
+    ref = native.OracleReference(lib, m["x"], m["y"], m["ref_zoom"], m["max_iter"], m["bits"],
+                                 m["image_series_order"], m["options"].get("escape_radius_mode", 0),
+                                 m["options"].get("coordinate_mode", 0))
+    if m["clone_zoom"]:
+        ref = ref.clone(m["clone_zoom"])
+    opts = native.RenderOptions.defaults(**m["options"])
+    return ref.render(m["width"], m["height"], m["zoom"], m["max_iter"], m["series_order"],
+                      m["series_block"], opts, planes=True, stats=stats)
+
+
+def render_abi(lib, case, stats=True, backend=2):
+    """Replays a golden case through a renderer.h-speaking library."""
+    m = case
+    opts = native.RenderOptions.defaults(backend=backend, **m["options"])
+    if m["clone_zoom"]:
+        handle = lib.fractal_create_reference_reusable_options(
+            m["x"].encode(), m["y"].encode(), m["ref_zoom"].encode(), m["max_iter"], m["bits"],
+            m["image_series_order"], ctypes.byref(opts))
+        assert handle, lib.fractal_last_error()
+        clone = lib.fractal_clone_reference(ctypes.c_void_p(handle), m["clone_zoom"].encode())
+        assert clone, lib.fractal_last_error()
+        lib.fractal_destroy_reference(ctypes.c_void_p(handle))
+        ref = native.AbiReference.__new__(native.AbiReference)
+        ref.lib, ref.handle = lib, clone
+    else:
+        ref = native.AbiReference(lib, m["x"], m["y"], m["ref_zoom"], m["max_iter"], m["bits"],
+                                  m["image_series_order"], opts)
+    return ref.render(m["width"], m["height"], m["zoom"], m["max_iter"], m["series_order"],
+                      m["series_block"], opts, planes=True, stats=stats)
+
+
+def stats_without_time(stats):
+    s = np.asarray(stats, dtype=np.uint64).copy()
+    s[22] = 0  # render_ns is wall time
+    return s
+
+
+def assert_bit_exact(golden, field, planes, stats):
+    """Field, planes and counters identical to the last bit."""
+    np.testing.assert_array_equal(field.reshape(-1).view(np.uint32), golden["field"].reshape(-1).view(np.uint32))
+    np.testing.assert_array_equal(planes.orbit_iteration, golden["orbit_iteration"])
+    for name in ("phase", "de_x", "de_y", "test1", "test2"):
+        np.testing.assert_array_equal(getattr(planes, name).view(np.uint64), golden[name].view(np.uint64),
+                                      err_msg=name)
+    np.testing.assert_array_equal(stats_without_time(stats), stats_without_time(golden["stats"]))
+
+
+# Tolerances of the HIP path against the FP64 CPU path.  Escape counts, the
+# NaN mask, test1/test2 and all work counters are bit-exact; the float32
+# field is bit-exact except that the reference forms the final smoothing
+# logarithm in x87 long double (renderer.cpp:8170-8181), so one float32 ulp
+# is allowed there; phase/DE use libm transcendentals and long double sums
+# (renderer.cpp:7845-7896), held to 1e-9 relative.
+FIELD_ULPS = 1
+PLANE_RTOL = 1e-9
+
+
+def assert_hip_parity(golden, field, planes, stats):
+    f = field.reshape(-1)
+    g = golden["field"].reshape(-1)
+    np.testing.assert_array_equal(np.isnan(f), np.isnan(g))
+    finite = ~np.isnan(g)
+    ulps = np.abs(f[finite].view(np.int32).astype(np.int64) - g[finite].view(np.int32).astype(np.int64))
+    assert ulps.size == 0 or int(ulps.max()) <= FIELD_ULPS, f"field differs by {int(ulps.max())} ulp"
+    np.testing.assert_array_equal(planes.orbit_iteration, golden["orbit_iteration"])
+    for name in ("test1", "test2"):
+        np.testing.assert_array_equal(getattr(planes, name).view(np.uint64), golden[name].view(np.uint64),
+                                      err_msg=name)
+    for name in ("phase", "de_x", "de_y"):
+        np.testing.assert_allclose(getattr(planes, name), golden[name], rtol=PLANE_RTOL, atol=0.0,
+                                   err_msg=name)
+    np.testing.assert_array_equal(stats_without_time(stats), stats_without_time(golden["stats"]))

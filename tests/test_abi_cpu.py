@@ -1,0 +1,149 @@
+"""The HIP library's C ABI without a GPU: it loads, exports every symbol
+include/fractal_hip.h declares, validates arguments like the reference
+(renderer.cpp:4729-4765, 19814-19860), builds references on the host, and
+fails loudly instead of falling back when no device is present."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import native
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "fractal_hip.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_hot_path_abi():
+    names = declared_functions()
+    for required in ("fractal_abi_version", "fractal_last_error", "fractal_create_reference",
+                     "fractal_destroy_reference", "render_mandelbrot_reference",
+                     "fractal_render_reference_ex_planes", "render_fractal_ex"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol(product):
+    for name in declared_functions():
+        assert hasattr(product, name), name
+
+
+def test_identity(product):
+    assert product.fractal_abi_version() == 10
+    assert product.fractal_render_options_version() == 2
+    opts = native.RenderOptions()
+    product.fractal_render_options_default(ctypes.byref(opts))
+    assert opts.struct_size == ctypes.sizeof(native.RenderOptions) == 72
+    assert (opts.version, opts.strict, opts.series_min_terms, opts.series_max_terms) == (2, 1, 8, 32)
+    assert (opts.max_bla_length, opts.max_linear_bla_length, opts.backend) == (64, 4096, 2)
+
+
+def test_capabilities_never_claim_cpu_backends(product):
+    caps = product.fractal_backend_capabilities()
+    assert caps & 3 == 0
+    assert caps in (0, 12)
+
+
+X = b"-0.743643887037158704752191506114774"
+Y = b"0.131825904205311970493132056385139"
+
+
+def test_reference_builds_on_host_and_reports_stats(product):
+    handle = product.fractal_create_reference(X, Y, b"1e12", 2000, 128, 8)
+    assert handle, product.fractal_last_error()
+    values = np.zeros(5, dtype=np.uint64)
+    assert product.fractal_get_reference_stats(ctypes.c_void_p(handle), values.ctypes.data_as(native.c_uint64_p), 5) == 5
+    assert values[4] == 8 and values[3] > 1  # series order and validated iteration
+    product.fractal_destroy_reference(ctypes.c_void_p(handle))
+    assert product.fractal_last_error() == b""
+    product.fractal_destroy_reference(ctypes.c_void_p(handle))  # stale handle
+    assert b"invalid or already-destroyed" in product.fractal_last_error()
+    product.fractal_destroy_reference(None)  # NULL is a no-op
+    assert product.fractal_last_error() == b""
+
+
+def test_reference_stats_match_oracle_series(product, oracle):
+    handle = product.fractal_create_reference(X, Y, b"1e12", 20000, 128, 8)
+    values = np.zeros(5, dtype=np.uint64)
+    product.fractal_get_reference_stats(ctypes.c_void_p(handle), values.ctypes.data_as(native.c_uint64_p), 5)
+    ref = native.OracleReference(oracle, X.decode(), Y.decode(), "1e12", 20000, 128, 8)
+    info = ref.info()
+    assert int(values[3]) == info["series_iteration"] and int(values[4]) == info["series_order"]
+    product.fractal_destroy_reference(ctypes.c_void_p(handle))
+
+
+@pytest.mark.parametrize("kwargs,message", [
+    (dict(backend=0), b"HIP backend"),
+    (dict(backend=1), b"HIP backend"),
+    (dict(backend=3), b"unknown native render backend"),
+    (dict(strict=2), b"strict render flag must be 0 or 1"),
+    (dict(escape_radius_mode=2), b"unknown escape-radius mode"),
+    (dict(time_budget_ms=-1), b"render time budget cannot be negative"),
+    (dict(output_bias=-1.0), b"output bias"),
+])
+def test_render_option_validation(product, kwargs, message):
+    handle = product.fractal_create_reference(X, Y, b"1e12", 200, 128, 8)
+    out = np.zeros(16, dtype=np.float32)
+    opts = native.RenderOptions.defaults(**{"backend": 2, **kwargs})
+    status = product.fractal_render_reference_ex(out.ctypes.data_as(native.c_float_p), 4, 4, b"1e12",
+                                                 ctypes.c_void_p(handle), 200, 0, 8, 32, ctypes.byref(opts))
+    assert status == 1
+    assert message in product.fractal_last_error()
+    product.fractal_destroy_reference(ctypes.c_void_p(handle))
+
+
+def test_invalid_handles_and_dimensions(product):
+    out = np.zeros(16, dtype=np.float32)
+    opts = native.RenderOptions.defaults(backend=2)
+    assert product.fractal_render_reference_ex(out.ctypes.data_as(native.c_float_p), 4, 4, b"1e12",
+                                               ctypes.c_void_p(0x7777), 200, 0, 8, 32, ctypes.byref(opts)) == 1
+    assert b"invalid or already-destroyed" in product.fractal_last_error()
+    handle = product.fractal_create_reference(X, Y, b"1e12", 200, 128, 8)
+    assert product.fractal_render_reference_ex(out.ctypes.data_as(native.c_float_p), 0, 4, b"1e12",
+                                               ctypes.c_void_p(handle), 200, 0, 8, 32, ctypes.byref(opts)) == 1
+    assert product.fractal_render_reference_ex(out.ctypes.data_as(native.c_float_p), 4, 4, b"1e12",
+                                               ctypes.c_void_p(handle), 201, 0, 8, 32, ctypes.byref(opts)) == 1
+    assert b"exceeds prepared reference" in product.fractal_last_error()
+    kalles = native.RenderOptions.defaults(backend=2, escape_radius_mode=1)
+    assert product.fractal_render_reference_ex(out.ctypes.data_as(native.c_float_p), 4, 4, b"1e12",
+                                               ctypes.c_void_p(handle), 200, 0, 8, 32, ctypes.byref(kalles)) == 1
+    assert b"escape-radius mode" in product.fractal_last_error()
+    product.fractal_destroy_reference(ctypes.c_void_p(handle))
+
+
+def test_invalid_reference_arguments(product):
+    assert not product.fractal_create_reference(X, Y, b"1e12", 0, 128, 8)
+    assert not product.fractal_create_reference(X, Y, b"1e12", 100, 64, 8)
+    assert not product.fractal_create_reference(b"not-a-number", Y, b"1e12", 100, 128, 8)
+    assert b"invalid real coordinate" in product.fractal_last_error()
+    assert not product.fractal_create_reference(X, Y, b"-5", 100, 128, 8)
+    assert not product.fractal_create_reference_ex(X, Y, b"1e12", 100, 128, 8, 1, b"0", b"0")
+    assert b"Mandelbrot" in product.fractal_last_error()
+
+
+def test_clone_requires_reusable_root(product):
+    plain = product.fractal_create_reference(X, Y, b"1e12", 200, 128, 8)
+    assert not product.fractal_clone_reference(ctypes.c_void_p(plain), b"1e13")
+    assert b"reusable tier root" in product.fractal_last_error()
+    root = product.fractal_create_reference_reusable(X, Y, b"1e12", 200, 128, 8)
+    clone = product.fractal_clone_reference(ctypes.c_void_p(root), b"1e13")
+    assert clone, product.fractal_last_error()
+    for h in (plain, root, clone):
+        product.fractal_destroy_reference(ctypes.c_void_p(h))
+
+
+def test_render_without_device_fails_loudly(product):
+    if product.fractal_backend_capabilities() & 8:
+        pytest.skip("a device is present")
+    handle = product.fractal_create_reference(X, Y, b"1e12", 200, 128, 8)
+    out = np.full(16, 7.0, dtype=np.float32)
+    opts = native.RenderOptions.defaults(backend=2)
+    status = product.fractal_render_reference_ex(out.ctypes.data_as(native.c_float_p), 4, 4, b"1e12",
+                                                 ctypes.c_void_p(handle), 200, 0, 8, 32, ctypes.byref(opts))
+    assert status == 1 and product.fractal_last_error()
+    assert np.all(out == 7.0)  # nothing was computed on the host
+    product.fractal_destroy_reference(ctypes.c_void_p(handle))
