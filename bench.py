@@ -63,13 +63,13 @@ def escape_count_sum(orbit_iteration, field, cap):
 
 def cpu_baseline(work, threads):
     """The reference's renderer.cpp (oracle/_ref, built from source) on the
-    host cores, over a 1/64-area sample of the same workload."""
+    host cores, over a 1/16-area sample of the same workload."""
     sys.path.insert(0, str(ROOT / "tests"))
     import native  # checker-side ctypes mirrors
 
     lib = native.load_reference()
     kind = "reference"
-    sample_w, sample_h = work["width"] // 8, work["height"] // 8
+    sample_w, sample_h = work["width"] // 4, work["height"] // 4
     opts = native.RenderOptions.defaults(backend=0, **work["options"])
     if lib is not None:
         ref = native.AbiReference(lib, work["x"], work["y"], work["zoom"], work["max_iter"],
@@ -88,7 +88,7 @@ def cpu_baseline(work, threads):
     seconds = time.perf_counter() - started
     total = escape_count_sum(planes.orbit_iteration, field.reshape(-1), work["max_iter"])
     return {"value": total / seconds / 1e9, "unit": "Gpixel-iter/s", "cores": threads, "kind": kind,
-            "sample": f"{sample_w}x{sample_h} of the same field (1/64 of the pixels), "
+            "sample": f"{sample_w}x{sample_h} of the same field (1/16 of the pixels), "
                       f"{seconds:.1f} s on {threads} threads ({platform.processor() or platform.machine()})"}
 
 

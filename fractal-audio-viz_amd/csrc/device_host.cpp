@@ -13,6 +13,7 @@
 
 #include "fav_host.h"
 #include "fav_render.h"
+#include "table_image.h"
 
 namespace fav {
 
@@ -100,34 +101,13 @@ DeviceTables::~DeviceTables() {
 
 namespace {
 
-void* upload_blob(const std::vector<unsigned char>& blob, hipStream_t stream) {
-    void* d = nullptr;
-    check(hipMalloc(&d, std::max<size_t>(blob.size(), 256)), "hipMalloc tables");
-    if (!blob.empty()) check(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, stream), "upload");
-    return d;
-}
-
-template <typename Step>
-void* upload_levels(const std::vector<std::vector<Step>>& levels, LevelDir& dir, const Step*& base,
-                    hipStream_t stream, size_t& bytes) {
-    if (levels.size() > static_cast<size_t>(kMaxLevels))
-        throw std::runtime_error("BLA hierarchy deeper than the device level directory");
-    std::vector<Step> flat;
-    dir = LevelDir{};
-    dir.count = static_cast<int32_t>(levels.size());
-    for (size_t l = 0; l < levels.size(); ++l) {
-        dir.offset[l] = static_cast<int64_t>(flat.size());
-        dir.size[l] = static_cast<int32_t>(levels[l].size());
-        flat.insert(flat.end(), levels[l].begin(), levels[l].end());
-    }
-    void* d = nullptr;
-    const size_t n = std::max<size_t>(flat.size(), 1) * sizeof(Step);
-    check(hipMalloc(&d, n), "hipMalloc BLA");
-    if (!flat.empty())
-        check(hipMemcpyAsync(d, flat.data(), flat.size() * sizeof(Step), hipMemcpyHostToDevice, stream), "upload BLA");
-    base = static_cast<const Step*>(d);
+template <typename T>
+void upload_vector(const std::vector<T>& v, void*& block, const T*& view, hipStream_t stream, size_t& bytes) {
+    const size_t n = std::max<size_t>(v.size(), 1) * sizeof(T);
+    check(hipMalloc(&block, n), "hipMalloc tables");
+    if (!v.empty()) check(hipMemcpyAsync(block, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream), "upload");
+    view = static_cast<const T*>(block);
     bytes += n;
-    return d;
 }
 
 }  // namespace
@@ -141,22 +121,20 @@ void ensure_device_tables(ReferenceContext& c) {
         init_locked(s);
     }
     const uint64_t t0 = now_ns();
+    TableImage img;
+    make_table_image(c, img);
     auto t = std::make_unique<DeviceTables>();
     t->device = s.device;
     const OrbitData& o = *c.orbit;
     std::vector<unsigned char> blob;
-    std::vector<double> re(o.scaled.size()), im(o.scaled.size());
-    std::vector<int32_t> ex(o.scaled.size());
-    for (size_t i = 0; i < o.scaled.size(); ++i) { re[i] = o.scaled[i].re; im[i] = o.scaled[i].im; ex[i] = o.scaled[i].e; }
-    const size_t off_re = append(blob, re), off_im = append(blob, im), off_e = append(blob, ex);
+    const size_t off_re = append(blob, img.re), off_im = append(blob, img.im), off_e = append(blob, img.e);
     const size_t off_zr_m = append(blob, o.zr_m), off_zi_m = append(blob, o.zi_m);
     const size_t off_zr_e = append(blob, o.zr_e), off_zi_e = append(blob, o.zi_e);
     const size_t off_mm = append(blob, o.margin_m), off_me = append(blob, o.margin_e);
     const size_t off_gm = append(blob, o.glitch_m), off_ge = append(blob, o.glitch_e);
     const size_t off_rd = append(blob, o.real_double), off_id = append(blob, o.imag_double);
-    t->orbit_block = upload_blob(blob, s.stream);
-    t->bytes += blob.size();
-    auto* base = static_cast<unsigned char*>(t->orbit_block);
+    const unsigned char* base = nullptr;
+    upload_vector(blob, t->orbit_block, base, s.stream, t->bytes);
     t->orbit.re = reinterpret_cast<const double*>(base + off_re);
     t->orbit.im = reinterpret_cast<const double*>(base + off_im);
     t->orbit.e = reinterpret_cast<const int32_t*>(base + off_e);
@@ -172,96 +150,52 @@ void ensure_device_tables(ReferenceContext& c) {
     t->orbit.imag_double = reinterpret_cast<const double*>(base + off_id);
     t->orbit.size = static_cast<int32_t>(o.scaled.size());
     t->orbit.map_end = c.map_end;
-    // series coefficients
     std::vector<unsigned char> sblob;
-    std::vector<double> sre, sim;
-    std::vector<int32_t> se;
-    for (const SC& k : c.series_coefficients) { sre.push_back(k.re); sim.push_back(k.im); se.push_back(k.e); }
-    const size_t s_re = append(sblob, sre), s_im = append(sblob, sim), s_e = append(sblob, se);
-    t->series_block = upload_blob(sblob, s.stream);
-    auto* sbase = static_cast<unsigned char*>(t->series_block);
+    const size_t s_re = append(sblob, img.series_re), s_im = append(sblob, img.series_im);
+    const size_t s_e = append(sblob, img.series_e);
+    const unsigned char* sbase = nullptr;
+    upload_vector(sblob, t->series_block, sbase, s.stream, t->bytes);
     t->series.re = reinterpret_cast<const double*>(sbase + s_re);
     t->series.im = reinterpret_cast<const double*>(sbase + s_im);
     t->series.e = reinterpret_cast<const int32_t*>(sbase + s_e);
-    t->series.count = static_cast<int32_t>(c.series_coefficients.size());
-    t->series.iteration = c.series_iteration;
-    t->series.radius2_m = c.series_radius_squared.m;
-    t->series.radius2_e = c.series_radius_squared.e;
-    t->cubic_block = upload_levels(c.cubic.levels, t->cubic_dir, t->cubic, s.stream, t->bytes);
-    for (int tier = 0; tier < 2; ++tier)
-        t->linear_block[tier] = upload_levels(c.linear[tier].levels, t->linear_dir[tier], t->linear[tier],
-                                              s.stream, t->bytes);
+    upload_vector(img.cubic, t->cubic_block, t->cubic, s.stream, t->bytes);
+    t->cubic_dir = img.cubic_dir;
+    for (int tier = 0; tier < 2; ++tier) {
+        upload_vector(img.linear[tier], t->linear_block[tier], t->linear[tier], s.stream, t->bytes);
+        t->linear_dir[tier] = img.linear_dir[tier];
+    }
     check(hipStreamSynchronize(s.stream), "table upload");
     c.upload_ns = now_ns() - t0;
     c.device = std::move(t);
 }
 
-// One deep field render on the device, renderer.cpp:12672-13583 host part.
+// One deep field render on the device, host part of renderer.cpp:12672-13583.
 void render_field(ReferenceContext& c, const FieldRequest& r) {
     const int w = r.width, h = r.height;
-    const FractalRenderOptions& opt = r.options;
-    const FE zoom = parse_zoom(r.zoom_text, c.precision_bits);
-    // viewport, renderer.cpp:12689-12705
-    const double factor = c.coordinate_mode == 1 ? 4.0 : 2.8;
-    const FE inverse_zoom = fe_div(fe_from_parts(1.0, 0), zoom);
-    const FE view_height = fe_muld(inverse_zoom, factor);
-    const FE view_width = fe_muld(view_height, static_cast<double>(w) / static_cast<double>(h));
-    const FE pixel_spacing = fe_div(view_height, fe_from_parts(static_cast<double>(h), 0));
-    const FE current_input_radius = fe_muld(inverse_zoom, factor);
-    const bool covers = fe_compare(current_input_radius, c.input_radius) <= 0;
-    const bool use_deep_linear = fe_compare(current_input_radius, c.deep_input_radius) <= 0;
-    const bool use_ultra = fe_compare(current_input_radius, fe_muld(c.input_radius, 1.0e-40)) <= 0;
-    // per-row / per-column offsets, renderer.cpp:12764-12783
-    std::vector<double> off_m(static_cast<size_t>(w) + h);
-    std::vector<int32_t> off_e(static_cast<size_t>(w) + h);
-    for (int px = 0; px < w; ++px) {
-        const double origin = opt.coordinate_mode == 1 ? static_cast<double>(w / 2) : static_cast<double>(w - 1) / 2.0;
-        const FE v = fe_muld(view_width, (static_cast<double>(px) - origin) / static_cast<double>(w));
-        off_m[static_cast<size_t>(px)] = v.m;
-        off_e[static_cast<size_t>(px)] = v.e;
-    }
-    for (int py = 0; py < h; ++py) {
-        const double origin = opt.coordinate_mode == 1 ? static_cast<double>(h / 2) : static_cast<double>(h - 1) / 2.0;
-        const FE v = fe_muld(view_height, -(static_cast<double>(py) - origin) / static_cast<double>(h));
-        off_m[static_cast<size_t>(w + py)] = v.m;
-        off_e[static_cast<size_t>(w + py)] = v.e;
-    }
+    PreparedRender prep;
+    prepare_render(c, r, prep);
     ensure_device_tables(c);
     const DeviceTables& t = *c.device;
-
-    RenderParams p{};
+    RenderParams p = prep.params;
     p.orbit = t.orbit;
+    const SeriesDev scalar_series = p.series;
     p.series = t.series;
-    p.series.available = c.series_enabled && c.series_order >= opt.series_min_terms
-        && c.series_order <= opt.series_max_terms
-        && fe_compare(current_input_radius, fe_sqrt(c.series_radius_squared)) <= 0
-        && c.series_iteration < r.max_iter;
-    const int tier = use_deep_linear ? 1 : 0;
-    p.linear = t.linear[tier];
-    p.linear_dir = t.linear_dir[tier];
+    p.series.count = scalar_series.count;
+    p.series.iteration = scalar_series.iteration;
+    p.series.available = scalar_series.available;
+    p.series.radius2_m = scalar_series.radius2_m;
+    p.series.radius2_e = scalar_series.radius2_e;
+    p.linear = t.linear[prep.linear_tier];
+    p.linear_dir = t.linear_dir[prep.linear_tier];
     p.cubic = t.cubic;
     p.cubic_dir = t.cubic_dir;
-    p.output_bias = opt.output_bias;
-    p.pixel_spacing_m = pixel_spacing.m;
-    p.pixel_spacing_e = pixel_spacing.e;
-    p.width = w;
-    p.height = h;
-    p.max_iter = r.max_iter;
-    p.escape_radius_mode = opt.escape_radius_mode;
-    p.strict = opt.strict != 0;
-    p.allow_recovery = opt.allow_recovery != 0;
-    p.disable_bla = !covers || opt.disable_bla != 0;
-    p.cycle_enabled = opt.disable_cycle == 0 && (opt.strict == 0 || opt.strict_cycle != 0);
-    p.strict_cycle = opt.strict_cycle != 0;
-    p.max_bla_length = std::clamp(std::min(r.series_block, opt.max_bla_length), 2, 64);
-    p.max_linear_length = std::clamp(std::min(r.series_block, opt.max_linear_bla_length), 2,
-                                     use_ultra ? 4096 : (use_deep_linear ? 1024 : 64));
-    p.approximation_order = std::clamp(r.series_order, 1, 3);
 
     LaunchState& s = state();
     std::lock_guard<std::mutex> lock(s.mutex);
     init_locked(s);
     const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
+    const std::vector<double>& off_m = prep.offset_m;
+    const std::vector<int32_t>& off_e = prep.offset_e;
     // offsets: doubles first, then the int32 exponents
     char* doff = static_cast<char*>(s.offsets.get(off_m.size() * 12 + 512));
     const size_t exp_offset = (off_m.size() * 8 + 255) & ~size_t(255);
@@ -302,8 +236,8 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     void* budget = s.budget.get(64);
     p.budget_start = static_cast<const unsigned long long*>(budget);
     p.budget_flag = reinterpret_cast<int*>(static_cast<char*>(budget) + 8);
-    p.budget_ticks = opt.time_budget_ms > 0
-        ? static_cast<unsigned long long>(opt.time_budget_ms * s.wall_clock_khz)
+    p.budget_ticks = r.options.time_budget_ms > 0
+        ? static_cast<unsigned long long>(r.options.time_budget_ms * s.wall_clock_khz)
         : 0ULL;
 
     const uint64_t wall0 = now_ns();

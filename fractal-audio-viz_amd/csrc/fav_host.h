@@ -77,7 +77,7 @@ struct ReferenceContext {
     uint64_t reference_build_ns = 0, series_build_ns = 0, bla_build_ns = 0;
     uint64_t upload_ns = 0;
     std::mutex device_mutex;
-    std::unique_ptr<DeviceTables> device;
+    std::shared_ptr<DeviceTables> device;  // deleter bound where the tables are uploaded
 };
 
 // Builders (reference_build.cpp).  They throw std::runtime_error.

@@ -1,0 +1,740 @@
+// field_core.h -- per-pixel deep-zoom state machine of the FP64 field path.
+//
+// Each pixel runs the reference's loop (renderer.cpp:12817-13544): image
+// series jump, then glitch test, escape test, optional Brent cycle check,
+// rebase, reference-escape switch, BLA lookup with endpoint replay,
+// plain-double tail and exact perturbation steps.  Every floating-point
+// operation on the control path is an IEEE add/sub/mul/compare or an exact
+// frexp/ldexp, compiled with -ffp-contract=off, so escape counts are
+// bit-identical to the scalar CPU path.
+//
+// The code is written once for the gfx950 kernel (render_kernel.hip) and
+// compiles unchanged as host C++ for a test harness that checks the kernel
+// logic against the oracle without a GPU (host_check.cpp, test-only).
+#pragma once
+
+#include "fav_device.h"
+#include "fav_numeric.h"
+
+#if !defined(__HIP_DEVICE_COMPILE__)
+#include <atomic>
+#include <chrono>
+#endif
+
+namespace fav {
+namespace core {
+
+constexpr double kLn2 = 0.693147180559945309417232121458176568;
+constexpr double kTwoPi = 6.283185307179586476925286766559005768;
+
+enum StatField {  // RenderStats order, renderer.cpp:4615-4643
+    ST_PIXELS, ST_LOGICAL, ST_BLA_BLOCKS, ST_LINEAR_BLOCKS, ST_CUBIC_BLOCKS, ST_EXACT,
+    ST_REPLAY, ST_BLA_RETRIES, ST_CYCLE_INSIDE, ST_TAIL_PIXELS, ST_BLA_DISABLED,
+    ST_TAIL_STEPS, ST_MAX_TAIL_STEPS, ST_TAIL_REBASES, ST_TAIL_FALLBACKS, ST_MAX_PIXEL_ITER,
+    ST_SERIES_PIXELS, ST_SERIES_JUMPS, ST_GLITCH, ST_UNRESOLVED, ST_DEADLINE, ST_SECONDARY,
+    ST_RENDER_NS, ST_HISTOGRAM
+};
+
+// ---- platform hooks ------------------------------------------------------
+FAV_HD unsigned long long clock_ticks() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return wall_clock64();
+#else
+    return static_cast<unsigned long long>(std::chrono::duration_cast<std::chrono::microseconds>(
+        std::chrono::steady_clock::now().time_since_epoch()).count()) * 100ULL;  // 100 MHz ticks
+#endif
+}
+
+FAV_HD int flag_load(int* flag) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+    return reinterpret_cast<std::atomic<int>*>(flag)->load(std::memory_order_relaxed);
+#endif
+}
+
+FAV_HD void flag_store(int* flag) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+    reinterpret_cast<std::atomic<int>*>(flag)->store(1, std::memory_order_relaxed);
+#endif
+}
+
+FAV_HD void counter_add(unsigned long long* out, unsigned long long v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicAdd(out, v);
+#else
+    reinterpret_cast<std::atomic<unsigned long long>*>(out)->fetch_add(v, std::memory_order_relaxed);
+#endif
+}
+
+FAV_HD void counter_max(unsigned long long* out, unsigned long long v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicMax(out, v);
+#else
+    auto* a = reinterpret_cast<std::atomic<unsigned long long>*>(out);
+    unsigned long long cur = a->load(std::memory_order_relaxed);
+    while (cur < v && !a->compare_exchange_weak(cur, v, std::memory_order_relaxed)) {}
+#endif
+}
+
+template <bool Stats>
+struct Counters {
+    unsigned long long v[kStatsFields];
+    FAV_HD void clear() {
+        for (int i = 0; i < kStatsFields; ++i) v[i] = 0;
+    }
+    FAV_HD void add(int field, unsigned long long n = 1) { v[field] += n; }
+    FAV_HD void max(int field, unsigned long long n) { v[field] = n > v[field] ? n : v[field]; }
+    FAV_HD void flush(unsigned long long* out) {
+        for (int i = 0; i < kStatsFields; ++i) {
+            if (v[i] == 0) continue;
+            if (i == ST_MAX_TAIL_STEPS || i == ST_MAX_PIXEL_ITER) counter_max(out + i, v[i]);
+            else counter_add(out + i, v[i]);
+        }
+    }
+};
+
+template <>
+struct Counters<false> {
+    FAV_HD void clear() {}
+    FAV_HD void add(int, unsigned long long = 1) {}
+    FAV_HD void max(int, unsigned long long) {}
+    FAV_HD void flush(unsigned long long*) {}
+};
+
+// ---- small exact helpers --------------------------------------------------
+
+// fe_mul(x, 2.0) (renderer.cpp:8268-8270) for the normalised operands of the
+// escape margin: the mantissa product x*0.5 lands in [0.25, 0.5), is doubled
+// back exactly and the exponent grows by one.
+FAV_HD FE fe_times_two(FE x) {
+    if (x.m == 0.0) return FE{0.0, 0};
+    if (!isfinite_(x.m)) return FE{x.m, 0};
+    return FE{x.m, sat_exp(static_cast<long long>(x.e) + 1)};
+}
+
+// |Z_i + d|^2 - R^2 = (|Z_i|^2 - R^2) + 2 Re(Z_i conj d) + |d|^2 with the
+// FloatExp parts of Z_i and |Z_i|^2 - R^2 read from the orbit tables
+// (sc_escape_margin_with_delta, renderer.cpp:8383-8431).
+FAV_HD FE margin_at(const OrbitDev& o, int i, SC delta) {
+    const FE zr{o.zr_m[i], o.zr_e[i]}, zi{o.zi_m[i], o.zi_e[i]};
+    const FEC d = sc_parts(delta);
+    const FE cross = fe_times_two(fe_add(fe_mul(zr, d.re), fe_mul(zi, d.im)));
+    const FE dn = fe_add(fe_mul(d.re, d.re), fe_mul(d.im, d.im));
+    return fe_add(fe_add(FE{o.margin_m[i], o.margin_e[i]}, cross), dn);
+}
+
+FAV_HD SC load_z(const OrbitDev& o, int i) { return SC{o.re[i], o.im[i], o.e[i]}; }
+
+FAV_HD float encode_iteration(int iteration, double bias) {
+    return static_cast<float>(static_cast<double>(iteration) - bias);
+}
+
+// smooth_escape_value / smooth_escape_scaled, renderer.cpp:8170-8188.  The
+// reference takes these logarithms in x87 long double; FP64 differs by at
+// most one ulp of the float32 result.
+FAV_HD float smooth_escape(int iteration, SN norm, double bias) {
+    double value = static_cast<double>(iteration);
+    if (norm.m != 0.0) {
+        const double lm = 0.5 * (::log(norm.m) + static_cast<double>(norm.e) * kLn2);
+        if (lm > 0.0 && isfinite_(lm)) value = static_cast<double>(iteration) - ::log(lm) / kLn2;
+    }
+    return static_cast<float>(value - bias);
+}
+
+struct PlaneOut {
+    int64_t* iteration;
+    double *phase, *de_x, *de_y, *test1, *test2;
+};
+
+FAV_HD void clear_planes(const PlaneOut& p, size_t i, int max_iter) {  // renderer.cpp:8084-8096
+    p.iteration[i] = max_iter;
+    p.phase[i] = 0.0; p.de_x[i] = 0.0; p.de_y[i] = 0.0;
+    p.test1[i] = 0.0; p.test2[i] = 0.0;
+}
+
+// render_plane_de, renderer.cpp:7826-7896 (Kalles orientation z*conj(dz)).
+FAV_HD bool plane_de(SC total, SC derivative, FE spacing, double& out_re, double& out_im) {
+    const double cm = ::hypot(total.re, total.im);
+    if (!(cm > 0.0) || !isfinite_(cm)) return false;
+    const double lm = ::log(cm) + static_cast<double>(total.e) * kLn2;
+    if (!(lm > 0.0) || !isfinite_(lm)) return false;
+    const SC unit = sc_from_fe(fe_from_parts(total.re / cm, 0), fe_from_parts(total.im / cm, 0));
+    const SC den = sc_mul(unit, sc_mul(sc_conj(derivative), sc_from_fe(spacing, fe_zero())));
+    const double den2 = den.re * den.re + den.im * den.im;
+    if (!(den2 > 0.0) || !isfinite_(den2)) return false;
+    SC r{cm * lm * den.re / den2, -cm * lm * den.im / den2,
+         sat_exp(static_cast<long long>(total.e) - den.e)};
+    sc_normalize(r);
+    out_re = sc_re_double(r);
+    out_im = sc_im_double(r);
+    return isfinite_(out_re) && isfinite_(out_im);
+}
+
+FAV_HD double plane_phase(SC v) {  // renderer.cpp:8075-8082
+    const double re = sc_re_double(v), im = sc_im_double(v);
+    if (!isfinite_(re) || !isfinite_(im)) return 0.0;
+    double phase = ::atan2(im, re) / kTwoPi;
+    phase -= ::floor(phase);
+    return isfinite_(phase) ? phase : 0.0;
+}
+
+FAV_HD void store_escape(const PlaneOut& p, size_t i, int iteration, SC total, SN test1, SN test2,
+                         SC derivative, FE spacing) {  // renderer.cpp:8106-8135
+    p.iteration[i] = iteration - 1 > 0 ? iteration - 1 : 0;  // Kalles antal = n-1
+    p.phase[i] = plane_phase(total);
+    double dx = 0.0, dy = 0.0;
+    if (!plane_de(total, derivative, spacing, dx, dy)) { dx = 0.0; dy = 0.0; }
+    p.de_x[i] = dx;
+    p.de_y[i] = dy;
+    p.test1[i] = sn_to_double(test1);
+    p.test2[i] = sn_to_double(test2);
+}
+
+struct Budget {  // renderer.cpp:11872-11890
+    const RenderParams* p;
+    unsigned ticks;
+    FAV_HD bool expired() {
+        if (p->budget_ticks == 0) return false;
+        if ((ticks++ & 255U) != 0U) return false;
+        if (flag_load(p->budget_flag)) return true;
+        if (clock_ticks() - *p->budget_start >= p->budget_ticks) {
+            flag_store(p->budget_flag);
+            return true;
+        }
+        return false;
+    }
+};
+
+FAV_HD int highest_level(int max_length) {  // floor(log2(max_length))
+    return max_length > 0 ? 31 - __builtin_clz(static_cast<unsigned>(max_length)) : 0;
+}
+
+// BlaLevels::lookup_linear / lookup, renderer.cpp:8816-8877: the highest
+// aligned level whose block is longer than one step and whose validity
+// radius exceeds |delta|^2.
+template <typename Step>
+FAV_HD int lookup_level(const Step* steps, const LevelDir& d, int start, FE dn, int max_length) {
+    if (start <= 0 || max_length <= 0 || !fe_finite(dn)) return -1;
+    const int base = d.count ? d.size[0] : 0;
+    if (start > base) return -1;
+    int h = highest_level(max_length);
+    if (h > d.count - 1) h = d.count - 1;
+    const int offset = start - 1;
+    for (int level = h; level >= 0; --level) {
+        if ((offset & ((1 << level) - 1)) != 0) continue;
+        const int index = offset >> level;
+        if (index >= d.size[level]) continue;
+        const int64_t k = d.offset[level] + index;
+        const Step& s = steps[k];
+        const FE r2{s.r2_m, s.r2_e};
+        if (s.length > 1 && fe_finite(r2) && fe_compare(dn, r2) < 0) return static_cast<int>(k);
+    }
+    return -1;
+}
+
+FAV_HD SC coef(const CubicStepDev& s, int i) { return SC{s.re[i], s.im[i], s.e[i]}; }
+
+// apply_bla_series, renderer.cpp:8686-8731.
+FAV_HD SC apply_cubic(const CubicStepDev& s, SC d, SC c, int order) {
+    if (order <= 1) return sc_add(sc_mul(coef(s, 0), d), sc_mul(coef(s, 1), c));
+    if (order == 2) {
+        const SC d_inner = sc_add(sc_mul(coef(s, 2), d), sc_mul(coef(s, 3), c));
+        const SC c_inner = sc_add(coef(s, 1), sc_mul(coef(s, 4), c));
+        return sc_add(sc_mul(d, sc_add(coef(s, 0), d_inner)), sc_mul(c, c_inner));
+    }
+    const SC d_cubic = sc_add(sc_add(sc_mul(coef(s, 5), d), sc_mul(coef(s, 6), c)), coef(s, 2));
+    const SC c_cubic = sc_add(coef(s, 3), sc_mul(coef(s, 7), c));
+    const SC d_inner = sc_add(coef(s, 0), sc_add(sc_mul(d, d_cubic), sc_mul(c, c_cubic)));
+    const SC c_inner = sc_add(coef(s, 1), sc_mul(c, sc_add(coef(s, 4), sc_mul(coef(s, 8), c))));
+    return sc_add(sc_mul(d, d_inner), sc_mul(c, c_inner));
+}
+
+// apply_bla_series_derivative, renderer.cpp:8733-8795.
+FAV_HD SC apply_cubic_derivative(const CubicStepDev& s, SC d, SC c, SC der, int order) {
+    if (order <= 1) return sc_add(sc_mul(coef(s, 0), der), coef(s, 1));
+    SC pd = coef(s, 0), pp = coef(s, 1);
+    pd = sc_add(pd, sc_add(sc_double(sc_mul(coef(s, 2), d)), sc_mul(coef(s, 3), c)));
+    pp = sc_add(pp, sc_add(sc_mul(coef(s, 3), d), sc_double(sc_mul(coef(s, 4), c))));
+    if (order >= 3) {
+        const SC d2 = sc_mul(d, d);
+        const SC c2 = sc_mul(c, c);
+        pd = sc_add(pd, sc_add(sc_mul(sc_add(sc_double(sc_mul(coef(s, 5), d)), sc_mul(coef(s, 5), d)), d),
+                               sc_add(sc_double(sc_mul(coef(s, 6), sc_mul(d, c))), sc_mul(coef(s, 7), c2))));
+        pp = sc_add(pp, sc_add(sc_mul(coef(s, 6), d2),
+                               sc_add(sc_double(sc_mul(coef(s, 7), sc_mul(d, c))),
+                                      sc_mul(sc_add(sc_double(sc_mul(coef(s, 8), c)), sc_mul(coef(s, 8), c)), c))));
+    }
+    return sc_add(sc_mul(pd, der), pp);
+}
+
+constexpr SC kOne{1.0, 0.0, 0};
+
+// render_scaled_double_tail, renderer.cpp:12450-12669.  Returns true when
+// the tail gave up (step ceiling or rebase storm).
+template <bool Planes, bool Stats>
+FAV_HD bool double_tail(const RenderParams& p, const PlaneOut& planes, size_t index, SC dc, int& iteration,
+                        int& ref, SC delta, SC& derivative, Budget& budget, bool& deadline, bool& unresolved,
+                        Counters<Stats>& st, SN initial_norm, float& out) {
+    const OrbitDev& o = p.orbit;
+    const double dc_re = sc_re_double(dc), dc_im = sc_im_double(dc);
+    const double escape_squared = p.escape_radius_mode == 1 ? 1.0e8 : 4.0;
+    const double safe_escape_squared = escape_squared + 1.0e-7;
+    double d_re = sc_re_double(delta), d_im = sc_im_double(delta);
+    out = encode_iteration(p.max_iter, p.output_bias);
+    double tortoise_re = 0.0, tortoise_im = 0.0;
+    int cycle_power = 1, cycle_length = 0, tail_steps = 0, tail_iterations = 0, tail_rebases = 0;
+    bool cycle_ready = false;
+    SN previous_norm = initial_norm;
+    const int orbit_size = o.size;
+    while (iteration < p.max_iter && ref >= 0 && ref + 1 < orbit_size) {
+        if (budget.expired()) {
+            out = __builtin_nan(""); deadline = true; unresolved = true;
+            st.add(ST_DEADLINE); st.add(ST_UNRESOLVED);
+            return false;
+        }
+        const double z_re = o.real_double[ref], z_im = o.imag_double[ref];
+        if constexpr (Planes) {
+            SC prior_total{z_re + d_re, z_im + d_im, 0};
+            sc_normalize(prior_total);
+            derivative = sc_add(sc_double(sc_mul(prior_total, derivative)), kOne);
+        }
+        const double lin_re = 2.0 * (z_re * d_re - z_im * d_im);
+        const double lin_im = 2.0 * (z_re * d_im + z_im * d_re);
+        const double sq_re = d_re * d_re - d_im * d_im;
+        const double sq_im = 2.0 * d_re * d_im;
+        d_re = lin_re + sq_re + dc_re;
+        d_im = lin_im + sq_im + dc_im;
+        ++ref;
+        ++iteration;
+        if (++tail_iterations > 65536) {
+            st.add(ST_TAIL_FALLBACKS); st.add(ST_UNRESOLVED);
+            unresolved = true;
+            out = p.strict ? __builtin_nan("") : encode_iteration(iteration, p.output_bias);
+            return true;
+        }
+        st.add(ST_LOGICAL); st.add(ST_EXACT);
+        const double t_re = o.real_double[ref] + d_re;
+        const double t_im = o.imag_double[ref] + d_im;
+        const double m2 = t_re * t_re + t_im * t_im;
+        if (!isfinite_(m2)) {
+            unresolved = true;
+            st.add(ST_GLITCH); st.add(ST_UNRESOLVED);
+            out = p.strict ? __builtin_nan("") : encode_iteration(iteration, p.output_bias);
+            return false;
+        }
+        if (m2 > escape_squared) {
+            const double magnitude = ::sqrt(m2 > safe_escape_squared ? m2 : safe_escape_squared);
+            const double q = ::log(::log(magnitude)) / kLn2;
+            out = static_cast<float>((static_cast<double>(iteration) - q) - p.output_bias);
+            if constexpr (Planes) {
+                SC escaped_total{t_re, t_im, 0};
+                sc_normalize(escaped_total);
+                store_escape(planes, index, iteration, escaped_total, sn_from_double(m2), previous_norm,
+                             derivative, FE{p.pixel_spacing_m, p.pixel_spacing_e});
+            }
+            return false;
+        }
+        previous_norm = sn_from_double(m2);
+        const double dm2 = d_re * d_re + d_im * d_im;
+        ++tail_steps;
+        if (p.cycle_enabled && tail_steps >= 64 && (tail_steps & 31) == 0) {
+            if (!cycle_ready) {
+                tortoise_re = t_re; tortoise_im = t_im;
+                cycle_power = 1; cycle_length = 0; cycle_ready = true;
+            } else {
+                const double cdr = t_re - tortoise_re, cdi = t_im - tortoise_im;
+                const double cd2 = cdr * cdr + cdi * cdi;
+                const int cmin = p.strict_cycle ? 2048 : 512;
+                const double ctol = p.strict_cycle ? 1.0e-24 : 1.0e-18;
+                if (iteration >= cmin && m2 < 3.0 && cd2 <= ctol * (1.0 < m2 ? m2 : 1.0)) {
+                    if constexpr (Planes) clear_planes(planes, index, p.max_iter);
+                    return false;
+                }
+                ++cycle_length;
+                if (cycle_length >= cycle_power) {
+                    tortoise_re = t_re; tortoise_im = t_im;
+                    cycle_power = cycle_power * 2 < (1 << 20) ? cycle_power * 2 : (1 << 20);
+                    cycle_length = 0;
+                }
+            }
+        }
+        if (m2 < dm2) {
+            d_re = t_re; d_im = t_im;
+            ref = 0;
+            tail_steps = 0;
+            cycle_ready = false;
+            st.add(ST_TAIL_REBASES);
+            if (++tail_rebases > 64) {
+                st.add(ST_TAIL_FALLBACKS); st.add(ST_UNRESOLVED);
+                unresolved = true;
+                out = p.strict ? __builtin_nan("") : encode_iteration(iteration, p.output_bias);
+                if constexpr (Planes) clear_planes(planes, index, p.max_iter);
+                return true;
+            }
+        }
+    }
+    if (iteration < p.max_iter) {
+        unresolved = true;
+        st.add(ST_UNRESOLVED);
+        out = p.strict ? __builtin_nan("") : encode_iteration(iteration, p.output_bias);
+        if constexpr (Planes) clear_planes(planes, index, p.max_iter);
+    }
+    return false;
+}
+
+// Direct scaled recurrence after the reference escaped or after a rebase
+// storm, renderer.cpp:13044-13128 / 13155-13250.
+template <bool Planes, bool Stats>
+FAV_HD void exact_tail(const RenderParams& p, const PlaneOut& planes, size_t index, SC dc, int& iteration,
+                       SC total, SN total_norm, SC& derivative, Budget& budget, bool& deadline, bool& unresolved,
+                       Counters<Stats>& st, float& out) {
+    const SC parameter = sc_add(load_z(p.orbit, 1), dc);
+    const int margin = p.strict_cycle ? 78 : 60;
+    const int cmin = p.strict_cycle ? 2048 : 512;
+    SC tortoise = sc_zero();
+    int power = 1, length = 0;
+    bool ready = false, escaped = false;
+    while (iteration < p.max_iter) {
+        if (budget.expired()) {
+            out = __builtin_nan(""); deadline = true; unresolved = true;
+            st.add(ST_DEADLINE); st.add(ST_UNRESOLVED);
+            return;
+        }
+        const SN prior_norm = total_norm;
+        if constexpr (Planes) derivative = sc_add(sc_double(sc_mul(total, derivative)), kOne);
+        total = sc_add(sc_mul(total, total), parameter);
+        ++iteration;
+        st.add(ST_LOGICAL); st.add(ST_EXACT);
+        total_norm = sc_norm_squared(total);
+        if (sc_outside_escape(total_norm, p.escape_radius_mode)) {
+            out = smooth_escape(iteration, total_norm, p.output_bias);
+            if constexpr (Planes)
+                store_escape(planes, index, iteration, total, total_norm, prior_norm, derivative,
+                             FE{p.pixel_spacing_m, p.pixel_spacing_e});
+            escaped = true;
+            break;
+        }
+        if (p.cycle_enabled && iteration >= cmin && (iteration & 31) == 0
+            && sc_compare_norm(total_norm, SN{0.75, 2}) < 0) {
+            if (!ready) {
+                tortoise = total; power = 1; length = 0; ready = true;
+            } else {
+                const SN distance = sc_norm_squared(sc_sub(total, tortoise));
+                const int scale_exponent = total_norm.e > 0 ? total_norm.e : 0;
+                if (distance.m == 0.0 || distance.e <= scale_exponent - margin) {
+                    out = encode_iteration(p.max_iter, p.output_bias);
+                    escaped = true;
+                    break;
+                }
+                ++length;
+                if (length >= power) {
+                    tortoise = total;
+                    power = power * 2 < (1 << 20) ? power * 2 : (1 << 20);
+                    length = 0;
+                }
+            }
+        }
+    }
+    if (!escaped) out = encode_iteration(p.max_iter, p.output_bias);
+}
+
+// One pixel of render_bla_impl, renderer.cpp:12817-13544.
+template <bool Planes, bool Stats>
+FAV_HD void render_pixel(const RenderParams& p, const PlaneOut& planes, size_t index, SC dc, Counters<Stats>& st) {
+    const OrbitDev& o = p.orbit;
+    const int max_iter = p.max_iter;
+    const int mode = p.escape_radius_mode;
+    const FE spacing{p.pixel_spacing_m, p.pixel_spacing_e};
+    st.add(ST_PIXELS);
+    if constexpr (Planes) clear_planes(planes, index, max_iter);
+    SC delta = dc;
+    SC derivative = kOne;
+    int ref = 1, iteration = 1;
+    if (p.series.available
+        && sc_compare_norm(sc_norm_squared(dc), SN{p.series.radius2_m, p.series.radius2_e}) <= 0) {
+        // evaluate_image_series_with_derivative, renderer.cpp:8611-8639
+        const int n = p.series.count;
+        SC value{p.series.re[n - 1], p.series.im[n - 1], p.series.e[n - 1]};
+        SC pd = sc_zero();
+        for (int k = n - 1; k > 1; --k) {
+            if constexpr (Planes) pd = sc_add(sc_mul(pd, dc), value);
+            value = sc_add(sc_mul(value, dc), SC{p.series.re[k - 1], p.series.im[k - 1], p.series.e[k - 1]});
+        }
+        if constexpr (Planes) derivative = sc_add(sc_mul(pd, dc), value);
+        delta = sc_mul(value, dc);
+        ref = p.series.iteration;
+        iteration = p.series.iteration;
+        st.add(ST_SERIES_PIXELS);
+        st.add(ST_SERIES_JUMPS, static_cast<unsigned long long>(iteration));
+    }
+    bool escaped = false, have_total = false;
+    SC total = sc_zero();
+    SN total_norm{0.0, 0}, previous_total_norm{0.0, 0};
+    FE margin{0.0, 0};  // margin of (Z[ref], delta) when have_total
+    SC tortoise = sc_zero();
+    int cycle_power = 1, cycle_length = 0;
+    bool cycle_ready = false;
+    bool pixel_disable_bla = false;
+    int rebases = 0;
+    Budget budget{&p, 0U};
+    bool deadline = false, unresolved = false;
+    float out = 0.0f;
+    const int cycle_min = p.strict_cycle ? 2048 : 512;
+    const int cycle_margin = p.strict_cycle ? 78 : 60;
+
+    while (iteration < max_iter) {
+        if (budget.expired()) {
+            out = __builtin_nan(""); deadline = true; unresolved = true;
+            st.add(ST_DEADLINE); st.add(ST_UNRESOLVED);
+            break;
+        }
+        if (ref < 0 || ref >= o.size) {
+            out = encode_iteration(iteration, p.output_bias);
+            if constexpr (Planes)
+                store_escape(planes, index, iteration, total, total_norm, previous_total_norm, derivative, spacing);
+            escaped = true;
+            break;
+        }
+        if (!have_total) {
+            total = sc_add(load_z(o, ref), delta);
+            margin = margin_at(o, ref, delta);
+            total_norm = sn_from_margin(margin, mode);
+        }
+        have_total = false;
+        // glitch: |Z+d|^2 < 1e-7 |Z|^2, renderer.cpp:12931-12988 (the
+        // threshold table holds 0 where |Z|^2 is zero or not finite)
+        const FE glitch{o.glitch_m[ref], o.glitch_e[ref]};
+        if (glitch.m != 0.0 && fe_compare(FE{total_norm.m, total_norm.e}, glitch) < 0) {
+            st.add(ST_GLITCH);
+            if (p.allow_recovery && sc_finite(total) && ++rebases <= 64) {
+                delta = total;
+                ref = 0;
+                continue;
+            }
+            out = __builtin_nan("");
+            unresolved = true;
+            st.add(ST_UNRESOLVED);
+            break;
+        }
+        // escape, renderer.cpp:12989-13006
+        if (sc_outside_escape(total_norm, mode) || fe_compare(margin, fe_zero()) > 0) {
+            out = smooth_escape(iteration, total_norm, p.output_bias);
+            if constexpr (Planes)
+                store_escape(planes, index, iteration, total, total_norm, previous_total_norm, derivative, spacing);
+            escaped = true;
+            break;
+        }
+        // Brent cycle check, renderer.cpp:13016-13042
+        if (p.cycle_enabled && iteration >= cycle_min && (iteration & 31) == 0
+            && sc_compare_norm(total_norm, SN{0.75, 2}) < 0) {
+            if (!cycle_ready) {
+                tortoise = total; cycle_power = 1; cycle_length = 0; cycle_ready = true;
+            } else {
+                const SN distance = sc_norm_squared(sc_sub(total, tortoise));
+                const int scale_exponent = total_norm.e > 0 ? total_norm.e : 0;
+                if (distance.m == 0.0 || distance.e <= scale_exponent - cycle_margin) {
+                    st.add(ST_CYCLE_INSIDE);
+                    out = encode_iteration(max_iter, p.output_bias);
+                    escaped = true;
+                    break;
+                }
+                ++cycle_length;
+                if (cycle_length >= cycle_power) {
+                    tortoise = total;
+                    cycle_power = cycle_power * 2 < (1 << 20) ? cycle_power * 2 : (1 << 20);
+                    cycle_length = 0;
+                }
+            }
+        }
+        // rebase when |Z+d| < |d|, renderer.cpp:13047-13153
+        const SN delta_norm = sc_norm_squared(delta);
+        if (sc_compare_norm(total_norm, delta_norm) < 0) {
+            if (++rebases > 64) {
+                exact_tail<Planes, Stats>(p, planes, index, dc, iteration, total, total_norm, derivative, budget,
+                                          deadline, unresolved, st, out);
+                if (!deadline) escaped = true;
+                break;
+            }
+            delta = total;
+            ref = 0;
+            continue;
+        }
+        // the reference orbit escaped: direct recurrence, renderer.cpp:13155-13250
+        if (o.map_end > 1 && ref >= o.map_end) {
+            exact_tail<Planes, Stats>(p, planes, index, dc, iteration, total, total_norm, derivative, budget,
+                                      deadline, unresolved, st, out);
+            if (!deadline) escaped = true;
+            break;
+        }
+        // BLA selection, renderer.cpp:13252-13273
+        const FE dn{delta_norm.m, delta_norm.e};
+        const int remaining = max_iter - iteration;
+        const int order = p.approximation_order >= 3 && delta_norm.e < -115
+            ? (delta_norm.e < -160 ? 1 : 2) : p.approximation_order;
+        int linear_k = -1, cubic_k = -1, map_length = 0;
+        if (!p.disable_bla && !pixel_disable_bla) {
+            if (order <= 1) {
+                linear_k = lookup_level(p.linear, p.linear_dir, ref, dn,
+                                        p.max_linear_length < remaining ? p.max_linear_length : remaining);
+                if (linear_k >= 0) map_length = p.linear[linear_k].length;
+            } else {
+                cubic_k = lookup_level(p.cubic, p.cubic_dir, ref, dn,
+                                       p.max_bla_length < remaining ? p.max_bla_length : remaining);
+                if (cubic_k >= 0) map_length = p.cubic[cubic_k].length;
+            }
+        }
+        // plain-double tail, renderer.cpp:13283-13362
+        if (!p.disable_bla && !pixel_disable_bla && (map_length <= 0 || ref + map_length > max_iter)
+            && delta_norm.e > -90) {
+            st.add(ST_TAIL_PIXELS);
+            const int tail_start = iteration;
+            const bool pathological = double_tail<Planes, Stats>(p, planes, index, dc, iteration, ref, delta,
+                                                                 derivative, budget, deadline, unresolved, st,
+                                                                 total_norm, out);
+            if (deadline || unresolved) break;
+            const unsigned long long steps =
+                static_cast<unsigned long long>(iteration - tail_start > 0 ? iteration - tail_start : 0);
+            st.add(ST_TAIL_STEPS, steps);
+            st.max(ST_MAX_TAIL_STEPS, steps);
+            if (pathological) {
+                delta = dc;
+                derivative = kOne;
+                ref = 1;
+                iteration = 1;
+                have_total = false;
+                pixel_disable_bla = true;
+                continue;
+            }
+            escaped = true;
+            break;
+        }
+        // BLA jump with endpoint guard and replay, renderer.cpp:13363-13514
+        if (map_length > 0 && ref + map_length <= max_iter) {
+            st.add(ST_BLA_BLOCKS);
+            st.add(linear_k >= 0 ? ST_LINEAR_BLOCKS : ST_CUBIC_BLOCKS);
+            st.add(ST_SERIES_JUMPS);
+            if constexpr (Stats) {
+                int bucket = 0;
+                unsigned v = static_cast<unsigned>(map_length);
+                while (v > 1U && bucket < 15) { v >>= 1U; ++bucket; }
+                st.add(ST_HISTOGRAM + bucket);
+            }
+            const SC previous_delta = delta, previous_derivative = derivative;
+            const int previous_ref = ref, previous_iteration = iteration;
+            if (linear_k >= 0) {
+                const LinearStepDev& s = p.linear[linear_k];
+                const SC a{s.a_re, s.a_im, s.a_e}, b{s.b_re, s.b_im, s.b_e};
+                delta = sc_add(sc_mul(a, previous_delta), sc_mul(b, dc));
+                if constexpr (Planes) derivative = sc_add(sc_mul(a, previous_derivative), b);
+            } else {
+                const CubicStepDev& s = p.cubic[cubic_k];
+                delta = apply_cubic(s, previous_delta, dc, order);
+                if constexpr (Planes)
+                    derivative = apply_cubic_derivative(s, previous_delta, dc, previous_derivative, order);
+            }
+            ref += map_length;
+            iteration += map_length;
+            const SC endpoint = sc_add(load_z(o, ref), delta);
+            const FE endpoint_margin = margin_at(o, ref, delta);
+            const SN endpoint_norm = sn_from_margin(endpoint_margin, mode);
+            if (!sc_finite(delta) || !sc_finite(endpoint)) {
+                delta = previous_delta; derivative = previous_derivative;
+                ref = previous_ref; iteration = previous_iteration;
+                st.add(ST_BLA_RETRIES);
+                if (!pixel_disable_bla) st.add(ST_BLA_DISABLED);
+                pixel_disable_bla = true;
+                have_total = false;
+                continue;
+            }
+            if (sc_compare_norm(endpoint_norm, SN{0.75, 2}) >= 0) {
+                delta = previous_delta; derivative = previous_derivative;
+                ref = previous_ref; iteration = previous_iteration;
+                SC replay_total = sc_zero();
+                SN replay_norm{0.0, 0};
+                FE replay_margin{0.0, 0};
+                bool retry = false;
+                for (int replay = 0; replay < map_length && iteration < max_iter; ++replay) {
+                    if (budget.expired()) {
+                        out = __builtin_nan(""); deadline = true; unresolved = true;
+                        st.add(ST_DEADLINE); st.add(ST_UNRESOLVED);
+                        break;
+                    }
+                    const SC zc = load_z(o, ref);
+                    if constexpr (Planes)
+                        derivative = sc_add(sc_double(sc_mul(sc_add(zc, delta), derivative)), kOne);
+                    delta = sc_add(sc_double(sc_mul(zc, delta)), sc_add(sc_mul(delta, delta), dc));
+                    ++ref;
+                    ++iteration;
+                    st.add(ST_REPLAY); st.add(ST_LOGICAL);
+                    replay_total = sc_add(load_z(o, ref), delta);
+                    replay_margin = margin_at(o, ref, delta);
+                    replay_norm = sn_from_margin(replay_margin, mode);
+                    if (!sc_finite(delta) || !sc_finite(replay_total)) { retry = true; break; }
+                    if (sc_outside_escape(replay_norm, mode) || fe_compare(replay_margin, fe_zero()) > 0) {
+                        out = smooth_escape(iteration, replay_norm, p.output_bias);
+                        if constexpr (Planes)
+                            store_escape(planes, index, iteration, replay_total, replay_norm, total_norm, derivative,
+                                         spacing);
+                        escaped = true;
+                        break;
+                    }
+                }
+                if (deadline || escaped) break;
+                if (retry) {
+                    delta = previous_delta; derivative = previous_derivative;
+                    ref = previous_ref; iteration = previous_iteration;
+                    st.add(ST_BLA_RETRIES);
+                    if (!pixel_disable_bla) st.add(ST_BLA_DISABLED);
+                    pixel_disable_bla = true;
+                    have_total = false;
+                    continue;
+                }
+                total = replay_total;
+                total_norm = replay_norm;
+                margin = replay_margin;
+                previous_total_norm = total_norm;
+                have_total = true;
+            } else {
+                total = endpoint;
+                total_norm = endpoint_norm;
+                margin = endpoint_margin;
+                previous_total_norm = total_norm;
+                have_total = true;
+                st.add(ST_LOGICAL, static_cast<unsigned long long>(map_length));
+            }
+            continue;
+        }
+        // exact perturbation step, renderer.cpp:13516-13535
+        const SC z = load_z(o, ref);
+        const SN prior_norm = total_norm;
+        if constexpr (Planes) derivative = sc_add(sc_double(sc_mul(sc_add(z, delta), derivative)), kOne);
+        delta = sc_add(sc_double(sc_mul(z, delta)), sc_add(sc_mul(delta, delta), dc));
+        ++ref;
+        ++iteration;
+        previous_total_norm = prior_norm;
+        st.add(ST_LOGICAL); st.add(ST_EXACT);
+    }
+    if (!escaped && !deadline && !unresolved) out = encode_iteration(max_iter, p.output_bias);
+    p.output[index] = out;
+    st.max(ST_MAX_PIXEL_ITER, static_cast<unsigned long long>(iteration > 0 ? iteration : 0));
+}
+
+// The pixel at (px, py): dc from the per-column / per-row offsets
+// (renderer.cpp:12838-12842), then the state machine.
+template <bool Planes, bool Stats>
+FAV_HD void render_one(const RenderParams& p, int px, int py) {
+    const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
+    const SC dc = sc_from_fe(FE{p.x_off_m[px], p.x_off_e[px]}, FE{p.y_off_m[py], p.y_off_e[py]});
+    const PlaneOut planes{p.plane_iteration, p.plane_phase, p.plane_de_x, p.plane_de_y, p.plane_test1,
+                          p.plane_test2};
+    Counters<Stats> st;
+    st.clear();
+    render_pixel<Planes, Stats>(p, planes, index, dc, st);
+    if constexpr (Stats) st.flush(p.stats);
+}
+
+}  // namespace core
+}  // namespace fav

@@ -1,0 +1,90 @@
+// host_check.cpp -- TEST-ONLY CPU build of the field kernel's per-pixel code.
+//
+// Links the product's host table builders with field_core.h compiled for the
+// CPU, so tests can check the kernel's logic (and any rewrite of it) against
+// the oracle bit for bit without a GPU.  It is built into its own shared
+// object (_build/libfav_host_check.so) that the product library never loads.
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "field_core.h"
+#include "table_image.h"
+
+extern "C" int fav_host_check_render(
+    const char* x, const char* y, const char* ref_zoom, int max_iter, int bits, int image_series_order,
+    const char* clone_zoom, const FractalRenderOptions* options, float* output, int width, int height,
+    const char* zoom, int series_order, int series_block, FractalRenderPlanes* planes, uint64_t* stats,
+    char* error, int error_capacity) {
+    using namespace fav;
+    try {
+        auto root = build_reference(x, y, ref_zoom, max_iter, bits, image_series_order, clone_zoom != nullptr, 0,
+                                    options->escape_radius_mode, options->coordinate_mode);
+        std::unique_ptr<ReferenceContext> clone;
+        if (clone_zoom) clone = clone_reference(*root, clone_zoom);
+        ReferenceContext& c = clone ? *clone : *root;
+        TableImage img;
+        make_table_image(c, img);
+        FieldRequest request{output, width, height, zoom, max_iter, series_order, series_block, *options,
+                             planes, stats, false};
+        PreparedRender prep;
+        prepare_render(c, request, prep);
+        RenderParams p = prep.params;
+        const OrbitData& o = *c.orbit;
+        p.orbit.re = img.re.data();
+        p.orbit.im = img.im.data();
+        p.orbit.e = img.e.data();
+        p.orbit.zr_m = o.zr_m.data();
+        p.orbit.zi_m = o.zi_m.data();
+        p.orbit.zr_e = o.zr_e.data();
+        p.orbit.zi_e = o.zi_e.data();
+        p.orbit.margin_m = o.margin_m.data();
+        p.orbit.margin_e = o.margin_e.data();
+        p.orbit.glitch_m = o.glitch_m.data();
+        p.orbit.glitch_e = o.glitch_e.data();
+        p.orbit.real_double = o.real_double.data();
+        p.orbit.imag_double = o.imag_double.data();
+        p.series.re = img.series_re.data();
+        p.series.im = img.series_im.data();
+        p.series.e = img.series_e.data();
+        p.linear = img.linear[prep.linear_tier].data();
+        p.linear_dir = img.linear_dir[prep.linear_tier];
+        p.cubic = img.cubic.data();
+        p.cubic_dir = img.cubic_dir;
+        p.x_off_m = prep.offset_m.data();
+        p.y_off_m = prep.offset_m.data() + width;
+        p.x_off_e = prep.offset_e.data();
+        p.y_off_e = prep.offset_e.data() + width;
+        p.output = output;
+        if (planes) {
+            p.plane_iteration = planes->orbit_iteration;
+            p.plane_phase = planes->phase;
+            p.plane_de_x = planes->de_x;
+            p.plane_de_y = planes->de_y;
+            p.plane_test1 = planes->test1;
+            p.plane_test2 = planes->test2;
+        }
+        std::vector<unsigned long long> counters(kStatsFields, 0);
+        p.stats = counters.data();
+        int flag = 0;
+        unsigned long long start = 0;
+        p.budget_flag = &flag;
+        p.budget_start = &start;
+        p.budget_ticks = 0;
+        const long long n = static_cast<long long>(width) * height;
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long long i = 0; i < n; ++i) {
+            const int py = static_cast<int>(i / width), px = static_cast<int>(i % width);
+            if (planes && stats) core::render_one<true, true>(p, px, py);
+            else if (planes) core::render_one<true, false>(p, px, py);
+            else if (stats) core::render_one<false, true>(p, px, py);
+            else core::render_one<false, false>(p, px, py);
+        }
+        if (stats) for (int i = 0; i < kStatsFields; ++i) stats[i] = counters[static_cast<size_t>(i)];
+        return 0;
+    } catch (const std::exception& e) {
+        std::snprintf(error, static_cast<size_t>(error_capacity), "%s", e.what());
+        return 1;
+    }
+}

@@ -1,0 +1,113 @@
+// table_image.cpp -- flat table images and per-render launch parameters
+// (renderer.cpp:12689-12783 for the viewport and offsets).
+#include "table_image.h"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace fav {
+
+namespace {
+
+template <typename Step>
+void flatten(const std::vector<std::vector<Step>>& levels, std::vector<Step>& flat, LevelDir& dir) {
+    if (levels.size() > static_cast<size_t>(kMaxLevels))
+        throw std::runtime_error("BLA hierarchy deeper than the device level directory");
+    flat.clear();
+    dir = LevelDir{};
+    dir.count = static_cast<int32_t>(levels.size());
+    for (size_t l = 0; l < levels.size(); ++l) {
+        dir.offset[l] = static_cast<int64_t>(flat.size());
+        dir.size[l] = static_cast<int32_t>(levels[l].size());
+        flat.insert(flat.end(), levels[l].begin(), levels[l].end());
+    }
+}
+
+}  // namespace
+
+void make_table_image(const ReferenceContext& c, TableImage& img) {
+    const OrbitData& o = *c.orbit;
+    const size_t n = o.scaled.size();
+    img.re.resize(n);
+    img.im.resize(n);
+    img.e.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        img.re[i] = o.scaled[i].re;
+        img.im[i] = o.scaled[i].im;
+        img.e[i] = o.scaled[i].e;
+    }
+    flatten(c.cubic.levels, img.cubic, img.cubic_dir);
+    for (int t = 0; t < 2; ++t) flatten(c.linear[t].levels, img.linear[t], img.linear_dir[t]);
+    img.series_re.clear();
+    img.series_im.clear();
+    img.series_e.clear();
+    for (const SC& k : c.series_coefficients) {
+        img.series_re.push_back(k.re);
+        img.series_im.push_back(k.im);
+        img.series_e.push_back(k.e);
+    }
+}
+
+void prepare_render(const ReferenceContext& c, const FieldRequest& r, PreparedRender& out) {
+    const int w = r.width, h = r.height;
+    const FractalRenderOptions& opt = r.options;
+    const FE zoom = parse_zoom(r.zoom_text, c.precision_bits);
+    // viewport, renderer.cpp:12689-12705
+    const double factor = c.coordinate_mode == 1 ? 4.0 : 2.8;
+    const FE inverse_zoom = fe_div(fe_from_parts(1.0, 0), zoom);
+    const FE view_height = fe_muld(inverse_zoom, factor);
+    const FE view_width = fe_muld(view_height, static_cast<double>(w) / static_cast<double>(h));
+    const FE pixel_spacing = fe_div(view_height, fe_from_parts(static_cast<double>(h), 0));
+    const FE current_input_radius = fe_muld(inverse_zoom, factor);
+    const bool covers = fe_compare(current_input_radius, c.input_radius) <= 0;
+    const bool use_deep_linear = fe_compare(current_input_radius, c.deep_input_radius) <= 0;
+    const bool use_ultra = fe_compare(current_input_radius, fe_muld(c.input_radius, 1.0e-40)) <= 0;
+    // per-column / per-row offsets, renderer.cpp:12764-12783
+    out.offset_m.resize(static_cast<size_t>(w) + h);
+    out.offset_e.resize(static_cast<size_t>(w) + h);
+    const bool kalles = opt.coordinate_mode == 1;
+    const double ox = kalles ? static_cast<double>(w / 2) : static_cast<double>(w - 1) / 2.0;
+    const double oy = kalles ? static_cast<double>(h / 2) : static_cast<double>(h - 1) / 2.0;
+    for (int px = 0; px < w; ++px) {
+        const FE v = fe_muld(view_width, (static_cast<double>(px) - ox) / static_cast<double>(w));
+        out.offset_m[static_cast<size_t>(px)] = v.m;
+        out.offset_e[static_cast<size_t>(px)] = v.e;
+    }
+    for (int py = 0; py < h; ++py) {
+        const FE v = fe_muld(view_height, -(static_cast<double>(py) - oy) / static_cast<double>(h));
+        out.offset_m[static_cast<size_t>(w + py)] = v.m;
+        out.offset_e[static_cast<size_t>(w + py)] = v.e;
+    }
+    RenderParams& p = out.params;
+    p = RenderParams{};
+    p.orbit.size = static_cast<int32_t>(c.orbit->scaled.size());
+    p.orbit.map_end = c.map_end;
+    p.series.count = static_cast<int32_t>(c.series_coefficients.size());
+    p.series.iteration = c.series_iteration;
+    p.series.radius2_m = c.series_radius_squared.m;
+    p.series.radius2_e = c.series_radius_squared.e;
+    p.series.available = c.series_enabled && c.series_order >= opt.series_min_terms
+        && c.series_order <= opt.series_max_terms
+        && fe_compare(current_input_radius, fe_sqrt(c.series_radius_squared)) <= 0
+        && c.series_iteration < r.max_iter;
+    out.linear_tier = use_deep_linear ? 1 : 0;
+    p.output_bias = opt.output_bias;
+    p.pixel_spacing_m = pixel_spacing.m;
+    p.pixel_spacing_e = pixel_spacing.e;
+    p.width = w;
+    p.height = h;
+    p.max_iter = r.max_iter;
+    p.escape_radius_mode = opt.escape_radius_mode;
+    p.strict = opt.strict != 0;
+    p.allow_recovery = opt.allow_recovery != 0;
+    p.disable_bla = !covers || opt.disable_bla != 0;
+    p.cycle_enabled = opt.disable_cycle == 0 && (opt.strict == 0 || opt.strict_cycle != 0);
+    p.strict_cycle = opt.strict_cycle != 0;
+    // renderer.cpp:12741-12750
+    p.max_bla_length = std::clamp(std::min(r.series_block, opt.max_bla_length), 2, 64);
+    p.max_linear_length = std::clamp(std::min(r.series_block, opt.max_linear_bla_length), 2,
+                                     use_ultra ? 4096 : (use_deep_linear ? 1024 : 64));
+    p.approximation_order = std::clamp(r.series_order, 1, 3);
+}
+
+}  // namespace fav

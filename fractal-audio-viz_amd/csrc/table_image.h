@@ -1,0 +1,39 @@
+// table_image.h -- host-side flat images of a reference's render tables and
+// the scalar launch parameters of one field render.  Nothing here touches
+// the HIP runtime: device_host.cpp copies the images into HBM, and the
+// test-only CPU harness (host_check.cpp) points the same kernel code at
+// them directly.
+#pragma once
+
+#include <vector>
+
+#include "fav_render.h"
+
+namespace fav {
+
+struct TableImage {
+    std::vector<double> re, im;  // ScaledComplex mantissas of the orbit
+    std::vector<int32_t> e;
+    std::vector<LinearStepDev> linear[2];
+    LevelDir linear_dir[2]{};
+    std::vector<CubicStepDev> cubic;
+    LevelDir cubic_dir{};
+    std::vector<double> series_re, series_im;
+    std::vector<int32_t> series_e;
+};
+
+void make_table_image(const ReferenceContext& context, TableImage& image);
+
+// Per-render parameters: everything in RenderParams except the table, output
+// and scratch pointers, plus the per-column / per-row FloatExp offsets
+// (width entries, then height entries).
+struct PreparedRender {
+    RenderParams params{};
+    int linear_tier = 0;
+    std::vector<double> offset_m;
+    std::vector<int32_t> offset_e;
+};
+
+void prepare_render(const ReferenceContext& context, const FieldRequest& request, PreparedRender& out);
+
+}  // namespace fav

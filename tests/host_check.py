@@ -1,0 +1,38 @@
+"""ctypes access to the test-only CPU build of the field kernel code
+(fractal-audio-viz_amd/_build/libfav_host_check.so)."""
+import ctypes
+
+import numpy as np
+
+import native
+
+HOST_CHECK_SO = native.ROOT / "fractal-audio-viz_amd" / "_build" / "libfav_host_check.so"
+
+
+def load():
+    lib = ctypes.CDLL(str(HOST_CHECK_SO))
+    lib.fav_host_check_render.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_char_p, ctypes.POINTER(native.RenderOptions), native.c_float_p, ctypes.c_int,
+        ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(native.RenderPlanes),
+        native.c_uint64_p, ctypes.c_char_p, ctypes.c_int]
+    lib.fav_host_check_render.restype = ctypes.c_int
+    return lib
+
+
+def render(lib, meta):
+    opts = native.RenderOptions.defaults(backend=2, **meta["options"])
+    w, h = meta["width"], meta["height"]
+    out = np.empty(w * h, dtype=np.float32)
+    planes = native.PlaneArrays(w, h)
+    stats = np.zeros(39, dtype=np.uint64)
+    err = ctypes.create_string_buffer(512)
+    status = lib.fav_host_check_render(
+        meta["x"].encode(), meta["y"].encode(), meta["ref_zoom"].encode(), meta["max_iter"], meta["bits"],
+        meta["image_series_order"], meta["clone_zoom"].encode() if meta["clone_zoom"] else None,
+        ctypes.byref(opts), out.ctypes.data_as(native.c_float_p), w, h, meta["zoom"].encode(),
+        meta["series_order"], meta["series_block"], ctypes.byref(planes.struct),
+        stats.ctypes.data_as(native.c_uint64_p), err, 512)
+    if status != 0:
+        raise RuntimeError(err.value.decode())
+    return out.reshape(h, w), planes, stats
