@@ -17,7 +17,8 @@
 
 namespace fav {
 
-hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, hipStream_t stream);
+hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, int persistent_blocks,
+                               hipStream_t stream);
 
 namespace {
 
@@ -56,7 +57,8 @@ struct LaunchState {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    DeviceBuffer offsets, output, planes, stats, budget;
+    DeviceBuffer offsets, output, planes, stats, budget, state;
+    int compute_units = 256;
     double wall_clock_khz = 0.0;
     LastTiming timing{};
 };
@@ -79,6 +81,9 @@ void init_locked(LaunchState& s) {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s.device) != hipSuccess || khz <= 0)
         khz = 100000;  // 100 MHz constant clock of gfx9 parts
     s.wall_clock_khz = static_cast<double>(khz);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s.device) == hipSuccess && cus > 0)
+        s.compute_units = cus;
     s.ready = true;
 }
 
@@ -233,16 +238,19 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
         p.stats = static_cast<unsigned long long*>(s.stats.get(kStatsFields * 8));
         check(hipMemsetAsync(p.stats, 0, kStatsFields * 8, s.stream), "stats clear");
     }
-    void* budget = s.budget.get(64);
+    void* budget = s.budget.get(256);
     p.budget_start = static_cast<const unsigned long long*>(budget);
     p.budget_flag = reinterpret_cast<int*>(static_cast<char*>(budget) + 8);
     p.budget_ticks = r.options.time_budget_ms > 0
         ? static_cast<unsigned long long>(r.options.time_budget_ms * s.wall_clock_khz)
         : 0ULL;
 
+    bind_state_store(p, s.state.get(pixels * kStateBytesPerPixel + 20 * 256), pixels);
+
     const uint64_t wall0 = now_ns();
     check(hipEventRecord(s.start, s.stream), "event");
-    check(launch_field_kernel(p, want_planes, r.stats != nullptr, s.stream), "field kernel launch");
+    check(launch_field_kernel(p, want_planes, r.stats != nullptr, s.compute_units * 8, s.stream),
+          "field kernel launch");
     check(hipEventRecord(s.stop, s.stream), "event");
     if (!r.device_pointers) {
         check(hipMemcpyAsync(r.output, p.output, pixels * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H field");

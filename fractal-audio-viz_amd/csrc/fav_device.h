@@ -75,8 +75,24 @@ struct SeriesDev {
     int32_t radius2_e;
 };
 
+// Resumable per-pixel state (structure of arrays, one slot per pixel) and
+// the two hand-over queues of the three field kernels.  counters[0]/[1] are
+// the general queue's length and read cursor, counters[2]/[3] the tail
+// queue's.
+struct StateStore {
+    double *d_re, *d_im, *g_re, *g_im, *t_re, *t_im, *tn_m, *pn_m, *mg_m;
+    int32_t *d_e, *g_e, *t_e, *tn_e, *pn_e, *mg_e, *ref, *iter, *misc;
+    uint32_t* general_queue;
+    uint32_t* tail_queue;
+    unsigned* counters;
+};
+
+// Bytes of StateStore storage per pixel (9 doubles, 9 ints, 2 queue slots).
+constexpr size_t kStateBytesPerPixel = 9 * 8 + 9 * 4 + 2 * 4;
+
 // Everything one field launch needs, passed by value as the kernel argument.
 struct RenderParams {
+    StateStore state;
     OrbitDev orbit;
     SeriesDev series;
     const LinearStepDev* linear;  // tier selected on the host (ordinary/deep)
@@ -112,6 +128,7 @@ struct RenderParams {
     int32_t max_bla_length;
     int32_t max_linear_length;
     int32_t approximation_order;
+    int32_t fast_linear;  // linear-BLA fast kernel on (off with cycle checks or a time budget)
 };
 
 }  // namespace fav

@@ -441,18 +441,78 @@ FAV_HD void exact_tail(const RenderParams& p, const PlaneOut& planes, size_t ind
     if (!escaped) out = encode_iteration(p.max_iter, p.output_bias);
 }
 
-// One pixel of render_bla_impl, renderer.cpp:12817-13544.
+// ---- resumable pixel state ----------------------------------------------
+// The loop-top state of one pixel of render_bla_impl (renderer.cpp:12817-
+// 12900).  The field is rendered by three kernels that pass this state
+// through HBM: a linear-BLA fast kernel, a general kernel with every other
+// branch of the loop, and a plain-double tail kernel.  A kernel hands a pixel
+// on only at a loop top (or at the tail entry), so the sequence of
+// floating-point operations is the one of the single loop.
+struct PixelState {
+    SC delta, derivative, total;
+    SN total_norm, previous_total_norm;
+    FE margin;  // margin of (Z[ref], delta) while have_total
+    int ref, iteration, rebases;
+    bool have_total, pixel_disable_bla;
+    unsigned ticks;  // time-budget cadence counter (renderer.cpp:11878-11890)
+};
+
+FAV_HD void save_state(const StateStore& q, size_t i, const PixelState& s) {
+    q.d_re[i] = s.delta.re; q.d_im[i] = s.delta.im; q.d_e[i] = s.delta.e;
+    q.g_re[i] = s.derivative.re; q.g_im[i] = s.derivative.im; q.g_e[i] = s.derivative.e;
+    q.t_re[i] = s.total.re; q.t_im[i] = s.total.im; q.t_e[i] = s.total.e;
+    q.tn_m[i] = s.total_norm.m; q.tn_e[i] = s.total_norm.e;
+    q.pn_m[i] = s.previous_total_norm.m; q.pn_e[i] = s.previous_total_norm.e;
+    q.mg_m[i] = s.margin.m; q.mg_e[i] = s.margin.e;
+    q.ref[i] = s.ref; q.iter[i] = s.iteration;
+    q.misc[i] = (s.rebases & 0xff) | (s.have_total ? 0x100 : 0) | (s.pixel_disable_bla ? 0x200 : 0)
+        | static_cast<int>((s.ticks & 0xffU) << 16);
+}
+
+FAV_HD PixelState load_state(const StateStore& q, size_t i) {
+    PixelState s;
+    s.delta = SC{q.d_re[i], q.d_im[i], q.d_e[i]};
+    s.derivative = SC{q.g_re[i], q.g_im[i], q.g_e[i]};
+    s.total = SC{q.t_re[i], q.t_im[i], q.t_e[i]};
+    s.total_norm = SN{q.tn_m[i], q.tn_e[i]};
+    s.previous_total_norm = SN{q.pn_m[i], q.pn_e[i]};
+    s.margin = FE{q.mg_m[i], q.mg_e[i]};
+    s.ref = q.ref[i];
+    s.iteration = q.iter[i];
+    const int misc = q.misc[i];
+    s.rebases = misc & 0xff;
+    s.have_total = (misc & 0x100) != 0;
+    s.pixel_disable_bla = (misc & 0x200) != 0;
+    s.ticks = static_cast<unsigned>(misc >> 16) & 0xffU;
+    return s;
+}
+
+FAV_HD SC pixel_dc(const RenderParams& p, int px, int py) {  // renderer.cpp:12838-12842
+    return sc_from_fe(FE{p.x_off_m[px], p.x_off_e[px]}, FE{p.y_off_m[py], p.y_off_e[py]});
+}
+
+FAV_HD PlaneOut plane_out(const RenderParams& p) {
+    return PlaneOut{p.plane_iteration, p.plane_phase, p.plane_de_x, p.plane_de_y, p.plane_test1, p.plane_test2};
+}
+
+// Pixel set-up and image-series jump, renderer.cpp:12836-12873.
 template <bool Planes, bool Stats>
-FAV_HD void render_pixel(const RenderParams& p, const PlaneOut& planes, size_t index, SC dc, Counters<Stats>& st) {
-    const OrbitDev& o = p.orbit;
-    const int max_iter = p.max_iter;
-    const int mode = p.escape_radius_mode;
-    const FE spacing{p.pixel_spacing_m, p.pixel_spacing_e};
+FAV_HD PixelState init_pixel(const RenderParams& p, size_t index, SC dc, Counters<Stats>& st) {
     st.add(ST_PIXELS);
-    if constexpr (Planes) clear_planes(planes, index, max_iter);
-    SC delta = dc;
-    SC derivative = kOne;
-    int ref = 1, iteration = 1;
+    if constexpr (Planes) clear_planes(plane_out(p), index, p.max_iter);
+    PixelState s;
+    s.delta = dc;
+    s.derivative = kOne;
+    s.total = sc_zero();
+    s.total_norm = SN{0.0, 0};
+    s.previous_total_norm = SN{0.0, 0};
+    s.margin = FE{0.0, 0};
+    s.ref = 1;
+    s.iteration = 1;
+    s.rebases = 0;
+    s.have_total = false;
+    s.pixel_disable_bla = false;
+    s.ticks = 0;
     if (p.series.available
         && sc_compare_norm(sc_norm_squared(dc), SN{p.series.radius2_m, p.series.radius2_e}) <= 0) {
         // evaluate_image_series_with_derivative, renderer.cpp:8611-8639
@@ -463,55 +523,138 @@ FAV_HD void render_pixel(const RenderParams& p, const PlaneOut& planes, size_t i
             if constexpr (Planes) pd = sc_add(sc_mul(pd, dc), value);
             value = sc_add(sc_mul(value, dc), SC{p.series.re[k - 1], p.series.im[k - 1], p.series.e[k - 1]});
         }
-        if constexpr (Planes) derivative = sc_add(sc_mul(pd, dc), value);
-        delta = sc_mul(value, dc);
-        ref = p.series.iteration;
-        iteration = p.series.iteration;
+        if constexpr (Planes) s.derivative = sc_add(sc_mul(pd, dc), value);
+        s.delta = sc_mul(value, dc);
+        s.ref = p.series.iteration;
+        s.iteration = p.series.iteration;
         st.add(ST_SERIES_PIXELS);
-        st.add(ST_SERIES_JUMPS, static_cast<unsigned long long>(iteration));
+        st.add(ST_SERIES_JUMPS, static_cast<unsigned long long>(s.iteration));
     }
-    bool escaped = false, have_total = false;
-    SC total = sc_zero();
-    SN total_norm{0.0, 0}, previous_total_norm{0.0, 0};
-    FE margin{0.0, 0};  // margin of (Z[ref], delta) when have_total
+    return s;
+}
+
+enum Outcome : int { kDone = 0, kHandoff = 1, kTail = 2 };
+
+// Linear-BLA fast path: the loop of renderer.cpp:12889-13536 restricted to
+// iterations that end in a committed linear block.  Any other branch (glitch,
+// escape, rebase, reference escape, cubic order, no block, replay, retry) is
+// left to the general kernel: the pixel is handed on with the state it had
+// at that loop top.  Interior pixels that reach the cap finish here.
+template <bool Planes, bool Stats>
+FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st) {
+    const OrbitDev& o = p.orbit;
+    const int max_iter = p.max_iter;
+    const int mode = p.escape_radius_mode;
+    if (!p.fast_linear) return kHandoff;
+    while (s.iteration < max_iter) {
+        if (s.ref < 0 || s.ref >= o.size) return kHandoff;
+        SC total = s.total;
+        SN total_norm = s.total_norm;
+        FE margin = s.margin;
+        if (!s.have_total) {
+            total = sc_add(load_z(o, s.ref), s.delta);
+            margin = margin_at(o, s.ref, s.delta);
+            total_norm = sn_from_margin(margin, mode);
+        }
+        const FE glitch{o.glitch_m[s.ref], o.glitch_e[s.ref]};
+        if (glitch.m != 0.0 && fe_compare(FE{total_norm.m, total_norm.e}, glitch) < 0) return kHandoff;
+        if (sc_outside_escape(total_norm, mode) || fe_compare(margin, fe_zero()) > 0) return kHandoff;
+        const SN delta_norm = sc_norm_squared(s.delta);
+        if (sc_compare_norm(total_norm, delta_norm) < 0) return kHandoff;
+        if (o.map_end > 1 && s.ref >= o.map_end) return kHandoff;
+        const int order = p.approximation_order >= 3 && delta_norm.e < -115
+            ? (delta_norm.e < -160 ? 1 : 2) : p.approximation_order;
+        if (order > 1 || p.disable_bla || s.pixel_disable_bla) return kHandoff;
+        const int remaining = max_iter - s.iteration;
+        const int k = lookup_level(p.linear, p.linear_dir, s.ref, FE{delta_norm.m, delta_norm.e},
+                                   p.max_linear_length < remaining ? p.max_linear_length : remaining);
+        if (k < 0) return kHandoff;
+        const LinearStepDev& step = p.linear[k];
+        const int length = step.length;
+        if (s.ref + length > max_iter) return kHandoff;
+        const SC a{step.a_re, step.a_im, step.a_e}, b{step.b_re, step.b_im, step.b_e};
+        const SC delta = sc_add(sc_mul(a, s.delta), sc_mul(b, dc));
+        const int ref = s.ref + length;
+        const SC endpoint = sc_add(load_z(o, ref), delta);
+        const FE endpoint_margin = margin_at(o, ref, delta);
+        const SN endpoint_norm = sn_from_margin(endpoint_margin, mode);
+        if (!sc_finite(delta) || !sc_finite(endpoint)) return kHandoff;
+        if (sc_compare_norm(endpoint_norm, SN{0.75, 2}) >= 0) return kHandoff;
+        // commit, renderer.cpp:13499-13511
+        if constexpr (Planes) s.derivative = sc_add(sc_mul(a, s.derivative), b);
+        s.delta = delta;
+        s.ref = ref;
+        s.iteration += length;
+        s.total = endpoint;
+        s.total_norm = endpoint_norm;
+        s.margin = endpoint_margin;
+        s.previous_total_norm = endpoint_norm;
+        s.have_total = true;
+        st.add(ST_BLA_BLOCKS);
+        st.add(ST_LINEAR_BLOCKS);
+        st.add(ST_SERIES_JUMPS);
+        if constexpr (Stats) {
+            int bucket = 0;
+            unsigned v = static_cast<unsigned>(length);
+            while (v > 1U && bucket < 15) { v >>= 1U; ++bucket; }
+            st.add(ST_HISTOGRAM + bucket);
+        }
+        st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
+    }
+    p.output[index] = encode_iteration(max_iter, p.output_bias);
+    st.max(ST_MAX_PIXEL_ITER, static_cast<unsigned long long>(s.iteration > 0 ? s.iteration : 0));
+    return kDone;
+}
+
+// The general loop of render_bla_impl, renderer.cpp:12889-13544, resumed
+// from a loop-top state.  With InlineTail the plain-double tail runs here;
+// otherwise the pixel is handed to the tail kernel at the tail entry
+// (renderer.cpp:13283-13290) and kTail is returned.
+template <bool Planes, bool Stats, bool InlineTail>
+FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st) {
+    const OrbitDev& o = p.orbit;
+    const PlaneOut planes = plane_out(p);
+    const int max_iter = p.max_iter;
+    const int mode = p.escape_radius_mode;
+    const FE spacing{p.pixel_spacing_m, p.pixel_spacing_e};
+    bool escaped = false;
     SC tortoise = sc_zero();
     int cycle_power = 1, cycle_length = 0;
     bool cycle_ready = false;
-    bool pixel_disable_bla = false;
-    int rebases = 0;
-    Budget budget{&p, 0U};
+    Budget budget{&p, s.ticks};
     bool deadline = false, unresolved = false;
     float out = 0.0f;
     const int cycle_min = p.strict_cycle ? 2048 : 512;
     const int cycle_margin = p.strict_cycle ? 78 : 60;
 
-    while (iteration < max_iter) {
+    while (s.iteration < max_iter) {
         if (budget.expired()) {
             out = __builtin_nan(""); deadline = true; unresolved = true;
             st.add(ST_DEADLINE); st.add(ST_UNRESOLVED);
             break;
         }
-        if (ref < 0 || ref >= o.size) {
-            out = encode_iteration(iteration, p.output_bias);
+        if (s.ref < 0 || s.ref >= o.size) {
+            out = encode_iteration(s.iteration, p.output_bias);
             if constexpr (Planes)
-                store_escape(planes, index, iteration, total, total_norm, previous_total_norm, derivative, spacing);
+                store_escape(planes, index, s.iteration, s.total, s.total_norm, s.previous_total_norm, s.derivative,
+                             spacing);
             escaped = true;
             break;
         }
-        if (!have_total) {
-            total = sc_add(load_z(o, ref), delta);
-            margin = margin_at(o, ref, delta);
-            total_norm = sn_from_margin(margin, mode);
+        if (!s.have_total) {
+            s.total = sc_add(load_z(o, s.ref), s.delta);
+            s.margin = margin_at(o, s.ref, s.delta);
+            s.total_norm = sn_from_margin(s.margin, mode);
         }
-        have_total = false;
+        s.have_total = false;
         // glitch: |Z+d|^2 < 1e-7 |Z|^2, renderer.cpp:12931-12988 (the
         // threshold table holds 0 where |Z|^2 is zero or not finite)
-        const FE glitch{o.glitch_m[ref], o.glitch_e[ref]};
-        if (glitch.m != 0.0 && fe_compare(FE{total_norm.m, total_norm.e}, glitch) < 0) {
+        const FE glitch{o.glitch_m[s.ref], o.glitch_e[s.ref]};
+        if (glitch.m != 0.0 && fe_compare(FE{s.total_norm.m, s.total_norm.e}, glitch) < 0) {
             st.add(ST_GLITCH);
-            if (p.allow_recovery && sc_finite(total) && ++rebases <= 64) {
-                delta = total;
-                ref = 0;
+            if (p.allow_recovery && sc_finite(s.total) && ++s.rebases <= 64) {
+                s.delta = s.total;
+                s.ref = 0;
                 continue;
             }
             out = __builtin_nan("");
@@ -520,21 +663,22 @@ FAV_HD void render_pixel(const RenderParams& p, const PlaneOut& planes, size_t i
             break;
         }
         // escape, renderer.cpp:12989-13006
-        if (sc_outside_escape(total_norm, mode) || fe_compare(margin, fe_zero()) > 0) {
-            out = smooth_escape(iteration, total_norm, p.output_bias);
+        if (sc_outside_escape(s.total_norm, mode) || fe_compare(s.margin, fe_zero()) > 0) {
+            out = smooth_escape(s.iteration, s.total_norm, p.output_bias);
             if constexpr (Planes)
-                store_escape(planes, index, iteration, total, total_norm, previous_total_norm, derivative, spacing);
+                store_escape(planes, index, s.iteration, s.total, s.total_norm, s.previous_total_norm, s.derivative,
+                             spacing);
             escaped = true;
             break;
         }
         // Brent cycle check, renderer.cpp:13016-13042
-        if (p.cycle_enabled && iteration >= cycle_min && (iteration & 31) == 0
-            && sc_compare_norm(total_norm, SN{0.75, 2}) < 0) {
+        if (p.cycle_enabled && s.iteration >= cycle_min && (s.iteration & 31) == 0
+            && sc_compare_norm(s.total_norm, SN{0.75, 2}) < 0) {
             if (!cycle_ready) {
-                tortoise = total; cycle_power = 1; cycle_length = 0; cycle_ready = true;
+                tortoise = s.total; cycle_power = 1; cycle_length = 0; cycle_ready = true;
             } else {
-                const SN distance = sc_norm_squared(sc_sub(total, tortoise));
-                const int scale_exponent = total_norm.e > 0 ? total_norm.e : 0;
+                const SN distance = sc_norm_squared(sc_sub(s.total, tortoise));
+                const int scale_exponent = s.total_norm.e > 0 ? s.total_norm.e : 0;
                 if (distance.m == 0.0 || distance.e <= scale_exponent - cycle_margin) {
                     st.add(ST_CYCLE_INSIDE);
                     out = encode_iteration(max_iter, p.output_bias);
@@ -543,76 +687,81 @@ FAV_HD void render_pixel(const RenderParams& p, const PlaneOut& planes, size_t i
                 }
                 ++cycle_length;
                 if (cycle_length >= cycle_power) {
-                    tortoise = total;
+                    tortoise = s.total;
                     cycle_power = cycle_power * 2 < (1 << 20) ? cycle_power * 2 : (1 << 20);
                     cycle_length = 0;
                 }
             }
         }
         // rebase when |Z+d| < |d|, renderer.cpp:13047-13153
-        const SN delta_norm = sc_norm_squared(delta);
-        if (sc_compare_norm(total_norm, delta_norm) < 0) {
-            if (++rebases > 64) {
-                exact_tail<Planes, Stats>(p, planes, index, dc, iteration, total, total_norm, derivative, budget,
-                                          deadline, unresolved, st, out);
+        const SN delta_norm = sc_norm_squared(s.delta);
+        if (sc_compare_norm(s.total_norm, delta_norm) < 0) {
+            if (++s.rebases > 64) {
+                exact_tail<Planes, Stats>(p, planes, index, dc, s.iteration, s.total, s.total_norm, s.derivative,
+                                          budget, deadline, unresolved, st, out);
                 if (!deadline) escaped = true;
                 break;
             }
-            delta = total;
-            ref = 0;
+            s.delta = s.total;
+            s.ref = 0;
             continue;
         }
         // the reference orbit escaped: direct recurrence, renderer.cpp:13155-13250
-        if (o.map_end > 1 && ref >= o.map_end) {
-            exact_tail<Planes, Stats>(p, planes, index, dc, iteration, total, total_norm, derivative, budget,
-                                      deadline, unresolved, st, out);
+        if (o.map_end > 1 && s.ref >= o.map_end) {
+            exact_tail<Planes, Stats>(p, planes, index, dc, s.iteration, s.total, s.total_norm, s.derivative,
+                                      budget, deadline, unresolved, st, out);
             if (!deadline) escaped = true;
             break;
         }
         // BLA selection, renderer.cpp:13252-13273
         const FE dn{delta_norm.m, delta_norm.e};
-        const int remaining = max_iter - iteration;
+        const int remaining = max_iter - s.iteration;
         const int order = p.approximation_order >= 3 && delta_norm.e < -115
             ? (delta_norm.e < -160 ? 1 : 2) : p.approximation_order;
         int linear_k = -1, cubic_k = -1, map_length = 0;
-        if (!p.disable_bla && !pixel_disable_bla) {
+        if (!p.disable_bla && !s.pixel_disable_bla) {
             if (order <= 1) {
-                linear_k = lookup_level(p.linear, p.linear_dir, ref, dn,
+                linear_k = lookup_level(p.linear, p.linear_dir, s.ref, dn,
                                         p.max_linear_length < remaining ? p.max_linear_length : remaining);
                 if (linear_k >= 0) map_length = p.linear[linear_k].length;
             } else {
-                cubic_k = lookup_level(p.cubic, p.cubic_dir, ref, dn,
+                cubic_k = lookup_level(p.cubic, p.cubic_dir, s.ref, dn,
                                        p.max_bla_length < remaining ? p.max_bla_length : remaining);
                 if (cubic_k >= 0) map_length = p.cubic[cubic_k].length;
             }
         }
         // plain-double tail, renderer.cpp:13283-13362
-        if (!p.disable_bla && !pixel_disable_bla && (map_length <= 0 || ref + map_length > max_iter)
+        if (!p.disable_bla && !s.pixel_disable_bla && (map_length <= 0 || s.ref + map_length > max_iter)
             && delta_norm.e > -90) {
             st.add(ST_TAIL_PIXELS);
-            const int tail_start = iteration;
-            const bool pathological = double_tail<Planes, Stats>(p, planes, index, dc, iteration, ref, delta,
-                                                                 derivative, budget, deadline, unresolved, st,
-                                                                 total_norm, out);
-            if (deadline || unresolved) break;
-            const unsigned long long steps =
-                static_cast<unsigned long long>(iteration - tail_start > 0 ? iteration - tail_start : 0);
-            st.add(ST_TAIL_STEPS, steps);
-            st.max(ST_MAX_TAIL_STEPS, steps);
-            if (pathological) {
-                delta = dc;
-                derivative = kOne;
-                ref = 1;
-                iteration = 1;
-                have_total = false;
-                pixel_disable_bla = true;
-                continue;
+            if constexpr (!InlineTail) {
+                s.ticks = budget.ticks;
+                return kTail;  // the tail kernel continues from (delta, ref, iteration, total_norm)
+            } else {
+                const int tail_start = s.iteration;
+                const bool pathological = double_tail<Planes, Stats>(
+                    p, planes, index, dc, s.iteration, s.ref, s.delta, s.derivative, budget, deadline, unresolved,
+                    st, s.total_norm, out);
+                if (deadline || unresolved) break;
+                const unsigned long long steps =
+                    static_cast<unsigned long long>(s.iteration - tail_start > 0 ? s.iteration - tail_start : 0);
+                st.add(ST_TAIL_STEPS, steps);
+                st.max(ST_MAX_TAIL_STEPS, steps);
+                if (pathological) {
+                    s.delta = dc;
+                    s.derivative = kOne;
+                    s.ref = 1;
+                    s.iteration = 1;
+                    s.have_total = false;
+                    s.pixel_disable_bla = true;
+                    continue;
+                }
+                escaped = true;
+                break;
             }
-            escaped = true;
-            break;
         }
         // BLA jump with endpoint guard and replay, renderer.cpp:13363-13514
-        if (map_length > 0 && ref + map_length <= max_iter) {
+        if (map_length > 0 && s.ref + map_length <= max_iter) {
             st.add(ST_BLA_BLOCKS);
             st.add(linear_k >= 0 ? ST_LINEAR_BLOCKS : ST_CUBIC_BLOCKS);
             st.add(ST_SERIES_JUMPS);
@@ -622,118 +771,128 @@ FAV_HD void render_pixel(const RenderParams& p, const PlaneOut& planes, size_t i
                 while (v > 1U && bucket < 15) { v >>= 1U; ++bucket; }
                 st.add(ST_HISTOGRAM + bucket);
             }
-            const SC previous_delta = delta, previous_derivative = derivative;
-            const int previous_ref = ref, previous_iteration = iteration;
+            const SC previous_delta = s.delta, previous_derivative = s.derivative;
+            const int previous_ref = s.ref, previous_iteration = s.iteration;
             if (linear_k >= 0) {
-                const LinearStepDev& s = p.linear[linear_k];
-                const SC a{s.a_re, s.a_im, s.a_e}, b{s.b_re, s.b_im, s.b_e};
-                delta = sc_add(sc_mul(a, previous_delta), sc_mul(b, dc));
-                if constexpr (Planes) derivative = sc_add(sc_mul(a, previous_derivative), b);
+                const LinearStepDev& step = p.linear[linear_k];
+                const SC a{step.a_re, step.a_im, step.a_e}, b{step.b_re, step.b_im, step.b_e};
+                s.delta = sc_add(sc_mul(a, previous_delta), sc_mul(b, dc));
+                if constexpr (Planes) s.derivative = sc_add(sc_mul(a, previous_derivative), b);
             } else {
-                const CubicStepDev& s = p.cubic[cubic_k];
-                delta = apply_cubic(s, previous_delta, dc, order);
+                const CubicStepDev& step = p.cubic[cubic_k];
+                s.delta = apply_cubic(step, previous_delta, dc, order);
                 if constexpr (Planes)
-                    derivative = apply_cubic_derivative(s, previous_delta, dc, previous_derivative, order);
+                    s.derivative = apply_cubic_derivative(step, previous_delta, dc, previous_derivative, order);
             }
-            ref += map_length;
-            iteration += map_length;
-            const SC endpoint = sc_add(load_z(o, ref), delta);
-            const FE endpoint_margin = margin_at(o, ref, delta);
+            s.ref += map_length;
+            s.iteration += map_length;
+            const SC endpoint = sc_add(load_z(o, s.ref), s.delta);
+            const FE endpoint_margin = margin_at(o, s.ref, s.delta);
             const SN endpoint_norm = sn_from_margin(endpoint_margin, mode);
-            if (!sc_finite(delta) || !sc_finite(endpoint)) {
-                delta = previous_delta; derivative = previous_derivative;
-                ref = previous_ref; iteration = previous_iteration;
+            if (!sc_finite(s.delta) || !sc_finite(endpoint)) {
+                s.delta = previous_delta; s.derivative = previous_derivative;
+                s.ref = previous_ref; s.iteration = previous_iteration;
                 st.add(ST_BLA_RETRIES);
-                if (!pixel_disable_bla) st.add(ST_BLA_DISABLED);
-                pixel_disable_bla = true;
-                have_total = false;
+                if (!s.pixel_disable_bla) st.add(ST_BLA_DISABLED);
+                s.pixel_disable_bla = true;
+                s.have_total = false;
                 continue;
             }
             if (sc_compare_norm(endpoint_norm, SN{0.75, 2}) >= 0) {
-                delta = previous_delta; derivative = previous_derivative;
-                ref = previous_ref; iteration = previous_iteration;
+                s.delta = previous_delta; s.derivative = previous_derivative;
+                s.ref = previous_ref; s.iteration = previous_iteration;
                 SC replay_total = sc_zero();
                 SN replay_norm{0.0, 0};
                 FE replay_margin{0.0, 0};
                 bool retry = false;
-                for (int replay = 0; replay < map_length && iteration < max_iter; ++replay) {
+                for (int replay = 0; replay < map_length && s.iteration < max_iter; ++replay) {
                     if (budget.expired()) {
                         out = __builtin_nan(""); deadline = true; unresolved = true;
                         st.add(ST_DEADLINE); st.add(ST_UNRESOLVED);
                         break;
                     }
-                    const SC zc = load_z(o, ref);
+                    const SC zc = load_z(o, s.ref);
                     if constexpr (Planes)
-                        derivative = sc_add(sc_double(sc_mul(sc_add(zc, delta), derivative)), kOne);
-                    delta = sc_add(sc_double(sc_mul(zc, delta)), sc_add(sc_mul(delta, delta), dc));
-                    ++ref;
-                    ++iteration;
+                        s.derivative = sc_add(sc_double(sc_mul(sc_add(zc, s.delta), s.derivative)), kOne);
+                    s.delta = sc_add(sc_double(sc_mul(zc, s.delta)), sc_add(sc_mul(s.delta, s.delta), dc));
+                    ++s.ref;
+                    ++s.iteration;
                     st.add(ST_REPLAY); st.add(ST_LOGICAL);
-                    replay_total = sc_add(load_z(o, ref), delta);
-                    replay_margin = margin_at(o, ref, delta);
+                    replay_total = sc_add(load_z(o, s.ref), s.delta);
+                    replay_margin = margin_at(o, s.ref, s.delta);
                     replay_norm = sn_from_margin(replay_margin, mode);
-                    if (!sc_finite(delta) || !sc_finite(replay_total)) { retry = true; break; }
+                    if (!sc_finite(s.delta) || !sc_finite(replay_total)) { retry = true; break; }
                     if (sc_outside_escape(replay_norm, mode) || fe_compare(replay_margin, fe_zero()) > 0) {
-                        out = smooth_escape(iteration, replay_norm, p.output_bias);
+                        out = smooth_escape(s.iteration, replay_norm, p.output_bias);
                         if constexpr (Planes)
-                            store_escape(planes, index, iteration, replay_total, replay_norm, total_norm, derivative,
-                                         spacing);
+                            store_escape(planes, index, s.iteration, replay_total, replay_norm, s.total_norm,
+                                         s.derivative, spacing);
                         escaped = true;
                         break;
                     }
                 }
                 if (deadline || escaped) break;
                 if (retry) {
-                    delta = previous_delta; derivative = previous_derivative;
-                    ref = previous_ref; iteration = previous_iteration;
+                    s.delta = previous_delta; s.derivative = previous_derivative;
+                    s.ref = previous_ref; s.iteration = previous_iteration;
                     st.add(ST_BLA_RETRIES);
-                    if (!pixel_disable_bla) st.add(ST_BLA_DISABLED);
-                    pixel_disable_bla = true;
-                    have_total = false;
+                    if (!s.pixel_disable_bla) st.add(ST_BLA_DISABLED);
+                    s.pixel_disable_bla = true;
+                    s.have_total = false;
                     continue;
                 }
-                total = replay_total;
-                total_norm = replay_norm;
-                margin = replay_margin;
-                previous_total_norm = total_norm;
-                have_total = true;
+                s.total = replay_total;
+                s.total_norm = replay_norm;
+                s.margin = replay_margin;
+                s.previous_total_norm = s.total_norm;
+                s.have_total = true;
             } else {
-                total = endpoint;
-                total_norm = endpoint_norm;
-                margin = endpoint_margin;
-                previous_total_norm = total_norm;
-                have_total = true;
+                s.total = endpoint;
+                s.total_norm = endpoint_norm;
+                s.margin = endpoint_margin;
+                s.previous_total_norm = s.total_norm;
+                s.have_total = true;
                 st.add(ST_LOGICAL, static_cast<unsigned long long>(map_length));
             }
             continue;
         }
         // exact perturbation step, renderer.cpp:13516-13535
-        const SC z = load_z(o, ref);
-        const SN prior_norm = total_norm;
-        if constexpr (Planes) derivative = sc_add(sc_double(sc_mul(sc_add(z, delta), derivative)), kOne);
-        delta = sc_add(sc_double(sc_mul(z, delta)), sc_add(sc_mul(delta, delta), dc));
-        ++ref;
-        ++iteration;
-        previous_total_norm = prior_norm;
+        const SC z = load_z(o, s.ref);
+        const SN prior_norm = s.total_norm;
+        if constexpr (Planes) s.derivative = sc_add(sc_double(sc_mul(sc_add(z, s.delta), s.derivative)), kOne);
+        s.delta = sc_add(sc_double(sc_mul(z, s.delta)), sc_add(sc_mul(s.delta, s.delta), dc));
+        ++s.ref;
+        ++s.iteration;
+        s.previous_total_norm = prior_norm;
         st.add(ST_LOGICAL); st.add(ST_EXACT);
     }
     if (!escaped && !deadline && !unresolved) out = encode_iteration(max_iter, p.output_bias);
     p.output[index] = out;
-    st.max(ST_MAX_PIXEL_ITER, static_cast<unsigned long long>(iteration > 0 ? iteration : 0));
+    st.max(ST_MAX_PIXEL_ITER, static_cast<unsigned long long>(s.iteration > 0 ? s.iteration : 0));
+    return kDone;
 }
 
-// The pixel at (px, py): dc from the per-column / per-row offsets
-// (renderer.cpp:12838-12842), then the state machine.
+// The tail kernel's part: render_scaled_double_tail from the handed-on state
+// and the bookkeeping the caller does after it (renderer.cpp:13283-13362).
 template <bool Planes, bool Stats>
-FAV_HD void render_one(const RenderParams& p, int px, int py) {
-    const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
-    const SC dc = sc_from_fe(FE{p.x_off_m[px], p.x_off_e[px]}, FE{p.y_off_m[py], p.y_off_e[py]});
-    const PlaneOut planes{p.plane_iteration, p.plane_phase, p.plane_de_x, p.plane_de_y, p.plane_test1,
-                          p.plane_test2};
-    Counters<Stats> st;
-    st.clear();
-    render_pixel<Planes, Stats>(p, planes, index, dc, st);
-    if constexpr (Stats) st.flush(p.stats);
+FAV_HD void run_tail(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st) {
+    Budget budget{&p, s.ticks};
+    bool deadline = false, unresolved = false;
+    float out = 0.0f;
+    const int tail_start = s.iteration;
+    (void)double_tail<Planes, Stats>(p, plane_out(p), index, dc, s.iteration, s.ref, s.delta, s.derivative, budget,
+                                     deadline, unresolved, st, s.total_norm, out);
+    // Both ways double_tail reports a pathological tail also mark the pixel
+    // unresolved (renderer.cpp:12536-12544, 12644-12655), so the caller's
+    // restart branch is never taken and the tail always ends the pixel.
+    if (!deadline && !unresolved) {
+        const unsigned long long steps =
+            static_cast<unsigned long long>(s.iteration - tail_start > 0 ? s.iteration - tail_start : 0);
+        st.add(ST_TAIL_STEPS, steps);
+        st.max(ST_MAX_TAIL_STEPS, steps);
+    }
+    p.output[index] = out;
+    st.max(ST_MAX_PIXEL_ITER, static_cast<unsigned long long>(s.iteration > 0 ? s.iteration : 0));
 }
 
 }  // namespace core

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "field_core.h"
@@ -73,14 +74,62 @@ extern "C" int fav_host_check_render(
         p.budget_start = &start;
         p.budget_ticks = 0;
         const long long n = static_cast<long long>(width) * height;
+        std::vector<unsigned char> state_block(static_cast<size_t>(n) * kStateBytesPerPixel + 20 * 256);
+        bind_state_store(p, state_block.data(), static_cast<size_t>(n));
+        // The three kernels of render_kernel.hip, run one after another on
+        // the CPU: fast pass over every pixel, then the general queue, then
+        // the tail queue.
+        std::vector<char> outcome(static_cast<size_t>(n), 0);
+        auto pass = [&](auto&& body) {
 #pragma omp parallel for schedule(dynamic, 64)
-        for (long long i = 0; i < n; ++i) {
-            const int py = static_cast<int>(i / width), px = static_cast<int>(i % width);
-            if (planes && stats) core::render_one<true, true>(p, px, py);
-            else if (planes) core::render_one<true, false>(p, px, py);
-            else if (stats) core::render_one<false, true>(p, px, py);
-            else core::render_one<false, false>(p, px, py);
-        }
+            for (long long i = 0; i < n; ++i) body(static_cast<size_t>(i));
+        };
+        auto run = [&](auto planes_tag, auto stats_tag) {
+            constexpr bool Planes = decltype(planes_tag)::value;
+            constexpr bool Stats = decltype(stats_tag)::value;
+            pass([&](size_t i) {
+                const int py = static_cast<int>(i / static_cast<size_t>(width));
+                const int px = static_cast<int>(i % static_cast<size_t>(width));
+                core::Counters<Stats> st;
+                st.clear();
+                const SC dc = core::pixel_dc(p, px, py);
+                core::PixelState s = core::init_pixel<Planes, Stats>(p, i, dc, st);
+                if (core::run_linear<Planes, Stats>(p, i, s, dc, st) == core::kHandoff) {
+                    core::save_state(p.state, i, s);
+                    outcome[i] = 1;
+                }
+                if constexpr (Stats) st.flush(p.stats);
+            });
+            pass([&](size_t i) {
+                if (outcome[i] != 1) return;
+                const int py = static_cast<int>(i / static_cast<size_t>(width));
+                const int px = static_cast<int>(i % static_cast<size_t>(width));
+                core::Counters<Stats> st;
+                st.clear();
+                const SC dc = core::pixel_dc(p, px, py);
+                core::PixelState s = core::load_state(p.state, i);
+                if (core::run_general<Planes, Stats, false>(p, i, s, dc, st) == core::kTail) {
+                    core::save_state(p.state, i, s);
+                    outcome[i] = 2;
+                }
+                if constexpr (Stats) st.flush(p.stats);
+            });
+            pass([&](size_t i) {
+                if (outcome[i] != 2) return;
+                const int py = static_cast<int>(i / static_cast<size_t>(width));
+                const int px = static_cast<int>(i % static_cast<size_t>(width));
+                core::Counters<Stats> st;
+                st.clear();
+                const SC dc = core::pixel_dc(p, px, py);
+                core::PixelState s = core::load_state(p.state, i);
+                core::run_tail<Planes, Stats>(p, i, s, dc, st);
+                if constexpr (Stats) st.flush(p.stats);
+            });
+        };
+        if (planes && stats) run(std::true_type{}, std::true_type{});
+        else if (planes) run(std::true_type{}, std::false_type{});
+        else if (stats) run(std::false_type{}, std::true_type{});
+        else run(std::false_type{}, std::false_type{});
         if (stats) for (int i = 0; i < kStatsFields; ++i) stats[i] = counters[static_cast<size_t>(i)];
         return 0;
     } catch (const std::exception& e) {

@@ -1,10 +1,23 @@
-// render_kernel.hip -- the FP64 deep-zoom field kernel for gfx950.
+// render_kernel.hip -- the FP64 deep-zoom field kernels for gfx950.
 //
-// Work layout: a 256-thread block covers a 16x16 pixel tile as four 8x8
-// wavefronts.  Neighbouring pixels follow near-identical orbits, so an 8x8
-// wave keeps its lanes on the same reference index and BLA level for most of
-// the iteration, and the orbit / table loads are wave-uniform broadcasts.
-// The per-pixel state machine lives in field_core.h.
+// A field is rendered by three kernels on one stream (field_core.h holds the
+// per-pixel code):
+//
+//   fast_kernel     every pixel: set-up, image-series jump, then the
+//                   linear-BLA loop.  A 256-thread block covers a 16x16 tile
+//                   as four 8x8 wavefronts, so a wave's lanes share their
+//                   reference index and BLA level and the table loads are
+//                   wave-uniform.  Interior pixels finish here; any other
+//                   branch hands the pixel to the general queue.
+//   general_kernel  the complete loop, resumed from the handed-on state.
+//                   Pixels that enter the plain-double tail go to the tail
+//                   queue.
+//   tail_kernel     the plain-double tail (renderer.cpp:12450-12669).
+//
+// Each kernel only carries the registers of its own part, so the common
+// paths run at higher occupancy than one kernel holding every branch.  The
+// general and tail kernels are persistent: a fixed grid whose waves take 64
+// queue entries at a time until the queue is drained.
 #include <hip/hip_runtime.h>
 
 #include "field_core.h"
@@ -12,36 +25,131 @@
 namespace fav {
 namespace {
 
-template <bool Planes, bool Stats>
-__global__ void __launch_bounds__(256) field_kernel(RenderParams p) {
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-    if (px >= p.width || py >= p.height) return;
-    core::render_one<Planes, Stats>(p, px, py);
+__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63); }
+
+// Appends `value` for every lane with `want`, one atomic per wave.  Called
+// by all lanes of the wave.
+__device__ void wave_push(uint32_t* queue, unsigned* count, uint32_t value, bool want) {
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0) return;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    unsigned base = 0;
+    if (lane_id() == leader) base = atomicAdd(count, static_cast<unsigned>(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (want) queue[base + __popcll(mask & ((1ULL << lane_id()) - 1ULL))] = value;
 }
 
-__global__ void budget_start_kernel(unsigned long long* start, int* flag) {
-    start[0] = wall_clock64();
-    flag[0] = 0;
+// Next 64 queue entries for this wave (all lanes call it).
+__device__ unsigned wave_grab(unsigned* cursor) {
+    unsigned base = 0;
+    if (lane_id() == 0) base = atomicAdd(cursor, 64U);
+    return __shfl(base, 0);
+}
+
+template <bool Planes, bool Stats>
+__global__ void __launch_bounds__(256) fast_kernel(RenderParams p) {
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const bool valid = px < p.width && py < p.height;
+    const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
+    bool handoff = false;
+    if (valid) {
+        core::Counters<Stats> st;
+        st.clear();
+        const SC dc = core::pixel_dc(p, px, py);
+        core::PixelState s = core::init_pixel<Planes, Stats>(p, index, dc, st);
+        if (core::run_linear<Planes, Stats>(p, index, s, dc, st) == core::kHandoff) {
+            core::save_state(p.state, index, s);
+            handoff = true;
+        }
+        if constexpr (Stats) st.flush(p.stats);
+    }
+    wave_push(p.state.general_queue, p.state.counters + 0, static_cast<uint32_t>(index), handoff);
+}
+
+template <bool Planes, bool Stats>
+__global__ void __launch_bounds__(256) general_kernel(RenderParams p) {
+    const unsigned count = p.state.counters[0];
+    for (;;) {
+        const unsigned base = wave_grab(p.state.counters + 1);
+        if (base >= count) break;
+        const unsigned slot = base + static_cast<unsigned>(lane_id());
+        bool to_tail = false;
+        uint32_t index = 0;
+        if (slot < count) {
+            index = p.state.general_queue[slot];
+            const int py = static_cast<int>(index / static_cast<uint32_t>(p.width));
+            const int px = static_cast<int>(index - static_cast<uint32_t>(py) * static_cast<uint32_t>(p.width));
+            core::Counters<Stats> st;
+            st.clear();
+            const SC dc = core::pixel_dc(p, px, py);
+            core::PixelState s = core::load_state(p.state, index);
+            if (core::run_general<Planes, Stats, false>(p, index, s, dc, st) == core::kTail) {
+                core::save_state(p.state, index, s);
+                to_tail = true;
+            }
+            if constexpr (Stats) st.flush(p.stats);
+        }
+        wave_push(p.state.tail_queue, p.state.counters + 2, index, to_tail);
+    }
+}
+
+template <bool Planes, bool Stats>
+__global__ void __launch_bounds__(256) tail_kernel(RenderParams p) {
+    const unsigned count = p.state.counters[2];
+    for (;;) {
+        const unsigned base = wave_grab(p.state.counters + 3);
+        if (base >= count) break;
+        const unsigned slot = base + static_cast<unsigned>(lane_id());
+        if (slot < count) {
+            const uint32_t index = p.state.tail_queue[slot];
+            const int py = static_cast<int>(index / static_cast<uint32_t>(p.width));
+            const int px = static_cast<int>(index - static_cast<uint32_t>(py) * static_cast<uint32_t>(p.width));
+            core::Counters<Stats> st;
+            st.clear();
+            const SC dc = core::pixel_dc(p, px, py);
+            core::PixelState s = core::load_state(p.state, index);
+            core::run_tail<Planes, Stats>(p, index, s, dc, st);
+            if constexpr (Stats) st.flush(p.stats);
+        }
+    }
+}
+
+// Clears the queue counters (a kernel rather than a memset so the sequence
+// can be captured in a HIP graph) and starts the time-budget clock.
+__global__ void prologue_kernel(unsigned* counters, unsigned long long* budget_start, int* budget_flag) {
+    if (threadIdx.x < 4) counters[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        budget_start[0] = wall_clock64();
+        budget_flag[0] = 0;
+    }
+}
+
+template <bool Planes, bool Stats>
+void launch_all(const RenderParams& p, int persistent_blocks, hipStream_t stream) {
+    prologue_kernel<<<1, 64, 0, stream>>>(p.state.counters, const_cast<unsigned long long*>(p.budget_start),
+                                          p.budget_flag);
+    fast_kernel<Planes, Stats><<<dim3((p.width + 15) / 16, (p.height + 15) / 16), dim3(256), 0, stream>>>(p);
+    general_kernel<Planes, Stats><<<dim3(persistent_blocks), dim3(256), 0, stream>>>(p);
+    tail_kernel<Planes, Stats><<<dim3(persistent_blocks), dim3(256), 0, stream>>>(p);
 }
 
 }  // namespace
 
-// Launches the field kernel on `stream`.  The caller validated the shapes:
-// width*height <= 1e8 and every pointer in `p` sized for width x height.
-hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, hipStream_t stream) {
-    if (p.budget_ticks != 0)
-        budget_start_kernel<<<1, 1, 0, stream>>>(const_cast<unsigned long long*>(p.budget_start), p.budget_flag);
-    const dim3 grid((p.width + 15) / 16, (p.height + 15) / 16);
-    const dim3 block(256);
+// Launches the field kernels on `stream`.  The caller validated the shapes:
+// width*height <= 1e8 and every pointer in `p` (outputs, planes, state
+// store) sized for width x height.  `persistent_blocks` is the grid of the
+// queue-driven kernels.
+hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, int persistent_blocks,
+                               hipStream_t stream) {
     if (planes) {
-        if (stats) field_kernel<true, true><<<grid, block, 0, stream>>>(p);
-        else field_kernel<true, false><<<grid, block, 0, stream>>>(p);
+        if (stats) launch_all<true, true>(p, persistent_blocks, stream);
+        else launch_all<true, false>(p, persistent_blocks, stream);
     } else {
-        if (stats) field_kernel<false, true><<<grid, block, 0, stream>>>(p);
-        else field_kernel<false, false><<<grid, block, 0, stream>>>(p);
+        if (stats) launch_all<false, true>(p, persistent_blocks, stream);
+        else launch_all<false, false>(p, persistent_blocks, stream);
     }
     return hipGetLastError();
 }

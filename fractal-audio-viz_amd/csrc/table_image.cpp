@@ -108,6 +108,31 @@ void prepare_render(const ReferenceContext& c, const FieldRequest& r, PreparedRe
     p.max_linear_length = std::clamp(std::min(r.series_block, opt.max_linear_bla_length), 2,
                                      use_ultra ? 4096 : (use_deep_linear ? 1024 : 64));
     p.approximation_order = std::clamp(r.series_order, 1, 3);
+    // the linear fast kernel skips the per-step budget clock and cycle checks
+    p.fast_linear = !(p.cycle_enabled || opt.time_budget_ms > 0);
+}
+
+}  // namespace fav
+
+namespace fav {
+
+// Carves the per-pixel state arrays and queues out of one block of at least
+// pixels * kStateBytesPerPixel + 4096 bytes.
+void bind_state_store(RenderParams& p, void* block, size_t pixels) {
+    char* cursor = static_cast<char*>(block);
+    auto take = [&](size_t bytes) {
+        char* here = cursor;
+        cursor += (bytes + 255) & ~size_t(255);
+        return here;
+    };
+    StateStore& q = p.state;
+    double** doubles[] = {&q.d_re, &q.d_im, &q.g_re, &q.g_im, &q.t_re, &q.t_im, &q.tn_m, &q.pn_m, &q.mg_m};
+    for (double** d : doubles) *d = reinterpret_cast<double*>(take(pixels * 8));
+    int32_t** ints[] = {&q.d_e, &q.g_e, &q.t_e, &q.tn_e, &q.pn_e, &q.mg_e, &q.ref, &q.iter, &q.misc};
+    for (int32_t** i : ints) *i = reinterpret_cast<int32_t*>(take(pixels * 4));
+    q.general_queue = reinterpret_cast<uint32_t*>(take(pixels * 4));
+    q.tail_queue = reinterpret_cast<uint32_t*>(take(pixels * 4));
+    q.counters = reinterpret_cast<unsigned*>(take(64));
 }
 
 }  // namespace fav

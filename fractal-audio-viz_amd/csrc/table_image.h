@@ -36,4 +36,8 @@ struct PreparedRender {
 
 void prepare_render(const ReferenceContext& context, const FieldRequest& request, PreparedRender& out);
 
+// Points p.state at per-pixel arrays inside `block`, which must hold
+// pixels * kStateBytesPerPixel + 20 * 256 bytes (each array is 256-byte aligned).
+void bind_state_store(RenderParams& p, void* block, size_t pixels);
+
 }  // namespace fav
