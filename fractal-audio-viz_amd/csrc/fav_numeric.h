@@ -27,15 +27,25 @@
 namespace fav {
 
 // ---- primitive scaling -------------------------------------------------
-// frexp/ldexp with the C library's conventions (0, +-inf and NaN return the
-// input with exponent 0).  On the device they map onto v_frexp_mant_f64,
-// v_frexp_exp_i32_f64 and v_ldexp_f64, which are exact for every finite
-// input including subnormals.
+// frexp with the C library's conventions (0, +-inf and NaN return the input
+// with exponent 0), split into its two results.  On the device these are
+// v_frexp_exp_i32_f64 / v_frexp_mant_f64 / v_ldexp_f64, exact for every
+// finite input including subnormals.
+FAV_HD int frexp_exp_(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (v == 0.0 || !__builtin_isfinite(v)) ? 0 : __builtin_amdgcn_frexp_exp(v);
+#else
+    int e = 0;
+    (void)std::frexp(v, &e);
+    return e;
+#endif
+}
+
 FAV_HD double frexp_(double v, int* e) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if (v == 0.0 || !__builtin_isfinite(v)) { *e = 0; return v; }
-    *e = __builtin_amdgcn_frexp_exp(v);
-    return __builtin_amdgcn_frexp_mant(v);
+    const bool ordinary = v != 0.0 && __builtin_isfinite(v);
+    *e = ordinary ? __builtin_amdgcn_frexp_exp(v) : 0;
+    return ordinary ? __builtin_amdgcn_frexp_mant(v) : v;
 #else
     return std::frexp(v, e);
 #endif
@@ -53,8 +63,32 @@ FAV_HD bool isfinite_(double v) { return __builtin_isfinite(v); }
 FAV_HD bool isnan_(double v) { return __builtin_isnan(v); }
 FAV_HD double fabs_(double v) { return __builtin_fabs(v); }
 
-FAV_HD int sat_exp(long long v) {  // renderer.cpp:7532-7540
+// ---- saturating exponent arithmetic ---------------------------------------
+// The reference forms exponent sums in long long and clamps them to int
+// (saturating_exponent, renderer.cpp:7532-7540).  These give the same
+// results with 32-bit operations.
+FAV_HD int sat_exp(long long v) {
     return v > INT_MAX ? INT_MAX : (v < INT_MIN ? INT_MIN : static_cast<int>(v));
+}
+
+// clamp(a + b)
+FAV_HD int sat_add(int a, int b) {
+#if defined(__clang__)
+    return __builtin_elementwise_add_sat(a, b);
+#else
+    int r;
+    return __builtin_add_overflow(a, b, &r) ? (b > 0 ? INT_MAX : INT_MIN) : r;
+#endif
+}
+
+// clamp(a + b + s) for s in {-1, 0, 1}: one saturating add of two exact
+// operands, choosing the operand that absorbs s without overflow.
+FAV_HD int sat_add3(int a, int b, int s) {
+    const bool b_edge = (s > 0 && b == INT_MAX) || (s < 0 && b == INT_MIN);
+    const bool a_edge = (s > 0 && a == INT_MAX) || (s < 0 && a == INT_MIN);
+    if (!b_edge) return sat_add(a, b + s);
+    if (!a_edge) return sat_add(a + s, b);
+    return s > 0 ? INT_MAX : INT_MIN;
 }
 
 // ---- FloatExp ------------------------------------------------------------
@@ -68,52 +102,58 @@ FAV_HD FE fe_zero() { return FE{0.0, 0}; }
 FAV_HD bool fe_is_zero(FE v) { return v.m == 0.0; }
 FAV_HD bool fe_finite(FE v) { return isfinite_(v.m); }
 
-// FloatExp::from_parts, renderer.cpp:7551-7560.
+// FloatExp::from_parts, renderer.cpp:7551-7560.  Zero and NaN keep
+// exponent 0; infinity keeps the given exponent (frexp leaves it alone).
 FAV_HD FE fe_from_parts(double value, int exponent) {
-    if (value == 0.0 || isnan_(value)) return FE{value, 0};
     int shift = 0;
     const double n = frexp_(value, &shift);
-    return FE{n, sat_exp(static_cast<long long>(exponent) + shift)};
+    const FE ordinary{n, sat_add(exponent, shift)};
+    return (value == 0.0 || isnan_(value)) ? FE{value, 0} : ordinary;
 }
 
 // escape_radius_squared_float_exp, renderer.cpp:7594-7598: 4 = 0.5*2^3,
-// 1e8 = 0.7450580596923828*2^27.
+// 1e8 = 0.7450580596923828125*2^27.
 FAV_HD FE fe_escape_r2(int mode) {
     return mode == 1 ? FE{0.7450580596923828125, 27} : FE{0.5, 3};
 }
 
 FAV_HD FE fe_neg(FE v) { return FE{-v.m, v.e}; }
 
-// fe_add, renderer.cpp:8198-8235.
+// fe_add, renderer.cpp:8198-8235.  The reference renormalises the aligned
+// sum by x0.5 when |m| >= 1, keeps it when |m| >= 0.5, and calls frexp
+// otherwise; all three are ldexp(m, -shift) with the shift chosen below.
+// The ordinary result is formed unconditionally and the special cases are
+// selected over it in the reference's order of precedence, so the code has
+// no divergent branches.
 FAV_HD FE fe_add(FE a, FE b) {
-    if (a.m == 0.0) return b;
-    if (b.m == 0.0) return a;
-    FE larger = a, smaller = b;
-    if (b.e > a.e) { larger = b; smaller = a; }
-    const long long diff = static_cast<long long>(larger.e) - smaller.e;
-    if (diff > 60) return larger;
-    const double mantissa = larger.m + ldexp_(smaller.m, -static_cast<int>(diff));
-    if (mantissa == 0.0 || !isfinite_(mantissa)) return fe_from_parts(mantissa, larger.e);
+    const bool swap = b.e > a.e;
+    const FE larger = swap ? b : a;
+    const FE smaller = swap ? a : b;
+    const unsigned diff = static_cast<unsigned>(larger.e) - static_cast<unsigned>(smaller.e);
+    const int align = static_cast<int>(diff < 64U ? diff : 64U);
+    const double mantissa = larger.m + ldexp_(smaller.m, -align);
     const double magnitude = fabs_(mantissa);
-    if (magnitude >= 1.0) return FE{mantissa * 0.5, sat_exp(static_cast<long long>(larger.e) + 1)};
-    if (magnitude >= 0.5) return FE{mantissa, larger.e};
-    int shift = 0;
-    const double n = frexp_(mantissa, &shift);
-    return FE{n, sat_exp(static_cast<long long>(larger.e) + shift)};
+    const int shift = magnitude >= 1.0 ? 1 : (magnitude >= 0.5 ? 0 : frexp_exp_(mantissa));
+    FE r{ldexp_(mantissa, -shift), sat_add(larger.e, shift)};
+    if (!isfinite_(mantissa)) r = FE{mantissa, larger.e};          // from_parts(+-inf, e)
+    if (mantissa == 0.0 || isnan_(mantissa)) r = FE{mantissa, 0};  // from_parts(0 or NaN, e)
+    if (diff > 60U) r = larger;
+    if (b.m == 0.0) r = a;
+    if (a.m == 0.0) r = b;
+    return r;
 }
 
 FAV_HD FE fe_sub(FE a, FE b) { return fe_add(a, fe_neg(b)); }
 
-// fe_mul, renderer.cpp:8241-8266.
+// fe_mul, renderer.cpp:8241-8266: one x0.5 or x2 correction of the product.
 FAV_HD FE fe_mul(FE a, FE b) {
-    if (a.m == 0.0 || b.m == 0.0) return FE{0.0, 0};
     const double mantissa = a.m * b.m;
-    if (!isfinite_(mantissa)) return fe_from_parts(mantissa, 0);
     const double magnitude = fabs_(mantissa);
-    const long long sum = static_cast<long long>(a.e) + b.e;
-    if (magnitude >= 1.0) return FE{mantissa * 0.5, sat_exp(sum + 1)};
-    if (magnitude < 0.5) return FE{mantissa * 2.0, sat_exp(sum - 1)};
-    return FE{mantissa, sat_exp(sum)};
+    const int shift = magnitude >= 1.0 ? 1 : (magnitude < 0.5 ? -1 : 0);
+    FE r{ldexp_(mantissa, -shift), sat_add3(a.e, b.e, shift)};
+    if (!isfinite_(mantissa)) r = FE{mantissa, 0};  // from_parts(inf or NaN, 0)
+    if (a.m == 0.0 || b.m == 0.0) r = FE{0.0, 0};
+    return r;
 }
 
 // fe_mul(FloatExp, double), renderer.cpp:8268-8270.
@@ -163,6 +203,16 @@ FAV_HD int fe_compare(FE a, FE b) {
     return (a.m > b.m) - (a.m < b.m);
 }
 
+// fe_compare(a, b) < 0 for finite, non-negative b (validity radii and
+// glitch thresholds).
+FAV_HD bool fe_less_nonneg(FE a, FE b) {
+    if (!isfinite_(a.m)) return false;
+    if (a.m == 0.0) return b.m != 0.0;
+    if (b.m == 0.0) return a.m < 0.0;
+    if (a.m < 0.0) return true;
+    return a.e != b.e ? a.e < b.e : a.m < b.m;
+}
+
 FAV_HD FE fe_min(FE a, FE b) { return fe_compare(a, b) <= 0 ? a : b; }  // renderer.cpp:11336
 
 // ---- FloatExpComplex (renderer.cpp:8351-8399) -----------------------------
@@ -210,35 +260,19 @@ FAV_HD SC sc(double re, double im, int e) { return SC{re, im, e}; }
 FAV_HD SC sc_zero() { return SC{0.0, 0.0, 0}; }
 FAV_HD bool sc_is_zero(SC v) { return v.re == 0.0 && v.im == 0.0; }
 
-// ScaledComplex::normalize, renderer.cpp:7610-7655.
+// ScaledComplex::normalize, renderer.cpp:7610-7655.  Each of the
+// reference's cases (frexp above 2, x0.5 in [1,2), none in [0.5,1), x2 in
+// [0.25,0.5), frexp below) scales both parts by 2^-shift with shift equal to
+// frexp's exponent of the larger magnitude, so that single shift reproduces
+// every case bit for bit; non-finite magnitudes keep shift 0.
 FAV_HD void sc_normalize(SC& v) {
     const double ar = fabs_(v.re), ai = fabs_(v.im);
     const double magnitude = (ar < ai) ? ai : ar;  // std::max(ar, ai)
-    if (magnitude == 0.0) { v.re = 0.0; v.im = 0.0; v.e = 0; return; }
-    if (magnitude >= 2.0) {
-        int shift = 0;
-        (void)frexp_(magnitude, &shift);
-        v.re = ldexp_(v.re, -shift);
-        v.im = ldexp_(v.im, -shift);
-        v.e = sat_exp(static_cast<long long>(v.e) + shift);
-        return;
-    }
-    if (magnitude >= 1.0) {
-        v.re *= 0.5; v.im *= 0.5;
-        v.e = sat_exp(static_cast<long long>(v.e) + 1);
-        return;
-    }
-    if (magnitude >= 0.5) return;
-    if (magnitude >= 0.25) {
-        v.re *= 2.0; v.im *= 2.0;
-        v.e = sat_exp(static_cast<long long>(v.e) - 1);
-        return;
-    }
-    int shift = 0;
-    (void)frexp_(magnitude, &shift);
-    v.re = ldexp_(v.re, -shift);
-    v.im = ldexp_(v.im, -shift);
-    v.e = sat_exp(static_cast<long long>(v.e) + shift);
+    const int shift = frexp_exp_(magnitude);
+    const bool zero = magnitude == 0.0;
+    v.re = zero ? 0.0 : ldexp_(v.re, -shift);
+    v.im = zero ? 0.0 : ldexp_(v.im, -shift);
+    v.e = zero ? 0 : sat_add(v.e, shift);
 }
 
 // ScaledComplex::from_float_exp, renderer.cpp:7657-7674.
@@ -253,17 +287,18 @@ FAV_HD SC sc_from_fe(FE re, FE im) {
     return r;
 }
 
-// sc_add, renderer.cpp:7676-7697.
+// sc_add, renderer.cpp:7676-7697 (special cases selected, no branches).
 FAV_HD SC sc_add(SC a, SC b) {
-    if (a.re == 0.0 && a.im == 0.0) return b;
-    if (b.re == 0.0 && b.im == 0.0) return a;
-    SC larger = a, smaller = b;
-    if (b.e > a.e) { larger = b; smaller = a; }
-    const long long diff = static_cast<long long>(larger.e) - smaller.e;
-    if (diff > 60) return larger;
-    const int d = static_cast<int>(diff);
+    const bool swap = b.e > a.e;
+    const SC larger = swap ? b : a;
+    const SC smaller = swap ? a : b;
+    const unsigned diff = static_cast<unsigned>(larger.e) - static_cast<unsigned>(smaller.e);
+    const int d = static_cast<int>(diff < 64U ? diff : 64U);
     SC r{larger.re + ldexp_(smaller.re, -d), larger.im + ldexp_(smaller.im, -d), larger.e};
     sc_normalize(r);
+    if (diff > 60U) r = larger;
+    if (b.re == 0.0 && b.im == 0.0) r = a;
+    if (a.re == 0.0 && a.im == 0.0) r = b;
     return r;
 }
 
@@ -273,17 +308,16 @@ FAV_HD SC sc_sub(SC a, SC b) { return sc_add(a, sc_neg(b)); }
 
 // sc_mul, renderer.cpp:7711-7722 (no fused multiply-add).
 FAV_HD SC sc_mul(SC a, SC b) {
-    if ((a.re == 0.0 && a.im == 0.0) || (b.re == 0.0 && b.im == 0.0)) return SC{0.0, 0.0, 0};
-    SC r{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re,
-         sat_exp(static_cast<long long>(a.e) + b.e)};
+    SC r{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re, sat_add(a.e, b.e)};
     sc_normalize(r);
+    if ((a.re == 0.0 && a.im == 0.0) || (b.re == 0.0 && b.im == 0.0)) r = SC{0.0, 0.0, 0};
     return r;
 }
 
 // sc_double, renderer.cpp:7724-7733: exact, no renormalisation.
 FAV_HD SC sc_double(SC v) {
     if (v.re == 0.0 && v.im == 0.0) return SC{0.0, 0.0, 0};
-    return SC{v.re, v.im, sat_exp(static_cast<long long>(v.e) + 1)};
+    return SC{v.re, v.im, sat_add(v.e, 1)};
 }
 
 FAV_HD bool sc_finite(SC v) { return isfinite_(v.re) && isfinite_(v.im); }

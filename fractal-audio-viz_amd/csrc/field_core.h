@@ -112,7 +112,7 @@ struct Counters<false> {
 FAV_HD FE fe_times_two(FE x) {
     if (x.m == 0.0) return FE{0.0, 0};
     if (!isfinite_(x.m)) return FE{x.m, 0};
-    return FE{x.m, sat_exp(static_cast<long long>(x.e) + 1)};
+    return FE{x.m, sat_add(x.e, 1)};
 }
 
 // |Z_i + d|^2 - R^2 = (|Z_i|^2 - R^2) + 2 Re(Z_i conj d) + |d|^2 with the
@@ -230,7 +230,8 @@ FAV_HD int lookup_level(const Step* steps, const LevelDir& d, int start, FE dn, 
         const int64_t k = d.offset[level] + index;
         const Step& s = steps[k];
         const FE r2{s.r2_m, s.r2_e};
-        if (s.length > 1 && fe_finite(r2) && fe_compare(dn, r2) < 0) return static_cast<int>(k);
+        // fe_compare(dn, r2) < 0 with both finite and r2 >= 0
+        if (s.length > 1 && fe_finite(r2) && fe_less_nonneg(dn, r2)) return static_cast<int>(k);
     }
     return -1;
 }
@@ -535,11 +536,12 @@ FAV_HD PixelState init_pixel(const RenderParams& p, size_t index, SC dc, Counter
 
 enum Outcome : int { kDone = 0, kHandoff = 1, kTail = 2 };
 
-// Linear-BLA fast path: the loop of renderer.cpp:12889-13536 restricted to
-// iterations that end in a committed linear block.  Any other branch (glitch,
-// escape, rebase, reference escape, cubic order, no block, replay, retry) is
-// left to the general kernel: the pixel is handed on with the state it had
-// at that loop top.  Interior pixels that reach the cap finish here.
+// Fast path: the loop of renderer.cpp:12889-13536 restricted to iterations
+// that end in a committed linear block or in an exact perturbation step.
+// Any other branch (glitch, escape, rebase, reference escape, cubic order,
+// plain-double tail, replay, retry) is left to the general kernel: the pixel
+// is handed on with its state at that loop top.  Interior pixels that reach
+// the cap finish here.
 template <bool Planes, bool Stats>
 FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st) {
     const OrbitDev& o = p.orbit;
@@ -548,58 +550,72 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
     if (!p.fast_linear) return kHandoff;
     while (s.iteration < max_iter) {
         if (s.ref < 0 || s.ref >= o.size) return kHandoff;
-        SC total = s.total;
-        SN total_norm = s.total_norm;
-        FE margin = s.margin;
         if (!s.have_total) {
-            total = sc_add(load_z(o, s.ref), s.delta);
-            margin = margin_at(o, s.ref, s.delta);
-            total_norm = sn_from_margin(margin, mode);
+            s.total = sc_add(load_z(o, s.ref), s.delta);
+            s.margin = margin_at(o, s.ref, s.delta);
+            s.total_norm = sn_from_margin(s.margin, mode);
         }
+        s.have_total = false;
         const FE glitch{o.glitch_m[s.ref], o.glitch_e[s.ref]};
-        if (glitch.m != 0.0 && fe_compare(FE{total_norm.m, total_norm.e}, glitch) < 0) return kHandoff;
-        if (sc_outside_escape(total_norm, mode) || fe_compare(margin, fe_zero()) > 0) return kHandoff;
+        if (glitch.m != 0.0 && fe_less_nonneg(FE{s.total_norm.m, s.total_norm.e}, glitch)) return kHandoff;
+        if (sc_outside_escape(s.total_norm, mode) || fe_compare(s.margin, fe_zero()) > 0) return kHandoff;
         const SN delta_norm = sc_norm_squared(s.delta);
-        if (sc_compare_norm(total_norm, delta_norm) < 0) return kHandoff;
+        if (sc_compare_norm(s.total_norm, delta_norm) < 0) return kHandoff;
         if (o.map_end > 1 && s.ref >= o.map_end) return kHandoff;
+        const bool bla = !p.disable_bla && !s.pixel_disable_bla;
         const int order = p.approximation_order >= 3 && delta_norm.e < -115
             ? (delta_norm.e < -160 ? 1 : 2) : p.approximation_order;
-        if (order > 1 || p.disable_bla || s.pixel_disable_bla) return kHandoff;
-        const int remaining = max_iter - s.iteration;
-        const int k = lookup_level(p.linear, p.linear_dir, s.ref, FE{delta_norm.m, delta_norm.e},
-                                   p.max_linear_length < remaining ? p.max_linear_length : remaining);
-        if (k < 0) return kHandoff;
-        const LinearStepDev& step = p.linear[k];
-        const int length = step.length;
-        if (s.ref + length > max_iter) return kHandoff;
-        const SC a{step.a_re, step.a_im, step.a_e}, b{step.b_re, step.b_im, step.b_e};
-        const SC delta = sc_add(sc_mul(a, s.delta), sc_mul(b, dc));
-        const int ref = s.ref + length;
-        const SC endpoint = sc_add(load_z(o, ref), delta);
-        const FE endpoint_margin = margin_at(o, ref, delta);
-        const SN endpoint_norm = sn_from_margin(endpoint_margin, mode);
-        if (!sc_finite(delta) || !sc_finite(endpoint)) return kHandoff;
-        if (sc_compare_norm(endpoint_norm, SN{0.75, 2}) >= 0) return kHandoff;
-        // commit, renderer.cpp:13499-13511
-        if constexpr (Planes) s.derivative = sc_add(sc_mul(a, s.derivative), b);
-        s.delta = delta;
-        s.ref = ref;
-        s.iteration += length;
-        s.total = endpoint;
-        s.total_norm = endpoint_norm;
-        s.margin = endpoint_margin;
-        s.previous_total_norm = endpoint_norm;
-        s.have_total = true;
-        st.add(ST_BLA_BLOCKS);
-        st.add(ST_LINEAR_BLOCKS);
-        st.add(ST_SERIES_JUMPS);
-        if constexpr (Stats) {
-            int bucket = 0;
-            unsigned v = static_cast<unsigned>(length);
-            while (v > 1U && bucket < 15) { v >>= 1U; ++bucket; }
-            st.add(ST_HISTOGRAM + bucket);
+        if (bla && order > 1) return kHandoff;  // cubic blocks
+        int k = -1;
+        if (bla) {
+            const int remaining = max_iter - s.iteration;
+            k = lookup_level(p.linear, p.linear_dir, s.ref, FE{delta_norm.m, delta_norm.e},
+                             p.max_linear_length < remaining ? p.max_linear_length : remaining);
         }
-        st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
+        const int length = k >= 0 ? p.linear[k].length : 0;
+        const bool block = length > 0 && s.ref + length <= max_iter;
+        if (bla && !block && delta_norm.e > -90) return kHandoff;  // plain-double tail
+        if (block) {
+            const LinearStepDev& step = p.linear[k];
+            const SC a{step.a_re, step.a_im, step.a_e}, b{step.b_re, step.b_im, step.b_e};
+            const SC delta = sc_add(sc_mul(a, s.delta), sc_mul(b, dc));
+            const int ref = s.ref + length;
+            const SC endpoint = sc_add(load_z(o, ref), delta);
+            const FE endpoint_margin = margin_at(o, ref, delta);
+            const SN endpoint_norm = sn_from_margin(endpoint_margin, mode);
+            if (!sc_finite(delta) || !sc_finite(endpoint)) return kHandoff;
+            if (sc_compare_norm(endpoint_norm, SN{0.75, 2}) >= 0) return kHandoff;
+            // commit, renderer.cpp:13499-13511
+            if constexpr (Planes) s.derivative = sc_add(sc_mul(a, s.derivative), b);
+            s.delta = delta;
+            s.ref = ref;
+            s.iteration += length;
+            s.total = endpoint;
+            s.total_norm = endpoint_norm;
+            s.margin = endpoint_margin;
+            s.previous_total_norm = endpoint_norm;
+            s.have_total = true;
+            st.add(ST_BLA_BLOCKS);
+            st.add(ST_LINEAR_BLOCKS);
+            st.add(ST_SERIES_JUMPS);
+            if constexpr (Stats) {
+                int bucket = 0;
+                unsigned v = static_cast<unsigned>(length);
+                while (v > 1U && bucket < 15) { v >>= 1U; ++bucket; }
+                st.add(ST_HISTOGRAM + bucket);
+            }
+            st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
+            continue;
+        }
+        // exact perturbation step, renderer.cpp:13516-13535
+        const SC z = load_z(o, s.ref);
+        if constexpr (Planes) s.derivative = sc_add(sc_double(sc_mul(s.total, s.derivative)), kOne);
+        s.delta = sc_add(sc_double(sc_mul(z, s.delta)), sc_add(sc_mul(s.delta, s.delta), dc));
+        ++s.ref;
+        ++s.iteration;
+        s.previous_total_norm = s.total_norm;
+        st.add(ST_LOGICAL);
+        st.add(ST_EXACT);
     }
     p.output[index] = encode_iteration(max_iter, p.output_bias);
     st.max(ST_MAX_PIXEL_ITER, static_cast<unsigned long long>(s.iteration > 0 ? s.iteration : 0));
@@ -650,7 +666,7 @@ FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, S
         // glitch: |Z+d|^2 < 1e-7 |Z|^2, renderer.cpp:12931-12988 (the
         // threshold table holds 0 where |Z|^2 is zero or not finite)
         const FE glitch{o.glitch_m[s.ref], o.glitch_e[s.ref]};
-        if (glitch.m != 0.0 && fe_compare(FE{s.total_norm.m, s.total_norm.e}, glitch) < 0) {
+        if (glitch.m != 0.0 && fe_less_nonneg(FE{s.total_norm.m, s.total_norm.e}, glitch)) {
             st.add(ST_GLITCH);
             if (p.allow_recovery && sc_finite(s.total) && ++s.rebases <= 64) {
                 s.delta = s.total;
