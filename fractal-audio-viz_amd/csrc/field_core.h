@@ -537,7 +537,7 @@ FAV_HD PixelState init_pixel(const RenderParams& p, size_t index, SC dc, Counter
 enum Outcome : int { kDone = 0, kHandoff = 1, kTail = 2 };
 
 // Fast path: the loop of renderer.cpp:12889-13536 restricted to iterations
-// that end in a committed linear block or in an exact perturbation step.
+// that end in a committed linear BLA block or in an exact perturbation step.
 // Any other branch (glitch, escape, rebase, reference escape, cubic order,
 // plain-double tail, replay, retry) is left to the general kernel: the pixel
 // is handed on with its state at that loop top.  Interior pixels that reach
@@ -565,20 +565,42 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
         const bool bla = !p.disable_bla && !s.pixel_disable_bla;
         const int order = p.approximation_order >= 3 && delta_norm.e < -115
             ? (delta_norm.e < -160 ? 1 : 2) : p.approximation_order;
-        if (bla && order > 1) return kHandoff;  // cubic blocks
-        int k = -1;
+        // With delta = dc = 0 (the pixel sitting on the reference) every
+        // term of the cubic block but A*delta + B*dc is an exact zero, and so
+        // is every term of its derivative but A*dz + B (renderer.cpp:8686-
+        // 8795), so the block is applied as a linear one; only its lookup
+        // uses the cubic table.
+        const bool on_reference = sc_is_zero(s.delta) && sc_is_zero(dc);
+        if (bla && order > 1 && !on_reference) return kHandoff;  // cubic blocks
+        int k = -1, length = 0;
         if (bla) {
             const int remaining = max_iter - s.iteration;
-            k = lookup_level(p.linear, p.linear_dir, s.ref, FE{delta_norm.m, delta_norm.e},
-                             p.max_linear_length < remaining ? p.max_linear_length : remaining);
+            const FE dn{delta_norm.m, delta_norm.e};
+            if (order <= 1) {
+                k = lookup_level(p.linear, p.linear_dir, s.ref, dn,
+                                 p.max_linear_length < remaining ? p.max_linear_length : remaining);
+                if (k >= 0) length = p.linear[k].length;
+            } else {
+                k = lookup_level(p.cubic, p.cubic_dir, s.ref, dn,
+                                 p.max_bla_length < remaining ? p.max_bla_length : remaining);
+                if (k >= 0) length = p.cubic[k].length;
+            }
         }
-        const int length = k >= 0 ? p.linear[k].length : 0;
         const bool block = length > 0 && s.ref + length <= max_iter;
         if (bla && !block && delta_norm.e > -90) return kHandoff;  // plain-double tail
         if (block) {
-            const LinearStepDev& step = p.linear[k];
-            const SC a{step.a_re, step.a_im, step.a_e}, b{step.b_re, step.b_im, step.b_e};
+            SC a, b;
+            if (order <= 1) {
+                const LinearStepDev& step = p.linear[k];
+                a = SC{step.a_re, step.a_im, step.a_e};
+                b = SC{step.b_re, step.b_im, step.b_e};
+            } else {
+                a = coef(p.cubic[k], 0);
+                b = coef(p.cubic[k], 1);
+            }
             const SC delta = sc_add(sc_mul(a, s.delta), sc_mul(b, dc));
+            SC derivative = s.derivative;
+            if constexpr (Planes) derivative = sc_add(sc_mul(a, s.derivative), b);
             const int ref = s.ref + length;
             const SC endpoint = sc_add(load_z(o, ref), delta);
             const FE endpoint_margin = margin_at(o, ref, delta);
@@ -586,7 +608,7 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
             if (!sc_finite(delta) || !sc_finite(endpoint)) return kHandoff;
             if (sc_compare_norm(endpoint_norm, SN{0.75, 2}) >= 0) return kHandoff;
             // commit, renderer.cpp:13499-13511
-            if constexpr (Planes) s.derivative = sc_add(sc_mul(a, s.derivative), b);
+            s.derivative = derivative;
             s.delta = delta;
             s.ref = ref;
             s.iteration += length;
@@ -596,7 +618,7 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
             s.previous_total_norm = endpoint_norm;
             s.have_total = true;
             st.add(ST_BLA_BLOCKS);
-            st.add(ST_LINEAR_BLOCKS);
+            st.add(order <= 1 ? ST_LINEAR_BLOCKS : ST_CUBIC_BLOCKS);
             st.add(ST_SERIES_JUMPS);
             if constexpr (Stats) {
                 int bucket = 0;

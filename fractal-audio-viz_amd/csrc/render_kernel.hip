@@ -47,7 +47,7 @@ __device__ unsigned wave_grab(unsigned* cursor) {
 }
 
 template <bool Planes, bool Stats>
-__global__ void __launch_bounds__(256) fast_kernel(RenderParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) fast_kernel(RenderParams p) {
     const int wave = threadIdx.x >> 6;
     const int lane = lane_id();
     const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
