@@ -138,6 +138,8 @@ void ensure_device_tables(ReferenceContext& c) {
     const size_t off_mm = append(blob, o.margin_m), off_me = append(blob, o.margin_e);
     const size_t off_gm = append(blob, o.glitch_m), off_ge = append(blob, o.glitch_e);
     const size_t off_rd = append(blob, o.real_double), off_id = append(blob, o.imag_double);
+    const size_t off_znm = append(blob, o.zn_m), off_zne = append(blob, o.zn_e);
+    const size_t off_small = append(blob, o.small_exp), off_flags = append(blob, o.flags);
     const unsigned char* base = nullptr;
     upload_vector(blob, t->orbit_block, base, s.stream, t->bytes);
     t->orbit.re = reinterpret_cast<const double*>(base + off_re);
@@ -153,6 +155,10 @@ void ensure_device_tables(ReferenceContext& c) {
     t->orbit.glitch_e = reinterpret_cast<const int32_t*>(base + off_ge);
     t->orbit.real_double = reinterpret_cast<const double*>(base + off_rd);
     t->orbit.imag_double = reinterpret_cast<const double*>(base + off_id);
+    t->orbit.zn_m = reinterpret_cast<const double*>(base + off_znm);
+    t->orbit.zn_e = reinterpret_cast<const int32_t*>(base + off_zne);
+    t->orbit.small_exp = reinterpret_cast<const int32_t*>(base + off_small);
+    t->orbit.flags = reinterpret_cast<const int32_t*>(base + off_flags);
     t->orbit.size = static_cast<int32_t>(o.scaled.size());
     t->orbit.map_end = c.map_end;
     std::vector<unsigned char> sblob;

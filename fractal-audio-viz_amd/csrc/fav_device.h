@@ -60,6 +60,10 @@ struct OrbitDev {
     const int32_t* glitch_e;
     const double* real_double;  // ldexp(re, e): the plain-double tail orbit
     const double* imag_double;
+    const double* zn_m;         // R^2 + (|Z|^2 - R^2): the loop-top norm for a negligible delta
+    const int32_t* zn_e;
+    const int32_t* small_exp;   // delta.e <= small_exp[n]: Z + delta, margin and norm are Z's own
+    const int32_t* flags;       // bit 0: glitch test fires, bit 1: escape test fires (negligible delta)
     int32_t size;
     int32_t map_end;
 };
@@ -88,6 +92,8 @@ struct StateStore {
 };
 
 // Bytes of StateStore storage per pixel (9 doubles, 9 ints, 2 queue slots).
+constexpr int32_t kOrbitGlitchFlag = 1;
+constexpr int32_t kOrbitEscapeFlag = 2;
 constexpr size_t kStateBytesPerPixel = 9 * 8 + 9 * 4 + 2 * 4;
 
 // Everything one field launch needs, passed by value as the kernel argument.

@@ -24,6 +24,13 @@ struct OrbitData {
     std::vector<double> real_double, imag_double;
     std::vector<double> zr_m, zi_m, margin_m, glitch_m;
     std::vector<int32_t> zr_e, zi_e, margin_e, glitch_e;
+    // Values of the loop top for a perturbation negligible against Z_n
+    // (see negligible_delta_exponent in reference_build.cpp): |Z_n|^2 as the
+    // norm R^2 + margin, the largest delta exponent for which Z_n + delta,
+    // the margin and that norm round to the orbit-only values, and the
+    // glitch / escape outcomes of the loop-top tests for them.
+    std::vector<double> zn_m;
+    std::vector<int32_t> zn_e, small_exp, flags;
 };
 
 struct CubicLevels {

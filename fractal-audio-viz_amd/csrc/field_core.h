@@ -128,6 +128,23 @@ FAV_HD FE margin_at(const OrbitDev& o, int i, SC delta) {
 
 FAV_HD SC load_z(const OrbitDev& o, int i) { return SC{o.re[i], o.im[i], o.e[i]}; }
 
+// Z_i + delta, the escape margin and the loop-top norm R^2 + margin
+// (renderer.cpp:12915-12923).  When delta is zero or delta.e <= small_exp[i]
+// all three equal the orbit-only values (the addends involving delta are
+// more than 60 binades below the terms they are added to, where FloatExp
+// sums return the larger operand), so they are read from the tables.
+FAV_HD void loop_top_values(const OrbitDev& o, int i, SC delta, int mode, SC& total, FE& margin, SN& norm) {
+    if (sc_is_zero(delta) || delta.e <= o.small_exp[i]) {
+        total = load_z(o, i);
+        margin = FE{o.margin_m[i], o.margin_e[i]};
+        norm = SN{o.zn_m[i], o.zn_e[i]};
+    } else {
+        total = sc_add(load_z(o, i), delta);
+        margin = margin_at(o, i, delta);
+        norm = sn_from_margin(margin, mode);
+    }
+}
+
 FAV_HD float encode_iteration(int iteration, double bias) {
     return static_cast<float>(static_cast<double>(iteration) - bias);
 }
@@ -550,11 +567,7 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
     if (!p.fast_linear) return kHandoff;
     while (s.iteration < max_iter) {
         if (s.ref < 0 || s.ref >= o.size) return kHandoff;
-        if (!s.have_total) {
-            s.total = sc_add(load_z(o, s.ref), s.delta);
-            s.margin = margin_at(o, s.ref, s.delta);
-            s.total_norm = sn_from_margin(s.margin, mode);
-        }
+        if (!s.have_total) loop_top_values(o, s.ref, s.delta, mode, s.total, s.margin, s.total_norm);
         s.have_total = false;
         const FE glitch{o.glitch_m[s.ref], o.glitch_e[s.ref]};
         if (glitch.m != 0.0 && fe_less_nonneg(FE{s.total_norm.m, s.total_norm.e}, glitch)) return kHandoff;
@@ -602,9 +615,10 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
             SC derivative = s.derivative;
             if constexpr (Planes) derivative = sc_add(sc_mul(a, s.derivative), b);
             const int ref = s.ref + length;
-            const SC endpoint = sc_add(load_z(o, ref), delta);
-            const FE endpoint_margin = margin_at(o, ref, delta);
-            const SN endpoint_norm = sn_from_margin(endpoint_margin, mode);
+            SC endpoint;
+            FE endpoint_margin;
+            SN endpoint_norm;
+            loop_top_values(o, ref, delta, mode, endpoint, endpoint_margin, endpoint_norm);
             if (!sc_finite(delta) || !sc_finite(endpoint)) return kHandoff;
             if (sc_compare_norm(endpoint_norm, SN{0.75, 2}) >= 0) return kHandoff;
             // commit, renderer.cpp:13499-13511
@@ -679,11 +693,7 @@ FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, S
             escaped = true;
             break;
         }
-        if (!s.have_total) {
-            s.total = sc_add(load_z(o, s.ref), s.delta);
-            s.margin = margin_at(o, s.ref, s.delta);
-            s.total_norm = sn_from_margin(s.margin, mode);
-        }
+        if (!s.have_total) loop_top_values(o, s.ref, s.delta, mode, s.total, s.margin, s.total_norm);
         s.have_total = false;
         // glitch: |Z+d|^2 < 1e-7 |Z|^2, renderer.cpp:12931-12988 (the
         // threshold table holds 0 where |Z|^2 is zero or not finite)
@@ -824,9 +834,10 @@ FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, S
             }
             s.ref += map_length;
             s.iteration += map_length;
-            const SC endpoint = sc_add(load_z(o, s.ref), s.delta);
-            const FE endpoint_margin = margin_at(o, s.ref, s.delta);
-            const SN endpoint_norm = sn_from_margin(endpoint_margin, mode);
+            SC endpoint;
+            FE endpoint_margin;
+            SN endpoint_norm;
+            loop_top_values(o, s.ref, s.delta, mode, endpoint, endpoint_margin, endpoint_norm);
             if (!sc_finite(s.delta) || !sc_finite(endpoint)) {
                 s.delta = previous_delta; s.derivative = previous_derivative;
                 s.ref = previous_ref; s.iteration = previous_iteration;

@@ -46,6 +46,10 @@ extern "C" int fav_host_check_render(
         p.orbit.glitch_e = o.glitch_e.data();
         p.orbit.real_double = o.real_double.data();
         p.orbit.imag_double = o.imag_double.data();
+        p.orbit.zn_m = o.zn_m.data();
+        p.orbit.zn_e = o.zn_e.data();
+        p.orbit.small_exp = o.small_exp.data();
+        p.orbit.flags = o.flags.data();
         p.series.re = img.series_re.data();
         p.series.im = img.series_im.data();
         p.series.e = img.series_e.data();

@@ -71,6 +71,34 @@ struct Mpfr {
     operator mpfr_ptr() { return v; }
 };
 
+// Largest exponent t such that every normalised perturbation d with
+// d.e <= t leaves the loop-top values of orbit index n unchanged:
+// sc_add(Z, d) returns Z, and the escape margin
+//   fe_add(fe_add(M, cross), dn),  cross = 2(zr*dr + zi*di),  dn = dr^2 + di^2
+// (renderer.cpp:8383-8431) returns M = |Z|^2 - R^2.  Each FloatExp sum
+// returns its larger operand unchanged when the exponents differ by more
+// than 60 (renderer.cpp:8209, 7688), and for normalised operands
+//   cross.e <= Z.e + d.e + 2   and   dn.e <= 2 d.e + 1
+// (a product's exponent is at most the sum of its factors', a sum's at most
+// one above the larger addend's, and the parts of Z and d have exponents at
+// most Z.e and d.e).  So the three conditions
+//   Z.e - d.e > 60,  M.e - (Z.e + d.e + 2) > 60,  M.e - (2 d.e + 1) > 60
+// make the loop top take Z, M and R^2 + M exactly.  INT_MIN when Z or M is
+// zero or the exponents are too large to bound safely.
+int32_t negligible_delta_exponent(SC z, FE margin) {
+    if ((z.re == 0.0 && z.im == 0.0) || margin.m == 0.0 || !fe_finite(margin) || !sc_finite(z)) return INT_MIN;
+    const long long limit = 1LL << 28;
+    if (z.e > limit || z.e < -limit || margin.e > limit || margin.e < -limit) return INT_MIN;
+    const long long a = static_cast<long long>(z.e) - 61;
+    const long long b = static_cast<long long>(margin.e) - z.e - 63;
+    const long long c2 = static_cast<long long>(margin.e) - 62;  // 2t <= c2
+    const long long c = c2 >= 0 ? c2 / 2 : -((-c2 + 1) / 2);     // floor(c2 / 2)
+    long long t = a < b ? a : b;
+    if (c < t) t = c;
+    if (t < -limit) t = -limit;
+    return static_cast<int32_t>(t);
+}
+
 // make_reference_orbit, Mandelbrot branch, renderer.cpp:10640-10922.
 void make_orbit(ReferenceContext& c, const char* x, const char* y, const char* zoom, int max_iter) {
     const long bits = c.precision_bits;
@@ -127,6 +155,10 @@ void make_orbit(ReferenceContext& c, const char* x, const char* y, const char* z
     orbit->zi_e.resize(static_cast<size_t>(n));
     orbit->margin_e.resize(static_cast<size_t>(n));
     orbit->glitch_e.resize(static_cast<size_t>(n));
+    orbit->zn_m.resize(static_cast<size_t>(n));
+    orbit->zn_e.resize(static_cast<size_t>(n));
+    orbit->small_exp.resize(static_cast<size_t>(n));
+    orbit->flags.resize(static_cast<size_t>(n));
     const FE glitch_scale = fe_from_parts(1.0e-7, 0);
     const int mode = c.escape_radius_mode;
     parallel_for(static_cast<size_t>(n), [&](size_t i) {
@@ -142,12 +174,20 @@ void make_orbit(ReferenceContext& c, const char* x, const char* y, const char* z
         orbit->margin_m[i] = margin.m; orbit->margin_e[i] = margin.e;
         // glitch threshold 1e-7*|Z|^2 when |Z|^2 is nonzero and finite
         // (renderer.cpp:12931-12942); zero disables the test
-        const SN zn = sc_norm_squared(z);
-        const FE reference_norm{zn.m, zn.e};
+        const SN zsq = sc_norm_squared(z);
+        const FE reference_norm{zsq.m, zsq.e};
         FE threshold{0.0, 0};
         if (reference_norm.m != 0.0 && fe_finite(reference_norm))
             threshold = fe_mul(reference_norm, glitch_scale);
         orbit->glitch_m[i] = threshold.m; orbit->glitch_e[i] = threshold.e;
+        const FE zn = fe_add(fe_escape_r2(mode), margin);  // sn_from_margin(margin)
+        orbit->zn_m[i] = zn.m;
+        orbit->zn_e[i] = zn.e;
+        orbit->small_exp[i] = negligible_delta_exponent(z, margin);
+        int32_t f = 0;
+        if (threshold.m != 0.0 && fe_compare(zn, threshold) < 0) f |= kOrbitGlitchFlag;
+        if (sc_outside_escape(SN{zn.m, zn.e}, mode) || fe_compare(margin, fe_zero()) > 0) f |= kOrbitEscapeFlag;
+        orbit->flags[i] = f;
     });
     c.orbit = std::move(orbit);
 }
