@@ -4,6 +4,7 @@
 // thread-local error text follow the reference's C ABI (renderer.h,
 // renderer.cpp:15870-15924, 19568-20474).  No exception crosses the ABI.
 #include <algorithm>
+#include <cstddef>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -18,6 +19,7 @@
 #include "../../include/fractal_hip.h"
 #include "fav_host.h"
 #include "fav_render.h"
+#include "kfp_prepare.h"
 
 namespace {
 
@@ -267,9 +269,72 @@ int one_shot(float* output, int width, int height, const char* zoom, const char*
     }
 }
 
+// fractal_colourise_kfp_impl for a scalar field, renderer.cpp:16314-16545.
+int colourise_kfp(const float* field, uint8_t* output, int width, int height, int max_iter, double phase, double vocal,
+                  double instrumental, double pitch, const FractalKfpOptions* options, const FractalKfpPlanes* planes,
+                  const uint8_t* lut, int lut_size, int threads, bool device_pointers) {
+    try {
+        if (!field || !output || !valid_dimensions(width, height) || !valid_iterations(max_iter)
+            || !valid_threads(threads) || !std::isfinite(phase) || !std::isfinite(vocal)
+            || !std::isfinite(instrumental) || !std::isfinite(pitch) || !lut || !fav::kfp::valid_options(options, lut_size))
+            throw std::runtime_error("invalid native KFP colour dimensions or controls");
+        if (planes != nullptr)
+            throw std::runtime_error(
+                "the HIP KFP colour pass takes the scalar field only; orbit/DE/phase/texture planes are not supported");
+        const fav::kfp::KfpParams k = fav::kfp::make_params(*options, field, output, width, height, max_iter, lut,
+                                                            lut_size);
+        fav::colourise_kfp_on_device(k, device_pointers);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native KFP colouriser failed with an unknown exception");
+        return 1;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+int fractal_colourise_kfp(const float* field, uint8_t* output, int width, int height, int max_iter, double phase,
+                          double vocal, double instrumental, double pitch, const FractalKfpOptions* options,
+                          const uint8_t* lut, int lut_size, int threads) {
+    return colourise_kfp(field, output, width, height, max_iter, phase, vocal, instrumental, pitch, options, nullptr,
+                         lut, lut_size, threads, false);
+}
+
+int fractal_colourise_kfp_opencl(const float* field, uint8_t* output, int width, int height, int max_iter,
+                                 double phase, double vocal, double instrumental, double pitch,
+                                 const FractalKfpOptions* options, const uint8_t* lut, int lut_size, int threads) {
+    return colourise_kfp(field, output, width, height, max_iter, phase, vocal, instrumental, pitch, options, nullptr,
+                         lut, lut_size, threads, false);
+}
+
+int fractal_colourise_kfp_precise(const float* field, uint8_t* output, int width, int height, int max_iter,
+                                  double phase, double vocal, double instrumental, double pitch,
+                                  const FractalKfpOptions* options, const uint8_t* lut, int lut_size, int threads) {
+    return colourise_kfp(field, output, width, height, max_iter, phase, vocal, instrumental, pitch, options, nullptr,
+                         lut, lut_size, threads, false);
+}
+
+int fractal_colourise_kfp_planes(const float* field, uint8_t* output, int width, int height, int max_iter,
+                                 double phase, double vocal, double instrumental, double pitch,
+                                 const FractalKfpOptions* options, const FractalKfpPlanes* planes,
+                                 const uint8_t* lut, int lut_size, int threads) {
+    return colourise_kfp(field, output, width, height, max_iter, phase, vocal, instrumental, pitch, options, planes,
+                         lut, lut_size, threads, false);
+}
+
+int fractal_hip_colourise_kfp_device(const float* device_field, uint8_t* device_rgb, int width, int height,
+                                     int max_iter, const FractalKfpOptions* options, const uint8_t* lut,
+                                     int lut_size) {
+    return colourise_kfp(device_field, device_rgb, width, height, max_iter, 0.0, 0.0, 0.0, 0.0, options, nullptr,
+                         lut, lut_size, 0, true);
+}
+
 
 int fractal_abi_version(void) { return FRACTAL_ABI_VERSION; }
 

@@ -14,11 +14,13 @@
 #include "fav_host.h"
 #include "fav_render.h"
 #include "table_image.h"
+#include "kfp_core.h"
 
 namespace fav {
 
 hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, int persistent_blocks,
                                hipStream_t stream);
+hipError_t launch_kfp_kernel(const kfp::KfpParams& params, void* scratch, int compute_units, hipStream_t stream);
 
 namespace {
 
@@ -57,7 +59,7 @@ struct LaunchState {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    DeviceBuffer offsets, output, planes, stats, budget, state;
+    DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch;
     int compute_units = 256;
     double wall_clock_khz = 0.0;
     LastTiming timing{};
@@ -279,6 +281,36 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
     s.timing.pixels = static_cast<double>(pixels);
     if (r.stats) r.stats[22] = static_cast<uint64_t>(static_cast<double>(kernel_ms) * 1e6);  // render_ns
+}
+
+void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const size_t pixels = static_cast<size_t>(host.width) * static_cast<size_t>(host.height);
+    kfp::KfpParams k = host;
+    if (!device_pointers) {
+        void* f = s.kfp_field.get(pixels * sizeof(float));
+        check(hipMemcpyAsync(f, host.field, pixels * sizeof(float), hipMemcpyHostToDevice, s.stream), "H2D field");
+        k.field = static_cast<const float*>(f);
+        k.output = static_cast<uint8_t*>(s.kfp_rgb.get(pixels * 3));
+    }
+    void* l = s.kfp_lut.get(static_cast<size_t>(host.lut_size) * 3);
+    check(hipMemcpyAsync(l, host.lut, static_cast<size_t>(host.lut_size) * 3, hipMemcpyHostToDevice, s.stream), "H2D lut");
+    k.lut = static_cast<const uint8_t*>(l);
+    void* scratch = s.kfp_scratch.get(64);
+    const uint64_t wall0 = now_ns();
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_kfp_kernel(k, scratch, s.compute_units, s.stream), "KFP colour launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    if (!device_pointers)
+        check(hipMemcpyAsync(host.output, k.output, pixels * 3, hipMemcpyDeviceToHost, s.stream), "D2H rgb");
+    check(hipStreamSynchronize(s.stream), "KFP colour");
+    float kernel_ms = 0.0f;
+    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    s.timing.kernel_ms = kernel_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
 }
 
 LastTiming last_timing() {

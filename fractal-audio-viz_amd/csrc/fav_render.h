@@ -29,6 +29,12 @@ struct LastTiming {
 };
 
 void render_field(ReferenceContext& context, const FieldRequest& request);
+
+// KFP colour transfer on the device.  `params` carries host pointers for
+// field / lut / output unless device_pointers is set (then field and output
+// are device pointers; the LUT is always a host pointer).
+namespace kfp { struct KfpParams; }
+void colourise_kfp_on_device(const kfp::KfpParams& params, bool device_pointers);
 LastTiming last_timing();
 bool device_available();
 

@@ -4,6 +4,7 @@
 // CPU, so tests can check the kernel's logic (and any rewrite of it) against
 // the oracle bit for bit without a GPU.  It is built into its own shared
 // object (_build/libfav_host_check.so) that the product library never loads.
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -138,6 +139,48 @@ extern "C" int fav_host_check_render(
         return 0;
     } catch (const std::exception& e) {
         std::snprintf(error, static_cast<size_t>(error_capacity), "%s", e.what());
+        return 1;
+    }
+}
+
+#include "kfp_prepare.h"
+
+// KFP colour pass of kfp_kernel.hip run on the CPU (test-only): the same
+// option validation and per-pixel code, with the method-4 range reduced
+// serially.  Returns 1 with `err` set when the options are rejected.
+extern "C" int fav_host_check_kfp(const float* field, uint8_t* output, int width, int height, int max_iter,
+                                  const FractalKfpOptions* options, const uint8_t* lut, int lut_size, char* err,
+                                  int cap) {
+    try {
+        if (!fav::kfp::valid_options(options, lut_size)) throw std::runtime_error("invalid KFP options");
+        fav::kfp::KfpParams k =
+            fav::kfp::make_params(*options, field, output, width, height, max_iter, lut, lut_size);
+        double bounds[2] = {0.0, 1.0};
+        if (k.color_method == 4) {
+            long long lo = 0, hi = 0;
+            bool any = false;
+            for (size_t i = 0; i < static_cast<size_t>(width) * height; ++i) {
+                long long v;
+                if (!fav::kfp::transfer_bound_value(k, i, v)) continue;
+                lo = any ? std::min(lo, v) : v;
+                hi = any ? std::max(hi, v) : v;
+                any = true;
+            }
+            if (any) {
+                bounds[0] = static_cast<double>(lo);
+                bounds[1] = static_cast<double>(hi);
+            }
+        }
+        k.bounds = bounds;
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < height; ++y)
+            for (int x = 0; x < width; ++x) fav::kfp::colour_pixel(k, x, y);
+        return 0;
+    } catch (const std::exception& e) {
+        if (err && cap > 0) {
+            std::strncpy(err, e.what(), static_cast<size_t>(cap) - 1);
+            err[cap - 1] = 0;
+        }
         return 1;
     }
 }

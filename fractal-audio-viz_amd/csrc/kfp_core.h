@@ -1,0 +1,304 @@
+// kfp_core.h -- per-pixel Kalles SetColor transfer of a scalar iteration
+// field (renderer.cpp:6696-7289 without orbit planes, phase or texture).
+//
+// Compiled for the gfx950 colour kernel (kfp_kernel.hip) and, unchanged, for
+// the test-only CPU harness.  Double and float stages follow the reference's
+// types and operation order (-ffp-contract=off); the transcendental
+// functions are the platform's, which is why device output may differ from
+// the CPU by one 8-bit level at a quantisation boundary.
+#pragma once
+
+#include <cstdint>
+
+#include "fav_numeric.h"
+
+namespace fav {
+namespace kfp {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// Per-launch constants (kfp_pixel_context / kfp_slope_direction /
+// kfp_transfer_bounds results, renderer.cpp:5723-5763, 7296-7334, 7509-7514).
+struct KfpParams {
+    const float* field;
+    const uint8_t* lut;
+    uint8_t* output;
+    const double* bounds;  // [0] minimum, [1] maximum for colour method 4
+    int32_t width, height, max_iter, lut_size;
+    double field_bias, smooth_offset, iter_div, color_offset;
+    double slope_power, slope_ratio, cosine, sine;
+    int32_t color_method, differences, smooth, flat, inverse_transition;
+    int32_t needs_difference, needs_slopes;
+    int32_t multi_color, blend_multi_color, multi_color_count;
+    int32_t interior_color[3];
+    double multi_color_period[256];
+    int32_t multi_color_type[256];
+};
+
+FAV_HD double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+FAV_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+FAV_HD double maxd(double a, double b) { return a < b ? b : a; }  // std::max
+FAV_HD double mind(double a, double b) { return b < a ? b : a; }  // std::min
+
+// kfp_colour_dither_mask + kfp_dithered_srgb_byte, renderer.cpp:5246-5336.
+FAV_HD uint8_t dithered_srgb_byte(float value, int x, int y, int channel) {
+    if (!__builtin_isfinite(value)) value = 0.0F;
+    const uint32_t coordinate = (static_cast<uint32_t>(x) + static_cast<uint32_t>(channel) * 67U
+                                 + static_cast<uint32_t>(y) * 236U) * 119U;
+    const float mask = static_cast<float>(coordinate & 255U) / 256.0F;
+    const float quantized = 255.0F * value + mask;
+    return static_cast<uint8_t>(clampi(static_cast<int>(::floorf(quantized)), 0, 255));
+}
+
+FAV_HD bool inside_value(float value, int max_iter, double bias) {  // renderer.cpp:5379-5387
+    return __builtin_isfinite(value) && static_cast<double>(value) >= static_cast<double>(max_iter) - bias;
+}
+
+// kfp_colour_sample, renderer.cpp:5434-5460 (the materialised stencil value).
+FAV_HD double colour_sample(const KfpParams& k, int x, int y) {
+    x = clampi(x, 0, k.width - 1);
+    y = clampi(y, 0, k.height - 1);
+    const float value = k.field[static_cast<size_t>(y) * static_cast<size_t>(k.width) + static_cast<size_t>(x)];
+    if (!__builtin_isfinite(value)) return maxd(0.0, k.smooth_offset);
+    if (static_cast<double>(value) >= static_cast<double>(k.max_iter) - k.field_bias)
+        return static_cast<double>(k.max_iter) + 1.0;
+    return maxd(0.0, clampd(static_cast<double>(value) + k.field_bias, 0.0, static_cast<double>(k.max_iter))
+                         + k.smooth_offset);
+}
+
+FAV_HD double reflected(const KfpParams& k, int x, int y, int ox, int oy, double centre) {
+    const int sx = x + ox, sy = y + oy;
+    if (sx >= 0 && sx < k.width && sy >= 0 && sy < k.height) return colour_sample(k, sx, sy);
+    const int px = x - ox, py = y - oy;
+    if (px >= 0 && px < k.width && py >= 0 && py < k.height) return 2.0 * centre - colour_sample(k, px, py);
+    return centre;
+}
+
+// kfp_difference_magnitude, renderer.cpp:5497-5589.
+FAV_HD double difference_magnitude(int d, double c, double l, double r, double u, double dn, double tl, double tr,
+                                   double bl, double br) {
+    const double inverse_sqrt_two = 0.7071067811865475244008443621048490;
+    if (d == 0) return __builtin_fabs(l - c) * 1.414 + __builtin_fabs(u - c) * 1.414 + __builtin_fabs(tl - c)
+        + __builtin_fabs(bl - c);
+    if (d == 1) {
+        const double sq = (l - c) * (l - c) + (r - c) * (r - c) + (u - c) * (u - c) + (dn - c) * (dn - c)
+            + ((tl - c) * (tl - c) + (br - c) * (br - c)) * inverse_sqrt_two * inverse_sqrt_two
+            + ((bl - c) * (bl - c) + (tr - c) * (tr - c)) * inverse_sqrt_two * inverse_sqrt_two;
+        return ::sqrt(maxd(0.0, sq * 0.25)) * 2.8284271247461903;
+    }
+    if (d == 2) {
+        const double sq = (r - l) * (r - l) * 0.25 + (dn - u) * (dn - u) * 0.25 + (br - tl) * (br - tl) * 0.125
+            + (tr - bl) * (tr - bl) * 0.125;
+        return ::sqrt(maxd(0.0, sq * 0.5)) * 2.8284271247461903;
+    }
+    if (d == 3) {
+        const double sq = (tl - c) * (tl - c) / 2.0 + (l - u) * (l - u) / 2.0;
+        return ::sqrt(maxd(0.0, sq)) * 2.8284271247461903;
+    }
+    if (d == 4) {
+        const double dx = ((u - tl) + (c - l)) * 0.5;
+        const double dy = ((l - tl) + (c - u)) * 0.5;
+        return ::sqrt(dx * dx + dy * dy) * 2.8284271247461903;
+    }
+    if (d == 5) {
+        const double dx = (r + tr + br - l - tl - bl) / 6.0;
+        const double dy = (dn + bl + br - u - tl - tr) / 6.0;
+        return ::sqrt(dx * dx + dy * dy) * 2.8284271247461903;
+    }
+    if (d == 6) {
+        const double lap = tl + 4.0 * u + tr + 4.0 * l - 20.0 * c + 4.0 * r + bl + 4.0 * dn + br;
+        return ::sqrt(__builtin_fabs(lap / 6.0 * 1.4426950408889634)) * 2.8284271247461903;
+    }
+    return 0.0;  // Differences_Analytic without DE planes
+}
+
+// kfp_hsv_to_rgb, renderer.cpp:5591-5644 (float, as Kalles' colour.h).
+FAV_HD void hsv_to_rgb(float hue, float saturation, float value, float& r, float& g, float& b) {
+    const float scaled_hue = hue * 6.0F;
+    const int sector = static_cast<int>(::floorf(scaled_hue));
+    float fraction = scaled_hue - static_cast<float>(sector);
+    if ((sector & 1) == 0) fraction = 1.0F - fraction;
+    const float minimum = value * (1.0F - saturation);
+    const float transition = value * (1.0F - saturation * fraction);
+    r = 0.0F; g = 0.0F; b = 0.0F;
+    switch (sector) {
+        case 0: case 6: r = minimum; g = transition; b = value; break;
+        case 1: r = minimum; g = value; b = transition; break;
+        case 2: r = transition; g = value; b = minimum; break;
+        case 3: r = value; g = transition; b = minimum; break;
+        case 4: r = value; g = minimum; b = transition; break;
+        case 5: r = transition; g = minimum; b = value; break;
+        default: break;
+    }
+}
+
+// palette_rgb, renderer.cpp:7070-7110: cyclic LUT lookup with optional
+// linear blend of the two neighbouring entries.
+FAV_HD void palette_rgb(const KfpParams& k, double input, float& r, float& g, float& b) {
+    double position = ::fmod(input, static_cast<double>(k.lut_size));
+    if (position < 0.0) position += static_cast<double>(k.lut_size);
+    const double lower_position = ::floor(position);
+    const int lower = clampi(static_cast<int>(lower_position), 0, k.lut_size - 1);
+    const uint8_t* lut = k.lut;
+    if (k.smooth) {
+        double fraction = position - lower_position;
+        if (k.inverse_transition) fraction = 1.0 - fraction;
+        const int upper = (lower + 1) % k.lut_size;
+        const double inv = 1.0 - fraction;
+        r = static_cast<float>((static_cast<double>(lut[lower * 3]) * inv + static_cast<double>(lut[upper * 3]) * fraction)
+                               / 255.0F);
+        g = static_cast<float>((static_cast<double>(lut[lower * 3 + 1]) * inv
+                                + static_cast<double>(lut[upper * 3 + 1]) * fraction) / 255.0F);
+        b = static_cast<float>((static_cast<double>(lut[lower * 3 + 2]) * inv
+                                + static_cast<double>(lut[upper * 3 + 2]) * fraction) / 255.0F);
+    } else {
+        r = static_cast<float>(lut[lower * 3]) / 255.0F;
+        g = static_cast<float>(lut[lower * 3 + 1]) / 255.0F;
+        b = static_cast<float>(lut[lower * 3 + 2]) / 255.0F;
+    }
+}
+
+// write_kfp_pixel without planes, renderer.cpp:6696-7289.
+FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
+    const size_t idx = static_cast<size_t>(y) * static_cast<size_t>(k.width) + static_cast<size_t>(x);
+    uint8_t* dst = k.output + idx * 3U;
+    const float raw = k.field[idx];
+    const bool inside = inside_value(raw, k.max_iter, k.field_bias);
+    if (inside) {
+        dst[0] = static_cast<uint8_t>(k.interior_color[0]);
+        dst[1] = static_cast<uint8_t>(k.interior_color[1]);
+        dst[2] = static_cast<uint8_t>(k.interior_color[2]);
+        return;
+    }
+    // the centre sample (materialised or not, the value is the same for an
+    // escaped pixel)
+    const double safe = __builtin_isfinite(raw)
+        ? clampd(static_cast<double>(raw) + k.field_bias, 0.0, static_cast<double>(k.max_iter)) : 0.0;
+    const double smooth_iter = maxd(0.0, safe + k.smooth_offset);
+    const double colour_iter = k.flat ? ::floor(smooth_iter) : smooth_iter;
+    const double c = smooth_iter;
+    double l = c, r = c, u = c, dn = c, tl = c, tr = c, bl = c, br = c;
+    if (k.needs_difference || k.needs_slopes) {
+        l = reflected(k, x, y, -1, 0, c);
+        u = reflected(k, x, y, 0, -1, c);
+        if (k.needs_difference) {
+            switch (k.differences) {
+                case 0:
+                    tl = reflected(k, x, y, -1, -1, c);
+                    bl = reflected(k, x, y, -1, 1, c);
+                    break;
+                case 1: case 2: case 5: case 6:
+                    r = reflected(k, x, y, 1, 0, c);
+                    dn = reflected(k, x, y, 0, 1, c);
+                    tl = reflected(k, x, y, -1, -1, c);
+                    tr = reflected(k, x, y, 1, -1, c);
+                    bl = reflected(k, x, y, -1, 1, c);
+                    br = reflected(k, x, y, 1, 1, c);
+                    break;
+                case 3: case 4:
+                    tl = reflected(k, x, y, -1, -1, c);
+                    break;
+                default:
+                    r = reflected(k, x, y, 1, 0, c);
+                    dn = reflected(k, x, y, 0, 1, c);
+                    break;
+            }
+        }
+        if (k.needs_slopes) {
+            if (x == 0) r = reflected(k, x, y, 1, 0, c);
+            if (y == 0) dn = reflected(k, x, y, 0, 1, c);
+        }
+    }
+    const double gradient = k.needs_difference
+        ? difference_magnitude(k.differences, c, l, r, u, dn, tl, tr, bl, br) : 0.0;
+    double distance = __builtin_isnan(gradient) ? 0.0 : gradient * static_cast<double>(k.width) / 640.0;
+    if (distance < 0.0) distance = 0.0;
+    double transfer = colour_iter;
+    switch (k.color_method) {
+        case 1: transfer = ::sqrt(maxd(0.0, colour_iter)); break;
+        case 2: transfer = ::pow(maxd(0.0, colour_iter), 1.0 / 3.0); break;
+        case 3: transfer = ::log(maxd(1.0, colour_iter)); break;
+        case 4:
+            transfer = 1024.0 * (colour_iter - k.bounds[0]) / maxd(k.bounds[1] - k.bounds[0], 1.0e-12);
+            break;
+        case 5: transfer = mind(distance, 1024.0); break;
+        case 6: {
+            const double dt = mind(::sqrt(maxd(0.0, distance)), 1024.0);
+            transfer = dt > k.iter_div ? smooth_iter : dt;
+            break;
+        }
+        case 7: transfer = ::log(maxd(1.0, distance + 1.0)); break;
+        case 8: transfer = ::sqrt(maxd(0.0, distance)); break;
+        case 9: transfer = ::log(1.0 + ::log(1.0 + maxd(0.0, colour_iter))); break;
+        case 10: transfer = ::atan(colour_iter); break;
+        case 11: transfer = ::sqrt(::sqrt(maxd(0.0, colour_iter))); break;
+        default: break;
+    }
+    if (__builtin_isnan(transfer) || transfer < 0.0) transfer = 0.0;
+    if (k.color_method == 5 || k.color_method == 7 || k.color_method == 8) transfer = clampd(transfer, 0.0, 1024.0);
+    double palette_value = transfer;
+    if (k.iter_div != 1.0) palette_value /= k.iter_div;
+    const double base = palette_value + k.color_offset;
+    float red, green, blue;
+    if (k.multi_color) {
+        const double wave_input = k.smooth ? base : ::floor(base);
+        double hs = 0.0, ss = 0.0, vs = 0.0;
+        int hc = 0, sc = 0, vc = 0;
+        for (int i = 0; i < k.multi_color_count; ++i) {
+            const double period = k.multi_color_period[i];
+            const double wave = period < 0.0 ? -period / 100.0 : 0.5 + 0.5 * ::sin(kPi * wave_input / period);
+            if (k.multi_color_type[i] == 0) { hs += wave; ++hc; }
+            else if (k.multi_color_type[i] == 1) { ss += wave; ++sc; }
+            else { vs += wave; ++vc; }
+        }
+        float mr, mg, mb;
+        hsv_to_rgb(static_cast<float>(hc > 0 ? hs / hc : 0.0), static_cast<float>(sc > 0 ? ss / sc : 0.0),
+                   static_cast<float>(vc > 0 ? vs / vc : 0.0), mr, mg, mb);
+        if (k.blend_multi_color) {
+            palette_rgb(k, base, red, green, blue);
+            red = (red + mr) * 0.5F;
+            green = (green + mg) * 0.5F;
+            blue = (blue + mb) * 0.5F;
+        } else {
+            red = mr; green = mg; blue = mb;
+        }
+    } else {
+        palette_rgb(k, base, red, green, blue);
+    }
+    if (k.needs_slopes) {
+        const double horizontal = x > 0 ? l - c : c - r;
+        const double vertical = y == 0 ? c - dn : u - c;
+        const double projected = ((horizontal * k.cosine) + (vertical * k.sine)) * k.slope_power
+            * static_cast<double>(k.width) / 640.0;
+        const double strength = ::atan(__builtin_fabs(projected)) / (kPi / 2.0) * k.slope_ratio / 100.0;
+        if (projected >= 0.0) {
+            red = static_cast<float>(static_cast<double>(red) * (1.0 - strength));
+            green = static_cast<float>(static_cast<double>(green) * (1.0 - strength));
+            blue = static_cast<float>(static_cast<double>(blue) * (1.0 - strength));
+        } else {
+            red = static_cast<float>(static_cast<double>(red) * (1.0 - strength) + strength);
+            green = static_cast<float>(static_cast<double>(green) * (1.0 - strength) + strength);
+            blue = static_cast<float>(static_cast<double>(blue) * (1.0 - strength) + strength);
+        }
+    }
+    dst[0] = dithered_srgb_byte(red, x, y, 0);
+    dst[1] = dithered_srgb_byte(green, x, y, 1);
+    dst[2] = dithered_srgb_byte(blue, x, y, 2);
+}
+
+// The value whose range kfp_transfer_bounds takes (renderer.cpp:7296-7334):
+// false when the pixel is excluded.  The value is an integer in
+// [floor(smooth_offset), max_iter - 2].
+FAV_HD bool transfer_bound_value(const KfpParams& k, size_t index, long long& value) {
+    const float raw = k.field[index];
+    if (inside_value(raw, k.max_iter, k.field_bias)) return false;
+    const double safe = __builtin_isfinite(raw)
+        ? clampd(static_cast<double>(raw) + k.field_bias, 0.0, static_cast<double>(k.max_iter)) : 0.0;
+    const double v = ::floor(safe + k.smooth_offset);
+    if (v >= static_cast<double>(k.max_iter - 1)) return false;
+    value = static_cast<long long>(v);
+    return true;
+}
+
+}  // namespace kfp
+}  // namespace fav

@@ -236,7 +236,47 @@ int render_fractal_ex_planes(
     double julia_real, double julia_imag,
     const FractalRenderOptions *options, FractalRenderPlanes *planes);
 
+/* ---- KFP colour transfer (HIP) ------------------------------------------ */
+
+/* renderer.h:452-465 / renderer.cpp:16547-16565 (fractal_colourise_kfp_impl,
+ * renderer.cpp:16314-16545).  Kalles SetColor of the scalar field with a
+ * lut_size-entry RGB LUT; phase/vocal/instrumental/pitch are validated and
+ * otherwise unused, as in the reference.  Output is W*H*3 bytes. */
+int fractal_colourise_kfp(
+    const float *field, uint8_t *output, int width, int height, int max_iter,
+    double phase, double vocal, double instrumental, double pitch,
+    const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
+    int threads);
+/* renderer.h:468-481 / renderer.cpp:16567-16600: the reference's optional
+ * GPU colour pass; here the same HIP pass. */
+int fractal_colourise_kfp_opencl(
+    const float *field, uint8_t *output, int width, int height, int max_iter,
+    double phase, double vocal, double instrumental, double pitch,
+    const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
+    int threads);
+/* renderer.h:483-498 / renderer.cpp:16729-16748.  planes must be NULL: the
+ * orbit/DE/phase/texture plane transfer is not part of the HIP pass and a
+ * non-NULL planes pointer fails with an error. */
+int fractal_colourise_kfp_planes(
+    const float *field, uint8_t *output, int width, int height, int max_iter,
+    double phase, double vocal, double instrumental, double pitch,
+    const FractalKfpOptions *options, const FractalKfpPlanes *planes,
+    const uint8_t *lut, int lut_size, int threads);
+/* renderer.h:502-515 / renderer.cpp:18252-18271. */
+int fractal_colourise_kfp_precise(
+    const float *field, uint8_t *output, int width, int height, int max_iter,
+    double phase, double vocal, double instrumental, double pitch,
+    const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
+    int threads);
+
 /* ---- Extensions of the HIP backend (no counterpart in renderer.h) ------ */
+
+/* fractal_colourise_kfp on device pointers: device_field (W*H floats) and
+ * device_rgb (W*H*3 bytes) already live in HBM; lut is a host pointer. */
+int fractal_hip_colourise_kfp_device(
+    const float *device_field, uint8_t *device_rgb, int width, int height,
+    int max_iter, const FractalKfpOptions *options, const uint8_t *lut,
+    int lut_size);
 
 /* Same contract as fractal_render_reference_ex_planes, but `device_output`
  * (width*height floats) and the plane arrays of `device_planes` (may be

@@ -1,12 +1,401 @@
 /*
- * fav_oracle_kfp.c -- placeholder translation unit for the KFP colour and
- * direct-render restatements.  TEST INFRASTRUCTURE ONLY.
+ * fav_oracle_kfp.c -- plain-C restatement of the reference's KFP colour
+ * transfer (Kalles SetColor) for a scalar iteration field, and of the
+ * 1024-entry cyclic palette LUT.  TEST INFRASTRUCTURE ONLY (fav_oracle.h).
+ *
+ * Scope: the scalar-field path of fractal_colourise_kfp_impl
+ * (renderer.cpp:16314-16545) with planes == NULL, i.e. write_kfp_pixel
+ * (renderer.cpp:6696-7289) without orbit planes, phase or texture.
  */
+#define _POSIX_C_SOURCE 200809L
 #include "fav_oracle.h"
 
-int fo_render_direct(float *output, int width, int height, double zoom, double x_center,
-                     double y_center, int max_iter, double output_bias, int escape_radius_mode,
-                     int coordinate_mode, FractalRenderPlanes *planes, int threads) {
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static const double PI_D = 3.14159265358979323846;
+
+static void kfp_error(char *error, int capacity, const char *message) {
+    if (error && capacity > 0) snprintf(error, (size_t)capacity, "%s", message);
+}
+
+static inline double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+static inline double maxd(double a, double b) { return a < b ? b : a; }  /* std::max */
+static inline double mind(double a, double b) { return b < a ? b : a; }  /* std::min */
+
+/* kfp_colour_dither_mask, renderer.cpp:5246-5260 */
+static inline float dither_mask(int x, int y, int channel) {
+    const unsigned coordinate = ((unsigned)x + (unsigned)channel * 67U + (unsigned)y * 236U) * 119U;
+    return (float)(coordinate & 255U) / 256.0F;
+}
+
+/* kfp_dithered_srgb_byte, renderer.cpp:5318-5336 */
+static inline unsigned char dithered_srgb_byte(float value, int x, int y, int channel) {
+    if (!isfinite(value)) value = 0.0F;
+    const float mask = dither_mask(x, y, channel);
+    const float quantized = 255.0F * value + mask;
+    return (unsigned char)clampi((int)floorf(quantized), 0, 255);
+}
+
+/* kfp_inside_value, renderer.cpp:5379-5387 */
+static inline int inside_value(float value, int max_iter, double bias) {
+    return isfinite(value) && (double)value >= (double)max_iter - bias;
+}
+
+/* kfp_smooth_offset, renderer.cpp:5400-5428 */
+static double smooth_offset_for(const FractalKfpOptions *o) {
+    if (o->smooth_method != 0 || !isfinite(o->power) || o->power <= 0.0 || fabs(o->power - 1.0) < 1.0e-12)
+        return 0.0;
+    double radius = 10000.0;
+    switch (o->bailout_radius_preset) {
+        case 0: radius = 10000.0; break;
+        case 1: radius = 2.0; break;
+        case 2: radius = pow(2.0, 1.0 / (o->power - 1.0)); break;
+        case 3: radius = o->bailout_radius_custom; break;
+        default: return 0.0;
+    }
+    if (!isfinite(radius) || radius <= 1.0) return 0.0;
+    return log(log(radius)) / log(o->power);
+}
+
+/* kfp_colour_sample, renderer.cpp:5434-5460 */
+static double colour_sample(const float *field, int w, int h, int x, int y, int max_iter, double offset,
+                            double bias) {
+    x = clampi(x, 0, w - 1);
+    y = clampi(y, 0, h - 1);
+    const float value = field[(size_t)y * (size_t)w + (size_t)x];
+    if (!isfinite(value)) return maxd(0.0, offset);
+    if ((double)value >= (double)max_iter - bias) return (double)max_iter + 1.0;
+    return maxd(0.0, clampd((double)value + bias, 0.0, (double)max_iter) + offset);
+}
+
+/* kfp_difference_magnitude, renderer.cpp:5497-5589 */
+static double difference_magnitude(int d, double c, double l, double r, double u, double dn, double tl,
+                                   double tr, double bl, double br) {
+    const double inverse_sqrt_two = 0.7071067811865475244008443621048490;
+    if (d == 0) {
+        const double axis_scale = 1.414;
+        return fabs(l - c) * axis_scale + fabs(u - c) * axis_scale + fabs(tl - c) + fabs(bl - c);
+    }
+    if (d == 1) {
+        const double sq = (l - c) * (l - c) + (r - c) * (r - c) + (u - c) * (u - c) + (dn - c) * (dn - c)
+            + ((tl - c) * (tl - c) + (br - c) * (br - c)) * inverse_sqrt_two * inverse_sqrt_two
+            + ((bl - c) * (bl - c) + (tr - c) * (tr - c)) * inverse_sqrt_two * inverse_sqrt_two;
+        return sqrt(maxd(0.0, sq * 0.25)) * 2.8284271247461903;
+    }
+    if (d == 2) {
+        const double sq = (r - l) * (r - l) * 0.25 + (dn - u) * (dn - u) * 0.25 + (br - tl) * (br - tl) * 0.125
+            + (tr - bl) * (tr - bl) * 0.125;
+        return sqrt(maxd(0.0, sq * 0.5)) * 2.8284271247461903;
+    }
+    if (d == 3) {
+        const double sq = (tl - c) * (tl - c) / 2.0 + (l - u) * (l - u) / 2.0;
+        return sqrt(maxd(0.0, sq)) * 2.8284271247461903;
+    }
+    if (d == 4) {
+        const double dx = ((u - tl) + (c - l)) * 0.5;
+        const double dy = ((l - tl) + (c - u)) * 0.5;
+        return sqrt(dx * dx + dy * dy) * 2.8284271247461903;
+    }
+    if (d == 5) {
+        const double dx = (r + tr + br - l - tl - bl) / 6.0;
+        const double dy = (dn + bl + br - u - tl - tr) / 6.0;
+        return sqrt(dx * dx + dy * dy) * 2.8284271247461903;
+    }
+    if (d == 6) {
+        const double lap = tl + 4.0 * u + tr + 4.0 * l - 20.0 * c + 4.0 * r + bl + 4.0 * dn + br;
+        return sqrt(fabs(lap / 6.0 * 1.4426950408889634)) * 2.8284271247461903;
+    }
+    return 0.0;
+}
+
+/* kfp_hsv_to_rgb, renderer.cpp:5591-5644 */
+static void hsv_to_rgb(float hue, float saturation, float value, float *r, float *g, float *b) {
+    const float scaled_hue = hue * 6.0F;
+    const int sector = (int)floorf(scaled_hue);
+    float fraction = scaled_hue - (float)sector;
+    if ((sector & 1) == 0) fraction = 1.0F - fraction;
+    const float minimum = value * (1.0F - saturation);
+    const float transition = value * (1.0F - saturation * fraction);
+    *r = 0.0F; *g = 0.0F; *b = 0.0F;
+    switch (sector) {
+        case 0: case 6: *r = minimum; *g = transition; *b = value; break;
+        case 1: *r = minimum; *g = value; *b = transition; break;
+        case 2: *r = transition; *g = value; *b = minimum; break;
+        case 3: *r = value; *g = transition; *b = minimum; break;
+        case 4: *r = value; *g = minimum; *b = transition; break;
+        case 5: *r = transition; *g = minimum; *b = value; break;
+        default: break;
+    }
+}
+
+typedef struct {
+    const float *field;
+    const double *samples;
+    int w, h, max_iter;
+    const FractalKfpOptions *o;
+    const unsigned char *lut;
+    int lut_size;
+    double smooth_offset, tmin, tmax, cosine, sine;
+    int needs_difference, needs_slopes;
+} kfp_ctx;
+
+static void palette_rgb(const kfp_ctx *k, double input, float *r, float *g, float *b) {
+    /* palette_rgb lambda, renderer.cpp:7070-7110 */
+    double position = fmod(input, (double)k->lut_size);
+    if (position < 0.0) position += (double)k->lut_size;
+    const double lower_position = floor(position);
+    const int lower = clampi((int)lower_position, 0, k->lut_size - 1);
+    const unsigned char *lut = k->lut;
+    if (k->o->smooth) {
+        double fraction = position - lower_position;
+        if (k->o->inverse_transition) fraction = 1.0 - fraction;
+        const int upper = (lower + 1) % k->lut_size;
+        const double inv = 1.0 - fraction;
+        *r = (float)(((double)lut[lower * 3] * inv + (double)lut[upper * 3] * fraction) / 255.0F);
+        *g = (float)(((double)lut[lower * 3 + 1] * inv + (double)lut[upper * 3 + 1] * fraction) / 255.0F);
+        *b = (float)(((double)lut[lower * 3 + 2] * inv + (double)lut[upper * 3 + 2] * fraction) / 255.0F);
+    } else {
+        *r = (float)lut[lower * 3] / 255.0F;
+        *g = (float)lut[lower * 3 + 1] / 255.0F;
+        *b = (float)lut[lower * 3 + 2] / 255.0F;
+    }
+}
+
+static double reflected(const kfp_ctx *k, int x, int y, int ox, int oy, double centre) {
+    const int sx = x + ox, sy = y + oy;
+    if (sx >= 0 && sx < k->w && sy >= 0 && sy < k->h) return k->samples[(size_t)sy * (size_t)k->w + (size_t)sx];
+    const int px = x - ox, py = y - oy;
+    if (px >= 0 && px < k->w && py >= 0 && py < k->h)
+        return 2.0 * centre - k->samples[(size_t)py * (size_t)k->w + (size_t)px];
+    return centre;
+}
+
+/* write_kfp_pixel without planes, renderer.cpp:6696-7289 */
+static void kfp_pixel(const kfp_ctx *k, int x, int y, unsigned char *dst) {
+    const FractalKfpOptions *o = k->o;
+    const size_t idx = (size_t)y * (size_t)k->w + (size_t)x;
+    const float raw = k->field[idx];
+    int inside;
+    double smooth_iter;
+    if (k->samples) {
+        smooth_iter = k->samples[idx];
+        inside = inside_value(raw, k->max_iter, o->field_bias);
+    } else {
+        inside = inside_value(raw, k->max_iter, o->field_bias);
+        const double safe = isfinite(raw) ? clampd((double)raw + o->field_bias, 0.0, (double)k->max_iter) : 0.0;
+        smooth_iter = maxd(0.0, safe + k->smooth_offset);
+    }
+    if (inside) {
+        dst[0] = (unsigned char)o->interior_color[0];
+        dst[1] = (unsigned char)o->interior_color[1];
+        dst[2] = (unsigned char)o->interior_color[2];
+        return;
+    }
+    const double colour_iter = o->flat ? floor(smooth_iter) : smooth_iter;
+    const double c = smooth_iter;
+    double l = c, r = c, u = c, dn = c, tl = c, tr = c, bl = c, br = c;
+    if (k->needs_difference || k->needs_slopes) {
+        l = reflected(k, x, y, -1, 0, c);
+        u = reflected(k, x, y, 0, -1, c);
+        if (k->needs_difference) {
+            switch (o->differences) {
+                case 0: tl = reflected(k, x, y, -1, -1, c); bl = reflected(k, x, y, -1, 1, c); break;
+                case 1: case 2: case 5: case 6:
+                    r = reflected(k, x, y, 1, 0, c); dn = reflected(k, x, y, 0, 1, c);
+                    tl = reflected(k, x, y, -1, -1, c); tr = reflected(k, x, y, 1, -1, c);
+                    bl = reflected(k, x, y, -1, 1, c); br = reflected(k, x, y, 1, 1, c);
+                    break;
+                case 3: case 4: tl = reflected(k, x, y, -1, -1, c); break;
+                default: r = reflected(k, x, y, 1, 0, c); dn = reflected(k, x, y, 0, 1, c); break;
+            }
+        }
+        if (k->needs_slopes) {
+            if (x == 0) r = reflected(k, x, y, 1, 0, c);
+            if (y == 0) dn = reflected(k, x, y, 0, 1, c);
+        }
+    }
+    double gradient = 0.0;
+    if (k->needs_difference) gradient = difference_magnitude(o->differences, c, l, r, u, dn, tl, tr, bl, br);
+    double distance = isnan(gradient) ? 0.0 : gradient * (double)k->w / 640.0;
+    if (distance < 0.0) distance = 0.0;
+    double transfer = colour_iter;
+    switch (o->color_method) {
+        case 1: transfer = sqrt(maxd(0.0, colour_iter)); break;
+        case 2: transfer = pow(maxd(0.0, colour_iter), 1.0 / 3.0); break;
+        case 3: transfer = log(maxd(1.0, colour_iter)); break;
+        case 4: transfer = 1024.0 * (colour_iter - k->tmin) / maxd(k->tmax - k->tmin, 1.0e-12); break;
+        case 5: transfer = mind(distance, 1024.0); break;
+        case 6: {
+            const double dt = mind(sqrt(maxd(0.0, distance)), 1024.0);
+            transfer = dt > o->iter_div ? smooth_iter : dt;
+            break;
+        }
+        case 7: transfer = log(maxd(1.0, distance + 1.0)); break;
+        case 8: transfer = sqrt(maxd(0.0, distance)); break;
+        case 9: transfer = log(1.0 + log(1.0 + maxd(0.0, colour_iter))); break;
+        case 10: transfer = atan(colour_iter); break;
+        case 11: transfer = sqrt(sqrt(maxd(0.0, colour_iter))); break;
+        default: break;
+    }
+    if (isnan(transfer) || transfer < 0.0) transfer = 0.0;
+    if (o->color_method == 5 || o->color_method == 7 || o->color_method == 8)
+        transfer = clampd(transfer, 0.0, 1024.0);
+    double palette_value = transfer;
+    if (o->iter_div != 1.0) palette_value /= o->iter_div;
+    float red, green, blue;
+    const double base = palette_value + o->color_offset;
+    if (o->multi_color) {
+        const double wave_input = o->smooth ? base : floor(base);
+        double hs = 0.0, ss = 0.0, vs = 0.0;
+        int hc = 0, sc = 0, vc = 0;
+        for (unsigned i = 0; i < o->multi_color_count; ++i) {
+            const double period = o->multi_color_period[i];
+            const double wave = period < 0.0 ? -period / 100.0 : 0.5 + 0.5 * sin(PI_D * wave_input / period);
+            switch (o->multi_color_type[i]) {
+                case 0: hs += wave; ++hc; break;
+                case 1: ss += wave; ++sc; break;
+                default: vs += wave; ++vc; break;
+            }
+        }
+        float mr, mg, mb;
+        hsv_to_rgb((float)(hc > 0 ? hs / hc : 0.0), (float)(sc > 0 ? ss / sc : 0.0),
+                   (float)(vc > 0 ? vs / vc : 0.0), &mr, &mg, &mb);
+        if (o->blend_multi_color) {
+            palette_rgb(k, base, &red, &green, &blue);
+            red = (red + mr) * 0.5F;
+            green = (green + mg) * 0.5F;
+            blue = (blue + mb) * 0.5F;
+        } else {
+            red = mr; green = mg; blue = mb;
+        }
+    } else {
+        palette_rgb(k, base, &red, &green, &blue);
+    }
+    if (k->needs_slopes) {
+        const double horizontal = x > 0 ? l - c : c - r;
+        const double vertical = y == 0 ? c - dn : u - c;
+        const double projected = ((horizontal * k->cosine) + (vertical * k->sine)) * o->slope_power
+            * (double)k->w / 640.0;
+        const double strength = atan(fabs(projected)) / (PI_D / 2.0) * o->slope_ratio / 100.0;
+        if (projected >= 0.0) {
+            red = (float)((double)red * (1.0 - strength));
+            green = (float)((double)green * (1.0 - strength));
+            blue = (float)((double)blue * (1.0 - strength));
+        } else {
+            red = (float)((double)red * (1.0 - strength) + strength);
+            green = (float)((double)green * (1.0 - strength) + strength);
+            blue = (float)((double)blue * (1.0 - strength) + strength);
+        }
+    }
+    dst[0] = dithered_srgb_byte(red, x, y, 0);
+    dst[1] = dithered_srgb_byte(green, x, y, 1);
+    dst[2] = dithered_srgb_byte(blue, x, y, 2);
+}
+
+int fo_colourise_kfp(const float *field, uint8_t *output, int width, int height, int max_iter,
+                     const FractalKfpOptions *o, const FractalKfpPlanes *planes, const uint8_t *lut,
+                     int lut_size, int threads, char *error, int error_capacity) {
+    if (!field || !output || !o || !lut || width <= 0 || height <= 0 || lut_size < 2) {
+        kfp_error(error, error_capacity, "invalid KFP colour arguments");
+        return 1;
+    }
+    if (planes) {
+        kfp_error(error, error_capacity, "the oracle restates the scalar-field KFP path only");
+        return 1;
+    }
+    if (o->field_bias > (double)max_iter) {
+        kfp_error(error, error_capacity, "native KFP field bias exceeds iteration cap");
+        return 1;
+    }
+    kfp_ctx k;
+    memset(&k, 0, sizeof(k));
+    k.field = field;
+    k.w = width;
+    k.h = height;
+    k.max_iter = max_iter;
+    k.o = o;
+    k.lut = lut;
+    k.lut_size = lut_size;
+    k.smooth_offset = smooth_offset_for(o);
+    k.cosine = 1.0;
+    k.sine = 0.0;
+    if (o->slopes) {  /* kfp_slope_direction, renderer.cpp:7509-7514 */
+        const double angle = o->slope_angle * 3.14159265358979323846 / 180.0;
+        k.cosine = cos(angle);
+        k.sine = sin(angle);
+    }
+    k.needs_difference = o->color_method >= 5 && o->color_method <= 8;
+    k.needs_slopes = o->slopes && o->slope_power > 0.0 && o->slope_ratio > 0.0;
+#ifdef _OPENMP
+    if (threads > 0) { omp_set_dynamic(0); omp_set_num_threads(threads); }
+#else
+    (void)threads;
+#endif
+    const long long n = (long long)width * height;
+    double *samples = NULL;
+    if (k.needs_difference || k.needs_slopes) {
+        samples = (double *)malloc(sizeof(double) * (size_t)n);
+        if (!samples) { kfp_error(error, error_capacity, "out of memory"); return 1; }
+#pragma omp parallel for schedule(static)
+        for (long long p = 0; p < n; ++p)
+            samples[p] = colour_sample(field, width, height, (int)(p % width), (int)(p / width), max_iter,
+                                       k.smooth_offset, o->field_bias);
+        k.samples = samples;
+    }
+    k.tmin = 0.0;
+    k.tmax = 1.0;
+    if (o->color_method == 4) {  /* kfp_transfer_bounds, renderer.cpp:7296-7334 */
+        double mn = INFINITY, mx = -INFINITY;
+        for (long long p = 0; p < n; ++p) {
+            const float raw = field[p];
+            if (inside_value(raw, max_iter, o->field_bias)) continue;
+            const double safe = isfinite(raw) ? clampd((double)raw + o->field_bias, 0.0, (double)max_iter) : 0.0;
+            const double value = floor(safe + k.smooth_offset);
+            if (value >= (double)(max_iter - 1)) continue;
+            mn = mind(mn, value);
+            mx = maxd(mx, value);
+        }
+        if (isfinite(mn) && isfinite(mx)) { k.tmin = mn; k.tmax = mx; }
+    }
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) kfp_pixel(&k, x, y, output + ((size_t)y * (size_t)width + (size_t)x) * 3U);
+    free(samples);
+    return 0;
+}
+
+/* _kfp_palette_lut, visualizer.py:13492-13521: Kalles' cyclic sine
+ * interpolation of the key colours, truncated to bytes. */
+int fo_kfp_palette_lut(const uint8_t *stops, int stop_count, uint8_t *lut, int lut_size) {
+    if (!stops || !lut || stop_count < 2 || lut_size < 2) return 1;
+    for (int i = 0; i < lut_size; ++i) {
+        const double position = (double)i * (double)stop_count / (double)lut_size;
+        const int left = (int)floor(position);
+        const int right = (left + 1) % stop_count;
+        double fraction = position - (double)left;
+        fraction = sin((fraction - 0.5) * PI_D) / 2.0 + 0.5;
+        for (int ch = 0; ch < 3; ++ch) {
+            double v = (double)stops[left * 3 + ch] * (1.0 - fraction) + (double)stops[right * 3 + ch] * fraction;
+            v = clampd(v, 0.0, 255.0);
+            lut[i * 3 + ch] = (uint8_t)v;
+        }
+    }
+    return 0;
+}
+
+/* ---- shallow direct render (renderer.cpp:10054-10136) --------------------
+ * Not restated yet: the HIP library renders zooms >= 1e6 only. */
+int fo_render_direct(float *output, int width, int height, double zoom, double x_center, double y_center,
+                     int max_iter, double output_bias, int escape_radius_mode, int coordinate_mode,
+                     FractalRenderPlanes *planes, int threads) {
     (void)output; (void)width; (void)height; (void)zoom; (void)x_center; (void)y_center;
     (void)max_iter; (void)output_bias; (void)escape_radius_mode; (void)coordinate_mode;
     (void)planes; (void)threads;

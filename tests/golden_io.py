@@ -97,3 +97,23 @@ def assert_hip_parity(golden, field, planes, stats):
         np.testing.assert_allclose(getattr(planes, name), golden[name], rtol=PLANE_RTOL, atol=0.0,
                                    err_msg=name)
     np.testing.assert_array_equal(stats_without_time(stats), stats_without_time(golden["stats"]))
+
+
+def load_kfp(name):
+    """A KFP colour golden case: field, lut, FractalKfpOptions, max_iter, rgb."""
+    with np.load(GOLDEN / f"kfp_{name}.npz", allow_pickle=False) as data:
+        out = {k: data[k] for k in data.files}
+    out["options"] = native.KfpOptions.from_dict(json.loads(str(out["options"])))
+    out["max_iter"] = int(out["max_iter"])
+    return out
+
+
+def kfp_case_names():
+    from kfp_cases import CASES as KFP_CASES
+    return [c[0] for c in KFP_CASES]
+
+
+def assert_rgb_within_one_level(rgb, expected, name=""):
+    """Colour bar: every channel within one 8-bit level of the reference."""
+    diff = np.abs(rgb.astype(np.int16) - expected.astype(np.int16))
+    assert int(diff.max()) <= 1, f"{name}: max RGB difference {int(diff.max())} at {np.argwhere(diff > 1)[:5]}"

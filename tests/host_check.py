@@ -36,3 +36,26 @@ def render(lib, meta):
     if status != 0:
         raise RuntimeError(err.value.decode())
     return out.reshape(h, w), planes, stats
+
+
+def bind_kfp(lib):
+    lib.fav_host_check_kfp.argtypes = [
+        native.c_float_p, native.c_uint8_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.POINTER(native.KfpOptions), native.c_uint8_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.fav_host_check_kfp.restype = ctypes.c_int
+    return lib
+
+
+def colourise(lib, field, max_iter, options, lut):
+    """The colour kernel's per-pixel code run on the CPU."""
+    h, w = field.shape
+    out = np.zeros((h, w, 3), dtype=np.uint8)
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    l = np.ascontiguousarray(lut, dtype=np.uint8)
+    err = ctypes.create_string_buffer(256)
+    status = lib.fav_host_check_kfp(f.ctypes.data_as(native.c_float_p), out.ctypes.data_as(native.c_uint8_p),
+                                    w, h, max_iter, ctypes.byref(options), l.ctypes.data_as(native.c_uint8_p),
+                                    l.shape[0], err, 256)
+    if status != 0:
+        raise RuntimeError(err.value.decode())
+    return out

@@ -186,7 +186,7 @@ def load_product():
     """The HIP library; raises if it has not been built."""
     if not PRODUCT_SO.exists():
         raise FileNotFoundError(f"{PRODUCT_SO} is not built; run __graft_entry__.build()")
-    return _bind_render_abi(ctypes.CDLL(str(PRODUCT_SO)))
+    return bind_kfp(_bind_render_abi(ctypes.CDLL(str(PRODUCT_SO))))
 
 
 def _b(text) -> bytes:
@@ -293,3 +293,143 @@ def escape_counts(field: np.ndarray, planes: PlaneArrays, max_iter: int) -> np.n
     counts[interior] = max_iter
     counts[np.isnan(field.reshape(-1))] = max_iter
     return counts.reshape(field.shape)
+
+
+class KfpOptions(ctypes.Structure):
+    """FractalKfpOptions, renderer.h:75-119 (version 3)."""
+
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("version", ctypes.c_uint32),
+        ("iter_div", ctypes.c_double), ("color_offset", ctypes.c_double), ("ratio", ctypes.c_double),
+        ("color_method", ctypes.c_int32), ("smooth_method", ctypes.c_int32), ("smooth", ctypes.c_int32),
+        ("flat", ctypes.c_int32), ("inverse_transition", ctypes.c_int32),
+        ("phase_color_strength", ctypes.c_double), ("multi_color", ctypes.c_int32),
+        ("blend_multi_color", ctypes.c_int32), ("multi_color_count", ctypes.c_uint32),
+        ("multi_color_period", ctypes.c_double * 256), ("multi_color_start", ctypes.c_int32 * 256),
+        ("multi_color_type", ctypes.c_int32 * 256), ("power", ctypes.c_double), ("slopes", ctypes.c_int32),
+        ("slope_power", ctypes.c_double), ("slope_ratio", ctypes.c_double), ("slope_angle", ctypes.c_double),
+        ("differences", ctypes.c_int32), ("interior_color", ctypes.c_int32 * 3), ("field_bias", ctypes.c_double),
+        ("bailout_radius_preset", ctypes.c_int32), ("bailout_radius_custom", ctypes.c_double),
+        ("bailout_norm_preset", ctypes.c_int32), ("bailout_norm_custom", ctypes.c_double),
+        ("texture_enabled", ctypes.c_int32), ("texture_merge", ctypes.c_double),
+        ("texture_power", ctypes.c_double), ("texture_ratio", ctypes.c_double),
+        ("texture_resize", ctypes.c_int32), ("use_opengl", ctypes.c_int32), ("use_srgb", ctypes.c_int32),
+        ("show_glitches", ctypes.c_int32),
+    ]
+
+    @classmethod
+    def make(cls, **kw) -> "KfpOptions":
+        """Kalles defaults (visualizer.py:1163-1215 NativeKfpOptions.from_profile
+        field meanings), overridden by keywords; multi_colors is a list of
+        (period, type) pairs."""
+        o = cls()
+        o.struct_size = ctypes.sizeof(cls)
+        o.version = 3
+        o.iter_div = 1.0
+        o.ratio = 360.0
+        o.smooth = 1
+        o.power = 2.0
+        o.slope_power = 50.0
+        o.slope_ratio = 20.0
+        o.slope_angle = 45.0
+        o.bailout_radius_custom = 2.0
+        o.bailout_norm_custom = 2.0
+        o.texture_merge = 1.0
+        o.texture_power = 200.0
+        o.texture_ratio = 100.0
+        o.show_glitches = 1
+        multi = kw.pop("multi_colors", None)
+        interior = kw.pop("interior_color", (0, 0, 0))
+        for key, value in kw.items():
+            setattr(o, key, value)
+        o.interior_color[0], o.interior_color[1], o.interior_color[2] = interior
+        if multi:
+            o.multi_color = 1
+            o.multi_color_count = len(multi)
+            for i, (period, kind) in enumerate(multi):
+                o.multi_color_period[i] = period
+                o.multi_color_type[i] = kind
+        return o
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, ctype in self._fields_:
+            value = getattr(self, name)
+            if hasattr(value, "__len__"):
+                value = list(value)
+            out[name] = value
+        return out
+
+    @classmethod
+    def from_dict(cls, data: dict) -> "KfpOptions":
+        o = cls()
+        for name, _ in cls._fields_:
+            value = data[name]
+            if isinstance(value, list):
+                arr = getattr(o, name)
+                for i, v in enumerate(value):
+                    arr[i] = v
+            else:
+                setattr(o, name, value)
+        return o
+
+
+def bind_kfp(lib):
+    """fractal_colourise_kfp[_planes] argument types (renderer.h:452-498)."""
+    lib.fractal_colourise_kfp.argtypes = [
+        c_float_p, c_uint8_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+        ctypes.c_double, ctypes.c_double, ctypes.POINTER(KfpOptions), c_uint8_p, ctypes.c_int, ctypes.c_int]
+    lib.fractal_colourise_kfp.restype = ctypes.c_int
+    lib.fractal_colourise_kfp_planes.argtypes = [
+        c_float_p, c_uint8_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+        ctypes.c_double, ctypes.c_double, ctypes.POINTER(KfpOptions), ctypes.c_void_p, c_uint8_p, ctypes.c_int,
+        ctypes.c_int]
+    lib.fractal_colourise_kfp_planes.restype = ctypes.c_int
+    for name in ("fractal_colourise_kfp_opencl", "fractal_colourise_kfp_precise"):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = lib.fractal_colourise_kfp.argtypes
+            getattr(lib, name).restype = ctypes.c_int
+    if hasattr(lib, "fractal_hip_colourise_kfp_device"):
+        lib.fractal_hip_colourise_kfp_device.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(KfpOptions),
+            c_uint8_p, ctypes.c_int]
+        lib.fractal_hip_colourise_kfp_device.restype = ctypes.c_int
+    return lib
+
+
+def colourise_abi(lib, field, max_iter, options, lut, threads=0, entry="fractal_colourise_kfp"):
+    h, w = field.shape
+    out = np.zeros((h, w, 3), dtype=np.uint8)
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    l = np.ascontiguousarray(lut, dtype=np.uint8)
+    status = getattr(lib, entry)(f.ctypes.data_as(c_float_p), out.ctypes.data_as(c_uint8_p), w, h,
+                                 max_iter, 0.0, 0.0, 0.0, 0.0, ctypes.byref(options),
+                                 l.ctypes.data_as(c_uint8_p), l.shape[0], threads)
+    if status != 0:
+        raise RuntimeError(lib.fractal_last_error().decode())
+    return out
+
+
+def colourise_oracle(lib, field, max_iter, options, lut, threads=0):
+    lib.fo_colourise_kfp.argtypes = [
+        c_float_p, c_uint8_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(KfpOptions),
+        ctypes.c_void_p, c_uint8_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    h, w = field.shape
+    out = np.zeros((h, w, 3), dtype=np.uint8)
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    l = np.ascontiguousarray(lut, dtype=np.uint8)
+    err = ctypes.create_string_buffer(256)
+    status = lib.fo_colourise_kfp(f.ctypes.data_as(c_float_p), out.ctypes.data_as(c_uint8_p), w, h, max_iter,
+                                  ctypes.byref(options), None, l.ctypes.data_as(c_uint8_p), l.shape[0], threads,
+                                  err, 256)
+    if status != 0:
+        raise RuntimeError(err.value.decode())
+    return out
+
+
+def palette_lut_oracle(lib, stops, size=1024):
+    lib.fo_kfp_palette_lut.argtypes = [c_uint8_p, ctypes.c_int, c_uint8_p, ctypes.c_int]
+    s = np.ascontiguousarray(stops, dtype=np.uint8)
+    lut = np.zeros((size, 3), dtype=np.uint8)
+    assert lib.fo_kfp_palette_lut(s.ctypes.data_as(c_uint8_p), s.shape[0], lut.ctypes.data_as(c_uint8_p), size) == 0
+    return lut

@@ -147,3 +147,43 @@ def test_render_without_device_fails_loudly(product):
     assert status == 1 and product.fractal_last_error()
     assert np.all(out == 7.0)  # nothing was computed on the host
     product.fractal_destroy_reference(ctypes.c_void_p(handle))
+
+
+def _kfp_call(product, options, field=None, lut_size=1024, max_iter=100, planes=None, phase=0.0):
+    field = np.full((4, 6), 10.0, dtype=np.float32) if field is None else field
+    lut = np.zeros((max(lut_size, 1), 3), dtype=np.uint8)
+    out = np.zeros((field.shape[0], field.shape[1], 3), dtype=np.uint8)
+    return product.fractal_colourise_kfp_planes(
+        field.ctypes.data_as(native.c_float_p), out.ctypes.data_as(native.c_uint8_p), field.shape[1],
+        field.shape[0], max_iter, phase, 0.0, 0.0, 0.0, ctypes.byref(options), planes,
+        lut.ctypes.data_as(native.c_uint8_p), lut_size, 0)
+
+
+@pytest.mark.parametrize("change", [
+    dict(iter_div=0.0), dict(color_method=12), dict(differences=8), dict(smooth_method=3), dict(power=0.0),
+    dict(slope_power=-1.0), dict(field_bias=-1.0), dict(bailout_radius_preset=4), dict(interior_color=(0, 0, 256)),
+    dict(multi_colors=[(0.0, 0)]), dict(multi_colors=[(1.0, 3)]), dict(show_glitches=2), dict(version=2)])
+def test_kfp_invalid_options_rejected_before_device(product, change):
+    """valid_kfp_options, renderer.cpp:7396-7454."""
+    opts = native.KfpOptions.make(**change)
+    assert _kfp_call(product, opts) == 1
+    assert b"invalid native KFP colour" in product.fractal_last_error()
+
+
+def test_kfp_rejects_bad_lut_and_phase(product):
+    opts = native.KfpOptions.make()
+    assert _kfp_call(product, opts, lut_size=1) == 1
+    assert _kfp_call(product, opts, phase=float("nan")) == 1
+
+
+def test_kfp_rejects_field_bias_above_cap(product):
+    opts = native.KfpOptions.make(field_bias=101.0)
+    assert _kfp_call(product, opts, max_iter=100) == 1
+    assert b"field bias exceeds" in product.fractal_last_error()
+
+
+def test_kfp_planes_pointer_rejected(product):
+    """The HIP pass colours the scalar field only: a planes pointer is an error."""
+    planes = ctypes.create_string_buffer(256)
+    assert _kfp_call(product, native.KfpOptions.make(), planes=ctypes.cast(planes, ctypes.c_void_p)) == 1
+    assert b"scalar field only" in product.fractal_last_error()

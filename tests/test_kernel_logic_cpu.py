@@ -2,6 +2,7 @@
 the test-only harness, against the golden vectors.  This checks the
 kernel's control flow and table layouts without a GPU; the device build
 is checked by test_gpu_parity.py."""
+import numpy as np
 import pytest
 
 import golden_io
@@ -20,3 +21,16 @@ def test_kernel_code_on_cpu_matches_golden(harness, name):
     golden = golden_io.load(name)
     field, planes, stats = host_check.render(harness, golden["meta"])
     golden_io.assert_hip_parity(golden, field, planes, stats)
+
+
+@pytest.fixture(scope="module")
+def kfp_harness(harness):
+    return host_check.bind_kfp(harness)
+
+
+@pytest.mark.parametrize("name", golden_io.kfp_case_names())
+def test_kfp_kernel_code_on_cpu_matches_golden(kfp_harness, name):
+    """kfp_core.h on the CPU: bit-exact with the reference's RGB (same libm)."""
+    g = golden_io.load_kfp(name)
+    rgb = host_check.colourise(kfp_harness, g["field"], g["max_iter"], g["options"], g["lut"])
+    np.testing.assert_array_equal(rgb, g["rgb"])
