@@ -3,9 +3,12 @@
 
 Workload (one step): the 1920x1080 field at zoom 1e100 with a 360-bit MPFR
 reference orbit, cap 100000, BLA on, rebase on, Kalles bailout/grid, render
-planes on, FP64 perturbation.  The reference orbit and BLA tables are built
-on the host once and copied to HBM before timing; each timed step is one
-field render into device-resident output buffers.
+planes on, FP64 perturbation, then Kalles colouring of the field (method 7
+field-difference distance, slopes at 45 degrees, seeded dither mask) with a
+1024-entry cyclic palette of 12 seeded stops.  The reference orbit and BLA
+tables are built on the host once and copied to HBM before timing; each
+timed step is one field render and one colour pass into device-resident
+buffers.
 
 value = Gpixel-iterations/s = sum over pixels of n_p (escape index, or the
 cap for interior / unresolved pixels) per second of wall time per step.
@@ -25,33 +28,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "fractal-audio-viz_amd"))
 
 import fav_hip  # noqa: E402
+import fav_palette  # noqa: E402
 import fav_workloads  # noqa: E402
 
 MI355X_FP64_VECTOR_PEAK_TFLOPS = 78.6
-
-
-class Hip:
-    def __init__(self):
-        self.lib = ctypes.CDLL("libamdhip64.so")
-
-    def malloc(self, nbytes):
-        ptr = ctypes.c_void_p()
-        status = self.lib.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes))
-        if status != 0:
-            raise RuntimeError(f"hipMalloc({nbytes}) failed with {status}")
-        return ptr
-
-    def free(self, ptr):
-        self.lib.hipFree(ptr)
-
-    def to_host(self, host_array, ptr):
-        status = self.lib.hipMemcpy(host_array.ctypes.data_as(ctypes.c_void_p), ptr,
-                                    ctypes.c_size_t(host_array.nbytes), 2)
-        if status != 0:
-            raise RuntimeError(f"hipMemcpy D2H failed with {status}")
-
-    def synchronize(self):
-        self.lib.hipDeviceSynchronize()
 
 
 def escape_count_sum(orbit_iteration, field, cap):
@@ -108,7 +88,7 @@ def main():
     lib = fav_hip.load_library()
     if not lib.fractal_backend_capabilities() & 8:
         raise SystemExit(f"no gfx950 device: {lib.fractal_last_error()!r}")
-    hip = Hip()
+    hip = fav_hip.HipRuntime()
     opts = fav_hip.NativeRenderOptions(**work["options"])
     t0 = time.perf_counter()
     ref = fav_hip.NativeReference(work["x"], work["y"], work["zoom"], work["max_iter"], work["bits"],
@@ -126,14 +106,23 @@ def main():
     for i, name in enumerate(("orbit_iteration", "phase", "de_x", "de_y", "test1", "test2")):
         setattr(planes, name, ctypes.cast(ctypes.c_void_p(base + i * n * 8), planes._fields_[2 + i][1]))
 
+    d_rgb = hip.malloc(n * 3)
+    kfp = fav_hip.NativeKfpOptions.kalles_default()
+    lut = fav_palette.kfp_palette_lut(fav_palette.seeded_stops(12, 2024), 1024)
+    lut_p = lut.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
     def step():
         status = lib.fractal_hip_render_reference_device(
             d_field, w, h, work["zoom"].encode(), ref.handle, work["max_iter"], work["series_order"],
             work["series_block"], ctypes.byref(opts), ctypes.byref(planes) if work["planes"] else None)
         if status != 0:
             raise RuntimeError(lib.fractal_last_error().decode())
-        timing = fav_hip.last_timing()
-        return timing["kernel_ms"]
+        field_ms = fav_hip.last_timing()["kernel_ms"]
+        status = lib.fractal_hip_colourise_kfp_device(d_field, d_rgb, w, h, work["max_iter"], ctypes.byref(kfp),
+                                                      lut_p, lut.shape[0])
+        if status != 0:
+            raise RuntimeError(lib.fractal_last_error().decode())
+        return field_ms, fav_hip.last_timing()["kernel_ms"]
 
     # event counters for the roofline numerator (deterministic, untimed)
     lib.fractal_set_stats_enabled(1)
@@ -151,18 +140,23 @@ def main():
     for _ in range(args.warmup):
         step()
     hip.synchronize()
-    kernel_ms = []
+    kernel_ms, colour_ms = [], []
     started = time.perf_counter()
     for _ in range(args.steps):
-        kernel_ms.append(step())
+        field_ms, kfp_ms = step()
+        kernel_ms.append(field_ms)
+        colour_ms.append(kfp_ms)
     hip.synchronize()
     elapsed = time.perf_counter() - started
     ms_per_step = elapsed / args.steps * 1e3
     value = pixel_iterations / (elapsed / args.steps) / 1e9
     mean_kernel_ms = float(np.mean(kernel_ms))
     achieved = flops / (mean_kernel_ms * 1e-3) / 1e12
+    rgb = np.empty(n * 3, dtype=np.uint8)
+    hip.to_host(rgb, d_rgb)
     hip.free(d_field)
     hip.free(d_planes_block)
+    hip.free(d_rgb)
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     baseline = None if args.no_cpu_baseline else cpu_baseline(work, threads)
@@ -182,7 +176,10 @@ def main():
         "config": {"workload": work["name"], "width": w, "height": h, "zoom": work["zoom"],
                    "max_iter": work["max_iter"], "precision_bits": work["bits"],
                    "path": "plain FP64 HIP kernel", "planes": work["planes"],
+                   "colouring": "KFP method 7, differences 3, slopes 45 deg, 12 seeded stops",
                    "pixel_iterations": pixel_iterations, "mean_kernel_ms": round(mean_kernel_ms, 3),
+                   "mean_colour_ms": round(float(np.mean(colour_ms)), 4),
+                   "rgb_checksum": int(rgb.astype(np.uint64).sum()),
                    "reference_build_s": round(build_s, 3),
                    "linear_blocks": int(stats[3]), "cubic_blocks": int(stats[4]),
                    "exact_steps": int(stats[5]), "tail_steps": int(stats[11]),

@@ -22,6 +22,7 @@ _c_float_p = ctypes.POINTER(ctypes.c_float)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int64_p = ctypes.POINTER(ctypes.c_int64)
 _c_uint64_p = ctypes.POINTER(ctypes.c_uint64)
+_c_uint8_p = ctypes.POINTER(ctypes.c_uint8)
 
 
 class NativeRenderOptions(ctypes.Structure):
@@ -75,6 +76,48 @@ class NativeRenderPlanes(ctypes.Structure):
     ]
 
 
+class NativeKfpOptions(ctypes.Structure):
+    """FractalKfpOptions version 3 (renderer.h:75-119); the keyword
+    defaults are Kalles' (visualizer.py:1163-1215 field meanings)."""
+
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("version", ctypes.c_uint32),
+        ("iter_div", ctypes.c_double), ("color_offset", ctypes.c_double), ("ratio", ctypes.c_double),
+        ("color_method", ctypes.c_int32), ("smooth_method", ctypes.c_int32), ("smooth", ctypes.c_int32),
+        ("flat", ctypes.c_int32), ("inverse_transition", ctypes.c_int32),
+        ("phase_color_strength", ctypes.c_double), ("multi_color", ctypes.c_int32),
+        ("blend_multi_color", ctypes.c_int32), ("multi_color_count", ctypes.c_uint32),
+        ("multi_color_period", ctypes.c_double * 256), ("multi_color_start", ctypes.c_int32 * 256),
+        ("multi_color_type", ctypes.c_int32 * 256), ("power", ctypes.c_double), ("slopes", ctypes.c_int32),
+        ("slope_power", ctypes.c_double), ("slope_ratio", ctypes.c_double), ("slope_angle", ctypes.c_double),
+        ("differences", ctypes.c_int32), ("interior_color", ctypes.c_int32 * 3), ("field_bias", ctypes.c_double),
+        ("bailout_radius_preset", ctypes.c_int32), ("bailout_radius_custom", ctypes.c_double),
+        ("bailout_norm_preset", ctypes.c_int32), ("bailout_norm_custom", ctypes.c_double),
+        ("texture_enabled", ctypes.c_int32), ("texture_merge", ctypes.c_double),
+        ("texture_power", ctypes.c_double), ("texture_ratio", ctypes.c_double),
+        ("texture_resize", ctypes.c_int32), ("use_opengl", ctypes.c_int32), ("use_srgb", ctypes.c_int32),
+        ("show_glitches", ctypes.c_int32),
+    ]
+
+    def __init__(self, **kw):
+        super().__init__()
+        self.struct_size = ctypes.sizeof(NativeKfpOptions)
+        self.version = 3
+        defaults = dict(iter_div=1.0, ratio=360.0, smooth=1, power=2.0, slope_power=50.0, slope_ratio=20.0,
+                        slope_angle=45.0, bailout_radius_custom=2.0, bailout_norm_custom=2.0, texture_merge=1.0,
+                        texture_power=200.0, texture_ratio=100.0, show_glitches=1)
+        defaults.update(kw)
+        for key, value in defaults.items():
+            setattr(self, key, value)
+
+    @classmethod
+    def kalles_default(cls) -> "NativeKfpOptions":
+        """KALLES_DEFAULT_KFP's transfer (visualizer.py:430-444): method 7
+        (field-difference distance), differences 3, slopes at 45 degrees."""
+        return cls(iter_div=0.01, ratio=360.0, color_method=7, smooth_method=0, smooth=1, power=2.0, slopes=1,
+                   slope_power=50.0, slope_ratio=20.0, slope_angle=45.0, differences=3)
+
+
 _library = None
 
 
@@ -111,6 +154,16 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.fractal_hip_last_timing.restype = ctypes.c_int
     lib.fractal_hip_reference_upload.argtypes = [ctypes.c_void_p]
     lib.fractal_hip_reference_upload.restype = ctypes.c_int
+    colour_args = [_c_float_p, _c_uint8_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.POINTER(NativeKfpOptions), _c_uint8_p,
+                   ctypes.c_int, ctypes.c_int]
+    for name in ("fractal_colourise_kfp", "fractal_colourise_kfp_opencl", "fractal_colourise_kfp_precise"):
+        getattr(lib, name).argtypes = colour_args
+        getattr(lib, name).restype = ctypes.c_int
+    lib.fractal_hip_colourise_kfp_device.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.POINTER(NativeKfpOptions), _c_uint8_p, ctypes.c_int]
+    lib.fractal_hip_colourise_kfp_device.restype = ctypes.c_int
     lib.fractal_set_stats_enabled.argtypes = [ctypes.c_int]
     lib.fractal_get_last_stats_ex.argtypes = [_c_uint64_p, ctypes.c_int]
     lib.fractal_get_last_stats_ex.restype = ctypes.c_int
@@ -176,6 +229,59 @@ class NativeReference:
 
     def __del__(self):
         self.close()
+
+
+class HipRuntime:
+    """Device memory through the HIP runtime that libfractal_hip.so links
+    (loading a second HIP runtime into the process, e.g. another copy
+    bundled with a framework, leaves that one without devices)."""
+
+    def __init__(self):
+        load_library()
+        self.lib = ctypes.CDLL("libamdhip64.so")
+        self.lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.lib.hipFree.argtypes = [ctypes.c_void_p]
+        self.lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+
+    def malloc(self, nbytes: int) -> ctypes.c_void_p:
+        ptr = ctypes.c_void_p()
+        status = self.lib.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes))
+        if status != 0:
+            raise RuntimeError(f"hipMalloc({nbytes}) failed with {status}")
+        return ptr
+
+    def free(self, ptr) -> None:
+        self.lib.hipFree(ptr)
+
+    def to_host(self, host_array: np.ndarray, ptr) -> None:
+        status = self.lib.hipMemcpy(host_array.ctypes.data_as(ctypes.c_void_p), ptr, host_array.nbytes, 2)
+        if status != 0:
+            raise RuntimeError(f"hipMemcpy D2H failed with {status}")
+
+    def to_device(self, ptr, host_array: np.ndarray) -> None:
+        source = np.ascontiguousarray(host_array)
+        status = self.lib.hipMemcpy(ptr, source.ctypes.data_as(ctypes.c_void_p), source.nbytes, 1)
+        if status != 0:
+            raise RuntimeError(f"hipMemcpy H2D failed with {status}")
+
+    def synchronize(self) -> None:
+        self.lib.hipDeviceSynchronize()
+
+
+def colourise_kfp(field: np.ndarray, max_iter: int, options: NativeKfpOptions, lut: np.ndarray,
+                  library=None) -> np.ndarray:
+    """(H, W, 3) uint8 Kalles colouring of a float32 field on the GPU."""
+    lib = library or load_library()
+    height, width = field.shape
+    source = np.ascontiguousarray(field, dtype=np.float32)
+    table = np.ascontiguousarray(lut, dtype=np.uint8)
+    out = np.empty((height, width, 3), dtype=np.uint8)
+    status = lib.fractal_colourise_kfp(source.ctypes.data_as(_c_float_p), out.ctypes.data_as(_c_uint8_p), width,
+                                       height, int(max_iter), 0.0, 0.0, 0.0, 0.0, ctypes.byref(options),
+                                       table.ctypes.data_as(_c_uint8_p), table.shape[0], 0)
+    if status != 0:
+        raise RuntimeError(lib.fractal_last_error().decode())
+    return out
 
 
 def last_timing() -> dict:

@@ -6,6 +6,7 @@ import pytest
 HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE))
 sys.path.insert(0, str(HERE / "golden"))
+sys.path.insert(0, str(HERE.parent / "fractal-audio-viz_amd"))
 
 
 def pytest_configure(config):

@@ -187,3 +187,21 @@ def test_kfp_planes_pointer_rejected(product):
     planes = ctypes.create_string_buffer(256)
     assert _kfp_call(product, native.KfpOptions.make(), planes=ctypes.cast(planes, ctypes.c_void_p)) == 1
     assert b"scalar field only" in product.fractal_last_error()
+
+
+def test_package_kfp_options_mirror_header_layout():
+    """fav_hip.NativeKfpOptions and the test mirror share renderer.h's layout."""
+    import fav_hip
+    assert ctypes.sizeof(fav_hip.NativeKfpOptions) == ctypes.sizeof(native.KfpOptions)
+    for (name, _), (other, _) in zip(fav_hip.NativeKfpOptions._fields_, native.KfpOptions._fields_):
+        assert name == other
+        assert getattr(fav_hip.NativeKfpOptions, name).offset == getattr(native.KfpOptions, name).offset
+    a = fav_hip.NativeKfpOptions.kalles_default()
+    b = native.KfpOptions.make(iter_div=0.01, color_method=7, slopes=1, differences=3)
+    assert bytes(a) == bytes(b)
+
+
+def test_package_loader_binds_colour_entries(product):
+    import fav_hip
+    lib = fav_hip.load_library()
+    assert lib.fractal_hip_colourise_kfp_device.restype is ctypes.c_int

@@ -49,17 +49,24 @@ def test_hip_kfp_full_hd_matches_oracle(gpu_product, oracle, method):
 
 def test_hip_kfp_device_pointers_equal_host_entry(gpu_product):
     """fractal_hip_colourise_kfp_device on HBM-resident field and RGB."""
-    import torch
+    import fav_hip
     print("[gpu] kfp device-pointer entry", flush=True)
     g = golden_io.load_kfp("kalles_default_c1")
     h, w = g["field"].shape
     host = native.colourise_abi(gpu_product, g["field"], g["max_iter"], g["options"], g["lut"])
-    field = torch.from_numpy(np.ascontiguousarray(g["field"], dtype=np.float32)).to("cuda:0")
-    rgb = torch.zeros(h * w * 3, dtype=torch.uint8, device="cuda:0")
-    torch.cuda.synchronize()
-    lut = np.ascontiguousarray(g["lut"], dtype=np.uint8)
-    status = gpu_product.fractal_hip_colourise_kfp_device(
-        ctypes.c_void_p(field.data_ptr()), ctypes.c_void_p(rgb.data_ptr()), w, h, g["max_iter"],
-        ctypes.byref(g["options"]), lut.ctypes.data_as(native.c_uint8_p), lut.shape[0])
-    assert status == 0, gpu_product.fractal_last_error()
-    np.testing.assert_array_equal(rgb.cpu().numpy().reshape(h, w, 3), host)
+    hip = fav_hip.HipRuntime()
+    d_field = hip.malloc(h * w * 4)
+    d_rgb = hip.malloc(h * w * 3)
+    try:
+        hip.to_device(d_field, np.ascontiguousarray(g["field"], dtype=np.float32))
+        lut = np.ascontiguousarray(g["lut"], dtype=np.uint8)
+        status = gpu_product.fractal_hip_colourise_kfp_device(
+            d_field, d_rgb, w, h, g["max_iter"], ctypes.byref(g["options"]),
+            lut.ctypes.data_as(native.c_uint8_p), lut.shape[0])
+        assert status == 0, gpu_product.fractal_last_error()
+        rgb = np.empty((h, w, 3), dtype=np.uint8)
+        hip.to_host(rgb, d_rgb)
+    finally:
+        hip.free(d_field)
+        hip.free(d_rgb)
+    np.testing.assert_array_equal(rgb, host)

@@ -71,3 +71,21 @@ def test_interior_pixels_take_the_interior_colour(oracle):
     rgb = native.colourise_oracle(oracle, field, 5000, opts, lut)
     assert (rgb.reshape(-1, 3)[1:] == [1, 2, 3]).all()
     assert tuple(rgb[0, 0]) != (1, 2, 3)
+
+
+@pytest.mark.parametrize("count,size,seed", [(2, 1024, 0), (8, 1024, 1), (12, 1024, 5), (3, 37, 2), (255, 1024, 9)])
+def test_product_palette_lut_equals_oracle(oracle, count, size, seed):
+    """The package's host-side LUT builder against the oracle's C table
+    (both pinned to the closed form above)."""
+    import fav_palette
+    stops = fav_palette.seeded_stops(count, seed)
+    np.testing.assert_array_equal(fav_palette.kfp_palette_lut(stops, size),
+                                  native.palette_lut_oracle(oracle, stops, size))
+
+
+def test_product_palette_lut_matches_golden_luts():
+    """The LUTs stored with the reference's golden RGB were built by the same rule."""
+    import fav_palette
+    from kfp_cases import KALLES_DEFAULT_STOPS
+    g = load_kfp("kalles_default_c1")
+    np.testing.assert_array_equal(fav_palette.kfp_palette_lut(KALLES_DEFAULT_STOPS, 1024), g["lut"])
