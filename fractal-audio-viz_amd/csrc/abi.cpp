@@ -20,6 +20,7 @@
 #include "fav_host.h"
 #include "fav_render.h"
 #include "kfp_prepare.h"
+#include "direct_prepare.h"
 
 namespace {
 
@@ -247,8 +248,25 @@ int one_shot(float* output, int width, int height, const char* zoom, const char*
             throw std::runtime_error("invalid native render dimensions, formula, or argument");
         const FractalRenderOptions o = checked_options(supplied);
         require_hip_backend(o);
-        if (!use_perturbation)
-            throw std::runtime_error("the direct (non-perturbation) renderer is not part of the HIP backend");
+        if (planes && formula != 0)  // renderer.cpp:20399-20405: planes off the scalar backend are Mandelbrot-only
+            throw std::runtime_error("render metadata requires the native scalar backend for this formula");
+        if (!use_perturbation) {  // render_direct[_with_planes], renderer.cpp:20311-20328, 20406-20425
+            fav::direct::DirectSetup d = fav::direct::make_setup(
+                width, height, zoom, x, y, max_iter, formula, julia_re, julia_im, o.output_bias,
+                o.escape_radius_mode, o.coordinate_mode, static_cast<uint32_t>(o.reserved[0]));
+            d.params.output = output;
+            if (planes) {
+                d.params.plane_iteration = planes->orbit_iteration;
+                d.params.plane_phase = planes->phase;
+                d.params.plane_de_x = planes->de_x;
+                d.params.plane_de_y = planes->de_y;
+                d.params.plane_test1 = planes->test1;
+                d.params.plane_test2 = planes->test2;
+            }
+            fav::render_direct_on_device(d.params, d.x.data(), d.y.data(), planes != nullptr, false);
+            set_error("");
+            return 0;
+        }
         char jr[64], ji[64];
         std::snprintf(jr, sizeof(jr), "%.17g", julia_re);
         std::snprintf(ji, sizeof(ji), "%.17g", julia_im);
@@ -481,6 +499,19 @@ int render_fractal_ex(float* output, int width, int height, const char* zoom, co
                       double julia_im, const FractalRenderOptions* options) {
     return one_shot(output, width, height, zoom, x, y, max_iter, bits, use_perturbation, threads, formula, julia_re,
                     julia_im, options, nullptr);
+}
+
+int render_mandelbrot_ex(float* output, int width, int height, const char* zoom, const char* x, const char* y,
+                         int max_iter, int bits, int use_perturbation, int threads,
+                         const FractalRenderOptions* options) {  // renderer.cpp:20478-20506
+    return one_shot(output, width, height, zoom, x, y, max_iter, bits, use_perturbation, threads, 0, 0.0, 0.0,
+                    options, nullptr);
+}
+
+int render_mandelbrot(float* output, int width, int height, const char* zoom, const char* x, const char* y,
+                      int max_iter, int bits, int use_perturbation, int threads) {  // renderer.cpp:20508-20533
+    return one_shot(output, width, height, zoom, x, y, max_iter, bits, use_perturbation, threads, 0, 0.0, 0.0,
+                    nullptr, nullptr);
 }
 
 int render_fractal_ex_planes(float* output, int width, int height, const char* zoom, const char* x, const char* y,

@@ -15,12 +15,14 @@
 #include "fav_render.h"
 #include "table_image.h"
 #include "kfp_core.h"
+#include "direct_core.h"
 
 namespace fav {
 
 hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, int persistent_blocks,
                                hipStream_t stream);
 hipError_t launch_kfp_kernel(const kfp::KfpParams& params, void* scratch, int compute_units, hipStream_t stream);
+hipError_t launch_direct_kernel(const direct::DirectParams& p, bool planes, hipStream_t stream);
 
 namespace {
 
@@ -59,7 +61,7 @@ struct LaunchState {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch;
+    DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords;
     int compute_units = 256;
     double wall_clock_khz = 0.0;
     LastTiming timing{};
@@ -306,6 +308,54 @@ void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers) {
     if (!device_pointers)
         check(hipMemcpyAsync(host.output, k.output, pixels * 3, hipMemcpyDeviceToHost, s.stream), "D2H rgb");
     check(hipStreamSynchronize(s.stream), "KFP colour");
+    float kernel_ms = 0.0f;
+    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    s.timing.kernel_ms = kernel_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
+}
+
+void render_direct_on_device(const direct::DirectParams& host, const double* x, const double* y, bool planes,
+                             bool device_pointers) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const size_t pixels = static_cast<size_t>(host.width) * static_cast<size_t>(host.height);
+    direct::DirectParams p = host;
+    double* coords = static_cast<double*>(s.direct_coords.get((static_cast<size_t>(host.width) + host.height) * 8));
+    check(hipMemcpyAsync(coords, x, static_cast<size_t>(host.width) * 8, hipMemcpyHostToDevice, s.stream), "H2D x");
+    check(hipMemcpyAsync(coords + host.width, y, static_cast<size_t>(host.height) * 8, hipMemcpyHostToDevice,
+                         s.stream), "H2D y");
+    p.x = coords;
+    p.y = coords + host.width;
+    if (!device_pointers) {
+        p.output = static_cast<float*>(s.output.get(pixels * sizeof(float)));
+        if (planes) {
+            char* base = static_cast<char*>(s.planes.get(pixels * 48));
+            p.plane_iteration = reinterpret_cast<int64_t*>(base);
+            p.plane_phase = reinterpret_cast<double*>(base + pixels * 8);
+            p.plane_de_x = p.plane_phase + pixels;
+            p.plane_de_y = p.plane_de_x + pixels;
+            p.plane_test1 = p.plane_de_y + pixels;
+            p.plane_test2 = p.plane_test1 + pixels;
+        }
+    }
+    const uint64_t wall0 = now_ns();
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_direct_kernel(p, planes, s.stream), "direct kernel launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    if (!device_pointers) {
+        check(hipMemcpyAsync(host.output, p.output, pixels * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H");
+        if (planes) {
+            check(hipMemcpyAsync(host.plane_iteration, p.plane_iteration, pixels * 8, hipMemcpyDeviceToHost, s.stream), "D2H");
+            check(hipMemcpyAsync(host.plane_phase, p.plane_phase, pixels * 8, hipMemcpyDeviceToHost, s.stream), "D2H");
+            check(hipMemcpyAsync(host.plane_de_x, p.plane_de_x, pixels * 8, hipMemcpyDeviceToHost, s.stream), "D2H");
+            check(hipMemcpyAsync(host.plane_de_y, p.plane_de_y, pixels * 8, hipMemcpyDeviceToHost, s.stream), "D2H");
+            check(hipMemcpyAsync(host.plane_test1, p.plane_test1, pixels * 8, hipMemcpyDeviceToHost, s.stream), "D2H");
+            check(hipMemcpyAsync(host.plane_test2, p.plane_test2, pixels * 8, hipMemcpyDeviceToHost, s.stream), "D2H");
+        }
+    }
+    check(hipStreamSynchronize(s.stream), "direct render");
     float kernel_ms = 0.0f;
     check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
     s.timing.kernel_ms = kernel_ms;

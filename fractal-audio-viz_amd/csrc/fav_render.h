@@ -34,6 +34,13 @@ void render_field(ReferenceContext& context, const FieldRequest& request);
 // field / lut / output unless device_pointers is set (then field and output
 // are device pointers; the LUT is always a host pointer).
 namespace kfp { struct KfpParams; }
+namespace direct { struct DirectParams; }
+
+// Shallow direct render on the device: x/y are the per-column/per-row
+// coordinates; `params` output/plane pointers are host pointers unless
+// device_pointers is set.
+void render_direct_on_device(const direct::DirectParams& params, const double* x, const double* y, bool planes,
+                             bool device_pointers);
 void colourise_kfp_on_device(const kfp::KfpParams& params, bool device_pointers);
 LastTiming last_timing();
 bool device_available();

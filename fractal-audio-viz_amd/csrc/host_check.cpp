@@ -184,3 +184,64 @@ extern "C" int fav_host_check_kfp(const float* field, uint8_t* output, int width
         return 1;
     }
 }
+
+#include "direct_prepare.h"
+
+// Shallow direct renderer of direct_kernel.hip run on the CPU (test-only):
+// the same set-up and per-pixel code.  Returns 1 with `err` set on a
+// rejected argument.
+extern "C" int fav_host_check_direct(float* output, int width, int height, const char* zoom, const char* x,
+                                     const char* y, int max_iter, int formula, double julia_re, double julia_im,
+                                     double output_bias, int escape_radius_mode, int coordinate_mode,
+                                     uint32_t plane_flags, FractalRenderPlanes* planes, char* err, int cap) {
+    try {
+        fav::direct::DirectSetup d =
+            fav::direct::make_setup(width, height, zoom, x, y, max_iter, formula, julia_re, julia_im, output_bias,
+                                    escape_radius_mode, coordinate_mode, plane_flags);
+        fav::direct::DirectParams p = d.params;
+        p.x = d.x.data();
+        p.y = d.y.data();
+        p.output = output;
+        if (planes) {
+            p.plane_iteration = planes->orbit_iteration;
+            p.plane_phase = planes->phase;
+            p.plane_de_x = planes->de_x;
+            p.plane_de_y = planes->de_y;
+            p.plane_test1 = planes->test1;
+            p.plane_test2 = planes->test2;
+        }
+        const bool de = planes && (plane_flags & fav::direct::kHintSkipAnalyticDe) == 0U;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int py = 0; py < height; ++py) {
+            for (int px = 0; px < width; ++px) {
+                switch (formula) {
+                    case 0:
+                        if (!planes) fav::direct::pixel<0, false, false>(p, px, py);
+                        else if (de) fav::direct::pixel<0, true, true>(p, px, py);
+                        else fav::direct::pixel<0, true, false>(p, px, py);
+                        break;
+                    case 1:
+                        if (!planes) fav::direct::pixel<1, false, false>(p, px, py);
+                        else if (de) fav::direct::pixel<1, true, true>(p, px, py);
+                        else fav::direct::pixel<1, true, false>(p, px, py);
+                        break;
+                    case 2:
+                        if (!planes) fav::direct::pixel<2, false, false>(p, px, py);
+                        else fav::direct::pixel<2, true, false>(p, px, py);
+                        break;
+                    default:
+                        if (!planes) fav::direct::pixel<3, false, false>(p, px, py);
+                        else fav::direct::pixel<3, true, false>(p, px, py);
+                        break;
+                }
+            }
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        if (err && cap > 0) {
+            std::strncpy(err, e.what(), static_cast<size_t>(cap) - 1);
+            err[cap - 1] = 0;
+        }
+        return 1;
+    }
+}

@@ -39,10 +39,14 @@ __device__ void wave_push(uint32_t* queue, unsigned* count, uint32_t value, bool
     if (want) queue[base + __popcll(mask & ((1ULL << lane_id()) - 1ULL))] = value;
 }
 
-// Next 64 queue entries for this wave (all lanes call it).
-__device__ unsigned wave_grab(unsigned* cursor) {
+// Next 64 queue entries for this wave (all lanes call it).  A plain read of
+// the cursor first lets waves of a drained queue leave without an atomic.
+__device__ unsigned wave_grab(unsigned* cursor, unsigned count) {
     unsigned base = 0;
-    if (lane_id() == 0) base = atomicAdd(cursor, 64U);
+    if (lane_id() == 0) {
+        base = __hip_atomic_load(cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (base < count) base = atomicAdd(cursor, 64U);
+    }
     return __shfl(base, 0);
 }
 
@@ -73,7 +77,7 @@ template <bool Planes, bool Stats>
 __global__ void __launch_bounds__(256) general_kernel(RenderParams p) {
     const unsigned count = p.state.counters[0];
     for (;;) {
-        const unsigned base = wave_grab(p.state.counters + 1);
+        const unsigned base = wave_grab(p.state.counters + 1, count);
         if (base >= count) break;
         const unsigned slot = base + static_cast<unsigned>(lane_id());
         bool to_tail = false;
@@ -100,7 +104,7 @@ template <bool Planes, bool Stats>
 __global__ void __launch_bounds__(256) tail_kernel(RenderParams p) {
     const unsigned count = p.state.counters[2];
     for (;;) {
-        const unsigned base = wave_grab(p.state.counters + 3);
+        const unsigned base = wave_grab(p.state.counters + 3, count);
         if (base >= count) break;
         const unsigned slot = base + static_cast<unsigned>(lane_id());
         if (slot < count) {

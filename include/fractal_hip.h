@@ -221,14 +221,32 @@ int fractal_render_reference_ex_planes(
 int render_mandelbrot_reference(
     float *output, int width, int height, const char *zoom_text, void *handle,
     int max_iter, int threads, int series_order, int series_block);
-/* renderer.h:338-353 / renderer.cpp:20279-20358.  use_perturbation=1 only. */
+/* renderer.h:338-353 / renderer.cpp:20279-20358.  use_perturbation=0 runs
+ * the direct double-precision renderer (formulas 0-3, render_direct,
+ * renderer.cpp:10138-10278); use_perturbation=1 builds a reference and runs
+ * the perturbation path (Mandelbrot only). */
 int render_fractal_ex(
     float *output, int width, int height, const char *zoom_text,
     const char *x_center, const char *y_center, int max_iter,
     int precision_bits, int use_perturbation, int threads, int formula,
     double julia_real, double julia_imag,
     const FractalRenderOptions *options);
-/* renderer.h:357-373 / renderer.cpp:20360-20474. */
+/* renderer.h:323-334 / renderer.cpp:20478-20506: render_fractal_ex with
+ * formula 0. */
+int render_mandelbrot_ex(
+    float *output, int width, int height, const char *zoom_text,
+    const char *x_center, const char *y_center, int max_iter,
+    int precision_bits, int use_perturbation, int threads,
+    const FractalRenderOptions *options);
+/* renderer.h:387-398 / renderer.cpp:20508-20533: default options. */
+int render_mandelbrot(
+    float *output, int width, int height, const char *zoom_text,
+    const char *x_center, const char *y_center, int max_iter,
+    int precision_bits, int use_perturbation, int threads);
+/* renderer.h:357-373 / renderer.cpp:20360-20474.  Planes with
+ * use_perturbation=0 follow render_direct_with_planes
+ * (renderer.cpp:10280-10426); planes are Mandelbrot-only on this backend,
+ * as on the reference's GPU backend. */
 int render_fractal_ex_planes(
     float *output, int width, int height, const char *zoom_text,
     const char *x_center, const char *y_center, int max_iter,

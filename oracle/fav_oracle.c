@@ -1689,3 +1689,167 @@ int fo_render(float *output, int width, int height, const char *zoom_text, const
     free(local); free(xo); free(yo);
     return 0;
 }
+
+/* ---- shallow direct render (renderer.cpp:10009-10426) -------------------- */
+
+/* parse_classic_long_double + range checks (renderer.cpp:214-241, 7516-7530). */
+static int parse_long_double(const char *text, long double *out) {
+    if (!text) return 0;
+    char *end = NULL;
+    const long double v = strtold(text, &end);
+    if (end == text) return 0;
+    while (*end == ' ' || *end == '\t' || *end == '\n' || *end == '\r' || *end == '\f' || *end == '\v') ++end;
+    if (*end != '\0' || !isfinite(v)) return 0;
+    *out = v;
+    return 1;
+}
+
+typedef struct {
+    const double *x, *y;
+    float *output;
+    FractalRenderPlanes *planes;
+    int width, height, max_iter, formula;
+    double julia_re, julia_im, bias, escape2, safe_escape2, log_power;
+    uint32_t flags;
+    fe_t spacing;
+} direct_ctx;
+
+/* iterate_direct_formula_with_derivative, renderer.cpp:251-398, with each
+ * product and sum rounded in the reference's order. */
+static void direct_step(int formula, int derivative, double zr, double zi, double pr, double pi, double dr,
+                        double di, double *nr, double *ni, double *ndr, double *ndi) {
+    if (formula == 2) {  /* burning ship */
+        const double ar = fabs(zr), ai = fabs(zi);
+        const double rs = ar * ar, is = ai * ai, cross = 2.0 * ar * ai;
+        *nr = (rs - is) + pr;
+        *ni = cross + pi;
+    } else if (formula == 3) {  /* tricorn */
+        const double rs = zr * zr, is = zi * zi, cross = -2.0 * zr * zi;
+        *nr = (rs - is) + pr;
+        *ni = cross + pi;
+    } else {  /* Mandelbrot and Julia share the map */
+        *nr = zr * zr - zi * zi + pr;
+        *ni = 2.0 * zr * zi + pi;
+    }
+    *ndr = 0.0;
+    *ndi = 0.0;
+    if (derivative) {  /* Mandelbrot (parameter plane, +1) and Julia (+0) */
+        const double add_real = formula == 1 ? 0.0 : 1.0;
+        *ndr = 2.0 * zr * dr - 2.0 * zi * di + add_real;
+        *ndi = 2.0 * zr * di + 2.0 * zi * dr + 0.0;
+    }
+}
+
+static void direct_inside(const direct_ctx *c, size_t index) {
+    c->output[index] = encode_iteration(c->max_iter, c->bias);
+    clear_planes_pixel(c->planes, index, c->max_iter);
+}
+
+static void direct_pixel(const direct_ctx *c, int px, int py) {
+    const double cx = c->x[px], cy = c->y[py];
+    const size_t index = (size_t)py * (size_t)c->width + (size_t)px;
+    const int julia = c->formula == 1;
+    const int derivative = c->planes && (c->flags & 2u) == 0 && (c->formula == 0 || julia);
+    if (c->formula == 0) { /* cardioid / period-2 bulb, renderer.cpp:10087-10097 */
+        const double q = (cx - 0.25) * (cx - 0.25) + cy * cy;
+        const int in_cardioid = q * (q + cx - 0.25) <= 0.25 * cy * cy;
+        const int in_bulb = (cx + 1.0) * (cx + 1.0) + cy * cy <= 0.0625;
+        if (in_cardioid || in_bulb) { direct_inside(c, index); return; }
+    }
+    double zr = julia ? cx : 0.0, zi = julia ? cy : 0.0;
+    const double pr = julia ? c->julia_re : cx, pi = julia ? c->julia_im : cy;
+    double dr = julia ? 1.0 : 0.0, di = 0.0;
+    double test1 = 0.0, test2 = 0.0;
+    int iteration = 0;
+    for (; iteration < c->max_iter; ++iteration) {
+        double nr, ni, ndr, ndi;
+        direct_step(c->formula, derivative, zr, zi, pr, pi, dr, di, &nr, &ni, &ndr, &ndi);
+        zr = nr; zi = ni; dr = ndr; di = ndi;
+        const double m2 = zr * zr + zi * zi;
+        test2 = test1;
+        test1 = m2;
+        if (m2 > c->escape2 || !isfinite(m2)) {
+            const double safe = isfinite(m2) ? (m2 < c->safe_escape2 ? c->safe_escape2 : m2) : DBL_MAX;
+            const double magnitude = sqrt(safe);
+            c->output[index] = (float)(((long double)(iteration + 1) - log(log(magnitude)) / c->log_power)
+                                       - (long double)c->bias);
+            if (c->planes) {
+                FractalRenderPlanes *p = c->planes;
+                p->orbit_iteration[index] = iteration;
+                p->phase[index] = 0.0;
+                if ((c->flags & 1u) == 0) {
+                    double value = 0.0;
+                    if (isfinite(zr) && isfinite(zi)) {
+                        value = atan2(zi, zr) / 6.283185307179586476925286766559005768;
+                        value -= floor(value);
+                    }
+                    p->phase[index] = value;
+                }
+                p->de_x[index] = 0.0;
+                p->de_y[index] = 0.0;
+                if (derivative && isfinite(zr) && isfinite(zi) && isfinite(dr) && isfinite(di)) {
+                    const sc_t total = sc_from_fe(fe_from_parts(zr, 0), fe_from_parts(zi, 0));
+                    const sc_t dz = sc_from_fe(fe_from_parts(dr, 0), fe_from_parts(di, 0));
+                    (void)render_plane_de(total, dz, c->spacing, &p->de_x[index], &p->de_y[index]);
+                }
+                p->test1[index] = test1;
+                p->test2[index] = test2;
+            }
+            return;
+        }
+    }
+    direct_inside(c, index);
+}
+
+int fo_render_direct(float *output, int width, int height, const char *zoom, const char *x_center,
+                     const char *y_center, int max_iter, int formula, double julia_real, double julia_imag,
+                     double output_bias, int escape_radius_mode, int coordinate_mode, uint32_t plane_flags,
+                     FractalRenderPlanes *planes, int threads, char *error, int cap) {
+    long double zl = 0.0L, xl = 0.0L, yl = 0.0L;
+    if (!output || width <= 0 || height <= 0 || max_iter <= 0 || formula < 0 || formula > 3
+        || !parse_long_double(zoom, &zl) || !(zl > 0.0L) || !parse_long_double(x_center, &xl)
+        || !parse_long_double(y_center, &yl)) {
+        fo_set_error(error, cap, "invalid direct render arguments");
+        return 1;
+    }
+    const long double l10 = log10l(zl);
+    if (!isfinite(l10) || l10 < -300.0L || l10 > 9800.0L) {
+        fo_set_error(error, cap, "Mandelbrot zoom is outside the supported range");
+        return 1;
+    }
+    const double zoom_value = (double)zl, centre_re = (double)xl, centre_im = (double)yl;
+    if (!isfinite(zoom_value) || zoom_value <= 0.0) {
+        fo_set_error(error, cap, "direct native coordinates or zoom exceed double range");
+        return 1;
+    }
+    const double height_span = viewport_height_factor(coordinate_mode) / zoom_value;
+    const double width_span = height_span * (double)width / (double)height;
+    if (!isfinite(height_span) || !isfinite(width_span)) {
+        fo_set_error(error, cap, "direct native viewport is outside double range");
+        return 1;
+    }
+    double *xs = malloc(sizeof(double) * (size_t)width);
+    double *ys = malloc(sizeof(double) * (size_t)height);
+    if (!xs || !ys) { free(xs); free(ys); fo_set_error(error, cap, "out of memory"); return 1; }
+    for (int px = 0; px < width; ++px)
+        xs[px] = centre_re + pixel_axis_offset(px, width, coordinate_mode) * width_span / (double)width;
+    for (int py = 0; py < height; ++py)
+        ys[py] = centre_im + -pixel_axis_offset(py, height, coordinate_mode) * height_span / (double)height;
+    direct_ctx c;
+    c.x = xs; c.y = ys; c.output = output; c.planes = planes;
+    c.width = width; c.height = height; c.max_iter = max_iter; c.formula = formula;
+    c.julia_re = julia_real; c.julia_im = julia_imag; c.bias = output_bias;
+    c.escape2 = mode_r2_double(escape_radius_mode);
+    c.safe_escape2 = c.escape2 + 1.0e-7;
+    c.log_power = log(2.0);
+    c.flags = plane_flags;
+    c.spacing = fe_from_parts(height_span / (double)height, 0);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int py = 0; py < height; ++py)
+        for (int px = 0; px < width; ++px) direct_pixel(&c, px, py);
+    free(xs); free(ys);
+    return 0;
+}

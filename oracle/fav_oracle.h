@@ -65,12 +65,16 @@ int fo_render(
     FractalRenderPlanes *planes, uint64_t *stats, int threads,
     char *error, int error_capacity);
 
-/* Shallow direct render (renderer.cpp:10054-10136, 10138-10278) for the
- * Mandelbrot formula with double coordinates. */
+/* Shallow direct render in plain doubles (render_direct / render_direct_with_planes,
+ * renderer.cpp:10009-10426): zoom and centre are decimal text parsed as
+ * long double and rounded to double; formula 0-3 (Mandelbrot, Julia,
+ * burning ship, tricorn); planes may be NULL; plane_flags are the
+ * FRACTAL_RENDER_HINT_* bits. */
 int fo_render_direct(
-    float *output, int width, int height, double zoom, double x_center,
-    double y_center, int max_iter, double output_bias, int escape_radius_mode,
-    int coordinate_mode, FractalRenderPlanes *planes, int threads);
+    float *output, int width, int height, const char *zoom, const char *x_center,
+    const char *y_center, int max_iter, int formula, double julia_real, double julia_imag,
+    double output_bias, int escape_radius_mode, int coordinate_mode, uint32_t plane_flags,
+    FractalRenderPlanes *planes, int threads, char *error, int error_capacity);
 
 /* KFP SetColor transfer of a scalar field (renderer.cpp:16314-16545). */
 int fo_colourise_kfp(

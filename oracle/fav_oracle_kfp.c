@@ -391,13 +391,3 @@ int fo_kfp_palette_lut(const uint8_t *stops, int stop_count, uint8_t *lut, int l
     return 0;
 }
 
-/* ---- shallow direct render (renderer.cpp:10054-10136) --------------------
- * Not restated yet: the HIP library renders zooms >= 1e6 only. */
-int fo_render_direct(float *output, int width, int height, double zoom, double x_center, double y_center,
-                     int max_iter, double output_bias, int escape_radius_mode, int coordinate_mode,
-                     FractalRenderPlanes *planes, int threads) {
-    (void)output; (void)width; (void)height; (void)zoom; (void)x_center; (void)y_center;
-    (void)max_iter; (void)output_bias; (void)escape_radius_mode; (void)coordinate_mode;
-    (void)planes; (void)threads;
-    return 1;
-}

@@ -117,3 +117,36 @@ def assert_rgb_within_one_level(rgb, expected, name=""):
     """Colour bar: every channel within one 8-bit level of the reference."""
     diff = np.abs(rgb.astype(np.int16) - expected.astype(np.int16))
     assert int(diff.max()) <= 1, f"{name}: max RGB difference {int(diff.max())} at {np.argwhere(diff > 1)[:5]}"
+
+
+def load_direct(name):
+    """A direct-render golden case: meta (the case dict), field, planes."""
+    with np.load(GOLDEN / f"{name}.npz", allow_pickle=False) as data:
+        out = {k: data[k] for k in data.files}
+    out["meta"] = json.loads(str(out["meta"]))
+    return out
+
+
+def direct_case_names():
+    from direct_cases import CASES as DIRECT_CASES
+    return [c["name"] for c in DIRECT_CASES]
+
+
+def assert_direct_parity(golden, field, planes, exact_field=False):
+    """Direct-render bar: iteration plane, test1/test2 and the interior set
+    bit-exact; the float field within one ulp (bit-exact when exact_field);
+    phase and DE within 1e-9 relative."""
+    f = field.reshape(-1)
+    g = golden["field"].reshape(-1)
+    if exact_field:
+        np.testing.assert_array_equal(f.view(np.uint32), g.view(np.uint32))
+    else:
+        ulps = np.abs(f.view(np.int32).astype(np.int64) - g.view(np.int32).astype(np.int64))
+        assert int(ulps.max()) <= 1, f"field differs by {int(ulps.max())} ulp"
+        assert np.array_equal(np.floor(f), np.floor(g)) or int(ulps.max()) <= 1
+    if "orbit_iteration" in golden:
+        np.testing.assert_array_equal(planes.orbit_iteration, golden["orbit_iteration"])
+        np.testing.assert_array_equal(planes.test1, golden["test1"])
+        np.testing.assert_array_equal(planes.test2, golden["test2"])
+        for name in ("phase", "de_x", "de_y"):
+            np.testing.assert_allclose(getattr(planes, name), golden[name], rtol=1e-9, atol=1e-300, err_msg=name)

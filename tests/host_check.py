@@ -59,3 +59,29 @@ def colourise(lib, field, max_iter, options, lut):
     if status != 0:
         raise RuntimeError(err.value.decode())
     return out
+
+
+def bind_direct(lib):
+    lib.fav_host_check_direct.argtypes = [
+        native.c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+        ctypes.c_uint32, ctypes.POINTER(native.RenderPlanes), ctypes.c_char_p, ctypes.c_int]
+    lib.fav_host_check_direct.restype = ctypes.c_int
+    return lib
+
+
+def render_direct(lib, case):
+    """The direct kernel's per-pixel code run on the CPU."""
+    w, h = case["width"], case["height"]
+    out = np.full(w * h, -7.0, dtype=np.float32)
+    o = case.get("options", {})
+    planes = native.PlaneArrays(w, h) if case["planes"] else None
+    err = ctypes.create_string_buffer(256)
+    status = lib.fav_host_check_direct(
+        out.ctypes.data_as(native.c_float_p), w, h, case["zoom"].encode(), case["x"].encode(), case["y"].encode(),
+        case["max_iter"], case["formula"], case.get("julia_re", 0.0), case.get("julia_im", 0.0),
+        o.get("output_bias", 0.0), o.get("escape_radius_mode", 0), o.get("coordinate_mode", 0),
+        o.get("plane_flags", 0), ctypes.byref(planes.struct) if planes is not None else None, err, 256)
+    if status != 0:
+        raise RuntimeError(err.value.decode())
+    return out.reshape(h, w), planes

@@ -154,7 +154,68 @@ def _bind_render_abi(lib: ctypes.CDLL) -> ctypes.CDLL:
         ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         ctypes.c_double, ctypes.c_double, ctypes.POINTER(RenderOptions)]
     lib.render_fractal_ex.restype = ctypes.c_int
+    if hasattr(lib, "render_fractal_ex_planes"):
+        lib.render_fractal_ex_planes.argtypes = lib.render_fractal_ex.argtypes + [ctypes.POINTER(RenderPlanes)]
+        lib.render_fractal_ex_planes.restype = ctypes.c_int
+    if hasattr(lib, "render_mandelbrot_ex"):
+        lib.render_mandelbrot_ex.argtypes = [
+            c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(RenderOptions)]
+        lib.render_mandelbrot_ex.restype = ctypes.c_int
+    if hasattr(lib, "render_mandelbrot"):
+        lib.render_mandelbrot.argtypes = [
+            c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.render_mandelbrot.restype = ctypes.c_int
     return lib
+
+
+def direct_options(case, backend):
+    """FractalRenderOptions of a direct-render case."""
+    o = case.get("options", {})
+    opts = RenderOptions.defaults(backend=backend, output_bias=o.get("output_bias", 0.0),
+                                  escape_radius_mode=o.get("escape_radius_mode", 0),
+                                  coordinate_mode=o.get("coordinate_mode", 0))
+    opts.reserved[0] = o.get("plane_flags", 0)
+    return opts
+
+
+def render_direct_abi(lib, case, backend=2):
+    """render_fractal_ex[_planes] with use_perturbation = 0."""
+    w, h = case["width"], case["height"]
+    out = np.full(w * h, -7.0, dtype=np.float32)
+    opts = direct_options(case, backend)
+    planes = PlaneArrays(w, h) if case["planes"] else None
+    args = [out.ctypes.data_as(c_float_p), w, h, case["zoom"].encode(), case["x"].encode(), case["y"].encode(),
+            case["max_iter"], 128, 0, 0, case["formula"], case.get("julia_re", 0.0), case.get("julia_im", 0.0),
+            ctypes.byref(opts)]
+    if planes is not None:
+        status = lib.render_fractal_ex_planes(*args, ctypes.byref(planes.struct))
+    else:
+        status = lib.render_fractal_ex(*args)
+    if status != 0:
+        raise RuntimeError(lib.fractal_last_error().decode())
+    return out.reshape(h, w), planes
+
+
+def render_direct_oracle(lib, case, threads=0):
+    lib.fo_render_direct.argtypes = [
+        c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+        ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+        ctypes.c_uint32, ctypes.POINTER(RenderPlanes), ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    w, h = case["width"], case["height"]
+    out = np.full(w * h, -7.0, dtype=np.float32)
+    o = case.get("options", {})
+    planes = PlaneArrays(w, h) if case["planes"] else None
+    err = ctypes.create_string_buffer(256)
+    status = lib.fo_render_direct(
+        out.ctypes.data_as(c_float_p), w, h, case["zoom"].encode(), case["x"].encode(), case["y"].encode(),
+        case["max_iter"], case["formula"], case.get("julia_re", 0.0), case.get("julia_im", 0.0),
+        o.get("output_bias", 0.0), o.get("escape_radius_mode", 0), o.get("coordinate_mode", 0),
+        o.get("plane_flags", 0), ctypes.byref(planes.struct) if planes is not None else None, threads, err, 256)
+    if status != 0:
+        raise RuntimeError(err.value.decode())
+    return out.reshape(h, w), planes
 
 
 def load_reference():

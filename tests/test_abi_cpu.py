@@ -205,3 +205,25 @@ def test_package_loader_binds_colour_entries(product):
     import fav_hip
     lib = fav_hip.load_library()
     assert lib.fractal_hip_colourise_kfp_device.restype is ctypes.c_int
+
+
+@pytest.mark.parametrize("zoom,x,err", [("0", "0", b"invalid Mandelbrot zoom"), ("1e-400", "0", b"outside the supported"),
+                                        ("1", "1,5", b"invalid real coordinate")])
+def test_direct_render_validation_before_device(product, zoom, x, err):
+    out = np.zeros(4, dtype=np.float32)
+    opts = native.RenderOptions.defaults(backend=2)
+    status = product.render_fractal_ex(out.ctypes.data_as(native.c_float_p), 2, 2, zoom.encode(), x.encode(), b"0",
+                                       100, 128, 0, 0, 0, 0.0, 0.0, ctypes.byref(opts))
+    assert status == 1
+    assert err in product.fractal_last_error()
+
+
+def test_direct_planes_need_mandelbrot(product):
+    """renderer.cpp:20399-20405: plane renders off the scalar backend are Mandelbrot-only."""
+    planes = native.PlaneArrays(2, 2)
+    out = np.zeros(4, dtype=np.float32)
+    opts = native.RenderOptions.defaults(backend=2)
+    status = product.render_fractal_ex_planes(out.ctypes.data_as(native.c_float_p), 2, 2, b"1", b"0", b"0", 100, 128,
+                                              0, 0, 2, 0.0, 0.0, ctypes.byref(opts), ctypes.byref(planes.struct))
+    assert status == 1
+    assert b"scalar backend for this formula" in product.fractal_last_error()

@@ -34,3 +34,16 @@ def test_kfp_kernel_code_on_cpu_matches_golden(kfp_harness, name):
     g = golden_io.load_kfp(name)
     rgb = host_check.colourise(kfp_harness, g["field"], g["max_iter"], g["options"], g["lut"])
     np.testing.assert_array_equal(rgb, g["rgb"])
+
+
+@pytest.fixture(scope="module")
+def direct_harness(harness):
+    return host_check.bind_direct(harness)
+
+
+@pytest.mark.parametrize("name", golden_io.direct_case_names())
+def test_direct_kernel_code_on_cpu_matches_golden(direct_harness, name):
+    """direct_core.h on the CPU: field, counts and test planes bit-exact."""
+    g = golden_io.load_direct(name)
+    field, planes = host_check.render_direct(direct_harness, g["meta"])
+    golden_io.assert_direct_parity(g, field, planes, exact_field=True)
