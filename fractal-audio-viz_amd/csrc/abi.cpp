@@ -248,7 +248,11 @@ int one_shot(float* output, int width, int height, const char* zoom, const char*
             throw std::runtime_error("invalid native render dimensions, formula, or argument");
         const FractalRenderOptions o = checked_options(supplied);
         require_hip_backend(o);
-        if (planes && formula != 0)  // renderer.cpp:20399-20405: planes off the scalar backend are Mandelbrot-only
+        // renderer.cpp:20399-20405 allows planes off the scalar backend for
+        // Mandelbrot only; the HIP direct kernel also carries Julia's
+        // analytic derivative, so Julia planes are accepted when rendered
+        // directly.  Burning ship / tricorn planes are rejected as there.
+        if (planes && formula != 0 && !(formula == 1 && !use_perturbation))
             throw std::runtime_error("render metadata requires the native scalar backend for this formula");
         if (!use_perturbation) {  // render_direct[_with_planes], renderer.cpp:20311-20328, 20406-20425
             fav::direct::DirectSetup d = fav::direct::make_setup(

@@ -245,8 +245,9 @@ int render_mandelbrot(
     int precision_bits, int use_perturbation, int threads);
 /* renderer.h:357-373 / renderer.cpp:20360-20474.  Planes with
  * use_perturbation=0 follow render_direct_with_planes
- * (renderer.cpp:10280-10426); planes are Mandelbrot-only on this backend,
- * as on the reference's GPU backend. */
+ * (renderer.cpp:10280-10426).  Planes are accepted for Mandelbrot, and for
+ * Julia when use_perturbation=0 (the reference's GPU backend takes
+ * Mandelbrot only); burning ship / tricorn planes fail as there. */
 int render_fractal_ex_planes(
     float *output, int width, int height, const char *zoom_text,
     const char *x_center, const char *y_center, int max_iter,
