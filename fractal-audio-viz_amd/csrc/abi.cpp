@@ -182,7 +182,7 @@ void* create(const char* x, const char* y, const char* zoom, int max_iter, int b
         parse_coordinate(y, "imaginary");
         auto context = fav::build_reference(x, y, zoom, max_iter, bits, series_order, reusable, formula,
                                             use_options ? o.escape_radius_mode : 0,
-                                            use_options ? o.coordinate_mode : 0);
+                                            use_options ? o.coordinate_mode : 0, julia_re, julia_im);
         void* handle = register_reference(std::move(context));
         set_error("");
         return handle;
@@ -207,6 +207,8 @@ int render_impl(float* output, int width, int height, const char* zoom, void* ha
         require_hip_backend(o);
         const auto context = acquire(handle);
         if (!context) throw std::runtime_error("invalid or already-destroyed reference handle");
+        if (planes != nullptr && context->formula != 0)  // renderer.cpp:19842-19848
+            throw std::runtime_error("render metadata requires the native scalar backend for this formula");
         if (context->escape_radius_mode != o.escape_radius_mode)
             throw std::runtime_error("reference escape-radius mode does not match render options");
         if (max_iter > context->requested_max_iter)
@@ -216,14 +218,34 @@ int render_impl(float* output, int width, int height, const char* zoom, void* ha
         const long double log10_zoom =
             (std::log(static_cast<long double>(z.m)) + static_cast<long double>(z.e) * 0.693147180559945309417232121458176568L)
             / std::log(10.0L);
-        if (log10_zoom < 6.0L)
-            throw std::runtime_error(
-                "zoom below 1e6 is the reference's direct renderer, which the HIP backend does not provide");
+        if (log10_zoom < 6.0L) {
+            // renderer.cpp:19920-19960: shallow views of a handle use the
+            // direct double-precision renderer at the handle's centre
+            fav::direct::DirectSetup d = fav::direct::make_setup_values(
+                width, height, fav::direct::parse_zoom_long(zoom), context->x_center, context->y_center, max_iter,
+                context->formula, context->julia_re, context->julia_im, o.output_bias, o.escape_radius_mode,
+                o.coordinate_mode, static_cast<uint32_t>(o.reserved[0]));
+            d.params.output = output;
+            if (planes) {
+                d.params.plane_iteration = planes->orbit_iteration;
+                d.params.plane_phase = planes->phase;
+                d.params.plane_de_x = planes->de_x;
+                d.params.plane_de_y = planes->de_y;
+                d.params.plane_test1 = planes->test1;
+                d.params.plane_test2 = planes->test2;
+            }
+            fav::render_direct_on_device(d.params, d.x.data(), d.y.data(), planes != nullptr, device_pointers);
+            set_error("");
+            return 0;
+        }
         uint64_t stats[fav::kStatsFields] = {};
         const bool collect = stats_enabled.load(std::memory_order_relaxed);
         fav::FieldRequest request{output, width, height, zoom, max_iter, series_order, series_block, o,
                                   planes, collect ? stats : nullptr, device_pointers};
-        fav::render_field(*context, request);
+        if (context->formula != 0)
+            fav::render_alternate(*context, request);
+        else
+            fav::render_field(*context, request);
         if (collect) publish_stats(stats);
         set_error("");
         return 0;

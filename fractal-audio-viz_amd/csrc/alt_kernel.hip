@@ -1,0 +1,38 @@
+// alt_kernel.hip -- alternate deep formulas (Julia, burning ship, tricorn)
+// on gfx950; the per-pixel loop is alt_core.h.
+//
+// One pixel per lane; a 256-thread block covers a 16x16 tile as four 8x8
+// wavefronts so neighbouring pixels, which share their reference index and
+// BLA level, run together.  FP64 VALU-bound like the Mandelbrot kernels.
+#include <hip/hip_runtime.h>
+
+#include "alt_core.h"
+
+namespace fav {
+namespace {
+
+__global__ void __launch_bounds__(256) alt_kernel(AltRenderParams p) {
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    if (px < p.width && py < p.height) alt::pixel(p, px, py);
+}
+
+__global__ void alt_prologue(unsigned long long* budget_start, int* budget_flag) {
+    budget_start[0] = wall_clock64();
+    budget_flag[0] = 0;
+}
+
+}  // namespace
+
+// Shapes validated by the caller: offsets hold width / height entries and
+// output width*height floats; the orbit, margins and BLA steps are the
+// reference's device tables.
+hipError_t launch_alt_kernel(const AltRenderParams& p, hipStream_t stream) {
+    alt_prologue<<<1, 1, 0, stream>>>(const_cast<unsigned long long*>(p.budget_start), p.budget_flag);
+    alt_kernel<<<dim3((p.width + 15) / 16, (p.height + 15) / 16), dim3(256), 0, stream>>>(p);
+    return hipGetLastError();
+}
+
+}  // namespace fav

@@ -23,6 +23,7 @@ hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, i
                                hipStream_t stream);
 hipError_t launch_kfp_kernel(const kfp::KfpParams& params, void* scratch, int compute_units, hipStream_t stream);
 hipError_t launch_direct_kernel(const direct::DirectParams& p, bool planes, hipStream_t stream);
+hipError_t launch_alt_kernel(const AltRenderParams& p, hipStream_t stream);
 
 namespace {
 
@@ -106,6 +107,7 @@ DeviceTables::~DeviceTables() {
     for (void* p : linear_block) if (p) (void)hipFree(p);
     if (cubic_block) (void)hipFree(cubic_block);
     if (series_block) (void)hipFree(series_block);
+    if (alt_block) (void)hipFree(alt_block);
 }
 
 namespace {
@@ -144,6 +146,7 @@ void ensure_device_tables(ReferenceContext& c) {
     const size_t off_rd = append(blob, o.real_double), off_id = append(blob, o.imag_double);
     const size_t off_znm = append(blob, o.zn_m), off_zne = append(blob, o.zn_e);
     const size_t off_small = append(blob, o.small_exp), off_flags = append(blob, o.flags);
+    const size_t off_am = append(blob, o.alt_margin_m), off_ae = append(blob, o.alt_margin_e);
     const unsigned char* base = nullptr;
     upload_vector(blob, t->orbit_block, base, s.stream, t->bytes);
     t->orbit.re = reinterpret_cast<const double*>(base + off_re);
@@ -165,6 +168,10 @@ void ensure_device_tables(ReferenceContext& c) {
     t->orbit.flags = reinterpret_cast<const int32_t*>(base + off_flags);
     t->orbit.size = static_cast<int32_t>(o.scaled.size());
     t->orbit.map_end = c.map_end;
+    t->alt_margin_m = reinterpret_cast<const double*>(base + off_am);
+    t->alt_margin_e = reinterpret_cast<const int32_t*>(base + off_ae);
+    upload_vector(img.alt, t->alt_block, t->alt, s.stream, t->bytes);
+    t->alt_dir = img.alt_dir;
     std::vector<unsigned char> sblob;
     const size_t s_re = append(sblob, img.series_re), s_im = append(sblob, img.series_im);
     const size_t s_e = append(sblob, img.series_e);
@@ -313,6 +320,68 @@ void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers) {
     s.timing.kernel_ms = kernel_ms;
     s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
     s.timing.pixels = static_cast<double>(pixels);
+}
+
+void render_alternate(ReferenceContext& c, const FieldRequest& r) {
+    PreparedAlternate prep;
+    prepare_alternate(c, r, prep);
+    ensure_device_tables(c);
+    const DeviceTables& t = *c.device;
+    AltRenderParams p = prep.params;
+    p.re = t.orbit.re;
+    p.im = t.orbit.im;
+    p.e = t.orbit.e;
+    p.margin_m = t.alt_margin_m;
+    p.margin_e = t.alt_margin_e;
+    p.steps = t.alt;
+    p.dir = t.alt_dir;
+    const SeriesDev scalar_series = p.series;
+    p.series = t.series;
+    p.series.count = scalar_series.count;
+    p.series.iteration = scalar_series.iteration;
+    p.series.radius2_m = scalar_series.radius2_m;
+    p.series.radius2_e = scalar_series.radius2_e;
+
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const int w = r.width, h = r.height;
+    const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
+    const std::vector<double>& off_m = prep.offset_m;
+    const std::vector<int32_t>& off_e = prep.offset_e;
+    char* doff = static_cast<char*>(s.offsets.get(off_m.size() * 12 + 512));
+    const size_t exp_offset = (off_m.size() * 8 + 255) & ~size_t(255);
+    check(hipMemcpyAsync(doff, off_m.data(), off_m.size() * 8, hipMemcpyHostToDevice, s.stream), "offsets");
+    check(hipMemcpyAsync(doff + exp_offset, off_e.data(), off_e.size() * 4, hipMemcpyHostToDevice, s.stream),
+          "offsets");
+    p.x_off_m = reinterpret_cast<const double*>(doff);
+    p.y_off_m = p.x_off_m + w;
+    p.x_off_e = reinterpret_cast<const int32_t*>(doff + exp_offset);
+    p.y_off_e = p.x_off_e + w;
+    p.output = r.device_pointers ? r.output : static_cast<float*>(s.output.get(pixels * sizeof(float)));
+    void* budget = s.budget.get(256);
+    p.budget_start = static_cast<const unsigned long long*>(budget);
+    p.budget_flag = reinterpret_cast<int*>(static_cast<char*>(budget) + 8);
+    p.budget_ticks = r.options.time_budget_ms > 0
+        ? static_cast<unsigned long long>(r.options.time_budget_ms * s.wall_clock_khz)
+        : 0ULL;
+    const uint64_t wall0 = now_ns();
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_alt_kernel(p, s.stream), "alternate kernel launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    if (!r.device_pointers)
+        check(hipMemcpyAsync(r.output, p.output, pixels * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H");
+    check(hipStreamSynchronize(s.stream), "alternate render");
+    float kernel_ms = 0.0f;
+    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    s.timing.kernel_ms = kernel_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
+    if (r.stats) {  // the alternate renderer reports pixels and time only (renderer.cpp:12439-12446)
+        std::fill(r.stats, r.stats + kStatsFields, 0ULL);
+        r.stats[0] = static_cast<uint64_t>(pixels);
+        r.stats[22] = static_cast<uint64_t>(static_cast<double>(kernel_ms) * 1e6);
+    }
 }
 
 void render_direct_on_device(const direct::DirectParams& host, const double* x, const double* y, bool planes,

@@ -60,12 +60,9 @@ struct DirectSetup {
 
 // The per-frame constants and coordinate vectors; pointers in `params` are
 // left null for the caller to bind.
-inline DirectSetup make_setup(int width, int height, const char* zoom_text, const char* x_text, const char* y_text,
-                              int max_iter, int formula, double julia_re, double julia_im, double output_bias,
-                              int escape_radius_mode, int coordinate_mode, uint32_t plane_flags) {
-    const long double zoom = parse_zoom_long(zoom_text);
-    const long double xl = parse_coordinate_long(x_text, "real");
-    const long double yl = parse_coordinate_long(y_text, "imaginary");
+inline DirectSetup make_setup_values(int width, int height, long double zoom, long double xl, long double yl,
+                                     int max_iter, int formula, double julia_re, double julia_im, double output_bias,
+                                     int escape_radius_mode, int coordinate_mode, uint32_t plane_flags) {
     if (escape_radius_mode != 0 && escape_radius_mode != 1) throw std::runtime_error("unknown escape-radius mode");
     if (coordinate_mode != 0 && coordinate_mode != 1) throw std::runtime_error("unknown coordinate mode");
     if (formula < 0 || formula > 3) throw std::runtime_error("unknown fractal formula");
@@ -102,6 +99,16 @@ inline DirectSetup make_setup(int width, int height, const char* zoom_text, cons
     p.spacing_m = spacing.m;
     p.spacing_e = spacing.e;
     return s;
+}
+
+inline DirectSetup make_setup(int width, int height, const char* zoom_text, const char* x_text, const char* y_text,
+                              int max_iter, int formula, double julia_re, double julia_im, double output_bias,
+                              int escape_radius_mode, int coordinate_mode, uint32_t plane_flags) {
+    const long double zoom = parse_zoom_long(zoom_text);
+    const long double xl = parse_coordinate_long(x_text, "real");
+    const long double yl = parse_coordinate_long(y_text, "imaginary");
+    return make_setup_values(width, height, zoom, xl, yl, max_iter, formula, julia_re, julia_im, output_bias,
+                             escape_radius_mode, coordinate_mode, plane_flags);
 }
 
 }  // namespace direct

@@ -96,6 +96,54 @@ constexpr int32_t kOrbitGlitchFlag = 1;
 constexpr int32_t kOrbitEscapeFlag = 2;
 constexpr size_t kStateBytesPerPixel = 9 * 8 + 9 * 4 + 2 * 4;
 
+// One alternate-formula linear BLA step (AlternateLinearBlaStep,
+// renderer.cpp:8520-8525): d' = M d + P dc with M = c[0..3] and P = c[4..7]
+// real 2x2 row-major FloatExp matrices, valid while |d|^2 < r2.
+struct alignas(64) AltStepDev {
+    double m[8];
+    double r2_m;
+    int32_t e[8];
+    int32_t r2_e, length;
+    int32_t pad[4];
+};
+static_assert(sizeof(AltStepDev) == 128, "alternate step is two cache lines");
+
+// Launch parameters of the alternate-formula kernel (Julia, burning ship,
+// tricorn; render_alternate_reference_impl, renderer.cpp:12064-12448).
+struct AltRenderParams {
+    const double* re;  // ScaledComplex orbit
+    const double* im;
+    const int32_t* e;
+    const double* margin_m;  // MPFR escape margin |Z|^2 - R^2 per orbit entry
+    const int32_t* margin_e;
+    int32_t orbit_size;
+    int32_t formula;
+    const AltStepDev* steps;
+    LevelDir dir;
+    SeriesDev series;  // Julia only
+    const double* x_off_m;
+    const int32_t* x_off_e;
+    const double* y_off_m;
+    const int32_t* y_off_e;
+    float* output;
+    int* budget_flag;
+    const unsigned long long* budget_start;
+    unsigned long long budget_ticks;
+    double parameter_re, parameter_im;  // c as ScaledComplex
+    int32_t parameter_e;
+    int32_t alt_start;
+    double input_r2_m;
+    int32_t input_r2_e;
+    int32_t width, height, max_iter;
+    int32_t escape_radius_mode;
+    int32_t strict;
+    int32_t cycle_enabled;
+    int32_t bla_enabled;
+    int32_t max_block;
+    int32_t series_available;
+    double output_bias;
+};
+
 // Everything one field launch needs, passed by value as the kernel argument.
 struct RenderParams {
     StateStore state;

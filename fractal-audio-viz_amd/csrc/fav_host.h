@@ -31,6 +31,14 @@ struct OrbitData {
     // glitch / escape outcomes of the loop-top tests for them.
     std::vector<double> zn_m;
     std::vector<int32_t> zn_e, small_exp, flags;
+    // alternate formulas: the MPFR escape margin |Z_n|^2 - R^2
+    // (renderer.cpp:10852-10860), read instead of the FloatExp one
+    std::vector<double> alt_margin_m;
+    std::vector<int32_t> alt_margin_e;
+};
+
+struct AltLevels {
+    std::vector<std::vector<AltStepDev>> levels;
 };
 
 struct CubicLevels {
@@ -54,6 +62,11 @@ struct DeviceTables {
     LevelDir linear_dir[2]{};
     const CubicStepDev* cubic = nullptr;
     LevelDir cubic_dir{};
+    void* alt_block = nullptr;
+    const AltStepDev* alt = nullptr;
+    const double* alt_margin_m = nullptr;  // inside orbit_block
+    const int32_t* alt_margin_e = nullptr;
+    LevelDir alt_dir{};
     size_t bytes = 0;
     ~DeviceTables();
 };
@@ -81,6 +94,14 @@ struct ReferenceContext {
     std::vector<SC> series_coefficients;
     CubicLevels cubic;
     LinearLevels linear[2];  // [0] ordinary, [1] deep (input radius * 1e-20)
+    // alternate formulas (renderer.cpp:8527-8531, 8881-8905)
+    SC parameter{0.0, 0.0, 0};
+    AltLevels alt;
+    FE alt_input_radius_squared{0.0, 0};
+    int alt_start = 1;
+    // centre and Julia constant as doubles, for the direct renderer below 1e6
+    long double x_center = 0.0L, y_center = 0.0L;
+    double julia_re = 0.0, julia_im = 0.0;
     uint64_t reference_build_ns = 0, series_build_ns = 0, bla_build_ns = 0;
     uint64_t upload_ns = 0;
     std::mutex device_mutex;
@@ -90,7 +111,13 @@ struct ReferenceContext {
 // Builders (reference_build.cpp).  They throw std::runtime_error.
 std::unique_ptr<ReferenceContext> build_reference(
     const char* x, const char* y, const char* zoom, int max_iter, int precision_bits,
-    int series_order, bool reusable, int formula, int escape_radius_mode, int coordinate_mode);
+    int series_order, bool reusable, int formula, int escape_radius_mode, int coordinate_mode,
+    const char* julia_re = "0", const char* julia_im = "0");
+
+// Alternate formulas (alt_build.cpp): the real 2x2 linear BLA hierarchy and
+// the settled-cycle repair of the projected orbit.
+void build_alternate_bla(ReferenceContext& context);
+void stabilize_alternate_cycle(ReferenceContext& context, OrbitData& orbit);
 std::unique_ptr<ReferenceContext> clone_reference(const ReferenceContext& source, const char* zoom);
 
 // parse_zoom_float_exp, renderer.cpp:11892-11915.

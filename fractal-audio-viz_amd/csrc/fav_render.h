@@ -30,6 +30,10 @@ struct LastTiming {
 
 void render_field(ReferenceContext& context, const FieldRequest& request);
 
+// Alternate-formula field (Julia, burning ship, tricorn) of a reference
+// handle; request.planes must be null.
+void render_alternate(ReferenceContext& context, const FieldRequest& request);
+
 // KFP colour transfer on the device.  `params` carries host pointers for
 // field / lut / output unless device_pointers is set (then field and output
 // are device pointers; the LUT is always a host pointer).

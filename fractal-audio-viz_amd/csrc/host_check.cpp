@@ -245,3 +245,55 @@ extern "C" int fav_host_check_direct(float* output, int width, int height, const
         return 1;
     }
 }
+
+#include "alt_core.h"
+
+// Alternate-formula field of alt_kernel.hip run on the CPU (test-only): the
+// same reference build, tables and per-pixel code.
+extern "C" int fav_host_check_alt(const char* x, const char* y, const char* ref_zoom, int max_iter, int bits,
+                                  int formula, const char* julia_re, const char* julia_im, const char* clone_zoom,
+                                  const FractalRenderOptions* options, float* output, int width, int height,
+                                  const char* zoom, char* error, int error_capacity) {
+    using namespace fav;
+    try {
+        auto root = build_reference(x, y, ref_zoom, max_iter, bits, 8, true, formula, options->escape_radius_mode,
+                                    options->coordinate_mode, julia_re, julia_im);
+        std::unique_ptr<ReferenceContext> clone;
+        if (clone_zoom) clone = clone_reference(*root, clone_zoom);
+        ReferenceContext& c = clone ? *clone : *root;
+        TableImage img;
+        make_table_image(c, img);
+        FieldRequest request{output, width, height, zoom, max_iter, 8, 32, *options, nullptr, nullptr, false};
+        PreparedAlternate prep;
+        prepare_alternate(c, request, prep);
+        AltRenderParams p = prep.params;
+        const OrbitData& o = *c.orbit;
+        p.re = img.re.data();
+        p.im = img.im.data();
+        p.e = img.e.data();
+        p.margin_m = o.alt_margin_m.data();
+        p.margin_e = o.alt_margin_e.data();
+        p.steps = img.alt.data();
+        p.dir = img.alt_dir;
+        p.series.re = img.series_re.data();
+        p.series.im = img.series_im.data();
+        p.series.e = img.series_e.data();
+        p.x_off_m = prep.offset_m.data();
+        p.y_off_m = prep.offset_m.data() + width;
+        p.x_off_e = prep.offset_e.data();
+        p.y_off_e = prep.offset_e.data() + width;
+        p.output = output;
+        int flag = 0;
+        unsigned long long start = 0;
+        p.budget_flag = &flag;
+        p.budget_start = &start;
+        p.budget_ticks = 0;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int py = 0; py < height; ++py)
+            for (int px = 0; px < width; ++px) alt::pixel(p, px, py);
+        return 0;
+    } catch (const std::exception& e) {
+        std::snprintf(error, static_cast<size_t>(error_capacity), "%s", e.what());
+        return 1;
+    }
+}

@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <locale>
+#include <sstream>
 #include <stdexcept>
 #include <thread>
 
@@ -69,7 +71,23 @@ struct Mpfr {
     Mpfr(const Mpfr&) = delete;
     Mpfr& operator=(const Mpfr&) = delete;
     operator mpfr_ptr() { return v; }
+    operator mpfr_srcptr() const { return v; }
 };
+
+// parse_coordinate, renderer.cpp:228-241 (locale-neutral long double).
+long double parse_long(const char* text, const char* label) {
+    std::istringstream stream(text ? text : "");
+    stream.imbue(std::locale::classic());
+    long double value = 0.0L;
+    stream >> value;
+    bool ok = !stream.fail();
+    if (ok) {
+        stream >> std::ws;
+        ok = stream.eof();
+    }
+    if (!ok || !std::isfinite(value)) throw std::runtime_error(std::string("invalid ") + label + " coordinate");
+    return value;
+}
 
 // Largest exponent t such that every normalised perturbation d with
 // d.e <= t leaves the loop-top values of orbit index n unchanged:
@@ -99,10 +117,77 @@ int32_t negligible_delta_exponent(SC z, FE margin) {
     return static_cast<int32_t>(t);
 }
 
-// make_reference_orbit, Mandelbrot branch, renderer.cpp:10640-10922.
-void make_orbit(ReferenceContext& c, const char* x, const char* y, const char* zoom, int max_iter) {
+// Repelling fixed point of z^2 + c, renderer.cpp:10736-10815: the root of
+// z^2 - z + c with the larger modulus, used as the Julia reference start when
+// the centre lies within 10^-digits of it (digits = max(32, ceil(log10 zoom)
+// + 16)).  Returns true and sets zr, zi when it applies.
+bool julia_fixed_point(const Mpfr& cx, const Mpfr& cy, const Mpfr& pr, const Mpfr& pi, const Mpfr& vz,
+                       Mpfr& zr, Mpfr& zi, long bits) {
+    Mpfr tmp(bits), dre(bits), dim(bits), norm(bits), dmag(bits), rre(bits), rim(bits), are(bits), aim(bits),
+        bre(bits), bim(bits), anorm(bits), bnorm(bits), nr(bits), ni(bits), mult(bits), vlog(bits), tol(bits),
+        dist(bits);
+    mpfr_set_ui(tmp, 4, MPFR_RNDN);
+    mpfr_mul(dre, pr, tmp, MPFR_RNDN);
+    mpfr_ui_sub(dre, 1, dre, MPFR_RNDN);
+    mpfr_mul(dim, pi, tmp, MPFR_RNDN);
+    mpfr_neg(dim, dim, MPFR_RNDN);
+    mpfr_mul(norm, dre, dre, MPFR_RNDN);
+    mpfr_mul(tmp, dim, dim, MPFR_RNDN);
+    mpfr_add(norm, norm, tmp, MPFR_RNDN);
+    mpfr_sqrt(dmag, norm, MPFR_RNDN);
+    mpfr_add(rre, dmag, dre, MPFR_RNDN);
+    mpfr_div_2ui(rre, rre, 1, MPFR_RNDN);
+    mpfr_sqrt(rre, rre, MPFR_RNDN);
+    mpfr_sub(rim, dmag, dre, MPFR_RNDN);
+    mpfr_div_2ui(rim, rim, 1, MPFR_RNDN);
+    mpfr_sqrt(rim, rim, MPFR_RNDN);
+    if (mpfr_sgn(dim.v) < 0) mpfr_neg(rim, rim, MPFR_RNDN);
+    mpfr_add_ui(are, rre, 1, MPFR_RNDN);
+    mpfr_div_2ui(are, are, 1, MPFR_RNDN);
+    mpfr_div_2ui(aim, rim, 1, MPFR_RNDN);
+    mpfr_set_ui(tmp, 1, MPFR_RNDN);
+    mpfr_sub(bre, tmp, rre, MPFR_RNDN);
+    mpfr_div_2ui(bre, bre, 1, MPFR_RNDN);
+    mpfr_neg(bim, aim, MPFR_RNDN);
+    mpfr_mul(anorm, are, are, MPFR_RNDN);
+    mpfr_mul(tmp, aim, aim, MPFR_RNDN);
+    mpfr_add(anorm, anorm, tmp, MPFR_RNDN);
+    mpfr_mul(bnorm, bre, bre, MPFR_RNDN);
+    mpfr_mul(tmp, bim, bim, MPFR_RNDN);
+    mpfr_add(bnorm, bnorm, tmp, MPFR_RNDN);
+    if (mpfr_cmp(anorm, bnorm) >= 0) {
+        mpfr_set(nr, are.v, MPFR_RNDN); mpfr_set(ni, aim.v, MPFR_RNDN); mpfr_set(norm, anorm.v, MPFR_RNDN);
+    } else {
+        mpfr_set(nr, bre.v, MPFR_RNDN); mpfr_set(ni, bim.v, MPFR_RNDN); mpfr_set(norm, bnorm.v, MPFR_RNDN);
+    }
+    mpfr_mul_ui(mult, norm, 4, MPFR_RNDN);
+    mpfr_log10(vlog, vz, MPFR_RNDN);
+    const double log_zoom = mpfr_get_d(vlog, MPFR_RNDN);
+    const int digits = std::max(32, static_cast<int>(std::ceil(std::max(0.0, log_zoom))) + 16);
+    mpfr_set_ui(tol, 10, MPFR_RNDN);
+    mpfr_pow_si(tol, tol, -static_cast<long>(digits), MPFR_RNDN);
+    mpfr_sub(tmp, cx, nr, MPFR_RNDN);
+    mpfr_mul(dist, tmp, tmp, MPFR_RNDN);
+    mpfr_sub(tmp, cy, ni, MPFR_RNDN);
+    mpfr_mul(norm, tmp, tmp, MPFR_RNDN);
+    mpfr_add(dist, dist, norm, MPFR_RNDN);
+    mpfr_mul(norm, tol, tol, MPFR_RNDN);
+    if (mpfr_cmp_ui(mult, 1) > 0 && mpfr_cmp(dist, norm) <= 0) {
+        mpfr_set(zr, nr.v, MPFR_RNDN);
+        mpfr_set(zi, ni.v, MPFR_RNDN);
+        return true;
+    }
+    return false;
+}
+
+// make_reference_orbit, renderer.cpp:10640-10922: the MPFR orbit of the
+// handle's formula (0 Mandelbrot, 1 Julia, 2 burning ship, 3 tricorn).
+void make_orbit(ReferenceContext& c, const char* x, const char* y, const char* zoom, int max_iter,
+                const char* julia_re, const char* julia_im, std::vector<FE>& alt_margins) {
     const long bits = c.precision_bits;
-    Mpfr cx(bits), cy(bits), vz(bits), vr(bits), zr(bits), zi(bits), nr(bits), ni(bits), tmp(bits);
+    const int formula = c.formula;
+    Mpfr cx(bits), cy(bits), vz(bits), vr(bits), zr(bits), zi(bits), nr(bits), ni(bits), tmp(bits), pr(bits),
+        pi(bits), ar(bits), ai(bits), norm(bits), margin(bits);
     if (mpfr_set_str(cx, x, 10, MPFR_RNDN) != 0 || mpfr_set_str(cy, y, 10, MPFR_RNDN) != 0)
         throw std::runtime_error("invalid MPFR Mandelbrot centre");
     if (zoom && mpfr_set_str(vz, zoom, 10, MPFR_RNDN) == 0 && mpfr_sgn(vz.v) > 0) {
@@ -112,22 +197,57 @@ void make_orbit(ReferenceContext& c, const char* x, const char* y, const char* z
     } else {
         c.input_radius = fe_from_parts(1.0, 0);
     }
-    mpfr_set_zero(zr, 0);
-    mpfr_set_zero(zi, 0);
+    if (formula == 1) {
+        if (mpfr_set_str(pr, julia_re, 10, MPFR_RNDN) != 0 || mpfr_set_str(pi, julia_im, 10, MPFR_RNDN) != 0)
+            throw std::runtime_error("invalid MPFR Julia constant");
+        mpfr_set(zr, cx.v, MPFR_RNDN);
+        mpfr_set(zi, cy.v, MPFR_RNDN);
+    } else {
+        mpfr_set(pr, cx.v, MPFR_RNDN);
+        mpfr_set(pi, cy.v, MPFR_RNDN);
+        mpfr_set_zero(zr, 0);
+        mpfr_set_zero(zi, 0);
+    }
+    c.parameter = sc_from_fe(fe_from_mpfr(pr.v), fe_from_mpfr(pi.v));
+    const bool fixed_point = formula == 1 && julia_fixed_point(cx, cy, pr, pi, vz, zr, zi, bits);
     c.fast_orbit.clear();
     c.fast_orbit.reserve(static_cast<size_t>(max_iter) + 1);
+    alt_margins.clear();
+    const double r2_double = c.escape_radius_mode == 1 ? 1.0e8 : 4.0;
     FE ore = fe_from_mpfr(zr.v), oim = fe_from_mpfr(zi.v);
     for (int i = 0; i <= max_iter; ++i) {
         if (!fe_finite(ore) || !fe_finite(oim)) break;
         c.fast_orbit.push_back(FEC{ore, oim});
-        // z <- z^2 + c, each MPFR operation correctly rounded (RNDN)
-        mpfr_mul(nr, zr, zr, MPFR_RNDN);
-        mpfr_mul(tmp, zi, zi, MPFR_RNDN);
-        mpfr_sub(nr, nr, tmp, MPFR_RNDN);
-        mpfr_mul(ni, zr, zi, MPFR_RNDN);
-        mpfr_mul_ui(ni, ni, 2, MPFR_RNDN);
-        mpfr_add(nr, nr, cx, MPFR_RNDN);
-        mpfr_add(ni, ni, cy, MPFR_RNDN);
+        if (formula != 0) {  // |Z|^2 - R^2 in MPFR (only the alternate path reads it)
+            mpfr_mul(norm, zr, zr, MPFR_RNDN);
+            mpfr_mul(tmp, zi, zi, MPFR_RNDN);
+            mpfr_add(norm, norm, tmp, MPFR_RNDN);
+            mpfr_set_d(tmp, r2_double, MPFR_RNDN);
+            mpfr_sub(margin, norm, tmp, MPFR_RNDN);
+            alt_margins.push_back(fe_from_mpfr(margin.v));
+        }
+        if (fixed_point) continue;
+        // each MPFR operation correctly rounded (RNDN)
+        if (formula == 2) {
+            mpfr_abs(ar, zr, MPFR_RNDN);
+            mpfr_abs(ai, zi, MPFR_RNDN);
+            mpfr_mul(nr, ar, ar, MPFR_RNDN);
+            mpfr_mul(tmp, ai, ai, MPFR_RNDN);
+            mpfr_sub(nr, nr, tmp, MPFR_RNDN);
+            mpfr_add(nr, nr, pr, MPFR_RNDN);
+            mpfr_mul(ni, ar, ai, MPFR_RNDN);
+            mpfr_mul_ui(ni, ni, 2, MPFR_RNDN);
+            mpfr_add(ni, ni, pi, MPFR_RNDN);
+        } else {
+            mpfr_mul(nr, zr, zr, MPFR_RNDN);
+            mpfr_mul(tmp, zi, zi, MPFR_RNDN);
+            mpfr_sub(nr, nr, tmp, MPFR_RNDN);
+            mpfr_mul(ni, zr, zi, MPFR_RNDN);
+            mpfr_mul_ui(ni, ni, 2, MPFR_RNDN);
+            if (formula == 3) mpfr_neg(ni, ni, MPFR_RNDN);
+            mpfr_add(nr, nr, pr, MPFR_RNDN);
+            mpfr_add(ni, ni, pi, MPFR_RNDN);
+        }
         mpfr_swap(zr, nr);
         mpfr_swap(zi, ni);
         ore = fe_from_mpfr(zr.v);
@@ -189,6 +309,24 @@ void make_orbit(ReferenceContext& c, const char* x, const char* y, const char* z
         if (sc_outside_escape(SN{zn.m, zn.e}, mode) || fe_compare(margin, fe_zero()) > 0) f |= kOrbitEscapeFlag;
         orbit->flags[i] = f;
     });
+    if (formula != 0) {
+        orbit->alt_margin_m.resize(static_cast<size_t>(n));
+        orbit->alt_margin_e.resize(static_cast<size_t>(n));
+        for (size_t i = 0; i < static_cast<size_t>(n); ++i) {
+            orbit->alt_margin_m[i] = alt_margins[i].m;
+            orbit->alt_margin_e[i] = alt_margins[i].e;
+        }
+        // renderer.cpp:10902-10920: repeat a settled cycle, then recompute
+        // the map end from the repaired orbit
+        stabilize_alternate_cycle(c, *orbit);
+        c.map_end = n - 1;
+        for (int i = 1; i < n; ++i) {
+            if (fe_compare(fec_norm_squared(c.fast_orbit[static_cast<size_t>(i)]), r2) > 0) {
+                c.map_end = i;
+                break;
+            }
+        }
+    }
     c.orbit = std::move(orbit);
 }
 
@@ -213,13 +351,16 @@ bool series_probe_is_safe(FEC reference, FEC exact, FEC approximate, FE minimum_
     return exact_inside && approx_inside;
 }
 
-// build_image_series, Mandelbrot branch, renderer.cpp:11206-11334.
+// build_image_series, renderer.cpp:11206-11334 (Mandelbrot and Julia; a
+// Julia pixel offset is the initial perturbation and c stays fixed).
 void build_image_series(ReferenceContext& c, const std::vector<FEC>& orbit) {
+    const bool julia = c.formula == 1;
     c.series_enabled = false;
     c.series_order = 0;
     c.series_iteration = 1;
     c.series_radius_squared = fe_zero();
     c.series_coefficients.clear();
+    if (c.formula != 0 && !julia) return;
     if (orbit.size() < 16 || c.requested_max_iter < 8) return;
     const int order = std::clamp(c.requested_series_order, 8, 32);
     const int map_end = std::clamp(c.map_end, 2, static_cast<int>(orbit.size()) - 1);
@@ -241,6 +382,10 @@ void build_image_series(ReferenceContext& c, const std::vector<FEC>& orbit) {
     std::vector<FEC> coefficients(n, FEC{z0, z0}), next(n, FEC{z0, z0}), best;
     FEC exact[12];
     for (auto& e : exact) e = FEC{z0, z0};
+    if (julia) {
+        coefficients[1] = FEC{fe_from_parts(1.0, 0), z0};
+        for (int p = 0; p < 12; ++p) exact[p] = probes[p];
+    }
     int best_iteration = 1;
     const FE tolerance_squared = fe_from_parts(std::ldexp(1.0, -38), 0);
     const FE minimum_scale_squared = fe_sqr(probe_radius);
@@ -254,7 +399,7 @@ void build_image_series(ReferenceContext& c, const std::vector<FEC>& orbit) {
             for (int left = 1; left < term; ++left)
                 value = fec_add(value, fec_mul(coefficients[static_cast<size_t>(left)],
                                                coefficients[static_cast<size_t>(term - left)]));
-            if (term == 1) value = fec_add(value, one);
+            if (!julia && term == 1) value = fec_add(value, one);
             next[static_cast<size_t>(term)] = value;
         }
         coefficients.swap(next);
@@ -262,7 +407,8 @@ void build_image_series(ReferenceContext& c, const std::vector<FEC>& orbit) {
         for (int probe = 0; probe < 12; ++probe) {
             const FEC dc = probes[probe];
             const FEC exact_next = fec_add(fec_mulf(fec_mul(reference, exact[probe]), two),
-                                           fec_add(fec_mul(exact[probe], exact[probe]), dc));
+                                           fec_add(fec_mul(exact[probe], exact[probe]),
+                                                   julia ? FEC{z0, z0} : dc));
             exact[probe] = exact_next;
             if (iteration + 1 >= order
                 && !series_probe_is_safe(orbit[static_cast<size_t>(iteration) + 1], exact_next,
@@ -532,9 +678,9 @@ FE parse_zoom(const char* text, long precision_bits) {  // renderer.cpp:11892-11
 
 std::unique_ptr<ReferenceContext> build_reference(
     const char* x, const char* y, const char* zoom, int max_iter, int precision_bits, int series_order,
-    bool reusable, int formula, int escape_radius_mode, int coordinate_mode) {
-    if (formula != 0)
-        throw std::runtime_error("the HIP backend renders the Mandelbrot formula only");
+    bool reusable, int formula, int escape_radius_mode, int coordinate_mode, const char* julia_re,
+    const char* julia_im) {
+    if (formula < 0 || formula > 3) throw std::runtime_error("invalid native reference formula or Julia constant");
     (void)parse_zoom(zoom, precision_bits);  // renderer.cpp:13638-13645
     auto c = std::make_unique<ReferenceContext>();
     c->formula = formula;
@@ -546,11 +692,29 @@ std::unique_ptr<ReferenceContext> build_reference(
     c->reusable = reusable;
     c->x_text = x;
     c->y_text = y;
+    c->x_center = parse_long(x, "real");
+    c->y_center = parse_long(y, "imaginary");
+    c->julia_re = static_cast<double>(parse_long(julia_re, "Julia real"));
+    c->julia_im = static_cast<double>(parse_long(julia_im, "Julia imaginary"));
     const uint64_t t0 = now_ns();
-    make_orbit(*c, x, y, zoom, max_iter);
+    std::vector<FE> alt_margins;
+    make_orbit(*c, x, y, zoom, max_iter, julia_re, julia_im, alt_margins);
     c->reference_build_ns = now_ns() - t0;
     const uint64_t t1 = now_ns();
-    build_bla(*c);
+    if (formula == 0) {
+        build_bla(*c);
+    } else {
+        // renderer.cpp:13683-13697: Julia's image series, then the real
+        // 2x2 hierarchy; alternate roots keep their builder orbit for clones
+        if (formula == 1) {
+            const uint64_t ts = now_ns();
+            build_image_series(*c, c->fast_orbit);
+            c->series_build_ns = now_ns() - ts;
+        }
+        build_alternate_bla(*c);
+        reusable = true;
+        c->reusable = true;
+    }
     c->bla_build_ns = now_ns() - t1;
     if (!reusable) {
         c->fast_orbit.clear();
@@ -580,6 +744,23 @@ std::unique_ptr<ReferenceContext> clone_reference(const ReferenceContext& s, con
     const uint64_t t1 = now_ns();
     const FE parsed = parse_zoom(zoom, c->precision_bits);
     c->input_radius = fe_div(fe_from_parts(viewport_height_factor(c->coordinate_mode), 0), parsed);
+    c->parameter = s.parameter;
+    c->x_center = s.x_center;
+    c->y_center = s.y_center;
+    c->julia_re = s.julia_re;
+    c->julia_im = s.julia_im;
+    if (s.formula != 0) {  // renderer.cpp:13739-13760
+        c->fast_orbit = s.fast_orbit;
+        c->reusable = true;
+        if (s.formula == 1) {
+            const uint64_t ts = now_ns();
+            build_image_series(*c, c->fast_orbit);
+            c->series_build_ns = now_ns() - ts;
+        }
+        build_alternate_bla(*c);
+        c->bla_build_ns = now_ns() - t1;
+        return c;
+    }
     const uint64_t series_started = now_ns();
     build_image_series(*c, s.fast_orbit);
     c->series_build_ns = now_ns() - series_started;

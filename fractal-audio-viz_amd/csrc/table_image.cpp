@@ -46,6 +46,56 @@ void make_table_image(const ReferenceContext& c, TableImage& img) {
         img.series_im.push_back(k.im);
         img.series_e.push_back(k.e);
     }
+    flatten(c.alt.levels, img.alt, img.alt_dir);
+}
+
+void prepare_alternate(const ReferenceContext& c, const FieldRequest& r, PreparedAlternate& out) {
+    const int w = r.width, h = r.height;
+    const FractalRenderOptions& opt = r.options;
+    const FE zoom = parse_zoom(r.zoom_text, c.precision_bits);
+    const double factor = opt.coordinate_mode == 1 ? 4.0 : 2.8;
+    const FE view_height = fe_muld(fe_div(fe_from_parts(1.0, 0), zoom), factor);
+    const FE view_width = fe_muld(view_height, static_cast<double>(w) / static_cast<double>(h));
+    out.offset_m.resize(static_cast<size_t>(w) + h);
+    out.offset_e.resize(static_cast<size_t>(w) + h);
+    const bool kalles = opt.coordinate_mode == 1;
+    const double ox = kalles ? static_cast<double>(w / 2) : static_cast<double>(w - 1) / 2.0;
+    const double oy = kalles ? static_cast<double>(h / 2) : static_cast<double>(h - 1) / 2.0;
+    for (int px = 0; px < w; ++px) {
+        const FE v = fe_muld(view_width, (static_cast<double>(px) - ox) / static_cast<double>(w));
+        out.offset_m[static_cast<size_t>(px)] = v.m;
+        out.offset_e[static_cast<size_t>(px)] = v.e;
+    }
+    for (int py = 0; py < h; ++py) {
+        const FE v = fe_muld(view_height, -(static_cast<double>(py) - oy) / static_cast<double>(h));
+        out.offset_m[static_cast<size_t>(w + py)] = v.m;
+        out.offset_e[static_cast<size_t>(w + py)] = v.e;
+    }
+    AltRenderParams& p = out.params;
+    p = AltRenderParams{};
+    p.orbit_size = static_cast<int32_t>(c.orbit->scaled.size());
+    p.formula = c.formula;
+    p.parameter_re = c.parameter.re;
+    p.parameter_im = c.parameter.im;
+    p.parameter_e = c.parameter.e;
+    p.alt_start = c.alt_start;
+    p.input_r2_m = c.alt_input_radius_squared.m;
+    p.input_r2_e = c.alt_input_radius_squared.e;
+    p.series.count = static_cast<int32_t>(c.series_coefficients.size());
+    p.series.iteration = c.series_iteration;
+    p.series.radius2_m = c.series_radius_squared.m;
+    p.series.radius2_e = c.series_radius_squared.e;
+    // renderer.cpp:12121-12128 (no point radius on a field render)
+    p.series_available = c.formula == 1 && c.series_enabled && c.series_iteration < r.max_iter;
+    p.width = w;
+    p.height = h;
+    p.max_iter = r.max_iter;
+    p.escape_radius_mode = opt.escape_radius_mode;
+    p.strict = opt.strict != 0;
+    p.cycle_enabled = opt.disable_cycle == 0 && (opt.strict == 0 || opt.strict_cycle != 0);
+    p.bla_enabled = opt.disable_bla == 0 && !c.alt.levels.empty();
+    p.max_block = std::clamp(opt.max_linear_bla_length, 2, 4096);
+    p.output_bias = opt.output_bias;
 }
 
 void prepare_render(const ReferenceContext& c, const FieldRequest& r, PreparedRender& out) {

@@ -20,6 +20,8 @@ struct TableImage {
     LevelDir cubic_dir{};
     std::vector<double> series_re, series_im;
     std::vector<int32_t> series_e;
+    std::vector<AltStepDev> alt;  // alternate formulas only
+    LevelDir alt_dir{};
 };
 
 void make_table_image(const ReferenceContext& context, TableImage& image);
@@ -35,6 +37,17 @@ struct PreparedRender {
 };
 
 void prepare_render(const ReferenceContext& context, const FieldRequest& request, PreparedRender& out);
+
+// Alternate formulas: the scalar part of AltRenderParams (table and output
+// pointers left null) and the per-column / per-row FloatExp offsets
+// (renderer.cpp:12082-12110, 12126-12140).
+struct PreparedAlternate {
+    AltRenderParams params{};
+    std::vector<double> offset_m;
+    std::vector<int32_t> offset_e;
+};
+
+void prepare_alternate(const ReferenceContext& context, const FieldRequest& request, PreparedAlternate& out);
 
 // Points p.state at per-pixel arrays inside `block`, which must hold
 // pixels * kStateBytesPerPixel + 20 * 256 bytes (each array is 256-byte aligned).

@@ -446,6 +446,15 @@ struct fo_reference {
     int requested_max_iter, requested_series_order;
     int escape_radius_mode, coordinate_mode;
     long precision_bits;
+    /* alternate formulas (renderer.cpp:8507-8565, 8881-8905) */
+    int formula;                 /* 0 Mandelbrot, 1 Julia, 2 burning ship, 3 tricorn */
+    sc_t parameter;              /* c (Julia constant or the centre) */
+    fe_t *margins;               /* MPFR escape margin |Z|^2 - R^2 per orbit entry */
+    int alt_levels;
+    size_t *alt_offset, *alt_size;
+    struct alt_step *alt;
+    fe_t alt_input_radius_squared;
+    int alt_start;
 };
 
 void fo_reference_destroy(fo_reference *r) {
@@ -454,6 +463,7 @@ void fo_reference_destroy(fo_reference *r) {
     free(r->cubic_offset); free(r->cubic_size); free(r->cubic);
     for (int t = 0; t < 2; ++t) { free(r->linear_offset[t]); free(r->linear_size[t]); free(r->linear[t]); }
     free(r->series_coefficients);
+    free(r->margins); free(r->alt_offset); free(r->alt_size); free(r->alt);
     free(r);
 }
 
@@ -482,49 +492,142 @@ static inline double pixel_axis_offset(int index, int dimension, int mode) { /* 
     return (double)index - origin;
 }
 
-/* make_reference_orbit, Mandelbrot branch, renderer.cpp:10640-10922. */
+static void stabilize_alternate_cycle(fo_reference *r);
+
+/* make_reference_orbit, renderer.cpp:10640-10922: MPFR iteration of the
+ * formula at the handle's precision; Julia starts from the repelling fixed
+ * point when the centre is within 10^-digits of it (10736-10815). */
 static int make_orbit(fo_reference *r, const char *x, const char *y, const char *zoom_text,
-                      int max_iter, int precision_bits, char *error, int cap) {
+                      int max_iter, int precision_bits, const char *julia_re, const char *julia_im,
+                      char *error, int cap) {
     precision_bits = precision_bits > 128 ? precision_bits : 128;
     r->precision_bits = precision_bits;
-    mpfr_t cx, cy, vz, vr, zr, zi, nr, ni, tmp, norm, margin;
-    mpfr_inits2(precision_bits, cx, cy, vz, vr, zr, zi, nr, ni, tmp, norm, margin, (mpfr_ptr)0);
+    const int formula = r->formula;
+    mpfr_t cx, cy, vz, vr, zr, zi, nr, ni, tmp, norm, margin, pr, pi, ar, ai;
+    mpfr_t dre, dim, dmag, rre, rim, are, aim, bre, bim, anorm, bnorm, dist, mult, tol, vlog;
+    mpfr_inits2(precision_bits, cx, cy, vz, vr, zr, zi, nr, ni, tmp, norm, margin, pr, pi, ar, ai,
+                dre, dim, dmag, rre, rim, are, aim, bre, bim, anorm, bnorm, dist, mult, tol, vlog, (mpfr_ptr)0);
     int ok = 0;
     if (mpfr_set_str(cx, x, 10, MPFR_RNDN) != 0 || mpfr_set_str(cy, y, 10, MPFR_RNDN) != 0) {
         fo_set_error(error, cap, "invalid MPFR Mandelbrot centre");
         goto done;
     }
-    if (zoom_text && mpfr_set_str(vz, zoom_text, 10, MPFR_RNDN) == 0 && mpfr_sgn(vz) > 0) {
+    const int have_zoom = zoom_text && mpfr_set_str(vz, zoom_text, 10, MPFR_RNDN) == 0 && mpfr_sgn(vz) > 0;
+    if (have_zoom) {
         mpfr_set_d(vr, viewport_height_factor(r->coordinate_mode), MPFR_RNDN);
         mpfr_div(vr, vr, vz, MPFR_RNDN);
         r->input_radius = fe_from_mpfr(vr);
     } else {
         r->input_radius = fe_from_parts(1.0, 0);
     }
-    mpfr_set_zero(zr, 0);
-    mpfr_set_zero(zi, 0);
+    if (formula == 1) {
+        if (mpfr_set_str(pr, julia_re, 10, MPFR_RNDN) != 0 || mpfr_set_str(pi, julia_im, 10, MPFR_RNDN) != 0) {
+            fo_set_error(error, cap, "invalid MPFR Julia constant");
+            goto done;
+        }
+        mpfr_set(zr, cx, MPFR_RNDN);
+        mpfr_set(zi, cy, MPFR_RNDN);
+    } else {
+        mpfr_set(pr, cx, MPFR_RNDN);
+        mpfr_set(pi, cy, MPFR_RNDN);
+        mpfr_set_zero(zr, 0);
+        mpfr_set_zero(zi, 0);
+    }
+    r->parameter = sc_from_fe(fe_from_mpfr(pr), fe_from_mpfr(pi));
+    int fixed_point = 0;
+    if (formula == 1) {
+        /* roots of z^2 - z + c: (1 +- sqrt(1 - 4c))/2, the larger |z| repels */
+        mpfr_set_ui(tmp, 4, MPFR_RNDN);
+        mpfr_mul(dre, pr, tmp, MPFR_RNDN);
+        mpfr_ui_sub(dre, 1, dre, MPFR_RNDN);
+        mpfr_mul(dim, pi, tmp, MPFR_RNDN);
+        mpfr_neg(dim, dim, MPFR_RNDN);
+        mpfr_mul(norm, dre, dre, MPFR_RNDN);
+        mpfr_mul(tmp, dim, dim, MPFR_RNDN);
+        mpfr_add(norm, norm, tmp, MPFR_RNDN);
+        mpfr_sqrt(dmag, norm, MPFR_RNDN);
+        mpfr_add(rre, dmag, dre, MPFR_RNDN);
+        mpfr_div_2ui(rre, rre, 1, MPFR_RNDN);
+        mpfr_sqrt(rre, rre, MPFR_RNDN);
+        mpfr_sub(rim, dmag, dre, MPFR_RNDN);
+        mpfr_div_2ui(rim, rim, 1, MPFR_RNDN);
+        mpfr_sqrt(rim, rim, MPFR_RNDN);
+        if (mpfr_sgn(dim) < 0) mpfr_neg(rim, rim, MPFR_RNDN);
+        mpfr_add_ui(are, rre, 1, MPFR_RNDN);
+        mpfr_div_2ui(are, are, 1, MPFR_RNDN);
+        mpfr_div_2ui(aim, rim, 1, MPFR_RNDN);
+        mpfr_set_ui(tmp, 1, MPFR_RNDN);
+        mpfr_sub(bre, tmp, rre, MPFR_RNDN);
+        mpfr_div_2ui(bre, bre, 1, MPFR_RNDN);
+        mpfr_neg(bim, aim, MPFR_RNDN);
+        mpfr_mul(anorm, are, are, MPFR_RNDN);
+        mpfr_mul(tmp, aim, aim, MPFR_RNDN);
+        mpfr_add(anorm, anorm, tmp, MPFR_RNDN);
+        mpfr_mul(bnorm, bre, bre, MPFR_RNDN);
+        mpfr_mul(tmp, bim, bim, MPFR_RNDN);
+        mpfr_add(bnorm, bnorm, tmp, MPFR_RNDN);
+        if (mpfr_cmp(anorm, bnorm) >= 0) {
+            mpfr_set(nr, are, MPFR_RNDN); mpfr_set(ni, aim, MPFR_RNDN); mpfr_set(norm, anorm, MPFR_RNDN);
+        } else {
+            mpfr_set(nr, bre, MPFR_RNDN); mpfr_set(ni, bim, MPFR_RNDN); mpfr_set(norm, bnorm, MPFR_RNDN);
+        }
+        mpfr_mul_ui(mult, norm, 4, MPFR_RNDN);
+        /* the zoom was validated, so viewport_zoom holds it here */
+        mpfr_log10(vlog, vz, MPFR_RNDN);
+        const double log_zoom = mpfr_get_d(vlog, MPFR_RNDN);
+        int digits = (int)ceil(log_zoom > 0.0 ? log_zoom : 0.0) + 16;
+        if (digits < 32) digits = 32;
+        mpfr_set_ui(tol, 10, MPFR_RNDN);
+        mpfr_pow_si(tol, tol, -(long)digits, MPFR_RNDN);
+        mpfr_sub(tmp, cx, nr, MPFR_RNDN);
+        mpfr_mul(dist, tmp, tmp, MPFR_RNDN);
+        mpfr_sub(tmp, cy, ni, MPFR_RNDN);
+        mpfr_mul(norm, tmp, tmp, MPFR_RNDN);
+        mpfr_add(dist, dist, norm, MPFR_RNDN);
+        mpfr_mul(norm, tol, tol, MPFR_RNDN);
+        if (mpfr_cmp_ui(mult, 1) > 0 && mpfr_cmp(dist, norm) <= 0) {
+            mpfr_set(zr, nr, MPFR_RNDN);
+            mpfr_set(zi, ni, MPFR_RNDN);
+            fixed_point = 1;
+        }
+    }
     r->fast_orbit = (fec_t *)malloc(sizeof(fec_t) * ((size_t)max_iter + 1));
-    if (!r->fast_orbit) { fo_set_error(error, cap, "out of memory"); goto done; }
+    r->margins = (fe_t *)malloc(sizeof(fe_t) * ((size_t)max_iter + 1));
+    if (!r->fast_orbit || !r->margins) { fo_set_error(error, cap, "out of memory"); goto done; }
     fe_t ore = fe_from_mpfr(zr), oim = fe_from_mpfr(zi);
     int finite_size = 0;
     const double r2 = mode_r2_double(r->escape_radius_mode);
     for (int i = 0; i <= max_iter; ++i) {
         if (!fe_finite(ore) || !fe_finite(oim)) break;
-        r->fast_orbit[finite_size++] = fec_make(ore, oim);
-        /* the signed escape margin is only consumed by alternate formulas;
-         * it is still computed to keep the MPFR sequence identical */
+        r->fast_orbit[finite_size] = fec_make(ore, oim);
         mpfr_mul(norm, zr, zr, MPFR_RNDN);
         mpfr_mul(tmp, zi, zi, MPFR_RNDN);
         mpfr_add(norm, norm, tmp, MPFR_RNDN);
         mpfr_set_d(tmp, r2, MPFR_RNDN);
         mpfr_sub(margin, norm, tmp, MPFR_RNDN);
-        mpfr_mul(nr, zr, zr, MPFR_RNDN);
-        mpfr_mul(tmp, zi, zi, MPFR_RNDN);
-        mpfr_sub(nr, nr, tmp, MPFR_RNDN);
-        mpfr_mul(ni, zr, zi, MPFR_RNDN);
-        mpfr_mul_ui(ni, ni, 2, MPFR_RNDN);
-        mpfr_add(nr, nr, cx, MPFR_RNDN);
-        mpfr_add(ni, ni, cy, MPFR_RNDN);
+        r->margins[finite_size] = fe_from_mpfr(margin);
+        ++finite_size;
+        if (fixed_point) continue;
+        if (formula == 2) {
+            mpfr_abs(ar, zr, MPFR_RNDN);
+            mpfr_abs(ai, zi, MPFR_RNDN);
+            mpfr_mul(nr, ar, ar, MPFR_RNDN);
+            mpfr_mul(tmp, ai, ai, MPFR_RNDN);
+            mpfr_sub(nr, nr, tmp, MPFR_RNDN);
+            mpfr_add(nr, nr, pr, MPFR_RNDN);
+            mpfr_mul(ni, ar, ai, MPFR_RNDN);
+            mpfr_mul_ui(ni, ni, 2, MPFR_RNDN);
+            mpfr_add(ni, ni, pi, MPFR_RNDN);
+        } else {
+            mpfr_mul(nr, zr, zr, MPFR_RNDN);
+            mpfr_mul(tmp, zi, zi, MPFR_RNDN);
+            mpfr_sub(nr, nr, tmp, MPFR_RNDN);
+            mpfr_mul(ni, zr, zi, MPFR_RNDN);
+            mpfr_mul_ui(ni, ni, 2, MPFR_RNDN);
+            if (formula == 3) mpfr_neg(ni, ni, MPFR_RNDN);
+            mpfr_add(nr, nr, pr, MPFR_RNDN);
+            mpfr_add(ni, ni, pi, MPFR_RNDN);
+        }
         mpfr_set(zr, nr, MPFR_RNDN);
         mpfr_set(zi, ni, MPFR_RNDN);
         ore = fe_from_mpfr(zr);
@@ -547,10 +650,71 @@ static int make_orbit(fo_reference *r, const char *x, const char *y, const char 
         r->real_double[i] = ldexp(r->scaled[i].re, r->scaled[i].e);
         r->imag_double[i] = ldexp(r->scaled[i].im, r->scaled[i].e);
     }
+    if (formula != 0) {
+        stabilize_alternate_cycle(r);
+        r->map_end = finite_size - 1;
+        for (int i = 1; i < finite_size; ++i) {
+            if (fe_compare(fec_norm_squared(r->fast_orbit[i]), er2) > 0) { r->map_end = i; break; }
+        }
+    }
     ok = 1;
 done:
-    mpfr_clears(cx, cy, vz, vr, zr, zi, nr, ni, tmp, norm, margin, (mpfr_ptr)0);
+    mpfr_clears(cx, cy, vz, vr, zr, zi, nr, ni, tmp, norm, margin, pr, pi, ar, ai,
+                dre, dim, dmag, rre, rim, are, aim, bre, bim, anorm, bnorm, dist, mult, tol, vlog, (mpfr_ptr)0);
     return ok;
+}
+
+/* stabilize_alternate_reference_cycle, renderer.cpp:9691-9788: a settled
+ * projected cycle (three matching periods within 1e-12 relative, period
+ * <= 64, found in the first 1024 entries) is repeated to the orbit end. */
+static void stabilize_alternate_cycle(fo_reference *r) {
+    const size_t count = (size_t)r->orbit_size;
+    if (count < 6) return;
+    size_t last_finite = count;
+    while (last_finite > 0) {
+        const size_t index = last_finite - 1;
+        if (isfinite(r->real_double[index]) && isfinite(r->imag_double[index])) break;
+        --last_finite;
+    }
+    if (last_finite < 6) return;
+    const size_t scan_limit = last_finite - 1 < 1024 ? last_finite - 1 : 1024;
+    const int max_period = (int)((scan_limit + 1) / 4) < 64 ? (int)((scan_limit + 1) / 4) : 64;
+    const double tolerance = 1.0e-12;
+    const double r2 = mode_r2_double(r->escape_radius_mode);
+    for (int period = 1; period <= max_period; ++period) {
+        for (size_t end = (size_t)(4 * period - 1); end <= scan_limit; ++end) {
+            const size_t cycle_start = end - (size_t)(3 * period) + 1U;
+            int matches = 1;
+            for (int offset = 0; offset < period && matches; ++offset) {
+                const size_t a = cycle_start + (size_t)offset, b = a + (size_t)period, c = b + (size_t)period;
+                const double ar = r->real_double[a], ai = r->imag_double[a];
+                const double br = r->real_double[b], bi = r->imag_double[b];
+                const double cr = r->real_double[c], ci = r->imag_double[c];
+                const double ra = hypot(ar, ai), rb = hypot(br, bi), rc = hypot(cr, ci);
+                if (!isfinite(ra) || !isfinite(rb) || !isfinite(rc) || ra * ra >= r2 || rb * rb >= r2
+                    || rc * rc >= r2) {
+                    matches = 0;
+                    break;
+                }
+                double scale = rb > rc ? rb : rc;
+                scale = ra > scale ? ra : scale;
+                scale = 1.0 > scale ? 1.0 : scale;
+                const double d1 = hypot(br - ar, bi - ai);
+                const double d2 = hypot(cr - br, ci - bi);
+                if (!(d1 <= tolerance * scale && d2 <= tolerance * scale)) matches = 0;
+            }
+            if (!matches) continue;
+            for (size_t target = cycle_start; target < count; ++target) {
+                const size_t source = cycle_start + (target - cycle_start) % (size_t)period;
+                r->fast_orbit[target] = r->fast_orbit[source];
+                r->scaled[target] = r->scaled[source];
+                r->real_double[target] = r->real_double[source];
+                r->imag_double[target] = r->imag_double[source];
+                r->margins[target] = r->margins[source];
+            }
+            return;
+        }
+    }
 }
 
 /* ---- image series (renderer.cpp:11162-11334) --------------------------- */
@@ -575,8 +739,10 @@ static int series_probe_is_safe(fec_t reference, fec_t exact, fec_t approximate,
     return exact_inside && approx_inside;
 }
 
-/* build_image_series, Mandelbrot branch, renderer.cpp:11206-11334. */
+/* build_image_series, renderer.cpp:11206-11334 (Mandelbrot and Julia; for
+ * Julia the pixel offset is the initial perturbation and c is fixed). */
 static int build_image_series(fo_reference *r, const fec_t *orbit, int orbit_size) {
+    const int julia = r->formula == 1;
     r->series_enabled = 0;
     r->series_order = 0;
     r->series_iteration = 1;
@@ -584,6 +750,7 @@ static int build_image_series(fo_reference *r, const fec_t *orbit, int orbit_siz
     free(r->series_coefficients);
     r->series_coefficients = NULL;
     r->series_count = 0;
+    if (r->formula != 0 && !julia) return 1;
     if (orbit_size < 16 || r->requested_max_iter < 8) return 1;
     int order = r->requested_series_order;
     order = order < 8 ? 8 : (order > 32 ? 32 : order);
@@ -611,6 +778,10 @@ static int build_image_series(fo_reference *r, const fec_t *orbit, int orbit_siz
     if (!coef || !next || !best) { free(coef); free(next); free(best); return 0; }
     fec_t exact[12];
     memset(exact, 0, sizeof(exact));
+    if (julia) {
+        coef[1] = fec_make(fe_from_parts(1.0, 0), FE_ZERO);
+        for (int p = 0; p < 12; ++p) exact[p] = probes[p];
+    }
     int best_iteration = 1, have_best = 0;
     const fe_t tolerance_squared = fe_from_parts(ldexp(1.0, -38), 0);
     const fe_t minimum_scale_squared = fe_sqr(probe_radius);
@@ -623,7 +794,7 @@ static int build_image_series(fo_reference *r, const fec_t *orbit, int orbit_siz
             fec_t value = fec_mulf(fec_mul(reference, coef[term]), two);
             for (int left = 1; left < term; ++left)
                 value = fec_add(value, fec_mul(coef[left], coef[term - left]));
-            if (term == 1) value = fec_add(value, one);
+            if (!julia && term == 1) value = fec_add(value, one);
             next[term] = value;
         }
         fec_t *swap = coef; coef = next; next = swap;
@@ -631,7 +802,8 @@ static int build_image_series(fo_reference *r, const fec_t *orbit, int orbit_siz
         for (int p = 0; p < 12; ++p) {
             const fec_t dc = probes[p];
             const fec_t exact_next = fec_add(fec_mulf(fec_mul(reference, exact[p]), two),
-                                             fec_add(fec_mul(exact[p], exact[p]), dc));
+                                             fec_add(fec_mul(exact[p], exact[p]),
+                                                     julia ? fec_make(FE_ZERO, FE_ZERO) : dc));
             exact[p] = exact_next;
             if (iteration + 1 >= order) {
                 const fec_t approximate = series_evaluate(coef, n, dc);
@@ -879,10 +1051,21 @@ static int build_bla(fo_reference *r) {
 }
 
 /* create_reference_context, Mandelbrot, renderer.cpp:13619-13702. */
+static int build_alternate_bla(fo_reference *r);
+
 fo_reference *fo_reference_create(const char *x, const char *y, const char *zoom, int max_iter,
                                   int precision_bits, int series_order, int escape_radius_mode,
                                   int coordinate_mode, char *error, int cap) {
-    if (!x || !y || !zoom || max_iter <= 0 || precision_bits < 128) {
+    return fo_reference_create_formula(x, y, zoom, max_iter, precision_bits, series_order, escape_radius_mode,
+                                       coordinate_mode, 0, "0", "0", error, cap);
+}
+
+fo_reference *fo_reference_create_formula(const char *x, const char *y, const char *zoom, int max_iter,
+                                          int precision_bits, int series_order, int escape_radius_mode,
+                                          int coordinate_mode, int formula, const char *julia_re,
+                                          const char *julia_im, char *error, int cap) {
+    if (!x || !y || !zoom || max_iter <= 0 || precision_bits < 128 || formula < 0 || formula > 3 || !julia_re
+        || !julia_im) {
         fo_set_error(error, cap, "invalid reference configuration");
         return NULL;
     }
@@ -897,7 +1080,10 @@ fo_reference *fo_reference_create(const char *x, const char *y, const char *zoom
     r->coordinate_mode = coordinate_mode;
     r->requested_series_order = series_order < 8 ? 8 : (series_order > 32 ? 32 : series_order);
     r->requested_max_iter = max_iter;
-    if (!make_orbit(r, x, y, zoom, max_iter, precision_bits, error, cap) || !build_bla(r)) {
+    r->formula = formula;
+    if (!make_orbit(r, x, y, zoom, max_iter, precision_bits, julia_re, julia_im, error, cap)
+        || !(formula == 0 ? build_bla(r)
+                          : build_image_series(r, r->fast_orbit, r->fast_orbit_size) && build_alternate_bla(r))) {
         fo_reference_destroy(r);
         return NULL;
     }
@@ -957,6 +1143,8 @@ fo_reference *fo_reference_clone(const fo_reference *s, const char *zoom, char *
     }
     fo_reference *r = (fo_reference *)calloc(1, sizeof(fo_reference));
     if (!r) return NULL;
+    r->formula = s->formula;
+    r->parameter = s->parameter;
     r->orbit_size = s->orbit_size;
     r->scaled = (sc_t *)malloc(sizeof(sc_t) * (size_t)s->orbit_size);
     r->real_double = (double *)malloc(sizeof(double) * (size_t)s->orbit_size);
@@ -971,6 +1159,20 @@ fo_reference *fo_reference_clone(const fo_reference *s, const char *zoom, char *
     r->coordinate_mode = s->coordinate_mode;
     r->precision_bits = s->precision_bits;
     r->input_radius = fe_div(fe_from_parts(viewport_height_factor(r->coordinate_mode), 0), parsed);
+    if (s->formula != 0) {
+        /* renderer.cpp:13739-13760: alternates rebuild series and matrices */
+        r->fast_orbit_size = s->fast_orbit_size;
+        r->fast_orbit = (fec_t *)malloc(sizeof(fec_t) * (size_t)s->fast_orbit_size);
+        r->margins = (fe_t *)malloc(sizeof(fe_t) * (size_t)s->orbit_size);
+        if (!r->fast_orbit || !r->margins) { fo_reference_destroy(r); return NULL; }
+        memcpy(r->fast_orbit, s->fast_orbit, sizeof(fec_t) * (size_t)s->fast_orbit_size);
+        memcpy(r->margins, s->margins, sizeof(fe_t) * (size_t)s->orbit_size);
+        if (!build_image_series(r, r->fast_orbit, r->fast_orbit_size) || !build_alternate_bla(r)) {
+            fo_reference_destroy(r);
+            return NULL;
+        }
+        return r;
+    }
     if (!build_image_series(r, s->fast_orbit, s->fast_orbit_size)) { fo_reference_destroy(r); return NULL; }
     /* cubic levels copied, then radii retargeted (renderer.cpp:11802-11820) */
     r->cubic_levels = s->cubic_levels;
@@ -1580,6 +1782,9 @@ static void render_pixel(const render_ctx *c, float *output, size_t index, sc_t 
         st[ST_MAX_PIXEL_ITER] = (uint64_t)(iteration > 0 ? iteration : 0);
 }
 
+static int render_alternate(float *output, int width, int height, fe_t zoom, const fo_reference *r, int max_iter,
+                            const FractalRenderOptions *o, uint64_t *stats, int threads, struct timespec started);
+
 int fo_render(float *output, int width, int height, const char *zoom_text, const fo_reference *r,
               int max_iter, int series_order, int series_block, const FractalRenderOptions *options,
               FractalRenderPlanes *planes, uint64_t *stats, int threads, char *error, int cap) {
@@ -1593,6 +1798,17 @@ int fo_render(float *output, int width, int height, const char *zoom_text, const
     if (!parse_zoom_fe(zoom_text, r->precision_bits, &zoom)) {
         fo_set_error(error, cap, "invalid deep Mandelbrot zoom");
         return 1;
+    }
+    if (r->formula != 0) {
+        if (planes) {
+            fo_set_error(error, cap, "render metadata requires the native scalar backend for this formula");
+            return 1;
+        }
+        if (render_alternate(output, width, height, zoom, r, max_iter, options, stats, threads, started)) {
+            fo_set_error(error, cap, "out of memory");
+            return 1;
+        }
+        return 0;
     }
     render_ctx c;
     memset(&c, 0, sizeof(c));
@@ -1851,5 +2067,413 @@ int fo_render_direct(float *output, int width, int height, const char *zoom, con
     for (int py = 0; py < height; ++py)
         for (int px = 0; px < width; ++px) direct_pixel(&c, px, py);
     free(xs); free(ys);
+    return 0;
+}
+
+/* ---- alternate formulas (renderer.cpp:11336-11530, 11917-12448) ---------- */
+
+/* AlternateLinearBlaStep, renderer.cpp:8520-8525: d' = M d + P dc with
+ * M = c[0..3] and P = c[4..7], both real 2x2 row-major. */
+typedef struct alt_step { fe_t c[8]; fe_t r2; int length; } alt_step_t;
+
+static inline fe_t fe_abs(fe_t v) { return fe_make(fabs(v.m), v.e); }
+
+static fe_t alt_matrix_norm(const fe_t *c, int offset) { /* renderer.cpp:11340-11350 */
+    fe_t sum = FE_ZERO;
+    for (int k = 0; k < 4; ++k) sum = fe_add(sum, fe_sqr(c[offset + k]));
+    return fe_sqrt(sum);
+}
+
+static alt_step_t merge_alt(const alt_step_t *y, const alt_step_t *x, fe_t input_parameter_radius) {
+    /* merge_alternate_linear_bla, renderer.cpp:11352-11397 */
+    const fe_t *yc = y->c, *xc = x->c;
+    alt_step_t out;
+    out.c[0] = fe_add(fe_mul(yc[0], xc[0]), fe_mul(yc[1], xc[2]));
+    out.c[1] = fe_add(fe_mul(yc[0], xc[1]), fe_mul(yc[1], xc[3]));
+    out.c[2] = fe_add(fe_mul(yc[2], xc[0]), fe_mul(yc[3], xc[2]));
+    out.c[3] = fe_add(fe_mul(yc[2], xc[1]), fe_mul(yc[3], xc[3]));
+    out.c[4] = fe_add(fe_add(fe_mul(yc[0], xc[4]), fe_mul(yc[1], xc[6])), yc[4]);
+    out.c[5] = fe_add(fe_add(fe_mul(yc[0], xc[5]), fe_mul(yc[1], xc[7])), yc[5]);
+    out.c[6] = fe_add(fe_add(fe_mul(yc[2], xc[4]), fe_mul(yc[3], xc[6])), yc[6]);
+    out.c[7] = fe_add(fe_add(fe_mul(yc[2], xc[5]), fe_mul(yc[3], xc[7])), yc[7]);
+    const fe_t x_state = alt_matrix_norm(xc, 0);
+    const fe_t x_parameter = alt_matrix_norm(xc, 4);
+    fe_t radius = fe_sqrt(x->r2);
+    const fe_t remaining = fe_sub(fe_sqrt(y->r2), fe_mul(x_parameter, input_parameter_radius));
+    if (fe_compare(remaining, FE_ZERO) > 0 && fe_compare(x_state, FE_ZERO) > 0)
+        radius = fe_min(radius, fe_div(remaining, x_state));
+    else
+        radius = FE_ZERO;
+    radius = fe_muld(radius, 1.0 - ldexp(1.0, -40));
+    out.r2 = fe_sqr(radius);
+    out.length = x->length + y->length;
+    return out;
+}
+
+/* build_alternate_linear_bla, renderer.cpp:11424-11526. */
+static int build_alternate_bla(fo_reference *r) {
+    free(r->alt); free(r->alt_offset); free(r->alt_size);
+    r->alt = NULL; r->alt_offset = NULL; r->alt_size = NULL;
+    r->alt_levels = 0;
+    r->alt_input_radius_squared = fe_sqr(r->input_radius);
+    r->alt_start = r->formula == 1 ? 0 : 1;
+    if (r->fast_orbit_size < 2) return 1;
+    const int first = r->alt_start;
+    const int last = r->map_end < r->fast_orbit_size - 1 ? r->map_end : r->fast_orbit_size - 1;
+    const int base_count = last - first;
+    if (base_count <= 0) return 1;
+    size_t sizes[64], total = 0;
+    int levels = 0;
+    sizes[levels++] = (size_t)base_count;
+    total += (size_t)base_count;
+    for (int level = 1; level < 64; ++level) {
+        if ((1ULL << level) > 4096ULL) break;  /* MAX_SAFE_LINEAR_BLA_LENGTH */
+        const size_t current = (sizes[level - 1] + 1) / 2;
+        if (current == 0) break;
+        sizes[levels++] = current;
+        total += current;
+    }
+    r->alt = (alt_step_t *)calloc(total, sizeof(alt_step_t));
+    r->alt_offset = (size_t *)malloc(sizeof(size_t) * (size_t)levels);
+    r->alt_size = (size_t *)malloc(sizeof(size_t) * (size_t)levels);
+    if (!r->alt || !r->alt_offset || !r->alt_size) return 0;
+    size_t offset = 0;
+    for (int l = 0; l < levels; ++l) { r->alt_offset[l] = offset; r->alt_size[l] = sizes[l]; offset += sizes[l]; }
+    r->alt_levels = levels;
+    const fe_t tolerance = fe_from_parts(ldexp(1.0, -38), 0);
+    const fe_t two = fe_from_parts(2.0, 0), quarter = fe_from_parts(0.25, 0), one = fe_from_parts(1.0, 0);
+    for (int k = 0; k < base_count; ++k) {
+        const fec_t ref = r->fast_orbit[first + k];
+        const fe_t re = ref.re, im = ref.im;
+        alt_step_t *b = &r->alt[k];
+        memset(b, 0, sizeof(*b));
+        if (r->formula == 3) {
+            b->c[0] = fe_mul(re, two);
+            b->c[1] = fe_muld(im, -2.0);
+            b->c[2] = fe_muld(im, -2.0);
+            b->c[3] = fe_muld(re, -2.0);
+        } else if (r->formula == 2) {
+            const fe_t are = fe_abs(re), aim = fe_abs(im);
+            const double rs = re.m < 0.0 ? -1.0 : 1.0, is = im.m < 0.0 ? -1.0 : 1.0;
+            b->c[0] = fe_muld(are, 2.0 * rs);
+            b->c[1] = fe_muld(aim, -2.0 * is);
+            b->c[2] = fe_muld(aim, 2.0 * rs);
+            b->c[3] = fe_muld(are, 2.0 * is);
+        } else {
+            b->c[0] = fe_mul(re, two);
+            b->c[1] = fe_muld(im, -2.0);
+            b->c[2] = fe_mul(im, two);
+            b->c[3] = fe_mul(re, two);
+        }
+        if (r->formula != 1) {
+            b->c[4] = one;
+            b->c[7] = one;
+        }
+        const fe_t magnitude = fe_sqrt(fec_norm_squared(ref));
+        fe_t radius = fe_mul(tolerance, fe_compare(magnitude, one) > 0 ? magnitude : one);
+        if (r->formula == 2) {
+            if (fe_zero(re) || fe_zero(im)) {
+                radius = FE_ZERO;
+            } else {
+                radius = fe_min(radius, fe_mul(fe_abs(re), quarter));
+                radius = fe_min(radius, fe_mul(fe_abs(im), quarter));
+            }
+        }
+        b->r2 = fe_sqr(radius);
+        b->length = 1;
+    }
+    for (int level = 1; level < levels; ++level) {
+        const alt_step_t *prev = r->alt + r->alt_offset[level - 1];
+        alt_step_t *cur = r->alt + r->alt_offset[level];
+        const size_t prev_size = r->alt_size[level - 1];
+        for (size_t index = 0; index < r->alt_size[level]; ++index) {
+            const size_t f = index * 2;
+            cur[index] = f + 1 < prev_size ? merge_alt(&prev[f + 1], &prev[f], r->input_radius) : prev[f];
+        }
+    }
+    return 1;
+}
+
+/* AlternateLinearBlaLevels::lookup, renderer.cpp:8533-8565. */
+static const alt_step_t *alt_lookup(const fo_reference *r, int start, fe_t dn, fe_t pn, int max_length) {
+    if (r->alt_levels == 0 || start < r->alt_start || max_length <= 0 || !fe_finite(dn) || !fe_finite(pn))
+        return NULL;
+    const int offset = start - r->alt_start;
+    if (offset < 0 || offset >= (int)r->alt_size[0]) return NULL;
+    int h = 31 - __builtin_clz((unsigned)max_length);
+    if (h > r->alt_levels - 1) h = r->alt_levels - 1;
+    for (int level = h; level >= 0; --level) {
+        if ((offset & ((1 << level) - 1)) != 0) continue;
+        const int index = offset >> level;
+        if (index >= (int)r->alt_size[level]) continue;
+        const alt_step_t *c = &r->alt[r->alt_offset[level] + (size_t)index];
+        if (c->length > 1 && fe_finite(c->r2) && fe_compare(dn, c->r2) < 0
+            && fe_compare(pn, r->alt_input_radius_squared) <= 0)
+            return c;
+    }
+    return NULL;
+}
+
+static sc_t alt_apply(const alt_step_t *s, sc_t delta, sc_t parameter) { /* renderer.cpp:11399-11416 */
+    const fe_t dr = sc_part(delta, 0), di = sc_part(delta, 1);
+    const fe_t pr = sc_part(parameter, 0), pi = sc_part(parameter, 1);
+    const fe_t rr = fe_add(fe_add(fe_mul(s->c[0], dr), fe_mul(s->c[1], di)),
+                           fe_add(fe_mul(s->c[4], pr), fe_mul(s->c[5], pi)));
+    const fe_t ri = fe_add(fe_add(fe_mul(s->c[2], dr), fe_mul(s->c[3], di)),
+                           fe_add(fe_mul(s->c[6], pr), fe_mul(s->c[7], pi)));
+    return sc_from_fe(rr, ri);
+}
+
+/* scaled_formula_step, renderer.cpp:11917-11931. */
+static sc_t alt_formula_step(int formula, sc_t v, sc_t parameter) {
+    if (formula == 2) {
+        const sc_t a = sc_make(fabs(v.re), fabs(v.im), v.e);
+        return sc_add(sc_mul(a, a), parameter);
+    }
+    const sc_t sq = formula == 3 ? sc_mul(sc_conj(v), sc_conj(v)) : sc_mul(v, v);
+    return sc_add(sq, parameter);
+}
+
+/* alternate_delta_step, renderer.cpp:11933-12034. */
+static sc_t alt_delta_step(const fo_reference *r, int i, sc_t delta, sc_t pdelta) {
+    const sc_t z = r->scaled[i];
+    if (r->formula == 3) {
+        const sc_t cz = sc_conj(z), cd = sc_conj(delta);
+        return sc_add(sc_double(sc_mul(cz, cd)), sc_add(sc_mul(cd, cd), pdelta));
+    }
+    if (r->formula == 2) {
+        const fe_t zr = sc_part(z, 0), zi = sc_part(z, 1);
+        const fe_t dr = sc_part(delta, 0), di = sc_part(delta, 1);
+        const fe_t azr = fe_abs(zr), azi = fe_abs(zi);
+        const fe_t tr = fe_add(zr, dr), ti = fe_add(zi, di);
+        const int crosses_real = (fe_compare(zr, FE_ZERO) >= 0) != (fe_compare(tr, FE_ZERO) >= 0);
+        const int crosses_imag = (fe_compare(zi, FE_ZERO) >= 0) != (fe_compare(ti, FE_ZERO) >= 0);
+        if (crosses_real || crosses_imag) {
+            const fe_t atr = fe_abs(tr), ati = fe_abs(ti);
+            const sc_t next = r->scaled[i + 1];
+            const fe_t por = fe_sub(sc_part(r->parameter, 0), sc_part(next, 0));
+            const fe_t poi = fe_sub(sc_part(r->parameter, 1), sc_part(next, 1));
+            const fe_t nr = fe_add(fe_sub(fe_sqr(atr), fe_sqr(ati)), fe_add(por, sc_part(pdelta, 0)));
+            const fe_t ni = fe_add(fe_muld(fe_mul(atr, ati), 2.0), fe_add(poi, sc_part(pdelta, 1)));
+            return sc_from_fe(nr, ni);
+        }
+        const double rs = zr.m < 0.0 ? -1.0 : 1.0, is = zi.m < 0.0 ? -1.0 : 1.0;
+        const fe_t rl = fe_sub(fe_muld(fe_mul(azr, dr), 2.0 * rs), fe_muld(fe_mul(azi, di), 2.0 * is));
+        const fe_t il = fe_add(fe_muld(fe_mul(azr, di), 2.0 * is), fe_muld(fe_mul(azi, dr), 2.0 * rs));
+        const fe_t sr = fe_sub(fe_sqr(dr), fe_sqr(di));
+        const fe_t si = fe_muld(fe_mul(dr, di), 2.0 * rs * is);
+        return sc_from_fe(fe_add(fe_add(rl, sr), sc_part(pdelta, 0)), fe_add(fe_add(il, si), sc_part(pdelta, 1)));
+    }
+    return sc_add(sc_double(sc_mul(z, delta)), sc_add(sc_mul(delta, delta), pdelta));
+}
+
+/* alternate_escape_margin via sc_escape_margin_with_reference_margin,
+ * renderer.cpp:8435-8451, 12036-12053. */
+static fe_t alt_margin(const fo_reference *r, int i, sc_t delta) {
+    const sc_t z = r->scaled[i];
+    const fe_t cross = fe_muld(fe_add(fe_mul(sc_part(z, 0), sc_part(delta, 0)),
+                                      fe_mul(sc_part(z, 1), sc_part(delta, 1))), 2.0);
+    const sn_t dn = sc_norm_squared(delta);
+    return fe_add(fe_add(r->margins[i], cross), fe_make(dn.m, dn.e));
+}
+
+typedef struct {
+    const fo_reference *r;
+    float *output;
+    int width, max_iter, mode, strict, cycle_enabled, series_available, bla_enabled, max_block;
+    double bias;
+    const fe_t *xo, *yo;
+    int budget_enabled;
+    struct timespec deadline;
+    atomic_int *budget_exceeded;
+} alt_ctx;
+
+static void alt_pixel(const alt_ctx *c, int px, int py) {
+    const fo_reference *r = c->r;
+    const size_t index = (size_t)py * (size_t)c->width + (size_t)px;
+    const int julia = r->formula == 1;
+    const int mode = c->mode;
+    const sc_t dc = sc_from_fe(c->xo[px], c->yo[py]);
+    const sc_t pdelta = julia ? SC_ZERO : dc;
+    const sn_t pnorm = sc_norm_squared(pdelta);
+    const fe_t pn = fe_make(pnorm.m, pnorm.e);
+    sc_t delta = dc;
+    int ref = julia ? 0 : 1, iteration = julia ? 0 : 1;
+    if (c->series_available) {
+        const sn_t dcn = sc_norm_squared(dc);
+        const sn_t rad = {r->series_radius_squared.m, r->series_radius_squared.e};
+        if (sc_compare_norm(dcn, rad) <= 0) {
+            sc_t unused;
+            delta = evaluate_series(r, dc, &unused, 0);
+            ref = r->series_iteration;
+            iteration = r->series_iteration;
+        }
+    }
+    unsigned ticks = 0;
+    sc_t total = sc_add(r->scaled[ref], delta);
+    sn_t total_norm = sc_norm_with_margin(alt_margin(r, ref, delta), mode);
+    (void)total;
+    if (sc_outside_escape(total_norm, mode)) {
+        c->output[index] = smooth_escape_scaled(iteration, total_norm, c->bias);
+        return;
+    }
+    sc_t tortoise = SC_ZERO;
+    int cycle_power = 1, cycle_length = 0, cycle_ready = 0, cycle_hits = 0;
+    while (iteration < c->max_iter) {
+        if (c->budget_enabled && (ticks++ & 255U) == 0U) {
+            int expired = atomic_load(c->budget_exceeded);
+            if (!expired) {
+                struct timespec now;
+                clock_gettime(CLOCK_MONOTONIC, &now);
+                if (now.tv_sec > c->deadline.tv_sec
+                    || (now.tv_sec == c->deadline.tv_sec && now.tv_nsec >= c->deadline.tv_nsec)) {
+                    atomic_store(c->budget_exceeded, 1);
+                    expired = 1;
+                }
+            }
+            if (expired) { c->output[index] = NAN; return; }
+        }
+        const int next = ref + 1;
+        if (next >= r->orbit_size) {
+            const sc_t parameter = julia ? r->parameter : sc_add(r->parameter, dc);
+            total = alt_formula_step(r->formula, total, parameter);
+            ++iteration;
+            total_norm = sc_norm_squared(total);
+            if (sc_outside_escape(total_norm, mode)) {
+                c->output[index] = smooth_escape_scaled(iteration, total_norm, c->bias);
+                return;
+            }
+            continue;
+        }
+        const sn_t dn = sc_norm_squared(delta);
+        const int remaining = c->max_iter - iteration;
+        const alt_step_t *step = c->bla_enabled
+            ? alt_lookup(r, ref, fe_make(dn.m, dn.e), pn, c->max_block < remaining ? c->max_block : remaining)
+            : NULL;
+        if (step && ref + step->length < r->orbit_size) {
+            const int end = ref + step->length;
+            const sc_t cand = alt_apply(step, delta, pdelta);
+            const fe_t cm = alt_margin(r, end, cand);
+            const sn_t cn = sc_norm_with_margin(cm, mode);
+            const sc_t ct = sc_add(r->scaled[end], cand);
+            const sn_t limit = {0.75, 2};
+            const int bad = !sc_finite(cand) || !fe_finite(cm) || !sc_finite(ct) || fe_compare(cm, FE_ZERO) > 0
+                || sc_compare_norm(cn, limit) >= 0;
+            if (!bad) {
+                delta = cand;
+                ref = end;
+                iteration += step->length;
+                total = ct;
+                total_norm = cn;
+                continue;
+            }
+            for (int replay = 0; replay < step->length && iteration < c->max_iter; ++replay) {
+                delta = alt_delta_step(r, ref, delta, pdelta);
+                ++ref;
+                ++iteration;
+                if (!sc_finite(delta)) {
+                    c->output[index] = c->strict ? NAN : encode_iteration(iteration, c->bias);
+                    return;
+                }
+                const fe_t m = alt_margin(r, ref, delta);
+                total_norm = sc_norm_with_margin(m, mode);
+                total = sc_add(r->scaled[ref], delta);
+                if (fe_compare(m, FE_ZERO) > 0 || sc_outside_escape(total_norm, mode)) {
+                    c->output[index] = smooth_escape_scaled(iteration, total_norm, c->bias);
+                    return;
+                }
+            }
+            continue;
+        }
+        delta = alt_delta_step(r, ref, delta, pdelta);
+        ref = next;
+        ++iteration;
+        if (!sc_finite(delta)) {
+            c->output[index] = c->strict ? NAN : encode_iteration(iteration, c->bias);
+            return;
+        }
+        const fe_t m = alt_margin(r, ref, delta);
+        total_norm = sc_norm_with_margin(m, mode);
+        total = sc_add(r->scaled[ref], delta);
+        if (fe_compare(m, FE_ZERO) > 0 || sc_outside_escape(total_norm, mode)) {
+            c->output[index] = smooth_escape_scaled(iteration, total_norm, c->bias);
+            return;
+        }
+        const sn_t limit = {0.75, 2};
+        if (c->cycle_enabled && iteration >= 2048 && (iteration & 31) == 0
+            && sc_compare_norm(total_norm, limit) < 0) {
+            if (!cycle_ready) {
+                tortoise = total; cycle_power = 1; cycle_length = 0; cycle_hits = 0; cycle_ready = 1;
+            } else {
+                const sn_t distance = sc_norm_squared(sc_sub(total, tortoise));
+                const int scale = total_norm.e > 0 ? total_norm.e : 0;
+                if (distance.m == 0.0 || distance.e <= scale - 78) {
+                    if (++cycle_hits >= 3) {
+                        c->output[index] = encode_iteration(c->max_iter, c->bias);
+                        return;
+                    }
+                } else {
+                    cycle_hits = 0;
+                }
+                if (++cycle_length >= cycle_power) {
+                    tortoise = total;
+                    cycle_power = cycle_power * 2 < (1 << 20) ? cycle_power * 2 : (1 << 20);
+                    cycle_length = 0;
+                }
+            }
+        }
+    }
+    c->output[index] = encode_iteration(c->max_iter, c->bias);
+}
+
+/* render_alternate_reference_impl without planes, renderer.cpp:12064-12448. */
+static int render_alternate(float *output, int width, int height, fe_t zoom, const fo_reference *r, int max_iter,
+                            const FractalRenderOptions *o, uint64_t *stats, int threads, struct timespec started) {
+    const double factor = viewport_height_factor(o->coordinate_mode);
+    const fe_t view_height = fe_muld(fe_div(fe_from_parts(1.0, 0), zoom), factor);
+    const fe_t view_width = fe_muld(view_height, (double)width / (double)height);
+    fe_t *xo = (fe_t *)malloc(sizeof(fe_t) * (size_t)width);
+    fe_t *yo = (fe_t *)malloc(sizeof(fe_t) * (size_t)height);
+    if (!xo || !yo) { free(xo); free(yo); return 1; }
+    for (int px = 0; px < width; ++px)
+        xo[px] = fe_muld(view_width, pixel_axis_offset(px, width, o->coordinate_mode) / (double)width);
+    for (int py = 0; py < height; ++py)
+        yo[py] = fe_muld(view_height, -pixel_axis_offset(py, height, o->coordinate_mode) / (double)height);
+    alt_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.r = r; c.output = output; c.width = width; c.max_iter = max_iter; c.mode = o->escape_radius_mode;
+    c.strict = o->strict != 0; c.bias = o->output_bias; c.xo = xo; c.yo = yo;
+    c.cycle_enabled = o->disable_cycle == 0 && (o->strict == 0 || o->strict_cycle != 0);
+    c.series_available = r->formula == 1 && r->series_enabled && r->series_iteration < max_iter;
+    c.bla_enabled = o->disable_bla == 0 && r->alt_levels > 0;
+    int mb = o->max_linear_bla_length;
+    c.max_block = mb < 2 ? 2 : (mb > 4096 ? 4096 : mb);
+    atomic_int exceeded;
+    atomic_init(&exceeded, 0);
+    c.budget_exceeded = &exceeded;
+    if (o->time_budget_ms > 0) {
+        c.budget_enabled = 1;
+        clock_gettime(CLOCK_MONOTONIC, &c.deadline);
+        long long ns = (long long)c.deadline.tv_nsec + (long long)o->time_budget_ms * 1000000LL;
+        c.deadline.tv_sec += (time_t)(ns / 1000000000LL);
+        c.deadline.tv_nsec = (long)(ns % 1000000000LL);
+    }
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256)
+#endif
+    for (long long lp = 0; lp < (long long)width * height; ++lp) {
+        const int py = (int)(lp / width);
+        alt_pixel(&c, (int)(lp - (long long)py * width), py);
+    }
+    if (stats) {
+        struct timespec ended;
+        clock_gettime(CLOCK_MONOTONIC, &ended);
+        memset(stats, 0, sizeof(uint64_t) * FO_STATS_FIELDS);
+        stats[0] = (uint64_t)width * (uint64_t)height;
+        stats[ST_RENDER_NS] = (uint64_t)((ended.tv_sec - started.tv_sec) * 1000000000LL
+                                         + (ended.tv_nsec - started.tv_nsec));
+    }
+    free(xo); free(yo);
     return 0;
 }

@@ -40,6 +40,14 @@ fo_reference *fo_reference_create(
     int escape_radius_mode, int coordinate_mode,
     char *error, int error_capacity);
 
+/* fractal_create_reference_ex_options with a formula (0 Mandelbrot, 1 Julia,
+ * 2 burning ship, 3 tricorn; renderer.cpp:13650-13700).  julia_re/julia_im
+ * are the Julia constant's decimal text (ignored otherwise). */
+fo_reference *fo_reference_create_formula(
+    const char *x, const char *y, const char *zoom, int max_iter, int precision_bits,
+    int series_order, int escape_radius_mode, int coordinate_mode, int formula,
+    const char *julia_re, const char *julia_im, char *error, int error_capacity);
+
 fo_reference *fo_reference_clone(
     const fo_reference *source, const char *viewport_zoom,
     char *error, int error_capacity);

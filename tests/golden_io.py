@@ -150,3 +150,49 @@ def assert_direct_parity(golden, field, planes, exact_field=False):
         np.testing.assert_array_equal(planes.test2, golden["test2"])
         for name in ("phase", "de_x", "de_y"):
             np.testing.assert_allclose(getattr(planes, name), golden[name], rtol=1e-9, atol=1e-300, err_msg=name)
+
+
+def alt_case_names():
+    from alt_cases import CASES as ALT_CASES
+    return [c["name"] for c in ALT_CASES]
+
+
+def render_alt_abi(lib, case, backend=2, stats=True):
+    """An alternate-formula case through a renderer.h-speaking library."""
+    m = case
+    opts = native.RenderOptions.defaults(backend=backend, **m["options"])
+    ref = native.AbiReference(lib, m["x"], m["y"], m["ref_zoom"], m["max_iter"], m["bits"], 8, opts,
+                              formula=m["formula"], julia=tuple(m["julia"]))
+    if m["clone_zoom"]:
+        clone = lib.fractal_clone_reference(ctypes.c_void_p(ref.handle), m["clone_zoom"].encode())
+        assert clone, lib.fractal_last_error()
+        ref = native.AbiReference.__new__(native.AbiReference)
+        ref.lib, ref.handle = lib, clone
+    field, _, st = ref.render(m["width"], m["height"], m["zoom"], m["max_iter"], 8, 32, opts, stats=stats)
+    return field, st
+
+
+def render_alt_oracle(lib, case):
+    m = case
+    o = m["options"]
+    ref = native.OracleReference(lib, m["x"], m["y"], m["ref_zoom"], m["max_iter"], m["bits"], 8,
+                                 o.get("escape_radius_mode", 0), o.get("coordinate_mode", 0),
+                                 formula=m["formula"], julia=tuple(m["julia"]))
+    if m["clone_zoom"]:
+        ref = ref.clone(m["clone_zoom"])
+    opts = native.RenderOptions.defaults(**o)
+    field, _, st = ref.render(m["width"], m["height"], m["zoom"], m["max_iter"], 8, 32, opts, stats=True)
+    return field, st
+
+
+def assert_alt_parity(golden, field):
+    """Alternate-formula bar: NaN mask and interior set bit-exact, integer
+    escape index bit-exact, float field within one ulp."""
+    f = np.asarray(field, dtype=np.float32).reshape(-1)
+    g = golden["field"].reshape(-1)
+    np.testing.assert_array_equal(np.isnan(f), np.isnan(g))
+    ok = ~np.isnan(g)
+    ulps = np.abs(f[ok].view(np.int32).astype(np.int64) - g[ok].view(np.int32).astype(np.int64))
+    assert ulps.size == 0 or int(ulps.max()) <= 1, f"field differs by {int(ulps.max())} ulp"
+    bias = golden["meta"]["options"].get("output_bias", 0.0)
+    np.testing.assert_array_equal(np.floor(f[ok].astype(np.float64) + bias), np.floor(g[ok].astype(np.float64) + bias))

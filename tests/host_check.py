@@ -85,3 +85,28 @@ def render_direct(lib, case):
     if status != 0:
         raise RuntimeError(err.value.decode())
     return out.reshape(h, w), planes
+
+
+def bind_alt(lib):
+    lib.fav_host_check_alt.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(native.RenderOptions), native.c_float_p,
+        ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+    lib.fav_host_check_alt.restype = ctypes.c_int
+    return lib
+
+
+def render_alt(lib, case):
+    """The alternate kernel's per-pixel code run on the CPU."""
+    m = case
+    w, h = m["width"], m["height"]
+    out = np.full(w * h, -7.0, dtype=np.float32)
+    opts = native.RenderOptions.defaults(backend=2, **m["options"])
+    err = ctypes.create_string_buffer(512)
+    status = lib.fav_host_check_alt(
+        m["x"].encode(), m["y"].encode(), m["ref_zoom"].encode(), m["max_iter"], m["bits"], m["formula"],
+        m["julia"][0].encode(), m["julia"][1].encode(), m["clone_zoom"].encode() if m["clone_zoom"] else None,
+        ctypes.byref(opts), out.ctypes.data_as(native.c_float_p), w, h, m["zoom"].encode(), err, 512)
+    if status != 0:
+        raise RuntimeError(err.value.decode())
+    return out.reshape(h, w)

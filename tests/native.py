@@ -258,13 +258,18 @@ class OracleReference:
     """A fo_reference owned by Python."""
 
     def __init__(self, lib, x, y, zoom, max_iter, precision_bits, series_order=8,
-                 escape_radius_mode=0, coordinate_mode=0, handle=None):
+                 escape_radius_mode=0, coordinate_mode=0, handle=None, formula=0, julia=("0", "0")):
         self.lib = lib
         if handle is None:
             err = ctypes.create_string_buffer(512)
-            handle = lib.fo_reference_create(_b(x), _b(y), _b(zoom), max_iter, precision_bits,
-                                              series_order, escape_radius_mode, coordinate_mode,
-                                              err, 512)
+            lib.fo_reference_create_formula.argtypes = [
+                ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                ctypes.c_int]
+            lib.fo_reference_create_formula.restype = ctypes.c_void_p
+            handle = lib.fo_reference_create_formula(_b(x), _b(y), _b(zoom), max_iter, precision_bits,
+                                                      series_order, escape_radius_mode, coordinate_mode,
+                                                      formula, _b(julia[0]), _b(julia[1]), err, 512)
             if not handle:
                 raise RuntimeError(err.value.decode())
         self.handle = handle
@@ -309,12 +314,12 @@ class AbiReference:
     """A reference handle created through a renderer.h-speaking library."""
 
     def __init__(self, lib, x, y, zoom, max_iter, precision_bits, series_order=8,
-                 options=None, formula=0):
+                 options=None, formula=0, julia=("0", "0")):
         self.lib = lib
         opts = options if options is not None else RenderOptions.defaults()
         handle = lib.fractal_create_reference_ex_options(
             _b(x), _b(y), _b(zoom), max_iter, precision_bits, series_order, formula,
-            b"0", b"0", ctypes.byref(opts))
+            _b(julia[0]), _b(julia[1]), ctypes.byref(opts))
         if not handle:
             raise RuntimeError(lib.fractal_last_error().decode())
         self.handle = handle

@@ -121,8 +121,8 @@ def test_invalid_reference_arguments(product):
     assert not product.fractal_create_reference(b"not-a-number", Y, b"1e12", 100, 128, 8)
     assert b"invalid real coordinate" in product.fractal_last_error()
     assert not product.fractal_create_reference(X, Y, b"-5", 100, 128, 8)
-    assert not product.fractal_create_reference_ex(X, Y, b"1e12", 100, 128, 8, 1, b"0", b"0")
-    assert b"Mandelbrot" in product.fractal_last_error()
+    assert not product.fractal_create_reference_ex(X, Y, b"1e12", 100, 128, 8, 4, b"0", b"0")
+    assert not product.fractal_create_reference_ex(X, Y, b"1e12", 100, 128, 8, -1, b"0", b"0")
 
 
 def test_clone_requires_reusable_root(product):
@@ -227,3 +227,31 @@ def test_direct_planes_need_mandelbrot(product):
                                               0, 0, 2, 0.0, 0.0, ctypes.byref(opts), ctypes.byref(planes.struct))
     assert status == 1
     assert b"scalar backend for this formula" in product.fractal_last_error()
+
+
+def test_alternate_reference_creation_on_host(product):
+    """Alternate handles build their MPFR orbit and real 2x2 BLA on the host
+    (no device needed) and reject plane renders on the GPU backend."""
+    opts = native.RenderOptions.defaults(backend=2)
+    for formula, julia in ((1, (b"-0.8", b"0.156")), (2, (b"0", b"0")), (3, (b"0", b"0"))):
+        handle = product.fractal_create_reference_ex_options(b"-0.99", b"0.05", b"1e7", 500, 128, 8, formula,
+                                                             julia[0], julia[1], ctypes.byref(opts))
+        assert handle, product.fractal_last_error()
+        planes = native.PlaneArrays(4, 4)
+        out = np.zeros(16, dtype=np.float32)
+        status = product.fractal_render_reference_ex_planes(out.ctypes.data_as(native.c_float_p), 4, 4, b"1e7",
+                                                            ctypes.c_void_p(handle), 500, 0, 8, 32,
+                                                            ctypes.byref(opts), ctypes.byref(planes.struct))
+        assert status == 1
+        assert b"scalar backend for this formula" in product.fractal_last_error()
+        clone = product.fractal_clone_reference(ctypes.c_void_p(handle), b"1e9")
+        assert clone, product.fractal_last_error()
+        product.fractal_destroy_reference(ctypes.c_void_p(clone))
+        product.fractal_destroy_reference(ctypes.c_void_p(handle))
+
+
+def test_alternate_rejects_bad_julia_constant(product):
+    opts = native.RenderOptions.defaults(backend=2)
+    handle = product.fractal_create_reference_ex_options(b"0", b"0", b"1e7", 500, 128, 8, 1, b"x", b"0",
+                                                         ctypes.byref(opts))
+    assert not handle

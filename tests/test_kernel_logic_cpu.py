@@ -47,3 +47,18 @@ def test_direct_kernel_code_on_cpu_matches_golden(direct_harness, name):
     g = golden_io.load_direct(name)
     field, planes = host_check.render_direct(direct_harness, g["meta"])
     golden_io.assert_direct_parity(g, field, planes, exact_field=True)
+
+
+@pytest.fixture(scope="module")
+def alt_harness(harness):
+    return host_check.bind_alt(harness)
+
+
+@pytest.mark.parametrize("name", golden_io.alt_case_names())
+def test_alt_kernel_code_on_cpu_matches_golden(alt_harness, name):
+    """alt_core.h and the product's alternate tables on the CPU: escape
+    counts bit-exact, float field within one ulp (double vs long double
+    final logarithm)."""
+    g = golden_io.load(name)
+    field = host_check.render_alt(alt_harness, g["meta"])
+    golden_io.assert_alt_parity(g, field)
