@@ -32,6 +32,19 @@ import fav_palette  # noqa: E402
 import fav_workloads  # noqa: E402
 
 MI355X_FP64_VECTOR_PEAK_TFLOPS = 78.6
+TRAFFIC_PROFILE = ROOT / "profiles" / "field_kernel_1920x1080_zoom1e100_traffic.md"
+
+
+def measured_traffic_bytes():
+    """HBM bytes per field launch from the committed FETCH_SIZE/WRITE_SIZE
+    profile (copy-calibrated), or None when that summary is absent."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def escape_count_sum(orbit_iteration, field, cap):
@@ -43,13 +56,14 @@ def escape_count_sum(orbit_iteration, field, cap):
 
 def cpu_baseline(work, threads):
     """The reference's renderer.cpp (oracle/_ref, built from source) on the
-    host cores, over a 1/16-area sample of the same workload."""
+    host cores, over a quarter-area sample (half width, half height) of the
+    same workload."""
     sys.path.insert(0, str(ROOT / "tests"))
     import native  # checker-side ctypes mirrors
 
     lib = native.load_reference()
     kind = "reference"
-    sample_w, sample_h = work["width"] // 4, work["height"] // 4
+    sample_w, sample_h = work["width"] // 2, work["height"] // 2
     opts = native.RenderOptions.defaults(backend=0, **work["options"])
     if lib is not None:
         ref = native.AbiReference(lib, work["x"], work["y"], work["zoom"], work["max_iter"],
@@ -68,8 +82,26 @@ def cpu_baseline(work, threads):
     seconds = time.perf_counter() - started
     total = escape_count_sum(planes.orbit_iteration, field.reshape(-1), work["max_iter"])
     return {"value": total / seconds / 1e9, "unit": "Gpixel-iter/s", "cores": threads, "kind": kind,
-            "sample": f"{sample_w}x{sample_h} of the same field (1/16 of the pixels), "
+            "sample": f"{sample_w}x{sample_h} of the same field (1/4 of the pixels), "
                       f"{seconds:.1f} s on {threads} threads ({platform.processor() or platform.machine()})"}
+
+
+def copy_calibration(mib, copies):
+    """Plain device-to-device copies of a known size: the reference point
+    for the FETCH_SIZE / WRITE_SIZE counters (each copy reads and writes
+    `mib` MiB)."""
+    hip = fav_hip.HipRuntime()
+    nbytes = mib << 20
+    src, dst = hip.malloc(nbytes), hip.malloc(nbytes)
+    for _ in range(copies):
+        status = hip.lib.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)
+        if status != 0:
+            raise RuntimeError(f"hipMemcpy D2D failed with {status}")
+    hip.synchronize()
+    hip.free(src)
+    hip.free(dst)
+    print(json.dumps({"copy_calibration_mib": mib, "copies": copies}))
+    return 0
 
 
 def main():
@@ -78,7 +110,11 @@ def main():
     parser.add_argument("--steps", type=int, default=10)
     parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--copy-calibration-mib", type=int, default=0,
+                        help="only run N device-to-device copies of this many MiB (HBM counter calibration)")
     args = parser.parse_args()
+    if args.copy_calibration_mib:
+        return copy_calibration(args.copy_calibration_mib, max(1, args.steps))
     if args.gpus != 1:
         print("bench.py measures one GPU: the field path has no cross-GPU exchange in this build "
               "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
@@ -186,7 +222,8 @@ def main():
                    "nan_pixels": int(np.isnan(field).sum())},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": MI355X_FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / MI355X_FP64_VECTOR_PEAK_TFLOPS, 4), "traffic": None},
+                     "frac": round(achieved / MI355X_FP64_VECTOR_PEAK_TFLOPS, 4),
+                     "traffic": measured_traffic_bytes()},
         "cpu_baseline": baseline,
     }
     print(json.dumps(result))
