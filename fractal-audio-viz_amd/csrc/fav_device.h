@@ -94,6 +94,10 @@ struct StateStore {
 // Bytes of StateStore storage per pixel (9 doubles, 9 ints, 2 queue slots).
 constexpr int32_t kOrbitGlitchFlag = 1;
 constexpr int32_t kOrbitEscapeFlag = 2;
+constexpr int32_t kOrbitEndpointFlag = 4;  // |Z_n|^2 >= 3: a BLA block may not end here
+// LinearStepDev::pad[0] bits, set when the tables are flattened
+constexpr int32_t kStepUsable = 1;           // length > 1 and 0 < r2 < inf
+constexpr int32_t kStepNormalCoefficients = 2;  // A and B are normal ScaledComplex values
 constexpr size_t kStateBytesPerPixel = 9 * 8 + 9 * 4 + 2 * 4;
 
 // One alternate-formula linear BLA step (AlternateLinearBlaStep,

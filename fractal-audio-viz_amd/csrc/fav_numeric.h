@@ -51,6 +51,17 @@ FAV_HD double frexp_(double v, int* e) {
 #endif
 }
 
+// frexp's exponent of a finite value (0 for zero; no infinity/NaN handling).
+FAV_HD int frexp_exp_nz(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_frexp_exp(v);
+#else
+    int e = 0;
+    (void)std::frexp(v, &e);
+    return e;
+#endif
+}
+
 FAV_HD double ldexp_(double v, int e) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_ldexp(v, e);
@@ -321,6 +332,59 @@ FAV_HD SC sc_double(SC v) {
 }
 
 FAV_HD bool sc_finite(SC v) { return isfinite_(v.re) && isfinite_(v.im); }
+
+// ---- operations on normalised nonzero operands ---------------------------
+// A ScaledComplex is "normal" when its larger part has magnitude in
+// [0.5, 1) and |e| <= 2^28: what normalize() leaves for every finite
+// nonzero value in the range the field path reaches.  For normal operands
+// the special cases of sc_mul / sc_add / sc_norm_squared cannot fire:
+//  * a product of two normal values has magnitude in [0.25, 2), so its
+//    larger part is in [0.17, 2) (each part is rounded once, so cancellation
+//    in one part leaves the other near |a||b|) -- never zero or infinite;
+//  * a sum of two normal values is below 2 in magnitude but may cancel to
+//    exactly zero, which normalize() maps to exponent 0;
+//  * |v|^2 of a normal value lies in [0.25, 2);
+//  * exponent sums stay far from the int limits, so no saturation.
+// The *_nn forms below compute the same roundings in the same order with
+// those cases removed, so for normal operands they equal sc_mul, sc_add and
+// sc_norm_squared bit for bit.
+FAV_HD bool sc_normal(SC v) {
+    const double ar = fabs_(v.re), ai = fabs_(v.im);
+    const double magnitude = (ar < ai) ? ai : ar;
+    return magnitude >= 0.5 && magnitude < 1.0 && v.e <= (1 << 28) && v.e >= -(1 << 28);
+}
+
+FAV_HD SC sc_mul_nn(SC a, SC b) {
+    const double re = a.re * b.re - a.im * b.im;
+    const double im = a.re * b.im + a.im * b.re;
+    const double ar = fabs_(re), ai = fabs_(im);
+    const int shift = frexp_exp_nz((ar < ai) ? ai : ar);
+    return SC{ldexp_(re, -shift), ldexp_(im, -shift), a.e + b.e + shift};
+}
+
+FAV_HD SC sc_add_nn(SC a, SC b) {
+    const bool swap = b.e > a.e;
+    const SC larger = swap ? b : a;
+    const SC smaller = swap ? a : b;
+    const int diff = larger.e - smaller.e;
+    const int d = diff < 64 ? diff : 64;
+    const double re = larger.re + ldexp_(smaller.re, -d);
+    const double im = larger.im + ldexp_(smaller.im, -d);
+    const double ar = fabs_(re), ai = fabs_(im);
+    const double magnitude = (ar < ai) ? ai : ar;
+    const int shift = frexp_exp_nz(magnitude);  // 0 for an exact cancellation
+    SC r{ldexp_(re, -shift), ldexp_(im, -shift), magnitude == 0.0 ? 0 : larger.e + shift};
+    if (diff > 60) r = larger;
+    return r;
+}
+
+FAV_HD SN sc_norm_nn(SC v) {
+    const double norm = v.re * v.re + v.im * v.im;
+    const int e2 = 2 * v.e;
+    if (norm >= 1.0) return SN{norm * 0.5, e2 + 1};
+    if (norm < 0.5) return SN{norm * 2.0, e2 - 1};
+    return SN{norm, e2};
+}
 
 // sc_norm_squared, renderer.cpp:7743-7774.
 FAV_HD SN sc_norm_squared(SC v) {

@@ -553,6 +553,91 @@ FAV_HD PixelState init_pixel(const RenderParams& p, size_t index, SC dc, Counter
 
 enum Outcome : int { kDone = 0, kHandoff = 1, kTail = 2 };
 
+// lookup_level for a normal |delta|^2 (positive, finite): the step flags
+// carry length > 1 and 0 < r2 < inf, so the comparison is exponent-first.
+template <typename Step>
+FAV_HD int lookup_level_normal(const Step* steps, const LevelDir& d, int start, SN dn, int max_length) {
+    if (start <= 0 || max_length <= 0) return -1;
+    const int base = d.count ? d.size[0] : 0;
+    if (start > base) return -1;
+    int h = highest_level(max_length);
+    if (h > d.count - 1) h = d.count - 1;
+    const int offset = start - 1;
+    for (int level = h; level >= 0; --level) {
+        if ((offset & ((1 << level) - 1)) != 0) continue;
+        const int index = offset >> level;
+        if (index >= d.size[level]) continue;
+        const int64_t k = d.offset[level] + index;
+        const Step& s = steps[k];
+        if ((s.pad[0] & kStepUsable) != 0 && (dn.e != s.r2_e ? dn.e < s.r2_e : dn.m < s.r2_m))
+            return static_cast<int>(k);
+    }
+    return -1;
+}
+
+// The common iterations of run_linear below, with the same results: while
+// the loop top is in the negligible-perturbation regime (Z + delta, the
+// margin and the norm are the orbit's own values, so the glitch / escape /
+// endpoint tests are the orbit's precomputed flags) and every operand of the
+// linear block is a normal ScaledComplex (fav_numeric.h), the block is formed
+// with the *_nn operations.  Any other case returns to run_linear's general
+// loop top with the state unchanged since the last committed block.
+template <bool Planes, bool Stats>
+FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Counters<Stats>& st) {
+    const OrbitDev& o = p.orbit;
+    const int max_iter = p.max_iter;
+    while (s.have_total && s.iteration < max_iter) {
+        const int ref = s.ref;
+        if (ref < 0 || ref >= o.size) return;
+        if (!sc_normal(s.delta)) return;
+        if constexpr (Planes) {
+            if (!sc_normal(s.derivative)) return;
+        }
+        if (s.delta.e > o.small_exp[ref]) return;
+        if ((o.flags[ref] & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) return;
+        const SN dn = sc_norm_nn(s.delta);
+        if (sc_compare_norm(s.total_norm, dn) < 0) return;  // rebase
+        if (o.map_end > 1 && ref >= o.map_end) return;
+        const int order = p.approximation_order >= 3 && dn.e < -115 ? (dn.e < -160 ? 1 : 2) : p.approximation_order;
+        if (order > 1) return;
+        const int remaining = max_iter - s.iteration;
+        const int k = lookup_level_normal(p.linear, p.linear_dir, ref, dn,
+                                          p.max_linear_length < remaining ? p.max_linear_length : remaining);
+        if (k < 0) return;
+        const LinearStepDev& step = p.linear[k];
+        const int length = step.length;
+        if (ref + length > max_iter || (step.pad[0] & kStepNormalCoefficients) == 0) return;
+        const SC a{step.a_re, step.a_im, step.a_e};
+        const SC b{step.b_re, step.b_im, step.b_e};
+        const SC delta = sc_add_nn(sc_mul_nn(a, s.delta), sc_mul_nn(b, dc));
+        SC derivative = s.derivative;
+        if constexpr (Planes) derivative = sc_add_nn(sc_mul_nn(a, s.derivative), b);
+        const int end = ref + length;
+        if (sc_is_zero(delta) || delta.e > o.small_exp[end]) return;
+        if ((o.flags[end] & kOrbitEndpointFlag) != 0) return;
+        // commit, renderer.cpp:13499-13511
+        const SN end_norm{o.zn_m[end], o.zn_e[end]};
+        s.derivative = derivative;
+        s.delta = delta;
+        s.ref = end;
+        s.iteration += length;
+        s.total = load_z(o, end);
+        s.total_norm = end_norm;
+        s.margin = FE{o.margin_m[end], o.margin_e[end]};
+        s.previous_total_norm = end_norm;
+        st.add(ST_BLA_BLOCKS);
+        st.add(ST_LINEAR_BLOCKS);
+        st.add(ST_SERIES_JUMPS);
+        if constexpr (Stats) {
+            int bucket = 0;
+            unsigned v = static_cast<unsigned>(length);
+            while (v > 1U && bucket < 15) { v >>= 1U; ++bucket; }
+            st.add(ST_HISTOGRAM + bucket);
+        }
+        st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
+    }
+}
+
 // Fast path: the loop of renderer.cpp:12889-13536 restricted to iterations
 // that end in a committed linear BLA block or in an exact perturbation step.
 // Any other branch (glitch, escape, rebase, reference escape, cubic order,
@@ -565,7 +650,12 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
     const int max_iter = p.max_iter;
     const int mode = p.escape_radius_mode;
     if (!p.fast_linear) return kHandoff;
+    const bool normal_blocks = sc_normal(dc) && !p.disable_bla && !s.pixel_disable_bla;
     while (s.iteration < max_iter) {
+        if (normal_blocks) {
+            run_normal_blocks<Planes, Stats>(p, s, dc, st);
+            if (s.iteration >= max_iter) break;
+        }
         if (s.ref < 0 || s.ref >= o.size) return kHandoff;
         if (!s.have_total) loop_top_values(o, s.ref, s.delta, mode, s.total, s.margin, s.total_norm);
         s.have_total = false;

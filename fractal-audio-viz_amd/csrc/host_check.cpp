@@ -297,3 +297,53 @@ extern "C" int fav_host_check_alt(const char* x, const char* y, const char* ref_
         return 1;
     }
 }
+
+#include <random>
+
+namespace {
+
+// A random normal ScaledComplex: larger part in [0.5, 1), the other part
+// anywhere from exact zero through tiny to comparable, either sign.
+fav::SC random_normal(std::mt19937_64& rng, int exponent_span) {
+    std::uniform_real_distribution<double> unit(0.5, 1.0);
+    std::uniform_int_distribution<int> pick(0, 7), expo(-exponent_span, exponent_span), small(-1100, 0);
+    const double big = unit(rng) * (pick(rng) & 1 ? -1.0 : 1.0);
+    double other = 0.0;
+    switch (pick(rng)) {
+        case 0: other = 0.0; break;
+        case 1: other = -0.0; break;
+        case 2: other = std::ldexp(unit(rng), small(rng)); break;
+        case 3: other = big; break;
+        case 4: other = -big; break;
+        default: other = unit(rng) * 2.0 - 1.0; break;
+    }
+    const bool swap = pick(rng) & 1;
+    return fav::SC{swap ? other : big, swap ? big : other, expo(rng)};
+}
+
+bool same_sc(fav::SC a, fav::SC b) {
+    return std::memcmp(&a.re, &b.re, 8) == 0 && std::memcmp(&a.im, &b.im, 8) == 0 && a.e == b.e;
+}
+
+}  // namespace
+
+// Test-only: the *_nn operations against the general ones on `count` random
+// normal operand pairs (including exact cancellations and exponent gaps
+// around the 60-binade cut).  Writes mismatch counts for mul, add and norm.
+extern "C" void fav_host_check_nn_ops(long long count, unsigned long long seed, long long* mismatches) {
+    using namespace fav;
+    std::mt19937_64 rng(seed);
+    mismatches[0] = mismatches[1] = mismatches[2] = 0;
+    std::uniform_int_distribution<int> gap(-70, 70), pick(0, 9);
+    for (long long i = 0; i < count; ++i) {
+        const SC a = random_normal(rng, 1 << 20);
+        SC b = random_normal(rng, 1 << 20);
+        if (pick(rng) < 6) b.e = a.e + gap(rng);  // exercise the alignment and the >60 cut
+        if (pick(rng) == 0) b = SC{-a.re, -a.im, a.e};  // exact cancellation
+        if (!sc_normal(a) || !sc_normal(b)) continue;
+        if (!same_sc(sc_mul_nn(a, b), sc_mul(a, b))) ++mismatches[0];
+        if (!same_sc(sc_add_nn(a, b), sc_add(a, b))) ++mismatches[1];
+        const SN x = sc_norm_nn(a), y = sc_norm_squared(a);
+        if (std::memcmp(&x.m, &y.m, 8) != 0 || x.e != y.e) ++mismatches[2];
+    }
+}

@@ -307,6 +307,7 @@ void make_orbit(ReferenceContext& c, const char* x, const char* y, const char* z
         int32_t f = 0;
         if (threshold.m != 0.0 && fe_compare(zn, threshold) < 0) f |= kOrbitGlitchFlag;
         if (sc_outside_escape(SN{zn.m, zn.e}, mode) || fe_compare(margin, fe_zero()) > 0) f |= kOrbitEscapeFlag;
+        if (sc_compare_norm(SN{zn.m, zn.e}, SN{0.75, 2}) >= 0) f |= kOrbitEndpointFlag;
         orbit->flags[i] = f;
     });
     if (formula != 0) {

@@ -3,6 +3,7 @@
 #include "table_image.h"
 
 #include <algorithm>
+#include <cmath>
 #include <stdexcept>
 
 namespace fav {
@@ -37,7 +38,15 @@ void make_table_image(const ReferenceContext& c, TableImage& img) {
         img.e[i] = o.scaled[i].e;
     }
     flatten(c.cubic.levels, img.cubic, img.cubic_dir);
-    for (int t = 0; t < 2; ++t) flatten(c.linear[t].levels, img.linear[t], img.linear_dir[t]);
+    for (int t = 0; t < 2; ++t) {
+        flatten(c.linear[t].levels, img.linear[t], img.linear_dir[t]);
+        for (LinearStepDev& s : img.linear[t]) {  // lookup and fast-path flags
+            int32_t f = 0;
+            if (s.length > 1 && std::isfinite(s.r2_m) && s.r2_m != 0.0) f |= kStepUsable;
+            if (sc_normal(SC{s.a_re, s.a_im, s.a_e}) && sc_normal(SC{s.b_re, s.b_im, s.b_e})) f |= kStepNormalCoefficients;
+            s.pad[0] = f;
+        }
+    }
     img.series_re.clear();
     img.series_im.clear();
     img.series_e.clear();

@@ -62,3 +62,15 @@ def test_alt_kernel_code_on_cpu_matches_golden(alt_harness, name):
     g = golden_io.load(name)
     field = host_check.render_alt(alt_harness, g["meta"])
     golden_io.assert_alt_parity(g, field)
+
+
+def test_normal_operand_fast_ops_equal_general_ops(harness):
+    """sc_mul_nn / sc_add_nn / sc_norm_nn are bit-identical to the general
+    ScaledComplex operations on normal operands (2 million random pairs with
+    zero parts, cancellations and exponent gaps around the 60-binade cut)."""
+    import ctypes
+    harness.fav_host_check_nn_ops.argtypes = [ctypes.c_longlong, ctypes.c_ulonglong,
+                                              ctypes.POINTER(ctypes.c_longlong)]
+    out = (ctypes.c_longlong * 3)()
+    harness.fav_host_check_nn_ops(2_000_000, 20261203, out)
+    assert list(out) == [0, 0, 0]
