@@ -229,6 +229,33 @@ FAV_HD int highest_level(int max_length) {  // floor(log2(max_length))
     return max_length > 0 ? 31 - __builtin_clz(static_cast<unsigned>(max_length)) : 0;
 }
 
+// Wave-level predicates over the active lanes (a single pixel is its own
+// "wave" on the host).  Loads indexed by a wave-uniform value take the
+// scalar path on the device.
+FAV_HD bool wave_all(bool v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ballot(v) == __ballot(1);
+#else
+    return v;
+#endif
+}
+
+FAV_HD bool wave_any(bool v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ballot(v) != 0ULL;
+#else
+    return v;
+#endif
+}
+
+FAV_HD int wave_first(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_readfirstlane(v);
+#else
+    return v;
+#endif
+}
+
 // BlaLevels::lookup_linear / lookup, renderer.cpp:8816-8877: the highest
 // aligned level whose block is longer than one step and whose validity
 // radius exceeds |delta|^2.
@@ -553,68 +580,71 @@ FAV_HD PixelState init_pixel(const RenderParams& p, size_t index, SC dc, Counter
 
 enum Outcome : int { kDone = 0, kHandoff = 1, kTail = 2 };
 
-// lookup_level for a normal |delta|^2 (positive, finite): the step flags
-// carry length > 1 and 0 < r2 < inf, so the comparison is exponent-first.
-template <typename Step>
-FAV_HD int lookup_level_normal(const Step* steps, const LevelDir& d, int start, SN dn, int max_length) {
-    if (start <= 0 || max_length <= 0) return -1;
-    const int base = d.count ? d.size[0] : 0;
-    if (start > base) return -1;
-    int h = highest_level(max_length);
-    if (h > d.count - 1) h = d.count - 1;
-    const int offset = start - 1;
-    for (int level = h; level >= 0; --level) {
-        if ((offset & ((1 << level) - 1)) != 0) continue;
-        const int index = offset >> level;
-        if (index >= d.size[level]) continue;
-        const int64_t k = d.offset[level] + index;
-        const Step& s = steps[k];
-        if ((s.pad[0] & kStepUsable) != 0 && (dn.e != s.r2_e ? dn.e < s.r2_e : dn.m < s.r2_m))
-            return static_cast<int>(k);
-    }
-    return -1;
-}
-
 // The common iterations of run_linear below, with the same results: while
 // the loop top is in the negligible-perturbation regime (Z + delta, the
 // margin and the norm are the orbit's own values, so the glitch / escape /
 // endpoint tests are the orbit's precomputed flags) and every operand of the
 // linear block is a normal ScaledComplex (fav_numeric.h), the block is formed
-// with the *_nn operations.  Any other case returns to run_linear's general
-// loop top with the state unchanged since the last committed block.
+// with the *_nn operations.  The decisions are taken for the whole wave:
+// when every active lane qualifies and all sit at the same reference index
+// and choose the same BLA level, the table reads are wave-uniform (scalar)
+// loads; any other case returns to run_linear's general loop top, where
+// each lane continues from the state of its last committed block.
 template <bool Planes, bool Stats>
 FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Counters<Stats>& st) {
     const OrbitDev& o = p.orbit;
     const int max_iter = p.max_iter;
-    while (s.have_total && s.iteration < max_iter) {
+    const LevelDir& dir = p.linear_dir;
+    for (;;) {
         const int ref = s.ref;
-        if (ref < 0 || ref >= o.size) return;
-        if (!sc_normal(s.delta)) return;
-        if constexpr (Planes) {
-            if (!sc_normal(s.derivative)) return;
-        }
-        if (s.delta.e > o.small_exp[ref]) return;
-        if ((o.flags[ref] & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) return;
+        bool ok = s.have_total && s.iteration < max_iter && sc_normal(s.delta);
+        if constexpr (Planes) ok = ok && sc_normal(s.derivative);
+        const int r = wave_first(ref);
+        ok = ok && ref == r;
+        if (!wave_all(ok)) return;
+        // wave-uniform from here: r is the reference index of every lane
+        if (r < 0 || r >= o.size || (o.map_end > 1 && r >= o.map_end)) return;
+        if ((o.flags[r] & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) return;
         const SN dn = sc_norm_nn(s.delta);
-        if (sc_compare_norm(s.total_norm, dn) < 0) return;  // rebase
-        if (o.map_end > 1 && ref >= o.map_end) return;
         const int order = p.approximation_order >= 3 && dn.e < -115 ? (dn.e < -160 ? 1 : 2) : p.approximation_order;
-        if (order > 1) return;
         const int remaining = max_iter - s.iteration;
-        const int k = lookup_level_normal(p.linear, p.linear_dir, ref, dn,
-                                          p.max_linear_length < remaining ? p.max_linear_length : remaining);
+        const int limit = p.max_linear_length < remaining ? p.max_linear_length : remaining;
+        ok = s.delta.e <= o.small_exp[r] && sc_compare_norm(s.total_norm, dn) >= 0 && order <= 1
+            && limit == wave_first(limit);
+        if (!wave_all(ok)) return;
+        // BLA level: every lane must take the same (highest valid) level
+        const int lim = wave_first(limit);
+        if (r <= 0 || lim <= 0 || dir.count == 0 || r > dir.size[0]) return;
+        int h = highest_level(lim);
+        if (h > dir.count - 1) h = dir.count - 1;
+        const int offset = r - 1;
+        int k = -1;
+        for (int level = h; level >= 0; --level) {
+            if ((offset & ((1 << level) - 1)) != 0) continue;
+            const int index = offset >> level;
+            if (index >= dir.size[level]) continue;
+            const int64_t kk = dir.offset[level] + index;
+            const LinearStepDev& cand = p.linear[kk];
+            if ((cand.pad[0] & kStepUsable) == 0) continue;
+            const bool fits = dn.e != cand.r2_e ? dn.e < cand.r2_e : dn.m < cand.r2_m;
+            if (wave_all(fits)) {
+                k = static_cast<int>(kk);
+                break;
+            }
+            if (wave_any(fits)) return;  // lanes disagree on the level
+        }
         if (k < 0) return;
         const LinearStepDev& step = p.linear[k];
         const int length = step.length;
-        if (ref + length > max_iter || (step.pad[0] & kStepNormalCoefficients) == 0) return;
+        if (r + length > max_iter || (step.pad[0] & kStepNormalCoefficients) == 0) return;
         const SC a{step.a_re, step.a_im, step.a_e};
         const SC b{step.b_re, step.b_im, step.b_e};
         const SC delta = sc_add_nn(sc_mul_nn(a, s.delta), sc_mul_nn(b, dc));
         SC derivative = s.derivative;
         if constexpr (Planes) derivative = sc_add_nn(sc_mul_nn(a, s.derivative), b);
-        const int end = ref + length;
-        if (sc_is_zero(delta) || delta.e > o.small_exp[end]) return;
+        const int end = r + length;
         if ((o.flags[end] & kOrbitEndpointFlag) != 0) return;
+        if (!wave_all(!sc_is_zero(delta) && delta.e <= o.small_exp[end])) return;
         // commit, renderer.cpp:13499-13511
         const SN end_norm{o.zn_m[end], o.zn_e[end]};
         s.derivative = derivative;
