@@ -108,6 +108,7 @@ DeviceTables::~DeviceTables() {
     if (cubic_block) (void)hipFree(cubic_block);
     if (series_block) (void)hipFree(series_block);
     if (alt_block) (void)hipFree(alt_block);
+    if (fast_block) (void)hipFree(fast_block);
 }
 
 namespace {
@@ -171,6 +172,7 @@ void ensure_device_tables(ReferenceContext& c) {
     t->alt_margin_m = reinterpret_cast<const double*>(base + off_am);
     t->alt_margin_e = reinterpret_cast<const int32_t*>(base + off_ae);
     upload_vector(img.alt, t->alt_block, t->alt, s.stream, t->bytes);
+    upload_vector(img.fast, t->fast_block, t->orbit.fast, s.stream, t->bytes);
     t->alt_dir = img.alt_dir;
     std::vector<unsigned char> sblob;
     const size_t s_re = append(sblob, img.series_re), s_im = append(sblob, img.series_im);

@@ -41,6 +41,17 @@ struct LevelDir {
     int64_t offset[kMaxLevels];
 };
 
+// The per-index values the negligible-regime block loop reads, gathered in
+// one 64-byte record so one wave-uniform scalar load fetches them all:
+// Z_n, |Z_n|^2 as the loop-top norm, the escape margin, the negligible
+// delta exponent and the loop-top flags.
+struct alignas(64) OrbitFastDev {
+    double z_re, z_im, zn_m, margin_m;
+    int32_t z_e, zn_e, margin_e, small_exp, flags;
+    int32_t pad[3];
+};
+static_assert(sizeof(OrbitFastDev) == 64, "orbit record is one cache line");
+
 // Device copy of one reference (orbit + tables), structure of arrays.  The
 // per-index FloatExp parts of Z, the reference escape margin |Z|^2-R^2 and
 // the glitch threshold 1e-7*|Z|^2 are pure functions of the orbit; they are
@@ -64,6 +75,7 @@ struct OrbitDev {
     const int32_t* zn_e;
     const int32_t* small_exp;   // delta.e <= small_exp[n]: Z + delta, margin and norm are Z's own
     const int32_t* flags;       // bit 0: glitch test fires, bit 1: escape test fires (negligible delta)
+    const OrbitFastDev* fast;   // the same values per index as 64-byte records
     int32_t size;
     int32_t map_end;
 };

@@ -63,6 +63,7 @@ struct DeviceTables {
     const CubicStepDev* cubic = nullptr;
     LevelDir cubic_dir{};
     void* alt_block = nullptr;
+    void* fast_block = nullptr;
     const AltStepDev* alt = nullptr;
     const double* alt_margin_m = nullptr;  // inside orbit_block
     const int32_t* alt_margin_e = nullptr;

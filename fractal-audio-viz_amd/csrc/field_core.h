@@ -595,25 +595,24 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
     const OrbitDev& o = p.orbit;
     const int max_iter = p.max_iter;
     const LevelDir& dir = p.linear_dir;
+    bool ok = s.have_total && s.iteration < max_iter && sc_normal(s.delta);
+    if constexpr (Planes) ok = ok && sc_normal(s.derivative);
+    int r = wave_first(s.ref);
+    if (!wave_all(ok && s.ref == r)) return;
+    if (r < 0 || r >= o.size) return;
+    OrbitFastDev top = o.fast[r];  // loop-top record of the wave's reference index
     for (;;) {
-        const int ref = s.ref;
-        bool ok = s.have_total && s.iteration < max_iter && sc_normal(s.delta);
-        if constexpr (Planes) ok = ok && sc_normal(s.derivative);
-        const int r = wave_first(ref);
-        ok = ok && ref == r;
-        if (!wave_all(ok)) return;
-        // wave-uniform from here: r is the reference index of every lane
-        if (r < 0 || r >= o.size || (o.map_end > 1 && r >= o.map_end)) return;
-        if ((o.flags[r] & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) return;
+        // wave-uniform: r is the reference index of every lane, top its record
+        if (o.map_end > 1 && r >= o.map_end) return;
+        if ((top.flags & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) return;
         const SN dn = sc_norm_nn(s.delta);
         const int order = p.approximation_order >= 3 && dn.e < -115 ? (dn.e < -160 ? 1 : 2) : p.approximation_order;
         const int remaining = max_iter - s.iteration;
         const int limit = p.max_linear_length < remaining ? p.max_linear_length : remaining;
-        ok = s.delta.e <= o.small_exp[r] && sc_compare_norm(s.total_norm, dn) >= 0 && order <= 1
-            && limit == wave_first(limit);
+        const int lim = wave_first(limit);
+        ok = s.delta.e <= top.small_exp && sc_compare_norm(s.total_norm, dn) >= 0 && order <= 1 && limit == lim;
         if (!wave_all(ok)) return;
         // BLA level: every lane must take the same (highest valid) level
-        const int lim = wave_first(limit);
         if (r <= 0 || lim <= 0 || dir.count == 0 || r > dir.size[0]) return;
         int h = highest_level(lim);
         if (h > dir.count - 1) h = dir.count - 1;
@@ -634,26 +633,32 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
             if (wave_any(fits)) return;  // lanes disagree on the level
         }
         if (k < 0) return;
-        const LinearStepDev& step = p.linear[k];
+        const LinearStepDev step = p.linear[k];
         const int length = step.length;
         if (r + length > max_iter || (step.pad[0] & kStepNormalCoefficients) == 0) return;
+        const int end = r + length;
+        const OrbitFastDev next = o.fast[end];
         const SC a{step.a_re, step.a_im, step.a_e};
         const SC b{step.b_re, step.b_im, step.b_e};
         const SC delta = sc_add_nn(sc_mul_nn(a, s.delta), sc_mul_nn(b, dc));
         SC derivative = s.derivative;
         if constexpr (Planes) derivative = sc_add_nn(sc_mul_nn(a, s.derivative), b);
-        const int end = r + length;
-        if ((o.flags[end] & kOrbitEndpointFlag) != 0) return;
-        if (!wave_all(!sc_is_zero(delta) && delta.e <= o.small_exp[end])) return;
+        if ((next.flags & kOrbitEndpointFlag) != 0) return;
+        // the results stay normal unless a sum cancelled to zero or an
+        // exponent left the normal range
+        bool normal = !sc_is_zero(delta) && delta.e <= next.small_exp && delta.e >= -(1 << 28);
+        if constexpr (Planes) normal = normal && !sc_is_zero(derivative) && derivative.e <= (1 << 28)
+                                       && derivative.e >= -(1 << 28);
+        if (!wave_all(normal)) return;
         // commit, renderer.cpp:13499-13511
-        const SN end_norm{o.zn_m[end], o.zn_e[end]};
+        const SN end_norm{next.zn_m, next.zn_e};
         s.derivative = derivative;
         s.delta = delta;
         s.ref = end;
         s.iteration += length;
-        s.total = load_z(o, end);
+        s.total = SC{next.z_re, next.z_im, next.z_e};
         s.total_norm = end_norm;
-        s.margin = FE{o.margin_m[end], o.margin_e[end]};
+        s.margin = FE{next.margin_m, next.margin_e};
         s.previous_total_norm = end_norm;
         st.add(ST_BLA_BLOCKS);
         st.add(ST_LINEAR_BLOCKS);
@@ -665,6 +670,9 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
             st.add(ST_HISTOGRAM + bucket);
         }
         st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
+        if (!wave_all(s.iteration < max_iter)) return;
+        r = end;
+        top = next;
     }
 }
 

@@ -51,6 +51,7 @@ extern "C" int fav_host_check_render(
         p.orbit.zn_e = o.zn_e.data();
         p.orbit.small_exp = o.small_exp.data();
         p.orbit.flags = o.flags.data();
+        p.orbit.fast = img.fast.data();
         p.series.re = img.series_re.data();
         p.series.im = img.series_im.data();
         p.series.e = img.series_e.data();

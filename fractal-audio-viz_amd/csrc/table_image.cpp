@@ -37,6 +37,21 @@ void make_table_image(const ReferenceContext& c, TableImage& img) {
         img.im[i] = o.scaled[i].im;
         img.e[i] = o.scaled[i].e;
     }
+    img.fast.assign(n, OrbitFastDev{});
+    if (o.zn_m.size() == n) {
+        for (size_t i = 0; i < n; ++i) {
+            OrbitFastDev& f = img.fast[i];
+            f.z_re = o.scaled[i].re;
+            f.z_im = o.scaled[i].im;
+            f.z_e = o.scaled[i].e;
+            f.zn_m = o.zn_m[i];
+            f.zn_e = o.zn_e[i];
+            f.margin_m = o.margin_m[i];
+            f.margin_e = o.margin_e[i];
+            f.small_exp = o.small_exp[i];
+            f.flags = o.flags[i];
+        }
+    }
     flatten(c.cubic.levels, img.cubic, img.cubic_dir);
     for (int t = 0; t < 2; ++t) {
         flatten(c.linear[t].levels, img.linear[t], img.linear_dir[t]);

@@ -20,6 +20,7 @@ struct TableImage {
     LevelDir cubic_dir{};
     std::vector<double> series_re, series_im;
     std::vector<int32_t> series_e;
+    std::vector<OrbitFastDev> fast;  // per-index records of the block loop
     std::vector<AltStepDev> alt;  // alternate formulas only
     LevelDir alt_dir{};
 };
