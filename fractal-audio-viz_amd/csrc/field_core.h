@@ -248,6 +248,20 @@ FAV_HD bool wave_any(bool v) {
 #endif
 }
 
+// Read-only table entry at a wave-uniform index: through the constant
+// address space on the device, so the load is a scalar (SMEM) load and the
+// branches on its fields are scalar branches.  Tables are written before
+// the launch and never during it.
+template <typename T>
+FAV_HD T load_uniform(const T* table, int64_t index) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef const T __attribute__((address_space(4)))* ConstantPtr;
+    return ((ConstantPtr)table)[index];
+#else
+    return table[index];
+#endif
+}
+
 FAV_HD int wave_first(int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_readfirstlane(v);
@@ -600,66 +614,76 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
     int r = wave_first(s.ref);
     if (!wave_all(ok && s.ref == r)) return;
     if (r < 0 || r >= o.size) return;
-    OrbitFastDev top = o.fast[r];  // loop-top record of the wave's reference index
+    // The loop-top record of the wave's reference index.  While blocks
+    // commit, the orbit-only loop-top values (total, its norm, the margin)
+    // are those of this record; they are written to the pixel state on exit.
+    OrbitFastDev top = load_uniform(o.fast, r);
+    SC delta = s.delta, derivative = s.derivative;
+    int iteration = s.iteration;
+    bool committed = false;
     for (;;) {
-        // wave-uniform: r is the reference index of every lane, top its record
-        if (o.map_end > 1 && r >= o.map_end) return;
-        if ((top.flags & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) return;
-        const SN dn = sc_norm_nn(s.delta);
+        if (o.map_end > 1 && r >= o.map_end) break;
+        if ((top.flags & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) break;
+        const SN dn = sc_norm_nn(delta);
         const int order = p.approximation_order >= 3 && dn.e < -115 ? (dn.e < -160 ? 1 : 2) : p.approximation_order;
-        const int remaining = max_iter - s.iteration;
+        const int remaining = max_iter - iteration;
         const int limit = p.max_linear_length < remaining ? p.max_linear_length : remaining;
         const int lim = wave_first(limit);
-        ok = s.delta.e <= top.small_exp && sc_compare_norm(s.total_norm, dn) >= 0 && order <= 1 && limit == lim;
-        if (!wave_all(ok)) return;
+        // the norm compare uses |Z|^2 of the record: equal to the state's
+        // total_norm whenever delta is negligible, which the first clause checks
+        ok = delta.e <= top.small_exp && sc_compare_norm(SN{top.zn_m, top.zn_e}, dn) >= 0 && order <= 1
+            && limit == lim;
+        if (!wave_all(ok)) break;
         // BLA level: every lane must take the same (highest valid) level
-        if (r <= 0 || lim <= 0 || dir.count == 0 || r > dir.size[0]) return;
+        if (r <= 0 || lim <= 0 || dir.count == 0 || r > dir.size[0]) break;
         int h = highest_level(lim);
         if (h > dir.count - 1) h = dir.count - 1;
         const int offset = r - 1;
+        // levels above the alignment of offset are skipped by the lookup
+        if (offset != 0 && __builtin_ctz(static_cast<unsigned>(offset)) < h)
+            h = __builtin_ctz(static_cast<unsigned>(offset));
         int k = -1;
+        bool mixed = false;
         for (int level = h; level >= 0; --level) {
-            if ((offset & ((1 << level) - 1)) != 0) continue;
             const int index = offset >> level;
             if (index >= dir.size[level]) continue;
             const int64_t kk = dir.offset[level] + index;
-            const LinearStepDev& cand = p.linear[kk];
+            const LinearStepDev cand = load_uniform(p.linear, kk);
             if ((cand.pad[0] & kStepUsable) == 0) continue;
             const bool fits = dn.e != cand.r2_e ? dn.e < cand.r2_e : dn.m < cand.r2_m;
             if (wave_all(fits)) {
                 k = static_cast<int>(kk);
                 break;
             }
-            if (wave_any(fits)) return;  // lanes disagree on the level
+            if (wave_any(fits)) {  // lanes disagree on the level
+                mixed = true;
+                break;
+            }
         }
-        if (k < 0) return;
-        const LinearStepDev step = p.linear[k];
+        if (k < 0 || mixed) break;
+        const LinearStepDev step = load_uniform(p.linear, k);
         const int length = step.length;
-        if (r + length > max_iter || (step.pad[0] & kStepNormalCoefficients) == 0) return;
+        if (r + length > max_iter || (step.pad[0] & kStepNormalCoefficients) == 0) break;
         const int end = r + length;
-        const OrbitFastDev next = o.fast[end];
+        const OrbitFastDev next = load_uniform(o.fast, end);
+        if ((next.flags & kOrbitEndpointFlag) != 0) break;
         const SC a{step.a_re, step.a_im, step.a_e};
         const SC b{step.b_re, step.b_im, step.b_e};
-        const SC delta = sc_add_nn(sc_mul_nn(a, s.delta), sc_mul_nn(b, dc));
-        SC derivative = s.derivative;
-        if constexpr (Planes) derivative = sc_add_nn(sc_mul_nn(a, s.derivative), b);
-        if ((next.flags & kOrbitEndpointFlag) != 0) return;
+        const SC d1 = sc_add_nn(sc_mul_nn(a, delta), sc_mul_nn(b, dc));
+        SC g1 = derivative;
+        if constexpr (Planes) g1 = sc_add_nn(sc_mul_nn(a, derivative), b);
         // the results stay normal unless a sum cancelled to zero or an
         // exponent left the normal range
-        bool normal = !sc_is_zero(delta) && delta.e <= next.small_exp && delta.e >= -(1 << 28);
-        if constexpr (Planes) normal = normal && !sc_is_zero(derivative) && derivative.e <= (1 << 28)
-                                       && derivative.e >= -(1 << 28);
-        if (!wave_all(normal)) return;
+        bool normal = !sc_is_zero(d1) && d1.e <= next.small_exp && d1.e >= -(1 << 28);
+        if constexpr (Planes) normal = normal && !sc_is_zero(g1) && g1.e <= (1 << 28) && g1.e >= -(1 << 28);
+        if (!wave_all(normal)) break;
         // commit, renderer.cpp:13499-13511
-        const SN end_norm{next.zn_m, next.zn_e};
-        s.derivative = derivative;
-        s.delta = delta;
-        s.ref = end;
-        s.iteration += length;
-        s.total = SC{next.z_re, next.z_im, next.z_e};
-        s.total_norm = end_norm;
-        s.margin = FE{next.margin_m, next.margin_e};
-        s.previous_total_norm = end_norm;
+        delta = d1;
+        derivative = g1;
+        iteration += length;
+        r = end;
+        top = next;
+        committed = true;
         st.add(ST_BLA_BLOCKS);
         st.add(ST_LINEAR_BLOCKS);
         st.add(ST_SERIES_JUMPS);
@@ -670,9 +694,18 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
             st.add(ST_HISTOGRAM + bucket);
         }
         st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
-        if (!wave_all(s.iteration < max_iter)) return;
-        r = end;
-        top = next;
+        if (!wave_all(iteration < max_iter)) break;
+    }
+    if (committed) {
+        const SN norm{top.zn_m, top.zn_e};
+        s.delta = delta;
+        s.derivative = derivative;
+        s.iteration = iteration;
+        s.ref = r;
+        s.total = SC{top.z_re, top.z_im, top.z_e};
+        s.total_norm = norm;
+        s.margin = FE{top.margin_m, top.margin_e};
+        s.previous_total_norm = norm;
     }
 }
 
