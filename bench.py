@@ -104,6 +104,47 @@ def copy_calibration(mib, copies):
     return 0
 
 
+def floatexp_leg(lib, hip, ref, work, opts, d_field, fp64_field, fp64_orbit_iteration, args):
+    """The FP32 floatexp path on the same field: throughput over the same
+    step count, and the fraction of pixels whose escape count differs from
+    the FP64 kernel's (plus the largest smooth-value difference where the
+    counts agree)."""
+    w, h = work["width"], work["height"]
+    n = w * h
+    d_counts = hip.malloc(n * 4)
+
+    def step():
+        status = lib.fractal_hip_render_reference_floatexp(
+            d_field, d_counts, w, h, work["zoom"].encode(), ref.handle, work["max_iter"], work["series_order"],
+            work["series_block"], ctypes.byref(opts), 1)
+        if status != 0:
+            raise RuntimeError(lib.fractal_last_error().decode())
+        return fav_hip.last_timing()["kernel_ms"]
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    hip.synchronize()
+    started = time.perf_counter()
+    kernel_ms = [step() for _ in range(args.steps)]
+    hip.synchronize()
+    elapsed = (time.perf_counter() - started) / args.steps
+    field = np.empty(n, dtype=np.float32)
+    counts = np.empty(n, dtype=np.int32)
+    hip.to_host(field, d_field)
+    hip.to_host(counts, d_counts)
+    hip.free(d_counts)
+    cap = work["max_iter"]
+    reference = np.where(np.isnan(fp64_field), -1,
+                         np.where(fp64_orbit_iteration >= cap, cap, fp64_orbit_iteration + 1))
+    agree = (counts == reference) & ~np.isnan(field) & ~np.isnan(fp64_field)
+    max_err = float(np.max(np.abs(field[agree].astype(np.float64) - fp64_field[agree]))) if agree.any() else 0.0
+    pixel_iterations = int(np.where(counts < 0, cap, counts).astype(np.int64).sum())
+    return {"value": round(pixel_iterations / elapsed / 1e9, 3), "unit": "Gpixel-iter/s",
+            "ms_per_step": round(elapsed * 1e3, 3), "mean_kernel_ms": round(float(np.mean(kernel_ms)), 3),
+            "pixels_differing_from_fp64_pct": round(float(np.mean(counts != reference)) * 100.0, 4),
+            "max_smooth_difference": max_err, "dtype": "f32 mantissa + i32 exponent deltas"}
+
+
 def main():
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -190,6 +231,7 @@ def main():
     achieved = flops / (mean_kernel_ms * 1e-3) / 1e12
     rgb = np.empty(n * 3, dtype=np.uint8)
     hip.to_host(rgb, d_rgb)
+    floatexp = floatexp_leg(lib, hip, ref, work, opts, d_field, field, orbit_iteration, args)
     hip.free(d_field)
     hip.free(d_planes_block)
     hip.free(d_rgb)
@@ -225,6 +267,7 @@ def main():
                      "frac": round(achieved / MI355X_FP64_VECTOR_PEAK_TFLOPS, 4),
                      "traffic": measured_traffic_bytes()},
         "cpu_baseline": baseline,
+        "floatexp": floatexp,
     }
     print(json.dumps(result))
     return 0

@@ -554,6 +554,37 @@ int render_fractal_ex_planes(float* output, int width, int height, const char* z
 
 // ---- extensions (fav_extensions.h) ----------------------------------------
 
+int fractal_hip_render_reference_floatexp(float* output, int32_t* counts, int width, int height, const char* zoom,
+                                          void* handle, int max_iter, int series_order, int series_block,
+                                          const FractalRenderOptions* options, int device_pointers) {
+    try {
+        if (!output || !counts || !zoom || !handle || !valid_dimensions(width, height) || !valid_iterations(max_iter)
+            || !valid_series(series_order, series_block) || (device_pointers != 0 && device_pointers != 1))
+            throw std::runtime_error("invalid native render dimensions or handle");
+        const FractalRenderOptions o = checked_options(options);
+        require_hip_backend(o);
+        const auto context = acquire(handle);
+        if (!context) throw std::runtime_error("invalid or already-destroyed reference handle");
+        if (context->formula != 0) throw std::runtime_error("the floatexp path renders the Mandelbrot formula only");
+        if (context->escape_radius_mode != o.escape_radius_mode)
+            throw std::runtime_error("reference escape-radius mode does not match render options");
+        if (max_iter > context->requested_max_iter)
+            throw std::runtime_error("render iteration count exceeds prepared reference");
+        if (!valid_text(zoom)) throw std::runtime_error("native zoom text is too long or null");
+        fav::FieldRequest request{output, width, height, zoom, max_iter, series_order, series_block, o,
+                                  nullptr, nullptr, device_pointers != 0};
+        fav::render_field_floatexp(*context, request, counts);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native floatexp render failed with an unknown exception");
+        return 1;
+    }
+}
+
 int fractal_hip_render_reference_device(float* device_output, int width, int height, const char* zoom, void* handle,
                                         int max_iter, int series_order, int series_block,
                                         const FractalRenderOptions* options, FractalRenderPlanes* device_planes) {

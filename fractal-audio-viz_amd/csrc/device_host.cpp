@@ -16,6 +16,7 @@
 #include "table_image.h"
 #include "kfp_core.h"
 #include "direct_core.h"
+#include "fx_core.h"
 
 namespace fav {
 
@@ -24,6 +25,7 @@ hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, i
 hipError_t launch_kfp_kernel(const kfp::KfpParams& params, void* scratch, int compute_units, hipStream_t stream);
 hipError_t launch_direct_kernel(const direct::DirectParams& p, bool planes, hipStream_t stream);
 hipError_t launch_alt_kernel(const AltRenderParams& p, hipStream_t stream);
+hipError_t launch_fx_kernel(const RenderParams& p, const fx::FxOut& out, hipStream_t stream);
 
 namespace {
 
@@ -62,7 +64,7 @@ struct LaunchState {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords;
+    DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords, fx_counts;
     int compute_units = 256;
     double wall_clock_khz = 0.0;
     LastTiming timing{};
@@ -317,6 +319,58 @@ void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers) {
     if (!device_pointers)
         check(hipMemcpyAsync(host.output, k.output, pixels * 3, hipMemcpyDeviceToHost, s.stream), "D2H rgb");
     check(hipStreamSynchronize(s.stream), "KFP colour");
+    float kernel_ms = 0.0f;
+    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    s.timing.kernel_ms = kernel_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
+}
+
+void render_field_floatexp(ReferenceContext& c, const FieldRequest& r, int32_t* counts) {
+    const int w = r.width, h = r.height;
+    PreparedRender prep;
+    prepare_render(c, r, prep);
+    ensure_device_tables(c);
+    const DeviceTables& t = *c.device;
+    RenderParams p = prep.params;
+    p.orbit = t.orbit;
+    const SeriesDev scalar_series = p.series;
+    p.series = t.series;
+    p.series.count = scalar_series.count;
+    p.series.iteration = scalar_series.iteration;
+    p.series.available = scalar_series.available;
+    p.series.radius2_m = scalar_series.radius2_m;
+    p.series.radius2_e = scalar_series.radius2_e;
+    p.linear = t.linear[prep.linear_tier];
+    p.linear_dir = t.linear_dir[prep.linear_tier];
+
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
+    const std::vector<double>& off_m = prep.offset_m;
+    const std::vector<int32_t>& off_e = prep.offset_e;
+    char* doff = static_cast<char*>(s.offsets.get(off_m.size() * 12 + 512));
+    const size_t exp_offset = (off_m.size() * 8 + 255) & ~size_t(255);
+    check(hipMemcpyAsync(doff, off_m.data(), off_m.size() * 8, hipMemcpyHostToDevice, s.stream), "offsets");
+    check(hipMemcpyAsync(doff + exp_offset, off_e.data(), off_e.size() * 4, hipMemcpyHostToDevice, s.stream),
+          "offsets");
+    p.x_off_m = reinterpret_cast<const double*>(doff);
+    p.y_off_m = p.x_off_m + w;
+    p.x_off_e = reinterpret_cast<const int32_t*>(doff + exp_offset);
+    p.y_off_e = p.x_off_e + w;
+    fx::FxOut out{};
+    out.field = r.device_pointers ? r.output : static_cast<float*>(s.output.get(pixels * sizeof(float)));
+    out.counts = r.device_pointers ? counts : static_cast<int32_t*>(s.fx_counts.get(pixels * sizeof(int32_t)));
+    const uint64_t wall0 = now_ns();
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_fx_kernel(p, out, s.stream), "floatexp kernel launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    if (!r.device_pointers) {
+        check(hipMemcpyAsync(r.output, out.field, pixels * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H");
+        check(hipMemcpyAsync(counts, out.counts, pixels * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream), "D2H");
+    }
+    check(hipStreamSynchronize(s.stream), "floatexp render");
     float kernel_ms = 0.0f;
     check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
     s.timing.kernel_ms = kernel_ms;

@@ -30,6 +30,11 @@ struct LastTiming {
 
 void render_field(ReferenceContext& context, const FieldRequest& request);
 
+// FP32 floatexp field of a Mandelbrot handle: request.output (field) and
+// `counts` (escape iteration per pixel, max_iter inside, -1 unresolved) are
+// host or device pointers as request.device_pointers says; no planes.
+void render_field_floatexp(ReferenceContext& context, const FieldRequest& request, int32_t* counts);
+
 // Alternate-formula field (Julia, burning ship, tricorn) of a reference
 // handle; request.planes must be null.
 void render_alternate(ReferenceContext& context, const FieldRequest& request);

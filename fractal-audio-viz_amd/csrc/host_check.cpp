@@ -348,3 +348,48 @@ extern "C" void fav_host_check_nn_ops(long long count, unsigned long long seed, 
         if (std::memcmp(&x.m, &y.m, 8) != 0 || x.e != y.e) ++mismatches[2];
     }
 }
+
+#include "fx_core.h"
+
+// The floatexp path's per-pixel code on the CPU (test-only), on the tables
+// of a Mandelbrot reference built exactly as for the FP64 path.
+extern "C" int fav_host_check_fx(const char* x, const char* y, const char* ref_zoom, int max_iter, int bits,
+                                 const FractalRenderOptions* options, float* field, int32_t* counts, int width,
+                                 int height, const char* zoom, int series_order, int series_block, char* error,
+                                 int error_capacity) {
+    using namespace fav;
+    try {
+        auto c = build_reference(x, y, ref_zoom, max_iter, bits, 8, false, 0, options->escape_radius_mode,
+                                 options->coordinate_mode);
+        TableImage img;
+        make_table_image(*c, img);
+        FieldRequest request{field, width, height, zoom, max_iter, series_order, series_block, *options, nullptr,
+                             nullptr, false};
+        PreparedRender prep;
+        prepare_render(*c, request, prep);
+        RenderParams p = prep.params;
+        const OrbitData& o = *c->orbit;
+        p.orbit.re = img.re.data();
+        p.orbit.im = img.im.data();
+        p.orbit.e = img.e.data();
+        p.orbit.glitch_m = o.glitch_m.data();
+        p.orbit.glitch_e = o.glitch_e.data();
+        p.series.re = img.series_re.data();
+        p.series.im = img.series_im.data();
+        p.series.e = img.series_e.data();
+        p.linear = img.linear[prep.linear_tier].data();
+        p.linear_dir = img.linear_dir[prep.linear_tier];
+        p.x_off_m = prep.offset_m.data();
+        p.y_off_m = prep.offset_m.data() + width;
+        p.x_off_e = prep.offset_e.data();
+        p.y_off_e = prep.offset_e.data() + width;
+        const fx::FxOut out{field, counts};
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int py = 0; py < height; ++py)
+            for (int px = 0; px < width; ++px) fx::pixel(p, out, px, py);
+        return 0;
+    } catch (const std::exception& e) {
+        std::snprintf(error, static_cast<size_t>(error_capacity), "%s", e.what());
+        return 1;
+    }
+}

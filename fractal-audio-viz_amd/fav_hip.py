@@ -150,6 +150,10 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int,
         ctypes.c_int, ctypes.c_int, ctypes.POINTER(NativeRenderOptions), ctypes.c_void_p]
     lib.fractal_hip_render_reference_device.restype = ctypes.c_int
+    lib.fractal_hip_render_reference_floatexp.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(NativeRenderOptions), ctypes.c_int]
+    lib.fractal_hip_render_reference_floatexp.restype = ctypes.c_int
     lib.fractal_hip_last_timing.argtypes = [_c_double_p, ctypes.c_int]
     lib.fractal_hip_last_timing.restype = ctypes.c_int
     lib.fractal_hip_reference_upload.argtypes = [ctypes.c_void_p]

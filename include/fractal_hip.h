@@ -178,8 +178,8 @@ void *fractal_create_reference_reusable_options(
     const char *x_center, const char *y_center, const char *viewport_zoom,
     int max_iter, int precision_bits, int series_order,
     const FractalRenderOptions *options);
-/* renderer.h:215-225 / renderer.cpp:19655-19685.  Mandelbrot only in this
- * library; other formula ids fail with an error. */
+/* renderer.h:215-225 / renderer.cpp:19655-19685.  Formula 0-3 (Mandelbrot,
+ * Julia, burning ship, tricorn). */
 void *fractal_create_reference_ex(
     const char *x_center, const char *y_center, const char *viewport_zoom,
     int max_iter, int precision_bits, int series_order, int formula,
@@ -289,6 +289,17 @@ int fractal_colourise_kfp_precise(
     int threads);
 
 /* ---- Extensions of the HIP backend (no counterpart in renderer.h) ------ */
+
+/* The FP32 floatexp perturbation path of a Mandelbrot handle (no
+ * counterpart in the reference): deltas in float mantissas with an int32
+ * exponent, FP64 orbit/BLA tables, FP64 escape/rebase tests and smoothing.
+ * Writes the float field and per-pixel escape counts (max_iter inside, -1
+ * unresolved); device_pointers = 1 when both already live in HBM. */
+int fractal_hip_render_reference_floatexp(
+    float *output, int32_t *counts, int width, int height,
+    const char *zoom_text, void *handle, int max_iter, int series_order,
+    int series_block, const FractalRenderOptions *options,
+    int device_pointers);
 
 /* fractal_colourise_kfp on device pointers: device_field (W*H floats) and
  * device_rgb (W*H*3 bytes) already live in HBM; lut is a host pointer. */

@@ -1,0 +1,53 @@
+"""The FP32 floatexp path against the FP64 kernel.
+
+This path is not the reference's arithmetic, so it is judged by the
+fraction of pixels whose escape count differs from the FP64 result and by
+the largest smooth-value difference on pixels whose counts agree: exact
+agreement where the field is interior or shallow in iterations, and a
+documented bound elsewhere."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import golden_io
+import native
+
+pytestmark = pytest.mark.gpu
+
+
+def render_fx(lib, meta):
+    w, h = meta["width"], meta["height"]
+    opts = native.RenderOptions.defaults(backend=2, **meta["options"])
+    ref = native.AbiReference(lib, meta["x"], meta["y"], meta["ref_zoom"], meta["max_iter"], meta["bits"],
+                              meta["image_series_order"], opts)
+    field = np.empty(w * h, dtype=np.float32)
+    counts = np.empty(w * h, dtype=np.int32)
+    lib.fractal_hip_render_reference_floatexp.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(native.RenderOptions), ctypes.c_int]
+    status = lib.fractal_hip_render_reference_floatexp(
+        field.ctypes.data, counts.ctypes.data, w, h, meta["zoom"].encode(), ctypes.c_void_p(ref.handle),
+        meta["max_iter"], meta["series_order"], meta["series_block"], ctypes.byref(opts), 0)
+    assert status == 0, lib.fractal_last_error()
+    return field, counts
+
+
+def fp64_counts(golden):
+    oi = golden["orbit_iteration"].reshape(-1)
+    f = golden["field"].reshape(-1)
+    cap = golden["meta"]["max_iter"]
+    return np.where(np.isnan(f), -1, np.where(oi >= cap, cap, oi + 1))
+
+
+@pytest.mark.parametrize("name,max_mismatch", [("c2_kalles", 0.0), ("deep_tier", 0.0), ("ultra_tier", 0.0),
+                                               ("cycle_1e100", 0.0), ("c1_series", 0.25), ("bla_1e25", 0.25)])
+def test_floatexp_count_mismatch_against_fp64(gpu_product, name, max_mismatch):
+    print(f"[gpu] floatexp {name}", flush=True)
+    g = golden_io.load(name)
+    field, counts = render_fx(gpu_product, g["meta"])
+    mismatch = float(np.mean(counts != fp64_counts(g)))
+    print(f"[gpu] floatexp {name}: {mismatch * 100:.2f}% of pixels differ from FP64", flush=True)
+    assert mismatch <= max_mismatch
+    if max_mismatch == 0.0:
+        np.testing.assert_allclose(field, g["field"].reshape(-1), rtol=0, atol=1e-3)
