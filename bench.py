@@ -139,7 +139,25 @@ def floatexp_leg(lib, hip, ref, work, opts, d_field, fp64_field, fp64_orbit_iter
     agree = (counts == reference) & ~np.isnan(field) & ~np.isnan(fp64_field)
     max_err = float(np.max(np.abs(field[agree].astype(np.float64) - fp64_field[agree]))) if agree.any() else 0.0
     pixel_iterations = int(np.where(counts < 0, cap, counts).astype(np.int64).sum())
+    # the floatexp path writes no planes: time the FP64 path without planes too
+    def fp64_step():
+        status = lib.fractal_hip_render_reference_device(
+            d_field, w, h, work["zoom"].encode(), ref.handle, work["max_iter"], work["series_order"],
+            work["series_block"], ctypes.byref(opts), None)
+        if status != 0:
+            raise RuntimeError(lib.fractal_last_error().decode())
+
+    fp64_step()
+    hip.synchronize()
+    started = time.perf_counter()
+    for _ in range(args.steps):
+        fp64_step()
+    hip.synchronize()
+    fp64_elapsed = (time.perf_counter() - started) / args.steps
+    fp64_pixel_iterations = int(np.where(reference < 0, cap, reference).astype(np.int64).sum())
     return {"value": round(pixel_iterations / elapsed / 1e9, 3), "unit": "Gpixel-iter/s",
+            "planes": False,
+            "fp64_without_planes_value": round(fp64_pixel_iterations / fp64_elapsed / 1e9, 3),
             "ms_per_step": round(elapsed * 1e3, 3), "mean_kernel_ms": round(float(np.mean(kernel_ms)), 3),
             "pixels_differing_from_fp64_pct": round(float(np.mean(counts != reference)) * 100.0, 4),
             "max_smooth_difference": max_err, "dtype": "f32 mantissa + i32 exponent deltas"}

@@ -76,6 +76,77 @@ FAV_HD SN norm2(FX v) {
     return SN{ldexp_(n, -shift), 2 * v.e + shift};
 }
 
+// Linear blocks in the negligible-perturbation regime, decided for the
+// whole wave as in the FP64 block loop (field_core.h run_normal_blocks):
+// the loop top's escape / glitch / endpoint outcomes are the orbit's flags,
+// all lanes share the reference index and BLA level, and the table reads are
+// scalar.  Returns when any lane needs the general loop below.
+FAV_HD void run_blocks(const RenderParams& p, FX& delta, FX dc, int& ref, int& iteration, int last) {
+    const OrbitDev& o = p.orbit;
+    const LevelDir& dir = p.linear_dir;
+    const int max_iter = p.max_iter;
+    int r = core::wave_first(ref);
+    if (!core::wave_all(ref == r && iteration < max_iter && !is_zero(delta)) || r <= 0 || r >= last) return;
+    OrbitFastDev top = core::load_uniform(o.fast, r);
+    FX d = delta;
+    int it = iteration;
+    bool committed = false;
+    for (;;) {
+        if ((top.flags & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) break;
+        const SN dn = norm2(d);
+        const int remaining = max_iter - it;
+        const int limit = p.max_linear_length < remaining ? p.max_linear_length : remaining;
+        const int lim = core::wave_first(limit);
+        const bool ok = d.e <= top.small_exp && sc_compare_norm(SN{top.zn_m, top.zn_e}, dn) >= 0 && limit == lim;
+        if (!core::wave_all(ok)) break;
+        if (lim <= 0 || dir.count == 0 || r > dir.size[0]) break;
+        int h = core::highest_level(lim);
+        if (h > dir.count - 1) h = dir.count - 1;
+        const int offset = r - 1;
+        if (offset != 0 && __builtin_ctz(static_cast<unsigned>(offset)) < h)
+            h = __builtin_ctz(static_cast<unsigned>(offset));
+        int k = -1;
+        bool mixed = false;
+        for (int level = h; level >= 0; --level) {
+            const int index = offset >> level;
+            if (index >= dir.size[level]) continue;
+            const int64_t kk = dir.offset[level] + index;
+            const LinearStepDev cand = core::load_uniform(p.linear, kk);
+            if ((cand.pad[0] & kStepUsable) == 0) continue;
+            const bool fits = dn.e != cand.r2_e ? dn.e < cand.r2_e : dn.m < cand.r2_m;
+            if (core::wave_all(fits)) {
+                k = static_cast<int>(kk);
+                break;
+            }
+            if (core::wave_any(fits)) {
+                mixed = true;
+                break;
+            }
+        }
+        if (k < 0 || mixed) break;
+        const LinearStepDev step = core::load_uniform(p.linear, k);
+        const int end = r + step.length;
+        if (end > max_iter || end > last) break;
+        const OrbitFastDev next = core::load_uniform(o.fast, end);
+        if ((next.flags & kOrbitEndpointFlag) != 0) break;
+        const FX a = from_sc(SC{step.a_re, step.a_im, step.a_e});
+        const FX b = from_sc(SC{step.b_re, step.b_im, step.b_e});
+        const FX d1 = add(mul(a, d), mul(b, dc));
+        if (!core::wave_all(!is_zero(d1))) break;
+        d = d1;
+        it += step.length;
+        r = end;
+        top = next;
+        committed = true;
+        if (!core::wave_all(it < max_iter) || r >= last) break;
+    }
+    if (committed) {
+        delta = d;
+        iteration = it;
+        ref = r;
+    }
+}
+
 struct FxOut {
     float* field;
     int32_t* counts;  // escape iteration, max_iter for interior, -1 for an unresolved (NaN) pixel
@@ -111,6 +182,10 @@ FAV_HD void pixel(const RenderParams& p, const FxOut& out, int px, int py) {
     // end of the stored orbit
     const int last = o.map_end > 0 && o.map_end < o.size ? o.map_end : o.size - 1;
     while (iteration < max_iter) {
+        if (!p.disable_bla) {
+            run_blocks(p, delta, dc, ref, iteration, last);
+            if (iteration >= max_iter) break;
+        }
         if (ref >= last) {  // continue against the orbit's start, Z_0 = 0
             delta = add(from_sc(core::load_z(o, ref)), delta);
             ref = 0;

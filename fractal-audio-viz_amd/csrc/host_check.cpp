@@ -374,6 +374,7 @@ extern "C" int fav_host_check_fx(const char* x, const char* y, const char* ref_z
         p.orbit.e = img.e.data();
         p.orbit.glitch_m = o.glitch_m.data();
         p.orbit.glitch_e = o.glitch_e.data();
+        p.orbit.fast = img.fast.data();
         p.series.re = img.series_re.data();
         p.series.im = img.series_im.data();
         p.series.e = img.series_e.data();
