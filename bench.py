@@ -104,6 +104,35 @@ def copy_calibration(mib, copies):
     return 0
 
 
+def floatexp_boundary_mismatch(lib):
+    """The floatexp path where escape counts are sensitive: the 640x360
+    zoom-1e12 field (cap 20000, every pixel near the boundary), against the
+    FP64 kernel's counts on the same inputs."""
+    work = fav_workloads.field_1e12_640x360()
+    opts = fav_hip.NativeRenderOptions(**work["options"])
+    ref = fav_hip.NativeReference(work["x"], work["y"], work["zoom"], work["max_iter"], work["bits"],
+                                  work["image_series_order"], opts)
+    w, h, cap = work["width"], work["height"], work["max_iter"]
+    fp64_field, planes = ref.render(w, h, work["zoom"], cap, work["series_order"], work["series_block"],
+                                    planes=True, options=opts)
+    oi = planes["orbit_iteration"].reshape(-1)
+    f64 = fp64_field.reshape(-1)
+    reference = np.where(np.isnan(f64), -1, np.where(oi >= cap, cap, oi + 1))
+    field = np.empty(w * h, dtype=np.float32)
+    counts = np.empty(w * h, dtype=np.int32)
+    status = lib.fractal_hip_render_reference_floatexp(
+        field.ctypes.data, counts.ctypes.data, w, h, work["zoom"].encode(), ref.handle, cap, work["series_order"],
+        work["series_block"], ctypes.byref(opts), 0)
+    if status != 0:
+        raise RuntimeError(lib.fractal_last_error().decode())
+    ref.close()
+    agree = (counts == reference) & ~np.isnan(field) & ~np.isnan(f64)
+    return {"workload": work["name"],
+            "pixels_differing_from_fp64_pct": round(float(np.mean(counts != reference)) * 100.0, 3),
+            "max_smooth_difference_where_counts_agree":
+                float(np.max(np.abs(field[agree].astype(np.float64) - f64[agree]))) if agree.any() else 0.0}
+
+
 def floatexp_leg(lib, hip, ref, work, opts, d_field, fp64_field, fp64_orbit_iteration, args):
     """The FP32 floatexp path on the same field: throughput over the same
     step count, and the fraction of pixels whose escape count differs from
@@ -155,8 +184,10 @@ def floatexp_leg(lib, hip, ref, work, opts, d_field, fp64_field, fp64_orbit_iter
     hip.synchronize()
     fp64_elapsed = (time.perf_counter() - started) / args.steps
     fp64_pixel_iterations = int(np.where(reference < 0, cap, reference).astype(np.int64).sum())
+    boundary = floatexp_boundary_mismatch(lib)
     return {"value": round(pixel_iterations / elapsed / 1e9, 3), "unit": "Gpixel-iter/s",
             "planes": False,
+            "boundary_field": boundary,
             "fp64_without_planes_value": round(fp64_pixel_iterations / fp64_elapsed / 1e9, 3),
             "ms_per_step": round(elapsed * 1e3, 3), "mean_kernel_ms": round(float(np.mean(kernel_ms)), 3),
             "pixels_differing_from_fp64_pct": round(float(np.mean(counts != reference)) * 100.0, 4),
