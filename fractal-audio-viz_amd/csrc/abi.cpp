@@ -134,6 +134,26 @@ void* register_reference(std::unique_ptr<ReferenceContext> context) {
     }
 }
 
+// batches share the token space (and counter) of reference handles
+std::unordered_map<uintptr_t, std::shared_ptr<fav::PreparedBatch>> batches;
+
+void* register_batch(std::shared_ptr<fav::PreparedBatch> batch) {
+    std::lock_guard<std::mutex> lock(registry_mutex);
+    for (;;) {
+        const uintptr_t token = next_handle++;
+        if (token == 0 || registry.count(token) || batches.count(token)) continue;
+        batches.emplace(token, std::move(batch));
+        return reinterpret_cast<void*>(token);
+    }
+}
+
+std::shared_ptr<fav::PreparedBatch> acquire_batch(void* handle) {
+    if (!handle) return {};
+    std::lock_guard<std::mutex> lock(registry_mutex);
+    auto it = batches.find(reinterpret_cast<uintptr_t>(handle));
+    return it == batches.end() ? nullptr : it->second;
+}
+
 std::shared_ptr<ReferenceContext> acquire(void* handle) {
     if (!handle) return {};
     std::lock_guard<std::mutex> lock(registry_mutex);
@@ -553,6 +573,99 @@ int render_fractal_ex_planes(float* output, int width, int height, const char* z
 }
 
 // ---- extensions (fav_extensions.h) ----------------------------------------
+
+void* fractal_hip_batch_create(const FractalHipBatchField* fields, int count, const FractalRenderOptions* options) {
+    try {
+        if (!fields || count <= 0 || count > 4095) throw std::runtime_error("invalid batch field list");
+        const FractalRenderOptions o = checked_options(options);
+        require_hip_backend(o);
+        std::vector<fav::BatchFieldRequest> list;
+        for (int i = 0; i < count; ++i) {
+            const FractalHipBatchField& f = fields[i];
+            if (!f.device_output || !f.zoom_text || !valid_dimensions(f.width, f.height)
+                || !valid_iterations(f.max_iter) || !valid_series(f.series_order, f.series_block)
+                || !valid_text(f.zoom_text))
+                throw std::runtime_error("invalid native render dimensions or handle");
+            auto context = acquire(f.handle);
+            if (!context) throw std::runtime_error("invalid or already-destroyed reference handle");
+            if (context->escape_radius_mode != o.escape_radius_mode)
+                throw std::runtime_error("reference escape-radius mode does not match render options");
+            if (f.max_iter > context->requested_max_iter)
+                throw std::runtime_error("render iteration count exceeds prepared reference");
+            const fav::FE z = fav::parse_zoom(f.zoom_text, context->precision_bits);
+            const long double log10_zoom =
+                (std::log(static_cast<long double>(z.m)) + static_cast<long double>(z.e) * 0.693147180559945309417232121458176568L)
+                / std::log(10.0L);
+            if (log10_zoom < 6.0L) throw std::runtime_error("batched fields are perturbation fields (zoom >= 1e6)");
+            fav::FieldRequest request{f.device_output, f.width, f.height, f.zoom_text, f.max_iter, f.series_order,
+                                      f.series_block, o, nullptr, nullptr, true};
+            list.push_back(fav::BatchFieldRequest{std::move(context), request});
+        }
+        void* handle = register_batch(fav::prepare_batch(std::move(list)));
+        set_error("");
+        return handle;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return nullptr;
+    } catch (...) {
+        set_error("native batch creation failed with an unknown exception");
+        return nullptr;
+    }
+}
+
+int fractal_hip_batch_tile_count(void* batch) {
+    const auto b = acquire_batch(batch);
+    if (!b) {
+        set_error("invalid or already-destroyed batch handle");
+        return -1;
+    }
+    set_error("");
+    return static_cast<int>(fav::batch_tile_count(*b));
+}
+
+int fractal_hip_batch_set_order(void* batch, int order, const float* tile_costs, int cost_count) {
+    try {
+        const auto b = acquire_batch(batch);
+        if (!b) throw std::runtime_error("invalid or already-destroyed batch handle");
+        fav::batch_set_order(*b, order, tile_costs, cost_count > 0 ? static_cast<size_t>(cost_count) : 0);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+
+int fractal_hip_batch_render(void* batch, int fast_blocks_per_cu, int use_graph) {
+    try {
+        const auto b = acquire_batch(batch);
+        if (!b) throw std::runtime_error("invalid or already-destroyed batch handle");
+        if (fast_blocks_per_cu < 0 || fast_blocks_per_cu > 64 || (use_graph != 0 && use_graph != 1))
+            throw std::runtime_error("invalid batch launch settings");
+        fav::batch_render(*b, fast_blocks_per_cu, use_graph != 0);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+
+void fractal_hip_batch_destroy(void* batch) {
+    if (!batch) return;
+    std::shared_ptr<fav::PreparedBatch> doomed;
+    {
+        std::lock_guard<std::mutex> lock(registry_mutex);
+        auto it = batches.find(reinterpret_cast<uintptr_t>(batch));
+        if (it == batches.end()) {
+            set_error("invalid or already-destroyed batch handle");
+            return;
+        }
+        doomed = std::move(it->second);
+        batches.erase(it);
+    }
+    set_error("");
+}
 
 int fractal_hip_render_reference_floatexp(float* output, int32_t* counts, int width, int height, const char* zoom,
                                           void* handle, int max_iter, int series_order, int series_block,

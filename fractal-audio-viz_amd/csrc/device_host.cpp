@@ -26,6 +26,7 @@ hipError_t launch_kfp_kernel(const kfp::KfpParams& params, void* scratch, int co
 hipError_t launch_direct_kernel(const direct::DirectParams& p, bool planes, hipStream_t stream);
 hipError_t launch_alt_kernel(const AltRenderParams& p, hipStream_t stream);
 hipError_t launch_fx_kernel(const RenderParams& p, const fx::FxOut& out, hipStream_t stream);
+hipError_t launch_batch(const BatchLaunch& q, int fast_blocks, int persistent_blocks, hipStream_t stream);
 
 namespace {
 
@@ -438,6 +439,190 @@ void render_alternate(ReferenceContext& c, const FieldRequest& r) {
         r.stats[0] = static_cast<uint64_t>(pixels);
         r.stats[22] = static_cast<uint64_t>(static_cast<double>(kernel_ms) * 1e6);
     }
+}
+
+// ---- batched fields --------------------------------------------------------
+
+class PreparedBatch {
+public:
+    std::vector<BatchFieldRequest> fields;
+    std::vector<RenderParams> params;      // host image of q.fields
+    std::vector<uint32_t> field_major;     // tile codes in field-major order
+    std::vector<uint32_t> order;           // tile codes in launch order
+    DeviceBuffer params_d, tiles_d, offsets_d, state_d, queues_d, counters_d;
+    BatchLaunch q{};
+    int fast_blocks_per_cu = 5;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_blocks = 0;
+    ~PreparedBatch() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (DeviceBuffer* b : {&params_d, &tiles_d, &offsets_d, &state_d, &queues_d, &counters_d})
+            if (b->ptr) (void)hipFree(b->ptr);
+    }
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fields) {
+    if (fields.empty()) throw std::runtime_error("empty batch");
+    if (fields.size() >= (1U << 12)) throw std::runtime_error("too many fields in one batch");
+    auto b = std::make_shared<PreparedBatch>();
+    std::vector<PreparedRender> preps(fields.size());
+    size_t pixels_total = 0, offsets_total = 0, state_total = 0;
+    for (size_t f = 0; f < fields.size(); ++f) {
+        const FieldRequest& r = fields[f].request;
+        if (!r.device_pointers || r.planes || r.stats) throw std::runtime_error("batch fields render to device pointers only");
+        if (fields[f].context->formula != 0) throw std::runtime_error("batched fields use the Mandelbrot formula");
+        const size_t tiles = static_cast<size_t>((r.width + 15) / 16) * static_cast<size_t>((r.height + 15) / 16);
+        if (tiles >= (1U << 20)) throw std::runtime_error("batch field too large");
+        prepare_render(*fields[f].context, r, preps[f]);
+        ensure_device_tables(*fields[f].context);
+        const size_t pixels = static_cast<size_t>(r.width) * static_cast<size_t>(r.height);
+        pixels_total += pixels;
+        offsets_total += (preps[f].offset_m.size() * 12 + 512 + 255) & ~size_t(255);
+        state_total += (pixels * kStateBytesPerPixel + 20 * 256 + 255) & ~size_t(255);
+        for (uint32_t t = 0; t < tiles; ++t) b->field_major.push_back((static_cast<uint32_t>(f) << 20) | t);
+    }
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    char* off_base = static_cast<char*>(b->offsets_d.get(offsets_total));
+    char* state_base = static_cast<char*>(b->state_d.get(state_total));
+    size_t off_at = 0, state_at = 0;
+    b->params.resize(fields.size());
+    for (size_t f = 0; f < fields.size(); ++f) {
+        const FieldRequest& r = fields[f].request;
+        const DeviceTables& t = *fields[f].context->device;
+        const PreparedRender& prep = preps[f];
+        RenderParams p = prep.params;
+        p.orbit = t.orbit;
+        const SeriesDev scalar_series = p.series;
+        p.series = t.series;
+        p.series.count = scalar_series.count;
+        p.series.iteration = scalar_series.iteration;
+        p.series.available = scalar_series.available;
+        p.series.radius2_m = scalar_series.radius2_m;
+        p.series.radius2_e = scalar_series.radius2_e;
+        p.linear = t.linear[prep.linear_tier];
+        p.linear_dir = t.linear_dir[prep.linear_tier];
+        p.cubic = t.cubic;
+        p.cubic_dir = t.cubic_dir;
+        char* doff = off_base + off_at;
+        const size_t exp_offset = (prep.offset_m.size() * 8 + 255) & ~size_t(255);
+        check(hipMemcpyAsync(doff, prep.offset_m.data(), prep.offset_m.size() * 8, hipMemcpyHostToDevice, s.stream),
+              "batch offsets");
+        check(hipMemcpyAsync(doff + exp_offset, prep.offset_e.data(), prep.offset_e.size() * 4,
+                             hipMemcpyHostToDevice, s.stream), "batch offsets");
+        off_at += (prep.offset_m.size() * 12 + 512 + 255) & ~size_t(255);
+        p.x_off_m = reinterpret_cast<const double*>(doff);
+        p.y_off_m = p.x_off_m + r.width;
+        p.x_off_e = reinterpret_cast<const int32_t*>(doff + exp_offset);
+        p.y_off_e = p.x_off_e + r.width;
+        p.output = r.output;
+        const size_t pixels = static_cast<size_t>(r.width) * static_cast<size_t>(r.height);
+        bind_state_store(p, state_base + state_at, pixels);
+        state_at += (pixels * kStateBytesPerPixel + 20 * 256 + 255) & ~size_t(255);
+        p.budget_ticks = 0;  // no time budget in a batch
+        b->params[f] = p;
+    }
+    void* params_d = b->params_d.get(b->params.size() * sizeof(RenderParams));
+    check(hipMemcpyAsync(params_d, b->params.data(), b->params.size() * sizeof(RenderParams), hipMemcpyHostToDevice,
+                         s.stream), "batch params");
+    void* queues = b->queues_d.get(pixels_total * 16);
+    void* counters = b->counters_d.get(256);
+    b->q.fields = static_cast<const RenderParams*>(params_d);
+    b->q.field_count = static_cast<unsigned>(fields.size());
+    b->q.general_queue = static_cast<unsigned long long*>(queues);
+    b->q.tail_queue = static_cast<unsigned long long*>(queues) + pixels_total;
+    b->q.counters = static_cast<unsigned*>(counters);
+    b->fields = std::move(fields);
+    b->order = b->field_major;
+    void* tiles = b->tiles_d.get(b->order.size() * 4);
+    check(hipMemcpyAsync(tiles, b->order.data(), b->order.size() * 4, hipMemcpyHostToDevice, s.stream), "tiles");
+    b->q.tiles = static_cast<const uint32_t*>(tiles);
+    b->q.tile_count = static_cast<unsigned>(b->order.size());
+    check(hipStreamSynchronize(s.stream), "batch upload");
+    return b;
+}
+
+void batch_set_order(PreparedBatch& b, int order, const float* costs, size_t cost_count) {
+    std::vector<uint32_t> next;
+    next.reserve(b.field_major.size());
+    if (order == 0) {
+        next = b.field_major;
+    } else if (order == 1) {  // round-robin over fields, one tile each
+        std::vector<std::vector<uint32_t>> per(b.fields.size());
+        for (uint32_t code : b.field_major) per[code >> 20].push_back(code);
+        for (size_t i = 0;; ++i) {
+            bool any = false;
+            for (auto& v : per)
+                if (i < v.size()) {
+                    next.push_back(v[i]);
+                    any = true;
+                }
+            if (!any) break;
+        }
+    } else if (order == 2) {
+        if (!costs || cost_count != b.field_major.size())
+            throw std::runtime_error("cost order needs one cost per tile");
+        std::vector<size_t> idx(b.field_major.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return costs[x] > costs[y]; });
+        for (size_t i : idx) next.push_back(b.field_major[i]);
+    } else {
+        throw std::runtime_error("unknown batch tile order");
+    }
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    b.order = std::move(next);
+    check(hipMemcpyAsync(b.tiles_d.ptr, b.order.data(), b.order.size() * 4, hipMemcpyHostToDevice, s.stream),
+          "tiles");
+    check(hipStreamSynchronize(s.stream), "tiles");
+}
+
+size_t batch_tile_count(const PreparedBatch& b) { return b.field_major.size(); }
+
+void batch_render(PreparedBatch& b, int fast_blocks_per_cu, bool use_graph) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    if (fast_blocks_per_cu > 0) b.fast_blocks_per_cu = fast_blocks_per_cu;
+    const int fast_blocks = s.compute_units * b.fast_blocks_per_cu;
+    const int persistent_blocks = s.compute_units * 8;
+    const uint64_t wall0 = now_ns();
+    if (use_graph) {
+        if (!b.graph_exec || b.graph_blocks != fast_blocks) {
+            b.drop_graph();
+            check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "graph capture");
+            const hipError_t launched = launch_batch(b.q, fast_blocks, persistent_blocks, s.stream);
+            const hipError_t ended = hipStreamEndCapture(s.stream, &b.graph);
+            check(launched, "batch launch (capture)");
+            check(ended, "graph capture end");
+            check(hipGraphInstantiate(&b.graph_exec, b.graph, nullptr, nullptr, 0), "graph instantiate");
+            b.graph_blocks = fast_blocks;
+        }
+        check(hipEventRecord(s.start, s.stream), "event");
+        check(hipGraphLaunch(b.graph_exec, s.stream), "graph launch");
+        check(hipEventRecord(s.stop, s.stream), "event");
+    } else {
+        check(hipEventRecord(s.start, s.stream), "event");
+        check(launch_batch(b.q, fast_blocks, persistent_blocks, s.stream), "batch launch");
+        check(hipEventRecord(s.stop, s.stream), "event");
+    }
+    check(hipStreamSynchronize(s.stream), "batch render");
+    float kernel_ms = 0.0f;
+    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    size_t pixels = 0;
+    for (const auto& f : b.fields) pixels += static_cast<size_t>(f.request.width) * f.request.height;
+    s.timing.kernel_ms = kernel_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
 }
 
 void render_direct_on_device(const direct::DirectParams& host, const double* x, const double* y, bool planes,

@@ -201,4 +201,15 @@ struct RenderParams {
     int32_t fast_linear;  // linear-BLA fast kernel on (off with cycle checks or a time budget)
 };
 
+// One batched launch over several fields (batch_kernel.hip).
+struct BatchLaunch {
+    const RenderParams* fields;  // field_count parameter sets in HBM
+    const uint32_t* tiles;       // tile_count codes: field << 20 | tile index (row-major 16x16 tiles)
+    unsigned tile_count;
+    unsigned field_count;
+    unsigned long long* general_queue;  // (field << 32 | pixel) hand-offs, one slot per batch pixel
+    unsigned long long* tail_queue;
+    unsigned* counters;  // [0] tile cursor, [1]/[2] general count/cursor, [3]/[4] tail count/cursor
+};
+
 }  // namespace fav

@@ -2,6 +2,9 @@
 // device launch code (device_host.cpp).
 #pragma once
 
+#include <memory>
+#include <vector>
+
 #include <cstdint>
 
 #include "../../include/fractal_hip.h"
@@ -34,6 +37,24 @@ void render_field(ReferenceContext& context, const FieldRequest& request);
 // `counts` (escape iteration per pixel, max_iter inside, -1 unresolved) are
 // host or device pointers as request.device_pointers says; no planes.
 void render_field_floatexp(ReferenceContext& context, const FieldRequest& request, int32_t* counts);
+
+// A prepared batch of Mandelbrot fields rendered by one launch sequence
+// (batch_kernel.hip).  Owns the device copies of every field's launch
+// parameters, offsets, state stores and the shared queues; the references
+// stay alive while the batch does.
+struct BatchFieldRequest {
+    std::shared_ptr<ReferenceContext> context;
+    FieldRequest request;  // device_pointers output, no planes, no stats
+};
+
+class PreparedBatch;
+std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fields);
+// tile order: 0 field-major, 1 fields interleaved tile by tile, 2 by the
+// given per-tile cost, heaviest first (costs in field-major tile order)
+void batch_set_order(PreparedBatch& batch, int order, const float* costs, size_t cost_count);
+size_t batch_tile_count(const PreparedBatch& batch);
+// one render; fast_blocks_per_cu <= 0 keeps the current setting
+void batch_render(PreparedBatch& batch, int fast_blocks_per_cu, bool use_graph);
 
 // Alternate-formula field (Julia, burning ship, tricorn) of a reference
 // handle; request.planes must be null.

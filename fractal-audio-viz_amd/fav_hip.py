@@ -154,6 +154,16 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p,
         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(NativeRenderOptions), ctypes.c_int]
     lib.fractal_hip_render_reference_floatexp.restype = ctypes.c_int
+    lib.fractal_hip_batch_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(NativeRenderOptions)]
+    lib.fractal_hip_batch_create.restype = ctypes.c_void_p
+    lib.fractal_hip_batch_tile_count.argtypes = [ctypes.c_void_p]
+    lib.fractal_hip_batch_tile_count.restype = ctypes.c_int
+    lib.fractal_hip_batch_set_order.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+    lib.fractal_hip_batch_set_order.restype = ctypes.c_int
+    lib.fractal_hip_batch_render.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    lib.fractal_hip_batch_render.restype = ctypes.c_int
+    lib.fractal_hip_batch_destroy.argtypes = [ctypes.c_void_p]
+    lib.fractal_hip_batch_destroy.restype = None
     lib.fractal_hip_last_timing.argtypes = [_c_double_p, ctypes.c_int]
     lib.fractal_hip_last_timing.restype = ctypes.c_int
     lib.fractal_hip_reference_upload.argtypes = [ctypes.c_void_p]
@@ -233,6 +243,16 @@ class NativeReference:
 
     def __del__(self):
         self.close()
+
+
+class NativeBatchField(ctypes.Structure):
+    """FractalHipBatchField (include/fractal_hip.h)."""
+
+    _fields_ = [
+        ("handle", ctypes.c_void_p), ("zoom_text", ctypes.c_char_p), ("device_output", ctypes.c_void_p),
+        ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("max_iter", ctypes.c_int32),
+        ("series_order", ctypes.c_int32), ("series_block", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
 
 
 class HipRuntime:

@@ -301,6 +301,39 @@ int fractal_hip_render_reference_floatexp(
     int series_block, const FractalRenderOptions *options,
     int device_pointers);
 
+/* ---- Batched fields (no counterpart in the reference) ------------------
+ * Several Mandelbrot fields with their own reference handles, sizes, zooms
+ * and caps rendered by one launch sequence over a shared device tile queue
+ * (the keyframe fields of one atlas segment).  Each field's output is
+ * bit-identical to its single-field render; outputs are device pointers;
+ * zooms must be >= 1e6 (perturbation fields). */
+typedef struct FractalHipBatchField {
+    void *handle;
+    const char *zoom_text;
+    float *device_output;
+    int32_t width;
+    int32_t height;
+    int32_t max_iter;
+    int32_t series_order;
+    int32_t series_block;
+    int32_t reserved;
+} FractalHipBatchField;
+
+/* Uploads every field's launch parameters, offsets and state storage; the
+ * handles stay referenced until the batch is destroyed. */
+void *fractal_hip_batch_create(const FractalHipBatchField *fields, int count,
+                               const FractalRenderOptions *options);
+/* Number of 16x16 tiles over all fields (field-major order). */
+int fractal_hip_batch_tile_count(void *batch);
+/* Tile order: 0 field-major, 1 fields interleaved tile by tile, 2 heaviest
+ * first by tile_costs (cost_count = tile count, field-major order). */
+int fractal_hip_batch_set_order(void *batch, int order, const float *tile_costs,
+                                int cost_count);
+/* One render of the batch; fast_blocks_per_cu 0 keeps the current grid,
+ * use_graph 1 replays a captured HIP graph of the launch sequence. */
+int fractal_hip_batch_render(void *batch, int fast_blocks_per_cu, int use_graph);
+void fractal_hip_batch_destroy(void *batch);
+
 /* fractal_colourise_kfp on device pointers: device_field (W*H floats) and
  * device_rgb (W*H*3 bytes) already live in HBM; lut is a host pointer. */
 int fractal_hip_colourise_kfp_device(

@@ -255,3 +255,29 @@ def test_alternate_rejects_bad_julia_constant(product):
     handle = product.fractal_create_reference_ex_options(b"0", b"0", b"1e7", 500, 128, 8, 1, b"x", b"0",
                                                          ctypes.byref(opts))
     assert not handle
+
+
+def test_atlas_segment_workload_rules():
+    """Keyframe zooms 10^(100k/47), caps 2000 + 1000/decade clamped to
+    [64, 100000], sizes from the three frame sizes, shallow fields dropped."""
+    import fav_batch
+    fields = fav_batch.segment_fields()
+    assert len(fields) == 45  # k = 0, 1, 2 are below 1e6
+    for f in fields:
+        assert f["zoom_log10"] >= 6.0
+        assert f["max_iter"] == min(100000, max(64, int(2000 + 1000 * f["zoom_log10"])))
+        assert (f["width"], f["height"]) in fav_batch.SIZES
+        assert float(f["zoom"]) == pytest.approx(10.0 ** f["zoom_log10"], rel=1e-9)
+    assert fields[-1]["zoom_log10"] == 100.0
+
+
+def test_batch_creation_validates_without_device(product):
+    import fav_hip
+    lib = fav_hip.load_library()
+    entries = (fav_hip.NativeBatchField * 1)()
+    entries[0].handle = None
+    entries[0].zoom_text = b"1e10"
+    entries[0].device_output = None
+    assert not lib.fractal_hip_batch_create(entries, 1, ctypes.byref(fav_hip.NativeRenderOptions()))
+    assert not lib.fractal_hip_batch_create(entries, 0, ctypes.byref(fav_hip.NativeRenderOptions()))
+    assert lib.fractal_hip_batch_tile_count(ctypes.c_void_p(0x7777)) == -1
