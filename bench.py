@@ -27,6 +27,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "fractal-audio-viz_amd"))
 
+import fav_batch  # noqa: E402
 import fav_hip  # noqa: E402
 import fav_palette  # noqa: E402
 import fav_workloads  # noqa: E402
@@ -102,6 +103,41 @@ def copy_calibration(mib, copies):
     hip.free(dst)
     print(json.dumps({"copy_calibration_mib": mib, "copies": copies}))
     return 0
+
+
+def batch_leg(lib, hip, steps):
+    """One atlas segment (45 keyframe fields from zoom 1e6.4 to 1e100, each
+    with its own 100-digit-centre reference, sizes drawn from 720p/1080p/4K,
+    caps 2000 + 1000/decade up to 100000) rendered as one autotuned,
+    graph-captured batch against one launch sequence per field."""
+    fields = fav_batch.segment_fields()
+    batch = fav_batch.Batch(fields, library=lib, hip=hip, threads=16)
+    try:
+        # pixel-iterations from the per-field statistics (logical iterations
+        # plus the first one of every pixel)
+        lib.fractal_set_stats_enabled(1)
+        pixel_iterations = 0
+        stats = np.zeros(39, dtype=np.uint64)
+        for i, f in enumerate(fields):
+            status = lib.fractal_hip_render_reference_device(
+                batch.outputs[i], f["width"], f["height"], f["zoom"].encode(), batch.references[i].handle,
+                f["max_iter"], 3, 256, ctypes.byref(batch.options), None)
+            if status != 0:
+                raise RuntimeError(lib.fractal_last_error().decode())
+            lib.fractal_get_last_stats_ex(stats.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 39)
+            pixel_iterations += int(stats[1]) + int(stats[0])
+        lib.fractal_set_stats_enabled(0)
+        per_field_ms = min(batch.render_per_field() for _ in range(max(1, steps)))
+        trials, best = fav_batch.autotune(batch, repeats=1)
+        batched_ms = min(batch.render(0, graph=True) for _ in range(max(1, steps)))
+        return {"workload": f"atlas segment: {len(fields)} fields, {batch.pixels() / 1e6:.1f} Mpx, zoom 1e6.4..1e100",
+                "value": round(pixel_iterations / (batched_ms * 1e-3) / 1e9, 3), "unit": "Gpixel-iter/s",
+                "batched_ms": round(batched_ms, 3), "per_field_launches_ms": round(per_field_ms, 3),
+                "speedup_vs_per_field": round(per_field_ms / batched_ms, 3),
+                "pixel_iterations": pixel_iterations, "reference_build_s": round(batch.reference_build_s, 3),
+                "autotuned": best, "trials": len(trials)}
+    finally:
+        batch.close()
 
 
 def floatexp_boundary_mismatch(lib):
@@ -200,6 +236,7 @@ def main():
     parser.add_argument("--steps", type=int, default=10)
     parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-batch", action="store_true", help="skip the batched atlas-segment leg")
     parser.add_argument("--copy-calibration-mib", type=int, default=0,
                         help="only run N device-to-device copies of this many MiB (HBM counter calibration)")
     args = parser.parse_args()
@@ -281,6 +318,7 @@ def main():
     rgb = np.empty(n * 3, dtype=np.uint8)
     hip.to_host(rgb, d_rgb)
     floatexp = floatexp_leg(lib, hip, ref, work, opts, d_field, field, orbit_iteration, args)
+    batched = None if args.no_batch else batch_leg(lib, hip, 2)
     hip.free(d_field)
     hip.free(d_planes_block)
     hip.free(d_rgb)
@@ -317,6 +355,7 @@ def main():
                      "traffic": measured_traffic_bytes()},
         "cpu_baseline": baseline,
         "floatexp": floatexp,
+        "batched": batched,
     }
     print(json.dumps(result))
     return 0
