@@ -30,6 +30,7 @@ sys.path.insert(0, str(ROOT / "fractal-audio-viz_amd"))
 import fav_batch  # noqa: E402
 import fav_hip  # noqa: E402
 import fav_palette  # noqa: E402
+import fav_sched  # noqa: E402
 import fav_workloads  # noqa: E402
 
 MI355X_FP64_VECTOR_PEAK_TFLOPS = 78.6
@@ -140,6 +141,44 @@ def batch_leg(lib, hip, steps):
         batch.close()
 
 
+def learned_leg(lib, repeats=2):
+    """The learned tile scheduler: a cost model trained on the GPU over the
+    seeded training set (12 Julia fields, 1920x1080, cap 20000), then Julia
+    fields at 3840x2160, zoom 1e100, cap 200000 (evaluation seeds 1, 2, 4,
+    disjoint from training) rendered in the plain grid and in the learned
+    heaviest-first order.  Net gain counts the preview, features and model
+    time; prediction error is against each field's measured tile costs."""
+    model, training = fav_sched.train_model(fav_sched.training_fields(), library=lib)
+    fields = []
+    for seed in (1, 2, 4):
+        case = fav_workloads.julia_field(seed=seed, width=3840, height=2160, zoom="1e100", max_iter=200000)
+        w, h, z, n = case["width"], case["height"], case["zoom"], case["max_iter"]
+        ref = fav_sched.reference_for(case, lib)
+        try:
+            _, work = ref.render_scheduled(w, h, z, n, work=True)
+            costs = fav_sched.tile_costs(work, w, h)
+            features, _ = fav_sched.preview_features(ref, w, h, z, n)
+            report = fav_sched.prediction_report(model.predict(features), costs)
+            plain = []
+            for _ in range(repeats):
+                ref.render_scheduled(w, h, z, n)
+                plain.append(fav_hip.last_timing()["kernel_ms"])
+            learned = [fav_sched.render_learned(ref, model, w, h, z, n)[2] for _ in range(repeats)]
+            best = min(learned, key=lambda t: t["preview_ms"] + t["model_ms"] + t["render_ms"])
+            total = best["preview_ms"] + best["model_ms"] + best["render_ms"]
+            fields.append({"seed": seed, "julia_c": list(case["julia"]), "plain_ms": round(min(plain), 3),
+                           "learned_total_ms": round(total, 3), "preview_ms": round(best["preview_ms"], 3),
+                           "model_ms": round(best["model_ms"], 3), "ordered_render_ms": round(best["render_ms"], 3),
+                           "net_gain_pct": round(100.0 * (min(plain) - total) / min(plain), 2),
+                           "prediction": {k: round(v, 4) if isinstance(v, float) else v for k, v in report.items()}})
+        finally:
+            ref.close()
+    return {"workload": "Julia at its repelling fixed point, 3840x2160, zoom 1e100, cap 200000, seeds 1 2 4",
+            "preview": f"quarter size, {fav_sched.PREVIEW_WORK_LIMIT} passes per pixel",
+            "training": {k: round(v, 4) if isinstance(v, float) else v for k, v in training.items()},
+            "mean_net_gain_pct": round(float(np.mean([f["net_gain_pct"] for f in fields])), 2), "fields": fields}
+
+
 def floatexp_boundary_mismatch(lib):
     """The floatexp path where escape counts are sensitive: the 640x360
     zoom-1e12 field (cap 20000, every pixel near the boundary), against the
@@ -237,6 +276,7 @@ def main():
     parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-batch", action="store_true", help="skip the batched atlas-segment leg")
+    parser.add_argument("--no-learned", action="store_true", help="skip the learned tile-scheduler leg")
     parser.add_argument("--copy-calibration-mib", type=int, default=0,
                         help="only run N device-to-device copies of this many MiB (HBM counter calibration)")
     args = parser.parse_args()
@@ -319,6 +359,7 @@ def main():
     hip.to_host(rgb, d_rgb)
     floatexp = floatexp_leg(lib, hip, ref, work, opts, d_field, field, orbit_iteration, args)
     batched = None if args.no_batch else batch_leg(lib, hip, 2)
+    learned = None if args.no_learned else learned_leg(lib)
     hip.free(d_field)
     hip.free(d_planes_block)
     hip.free(d_rgb)
@@ -356,6 +397,7 @@ def main():
         "cpu_baseline": baseline,
         "floatexp": floatexp,
         "batched": batched,
+        "learned_schedule": learned,
     }
     print(json.dumps(result))
     return 0

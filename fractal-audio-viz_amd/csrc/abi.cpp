@@ -8,6 +8,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <locale>
 #include <memory>
 #include <mutex>
@@ -19,6 +20,7 @@
 #include "../../include/fractal_hip.h"
 #include "fav_host.h"
 #include "fav_render.h"
+#include "sched_mlp.h"
 #include "kfp_prepare.h"
 #include "direct_prepare.h"
 
@@ -574,6 +576,104 @@ int render_fractal_ex_planes(float* output, int width, int height, const char* z
 
 // ---- extensions (fav_extensions.h) ----------------------------------------
 
+}  // extern "C"
+
+namespace {
+
+constexpr int kMaxPreviewWork = 1 << 24;
+
+// Checks shared by the scheduled, preview and learned entries: an
+// alternate-formula handle, zoom >= 1e6 (perturbation fields), a prepared
+// cap and matching escape mode.
+std::shared_ptr<ReferenceContext> alternate_context(void* handle, const char* zoom, int width, int height,
+                                                    int max_iter, int series_order, int series_block,
+                                                    const FractalRenderOptions& o) {
+    if (!zoom || !handle || !valid_dimensions(width, height) || !valid_iterations(max_iter)
+        || !valid_series(series_order, series_block))
+        throw std::runtime_error("invalid native render dimensions or handle");
+    require_hip_backend(o);
+    auto context = acquire(handle);
+    if (!context) throw std::runtime_error("invalid or already-destroyed reference handle");
+    if (context->formula == 0)
+        throw std::runtime_error("scheduled single-field renders are for the alternate formulas; "
+                                 "Mandelbrot fields take a tile order through a batch");
+    if (context->escape_radius_mode != o.escape_radius_mode)
+        throw std::runtime_error("reference escape-radius mode does not match render options");
+    if (max_iter > context->requested_max_iter)
+        throw std::runtime_error("render iteration count exceeds prepared reference");
+    if (!valid_text(zoom)) throw std::runtime_error("native zoom text is too long or null");
+    const fav::FE z = fav::parse_zoom(zoom, context->precision_bits);
+    const long double log10_zoom =
+        (std::log(static_cast<long double>(z.m)) + static_cast<long double>(z.e) * 0.693147180559945309417232121458176568L)
+        / std::log(10.0L);
+    if (log10_zoom < 6.0L) throw std::runtime_error("scheduled renders are perturbation fields (zoom >= 1e6)");
+    return context;
+}
+
+int guarded(const char* what, const std::function<void()>& body) {
+    try {
+        body();
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error(std::string(what) + " failed with an unknown exception");
+        return 1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fractal_hip_render_reference_scheduled(float* output, int width, int height, const char* zoom, void* handle,
+                                           int max_iter, int series_order, int series_block,
+                                           const FractalRenderOptions* options, const uint32_t* tile_order,
+                                           int tile_count, int32_t* work) {
+    return guarded("native scheduled render", [&] {
+        if (!output || tile_count < 0 || (tile_order && tile_count == 0))
+            throw std::runtime_error("invalid native render dimensions or handle");
+        const FractalRenderOptions o = checked_options(options);
+        const auto context = alternate_context(handle, zoom, width, height, max_iter, series_order, series_block, o);
+        fav::FieldRequest request{output, width, height, zoom, max_iter, series_order, series_block, o,
+                                  nullptr, nullptr, false};
+        fav::render_alternate(*context, request, tile_order, static_cast<size_t>(tile_count), work);
+    });
+}
+
+int fractal_hip_preview_features(float* features, int width, int height, const char* zoom, void* handle,
+                                 int max_iter, int series_order, int series_block,
+                                 const FractalRenderOptions* options, int work_limit) {
+    return guarded("native preview", [&] {
+        if (!features) throw std::runtime_error("feature buffer must not be null");
+        if (work_limit <= 0 || work_limit > kMaxPreviewWork)
+            throw std::runtime_error("preview work limit must be in [1, 16777216]");
+        const FractalRenderOptions o = checked_options(options);
+        const auto context = alternate_context(handle, zoom, width, height, max_iter, series_order, series_block, o);
+        fav::FieldRequest request{nullptr, width, height, zoom, max_iter, series_order, series_block, o,
+                                  nullptr, nullptr, false};
+        fav::alternate_preview_features(*context, request, work_limit, features);
+    });
+}
+
+int fractal_hip_render_reference_learned(float* output, int width, int height, const char* zoom, void* handle,
+                                         int max_iter, int series_order, int series_block,
+                                         const FractalRenderOptions* options, const float* weights, int work_limit,
+                                         uint32_t* tile_order, double* timings) {
+    return guarded("native learned render", [&] {
+        if (!output || !weights) throw std::runtime_error("output and weights must not be null");
+        if (work_limit <= 0 || work_limit > kMaxPreviewWork)
+            throw std::runtime_error("preview work limit must be in [1, 16777216]");
+        const FractalRenderOptions o = checked_options(options);
+        const auto context = alternate_context(handle, zoom, width, height, max_iter, series_order, series_block, o);
+        fav::FieldRequest request{output, width, height, zoom, max_iter, series_order, series_block, o,
+                                  nullptr, nullptr, false};
+        fav::render_alternate_learned(*context, request, weights, work_limit, tile_order, timings);
+    });
+}
+
 void* fractal_hip_batch_create(const FractalHipBatchField* fields, int count, const FractalRenderOptions* options) {
     try {
         if (!fields || count <= 0 || count > 4095) throw std::runtime_error("invalid batch field list");
@@ -696,6 +796,55 @@ int fractal_hip_render_reference_floatexp(float* output, int32_t* counts, int wi
         set_error("native floatexp render failed with an unknown exception");
         return 1;
     }
+}
+
+// ---- learned tile-cost model (no counterpart in the reference) ----
+
+namespace {
+
+constexpr int kMaxMlpSamples = 1 << 22;
+
+void check_samples(int count) {
+    if (count <= 0 || count > kMaxMlpSamples)
+        throw std::runtime_error("cost model sample count must be in [1, 4194304]");
+}
+
+}  // namespace
+
+int fractal_hip_cost_model_parameter_count(void) { return fav::mlp::kParams; }
+
+int fractal_hip_cost_model_forward(const float* weights, const float* features, int count, float* costs) {
+    return guarded("cost model forward", [&] {
+        if (!weights || !features || !costs) throw std::runtime_error("cost model buffers must not be null");
+        check_samples(count);
+        fav::mlp_forward(weights, features, count, costs);
+    });
+}
+
+int fractal_hip_cost_model_gradient(const float* weights, const float* features, const float* targets, int count,
+                                    float* gradient, float* loss) {
+    return guarded("cost model gradient", [&] {
+        if (!weights || !features || !targets || !gradient || !loss)
+            throw std::runtime_error("cost model buffers must not be null");
+        check_samples(count);
+        fav::mlp_gradient(weights, features, targets, count, gradient, loss);
+    });
+}
+
+int fractal_hip_cost_model_train(float* weights, float* adam_m, float* adam_v, const float* features,
+                                 const float* targets, int count, int steps, float learning_rate, int first_step,
+                                 float* loss_history) {
+    return guarded("cost model training", [&] {
+        if (!weights || !adam_m || !adam_v || !features || !targets || !loss_history)
+            throw std::runtime_error("cost model buffers must not be null");
+        check_samples(count);
+        if (steps <= 0 || steps > 100000) throw std::runtime_error("training steps must be in [1, 100000]");
+        if (first_step < 1 || first_step > 100000000) throw std::runtime_error("first Adam step must be >= 1");
+        if (!(learning_rate > 0.0F) || !(learning_rate < 1.0F))
+            throw std::runtime_error("learning rate must be in (0, 1)");
+        fav::mlp_train(weights, adam_m, adam_v, features, targets, count, steps, learning_rate, first_step,
+                       loss_history);
+    });
 }
 
 int fractal_hip_render_reference_device(float* device_output, int width, int height, const char* zoom, void* handle,

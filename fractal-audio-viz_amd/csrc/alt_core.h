@@ -134,8 +134,9 @@ FAV_HD SC pixel_dc(const AltRenderParams& p, int px, int py) {
     return sc_from_fe(FE{p.x_off_m[px], p.x_off_e[px]}, FE{p.y_off_m[py], p.y_off_e[py]});
 }
 
-// One pixel of render_alternate_reference_impl; writes p.output[index].
-FAV_HD void pixel(const AltRenderParams& p, int px, int py) {
+// One pixel of render_alternate_reference_impl; writes p.output[index] and
+// counts the loop passes (blocks, exact and replayed steps) in `work`.
+FAV_HD void pixel_body(const AltRenderParams& p, int px, int py, int& work) {
     const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + static_cast<size_t>(px);
     const bool julia = p.formula == 1;
     const int mode = p.escape_radius_mode;
@@ -163,6 +164,11 @@ FAV_HD void pixel(const AltRenderParams& p, int px, int py) {
     int cycle_power = 1, cycle_length = 0, cycle_hits = 0;
     bool cycle_ready = false;
     while (iteration < p.max_iter) {
+        if (p.work_limit > 0 && work >= p.work_limit) {
+            p.output[index] = __builtin_nanf("");
+            return;
+        }
+        ++work;
         if (budget.expired()) {
             p.output[index] = __builtin_nanf("");
             return;
@@ -202,6 +208,11 @@ FAV_HD void pixel(const AltRenderParams& p, int px, int py) {
             }
             // replay the block one exact step at a time
             for (int replay = 0; replay < step.length && iteration < p.max_iter; ++replay) {
+                if (p.work_limit > 0 && work >= p.work_limit) {
+                    p.output[index] = __builtin_nanf("");
+                    return;
+                }
+                ++work;
                 delta = delta_step(p, ref, delta, pdelta);
                 ++ref;
                 ++iteration;
@@ -260,6 +271,12 @@ FAV_HD void pixel(const AltRenderParams& p, int px, int py) {
         }
     }
     p.output[index] = core::encode_iteration(p.max_iter, bias);
+}
+
+FAV_HD void pixel(const AltRenderParams& p, int px, int py) {
+    int work = 0;
+    pixel_body(p, px, py, work);
+    if (p.work) p.work[static_cast<size_t>(py) * static_cast<size_t>(p.width) + static_cast<size_t>(px)] = work;
 }
 
 }  // namespace alt

@@ -14,8 +14,18 @@ namespace {
 __global__ void __launch_bounds__(256) alt_kernel(AltRenderParams p) {
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (p.tile_order) {
+        // blocks in launch order cover the 32x32 tiles of tile_order, four
+        // 16x16 quarters each; the hardware dispatches blocks in index order
+        const uint32_t tile = p.tile_order[blockIdx.x >> 2];
+        const int tiles_x = (p.width + 31) / 32;
+        const int quarter = blockIdx.x & 3;
+        bx = static_cast<int>(tile % static_cast<uint32_t>(tiles_x)) * 2 + (quarter & 1);
+        by = static_cast<int>(tile / static_cast<uint32_t>(tiles_x)) * 2 + (quarter >> 1);
+    }
+    const int px = bx * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = by * 16 + (wave >> 1) * 8 + (lane >> 3);
     if (px < p.width && py < p.height) alt::pixel(p, px, py);
 }
 
@@ -27,11 +37,15 @@ __global__ void alt_prologue(unsigned long long* budget_start, int* budget_flag)
 }  // namespace
 
 // Shapes validated by the caller: offsets hold width / height entries and
-// output width*height floats; the orbit, margins and BLA steps are the
-// reference's device tables.
+// output (and work) width*height entries; tile_order, when set, is a
+// permutation of the ceil(w/32)*ceil(h/32) tiles; the orbit, margins and BLA
+// steps are the reference's device tables.
 hipError_t launch_alt_kernel(const AltRenderParams& p, hipStream_t stream) {
     alt_prologue<<<1, 1, 0, stream>>>(const_cast<unsigned long long*>(p.budget_start), p.budget_flag);
-    alt_kernel<<<dim3((p.width + 15) / 16, (p.height + 15) / 16), dim3(256), 0, stream>>>(p);
+    if (p.tile_order)
+        alt_kernel<<<dim3(static_cast<unsigned>(p.tile_count) * 4U), dim3(256), 0, stream>>>(p);
+    else
+        alt_kernel<<<dim3((p.width + 15) / 16, (p.height + 15) / 16), dim3(256), 0, stream>>>(p);
     return hipGetLastError();
 }
 

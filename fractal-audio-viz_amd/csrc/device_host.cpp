@@ -4,6 +4,7 @@
 // timing and the copy of results back to the caller.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -17,6 +18,7 @@
 #include "kfp_core.h"
 #include "direct_core.h"
 #include "fx_core.h"
+#include "sched_mlp.h"
 
 namespace fav {
 
@@ -65,7 +67,10 @@ struct LaunchState {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords, fx_counts;
+    hipEvent_t mark[3] = {nullptr, nullptr, nullptr};
+    DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords, fx_counts, tile_order, work,
+        mlp_y, mlp_act, mlp_delta, mlp_err2, mlp_partial, mlp_grad, mlp_features, mlp_weights, preview_offsets,
+        preview_output, preview_work;
     int compute_units = 256;
     double wall_clock_khz = 0.0;
     LastTiming timing{};
@@ -85,6 +90,7 @@ void init_locked(LaunchState& s) {
     check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipEventCreate(&s.start), "hipEventCreate");
     check(hipEventCreate(&s.stop), "hipEventCreate");
+    for (hipEvent_t& e : s.mark) check(hipEventCreate(&e), "hipEventCreate");
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s.device) != hipSuccess || khz <= 0)
         khz = 100000;  // 100 MHz constant clock of gfx9 parts
@@ -379,10 +385,23 @@ void render_field_floatexp(ReferenceContext& c, const FieldRequest& r, int32_t* 
     s.timing.pixels = static_cast<double>(pixels);
 }
 
-void render_alternate(ReferenceContext& c, const FieldRequest& r) {
-    PreparedAlternate prep;
-    prepare_alternate(c, r, prep);
-    ensure_device_tables(c);
+namespace {
+
+void check_tile_order(const uint32_t* tile_order, size_t tile_count, int width, int height) {
+    const size_t tiles = static_cast<size_t>((width + 31) / 32) * static_cast<size_t>((height + 31) / 32);
+    if (tile_count != tiles) throw std::runtime_error("tile order must list every 32x32 tile once");
+    std::vector<uint32_t> sorted(tile_order, tile_order + tile_count);
+    std::sort(sorted.begin(), sorted.end());
+    for (size_t i = 0; i < tiles; ++i)
+        if (sorted[i] != i) throw std::runtime_error("tile order must list every 32x32 tile once");
+}
+
+// Launch parameters of one alternate-formula render with the reference's
+// device tables bound and the per-column / per-row offsets uploaded to
+// `offsets` on the library stream (lock held; `prep` must stay alive until
+// the stream has run the copy).
+AltRenderParams bind_alternate(const ReferenceContext& c, const PreparedAlternate& prep, LaunchState& s,
+                               DeviceBuffer& offsets) {
     const DeviceTables& t = *c.device;
     AltRenderParams p = prep.params;
     p.re = t.orbit.re;
@@ -398,39 +417,104 @@ void render_alternate(ReferenceContext& c, const FieldRequest& r) {
     p.series.iteration = scalar_series.iteration;
     p.series.radius2_m = scalar_series.radius2_m;
     p.series.radius2_e = scalar_series.radius2_e;
-
-    LaunchState& s = state();
-    std::lock_guard<std::mutex> lock(s.mutex);
-    init_locked(s);
-    const int w = r.width, h = r.height;
-    const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
     const std::vector<double>& off_m = prep.offset_m;
     const std::vector<int32_t>& off_e = prep.offset_e;
-    char* doff = static_cast<char*>(s.offsets.get(off_m.size() * 12 + 512));
+    char* doff = static_cast<char*>(offsets.get(off_m.size() * 12 + 512));
     const size_t exp_offset = (off_m.size() * 8 + 255) & ~size_t(255);
     check(hipMemcpyAsync(doff, off_m.data(), off_m.size() * 8, hipMemcpyHostToDevice, s.stream), "offsets");
     check(hipMemcpyAsync(doff + exp_offset, off_e.data(), off_e.size() * 4, hipMemcpyHostToDevice, s.stream),
           "offsets");
     p.x_off_m = reinterpret_cast<const double*>(doff);
-    p.y_off_m = p.x_off_m + w;
+    p.y_off_m = p.x_off_m + p.width;
     p.x_off_e = reinterpret_cast<const int32_t*>(doff + exp_offset);
-    p.y_off_e = p.x_off_e + w;
-    p.output = r.device_pointers ? r.output : static_cast<float*>(s.output.get(pixels * sizeof(float)));
+    p.y_off_e = p.x_off_e + p.width;
+    return p;
+}
+
+void bind_budget(AltRenderParams& p, LaunchState& s, int time_budget_ms) {
     void* budget = s.budget.get(256);
     p.budget_start = static_cast<const unsigned long long*>(budget);
     p.budget_flag = reinterpret_cast<int*>(static_cast<char*>(budget) + 8);
-    p.budget_ticks = r.options.time_budget_ms > 0
-        ? static_cast<unsigned long long>(r.options.time_budget_ms * s.wall_clock_khz)
-        : 0ULL;
+    p.budget_ticks = time_budget_ms > 0 ? static_cast<unsigned long long>(time_budget_ms * s.wall_clock_khz) : 0ULL;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, a, b), "event time");
+    return ms;
+}
+
+double log10_zoom(const ReferenceContext& c, const char* zoom_text) {
+    const FE z = parse_zoom(zoom_text, c.precision_bits);
+    return (std::log(z.m) + static_cast<double>(z.e) * 0.69314718055994530942) / 2.30258509299404568402;
+}
+
+FieldRequest preview_request(const FieldRequest& r) {
+    FieldRequest q = r;
+    q.width = (r.width + 3) / 4;
+    q.height = (r.height + 3) / 4;
+    q.output = nullptr;
+    q.planes = nullptr;
+    q.stats = nullptr;
+    q.device_pointers = true;
+    return q;
+}
+
+// Preview render (quarter size, per-pixel pass budget) and the tile
+// features of the full field `r` into s.mlp_features; lock held.  Returns
+// the device features pointer.
+float* launch_preview_features(const ReferenceContext& c, const FieldRequest& r, const PreparedAlternate& prep,
+                               int work_limit, LaunchState& s, hipEvent_t before, hipEvent_t after) {
+    const FieldRequest q = preview_request(r);
+    AltRenderParams p = bind_alternate(c, prep, s, s.preview_offsets);
+    const size_t pixels = static_cast<size_t>(q.width) * static_cast<size_t>(q.height);
+    p.output = static_cast<float*>(s.preview_output.get(pixels * sizeof(float)));
+    p.work = static_cast<int32_t*>(s.preview_work.get(pixels * 4));
+    p.work_limit = work_limit;
+    bind_budget(p, s, 0);
+    const int tiles_x = (r.width + mlp::kTile - 1) / mlp::kTile, tiles_y = (r.height + mlp::kTile - 1) / mlp::kTile;
+    float* features = static_cast<float*>(
+        s.mlp_features.get(static_cast<size_t>(tiles_x) * tiles_y * mlp::kInputs * sizeof(float)));
+    check(hipEventRecord(before, s.stream), "event");
+    check(launch_alt_kernel(p, s.stream), "preview kernel launch");
+    check(mlp::features(p.work, q.width, q.height, tiles_x, tiles_y,
+                        static_cast<float>(log10_zoom(c, r.zoom_text) / 100.0), features, s.stream),
+          "tile features");
+    check(hipEventRecord(after, s.stream), "event");
+    return features;
+}
+
+}  // namespace
+
+void render_alternate(ReferenceContext& c, const FieldRequest& r, const uint32_t* tile_order, size_t tile_count,
+                      int32_t* work) {
+    PreparedAlternate prep;
+    prepare_alternate(c, r, prep);
+    if (tile_order) check_tile_order(tile_order, tile_count, r.width, r.height);
+    ensure_device_tables(c);
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const size_t pixels = static_cast<size_t>(r.width) * static_cast<size_t>(r.height);
+    AltRenderParams p = bind_alternate(c, prep, s, s.offsets);
+    p.output = r.device_pointers ? r.output : static_cast<float*>(s.output.get(pixels * sizeof(float)));
+    bind_budget(p, s, r.options.time_budget_ms);
+    if (tile_order) {
+        void* d = s.tile_order.get(tile_count * 4);
+        check(hipMemcpyAsync(d, tile_order, tile_count * 4, hipMemcpyHostToDevice, s.stream), "H2D tile order");
+        p.tile_order = static_cast<const uint32_t*>(d);
+        p.tile_count = static_cast<int32_t>(tile_count);
+    }
+    if (work) p.work = static_cast<int32_t*>(s.work.get(pixels * 4));
     const uint64_t wall0 = now_ns();
     check(hipEventRecord(s.start, s.stream), "event");
     check(launch_alt_kernel(p, s.stream), "alternate kernel launch");
     check(hipEventRecord(s.stop, s.stream), "event");
     if (!r.device_pointers)
         check(hipMemcpyAsync(r.output, p.output, pixels * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H");
+    if (work) check(hipMemcpyAsync(work, p.work, pixels * 4, hipMemcpyDeviceToHost, s.stream), "D2H work");
     check(hipStreamSynchronize(s.stream), "alternate render");
-    float kernel_ms = 0.0f;
-    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    const float kernel_ms = elapsed(s.start, s.stop);
     s.timing.kernel_ms = kernel_ms;
     s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
     s.timing.pixels = static_cast<double>(pixels);
@@ -439,6 +523,139 @@ void render_alternate(ReferenceContext& c, const FieldRequest& r) {
         r.stats[0] = static_cast<uint64_t>(pixels);
         r.stats[22] = static_cast<uint64_t>(static_cast<double>(kernel_ms) * 1e6);
     }
+}
+
+void alternate_preview_features(ReferenceContext& c, const FieldRequest& r, int work_limit, float* features) {
+    PreparedAlternate prep;
+    prepare_alternate(c, preview_request(r), prep);
+    ensure_device_tables(c);
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const size_t count = static_cast<size_t>((r.width + 31) / 32) * ((r.height + 31) / 32) * mlp::kInputs;
+    const float* d = launch_preview_features(c, r, prep, work_limit, s, s.start, s.stop);
+    check(hipMemcpyAsync(features, d, count * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H features");
+    check(hipStreamSynchronize(s.stream), "preview");
+    const float ms = elapsed(s.start, s.stop);
+    s.timing = LastTiming{ms, ms, static_cast<double>(count / mlp::kInputs)};
+}
+
+void render_alternate_learned(ReferenceContext& c, const FieldRequest& r, const float* weights, int work_limit,
+                              uint32_t* order_out, double* timings) {
+    PreparedAlternate preview_prep, prep;
+    prepare_alternate(c, preview_request(r), preview_prep);
+    prepare_alternate(c, r, prep);
+    ensure_device_tables(c);
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const uint64_t wall0 = now_ns();
+    const int tiles = ((r.width + 31) / 32) * ((r.height + 31) / 32);
+    // preview, features, cost model
+    const float* features = launch_preview_features(c, r, preview_prep, work_limit, s, s.mark[0], s.mark[1]);
+    float* w = static_cast<float*>(s.mlp_weights.get(mlp::kParams * sizeof(float)));
+    float* y = static_cast<float*>(s.mlp_y.get(static_cast<size_t>(tiles) * sizeof(float)));
+    check(hipMemcpyAsync(w, weights, mlp::kParams * sizeof(float), hipMemcpyHostToDevice, s.stream), "H2D weights");
+    check(mlp::forward(w, features, tiles, y, nullptr, s.stream), "cost model forward");
+    check(hipEventRecord(s.mark[2], s.stream), "event");
+    std::vector<float> predicted(static_cast<size_t>(tiles));
+    check(hipMemcpyAsync(predicted.data(), y, predicted.size() * sizeof(float), hipMemcpyDeviceToHost, s.stream),
+          "D2H costs");
+    check(hipStreamSynchronize(s.stream), "cost model");
+    // heaviest first; equal (or NaN) predictions keep row-major order
+    std::vector<uint32_t> order(static_cast<size_t>(tiles));
+    for (int i = 0; i < tiles; ++i) order[static_cast<size_t>(i)] = static_cast<uint32_t>(i);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const float ca = predicted[a], cb = predicted[b];
+        return (ca == ca ? ca : -INFINITY) > (cb == cb ? cb : -INFINITY);
+    });
+    if (order_out) std::copy(order.begin(), order.end(), order_out);
+    const uint64_t sorted_ns = now_ns();
+    // the field in that order
+    const size_t pixels = static_cast<size_t>(r.width) * static_cast<size_t>(r.height);
+    AltRenderParams p = bind_alternate(c, prep, s, s.offsets);
+    p.output = r.device_pointers ? r.output : static_cast<float*>(s.output.get(pixels * sizeof(float)));
+    bind_budget(p, s, r.options.time_budget_ms);
+    void* d_order = s.tile_order.get(order.size() * 4);
+    check(hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s.stream), "H2D order");
+    p.tile_order = static_cast<const uint32_t*>(d_order);
+    p.tile_count = tiles;
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_alt_kernel(p, s.stream), "alternate kernel launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    if (!r.device_pointers)
+        check(hipMemcpyAsync(r.output, p.output, pixels * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H");
+    check(hipStreamSynchronize(s.stream), "learned render");
+    const float render_ms = elapsed(s.start, s.stop);
+    s.timing.kernel_ms = render_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
+    if (timings) {
+        timings[0] = elapsed(s.mark[0], s.mark[1]);  // preview render + features
+        timings[1] = elapsed(s.mark[1], s.mark[2]);  // weights upload + cost model
+        timings[2] = render_ms;                      // ordered render
+        timings[3] = static_cast<double>(sorted_ns - wall0) * 1e-6;  // wall time up to the ordered launch
+        timings[4] = s.timing.total_ms;
+    }
+}
+
+// ---- learned tile-cost model -------------------------------------------------
+
+namespace {
+
+mlp::Workspace mlp_workspace(LaunchState& s, int n) {
+    const size_t count = static_cast<size_t>(n);
+    mlp::Workspace ws{};
+    ws.y = static_cast<float*>(s.mlp_y.get(count * 4));
+    ws.act = static_cast<float*>(s.mlp_act.get(count * mlp::kActivations * 4));
+    ws.delta = static_cast<float*>(s.mlp_delta.get(count * mlp::kDeltas * 4));
+    ws.err2 = static_cast<float*>(s.mlp_err2.get(count * 4));
+    ws.partial = static_cast<float*>(
+        s.mlp_partial.get(static_cast<size_t>(mlp::chunk_count(n)) * mlp::kParams * 4));
+    return ws;
+}
+
+}  // namespace
+
+void mlp_forward(const float* weights, const float* x, int n, float* y) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(mlp::forward(weights, x, n, y, nullptr, s.stream), "cost model forward");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    check(hipStreamSynchronize(s.stream), "cost model forward");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, s.start, s.stop), "event time");
+    s.timing = LastTiming{ms, ms, static_cast<double>(n)};
+}
+
+void mlp_gradient(const float* weights, const float* x, const float* t, int n, float* grad, float* loss) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const mlp::Workspace ws = mlp_workspace(s, n);
+    check(mlp::gradient(weights, x, t, n, ws, grad, loss, s.stream), "cost model gradient");
+    check(hipStreamSynchronize(s.stream), "cost model gradient");
+}
+
+void mlp_train(float* weights, float* adam_m, float* adam_v, const float* x, const float* t, int n, int steps,
+               float learning_rate, int first_step, float* loss_history) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const mlp::Workspace ws = mlp_workspace(s, n);
+    float* grad = static_cast<float*>(s.mlp_grad.get(mlp::kParams * 4));
+    check(hipEventRecord(s.start, s.stream), "event");
+    for (int k = 0; k < steps; ++k) {
+        check(mlp::gradient(weights, x, t, n, ws, grad, loss_history + k, s.stream), "cost model gradient");
+        check(mlp::adam(weights, grad, adam_m, adam_v, learning_rate, first_step + k, s.stream), "cost model step");
+    }
+    check(hipEventRecord(s.stop, s.stream), "event");
+    check(hipStreamSynchronize(s.stream), "cost model training");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, s.start, s.stop), "event time");
+    s.timing = LastTiming{ms, ms, static_cast<double>(n) * steps};
 }
 
 // ---- batched fields --------------------------------------------------------

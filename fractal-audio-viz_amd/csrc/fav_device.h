@@ -158,6 +158,14 @@ struct AltRenderParams {
     int32_t max_block;
     int32_t series_available;
     double output_bias;
+    // schedule: 32x32 tiles launched in tile_order (null = row-major grid);
+    // work, when not null, receives each pixel's loop passes
+    const uint32_t* tile_order;
+    int32_t tile_count;
+    int32_t* work;
+    // preview budget: when > 0 a pixel stops after this many loop passes
+    // (output NaN, work = the budget)
+    int32_t work_limit;
 };
 
 // Everything one field launch needs, passed by value as the kernel argument.

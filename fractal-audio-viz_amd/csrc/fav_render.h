@@ -58,7 +58,27 @@ void batch_render(PreparedBatch& batch, int fast_blocks_per_cu, bool use_graph);
 
 // Alternate-formula field (Julia, burning ship, tricorn) of a reference
 // handle; request.planes must be null.
-void render_alternate(ReferenceContext& context, const FieldRequest& request);
+void render_alternate(ReferenceContext& context, const FieldRequest& request, const uint32_t* tile_order = nullptr,
+                      size_t tile_count = 0, int32_t* work = nullptr);
+// Learned schedule of an alternate-formula field: the quarter-size preview
+// with `work_limit` passes per pixel, its tile features (host `features`,
+// tiles * 65 floats) ...
+void alternate_preview_features(ReferenceContext& context, const FieldRequest& request, int work_limit,
+                                float* features);
+// ... and the whole pipeline: preview, features, cost model (host weights),
+// heaviest-first order (copied to order_out when set) and the ordered
+// render.  timings (5 doubles, when set): preview + features ms, model ms,
+// ordered render ms, wall ms up to the ordered launch, total wall ms.
+void render_alternate_learned(ReferenceContext& context, const FieldRequest& request, const float* weights,
+                              int work_limit, uint32_t* order_out, double* timings);
+
+// Learned tile-cost model (sched_mlp.h shapes); every pointer is device
+// memory and the caller checks n and steps.  loss_history gets one mean
+// squared error per training step (the loss before that step's update).
+void mlp_forward(const float* weights, const float* x, int n, float* y);
+void mlp_gradient(const float* weights, const float* x, const float* t, int n, float* grad, float* loss);
+void mlp_train(float* weights, float* adam_m, float* adam_v, const float* x, const float* t, int n, int steps,
+               float learning_rate, int first_step, float* loss_history);
 
 // KFP colour transfer on the device.  `params` carries host pointers for
 // field / lut / output unless device_pointers is set (then field and output

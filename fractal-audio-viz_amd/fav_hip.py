@@ -178,6 +178,29 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         ctypes.POINTER(NativeKfpOptions), _c_uint8_p, ctypes.c_int]
     lib.fractal_hip_colourise_kfp_device.restype = ctypes.c_int
+    lib.fractal_hip_render_reference_scheduled.argtypes = [
+        _c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.POINTER(NativeRenderOptions), ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    lib.fractal_hip_render_reference_scheduled.restype = ctypes.c_int
+    lib.fractal_hip_preview_features.argtypes = [
+        _c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.POINTER(NativeRenderOptions), ctypes.c_int]
+    lib.fractal_hip_preview_features.restype = ctypes.c_int
+    lib.fractal_hip_render_reference_learned.argtypes = [
+        _c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.POINTER(NativeRenderOptions), _c_float_p, ctypes.c_int, ctypes.c_void_p, _c_double_p]
+    lib.fractal_hip_render_reference_learned.restype = ctypes.c_int
+    lib.fractal_hip_cost_model_parameter_count.argtypes = []
+    lib.fractal_hip_cost_model_parameter_count.restype = ctypes.c_int
+    lib.fractal_hip_cost_model_forward.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    lib.fractal_hip_cost_model_forward.restype = ctypes.c_int
+    lib.fractal_hip_cost_model_gradient.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                    ctypes.c_void_p, ctypes.c_void_p]
+    lib.fractal_hip_cost_model_gradient.restype = ctypes.c_int
+    lib.fractal_hip_cost_model_train.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+        ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_void_p]
+    lib.fractal_hip_cost_model_train.restype = ctypes.c_int
     lib.fractal_set_stats_enabled.argtypes = [ctypes.c_int]
     lib.fractal_get_last_stats_ex.argtypes = [_c_uint64_p, ctypes.c_int]
     lib.fractal_get_last_stats_ex.restype = ctypes.c_int
@@ -192,12 +215,15 @@ class NativeReference:
 
     def __init__(self, x_center: str, y_center: str, viewport_zoom: str, max_iter: int,
                  precision_bits: int, image_series_order: int = 8,
-                 options: NativeRenderOptions | None = None, library=None):
+                 options: NativeRenderOptions | None = None, library=None, formula: int = 0,
+                 julia: tuple[str, str] = ("0", "0")):
         self.library = library or load_library()
         self.options = options or NativeRenderOptions()
+        self.formula = int(formula)
         handle = self.library.fractal_create_reference_ex_options(
             x_center.encode(), y_center.encode(), viewport_zoom.encode(), int(max_iter),
-            int(precision_bits), int(image_series_order), 0, b"0", b"0", ctypes.byref(self.options))
+            int(precision_bits), int(image_series_order), self.formula, str(julia[0]).encode(),
+            str(julia[1]).encode(), ctypes.byref(self.options))
         if not handle:
             raise RuntimeError(self.library.fractal_last_error().decode())
         self.handle = handle
@@ -235,6 +261,60 @@ class NativeReference:
         if planes:
             return field, {k: v.reshape(height, width) for k, v in plane_arrays.items()}
         return field
+
+    def render_scheduled(self, width: int, height: int, zoom: str, max_iter: int, tile_order=None,
+                         work: bool = False, series_order: int = 3, series_block: int = 256,
+                         options: NativeRenderOptions | None = None):
+        """Alternate-formula field (H, W) with its 32x32 tiles launched in
+        `tile_order` (None = the plain grid); with work=True also the (H, W)
+        int32 map of per-pixel loop passes."""
+        opts = options or self.options
+        out = np.empty((int(height), int(width)), dtype=np.float32)
+        work_map = np.empty((int(height), int(width)), dtype=np.int32) if work else None
+        order = None if tile_order is None else np.ascontiguousarray(tile_order, dtype=np.uint32)
+        status = self.library.fractal_hip_render_reference_scheduled(
+            out.ctypes.data_as(_c_float_p), int(width), int(height), zoom.encode(), self.handle, int(max_iter),
+            int(series_order), int(series_block), ctypes.byref(opts),
+            order.ctypes.data if order is not None else None, 0 if order is None else order.size,
+            work_map.ctypes.data if work_map is not None else None)
+        if status != 0:
+            raise RuntimeError(self.library.fractal_last_error().decode())
+        return (out, work_map) if work else out
+
+    def preview_features(self, width: int, height: int, zoom: str, max_iter: int, work_limit: int,
+                         series_order: int = 3, series_block: int = 256,
+                         options: NativeRenderOptions | None = None) -> np.ndarray:
+        """(tiles, 65) float32 cost-model inputs of an alternate-formula field
+        from its quarter-size preview with `work_limit` passes per pixel."""
+        opts = options or self.options
+        tiles = ((int(width) + 31) // 32) * ((int(height) + 31) // 32)
+        out = np.empty((tiles, 65), dtype=np.float32)
+        status = self.library.fractal_hip_preview_features(
+            out.ctypes.data_as(_c_float_p), int(width), int(height), zoom.encode(), self.handle, int(max_iter),
+            int(series_order), int(series_block), ctypes.byref(opts), int(work_limit))
+        if status != 0:
+            raise RuntimeError(self.library.fractal_last_error().decode())
+        return out
+
+    def render_learned(self, width: int, height: int, zoom: str, max_iter: int, weights: np.ndarray,
+                       work_limit: int, series_order: int = 3, series_block: int = 256,
+                       options: NativeRenderOptions | None = None):
+        """Alternate-formula field (H, W) rendered in the cost model's
+        heaviest-first tile order; returns (field, order, timings dict)."""
+        opts = options or self.options
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        out = np.empty((int(height), int(width)), dtype=np.float32)
+        tiles = ((int(width) + 31) // 32) * ((int(height) + 31) // 32)
+        order = np.empty(tiles, dtype=np.uint32)
+        timings = (ctypes.c_double * 5)()
+        status = self.library.fractal_hip_render_reference_learned(
+            out.ctypes.data_as(_c_float_p), int(width), int(height), zoom.encode(), self.handle, int(max_iter),
+            int(series_order), int(series_block), ctypes.byref(opts), w.ctypes.data_as(_c_float_p), int(work_limit),
+            order.ctypes.data, timings)
+        if status != 0:
+            raise RuntimeError(self.library.fractal_last_error().decode())
+        names = ("preview_ms", "model_ms", "render_ms", "wall_to_launch_ms", "wall_ms")
+        return out, order, dict(zip(names, list(timings)))
 
     def close(self) -> None:
         if getattr(self, "handle", None):

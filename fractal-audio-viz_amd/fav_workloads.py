@@ -65,3 +65,45 @@ def flops_from_stats(stats, planes: bool) -> float:
     if planes:
         total += (scaled_exact + replay + tail) * FLOPS["derivative"] + linear * FLOPS["derivative_linear"]
     return total
+
+
+def julia_parameter(seed: int) -> tuple[str, str]:
+    """Julia parameter c = e^{i theta}/2 - e^{2 i theta}/4 + 1e-3 e^{i phi},
+    theta and phi drawn from random.Random(seed): a point 1e-3 from the main
+    cardioid's boundary, as decimal text of the float64 values."""
+    import math
+    rng = random.Random(seed)
+    theta, phi = rng.uniform(0.0, 2.0 * math.pi), rng.uniform(0.0, 2.0 * math.pi)
+    re = math.cos(theta) / 2 - math.cos(2 * theta) / 4 + 1e-3 * math.cos(phi)
+    im = math.sin(theta) / 2 - math.sin(2 * theta) / 4 + 1e-3 * math.sin(phi)
+    return repr(re), repr(im)
+
+
+def julia_fixed_point(c_re: str, c_im: str, digits: int = 384) -> tuple[str, str]:
+    """The fixed point of z^2 + c with the larger modulus (the root of
+    z^2 - z + c, (1 +- sqrt(1 - 4c)) / 2), as `digits`-digit decimal text:
+    a point of the Julia set, whose orbit is constant."""
+    from decimal import Decimal, localcontext
+    with localcontext() as ctx:
+        ctx.prec = digits + 20
+        cr, ci = Decimal(c_re), Decimal(c_im)
+        a, b = 1 - 4 * cr, -4 * ci
+        modulus = (a * a + b * b).sqrt()
+        sr = ((modulus + a) / 2).sqrt()
+        si = ((modulus - a) / 2).sqrt()
+        if b < 0:
+            si = -si
+        roots = [((1 + sr) / 2, si / 2), ((1 - sr) / 2, -si / 2)]
+        zr, zi = max(roots, key=lambda z: z[0] * z[0] + z[1] * z[1])
+        ctx.prec = digits
+        return str(+zr), str(+zi)
+
+
+def julia_field(seed: int = 1, width: int = 3840, height: int = 2160, zoom: str = "1e100",
+                max_iter: int = 200000):
+    """A Julia field viewed at its repelling fixed point (the reference's
+    Julia view centre), 400-bit orbit, default render options."""
+    c = julia_parameter(seed)
+    x, y = julia_fixed_point(*c)
+    return dict(name=f"julia_{width}x{height}_zoom{zoom}_cap{max_iter}_seed{seed}", formula=1, julia=c, x=x, y=y,
+                zoom=zoom, width=width, height=height, max_iter=max_iter, bits=400)

@@ -301,6 +301,76 @@ int fractal_hip_render_reference_floatexp(
     int series_block, const FractalRenderOptions *options,
     int device_pointers);
 
+/* An alternate-formula field (Julia, burning ship, tricorn; zoom >= 1e6)
+ * with its 32x32 tiles launched in tile_order (a permutation of the
+ * ceil(w/32)*ceil(h/32) row-major tiles; NULL = the plain grid), and, when
+ * work is not NULL, each pixel's loop passes written to it.  Host pointers;
+ * the field is identical for every order.  No counterpart in the reference:
+ * the schedule hook of the learned tile-cost model. */
+int fractal_hip_render_reference_scheduled(
+    float *output, int width, int height, const char *zoom_text, void *handle,
+    int max_iter, int series_order, int series_block,
+    const FractalRenderOptions *options, const uint32_t *tile_order,
+    int tile_count, int32_t *work);
+
+/* Tile features of an alternate-formula field for the cost model below:
+ * renders the quarter-size preview ((w+3)/4 x (h+3)/4, same view) with at
+ * most work_limit loop passes per pixel and writes, per 32x32 tile of the
+ * w x h field (row-major), its 8x8 patch of log(1 + preview passes) / 8
+ * (edge pixels repeated past the preview's border) and log10(zoom) / 100:
+ * ceil(w/32) * ceil(h/32) * 65 floats to the host buffer `features`.
+ * No counterpart in the reference. */
+int fractal_hip_preview_features(
+    float *features, int width, int height, const char *zoom_text,
+    void *handle, int max_iter, int series_order, int series_block,
+    const FractalRenderOptions *options, int work_limit);
+
+/* The learned schedule end to end: preview and features as above, the cost
+ * model with `weights` (host, fractal_hip_cost_model_parameter_count()
+ * floats), 32x32 tiles sorted by decreasing predicted cost (ties and NaN
+ * in row-major order), and the field rendered in that order into the host
+ * buffer `output` -- identical to fractal_hip_render_reference_scheduled
+ * with any order.  tile_order (optional) receives the order; timings
+ * (optional, 5 doubles): preview + features ms, cost model ms, ordered
+ * render ms (HIP events), wall ms up to the ordered launch, total wall ms.
+ * No counterpart in the reference. */
+int fractal_hip_render_reference_learned(
+    float *output, int width, int height, const char *zoom_text, void *handle,
+    int max_iter, int series_order, int series_block,
+    const FractalRenderOptions *options, const float *weights, int work_limit,
+    uint32_t *tile_order, double *timings);
+
+/* ---- Learned tile-cost model (no counterpart in the reference) ---------
+ * A 65 -> 64 -> 64 -> 1 ReLU network with bf16 weights and inputs and fp32
+ * sums that predicts log(1 + cost) of a 32x32 tile from an 8x8 patch of a
+ * quarter-resolution preview's per-pixel work (log(1 + passes)) plus
+ * log10(zoom) / 100.  It only orders tiles, so renders stay identical.
+ * weights: fractal_hip_cost_model_parameter_count() fp32 values, laid out
+ * W1[65][64], b1[64], W2[64][64], b2[64], W3[64], b3 and rounded to bf16
+ * on use.  Every pointer below is device memory; features hold count * 65
+ * floats, row-major.  Calls on one process are serialised. */
+int fractal_hip_cost_model_parameter_count(void);
+
+/* costs[i] = network(features[i]).  count in [1, 2^22]. */
+int fractal_hip_cost_model_forward(const float *weights, const float *features,
+                                   int count, float *costs);
+
+/* gradient (parameter_count floats) of loss = mean((network(x) - t)^2)
+ * with respect to the bf16 weights, and loss (one float). */
+int fractal_hip_cost_model_gradient(const float *weights,
+                                    const float *features,
+                                    const float *targets, int count,
+                                    float *gradient, float *loss);
+
+/* steps full-batch Adam updates (beta 0.9 / 0.999, eps 1e-8) of the fp32
+ * master weights, Adam step numbers first_step, first_step + 1, ...;
+ * adam_m / adam_v are the moment buffers (zero before the first step);
+ * loss_history[k] receives the loss before update k.  Deterministic. */
+int fractal_hip_cost_model_train(float *weights, float *adam_m, float *adam_v,
+                                 const float *features, const float *targets,
+                                 int count, int steps, float learning_rate,
+                                 int first_step, float *loss_history);
+
 /* ---- Batched fields (no counterpart in the reference) ------------------
  * Several Mandelbrot fields with their own reference handles, sizes, zooms
  * and caps rendered by one launch sequence over a shared device tile queue
