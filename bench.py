@@ -143,15 +143,18 @@ def batch_leg(lib, hip, steps):
 
 def learned_leg(lib, repeats=2):
     """The learned tile scheduler: a cost model trained on the GPU over the
-    seeded training set (12 Julia fields, 1920x1080, cap 20000), then Julia
-    fields at 3840x2160, zoom 1e100, cap 200000 (evaluation seeds 1, 2, 4,
-    disjoint from training) rendered in the plain grid and in the learned
-    heaviest-first order.  Net gain counts the preview, features and model
-    time; prediction error is against each field's measured tile costs."""
+    seeded training set (12 Julia and 6 tricorn fields, 1920x1080, cap
+    20000), then held-out fields at 3840x2160, zoom 1e100, cap 200000 (the
+    tricorn branching junction and Julia seeds 1, 2, 4) rendered in the
+    plain grid and in the learned heaviest-first order.  Net gain counts the
+    preview, features and model time; prediction error is against each
+    field's measured tile costs."""
     model, training = fav_sched.train_model(fav_sched.training_fields(), library=lib)
     fields = []
-    for seed in (1, 2, 4):
-        case = fav_workloads.julia_field(seed=seed, width=3840, height=2160, zoom="1e100", max_iter=200000)
+    cases = [fav_workloads.tricorn_field("branching-junction", 3840, 2160, "1e100", 200000)] + [
+        fav_workloads.julia_field(seed=seed, width=3840, height=2160, zoom="1e100", max_iter=200000)
+        for seed in (1, 2, 4)]
+    for case in cases:
         w, h, z, n = case["width"], case["height"], case["zoom"], case["max_iter"]
         ref = fav_sched.reference_for(case, lib)
         try:
@@ -166,14 +169,15 @@ def learned_leg(lib, repeats=2):
             learned = [fav_sched.render_learned(ref, model, w, h, z, n)[2] for _ in range(repeats)]
             best = min(learned, key=lambda t: t["preview_ms"] + t["model_ms"] + t["render_ms"])
             total = best["preview_ms"] + best["model_ms"] + best["render_ms"]
-            fields.append({"seed": seed, "julia_c": list(case["julia"]), "plain_ms": round(min(plain), 3),
+            fields.append({"field": case["name"], "plain_ms": round(min(plain), 3),
                            "learned_total_ms": round(total, 3), "preview_ms": round(best["preview_ms"], 3),
                            "model_ms": round(best["model_ms"], 3), "ordered_render_ms": round(best["render_ms"], 3),
                            "net_gain_pct": round(100.0 * (min(plain) - total) / min(plain), 2),
                            "prediction": {k: round(v, 4) if isinstance(v, float) else v for k, v in report.items()}})
         finally:
             ref.close()
-    return {"workload": "Julia at its repelling fixed point, 3840x2160, zoom 1e100, cap 200000, seeds 1 2 4",
+    return {"workload": "3840x2160, zoom 1e100, cap 200000: tricorn branching junction; Julia seeds 1 2 4 "
+                        "at their repelling fixed point",
             "preview": f"quarter size, {fav_sched.PREVIEW_WORK_LIMIT} passes per pixel",
             "training": {k: round(v, 4) if isinstance(v, float) else v for k, v in training.items()},
             "mean_net_gain_pct": round(float(np.mean([f["net_gain_pct"] for f in fields])), 2), "fields": fields}
@@ -277,6 +281,10 @@ def main():
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-batch", action="store_true", help="skip the batched atlas-segment leg")
     parser.add_argument("--no-learned", action="store_true", help="skip the learned tile-scheduler leg")
+    parser.add_argument("--learned-only", action="store_true",
+                        help="run only the learned tile-scheduler leg and print its JSON")
+    parser.add_argument("--batch-only", action="store_true",
+                        help="run only the batched atlas-segment leg and print its JSON")
     parser.add_argument("--copy-calibration-mib", type=int, default=0,
                         help="only run N device-to-device copies of this many MiB (HBM counter calibration)")
     args = parser.parse_args()
@@ -292,6 +300,12 @@ def main():
     if not lib.fractal_backend_capabilities() & 8:
         raise SystemExit(f"no gfx950 device: {lib.fractal_last_error()!r}")
     hip = fav_hip.HipRuntime()
+    if args.learned_only:
+        print(json.dumps({"learned_schedule": learned_leg(lib)}))
+        return 0
+    if args.batch_only:
+        print(json.dumps({"batched": batch_leg(lib, hip, 2)}))
+        return 0
     opts = fav_hip.NativeRenderOptions(**work["options"])
     t0 = time.perf_counter()
     ref = fav_hip.NativeReference(work["x"], work["y"], work["zoom"], work["max_iter"], work["bits"],

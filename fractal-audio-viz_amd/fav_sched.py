@@ -178,7 +178,7 @@ class FieldSample:
     render_ms: float
 
 
-PREVIEW_WORK_LIMIT = 512
+PREVIEW_WORK_LIMIT = 128
 
 
 def preview_features(reference: fav_hip.NativeReference, width: int, height: int, zoom: str, max_iter: int,
@@ -246,13 +246,18 @@ def render_learned(reference: fav_hip.NativeReference, model: CostModel | None, 
 
 
 def training_fields(count: int = 12, width: int = 1920, height: int = 1080, max_iter: int = 20000) -> list[dict]:
-    """The seeded training set: Julia fields (seeds 10, 11, ...) viewed at
-    their repelling fixed point, zooms cycling 1e30 / 1e60 / 1e100; disjoint
-    from the evaluation seeds 0-9."""
+    """The seeded training set: `count` Julia fields (seeds 10, 11, ...)
+    viewed at their repelling fixed point, zooms cycling 1e30 / 1e60 / 1e100,
+    and the tricorn preperiodic fork and antler junction at the same three
+    zooms.  Disjoint from the evaluation fields (Julia seeds 0-9, the
+    tricorn branching junction)."""
     import fav_workloads
     zooms = ("1e30", "1e60", "1e100")
-    return [fav_workloads.julia_field(seed=10 + i, width=width, height=height, zoom=zooms[i % 3], max_iter=max_iter)
-            for i in range(count)]
+    julia = [fav_workloads.julia_field(seed=10 + i, width=width, height=height, zoom=zooms[i % 3],
+                                       max_iter=max_iter) for i in range(count)]
+    tricorn = [fav_workloads.tricorn_field(point, width, height, zoom, max_iter)
+               for point in ("preperiodic-fork", "antler-junction") for zoom in zooms]
+    return julia + tricorn
 
 
 def reference_for(case: dict, library=None) -> fav_hip.NativeReference:

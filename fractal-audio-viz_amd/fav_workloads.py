@@ -107,3 +107,25 @@ def julia_field(seed: int = 1, width: int = 3840, height: int = 2160, zoom: str 
     x, y = julia_fixed_point(*c)
     return dict(name=f"julia_{width}x{height}_zoom{zoom}_cap{max_iter}_seed{seed}", formula=1, julia=c, x=x, y=y,
                 zoom=zoom, width=width, height=height, max_iter=max_iter, bits=400)
+
+
+# Tricorn preperiodic boundary roots of the reference's point catalogue
+# (deep_zoom_points.py:286-318), truncated to 128 digits: zoom 1e100 views
+# a 1e-100 neighbourhood, far above the truncation.
+TRICORN_POINTS = {
+    "branching-junction": ("-1.09692656551687871996070994567030699382726636856135252864063809062918671538233200238925924425047165728705144577721323870097123830",
+        "0.15924841985780843944738929716460993014535067284697776610028746670435199873283411089165347426523171986788898191180737561810646406"),
+    "preperiodic-fork": ("-1.22224542629255890298019006722054650655408214278112662131889956070689001518195800647023583458966697095257488470249030425971271679",
+        "0.18411010266019595907154534797192084995346033751489213057770255983684726093175172244070707373577615831859868664198973001227361970"),
+    "antler-junction": ("-0.89496367244425123674835666927614371605103297638091872536445395141229693274347387622099879227927795268786190095948036005167118619",
+        "0.16233379791644897545645068663668796188752900772412238190448090899609106894212265824125641030767828484157713045472396312957550898"),
+}
+
+
+def tricorn_field(point: str = "branching-junction", width: int = 3840, height: int = 2160, zoom: str = "1e100",
+                  max_iter: int = 200000):
+    """A tricorn field centred on a catalogue boundary root, 480-bit orbit,
+    default render options."""
+    x, y = TRICORN_POINTS[point]
+    return dict(name=f"tricorn_{point}_{width}x{height}_zoom{zoom}_cap{max_iter}", formula=3, julia=("0", "0"),
+                x=x, y=y, zoom=zoom, width=width, height=height, max_iter=max_iter, bits=480)
