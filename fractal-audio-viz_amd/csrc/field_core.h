@@ -210,6 +210,21 @@ FAV_HD void store_escape(const PlaneOut& p, size_t i, int iteration, SC total, S
     p.test2[i] = sn_to_double(test2);
 }
 
+// store_escape kept out of line in the tail loop (its atan2 / log code would
+// otherwise raise the register count of the whole loop).
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __attribute__((noinline)) void store_escape_outlined(const PlaneOut& p, size_t i, int iteration,
+                                                                SC total, SN test1, SN test2, SC derivative,
+                                                                FE spacing) {
+    store_escape(p, i, iteration, total, test1, test2, derivative, spacing);
+}
+#else
+inline void store_escape_outlined(const PlaneOut& p, size_t i, int iteration, SC total, SN test1, SN test2,
+                                  SC derivative, FE spacing) {
+    store_escape(p, i, iteration, total, test1, test2, derivative, spacing);
+}
+#endif
+
 struct Budget {  // renderer.cpp:11872-11890
     const RenderParams* p;
     unsigned ticks;
@@ -331,116 +346,192 @@ FAV_HD SC apply_cubic_derivative(const CubicStepDev& s, SC d, SC c, SC der, int 
 
 constexpr SC kOne{1.0, 0.0, 0};
 
-// render_scaled_double_tail, renderer.cpp:12450-12669.  Returns true when
-// the tail gave up (step ceiling or rebase storm).
+// render_scaled_double_tail, renderer.cpp:12450-12669, split into a
+// resumable state and one loop iteration so that a wavefront can run the
+// tails of a queue of pixels with lanes refilled as pixels finish.
+//
+// The launch values the tail reads.  Filled per pixel from that pixel's
+// RenderParams (fields of a batch differ).
+struct TailParams {
+    const double* z_re;
+    const double* z_im;
+    int32_t orbit_size;
+    int32_t max_iter;
+    int32_t escape_radius_mode;
+    int32_t strict;
+    int32_t cycle_enabled;
+    int32_t strict_cycle;
+    int32_t timed;  // a time budget is set (Budget::expired can return true)
+    double output_bias;
+    FE pixel_spacing;
+};
+
+FAV_HD TailParams tail_params(const RenderParams& p) {
+    return TailParams{p.orbit.real_double, p.orbit.imag_double, p.orbit.size, p.max_iter, p.escape_radius_mode,
+                      p.strict, p.cycle_enabled, p.strict_cycle, p.budget_ticks != 0 ? 1 : 0, p.output_bias,
+                      FE{p.pixel_spacing_m, p.pixel_spacing_e}};
+}
+
+// double_tail's loop variables.
+struct TailState {
+    double dc_re, dc_im, d_re, d_im;
+    double tortoise_re, tortoise_im;
+    SN previous_norm;
+    int cycle_power, cycle_length, tail_steps, tail_iterations, tail_rebases;
+    bool cycle_ready;
+};
+
+FAV_HD TailState tail_begin(SC dc, SC delta, SN initial_norm) {
+    TailState t;
+    t.dc_re = sc_re_double(dc);
+    t.dc_im = sc_im_double(dc);
+    t.d_re = sc_re_double(delta);
+    t.d_im = sc_im_double(delta);
+    t.tortoise_re = 0.0;
+    t.tortoise_im = 0.0;
+    t.previous_norm = initial_norm;
+    t.cycle_power = 1;
+    t.cycle_length = 0;
+    t.tail_steps = 0;
+    t.tail_iterations = 0;
+    t.tail_rebases = 0;
+    t.cycle_ready = false;
+    return t;
+}
+
+// The loop condition of the tail.
+FAV_HD bool tail_continues(const TailParams& q, int iteration, int ref) {
+    return iteration < q.max_iter && ref >= 0 && ref + 1 < q.orbit_size;
+}
+
+enum TailStatus { kTailRunning = 0, kTailEnded = 1, kTailGaveUp = 2 };
+
+// One iteration of the tail loop (the caller checked tail_continues).
+// kTailEnded / kTailGaveUp end the pixel with `out` (and planes) written as
+// double_tail's `return false` / `return true` do.
+template <bool Planes, bool Stats>
+FAV_HD TailStatus tail_step(const TailParams& q, const PlaneOut& planes, size_t index, int& iteration, int& ref,
+                            SC& derivative, Budget& budget, bool& deadline, bool& unresolved, Counters<Stats>& st,
+                            TailState& t, float& out) {
+    const double escape_squared = q.escape_radius_mode == 1 ? 1.0e8 : 4.0;
+    const double safe_escape_squared = escape_squared + 1.0e-7;
+    // Budget::expired returns false without counting a tick when no budget
+    // is set, so skipping the call then changes nothing
+    if (q.timed && budget.expired()) {
+        out = __builtin_nan(""); deadline = true; unresolved = true;
+        st.add(ST_DEADLINE); st.add(ST_UNRESOLVED);
+        return kTailEnded;
+    }
+    const double z_re = q.z_re[ref], z_im = q.z_im[ref];
+    if constexpr (Planes) {
+        SC prior_total{z_re + t.d_re, z_im + t.d_im, 0};
+        sc_normalize(prior_total);
+        derivative = sc_add(sc_double(sc_mul(prior_total, derivative)), kOne);
+    }
+    const double lin_re = 2.0 * (z_re * t.d_re - z_im * t.d_im);
+    const double lin_im = 2.0 * (z_re * t.d_im + z_im * t.d_re);
+    const double sq_re = t.d_re * t.d_re - t.d_im * t.d_im;
+    const double sq_im = 2.0 * t.d_re * t.d_im;
+    t.d_re = lin_re + sq_re + t.dc_re;
+    t.d_im = lin_im + sq_im + t.dc_im;
+    ++ref;
+    ++iteration;
+    if (++t.tail_iterations > 65536) {
+        st.add(ST_TAIL_FALLBACKS); st.add(ST_UNRESOLVED);
+        unresolved = true;
+        out = q.strict ? __builtin_nan("") : encode_iteration(iteration, q.output_bias);
+        return kTailGaveUp;
+    }
+    st.add(ST_LOGICAL); st.add(ST_EXACT);
+    const double t_re = q.z_re[ref] + t.d_re;
+    const double t_im = q.z_im[ref] + t.d_im;
+    const double m2 = t_re * t_re + t_im * t_im;
+    if (!isfinite_(m2)) {
+        unresolved = true;
+        st.add(ST_GLITCH); st.add(ST_UNRESOLVED);
+        out = q.strict ? __builtin_nan("") : encode_iteration(iteration, q.output_bias);
+        return kTailEnded;
+    }
+    if (m2 > escape_squared) {
+        const double magnitude = ::sqrt(m2 > safe_escape_squared ? m2 : safe_escape_squared);
+        const double qlog = ::log(::log(magnitude)) / kLn2;
+        out = static_cast<float>((static_cast<double>(iteration) - qlog) - q.output_bias);
+        if constexpr (Planes) {
+            SC escaped_total{t_re, t_im, 0};
+            sc_normalize(escaped_total);
+            store_escape_outlined(planes, index, iteration, escaped_total, sn_from_double(m2), t.previous_norm,
+                                  derivative, q.pixel_spacing);
+        }
+        return kTailEnded;
+    }
+    t.previous_norm = sn_from_double(m2);
+    const double dm2 = t.d_re * t.d_re + t.d_im * t.d_im;
+    ++t.tail_steps;
+    if (q.cycle_enabled && t.tail_steps >= 64 && (t.tail_steps & 31) == 0) {
+        if (!t.cycle_ready) {
+            t.tortoise_re = t_re; t.tortoise_im = t_im;
+            t.cycle_power = 1; t.cycle_length = 0; t.cycle_ready = true;
+        } else {
+            const double cdr = t_re - t.tortoise_re, cdi = t_im - t.tortoise_im;
+            const double cd2 = cdr * cdr + cdi * cdi;
+            const int cmin = q.strict_cycle ? 2048 : 512;
+            const double ctol = q.strict_cycle ? 1.0e-24 : 1.0e-18;
+            if (iteration >= cmin && m2 < 3.0 && cd2 <= ctol * (1.0 < m2 ? m2 : 1.0)) {
+                if constexpr (Planes) clear_planes(planes, index, q.max_iter);
+                return kTailEnded;
+            }
+            ++t.cycle_length;
+            if (t.cycle_length >= t.cycle_power) {
+                t.tortoise_re = t_re; t.tortoise_im = t_im;
+                t.cycle_power = t.cycle_power * 2 < (1 << 20) ? t.cycle_power * 2 : (1 << 20);
+                t.cycle_length = 0;
+            }
+        }
+    }
+    if (m2 < dm2) {
+        t.d_re = t_re; t.d_im = t_im;
+        ref = 0;
+        t.tail_steps = 0;
+        t.cycle_ready = false;
+        st.add(ST_TAIL_REBASES);
+        if (++t.tail_rebases > 64) {
+            st.add(ST_TAIL_FALLBACKS); st.add(ST_UNRESOLVED);
+            unresolved = true;
+            out = q.strict ? __builtin_nan("") : encode_iteration(iteration, q.output_bias);
+            if constexpr (Planes) clear_planes(planes, index, q.max_iter);
+            return kTailGaveUp;
+        }
+    }
+    return kTailRunning;
+}
+
+// After the loop condition failed: the reference ran out before the cap.
+template <bool Planes, bool Stats>
+FAV_HD void tail_exhausted(const TailParams& q, const PlaneOut& planes, size_t index, int iteration, bool& unresolved,
+                           Counters<Stats>& st, float& out) {
+    if (iteration < q.max_iter) {
+        unresolved = true;
+        st.add(ST_UNRESOLVED);
+        out = q.strict ? __builtin_nan("") : encode_iteration(iteration, q.output_bias);
+        if constexpr (Planes) clear_planes(planes, index, q.max_iter);
+    }
+}
+
+// The whole tail of one pixel.  Returns true when the tail gave up (step
+// ceiling or rebase storm).
 template <bool Planes, bool Stats>
 FAV_HD bool double_tail(const RenderParams& p, const PlaneOut& planes, size_t index, SC dc, int& iteration,
                         int& ref, SC delta, SC& derivative, Budget& budget, bool& deadline, bool& unresolved,
                         Counters<Stats>& st, SN initial_norm, float& out) {
-    const OrbitDev& o = p.orbit;
-    const double dc_re = sc_re_double(dc), dc_im = sc_im_double(dc);
-    const double escape_squared = p.escape_radius_mode == 1 ? 1.0e8 : 4.0;
-    const double safe_escape_squared = escape_squared + 1.0e-7;
-    double d_re = sc_re_double(delta), d_im = sc_im_double(delta);
-    out = encode_iteration(p.max_iter, p.output_bias);
-    double tortoise_re = 0.0, tortoise_im = 0.0;
-    int cycle_power = 1, cycle_length = 0, tail_steps = 0, tail_iterations = 0, tail_rebases = 0;
-    bool cycle_ready = false;
-    SN previous_norm = initial_norm;
-    const int orbit_size = o.size;
-    while (iteration < p.max_iter && ref >= 0 && ref + 1 < orbit_size) {
-        if (budget.expired()) {
-            out = __builtin_nan(""); deadline = true; unresolved = true;
-            st.add(ST_DEADLINE); st.add(ST_UNRESOLVED);
-            return false;
-        }
-        const double z_re = o.real_double[ref], z_im = o.imag_double[ref];
-        if constexpr (Planes) {
-            SC prior_total{z_re + d_re, z_im + d_im, 0};
-            sc_normalize(prior_total);
-            derivative = sc_add(sc_double(sc_mul(prior_total, derivative)), kOne);
-        }
-        const double lin_re = 2.0 * (z_re * d_re - z_im * d_im);
-        const double lin_im = 2.0 * (z_re * d_im + z_im * d_re);
-        const double sq_re = d_re * d_re - d_im * d_im;
-        const double sq_im = 2.0 * d_re * d_im;
-        d_re = lin_re + sq_re + dc_re;
-        d_im = lin_im + sq_im + dc_im;
-        ++ref;
-        ++iteration;
-        if (++tail_iterations > 65536) {
-            st.add(ST_TAIL_FALLBACKS); st.add(ST_UNRESOLVED);
-            unresolved = true;
-            out = p.strict ? __builtin_nan("") : encode_iteration(iteration, p.output_bias);
-            return true;
-        }
-        st.add(ST_LOGICAL); st.add(ST_EXACT);
-        const double t_re = o.real_double[ref] + d_re;
-        const double t_im = o.imag_double[ref] + d_im;
-        const double m2 = t_re * t_re + t_im * t_im;
-        if (!isfinite_(m2)) {
-            unresolved = true;
-            st.add(ST_GLITCH); st.add(ST_UNRESOLVED);
-            out = p.strict ? __builtin_nan("") : encode_iteration(iteration, p.output_bias);
-            return false;
-        }
-        if (m2 > escape_squared) {
-            const double magnitude = ::sqrt(m2 > safe_escape_squared ? m2 : safe_escape_squared);
-            const double q = ::log(::log(magnitude)) / kLn2;
-            out = static_cast<float>((static_cast<double>(iteration) - q) - p.output_bias);
-            if constexpr (Planes) {
-                SC escaped_total{t_re, t_im, 0};
-                sc_normalize(escaped_total);
-                store_escape(planes, index, iteration, escaped_total, sn_from_double(m2), previous_norm,
-                             derivative, FE{p.pixel_spacing_m, p.pixel_spacing_e});
-            }
-            return false;
-        }
-        previous_norm = sn_from_double(m2);
-        const double dm2 = d_re * d_re + d_im * d_im;
-        ++tail_steps;
-        if (p.cycle_enabled && tail_steps >= 64 && (tail_steps & 31) == 0) {
-            if (!cycle_ready) {
-                tortoise_re = t_re; tortoise_im = t_im;
-                cycle_power = 1; cycle_length = 0; cycle_ready = true;
-            } else {
-                const double cdr = t_re - tortoise_re, cdi = t_im - tortoise_im;
-                const double cd2 = cdr * cdr + cdi * cdi;
-                const int cmin = p.strict_cycle ? 2048 : 512;
-                const double ctol = p.strict_cycle ? 1.0e-24 : 1.0e-18;
-                if (iteration >= cmin && m2 < 3.0 && cd2 <= ctol * (1.0 < m2 ? m2 : 1.0)) {
-                    if constexpr (Planes) clear_planes(planes, index, p.max_iter);
-                    return false;
-                }
-                ++cycle_length;
-                if (cycle_length >= cycle_power) {
-                    tortoise_re = t_re; tortoise_im = t_im;
-                    cycle_power = cycle_power * 2 < (1 << 20) ? cycle_power * 2 : (1 << 20);
-                    cycle_length = 0;
-                }
-            }
-        }
-        if (m2 < dm2) {
-            d_re = t_re; d_im = t_im;
-            ref = 0;
-            tail_steps = 0;
-            cycle_ready = false;
-            st.add(ST_TAIL_REBASES);
-            if (++tail_rebases > 64) {
-                st.add(ST_TAIL_FALLBACKS); st.add(ST_UNRESOLVED);
-                unresolved = true;
-                out = p.strict ? __builtin_nan("") : encode_iteration(iteration, p.output_bias);
-                if constexpr (Planes) clear_planes(planes, index, p.max_iter);
-                return true;
-            }
-        }
+    const TailParams q = tail_params(p);
+    TailState t = tail_begin(dc, delta, initial_norm);
+    out = encode_iteration(q.max_iter, q.output_bias);
+    while (tail_continues(q, iteration, ref)) {
+        const TailStatus r = tail_step<Planes, Stats>(q, planes, index, iteration, ref, derivative, budget, deadline,
+                                                      unresolved, st, t, out);
+        if (r != kTailRunning) return r == kTailGaveUp;
     }
-    if (iteration < p.max_iter) {
-        unresolved = true;
-        st.add(ST_UNRESOLVED);
-        out = p.strict ? __builtin_nan("") : encode_iteration(iteration, p.output_bias);
-        if constexpr (Planes) clear_planes(planes, index, p.max_iter);
-    }
+    tail_exhausted<Planes, Stats>(q, planes, index, iteration, unresolved, st, out);
     return false;
 }
 
