@@ -354,36 +354,47 @@ FAV_HD bool sc_normal(SC v) {
     return magnitude >= 0.5 && magnitude < 1.0 && v.e <= (1 << 28) && v.e >= -(1 << 28);
 }
 
+// max(|x|, |y|) of finite values: the larger magnitude, as the reference's
+// (|x| < |y|) ? |y| : |x| selects it (one instruction on the device).
+FAV_HD double max_abs(double x, double y) { return __builtin_fmax(fabs_(x), fabs_(y)); }
+
 FAV_HD SC sc_mul_nn(SC a, SC b) {
     const double re = a.re * b.re - a.im * b.im;
     const double im = a.re * b.im + a.im * b.re;
-    const double ar = fabs_(re), ai = fabs_(im);
-    const int shift = frexp_exp_nz((ar < ai) ? ai : ar);
+    const int shift = frexp_exp_nz(max_abs(re, im));
     return SC{ldexp_(re, -shift), ldexp_(im, -shift), a.e + b.e + shift};
 }
 
+// Both operands are aligned to the larger exponent (the larger one by a
+// shift of 0), which gives the same sums as adding the shifted smaller
+// operand to the larger (IEEE addition is commutative) without swapping.
 FAV_HD SC sc_add_nn(SC a, SC b) {
-    const bool swap = b.e > a.e;
-    const SC larger = swap ? b : a;
-    const SC smaller = swap ? a : b;
-    const int diff = larger.e - smaller.e;
-    const int d = diff < 64 ? diff : 64;
-    const double re = larger.re + ldexp_(smaller.re, -d);
-    const double im = larger.im + ldexp_(smaller.im, -d);
-    const double ar = fabs_(re), ai = fabs_(im);
-    const double magnitude = (ar < ai) ? ai : ar;
+    const int top = a.e > b.e ? a.e : b.e;
+    const int da = top - a.e < 64 ? top - a.e : 64;
+    const int db = top - b.e < 64 ? top - b.e : 64;
+    const double re = ldexp_(a.re, -da) + ldexp_(b.re, -db);
+    const double im = ldexp_(a.im, -da) + ldexp_(b.im, -db);
+    const double magnitude = max_abs(re, im);
     const int shift = frexp_exp_nz(magnitude);  // 0 for an exact cancellation
-    SC r{ldexp_(re, -shift), ldexp_(im, -shift), magnitude == 0.0 ? 0 : larger.e + shift};
-    if (diff > 60) r = larger;
+    SC r{ldexp_(re, -shift), ldexp_(im, -shift), magnitude == 0.0 ? 0 : top + shift};
+    if (da + db > 60) r = b.e > a.e ? b : a;  // the larger operand unchanged
     return r;
 }
 
+// |v|^2 of a normal value: the sum lies in [0.25, 2), so the frexp exponent
+// k is -1, 0 or 1 and scaling by 2^-k is the reference's *2, *1 or *0.5.
 FAV_HD SN sc_norm_nn(SC v) {
     const double norm = v.re * v.re + v.im * v.im;
-    const int e2 = 2 * v.e;
-    if (norm >= 1.0) return SN{norm * 0.5, e2 + 1};
-    if (norm < 0.5) return SN{norm * 2.0, e2 - 1};
-    return SN{norm, e2};
+    const int k = frexp_exp_nz(norm);
+    return SN{ldexp_(norm, -k), 2 * v.e + k};
+}
+
+// sc_compare_norm(a, b) >= 0 for a finite-or-not `a` and a normal, nonzero,
+// finite `b` (renderer.cpp:7776-7784 restricted to that case).
+FAV_HD bool sn_at_least(SN a, SN b) {
+    if (!isfinite_(a.m)) return true;
+    if (a.m == 0.0) return false;
+    return a.e != b.e ? a.e > b.e : a.m >= b.m;
 }
 
 // sc_norm_squared, renderer.cpp:7743-7774.

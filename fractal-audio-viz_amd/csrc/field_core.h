@@ -722,8 +722,7 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
         const int lim = wave_first(limit);
         // the norm compare uses |Z|^2 of the record: equal to the state's
         // total_norm whenever delta is negligible, which the first clause checks
-        ok = delta.e <= top.small_exp && sc_compare_norm(SN{top.zn_m, top.zn_e}, dn) >= 0 && order <= 1
-            && limit == lim;
+        ok = delta.e <= top.small_exp && sn_at_least(SN{top.zn_m, top.zn_e}, dn) && order <= 1 && limit == lim;
         if (!wave_all(ok)) break;
         // BLA level: every lane must take the same (highest valid) level
         if (r <= 0 || lim <= 0 || dir.count == 0 || r > dir.size[0]) break;

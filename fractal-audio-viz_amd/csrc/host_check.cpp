@@ -330,11 +330,13 @@ bool same_sc(fav::SC a, fav::SC b) {
 
 // Test-only: the *_nn operations against the general ones on `count` random
 // normal operand pairs (including exact cancellations and exponent gaps
-// around the 60-binade cut).  Writes mismatch counts for mul, add and norm.
+// around the 60-binade cut).  Writes mismatch counts for mul, add, norm and
+// the block loop's norm comparison (sn_at_least against sc_compare_norm on
+// a normal norm and another norm that may be zero, infinite or equal).
 extern "C" void fav_host_check_nn_ops(long long count, unsigned long long seed, long long* mismatches) {
     using namespace fav;
     std::mt19937_64 rng(seed);
-    mismatches[0] = mismatches[1] = mismatches[2] = 0;
+    mismatches[0] = mismatches[1] = mismatches[2] = mismatches[3] = 0;
     std::uniform_int_distribution<int> gap(-70, 70), pick(0, 9);
     for (long long i = 0; i < count; ++i) {
         const SC a = random_normal(rng, 1 << 20);
@@ -346,6 +348,13 @@ extern "C" void fav_host_check_nn_ops(long long count, unsigned long long seed, 
         if (!same_sc(sc_add_nn(a, b), sc_add(a, b))) ++mismatches[1];
         const SN x = sc_norm_nn(a), y = sc_norm_squared(a);
         if (std::memcmp(&x.m, &y.m, 8) != 0 || x.e != y.e) ++mismatches[2];
+        SN z = sc_norm_squared(b);
+        const int kind = pick(rng);
+        if (kind == 0) z = SN{0.0, 0};
+        else if (kind == 1) z = SN{__builtin_inf(), INT_MAX};
+        else if (kind == 2) z = x;
+        else if (kind == 3) z.e = x.e;
+        if (sn_at_least(z, x) != (sc_compare_norm(z, x) >= 0)) ++mismatches[3];
     }
 }
 

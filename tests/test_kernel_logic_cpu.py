@@ -71,6 +71,6 @@ def test_normal_operand_fast_ops_equal_general_ops(harness):
     import ctypes
     harness.fav_host_check_nn_ops.argtypes = [ctypes.c_longlong, ctypes.c_ulonglong,
                                               ctypes.POINTER(ctypes.c_longlong)]
-    out = (ctypes.c_longlong * 3)()
+    out = (ctypes.c_longlong * 4)()
     harness.fav_host_check_nn_ops(2_000_000, 20261203, out)
-    assert list(out) == [0, 0, 0]
+    assert list(out) == [0, 0, 0, 0]
