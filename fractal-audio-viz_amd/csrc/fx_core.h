@@ -42,8 +42,8 @@ FAV_HD float ldexp_f(float v, int e) {
 
 // Larger part scaled into [0.5, 1); zero maps to exponent 0.
 FAV_HD FX normalize(float re, float im, int e) {
-    const float ar = __builtin_fabsf(re), ai = __builtin_fabsf(im);
-    const float magnitude = ar < ai ? ai : ar;
+    // the larger magnitude (operands are finite): one max instruction
+    const float magnitude = __builtin_fmaxf(__builtin_fabsf(re), __builtin_fabsf(im));
     const int shift = magnitude == 0.0F ? 0 : frexp_exp_f(magnitude);
     return FX{ldexp_f(re, -shift), ldexp_f(im, -shift), magnitude == 0.0F ? 0 : e + shift};
 }
