@@ -291,8 +291,9 @@ def main():
     if args.copy_calibration_mib:
         return copy_calibration(args.copy_calibration_mib, max(1, args.steps))
     if args.gpus != 1:
-        print("bench.py measures one GPU: the field path has no cross-GPU exchange in this build "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build renders on a single device (fields and tiles "
+              "shard across GPUs without exchange, but no multi-GPU path is built); run with --gpus 1",
+              file=sys.stderr)
         return 2
 
     work = fav_workloads.field_1e100_1080p()

@@ -1,6 +1,6 @@
 """The FP32 floatexp path against the FP64 kernel.
 
-This path is not the reference's arithmetic, so it is judged by the
+This path is not the reference's arithmetic, so it is measured by the
 fraction of pixels whose escape count differs from the FP64 result and by
 the largest smooth-value difference on pixels whose counts agree: exact
 agreement where the field is interior or shallow in iterations, and a
