@@ -21,6 +21,7 @@
 #include "fav_host.h"
 #include "fav_render.h"
 #include "sched_mlp.h"
+#include "atlas_core.h"
 #include "kfp_prepare.h"
 #include "direct_prepare.h"
 
@@ -361,6 +362,86 @@ int colourise_kfp(const float* field, uint8_t* output, int width, int height, in
     }
 }
 
+// fractal_atlas_colourise_kfp_raw, renderer.cpp:17969-18265: the same
+// validation and geometry, the compose step and the KFP pass on the device.
+int atlas_kfp_raw(const float* parent, int parent_width, int parent_height, int parent_max_iter, const float* child,
+                  int child_width, int child_height, int child_max_iter, uint8_t* output, int output_width,
+                  int output_height, double parent_zoom, double child_fraction, double child_zoom, int max_iter,
+                  double phase, double vocal, double instrumental, double pitch, const FractalKfpOptions* options,
+                  const uint8_t* lut, int lut_size, int threads) {
+    try {
+        if (!parent || !output || !valid_dimensions(parent_width, parent_height)
+            || !valid_dimensions(output_width, output_height) || !valid_iterations(parent_max_iter)
+            || !valid_iterations(max_iter) || !valid_threads(threads) || !std::isfinite(parent_zoom)
+            || parent_zoom <= 0.0 || !std::isfinite(child_fraction) || child_fraction < 0.0 || child_fraction > 1.0
+            || !std::isfinite(child_zoom) || child_zoom <= 0.0 || !std::isfinite(phase) || !std::isfinite(vocal)
+            || !std::isfinite(instrumental) || !std::isfinite(pitch) || !lut
+            || !fav::kfp::valid_options(options, lut_size))
+            throw std::runtime_error("invalid native raw KFP atlas dimensions or controls");
+        parent_zoom = std::max(parent_zoom, 1.0);
+        child_zoom = std::max(child_zoom, 1.0);
+        const bool use_child = child != nullptr && child_fraction > 0.0;
+        if (use_child && (!valid_dimensions(child_width, child_height) || !valid_iterations(child_max_iter)))
+            throw std::runtime_error("invalid native raw KFP atlas child tile");
+        if (!use_child && child != nullptr)
+            throw std::runtime_error("raw KFP atlas child needs a positive visible fraction");
+        if (!use_child && (child_width != 0 || child_height != 0 || child_max_iter != 0))
+            throw std::runtime_error("invalid native raw KFP atlas child absence");
+        if (options->field_bias > static_cast<double>(max_iter))
+            throw std::runtime_error("native KFP field bias exceeds iteration cap");
+        // centred tiles are stored relative to their own caps
+        const bool centered = options->field_bias > 0.0;
+        fav::atlas::ComposeParams a{};
+        a.parent = parent;
+        a.child = use_child ? child : nullptr;
+        a.parent_bias = centered ? static_cast<double>(parent_max_iter) : 0.0;
+        a.child_bias = centered ? static_cast<double>(child_max_iter) : 0.0;
+        a.field_bias = options->field_bias;
+        a.parent_width = parent_width;
+        a.parent_max_iter = parent_max_iter;
+        a.child_width = use_child ? child_width : 0;
+        a.child_max_iter = use_child ? child_max_iter : 0;
+        a.width = output_width;
+        a.height = output_height;
+        a.max_iter = max_iter;
+        fav::atlas::Axis axes[4];
+        fav::atlas::fill_axis(axes[0], parent_width, output_width, parent_zoom);
+        fav::atlas::fill_axis(axes[1], parent_height, output_height, parent_zoom);
+        if (use_child) {
+            const int visible_w = std::max(1, static_cast<int>(std::lround(static_cast<double>(output_width) * child_fraction)));
+            const int visible_h = std::max(1, static_cast<int>(std::lround(static_cast<double>(output_height) * child_fraction)));
+            const int left = (output_width - visible_w) / 2, top = (output_height - visible_h) / 2;
+            const int halo = std::min(2, std::min(std::min(left, output_width - left - visible_w),
+                                                  std::min(top, output_height - top - visible_h)));
+            a.use_child = 1;
+            a.visible_left = left;
+            a.visible_top = top;
+            a.visible_width = visible_w;
+            a.visible_height = visible_h;
+            a.child_left = left - halo;
+            a.child_top = top - halo;
+            a.feather = fav::atlas::seam_feather(visible_w, visible_h);
+            fav::atlas::fill_axis_window(axes[2], child_width, visible_w + 2 * halo, -static_cast<double>(halo),
+                                         static_cast<double>(visible_w), child_zoom);
+            fav::atlas::fill_axis_window(axes[3], child_height, visible_h + 2 * halo, -static_cast<double>(halo),
+                                         static_cast<double>(visible_h), child_zoom);
+        }
+        const fav::kfp::KfpParams k = fav::kfp::make_params(*options, nullptr, output, output_width, output_height,
+                                                            max_iter, lut, lut_size);
+        fav::atlas_colourise_kfp_raw(
+            a, static_cast<size_t>(parent_width) * static_cast<size_t>(parent_height),
+            use_child ? static_cast<size_t>(child_width) * static_cast<size_t>(child_height) : 0, axes, k, output);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native raw KFP atlas colourizer failed with an unknown exception");
+        return 1;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -392,6 +473,31 @@ int fractal_colourise_kfp_planes(const float* field, uint8_t* output, int width,
                                  const uint8_t* lut, int lut_size, int threads) {
     return colourise_kfp(field, output, width, height, max_iter, phase, vocal, instrumental, pitch, options, planes,
                          lut, lut_size, threads, false);
+}
+
+int fractal_atlas_colourise_kfp_raw(const float* parent, int parent_width, int parent_height, int parent_max_iter,
+                                    const float* child, int child_width, int child_height, int child_max_iter,
+                                    uint8_t* output, int output_width, int output_height, double parent_zoom,
+                                    double child_fraction, double child_zoom, int max_iter, double phase,
+                                    double vocal, double instrumental, double pitch,
+                                    const FractalKfpOptions* options, const uint8_t* lut, int lut_size,
+                                    int threads) {
+    return atlas_kfp_raw(parent, parent_width, parent_height, parent_max_iter, child, child_width, child_height,
+                         child_max_iter, output, output_width, output_height, parent_zoom, child_fraction,
+                         child_zoom, max_iter, phase, vocal, instrumental, pitch, options, lut, lut_size, threads);
+}
+
+int fractal_atlas_colourise_kfp_raw_precise(const float* parent, int parent_width, int parent_height,
+                                            int parent_max_iter, const float* child, int child_width,
+                                            int child_height, int child_max_iter, uint8_t* output,
+                                            int output_width, int output_height, double parent_zoom,
+                                            double child_fraction, double child_zoom, int max_iter, double phase,
+                                            double vocal, double instrumental, double pitch,
+                                            const FractalKfpOptions* options, const uint8_t* lut, int lut_size,
+                                            int threads) {
+    return atlas_kfp_raw(parent, parent_width, parent_height, parent_max_iter, child, child_width, child_height,
+                         child_max_iter, output, output_width, output_height, parent_zoom, child_fraction,
+                         child_zoom, max_iter, phase, vocal, instrumental, pitch, options, lut, lut_size, threads);
 }
 
 int fractal_hip_colourise_kfp_device(const float* device_field, uint8_t* device_rgb, int width, int height,

@@ -19,6 +19,7 @@
 #include "direct_core.h"
 #include "fx_core.h"
 #include "sched_mlp.h"
+#include "atlas_core.h"
 
 namespace fav {
 
@@ -29,6 +30,7 @@ hipError_t launch_direct_kernel(const direct::DirectParams& p, bool planes, hipS
 hipError_t launch_alt_kernel(const AltRenderParams& p, hipStream_t stream);
 hipError_t launch_fx_kernel(const RenderParams& p, const fx::FxOut& out, hipStream_t stream);
 hipError_t launch_batch(const BatchLaunch& q, int fast_blocks, int persistent_blocks, hipStream_t stream);
+hipError_t launch_atlas_compose(const atlas::ComposeParams& a, hipStream_t stream);
 
 namespace {
 
@@ -70,7 +72,8 @@ struct LaunchState {
     hipEvent_t mark[3] = {nullptr, nullptr, nullptr};
     DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords, fx_counts, tile_order, work,
         mlp_y, mlp_act, mlp_delta, mlp_err2, mlp_partial, mlp_grad, mlp_features, mlp_weights, preview_offsets,
-        preview_output, preview_work;
+        preview_output, preview_work,
+        atlas_tiles, atlas_axes, atlas_field;
     int compute_units = 256;
     double wall_clock_khz = 0.0;
     LastTiming timing{};
@@ -326,6 +329,68 @@ void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers) {
     if (!device_pointers)
         check(hipMemcpyAsync(host.output, k.output, pixels * 3, hipMemcpyDeviceToHost, s.stream), "D2H rgb");
     check(hipStreamSynchronize(s.stream), "KFP colour");
+    float kernel_ms = 0.0f;
+    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    s.timing.kernel_ms = kernel_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
+}
+
+void atlas_colourise_kfp_raw(const atlas::ComposeParams& host, size_t parent_count, size_t child_count,
+                             const atlas::Axis* axes, const kfp::KfpParams& colour, uint8_t* output) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const size_t pixels = static_cast<size_t>(host.width) * static_cast<size_t>(host.height);
+    atlas::ComposeParams a = host;
+    // tiles: parent then child, 256-byte aligned
+    const size_t child_at = (parent_count * 4 + 255) & ~size_t(255);
+    char* tiles = static_cast<char*>(s.atlas_tiles.get(child_at + child_count * 4 + 256));
+    check(hipMemcpyAsync(tiles, host.parent, parent_count * 4, hipMemcpyHostToDevice, s.stream), "H2D parent");
+    a.parent = reinterpret_cast<const float*>(tiles);
+    a.child = nullptr;
+    if (child_count) {
+        check(hipMemcpyAsync(tiles + child_at, host.child, child_count * 4, hipMemcpyHostToDevice, s.stream),
+              "H2D child");
+        a.child = reinterpret_cast<const float*>(tiles + child_at);
+    }
+    // the four axes: index0, index1, weight of each, packed
+    size_t bytes = 0;
+    for (int i = 0; i < 4; ++i) bytes += ((axes[i].weight.size() * 4 + 255) & ~size_t(255)) * 3;
+    char* ax = static_cast<char*>(s.atlas_axes.get(bytes + 256));
+    atlas::AxisDev* views[4] = {&a.parent_x, &a.parent_y, &a.child_x, &a.child_y};
+    size_t at = 0;
+    for (int i = 0; i < 4; ++i) {
+        const size_t n = axes[i].weight.size();
+        const size_t stride = (n * 4 + 255) & ~size_t(255);
+        *views[i] = atlas::AxisDev{nullptr, nullptr, nullptr};
+        if (n == 0) continue;
+        check(hipMemcpyAsync(ax + at, axes[i].index0.data(), n * 4, hipMemcpyHostToDevice, s.stream), "H2D axis");
+        check(hipMemcpyAsync(ax + at + stride, axes[i].index1.data(), n * 4, hipMemcpyHostToDevice, s.stream),
+              "H2D axis");
+        check(hipMemcpyAsync(ax + at + 2 * stride, axes[i].weight.data(), n * 4, hipMemcpyHostToDevice, s.stream),
+              "H2D axis");
+        *views[i] = atlas::AxisDev{reinterpret_cast<const int32_t*>(ax + at),
+                                   reinterpret_cast<const int32_t*>(ax + at + stride),
+                                   reinterpret_cast<const float*>(ax + at + 2 * stride)};
+        at += 3 * stride;
+    }
+    a.out = static_cast<float*>(s.atlas_field.get(pixels * sizeof(float)));
+    kfp::KfpParams k = colour;
+    k.field = a.out;
+    k.output = static_cast<uint8_t*>(s.kfp_rgb.get(pixels * 3));
+    void* l = s.kfp_lut.get(static_cast<size_t>(colour.lut_size) * 3);
+    check(hipMemcpyAsync(l, colour.lut, static_cast<size_t>(colour.lut_size) * 3, hipMemcpyHostToDevice, s.stream),
+          "H2D lut");
+    k.lut = static_cast<const uint8_t*>(l);
+    void* scratch = s.kfp_scratch.get(64);
+    const uint64_t wall0 = now_ns();
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_atlas_compose(a, s.stream), "atlas compose launch");
+    check(launch_kfp_kernel(k, scratch, s.compute_units, s.stream), "KFP colour launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    check(hipMemcpyAsync(output, k.output, pixels * 3, hipMemcpyDeviceToHost, s.stream), "D2H rgb");
+    check(hipStreamSynchronize(s.stream), "atlas colour");
     float kernel_ms = 0.0f;
     check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
     s.timing.kernel_ms = kernel_ms;

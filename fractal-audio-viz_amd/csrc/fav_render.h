@@ -86,6 +86,16 @@ void mlp_train(float* weights, float* adam_m, float* adam_v, const float* x, con
 namespace kfp { struct KfpParams; }
 namespace direct { struct DirectParams; }
 
+// Raw KFP atlas colour pass (atlas_core.h): `compose` carries host pointers
+// to the parent (parent_count floats) and child (child_count floats) tiles
+// and the geometry; axes[0..3] are the parent x / y and child x / y maps;
+// `colour` the KFP parameters (its field and output are replaced).  RGB is
+// copied to the host buffer `output`.
+namespace atlas { struct ComposeParams; struct Axis; }
+void atlas_colourise_kfp_raw(const atlas::ComposeParams& compose, size_t parent_count, size_t child_count,
+                             const atlas::Axis* axes, const kfp::KfpParams& colour, uint8_t* output);
+
+
 // Shallow direct render on the device: x/y are the per-column/per-row
 // coordinates; `params` output/plane pointers are host pointers unless
 // device_pointers is set.

@@ -288,6 +288,30 @@ int fractal_colourise_kfp_precise(
     const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
     int threads);
 
+/* Raw KFP atlas colour pass: the parent tile (zoomed by parent_zoom) and
+ * the child tile (its centre child_fraction of the frame, zoomed by
+ * child_zoom, feathered in over up to 64 pixels) reprojected into one
+ * output-sized scalar field with interior-preserving bilinear sampling,
+ * then one KFP pass.  Tiles stored relative to their own caps when
+ * options->field_bias > 0.  renderer.h:663-686 / renderer.cpp:17969-18265. */
+int fractal_atlas_colourise_kfp_raw(
+    const float *parent, int parent_width, int parent_height,
+    int parent_max_iter, const float *child, int child_width, int child_height,
+    int child_max_iter, uint8_t *output, int output_width, int output_height,
+    double parent_zoom, double child_fraction, double child_zoom, int max_iter,
+    double phase, double vocal, double instrumental, double pitch,
+    const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
+    int threads);
+/* renderer.h:688-711 / renderer.cpp:18366-18395: the same pass. */
+int fractal_atlas_colourise_kfp_raw_precise(
+    const float *parent, int parent_width, int parent_height,
+    int parent_max_iter, const float *child, int child_width, int child_height,
+    int child_max_iter, uint8_t *output, int output_width, int output_height,
+    double parent_zoom, double child_fraction, double child_zoom, int max_iter,
+    double phase, double vocal, double instrumental, double pitch,
+    const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
+    int threads);
+
 /* ---- Extensions of the HIP backend (no counterpart in renderer.h) ------ */
 
 /* The FP32 floatexp perturbation path of a Mandelbrot handle (no

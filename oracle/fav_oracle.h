@@ -91,6 +91,17 @@ int fo_colourise_kfp(
     const uint8_t *lut, int lut_size, int threads,
     char *error, int error_capacity);
 
+/* Raw KFP atlas colour pass (renderer.cpp:17969-18265): parent and child
+ * scalar tiles reprojected into one frame, the child feathered into the
+ * centre, then the scalar-field KFP transfer. */
+int fo_atlas_colourise_kfp_raw(
+    const float *parent, int parent_width, int parent_height, int parent_max_iter,
+    const float *child, int child_width, int child_height, int child_max_iter,
+    uint8_t *output, int output_width, int output_height, double parent_zoom,
+    double child_fraction, double child_zoom, int max_iter,
+    const FractalKfpOptions *options, const uint8_t *lut, int lut_size, int threads,
+    char *error, int error_capacity);
+
 /* 1024-entry cyclic sine-interpolated palette (visualizer.py:13492-13521). */
 int fo_kfp_palette_lut(const uint8_t *stops, int stop_count, uint8_t *lut,
                        int lut_size);

@@ -391,3 +391,191 @@ int fo_kfp_palette_lut(const uint8_t *stops, int stop_count, uint8_t *lut, int l
     return 0;
 }
 
+
+/* ---- raw KFP atlas colour pass ------------------------------------------ */
+
+/* fill_bilinear_axis_window, renderer.cpp:4969-5002 (fill_bilinear_axis,
+ * renderer.cpp:4931-4962, is the window with origin 0 and span equal to the
+ * destination size). */
+static void atlas_axis(int *index0, int *index1, float *weight, int source_size, int destination_size,
+                       double origin, double span, double zoom) {
+    zoom = zoom > 1.0 ? zoom : 1.0;
+    const double crop = (double)source_size / zoom;
+    const double left = ((double)source_size - crop) * 0.5;
+    for (int d = 0; d < destination_size; ++d) {
+        const double logical = origin + (double)d + 0.5;
+        double source = clampd(left + logical * crop / span - 0.5, 0.0, (double)(source_size - 1));
+        const int i0 = (int)floor(source);
+        index0[d] = i0;
+        index1[d] = i0 + 1 < source_size - 1 ? i0 + 1 : source_size - 1;
+        weight[d] = (float)(source - (double)i0);
+    }
+}
+
+/* fill_bilinear_axis, renderer.cpp:4931-4962: the plain centred crop. */
+static void atlas_axis_plain(int *index0, int *index1, float *weight, int source_size, int destination_size,
+                             double zoom) {
+    zoom = zoom > 1.0 ? zoom : 1.0;
+    const double inverse_zoom = 1.0 / zoom;
+    const double crop = (double)source_size * inverse_zoom;
+    const double left = ((double)source_size - crop) * 0.5;
+    const double step = crop / (double)destination_size;
+    for (int d = 0; d < destination_size; ++d) {
+        double source = clampd(left + ((double)d + 0.5) * step - 0.5, 0.0, (double)(source_size - 1));
+        const int i0 = (int)floor(source);
+        index0[d] = i0;
+        index1[d] = i0 + 1 < source_size - 1 ? i0 + 1 : source_size - 1;
+        weight[d] = (float)(source - (double)i0);
+    }
+}
+
+/* sample_bilinear_mapped_preserving_interior, renderer.cpp:5146-5208. */
+static float atlas_sample(const float *src, int width, const int *x0, const int *x1, const float *xw,
+                          const int *y0, const int *y1, const float *yw, int x, int y, int max_iter, int *inside,
+                          double bias) {
+    const double wx = (double)xw[x], wy = (double)yw[y];
+    const float *top = src + (size_t)y0[y] * (size_t)width;
+    const float *bottom = src + (size_t)y1[y] * (size_t)width;
+    const float values[4] = {top[x0[x]], top[x1[x]], bottom[x0[x]], bottom[x1[x]]};
+    const double weights[4] = {(1.0 - wy) * (1.0 - wx), (1.0 - wy) * wx, wy * (1.0 - wx), wy * wx};
+    double interior = 0.0, exterior = 0.0, value = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        const float v = values[k];
+        if (isfinite(v) && (double)v >= (double)max_iter - bias) {
+            interior += weights[k];
+        } else if (isfinite(v)) {
+            exterior += weights[k];
+            value += ((double)v + bias) * weights[k];
+        }
+    }
+    if (interior >= 0.5) {
+        *inside = 1;
+        return (float)max_iter;
+    }
+    *inside = 0;
+    if (exterior <= 1.0e-12) return 0.0f;
+    return (float)(value / exterior);
+}
+
+/* encode_render_value, renderer.cpp:119-125 (long double subtraction). */
+static float atlas_encode(long double value, double bias) { return (float)(value - (long double)bias); }
+
+/* kfp_atlas_seam_feather, renderer.cpp:99-110 (maximum 64, divisor 2). */
+static int atlas_feather(int width, int height) {
+    const int minimum = width < height ? width : height;
+    if (minimum <= 0) return 0;
+    int requested = minimum / 2 < 64 ? minimum / 2 : 64;
+    if (requested < 2) requested = 2;
+    const int plateau = (minimum - 1) / 2 > 0 ? (minimum - 1) / 2 : 0;
+    return requested < plateau ? requested : plateau;
+}
+
+int fo_atlas_colourise_kfp_raw(const float *parent, int parent_width, int parent_height, int parent_max_iter,
+                               const float *child, int child_width, int child_height, int child_max_iter,
+                               uint8_t *output, int output_width, int output_height, double parent_zoom,
+                               double child_fraction, double child_zoom, int max_iter,
+                               const FractalKfpOptions *o, const uint8_t *lut, int lut_size, int threads,
+                               char *error, int error_capacity) {
+    /* fractal_atlas_colourise_kfp_raw, renderer.cpp:17969-18265 */
+    if (!parent || !output || !o || !lut || parent_width <= 0 || parent_height <= 0 || output_width <= 0
+        || output_height <= 0 || !isfinite(parent_zoom) || parent_zoom <= 0.0 || !isfinite(child_fraction)
+        || child_fraction < 0.0 || child_fraction > 1.0 || !isfinite(child_zoom) || child_zoom <= 0.0) {
+        kfp_error(error, error_capacity, "invalid native raw KFP atlas dimensions or controls");
+        return 1;
+    }
+    parent_zoom = parent_zoom > 1.0 ? parent_zoom : 1.0;
+    child_zoom = child_zoom > 1.0 ? child_zoom : 1.0;
+    const int use_child = child != NULL && child_fraction > 0.0;
+    if (use_child && (child_width <= 0 || child_height <= 0 || child_max_iter <= 0)) {
+        kfp_error(error, error_capacity, "invalid native raw KFP atlas child tile");
+        return 1;
+    }
+    if (o->field_bias > (double)max_iter) {
+        kfp_error(error, error_capacity, "native KFP field bias exceeds iteration cap");
+        return 1;
+    }
+    const int centered = o->field_bias > 0.0;
+    const double parent_bias = centered ? (double)parent_max_iter : 0.0;
+    const double child_bias = centered ? (double)child_max_iter : 0.0;
+    int visible_w = 0, visible_h = 0, visible_left = 0, visible_top = 0, halo = 0;
+    if (use_child) {
+        visible_w = (int)lround((double)output_width * child_fraction);
+        visible_h = (int)lround((double)output_height * child_fraction);
+        if (visible_w < 1) visible_w = 1;
+        if (visible_h < 1) visible_h = 1;
+        visible_left = (output_width - visible_w) / 2;
+        visible_top = (output_height - visible_h) / 2;
+        int m = visible_left;
+        if (output_width - visible_left - visible_w < m) m = output_width - visible_left - visible_w;
+        if (visible_top < m) m = visible_top;
+        if (output_height - visible_top - visible_h < m) m = output_height - visible_top - visible_h;
+        halo = m < 2 ? m : 2;
+    }
+    const int child_left = visible_left - halo, child_top = visible_top - halo;
+    const int sampled_w = use_child ? visible_w + 2 * halo : 0, sampled_h = use_child ? visible_h + 2 * halo : 0;
+    const int feather = use_child ? atlas_feather(visible_w, visible_h) : 0;
+
+    const size_t n = (size_t)output_width * (size_t)output_height;
+    int *px0 = malloc(sizeof(int) * (size_t)output_width), *px1 = malloc(sizeof(int) * (size_t)output_width);
+    int *py0 = malloc(sizeof(int) * (size_t)output_height), *py1 = malloc(sizeof(int) * (size_t)output_height);
+    float *pxw = malloc(sizeof(float) * (size_t)output_width), *pyw = malloc(sizeof(float) * (size_t)output_height);
+    float *view = malloc(sizeof(float) * n);
+    int sw = sampled_w > 0 ? sampled_w : 1, sh = sampled_h > 0 ? sampled_h : 1;
+    int *cx0 = malloc(sizeof(int) * (size_t)sw), *cx1 = malloc(sizeof(int) * (size_t)sw);
+    int *cy0 = malloc(sizeof(int) * (size_t)sh), *cy1 = malloc(sizeof(int) * (size_t)sh);
+    float *cxw = malloc(sizeof(float) * (size_t)sw), *cyw = malloc(sizeof(float) * (size_t)sh);
+    float *child_view = malloc(sizeof(float) * (size_t)sw * (size_t)sh);
+    int status = 1;
+    if (!px0 || !px1 || !py0 || !py1 || !pxw || !pyw || !view || !cx0 || !cx1 || !cy0 || !cy1 || !cxw || !cyw
+        || !child_view) {
+        kfp_error(error, error_capacity, "out of memory");
+        goto done;
+    }
+    atlas_axis_plain(px0, px1, pxw, parent_width, output_width, parent_zoom);
+    atlas_axis_plain(py0, py1, pyw, parent_height, output_height, parent_zoom);
+    for (int y = 0; y < output_height; ++y)
+        for (int x = 0; x < output_width; ++x) {
+            int inside = 0;
+            const float smooth = atlas_sample(parent, parent_width, px0, px1, pxw, py0, py1, pyw, x, y,
+                                              parent_max_iter, &inside, parent_bias);
+            view[(size_t)y * (size_t)output_width + (size_t)x] = inside
+                ? atlas_encode((long double)max_iter, o->field_bias)
+                : atlas_encode((long double)smooth, o->field_bias);
+        }
+    if (use_child) {
+        atlas_axis(cx0, cx1, cxw, child_width, sampled_w, -(double)halo, (double)visible_w, child_zoom);
+        atlas_axis(cy0, cy1, cyw, child_height, sampled_h, -(double)halo, (double)visible_h, child_zoom);
+        for (int y = 0; y < sampled_h; ++y)
+            for (int x = 0; x < sampled_w; ++x) {
+                int inside = 0;
+                const float smooth = atlas_sample(child, child_width, cx0, cx1, cxw, cy0, cy1, cyw, x, y,
+                                                  child_max_iter, &inside, child_bias);
+                child_view[(size_t)y * (size_t)sampled_w + (size_t)x] = inside
+                    ? atlas_encode((long double)max_iter, o->field_bias)
+                    : atlas_encode((long double)smooth, o->field_bias);
+            }
+        for (int y = visible_top; y < visible_top + visible_h; ++y)
+            for (int x = visible_left; x < visible_left + visible_w; ++x) {
+                const size_t at = (size_t)y * (size_t)output_width + (size_t)x;
+                const float cv = child_view[(size_t)(y - child_top) * (size_t)sampled_w + (size_t)(x - child_left)];
+                float alpha = 1.0f;
+                if (feather >= 2) {
+                    const int vx = x - visible_left, vy = y - visible_top;
+                    int edge = vx < visible_w - 1 - vx ? vx : visible_w - 1 - vx;
+                    const int ey = vy < visible_h - 1 - vy ? vy : visible_h - 1 - vy;
+                    if (ey < edge) edge = ey;
+                    float linear = (float)edge / (float)feather;
+                    linear = linear < 0.0f ? 0.0f : (linear > 1.0f ? 1.0f : linear);
+                    alpha = linear * linear * (3.0f - 2.0f * linear);
+                }
+                if (alpha >= 0.999999f) view[at] = cv;
+                else view[at] = view[at] * (1.0f - alpha) + cv * alpha;
+            }
+    }
+    status = fo_colourise_kfp(view, output, output_width, output_height, max_iter, o, NULL, lut, lut_size, threads,
+                              error, error_capacity);
+done:
+    free(px0); free(px1); free(py0); free(py1); free(pxw); free(pyw); free(view);
+    free(cx0); free(cx1); free(cy0); free(cy1); free(cxw); free(cyw); free(child_view);
+    return status;
+}
