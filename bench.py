@@ -183,6 +183,36 @@ def learned_leg(lib, repeats=2):
             "mean_net_gain_pct": round(float(np.mean([f["net_gain_pct"] for f in fields])), 2), "fields": fields}
 
 
+def atlas_leg(lib, repeats=5):
+    """The raw KFP atlas colour pass of one video frame: 1920x1080 output
+    from two 1920x1080 scalar tiles (parent zoomed 1.5x, child a third of
+    the frame), Kalles-default colouring; HIP-event time of the compose and
+    colour kernels with the tiles resident on the host (each call copies
+    them in and the RGB out, as the reference's entry does)."""
+    parent = fav_workloads.atlas_tile(1920, 1080, 20000, seed=41)
+    child = fav_workloads.atlas_tile(1920, 1080, 24000, seed=42)
+    opts = fav_hip.NativeKfpOptions.kalles_default()
+    lut = fav_palette.kfp_palette_lut(fav_palette.seeded_stops(12, 2024), 1024)
+    out = np.empty((1080, 1920, 3), dtype=np.uint8)
+    fn = lib.fractal_atlas_colourise_kfp_raw
+    times, walls = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        status = fn(parent.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 1920, 1080, 20000,
+                    child.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 1920, 1080, 24000,
+                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), 1920, 1080, 1.5, 0.33, 1.6, 24000,
+                    0.0, 0.0, 0.0, 0.0, ctypes.byref(opts), lut.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                    lut.shape[0], 0)
+        walls.append((time.perf_counter() - t0) * 1e3)
+        if status != 0:
+            raise RuntimeError(lib.fractal_last_error().decode())
+        times.append(fav_hip.last_timing()["kernel_ms"])
+    kernel_ms = min(times)
+    return {"workload": "1920x1080 frame from 1920x1080 parent (zoom 1.5) and child (1/3 of the frame, zoom 1.6)",
+            "kernel_ms": round(kernel_ms, 4), "call_ms": round(min(walls), 3),
+            "frames_per_s_kernel": round(1e3 / kernel_ms, 1), "rgb_checksum": int(out.astype(np.uint64).sum())}
+
+
 def floatexp_boundary_mismatch(lib):
     """The floatexp path where escape counts are sensitive: the 640x360
     zoom-1e12 field (cap 20000, every pixel near the boundary), against the
@@ -375,6 +405,7 @@ def main():
     floatexp = floatexp_leg(lib, hip, ref, work, opts, d_field, field, orbit_iteration, args)
     batched = None if args.no_batch else batch_leg(lib, hip, 2)
     learned = None if args.no_learned else learned_leg(lib)
+    atlas = atlas_leg(lib)
     hip.free(d_field)
     hip.free(d_planes_block)
     hip.free(d_rgb)
@@ -413,6 +444,7 @@ def main():
         "floatexp": floatexp,
         "batched": batched,
         "learned_schedule": learned,
+        "atlas_colour": atlas,
     }
     print(json.dumps(result))
     return 0

@@ -201,6 +201,14 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
         ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_void_p]
     lib.fractal_hip_cost_model_train.restype = ctypes.c_int
+    lib.fractal_atlas_colourise_kfp_raw.argtypes = [
+        _c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        _c_uint8_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.POINTER(NativeKfpOptions),
+        _c_uint8_p, ctypes.c_int, ctypes.c_int]
+    lib.fractal_atlas_colourise_kfp_raw.restype = ctypes.c_int
+    lib.fractal_atlas_colourise_kfp_raw_precise.argtypes = lib.fractal_atlas_colourise_kfp_raw.argtypes
+    lib.fractal_atlas_colourise_kfp_raw_precise.restype = ctypes.c_int
     lib.fractal_set_stats_enabled.argtypes = [ctypes.c_int]
     lib.fractal_get_last_stats_ex.argtypes = [_c_uint64_p, ctypes.c_int]
     lib.fractal_get_last_stats_ex.restype = ctypes.c_int

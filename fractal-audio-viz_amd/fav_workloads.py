@@ -129,3 +129,22 @@ def tricorn_field(point: str = "branching-junction", width: int = 3840, height: 
     x, y = TRICORN_POINTS[point]
     return dict(name=f"tricorn_{point}_{width}x{height}_zoom{zoom}_cap{max_iter}", formula=3, julia=("0", "0"),
                 x=x, y=y, zoom=zoom, width=width, height=height, max_iter=max_iter, bits=480)
+
+
+def atlas_tile(width, height, max_iter, seed, centered=False):
+    """A synthetic scalar atlas tile: smooth escape ramps with an interior
+    block, an interior left edge and one glitch sentinel; with `centered`,
+    stored relative to the tile's own cap (value - max_iter) as the
+    production atlas encoding does."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:height, 0:width]
+    field = (300.0 + 40.0 * np.sin(x / 6.0) * np.cos(y / 5.0) + 1.3 * x - 0.7 * y
+             + rng.normal(0, 1.5, (height, width)))
+    field[height // 3:height // 3 + 7, width // 2:width // 2 + 9] = max_iter   # interior block
+    field[:, :3] = max_iter                                                  # interior left edge
+    field[height - 2, width - 5] = np.nan                                    # glitch sentinel
+    field = field.astype(np.float64)
+    if centered:
+        field = field - max_iter
+    return field.astype(np.float32)

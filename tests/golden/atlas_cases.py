@@ -2,26 +2,8 @@
 (fractal_atlas_colourise_kfp_raw, renderer.cpp:17969-18265): a parent and a
 child scalar field reprojected into one frame (bilinear, interior-
 preserving), the child feathered into the centre, then one KFP pass."""
-import numpy as np
-
+from fav_workloads import atlas_tile as synthetic_tile  # noqa: F401
 from kfp_cases import KALLES_DEFAULT
-
-
-def synthetic_tile(width, height, max_iter, seed, centered=False):
-    """Smooth escape ramps with interior blocks and a few glitch sentinels;
-    with `centered`, stored relative to the tile's own cap (value - max_iter)
-    as the production atlas encoding does."""
-    rng = np.random.default_rng(seed)
-    y, x = np.mgrid[0:height, 0:width]
-    field = (300.0 + 40.0 * np.sin(x / 6.0) * np.cos(y / 5.0) + 1.3 * x - 0.7 * y
-             + rng.normal(0, 1.5, (height, width)))
-    field[height // 3:height // 3 + 7, width // 2:width // 2 + 9] = max_iter   # interior block
-    field[:, :3] = max_iter                                                  # interior left edge
-    field[height - 2, width - 5] = np.nan                                    # glitch sentinel
-    field = field.astype(np.float64)
-    if centered:
-        field = field - max_iter
-    return field.astype(np.float32)
 
 
 def _case(name, output=(192, 108), parent=(160, 90, 5000), child=(120, 68, 6000), parent_zoom=1.7,

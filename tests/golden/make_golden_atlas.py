@@ -13,6 +13,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE.parent))
 sys.path.insert(0, str(HERE))
+sys.path.insert(0, str(HERE.parents[1] / "fractal-audio-viz_amd"))
 
 import atlas_io  # noqa: E402
 import native  # noqa: E402

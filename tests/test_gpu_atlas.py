@@ -8,7 +8,7 @@ import pytest
 
 import atlas_io
 import golden_io
-from atlas_cases import synthetic_tile
+from fav_workloads import atlas_tile as synthetic_tile
 
 pytestmark = pytest.mark.gpu
 
