@@ -33,6 +33,7 @@ constexpr int kMaxIterations = 10'000'000;   // renderer.cpp:88-93
 constexpr int kMaxThreads = 4096;
 constexpr int kMaxPrecisionBits = 131'072;
 constexpr int kMaxPixels = 100'000'000;
+constexpr int kMaxPoints = 100000000;  // renderer.cpp:92
 constexpr size_t kMaxTextLength = 50'000;
 
 thread_local std::string last_error;  // renderer.cpp:4596-4599
@@ -277,6 +278,67 @@ int render_impl(float* output, int width, int height, const char* zoom, void* ha
         return 1;
     } catch (...) {
         set_error("native reference render failed with an unknown exception");
+        return 1;
+    }
+}
+
+// fractal_render_points_impl, renderer.cpp:20019-20186: the same checks and
+// the deep field loop over caller-supplied dc offsets instead of a pixel
+// grid (Mandelbrot handles, project coordinates, no planes on this
+// backend).
+int render_points_impl(float* output, int point_count, const char* zoom, const double* real_mantissa,
+                       const double* imag_mantissa, const int32_t* exponents, void* handle, int max_iter,
+                       int threads, int series_order, int series_block, const FractalRenderOptions* supplied,
+                       FractalRenderPlanes* planes) {
+    try {
+        if (!output || point_count <= 0 || point_count > kMaxPoints || !zoom || !real_mantissa || !imag_mantissa
+            || !exponents || !handle || (planes != nullptr && !valid_planes(planes, point_count, 1)))
+            throw std::runtime_error("invalid native point-render arguments");
+        if (!valid_iterations(max_iter) || !valid_threads(threads) || !valid_series(series_order, series_block))
+            throw std::runtime_error("invalid native point-render limits");
+        const FractalRenderOptions o = checked_options(supplied);
+        if (o.backend != 0 && o.backend != 1 && o.backend != 2) throw std::runtime_error("unknown native render backend");
+        if (planes != nullptr && o.backend != 0)
+            throw std::runtime_error("point metadata requires the native scalar backend");
+        require_hip_backend(o);
+        const auto context = acquire(handle);
+        if (!context) throw std::runtime_error("invalid or already-destroyed reference handle");
+        if (max_iter > context->requested_max_iter)
+            throw std::runtime_error("point render iteration count exceeds prepared reference");
+        if (!valid_text(zoom)) throw std::runtime_error("native zoom text is too long or null");
+        (void)fav::parse_zoom(zoom, context->precision_bits);  // malformed zoom text fails before the copy
+        std::vector<fav::SC> points(static_cast<size_t>(point_count));
+        fav::SN largest{0.0, 0};
+        for (int i = 0; i < point_count; ++i) {
+            fav::SC point{real_mantissa[i], imag_mantissa[i], exponents[i]};
+            if (!fav::sc_finite(point)) throw std::runtime_error("non-finite point offset");
+            fav::sc_normalize(point);
+            points[static_cast<size_t>(i)] = point;
+            const fav::SN radius = fav::sc_norm_squared(point);
+            if (fav::sc_compare_norm(radius, largest) > 0) largest = radius;
+        }
+        if (o.coordinate_mode != 0)  // as the reference's GPU point path (renderer.cpp:20093-20097)
+            throw std::runtime_error("point rendering on the HIP backend requires project coordinates");
+        if (context->formula != 0)
+            throw std::runtime_error("point rendering on the HIP backend takes Mandelbrot handles");
+        // the points as rows of up to 256, so wavefronts are filled
+        const int row = point_count < 256 ? point_count : 256;
+        uint64_t stats[fav::kStatsFields] = {};
+        const bool collect = stats_enabled.load(std::memory_order_relaxed);
+        fav::FieldRequest request{output, row, (point_count + row - 1) / row, zoom, max_iter, series_order,
+                                  series_block, o, nullptr, collect ? stats : nullptr, false};
+        request.points = points.data();
+        request.point_count = point_count;
+        request.point_radius = fav::fe_sqrt(fav::FE{largest.m, largest.e});
+        fav::render_field(*context, request);
+        if (collect) publish_stats(stats);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native point render failed with an unknown exception");
         return 1;
     }
 }
@@ -951,6 +1013,21 @@ int fractal_hip_cost_model_train(float* weights, float* adam_m, float* adam_v, c
         fav::mlp_train(weights, adam_m, adam_v, features, targets, count, steps, learning_rate, first_step,
                        loss_history);
     });
+}
+
+int fractal_render_points(float* output, int point_count, const char* zoom, const double* real_mantissa,
+                          const double* imag_mantissa, const int32_t* exponents, void* handle, int max_iter,
+                          int threads, int series_order, int series_block, const FractalRenderOptions* options) {
+    return render_points_impl(output, point_count, zoom, real_mantissa, imag_mantissa, exponents, handle, max_iter,
+                              threads, series_order, series_block, options, nullptr);
+}
+
+int fractal_render_points_ex_planes(float* output, int point_count, const char* zoom, const double* real_mantissa,
+                                    const double* imag_mantissa, const int32_t* exponents, void* handle,
+                                    int max_iter, int threads, int series_order, int series_block,
+                                    const FractalRenderOptions* options, FractalRenderPlanes* planes) {
+    return render_points_impl(output, point_count, zoom, real_mantissa, imag_mantissa, exponents, handle, max_iter,
+                              threads, series_order, series_block, options, planes);
 }
 
 int fractal_hip_render_reference_device(float* device_output, int width, int height, const char* zoom, void* handle,

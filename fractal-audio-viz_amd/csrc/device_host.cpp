@@ -73,7 +73,7 @@ struct LaunchState {
     DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords, fx_counts, tile_order, work,
         mlp_y, mlp_act, mlp_delta, mlp_err2, mlp_partial, mlp_grad, mlp_features, mlp_weights, preview_offsets,
         preview_output, preview_work,
-        atlas_tiles, atlas_axes, atlas_field;
+        atlas_tiles, atlas_axes, atlas_field, points;
     int compute_units = 256;
     double wall_clock_khz = 0.0;
     LastTiming timing{};
@@ -241,6 +241,28 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     p.y_off_m = p.x_off_m + w;
     p.x_off_e = reinterpret_cast<const int32_t*>(doff + exp_offset);
     p.y_off_e = p.x_off_e + w;
+    std::vector<double> point_m;
+    std::vector<int32_t> point_e;
+    size_t out_count = pixels;
+    if (r.points) {  // per-point offsets: re then im (doubles), then exponents
+        const size_t n = static_cast<size_t>(r.point_count);
+        point_m.resize(2 * n);
+        point_e.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            point_m[i] = r.points[i].re;
+            point_m[n + i] = r.points[i].im;
+            point_e[i] = r.points[i].e;
+        }
+        const size_t e_at = (2 * n * 8 + 255) & ~size_t(255);
+        char* dp = static_cast<char*>(s.points.get(e_at + n * 4 + 256));
+        check(hipMemcpyAsync(dp, point_m.data(), 2 * n * 8, hipMemcpyHostToDevice, s.stream), "points");
+        check(hipMemcpyAsync(dp + e_at, point_e.data(), n * 4, hipMemcpyHostToDevice, s.stream), "points");
+        p.point_re = reinterpret_cast<const double*>(dp);
+        p.point_im = p.point_re + n;
+        p.point_e = reinterpret_cast<const int32_t*>(dp + e_at);
+        p.point_count = r.point_count;
+        out_count = n;
+    }
 
     const bool want_planes = r.planes != nullptr;
     if (r.device_pointers) {
@@ -284,7 +306,8 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
           "field kernel launch");
     check(hipEventRecord(s.stop, s.stream), "event");
     if (!r.device_pointers) {
-        check(hipMemcpyAsync(r.output, p.output, pixels * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H field");
+        check(hipMemcpyAsync(r.output, p.output, out_count * sizeof(float), hipMemcpyDeviceToHost, s.stream),
+              "D2H field");
         if (want_planes) {
             check(hipMemcpyAsync(r.planes->orbit_iteration, p.plane_iteration, pixels * 8, hipMemcpyDeviceToHost, s.stream), "D2H");
             check(hipMemcpyAsync(r.planes->phase, p.plane_phase, pixels * 8, hipMemcpyDeviceToHost, s.stream), "D2H");

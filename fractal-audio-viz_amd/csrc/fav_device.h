@@ -173,6 +173,10 @@ struct RenderParams {
     StateStore state;
     OrbitDev orbit;
     SeriesDev series;
+    const double* point_re;       // point renders: per-point dc (null for a pixel grid)
+    const double* point_im;
+    const int32_t* point_e;
+    int32_t point_count;          // valid pixel indices are below this when point_re is set
     const LinearStepDev* linear;  // tier selected on the host (ordinary/deep)
     LevelDir linear_dir;
     const CubicStepDev* cubic;

@@ -23,6 +23,12 @@ struct FieldRequest {
     FractalRenderPlanes* planes;   // null when no planes are requested
     uint64_t* stats;               // kStatsFields host counters or null
     bool device_pointers;          // output/planes already live in HBM
+    // point renders (renderer.cpp:20019-20186): point_count normalised dc
+    // offsets; the field is then point_count values, laid out by the device
+    // as a width x height grid of whole rows (the last row partly used)
+    const SC* points = nullptr;
+    int point_count = 0;
+    FE point_radius{};             // sqrt of the largest |dc|^2
 };
 
 struct LastTiming {

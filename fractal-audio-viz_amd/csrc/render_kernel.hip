@@ -56,8 +56,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     const int lane = lane_id();
     const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-    const bool valid = px < p.width && py < p.height;
     const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
+    const bool valid = px < p.width && py < p.height && (!p.point_re || index < static_cast<size_t>(p.point_count));
     bool handoff = false;
     if (valid) {
         core::Counters<Stats> st;

@@ -131,8 +131,10 @@ void prepare_render(const ReferenceContext& c, const FieldRequest& r, PreparedRe
     const FE inverse_zoom = fe_div(fe_from_parts(1.0, 0), zoom);
     const FE view_height = fe_muld(inverse_zoom, factor);
     const FE view_width = fe_muld(view_height, static_cast<double>(w) / static_cast<double>(h));
-    const FE pixel_spacing = fe_div(view_height, fe_from_parts(static_cast<double>(h), 0));
-    const FE current_input_radius = fe_muld(inverse_zoom, factor);
+    // a point render is one row of point_count values (renderer.cpp:20099-20101)
+    const FE pixel_spacing = fe_div(view_height, fe_from_parts(r.points ? 1.0 : static_cast<double>(h), 0));
+    // a point render uses the radius of its farthest point (renderer.cpp:12701-12705)
+    const FE current_input_radius = r.points ? r.point_radius : fe_muld(inverse_zoom, factor);
     const bool covers = fe_compare(current_input_radius, c.input_radius) <= 0;
     const bool use_deep_linear = fe_compare(current_input_radius, c.deep_input_radius) <= 0;
     const bool use_ultra = fe_compare(current_input_radius, fe_muld(c.input_radius, 1.0e-40)) <= 0;

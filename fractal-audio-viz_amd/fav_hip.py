@@ -23,6 +23,7 @@ _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int64_p = ctypes.POINTER(ctypes.c_int64)
 _c_uint64_p = ctypes.POINTER(ctypes.c_uint64)
 _c_uint8_p = ctypes.POINTER(ctypes.c_uint8)
+_c_int32_p = ctypes.POINTER(ctypes.c_int32)
 
 
 class NativeRenderOptions(ctypes.Structure):
@@ -146,6 +147,10 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(NativeRenderOptions),
         ctypes.POINTER(NativeRenderPlanes)]
     lib.fractal_render_reference_ex_planes.restype = ctypes.c_int
+    lib.fractal_render_points.argtypes = [
+        _c_float_p, ctypes.c_int, ctypes.c_char_p, _c_double_p, _c_double_p, _c_int32_p, ctypes.c_void_p,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(NativeRenderOptions)]
+    lib.fractal_render_points.restype = ctypes.c_int
     lib.fractal_hip_render_reference_device.argtypes = [
         ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int,
         ctypes.c_int, ctypes.c_int, ctypes.POINTER(NativeRenderOptions), ctypes.c_void_p]
@@ -269,6 +274,27 @@ class NativeReference:
         if planes:
             return field, {k: v.reshape(height, width) for k, v in plane_arrays.items()}
         return field
+
+    def render_points(self, real_mantissa, imag_mantissa, exponents, zoom: str, max_iter: int,
+                      series_order: int = 3, series_block: int = 256,
+                      options: NativeRenderOptions | None = None) -> np.ndarray:
+        """float32 value per point of the offsets dc = (real, imag) * 2^exponent
+        from this reference's centre (fractal_render_points, renderer.cpp:
+        20019-20186; the call visualizer.py:6674-6790 makes for glitch cells)."""
+        re = np.ascontiguousarray(real_mantissa, dtype=np.float64).ravel()
+        im = np.ascontiguousarray(imag_mantissa, dtype=np.float64).ravel()
+        e = np.ascontiguousarray(exponents, dtype=np.int32).ravel()
+        if not (re.size == im.size == e.size):
+            raise ValueError("point mantissas and exponents must have one entry per point")
+        out = np.empty(re.size, dtype=np.float32)
+        opts = options or self.options
+        status = self.library.fractal_render_points(
+            out.ctypes.data_as(_c_float_p), int(re.size), zoom.encode(), re.ctypes.data_as(_c_double_p),
+            im.ctypes.data_as(_c_double_p), e.ctypes.data_as(_c_int32_p), self.handle, int(max_iter), 0,
+            int(series_order), int(series_block), ctypes.byref(opts))
+        if status != 0:
+            raise RuntimeError(self.library.fractal_last_error().decode())
+        return out
 
     def render_scheduled(self, width: int, height: int, zoom: str, max_iter: int, tile_order=None,
                          work: bool = False, series_order: int = 3, series_block: int = 256,

@@ -312,6 +312,27 @@ int fractal_atlas_colourise_kfp_raw_precise(
     const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
     int threads);
 
+/* Deep render of caller-supplied dc offsets (mantissas and exponents, one
+ * value per point) against a reference handle: the field loop of
+ * fractal_render_reference_ex with the points in place of a pixel grid and
+ * the farthest point's radius in place of the viewport's.  On this backend:
+ * Mandelbrot handles, project coordinates, no planes (as the reference's
+ * GPU point path).  renderer.h:287-300 / renderer.cpp:20019-20216. */
+int fractal_render_points(
+    float *output, int point_count, const char *zoom_text,
+    const double *real_mantissa, const double *imag_mantissa,
+    const int32_t *exponents, void *handle, int max_iter, int threads,
+    int series_order, int series_block, const FractalRenderOptions *options);
+/* renderer.h:307-321 / renderer.cpp:20218-20247: the same; non-NULL planes
+ * are refused on this backend ("point metadata requires the native scalar
+ * backend"). */
+int fractal_render_points_ex_planes(
+    float *output, int point_count, const char *zoom_text,
+    const double *real_mantissa, const double *imag_mantissa,
+    const int32_t *exponents, void *handle, int max_iter, int threads,
+    int series_order, int series_block, const FractalRenderOptions *options,
+    FractalRenderPlanes *planes);
+
 /* ---- Extensions of the HIP backend (no counterpart in renderer.h) ------ */
 
 /* The FP32 floatexp perturbation path of a Mandelbrot handle (no

@@ -1785,31 +1785,13 @@ static void render_pixel(const render_ctx *c, float *output, size_t index, sc_t 
 static int render_alternate(float *output, int width, int height, fe_t zoom, const fo_reference *r, int max_iter,
                             const FractalRenderOptions *o, uint64_t *stats, int threads, struct timespec started);
 
-int fo_render(float *output, int width, int height, const char *zoom_text, const fo_reference *r,
-              int max_iter, int series_order, int series_block, const FractalRenderOptions *options,
-              FractalRenderPlanes *planes, uint64_t *stats, int threads, char *error, int cap) {
-    struct timespec started;
-    clock_gettime(CLOCK_MONOTONIC, &started);
-    if (!output || !r || !options || width <= 0 || height <= 0) {
-        fo_set_error(error, cap, "invalid render arguments");
-        return 1;
-    }
-    fe_t zoom;
-    if (!parse_zoom_fe(zoom_text, r->precision_bits, &zoom)) {
-        fo_set_error(error, cap, "invalid deep Mandelbrot zoom");
-        return 1;
-    }
-    if (r->formula != 0) {
-        if (planes) {
-            fo_set_error(error, cap, "render metadata requires the native scalar backend for this formula");
-            return 1;
-        }
-        if (render_alternate(output, width, height, zoom, r, max_iter, options, stats, threads, started)) {
-            fo_set_error(error, cap, "out of memory");
-            return 1;
-        }
-        return 0;
-    }
+/* render_bla_impl's field loop (renderer.cpp:12672-13583) over a grid, or
+ * over caller-supplied normalised dc offsets (width = count, height = 1)
+ * when points is set. */
+static int render_mandel_field(float *output, int width, int height, fe_t zoom, const fo_reference *r, int max_iter,
+                             int series_order, int series_block, const FractalRenderOptions *options,
+                             FractalRenderPlanes *planes, uint64_t *stats, int threads, const sc_t *points,
+                             const fe_t *point_radius, struct timespec started, char *error, int cap) {
     render_ctx c;
     memset(&c, 0, sizeof(c));
     c.r = r;
@@ -1825,7 +1807,8 @@ int fo_render(float *output, int width, int height, const char *zoom_text, const
     const fe_t view_height = fe_muld(inverse_zoom, factor);
     const fe_t view_width = fe_muld(view_height, (double)width / (double)height);
     c.pixel_spacing = fe_div(view_height, fe_from_parts((double)height, 0));
-    const fe_t current_input_radius = fe_muld(inverse_zoom, factor);
+    /* a point render uses its farthest point's radius (renderer.cpp:12701-12705) */
+    const fe_t current_input_radius = points ? *point_radius : fe_muld(inverse_zoom, factor);
     const int covers = fe_compare(current_input_radius, r->input_radius) <= 0;
     c.disable_bla = !covers || options->disable_bla != 0;
     c.cycle_enabled = options->disable_cycle == 0 && (options->strict == 0 || options->strict_cycle != 0);
@@ -1857,12 +1840,12 @@ int fo_render(float *output, int width, int height, const char *zoom_text, const
         && fe_compare(current_input_radius, fe_sqrt(r->series_radius_squared)) <= 0
         && r->series_iteration < max_iter;
     /* per-row / per-column offsets, renderer.cpp:12764-12783 */
-    fe_t *xo = (fe_t *)malloc(sizeof(fe_t) * (size_t)width);
-    fe_t *yo = (fe_t *)malloc(sizeof(fe_t) * (size_t)height);
+    fe_t *xo = (fe_t *)malloc(sizeof(fe_t) * (size_t)(points ? 1 : width));
+    fe_t *yo = (fe_t *)malloc(sizeof(fe_t) * (size_t)(points ? 1 : height));
     if (!xo || !yo) { free(xo); free(yo); fo_set_error(error, cap, "out of memory"); return 1; }
-    for (int py = 0; py < height; ++py)
+    for (int py = 0; py < height && !points; ++py)
         yo[py] = fe_muld(view_height, -pixel_axis_offset(py, height, options->coordinate_mode) / (double)height);
-    for (int px = 0; px < width; ++px)
+    for (int px = 0; px < width && !points; ++px)
         xo[px] = fe_muld(view_width, pixel_axis_offset(px, width, options->coordinate_mode) / (double)width);
 #ifdef _OPENMP
     if (threads > 0) { omp_set_dynamic(0); omp_set_num_threads(threads); }
@@ -1884,7 +1867,7 @@ int fo_render(float *output, int width, int height, const char *zoom_text, const
 #else
         uint64_t *st = local;
 #endif
-        render_pixel(&c, output, (size_t)lp, sc_from_fe(xo[px], yo[py]), st);
+        render_pixel(&c, output, (size_t)lp, points ? points[lp] : sc_from_fe(xo[px], yo[py]), st);
     }
     if (stats) {
         memset(stats, 0, sizeof(uint64_t) * FO_STATS_FIELDS);
@@ -1904,6 +1887,83 @@ int fo_render(float *output, int width, int height, const char *zoom_text, const
     }
     free(local); free(xo); free(yo);
     return 0;
+}
+
+int fo_render(float *output, int width, int height, const char *zoom_text, const fo_reference *r,
+              int max_iter, int series_order, int series_block, const FractalRenderOptions *options,
+              FractalRenderPlanes *planes, uint64_t *stats, int threads, char *error, int cap) {
+    struct timespec started;
+    clock_gettime(CLOCK_MONOTONIC, &started);
+    if (!output || !r || !options || width <= 0 || height <= 0) {
+        fo_set_error(error, cap, "invalid render arguments");
+        return 1;
+    }
+    fe_t zoom;
+    if (!parse_zoom_fe(zoom_text, r->precision_bits, &zoom)) {
+        fo_set_error(error, cap, "invalid deep Mandelbrot zoom");
+        return 1;
+    }
+    if (r->formula != 0) {
+        if (planes) {
+            fo_set_error(error, cap, "render metadata requires the native scalar backend for this formula");
+            return 1;
+        }
+        if (render_alternate(output, width, height, zoom, r, max_iter, options, stats, threads, started)) {
+            fo_set_error(error, cap, "out of memory");
+            return 1;
+        }
+        return 0;
+    }
+    return render_mandel_field(output, width, height, zoom, r, max_iter, series_order, series_block, options, planes,
+                             stats, threads, NULL, NULL, started, error, cap);
+}
+
+/* fractal_render_points_impl, renderer.cpp:20019-20186, for Mandelbrot
+ * handles: each point is checked finite and normalised, the largest
+ * |dc|^2 gives the radius that replaces the viewport's in the BLA, series
+ * and tier decisions.  The scalar backend's strict-mode MPFR repair of a
+ * glitched point (renderer.cpp:12944-12966, at most 8192 points) is not
+ * restated: glitched points stay NaN, as on the reference's GPU point path
+ * (renderer.cpp:996-1010). */
+int fo_render_points(float *output, int point_count, const char *zoom_text, const double *real_mantissa,
+                     const double *imag_mantissa, const int32_t *exponents, const fo_reference *r, int max_iter,
+                     int series_order, int series_block, const FractalRenderOptions *options, uint64_t *stats,
+                     int threads, char *error, int cap) {
+    struct timespec started;
+    clock_gettime(CLOCK_MONOTONIC, &started);
+    if (!output || !r || !options || point_count <= 0 || !real_mantissa || !imag_mantissa || !exponents) {
+        fo_set_error(error, cap, "invalid native point-render arguments");
+        return 1;
+    }
+    if (r->formula != 0) {
+        fo_set_error(error, cap, "point renders of the oracle take Mandelbrot references");
+        return 1;
+    }
+    fe_t zoom;
+    if (!parse_zoom_fe(zoom_text, r->precision_bits, &zoom)) {
+        fo_set_error(error, cap, "invalid deep Mandelbrot zoom");
+        return 1;
+    }
+    sc_t *points = (sc_t *)malloc(sizeof(sc_t) * (size_t)point_count);
+    if (!points) { fo_set_error(error, cap, "out of memory"); return 1; }
+    sn_t largest = {0.0, 0};
+    for (int i = 0; i < point_count; ++i) {
+        sc_t point = {real_mantissa[i], imag_mantissa[i], exponents[i]};
+        if (!sc_finite(point)) {
+            free(points);
+            fo_set_error(error, cap, "non-finite point offset");
+            return 1;
+        }
+        sc_normalize(&point);
+        points[i] = point;
+        const sn_t radius = sc_norm_squared(point);
+        if (sc_compare_norm(radius, largest) > 0) largest = radius;
+    }
+    const fe_t point_radius = fe_sqrt((fe_t){largest.m, largest.e});
+    const int status = render_mandel_field(output, point_count, 1, zoom, r, max_iter, series_order, series_block,
+                                         options, NULL, stats, threads, points, &point_radius, started, error, cap);
+    free(points);
+    return status;
 }
 
 /* ---- shallow direct render (renderer.cpp:10009-10426) -------------------- */

@@ -73,6 +73,17 @@ int fo_render(
     FractalRenderPlanes *planes, uint64_t *stats, int threads,
     char *error, int error_capacity);
 
+/* Deep Mandelbrot render of caller-supplied dc offsets (mantissa pairs and
+ * exponents, one value per point; fractal_render_points, renderer.cpp:
+ * 20019-20186).  Glitched points stay NaN in strict mode (no MPFR repair).
+ * Returns 0 on success, 1 on failure. */
+int fo_render_points(
+    float *output, int point_count, const char *zoom_text,
+    const double *real_mantissa, const double *imag_mantissa,
+    const int32_t *exponents, const fo_reference *reference, int max_iter,
+    int series_order, int series_block, const FractalRenderOptions *options,
+    uint64_t *stats, int threads, char *error, int error_capacity);
+
 /* Shallow direct render in plain doubles (render_direct / render_direct_with_planes,
  * renderer.cpp:10009-10426): zoom and centre are decimal text parsed as
  * long double and rounded to double; formula 0-3 (Mandelbrot, Julia,

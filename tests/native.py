@@ -27,6 +27,7 @@ c_double_p = ctypes.POINTER(ctypes.c_double)
 c_int64_p = ctypes.POINTER(ctypes.c_int64)
 c_uint64_p = ctypes.POINTER(ctypes.c_uint64)
 c_uint8_p = ctypes.POINTER(ctypes.c_uint8)
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
 
 
 class RenderOptions(ctypes.Structure):
@@ -162,6 +163,15 @@ def _bind_render_abi(lib: ctypes.CDLL) -> ctypes.CDLL:
             c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(RenderOptions)]
         lib.render_mandelbrot_ex.restype = ctypes.c_int
+    for name in ("fractal_render_points", "fractal_render_points_ex_planes"):
+        if hasattr(lib, name):  # renderer.h:287-321
+            fn = getattr(lib, name)
+            fn.argtypes = [c_float_p, ctypes.c_int, ctypes.c_char_p, c_double_p, c_double_p,
+                           c_int32_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_int, ctypes.POINTER(RenderOptions)]
+            if name.endswith("planes"):
+                fn.argtypes = fn.argtypes + [ctypes.POINTER(RenderPlanes)]
+            fn.restype = ctypes.c_int
     if hasattr(lib, "render_mandelbrot"):
         lib.render_mandelbrot.argtypes = [
             c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
@@ -240,6 +250,11 @@ def load_oracle():
         ctypes.c_int, ctypes.c_int, ctypes.POINTER(RenderOptions), ctypes.POINTER(RenderPlanes),
         c_uint64_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
     lib.fo_render.restype = ctypes.c_int
+    lib.fo_render_points.argtypes = [
+        c_float_p, ctypes.c_int, ctypes.c_char_p, c_double_p, c_double_p, c_int32_p, ctypes.c_void_p,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(RenderOptions), c_uint64_p, ctypes.c_int,
+        ctypes.c_char_p, ctypes.c_int]
+    lib.fo_render_points.restype = ctypes.c_int
     return lib
 
 

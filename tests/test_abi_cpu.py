@@ -281,3 +281,74 @@ def test_batch_creation_validates_without_device(product):
     assert not lib.fractal_hip_batch_create(entries, 1, ctypes.byref(fav_hip.NativeRenderOptions()))
     assert not lib.fractal_hip_batch_create(entries, 0, ctypes.byref(fav_hip.NativeRenderOptions()))
     assert lib.fractal_hip_batch_tile_count(ctypes.c_void_p(0x7777)) == -1
+
+
+def _points_call(product, handle, count=4, options=None, planes=None, zoom=b"1e12", max_iter=200, real=None,
+                 exponents=None):
+    re = np.full(count, 0.25) if real is None else real
+    im = np.zeros(count)
+    e = np.full(count, -42, dtype=np.int32) if exponents is None else exponents
+    out = np.full(max(count, 1), 7.0, dtype=np.float32)
+    opts = options if options is not None else native.RenderOptions.defaults(backend=2)
+    args = (out.ctypes.data_as(native.c_float_p), count, zoom, re.ctypes.data_as(native.c_double_p),
+            im.ctypes.data_as(native.c_double_p), e.ctypes.data_as(native.c_int32_p), ctypes.c_void_p(handle),
+            max_iter, 0, 8, 32, ctypes.byref(opts))
+    if planes is not None:
+        return product.fractal_render_points_ex_planes(*args, planes), out
+    return product.fractal_render_points(*args), out
+
+
+@pytest.mark.parametrize("change, message", [
+    (dict(count=0), b"invalid native point-render arguments"),
+    (dict(count=100_000_001), b"invalid native point-render arguments"),
+    (dict(max_iter=0), b"invalid native point-render limits"),
+    (dict(max_iter=201), b"point render iteration count exceeds prepared reference"),
+    (dict(options=dict(backend=0)), b"HIP backend"),
+    (dict(options=dict(backend=5)), b"unknown native render backend"),
+    (dict(options=dict(coordinate_mode=1)), b"requires project coordinates"),
+    (dict(zoom=b"zoom"), b"zoom"),
+    (dict(real=np.array([0.25, np.inf, 0.0, 0.0])), b"non-finite point offset"),
+])
+def test_point_render_validation(product, change, message):
+    """fractal_render_points_impl's checks and messages (renderer.cpp:20034-20080),
+    then the HIP point path's own limits; no device work is started."""
+    handle = product.fractal_create_reference(X, Y, b"1e12", 200, 128, 8)
+    kwargs = dict(change)
+    if "options" in kwargs:
+        kwargs["options"] = native.RenderOptions.defaults(**{"backend": 2, **kwargs["options"]})
+    count = kwargs.pop("count", 4)
+    if count > 1000:  # the count check comes before any pointer is read
+        status = product.fractal_render_points(None, count, b"1e12", None, None, None, ctypes.c_void_p(handle),
+                                               200, 0, 8, 32, None)
+        out = np.full(4, 7.0, dtype=np.float32)
+    else:
+        status, out = _points_call(product, handle, count=count, **kwargs)
+    assert status == 1
+    assert message in product.fractal_last_error()
+    assert np.all(out == 7.0)
+    product.fractal_destroy_reference(ctypes.c_void_p(handle))
+
+
+def test_point_render_rejects_planes_and_other_formulas(product):
+    handle = product.fractal_create_reference(X, Y, b"1e12", 200, 128, 8)
+    planes = native.PlaneArrays(4, 1)
+    status, _ = _points_call(product, handle, planes=ctypes.byref(planes.struct))
+    assert status == 1 and b"point metadata requires the native scalar backend" in product.fractal_last_error()
+    status, _ = _points_call(product, 0x7777)
+    assert status == 1 and b"invalid or already-destroyed" in product.fractal_last_error()
+    product.fractal_destroy_reference(ctypes.c_void_p(handle))
+    julia = product.fractal_create_reference_ex(X, Y, b"1e12", 200, 128, 8, 1, b"-0.8", b"0.156")
+    assert julia, product.fractal_last_error()
+    status, _ = _points_call(product, julia)
+    assert status == 1 and b"Mandelbrot handles" in product.fractal_last_error()
+    product.fractal_destroy_reference(ctypes.c_void_p(julia))
+
+
+def test_point_render_without_device_fails_loudly(product):
+    if product.fractal_backend_capabilities() & 8:
+        pytest.skip("a device is present")
+    handle = product.fractal_create_reference(X, Y, b"1e12", 200, 128, 8)
+    status, out = _points_call(product, handle)
+    assert status == 1 and product.fractal_last_error()
+    assert np.all(out == 7.0)
+    product.fractal_destroy_reference(ctypes.c_void_p(handle))
