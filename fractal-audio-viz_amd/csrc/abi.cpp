@@ -283,8 +283,8 @@ int render_impl(float* output, int width, int height, const char* zoom, void* ha
 }
 
 // fractal_render_points_impl, renderer.cpp:20019-20186: the same checks and
-// the deep field loop over caller-supplied dc offsets instead of a pixel
-// grid (Mandelbrot handles, project coordinates, no planes on this
+// the deep field loop (Mandelbrot or alternate formula) over caller-supplied
+// dc offsets instead of a pixel grid (project coordinates, no planes on this
 // backend).
 int render_points_impl(float* output, int point_count, const char* zoom, const double* real_mantissa,
                        const double* imag_mantissa, const int32_t* exponents, void* handle, int max_iter,
@@ -319,8 +319,6 @@ int render_points_impl(float* output, int point_count, const char* zoom, const d
         }
         if (o.coordinate_mode != 0)  // as the reference's GPU point path (renderer.cpp:20093-20097)
             throw std::runtime_error("point rendering on the HIP backend requires project coordinates");
-        if (context->formula != 0)
-            throw std::runtime_error("point rendering on the HIP backend takes Mandelbrot handles");
         // the points as rows of up to 256, so wavefronts are filled
         const int row = point_count < 256 ? point_count : 256;
         uint64_t stats[fav::kStatsFields] = {};
@@ -330,7 +328,10 @@ int render_points_impl(float* output, int point_count, const char* zoom, const d
         request.points = points.data();
         request.point_count = point_count;
         request.point_radius = fav::fe_sqrt(fav::FE{largest.m, largest.e});
-        fav::render_field(*context, request);
+        if (context->formula != 0)  // render_alternate_reference_impl, renderer.cpp:20105-20136
+            fav::render_alternate(*context, request);
+        else
+            fav::render_field(*context, request);
         if (collect) publish_stats(stats);
         set_error("");
         return 0;

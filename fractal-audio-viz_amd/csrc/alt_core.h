@@ -130,7 +130,11 @@ struct AltBudget {  // render_time_budget_expired, renderer.cpp:11878-11890
     }
 };
 
-FAV_HD SC pixel_dc(const AltRenderParams& p, int px, int py) {
+FAV_HD SC pixel_dc(const AltRenderParams& p, int px, int py) {  // renderer.cpp:12148-12152
+    if (p.point_re) {  // point render: the caller's normalised offsets
+        const size_t i = static_cast<size_t>(py) * static_cast<size_t>(p.width) + static_cast<size_t>(px);
+        return SC{p.point_re[i], p.point_im[i], p.point_e[i]};
+    }
     return sc_from_fe(FE{p.x_off_m[px], p.x_off_e[px]}, FE{p.y_off_m[py], p.y_off_e[py]});
 }
 

@@ -26,7 +26,11 @@ __global__ void __launch_bounds__(256) alt_kernel(AltRenderParams p) {
     }
     const int px = bx * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = by * 16 + (wave >> 1) * 8 + (lane >> 3);
-    if (px < p.width && py < p.height) alt::pixel(p, px, py);
+    if (px >= p.width || py >= p.height) return;
+    if (p.point_re && static_cast<size_t>(py) * static_cast<size_t>(p.width) + static_cast<size_t>(px)
+                          >= static_cast<size_t>(p.point_count))
+        return;  // the unused end of a point render's last row
+    alt::pixel(p, px, py);
 }
 
 __global__ void alt_prologue(unsigned long long* budget_start, int* budget_flag) {

@@ -206,6 +206,32 @@ void ensure_device_tables(ReferenceContext& c) {
 }
 
 // One deep field render on the device, host part of renderer.cpp:12672-13583.
+// A point render's offsets in HBM: re then im (doubles), then exponents.
+// The host copies stay alive in `staging` until the caller synchronises.
+struct PointStaging {
+    std::vector<double> m;
+    std::vector<int32_t> e;
+};
+
+void upload_points(LaunchState& s, const FieldRequest& r, PointStaging& staging, const double*& re,
+                   const double*& im, const int32_t*& e) {
+    const size_t n = static_cast<size_t>(r.point_count);
+    staging.m.resize(2 * n);
+    staging.e.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        staging.m[i] = r.points[i].re;
+        staging.m[n + i] = r.points[i].im;
+        staging.e[i] = r.points[i].e;
+    }
+    const size_t e_at = (2 * n * 8 + 255) & ~size_t(255);
+    char* dp = static_cast<char*>(s.points.get(e_at + n * 4 + 256));
+    check(hipMemcpyAsync(dp, staging.m.data(), 2 * n * 8, hipMemcpyHostToDevice, s.stream), "H2D points");
+    check(hipMemcpyAsync(dp + e_at, staging.e.data(), n * 4, hipMemcpyHostToDevice, s.stream), "H2D points");
+    re = reinterpret_cast<const double*>(dp);
+    im = re + n;
+    e = reinterpret_cast<const int32_t*>(dp + e_at);
+}
+
 void render_field(ReferenceContext& c, const FieldRequest& r) {
     const int w = r.width, h = r.height;
     PreparedRender prep;
@@ -241,27 +267,12 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     p.y_off_m = p.x_off_m + w;
     p.x_off_e = reinterpret_cast<const int32_t*>(doff + exp_offset);
     p.y_off_e = p.x_off_e + w;
-    std::vector<double> point_m;
-    std::vector<int32_t> point_e;
+    PointStaging points;
     size_t out_count = pixels;
-    if (r.points) {  // per-point offsets: re then im (doubles), then exponents
-        const size_t n = static_cast<size_t>(r.point_count);
-        point_m.resize(2 * n);
-        point_e.resize(n);
-        for (size_t i = 0; i < n; ++i) {
-            point_m[i] = r.points[i].re;
-            point_m[n + i] = r.points[i].im;
-            point_e[i] = r.points[i].e;
-        }
-        const size_t e_at = (2 * n * 8 + 255) & ~size_t(255);
-        char* dp = static_cast<char*>(s.points.get(e_at + n * 4 + 256));
-        check(hipMemcpyAsync(dp, point_m.data(), 2 * n * 8, hipMemcpyHostToDevice, s.stream), "points");
-        check(hipMemcpyAsync(dp + e_at, point_e.data(), n * 4, hipMemcpyHostToDevice, s.stream), "points");
-        p.point_re = reinterpret_cast<const double*>(dp);
-        p.point_im = p.point_re + n;
-        p.point_e = reinterpret_cast<const int32_t*>(dp + e_at);
+    if (r.points) {
+        upload_points(s, r, points, p.point_re, p.point_im, p.point_e);
         p.point_count = r.point_count;
-        out_count = n;
+        out_count = static_cast<size_t>(r.point_count);
     }
 
     const bool want_planes = r.planes != nullptr;
@@ -594,12 +605,20 @@ void render_alternate(ReferenceContext& c, const FieldRequest& r, const uint32_t
         p.tile_count = static_cast<int32_t>(tile_count);
     }
     if (work) p.work = static_cast<int32_t*>(s.work.get(pixels * 4));
+    PointStaging points;
+    size_t out_count = pixels;
+    if (r.points) {
+        upload_points(s, r, points, p.point_re, p.point_im, p.point_e);
+        p.point_count = r.point_count;
+        out_count = static_cast<size_t>(r.point_count);
+    }
     const uint64_t wall0 = now_ns();
     check(hipEventRecord(s.start, s.stream), "event");
     check(launch_alt_kernel(p, s.stream), "alternate kernel launch");
     check(hipEventRecord(s.stop, s.stream), "event");
     if (!r.device_pointers)
-        check(hipMemcpyAsync(r.output, p.output, pixels * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H");
+        check(hipMemcpyAsync(r.output, p.output, out_count * sizeof(float), hipMemcpyDeviceToHost, s.stream),
+              "D2H");
     if (work) check(hipMemcpyAsync(work, p.work, pixels * 4, hipMemcpyDeviceToHost, s.stream), "D2H work");
     check(hipStreamSynchronize(s.stream), "alternate render");
     const float kernel_ms = elapsed(s.start, s.stop);
@@ -608,7 +627,7 @@ void render_alternate(ReferenceContext& c, const FieldRequest& r, const uint32_t
     s.timing.pixels = static_cast<double>(pixels);
     if (r.stats) {  // the alternate renderer reports pixels and time only (renderer.cpp:12439-12446)
         std::fill(r.stats, r.stats + kStatsFields, 0ULL);
-        r.stats[0] = static_cast<uint64_t>(pixels);
+        r.stats[0] = static_cast<uint64_t>(out_count);
         r.stats[22] = static_cast<uint64_t>(static_cast<double>(kernel_ms) * 1e6);
     }
 }

@@ -166,6 +166,12 @@ struct AltRenderParams {
     // preview budget: when > 0 a pixel stops after this many loop passes
     // (output NaN, work = the budget)
     int32_t work_limit;
+    // point renders: per-point dc (null for a pixel grid); valid pixel
+    // indices are below point_count when point_re is set
+    const double* point_re;
+    const double* point_im;
+    const int32_t* point_e;
+    int32_t point_count;
 };
 
 // Everything one field launch needs, passed by value as the kernel argument.

@@ -109,8 +109,10 @@ void prepare_alternate(const ReferenceContext& c, const FieldRequest& r, Prepare
     p.series.iteration = c.series_iteration;
     p.series.radius2_m = c.series_radius_squared.m;
     p.series.radius2_e = c.series_radius_squared.e;
-    // renderer.cpp:12121-12128 (no point radius on a field render)
-    p.series_available = c.formula == 1 && c.series_enabled && c.series_iteration < r.max_iter;
+    // renderer.cpp:12121-12128: a point render also needs its farthest point
+    // inside the series radius
+    p.series_available = c.formula == 1 && c.series_enabled && c.series_iteration < r.max_iter
+        && (!r.points || fe_compare(r.point_radius, fe_sqrt(c.series_radius_squared)) <= 0);
     p.width = w;
     p.height = h;
     p.max_iter = r.max_iter;

@@ -315,9 +315,10 @@ int fractal_atlas_colourise_kfp_raw_precise(
 /* Deep render of caller-supplied dc offsets (mantissas and exponents, one
  * value per point) against a reference handle: the field loop of
  * fractal_render_reference_ex with the points in place of a pixel grid and
- * the farthest point's radius in place of the viewport's.  On this backend:
- * Mandelbrot handles, project coordinates, no planes (as the reference's
- * GPU point path).  renderer.h:287-300 / renderer.cpp:20019-20216. */
+ * the farthest point's radius in place of the viewport's, for Mandelbrot
+ * and alternate-formula handles.  On this backend: project coordinates, no
+ * planes (as the reference's GPU point path).
+ * renderer.h:287-300 / renderer.cpp:20019-20216. */
 int fractal_render_points(
     float *output, int point_count, const char *zoom_text,
     const double *real_mantissa, const double *imag_mantissa,

@@ -1783,7 +1783,8 @@ static void render_pixel(const render_ctx *c, float *output, size_t index, sc_t 
 }
 
 static int render_alternate(float *output, int width, int height, fe_t zoom, const fo_reference *r, int max_iter,
-                            const FractalRenderOptions *o, uint64_t *stats, int threads, struct timespec started);
+                            const FractalRenderOptions *o, uint64_t *stats, int threads, struct timespec started,
+                            const sc_t *points, const fe_t *point_radius);
 
 /* render_bla_impl's field loop (renderer.cpp:12672-13583) over a grid, or
  * over caller-supplied normalised dc offsets (width = count, height = 1)
@@ -1908,7 +1909,8 @@ int fo_render(float *output, int width, int height, const char *zoom_text, const
             fo_set_error(error, cap, "render metadata requires the native scalar backend for this formula");
             return 1;
         }
-        if (render_alternate(output, width, height, zoom, r, max_iter, options, stats, threads, started)) {
+        if (render_alternate(output, width, height, zoom, r, max_iter, options, stats, threads, started, NULL,
+                             NULL)) {
             fo_set_error(error, cap, "out of memory");
             return 1;
         }
@@ -1918,10 +1920,10 @@ int fo_render(float *output, int width, int height, const char *zoom_text, const
                              stats, threads, NULL, NULL, started, error, cap);
 }
 
-/* fractal_render_points_impl, renderer.cpp:20019-20186, for Mandelbrot
- * handles: each point is checked finite and normalised, the largest
- * |dc|^2 gives the radius that replaces the viewport's in the BLA, series
- * and tier decisions.  The scalar backend's strict-mode MPFR repair of a
+/* fractal_render_points_impl, renderer.cpp:20019-20186, for every formula:
+ * each point is checked finite and normalised, the largest |dc|^2 gives the
+ * radius that replaces the viewport's in the BLA, series and tier
+ * decisions.  The scalar backend's strict-mode MPFR repair of a
  * glitched point (renderer.cpp:12944-12966, at most 8192 points) is not
  * restated: glitched points stay NaN, as on the reference's GPU point path
  * (renderer.cpp:996-1010). */
@@ -1933,10 +1935,6 @@ int fo_render_points(float *output, int point_count, const char *zoom_text, cons
     clock_gettime(CLOCK_MONOTONIC, &started);
     if (!output || !r || !options || point_count <= 0 || !real_mantissa || !imag_mantissa || !exponents) {
         fo_set_error(error, cap, "invalid native point-render arguments");
-        return 1;
-    }
-    if (r->formula != 0) {
-        fo_set_error(error, cap, "point renders of the oracle take Mandelbrot references");
         return 1;
     }
     fe_t zoom;
@@ -1960,6 +1958,13 @@ int fo_render_points(float *output, int point_count, const char *zoom_text, cons
         if (sc_compare_norm(radius, largest) > 0) largest = radius;
     }
     const fe_t point_radius = fe_sqrt((fe_t){largest.m, largest.e});
+    if (r->formula != 0) { /* renderer.cpp:20105-20136 */
+        const int failed = render_alternate(output, point_count, 1, zoom, r, max_iter, options, stats, threads,
+                                            started, points, &point_radius);
+        free(points);
+        if (failed) fo_set_error(error, cap, "out of memory");
+        return failed;
+    }
     const int status = render_mandel_field(output, point_count, 1, zoom, r, max_iter, series_order, series_block,
                                          options, NULL, stats, threads, points, &point_radius, started, error, cap);
     free(points);
@@ -2343,6 +2348,7 @@ typedef struct {
     int width, max_iter, mode, strict, cycle_enabled, series_available, bla_enabled, max_block;
     double bias;
     const fe_t *xo, *yo;
+    const sc_t *points; /* point render: one normalised dc per pixel index, or NULL */
     int budget_enabled;
     struct timespec deadline;
     atomic_int *budget_exceeded;
@@ -2353,7 +2359,8 @@ static void alt_pixel(const alt_ctx *c, int px, int py) {
     const size_t index = (size_t)py * (size_t)c->width + (size_t)px;
     const int julia = r->formula == 1;
     const int mode = c->mode;
-    const sc_t dc = sc_from_fe(c->xo[px], c->yo[py]);
+    const sc_t dc = c->points ? c->points[(size_t)py * (size_t)c->width + (size_t)px]
+                              : sc_from_fe(c->xo[px], c->yo[py]);
     const sc_t pdelta = julia ? SC_ZERO : dc;
     const sn_t pnorm = sc_norm_squared(pdelta);
     const fe_t pn = fe_make(pnorm.m, pnorm.e);
@@ -2486,9 +2493,12 @@ static void alt_pixel(const alt_ctx *c, int px, int py) {
     c->output[index] = encode_iteration(c->max_iter, c->bias);
 }
 
-/* render_alternate_reference_impl without planes, renderer.cpp:12064-12448. */
+/* render_alternate_reference_impl without planes, renderer.cpp:12064-12448,
+ * over a grid or over caller-supplied dc offsets (points, width = count,
+ * height = 1). */
 static int render_alternate(float *output, int width, int height, fe_t zoom, const fo_reference *r, int max_iter,
-                            const FractalRenderOptions *o, uint64_t *stats, int threads, struct timespec started) {
+                            const FractalRenderOptions *o, uint64_t *stats, int threads, struct timespec started,
+                            const sc_t *points, const fe_t *point_radius) {
     const double factor = viewport_height_factor(o->coordinate_mode);
     const fe_t view_height = fe_muld(fe_div(fe_from_parts(1.0, 0), zoom), factor);
     const fe_t view_width = fe_muld(view_height, (double)width / (double)height);
@@ -2502,9 +2512,11 @@ static int render_alternate(float *output, int width, int height, fe_t zoom, con
     alt_ctx c;
     memset(&c, 0, sizeof(c));
     c.r = r; c.output = output; c.width = width; c.max_iter = max_iter; c.mode = o->escape_radius_mode;
-    c.strict = o->strict != 0; c.bias = o->output_bias; c.xo = xo; c.yo = yo;
+    c.strict = o->strict != 0; c.bias = o->output_bias; c.xo = xo; c.yo = yo; c.points = points;
     c.cycle_enabled = o->disable_cycle == 0 && (o->strict == 0 || o->strict_cycle != 0);
-    c.series_available = r->formula == 1 && r->series_enabled && r->series_iteration < max_iter;
+    /* renderer.cpp:12121-12128 */
+    c.series_available = r->formula == 1 && r->series_enabled && r->series_iteration < max_iter
+        && (!points || fe_compare(*point_radius, fe_sqrt(r->series_radius_squared)) <= 0);
     c.bla_enabled = o->disable_bla == 0 && r->alt_levels > 0;
     int mb = o->max_linear_bla_length;
     c.max_block = mb < 2 ? 2 : (mb > 4096 ? 4096 : mb);

@@ -73,7 +73,7 @@ int fo_render(
     FractalRenderPlanes *planes, uint64_t *stats, int threads,
     char *error, int error_capacity);
 
-/* Deep Mandelbrot render of caller-supplied dc offsets (mantissa pairs and
+/* Deep render (Mandelbrot or alternate formula) of caller-supplied dc offsets (mantissa pairs and
  * exponents, one value per point; fractal_render_points, renderer.cpp:
  * 20019-20186).  Glitched points stay NaN in strict mode (no MPFR repair).
  * Returns 0 on success, 1 on failure. */

@@ -20,15 +20,17 @@ import math
 
 import numpy as np
 
+from alt_cases import BS_X, BS_Y, JULIA_C, JULIA_X, JULIA_Y, TC_X, TC_Y
 from cases import BASE_X, BASE_Y, X100, Y100
 
 
 def _case(name, x, y, ref_zoom, zoom, max_iter, bits, count, spread, seed, series_order=8,
-          series_block=32, image_series_order=8, specials=True, fixed=None, **options):
+          series_block=32, image_series_order=8, specials=True, fixed=None, formula=0, julia=("0", "0"),
+          far=None, **options):
     return dict(name=name, x=x, y=y, ref_zoom=ref_zoom, zoom=zoom, max_iter=max_iter, bits=bits,
                 count=count, spread=spread, seed=seed, series_order=series_order,
                 series_block=series_block, image_series_order=image_series_order,
-                specials=specials, fixed=fixed, options=options)
+                specials=specials, fixed=fixed, formula=formula, julia=list(julia), far=far, options=options)
 
 
 def half_height(zoom_text: str) -> float:
@@ -60,6 +62,10 @@ def make_points(case):
         re[4], im[4], e[4] = -0.75, 0.0, base - 1       # real axis
         re[5], im[5], e[5] = 3.0, -5.0, base - 4        # mantissas well above one
         re[6], im[6], e[6] = 0.015625, 0.03125, base    # mantissas well below one half
+    if case.get("far"):  # one more point at `far` half-heights: it alone sets the radius
+        far_base = math.frexp(half_height(case["zoom"]) * case["far"])[1]
+        re, im = np.append(re, 0.6), np.append(im, -0.6)
+        e = np.append(e, np.int32(far_base))
     return re, im, e
 
 
@@ -93,4 +99,18 @@ CASES = [
     # exactly (8192): every glitched point is NaN on both paths
     _case("points_1e100_glitch_many", X100, Y100, "1e100", "1e100", 100000, 360, 8200, 1.0, 17,
           series_order=3, series_block=256, series_min_terms=32, disable_bla=1),
+    # alternate formulas (render_alternate_reference_impl with points,
+    # renderer.cpp:12064-12160): burning ship and tricorn BLA, the tricorn
+    # cycle check, Julia with the image series on (points well inside its
+    # radius) and off (the farthest point outside it)
+    _case("points_burning_ship_1e7", BS_X, BS_Y, "1e7", "1e7", 4000, 200, 300, 1.0, 31, formula=2),
+    _case("points_tricorn_cycle", TC_X, TC_Y, "2e7", "2e7", 5000, 200, 300, 1.0, 32, formula=3, strict=0),
+    _case("points_julia_series", JULIA_X, JULIA_Y, "1e8", "1e8", 4000, 200, 300, 0.02, 33, formula=1,
+          julia=JULIA_C),
+    _case("points_julia_wide", JULIA_X, JULIA_Y, "1e8", "1e8", 4000, 200, 300, 1.0, 34, formula=1,
+          julia=JULIA_C),
+    # the same points and one far point: its radius turns the series off
+    # (renderer.cpp:12121-12128), which changes some of the other values
+    _case("points_julia_far", JULIA_X, JULIA_Y, "1e8", "1e8", 4000, 200, 300, 1.0, 34, formula=1,
+          julia=JULIA_C, far=4.0),
 ]

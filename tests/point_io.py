@@ -31,7 +31,8 @@ def render_oracle(lib, case, points, stats=True):
     opts = native.RenderOptions.defaults(**case["options"])
     ref = native.OracleReference(lib, case["x"], case["y"], case["ref_zoom"], case["max_iter"], case["bits"],
                                  case["image_series_order"], case["options"].get("escape_radius_mode", 0),
-                                 case["options"].get("coordinate_mode", 0))
+                                 case["options"].get("coordinate_mode", 0), formula=case.get("formula", 0),
+                                 julia=tuple(case.get("julia", ("0", "0"))))
     out = np.empty(len(re), dtype=np.float32)
     counters = np.zeros(39, dtype=np.uint64)
     err = ctypes.create_string_buffer(512)
@@ -53,7 +54,8 @@ def render_abi(lib, case, points, backend=2, stats=True, handle=None):
     ref = None
     if handle is None:
         ref = native.AbiReference(lib, case["x"], case["y"], case["ref_zoom"], case["max_iter"], case["bits"],
-                                  case["image_series_order"], opts)
+                                  case["image_series_order"], opts, formula=case.get("formula", 0),
+                                  julia=tuple(case.get("julia", ("0", "0"))))
         handle = ref.handle
     out = np.empty(len(re), dtype=np.float32)
     if stats:

@@ -329,7 +329,7 @@ def test_point_render_validation(product, change, message):
     product.fractal_destroy_reference(ctypes.c_void_p(handle))
 
 
-def test_point_render_rejects_planes_and_other_formulas(product):
+def test_point_render_rejects_planes_and_stale_handles(product):
     handle = product.fractal_create_reference(X, Y, b"1e12", 200, 128, 8)
     planes = native.PlaneArrays(4, 1)
     status, _ = _points_call(product, handle, planes=ctypes.byref(planes.struct))
@@ -337,17 +337,14 @@ def test_point_render_rejects_planes_and_other_formulas(product):
     status, _ = _points_call(product, 0x7777)
     assert status == 1 and b"invalid or already-destroyed" in product.fractal_last_error()
     product.fractal_destroy_reference(ctypes.c_void_p(handle))
-    julia = product.fractal_create_reference_ex(X, Y, b"1e12", 200, 128, 8, 1, b"-0.8", b"0.156")
-    assert julia, product.fractal_last_error()
-    status, _ = _points_call(product, julia)
-    assert status == 1 and b"Mandelbrot handles" in product.fractal_last_error()
-    product.fractal_destroy_reference(ctypes.c_void_p(julia))
 
 
-def test_point_render_without_device_fails_loudly(product):
+@pytest.mark.parametrize("formula", [0, 1, 2, 3])
+def test_point_render_without_device_fails_loudly(product, formula):
     if product.fractal_backend_capabilities() & 8:
         pytest.skip("a device is present")
-    handle = product.fractal_create_reference(X, Y, b"1e12", 200, 128, 8)
+    handle = product.fractal_create_reference_ex(X, Y, b"1e12", 200, 128, 8, formula, b"-0.8", b"0.156")
+    assert handle, product.fractal_last_error()
     status, out = _points_call(product, handle)
     assert status == 1 and product.fractal_last_error()
     assert np.all(out == 7.0)

@@ -62,3 +62,14 @@ def test_hip_single_point(gpu_product, oracle):
     expected, expected_stats = point_io.render_oracle(oracle, case, one)
     field, stats = point_io.render_abi(gpu_product, case, one)
     assert_points_parity(field, stats, expected, expected_stats)
+
+
+@pytest.mark.parametrize("name", ["points_burning_ship_1e7", "points_julia_wide"])
+def test_hip_alternate_points_many_rows_match_oracle(gpu_product, oracle, name):
+    """Alternate-formula point renders of 3000 points (11 full rows and a
+    partial one) against the oracle."""
+    case = dict(point_io.case_by_name(name), count=3000, seed=41)
+    points = make_points(case)
+    expected, expected_stats = point_io.render_oracle(oracle, case, points)
+    field, stats = point_io.render_abi(gpu_product, case, points)
+    assert_points_parity(field, stats, expected, expected_stats)

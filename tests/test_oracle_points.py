@@ -70,3 +70,13 @@ def test_dropped_radius_is_caught(oracle):
     assert 0 < near.sum() < near.size
     _, stats = point_io.render_oracle(oracle, case, (re[near], im[near], e[near]))
     assert stats[16] == near.sum() and golden["stats"][16] == 0  # series pixels
+
+
+def test_far_point_turns_the_julia_series_off():
+    """The Julia series is taken only when the farthest point lies inside its
+    radius (renderer.cpp:12121-12128): one far point changes values of the
+    other points, so the fixtures pin that check."""
+    wide = point_io.load("points_julia_wide")["field"]
+    far = point_io.load("points_julia_far")["field"]
+    assert far.size == wide.size + 1
+    assert (wide.view(np.uint32) != far[:-1].view(np.uint32)).sum() > 0
