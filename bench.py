@@ -67,9 +67,12 @@ def cpu_baseline(work, threads):
     kind = "reference"
     sample_w, sample_h = work["width"] // 2, work["height"] // 2
     opts = native.RenderOptions.defaults(backend=0, **work["options"])
+    build_s = None
     if lib is not None:
+        t0 = time.perf_counter()
         ref = native.AbiReference(lib, work["x"], work["y"], work["zoom"], work["max_iter"],
                                   work["bits"], work["image_series_order"], opts)
+        build_s = time.perf_counter() - t0
     else:
         kind = "port"
         lib = native.load_oracle()
@@ -85,7 +88,7 @@ def cpu_baseline(work, threads):
     total = escape_count_sum(planes.orbit_iteration, field.reshape(-1), work["max_iter"])
     return {"value": total / seconds / 1e9, "unit": "Gpixel-iter/s", "cores": threads, "kind": kind,
             "sample": f"{sample_w}x{sample_h} of the same field (1/4 of the pixels), "
-                      f"{seconds:.1f} s on {threads} threads ({platform.processor() or platform.machine()})"}
+                      f"{seconds:.1f} s on {threads} threads ({platform.processor() or platform.machine()})"}, build_s
 
 
 def copy_calibration(mib, copies):
@@ -411,7 +414,13 @@ def main():
     hip.free(d_rgb)
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    baseline = None if args.no_cpu_baseline else cpu_baseline(work, threads)
+    baseline, reference_build_s = (None, None) if args.no_cpu_baseline else cpu_baseline(work, threads)
+    # host setup of the same reference (MPFR orbit, image series, BLA tables):
+    # this library against the reference's renderer.cpp on the same host
+    host_setup = {"reference_build_s": round(build_s, 3),
+                  "reference_build_s_renderer_cpp": None if reference_build_s is None else round(reference_build_s, 3),
+                  "speedup": None if not reference_build_s else round(reference_build_s / build_s, 2),
+                  "threads": threads}
     result = {
         "metric": "Gpixel-iterations/s at 1e100 zoom (FP64 vs floatexp/batched), % pixels ≠ FP64",
         "value": round(value, 3),
@@ -445,6 +454,7 @@ def main():
         "batched": batched,
         "learned_schedule": learned,
         "atlas_colour": atlas,
+        "host_setup": host_setup,
     }
     print(json.dumps(result))
     return 0

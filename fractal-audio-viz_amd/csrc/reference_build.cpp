@@ -392,38 +392,89 @@ void build_image_series(ReferenceContext& c, const std::vector<FEC>& orbit) {
     const FE minimum_scale_squared = fe_sqr(probe_radius);
     const FE two = fe_from_parts(2.0, 0);
     const FEC one{fe_from_parts(1.0, 0), z0};
-    for (int iteration = 0; iteration < last_candidate; ++iteration) {
-        const FEC reference = orbit[static_cast<size_t>(iteration)];
-        std::fill(next.begin(), next.end(), FEC{z0, z0});
-        for (int term = 1; term <= order; ++term) {
-            FEC value = fec_mulf(fec_mul(reference, coefficients[static_cast<size_t>(term)]), two);
-            for (int left = 1; left < term; ++left)
-                value = fec_add(value, fec_mul(coefficients[static_cast<size_t>(left)],
-                                               coefficients[static_cast<size_t>(term - left)]));
-            if (!julia && term == 1) value = fec_add(value, one);
-            next[static_cast<size_t>(term)] = value;
-        }
-        coefficients.swap(next);
-        bool safe = true;
-        for (int probe = 0; probe < 12; ++probe) {
-            const FEC dc = probes[probe];
-            const FEC exact_next = fec_add(fec_mulf(fec_mul(reference, exact[probe]), two),
-                                           fec_add(fec_mul(exact[probe], exact[probe]),
-                                                   julia ? FEC{z0, z0} : dc));
-            exact[probe] = exact_next;
-            if (iteration + 1 >= order
-                && !series_probe_is_safe(orbit[static_cast<size_t>(iteration) + 1], exact_next,
-                                         series_evaluate(coefficients, dc), minimum_scale_squared,
-                                         tolerance_squared, c.escape_radius_mode)) {
-                safe = false;
-                break;
+    // The reference's loop (renderer.cpp:11280-11322) advances the series
+    // and then each probe's exact orbit by one iteration, and stops at the
+    // first iteration where some probe is unsafe.  The series recursion
+    // does not depend on the probes and each probe's orbit depends only on
+    // itself, so the iterations are taken in chunks: while the twelve probes
+    // check chunk i on their own threads, this thread computes the series of
+    // chunk i + 1.  The stop is the earliest unsafe iteration of any probe,
+    // and the kept series is the one after the iteration before it -- the
+    // same iteration and coefficients as the sequential loop.  Chunks grow
+    // from 64 to 2048 iterations, so a series that stops early computes
+    // little beyond its stop.
+    constexpr int kChunk = 2048;
+    int chunk_size = 64;
+    std::vector<FEC> chunk[2] = {std::vector<FEC>(static_cast<size_t>(kChunk) * n),
+                                 std::vector<FEC>(static_cast<size_t>(kChunk) * n)};
+    auto advance_series = [&](int first, int end, std::vector<FEC>& out) {
+        for (int iteration = first; iteration < end; ++iteration) {
+            const FEC reference = orbit[static_cast<size_t>(iteration)];
+            std::fill(next.begin(), next.end(), FEC{z0, z0});
+            for (int term = 1; term <= order; ++term) {
+                FEC value = fec_mulf(fec_mul(reference, coefficients[static_cast<size_t>(term)]), two);
+                for (int left = 1; left < term; ++left)
+                    value = fec_add(value, fec_mul(coefficients[static_cast<size_t>(left)],
+                                                   coefficients[static_cast<size_t>(term - left)]));
+                if (!julia && term == 1) value = fec_add(value, one);
+                next[static_cast<size_t>(term)] = value;
             }
+            coefficients.swap(next);
+            std::copy(coefficients.begin(), coefficients.end(),
+                      out.begin() + static_cast<ptrdiff_t>(static_cast<size_t>(iteration - first) * n));
         }
-        if (!safe) break;
-        if (iteration + 1 >= order) {
-            best_iteration = iteration + 1;
-            best = coefficients;
+    };
+    int first = 0;
+    int end = std::min(last_candidate, chunk_size);
+    if (first < end) advance_series(first, end, chunk[0]);
+    for (int index = 0; first < end; ++index) {
+        const std::vector<FEC>& series_of = chunk[index & 1];
+        int unsafe_at[12];
+        auto run_probe = [&, first, end](int probe) {
+            const FEC dc = probes[probe];
+            std::vector<FEC> k(n);
+            unsafe_at[probe] = end;
+            for (int iteration = first; iteration < end; ++iteration) {
+                const FEC reference = orbit[static_cast<size_t>(iteration)];
+                const FEC exact_next = fec_add(fec_mulf(fec_mul(reference, exact[probe]), two),
+                                               fec_add(fec_mul(exact[probe], exact[probe]),
+                                                       julia ? FEC{z0, z0} : dc));
+                exact[probe] = exact_next;
+                if (iteration + 1 < order) continue;
+                const FEC* series = series_of.data() + static_cast<size_t>(iteration - first) * n;
+                k.assign(series, series + n);
+                if (!series_probe_is_safe(orbit[static_cast<size_t>(iteration) + 1], exact_next,
+                                          series_evaluate(k, dc), minimum_scale_squared, tolerance_squared,
+                                          c.escape_radius_mode)) {
+                    unsafe_at[probe] = iteration;
+                    return;
+                }
+            }
+        };
+        // short chunks (the first few) are cheaper than starting threads
+        const bool threaded = end - first >= 1024;
+        std::vector<std::thread> pool;
+        if (threaded) {
+            for (int probe = 0; probe < 12; ++probe) pool.emplace_back(run_probe, probe);
+        } else {
+            for (int probe = 0; probe < 12; ++probe) run_probe(probe);
         }
+        chunk_size = std::min(kChunk, chunk_size * 2);
+        const int next_first = end;
+        const int next_end = std::min(last_candidate, end + chunk_size);
+        if (threaded && next_first < next_end) advance_series(next_first, next_end, chunk[(index + 1) & 1]);
+        for (auto& t : pool) t.join();
+        const int stop = *std::min_element(unsafe_at, unsafe_at + 12);
+        // iterations first .. stop-1 were safe for every probe
+        if (stop > first && stop >= order) {
+            best_iteration = stop;
+            const FEC* series = series_of.data() + static_cast<size_t>(stop - 1 - first) * n;
+            best.assign(series, series + n);
+        }
+        if (stop < end) break;
+        if (!threaded && next_first < next_end) advance_series(next_first, next_end, chunk[(index + 1) & 1]);
+        first = next_first;
+        end = next_end;
     }
     if (best_iteration <= 1 || best.empty()) return;
     c.series_enabled = true;
@@ -433,6 +484,26 @@ void build_image_series(ReferenceContext& c, const std::vector<FEC>& orbit) {
     c.series_coefficients.reserve(best.size());
     for (const FEC& k : best) c.series_coefficients.push_back(sc_from_fe(k.re, k.im));
 }
+
+// The image series on its own thread, timed into series_build_ns, joined
+// when this object ends.  The series and the BLA tables read the orbit and
+// write disjoint fields of the context, so the caller builds the tables
+// meanwhile.
+class SeriesBuild {
+public:
+    SeriesBuild(ReferenceContext& c, const std::vector<FEC>& orbit)
+        : thread_([&c, &orbit] {
+              const uint64_t started = now_ns();
+              build_image_series(c, orbit);
+              c.series_build_ns = now_ns() - started;
+          }) {}
+    ~SeriesBuild() { thread_.join(); }
+    SeriesBuild(const SeriesBuild&) = delete;
+    SeriesBuild& operator=(const SeriesBuild&) = delete;
+
+private:
+    std::thread thread_;
+};
 
 // BlaStep (renderer.cpp:8476-8493) while the hierarchy is being composed.
 struct CubicBuilder {
@@ -577,9 +648,7 @@ void build_bla(ReferenceContext& c) {
     const int max_iter = static_cast<int>(c.fast_orbit.size()) - 1;
     const int map_end = std::clamp(c.map_end, 0, max_iter);
     const int base_count = std::max(0, std::min(max_iter, map_end) - 1);
-    const uint64_t series_started = now_ns();
-    build_image_series(c, c.fast_orbit);
-    c.series_build_ns = now_ns() - series_started;
+    const SeriesBuild series(c, c.fast_orbit);
     c.cubic.levels.clear();
     c.linear[0].levels.clear();
     c.linear[1].levels.clear();
@@ -707,12 +776,11 @@ std::unique_ptr<ReferenceContext> build_reference(
     } else {
         // renderer.cpp:13683-13697: Julia's image series, then the real
         // 2x2 hierarchy; alternate roots keep their builder orbit for clones
-        if (formula == 1) {
-            const uint64_t ts = now_ns();
-            build_image_series(*c, c->fast_orbit);
-            c->series_build_ns = now_ns() - ts;
+        {
+            std::unique_ptr<SeriesBuild> series;
+            if (formula == 1) series = std::make_unique<SeriesBuild>(*c, c->fast_orbit);
+            build_alternate_bla(*c);
         }
-        build_alternate_bla(*c);
         reusable = true;
         c->reusable = true;
     }
@@ -753,18 +821,15 @@ std::unique_ptr<ReferenceContext> clone_reference(const ReferenceContext& s, con
     if (s.formula != 0) {  // renderer.cpp:13739-13760
         c->fast_orbit = s.fast_orbit;
         c->reusable = true;
-        if (s.formula == 1) {
-            const uint64_t ts = now_ns();
-            build_image_series(*c, c->fast_orbit);
-            c->series_build_ns = now_ns() - ts;
+        {
+            std::unique_ptr<SeriesBuild> series;
+            if (s.formula == 1) series = std::make_unique<SeriesBuild>(*c, c->fast_orbit);
+            build_alternate_bla(*c);
         }
-        build_alternate_bla(*c);
         c->bla_build_ns = now_ns() - t1;
         return c;
     }
-    const uint64_t series_started = now_ns();
-    build_image_series(*c, s.fast_orbit);
-    c->series_build_ns = now_ns() - series_started;
+    const SeriesBuild series(*c, s.fast_orbit);
     c->cubic = s.cubic;
     for (size_t level = 1; level < c->cubic.levels.size(); ++level) {
         const auto& prev = c->cubic.levels[level - 1];

@@ -66,11 +66,20 @@ def test_reference_builds_on_host_and_reports_stats(product):
     assert product.fractal_last_error() == b""
 
 
-def test_reference_stats_match_oracle_series(product, oracle):
-    handle = product.fractal_create_reference(X, Y, b"1e12", 20000, 128, 8)
+@pytest.mark.parametrize("zoom, max_iter, bits, order", [
+    ("1e12", 20000, 128, 8),     # the series stops inside the first 2048-iteration chunk
+    ("1e12", 20000, 128, 24),
+    ("1e16", 30000, 192, 8),     # ... in the second chunk
+    ("1e25", 50000, 256, 8),     # ... in the sixth
+    ("1e25", 50000, 256, 16),
+])
+def test_reference_stats_match_oracle_series(product, oracle, zoom, max_iter, bits, order):
+    """The image series' stopping iteration and order (built in chunks with
+    the probes on several threads) equal the sequential restatement's."""
+    handle = product.fractal_create_reference(X, Y, zoom.encode(), max_iter, bits, order)
     values = np.zeros(5, dtype=np.uint64)
     product.fractal_get_reference_stats(ctypes.c_void_p(handle), values.ctypes.data_as(native.c_uint64_p), 5)
-    ref = native.OracleReference(oracle, X.decode(), Y.decode(), "1e12", 20000, 128, 8)
+    ref = native.OracleReference(oracle, X.decode(), Y.decode(), zoom, max_iter, bits, order)
     info = ref.info()
     assert int(values[3]) == info["series_iteration"] and int(values[4]) == info["series_order"]
     product.fractal_destroy_reference(ctypes.c_void_p(handle))
