@@ -507,6 +507,12 @@ int atlas_kfp_raw(const float* parent, int parent_width, int parent_height, int 
 
 }  // namespace
 
+namespace fav {
+// The thread-local error text of the C ABI, for entry points defined in
+// other files.
+void set_last_error(const std::string& message) { set_error(message); }
+}  // namespace fav
+
 extern "C" {
 
 int fractal_colourise_kfp(const float* field, uint8_t* output, int width, int height, int max_iter, double phase,

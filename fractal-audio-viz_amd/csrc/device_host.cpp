@@ -20,6 +20,7 @@
 #include "fx_core.h"
 #include "sched_mlp.h"
 #include "atlas_core.h"
+#include "aurora_core.h"
 
 namespace fav {
 
@@ -31,6 +32,7 @@ hipError_t launch_alt_kernel(const AltRenderParams& p, hipStream_t stream);
 hipError_t launch_fx_kernel(const RenderParams& p, const fx::FxOut& out, hipStream_t stream);
 hipError_t launch_batch(const BatchLaunch& q, int fast_blocks, int persistent_blocks, hipStream_t stream);
 hipError_t launch_atlas_compose(const atlas::ComposeParams& a, hipStream_t stream);
+hipError_t launch_aurora(const aurora::Params& p, hipStream_t stream);
 
 namespace {
 
@@ -73,7 +75,7 @@ struct LaunchState {
     DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords, fx_counts, tile_order, work,
         mlp_y, mlp_act, mlp_delta, mlp_err2, mlp_partial, mlp_grad, mlp_features, mlp_weights, preview_offsets,
         preview_output, preview_work,
-        atlas_tiles, atlas_axes, atlas_field, points;
+        atlas_tiles, atlas_axes, atlas_field, points, aurora_in, aurora_out;
     int compute_units = 256;
     double wall_clock_khz = 0.0;
     LastTiming timing{};
@@ -363,6 +365,82 @@ void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers) {
     if (!device_pointers)
         check(hipMemcpyAsync(host.output, k.output, pixels * 3, hipMemcpyDeviceToHost, s.stream), "D2H rgb");
     check(hipStreamSynchronize(s.stream), "KFP colour");
+    float kernel_ms = 0.0f;
+    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    s.timing.kernel_ms = kernel_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
+}
+
+void aurora_run(const aurora::Params& host, const AuroraInputs& in) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const size_t pixels = static_cast<size_t>(host.width) * static_cast<size_t>(host.height);
+    aurora::Params p = host;
+    // every input in one block, each piece 256-byte aligned
+    struct Piece {
+        const void* data;
+        size_t bytes;
+    };
+    std::vector<Piece> pieces;
+    auto add = [&](const void* data, size_t bytes) {
+        pieces.push_back(Piece{data, bytes});
+        return pieces.size() - 1;
+    };
+    const size_t parent_field = add(in.parent_field, in.parent_field_count * 4);
+    const size_t child_field = add(in.child_field, in.child_field_count * 4);
+    const size_t parent_rgb = add(in.parent_rgb, in.parent_rgb_bytes);
+    const size_t child_rgb = add(in.child_rgb, in.child_rgb_bytes);
+    size_t axis_piece[4][3];
+    for (int a = 0; a < 4; ++a) {
+        const atlas::Axis* axis = in.axes[a];
+        const size_t n = axis ? axis->weight.size() : 0;
+        axis_piece[a][0] = add(n ? axis->index0.data() : nullptr, n * 4);
+        axis_piece[a][1] = add(n ? axis->index1.data() : nullptr, n * 4);
+        axis_piece[a][2] = add(n ? axis->weight.data() : nullptr, n * 4);
+    }
+    const size_t edge_x = add(in.edge_x ? in.edge_x->data() : nullptr, in.edge_x ? in.edge_x->size() * 4 : 0);
+    const size_t edge_y = add(in.edge_y ? in.edge_y->data() : nullptr, in.edge_y ? in.edge_y->size() * 4 : 0);
+    const size_t palette = add(in.palette ? in.palette->data() : nullptr, in.palette ? in.palette->size() : 0);
+    std::vector<size_t> at(pieces.size());
+    size_t total = 0;
+    for (size_t i = 0; i < pieces.size(); ++i) {
+        at[i] = total;
+        total += (pieces[i].bytes + 255) & ~size_t(255);
+    }
+    char* base = static_cast<char*>(s.aurora_in.get(total + 256));
+    auto device = [&](size_t i) -> const void* { return pieces[i].bytes ? base + at[i] : nullptr; };
+    for (size_t i = 0; i < pieces.size(); ++i)
+        if (pieces[i].bytes)
+            check(hipMemcpyAsync(base + at[i], pieces[i].data, pieces[i].bytes, hipMemcpyHostToDevice, s.stream),
+                  "H2D compositor input");
+    p.parent_field = static_cast<const float*>(device(parent_field));
+    p.child_field = static_cast<const float*>(device(child_field));
+    p.parent_rgb = static_cast<const uint8_t*>(device(parent_rgb));
+    p.child_rgb = static_cast<const uint8_t*>(device(child_rgb));
+    atlas::AxisDev* views[4] = {&p.parent_x, &p.parent_y, &p.child_x, &p.child_y};
+    for (int a = 0; a < 4; ++a)
+        *views[a] = atlas::AxisDev{static_cast<const int32_t*>(device(axis_piece[a][0])),
+                                   static_cast<const int32_t*>(device(axis_piece[a][1])),
+                                   static_cast<const float*>(device(axis_piece[a][2]))};
+    p.edge_x = static_cast<const float*>(device(edge_x));
+    p.edge_y = static_cast<const float*>(device(edge_y));
+    p.palette = static_cast<const uint8_t*>(device(palette));
+    const bool rgb_out = in.out_rgb != nullptr;
+    void* out = s.aurora_out.get(pixels * (rgb_out ? 3 : 4));
+    p.out_field = rgb_out ? nullptr : static_cast<float*>(out);
+    p.out_rgb = rgb_out ? static_cast<uint8_t*>(out) : nullptr;
+    if (p.mode == aurora::kAccents)  // in place on the caller's RGB
+        check(hipMemcpyAsync(out, in.out_rgb, pixels * 3, hipMemcpyHostToDevice, s.stream), "H2D rgb");
+    const uint64_t wall0 = now_ns();
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_aurora(p, s.stream), "compositor launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    check(hipMemcpyAsync(rgb_out ? static_cast<void*>(in.out_rgb) : static_cast<void*>(in.out_field), out,
+                         pixels * (rgb_out ? 3 : 4), hipMemcpyDeviceToHost, s.stream),
+          "D2H compositor output");
+    check(hipStreamSynchronize(s.stream), "compositor");
     float kernel_ms = 0.0f;
     check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
     s.timing.kernel_ms = kernel_ms;

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include <cstdint>
@@ -11,6 +12,9 @@
 #include "fav_host.h"
 
 namespace fav {
+
+// Sets the text fractal_last_error() returns on this thread.
+void set_last_error(const std::string& message);
 
 struct FieldRequest {
     float* output;                 // host or device pointer (see device_pointers)
@@ -101,6 +105,30 @@ namespace atlas { struct ComposeParams; struct Axis; }
 void atlas_colourise_kfp_raw(const atlas::ComposeParams& compose, size_t parent_count, size_t child_count,
                              const atlas::Axis* axes, const kfp::KfpParams& colour, uint8_t* output);
 
+
+// Ordinary-palette colouriser and crop / atlas compositors (aurora_core.h).
+// `params` carries the mode, geometry and palette controls; the sources,
+// axes (parent x / y, child x / y; empty when unused), seam weights and
+// palette are host arrays uploaded here; results go to the host buffers
+// out_field (float modes) or out_rgb (RGB modes; kAccents reads it too).
+namespace aurora { struct Params; }
+struct AuroraInputs {
+    const float* parent_field = nullptr;
+    size_t parent_field_count = 0;
+    const float* child_field = nullptr;
+    size_t child_field_count = 0;
+    const uint8_t* parent_rgb = nullptr;
+    size_t parent_rgb_bytes = 0;
+    const uint8_t* child_rgb = nullptr;
+    size_t child_rgb_bytes = 0;
+    const atlas::Axis* axes[4] = {nullptr, nullptr, nullptr, nullptr};
+    const std::vector<float>* edge_x = nullptr;
+    const std::vector<float>* edge_y = nullptr;
+    const std::vector<uint8_t>* palette = nullptr;
+    float* out_field = nullptr;
+    uint8_t* out_rgb = nullptr;
+};
+void aurora_run(const aurora::Params& params, const AuroraInputs& inputs);
 
 // Shallow direct render on the device: x/y are the per-column/per-row
 // coordinates; `params` output/plane pointers are host pointers unless

@@ -214,6 +214,26 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.fractal_atlas_colourise_kfp_raw.restype = ctypes.c_int
     lib.fractal_atlas_colourise_kfp_raw_precise.argtypes = lib.fractal_atlas_colourise_kfp_raw.argtypes
     lib.fractal_atlas_colourise_kfp_raw_precise.restype = ctypes.c_int
+    # ordinary palette and per-frame compositors (renderer.h:400-449, 713-835, 943-1068)
+    F, U8, I, D = _c_float_p, _c_uint8_p, ctypes.c_int, ctypes.c_double
+    for name, args in {
+        "fractal_colourise": [F, U8, I, I, I, D, D, D, D, I],
+        "fractal_colourise_accents": [F, U8, I, I, I, D, D, D, D, U8, I, I, I, I],
+        "fractal_apply_aurora_accents": [U8, I, I, U8, D, I],
+        "fractal_crop_rgb": [U8, I, I, U8, I, I, D, I],
+        "fractal_atlas_composite_rgb": [U8, I, I, U8, I, I, U8, I, I, D, D, D, I, I],
+        "fractal_crop_field": [F, I, I, F, I, I, D, I],
+        "fractal_atlas_field": [F, I, I, I, F, I, I, I, F, I, I, D, D, I, I],
+        "fractal_atlas_field_ex": [F, I, I, I, F, I, I, I, F, I, I, D, D, D, D, D, D, I, I, I],
+        "fractal_crop_colourise": [F, I, I, U8, I, I, D, I, D, D, D, D, I],
+        "fractal_crop_colourise_interior": [F, I, I, U8, I, I, D, I, D, D, D, D, I, I, I, I],
+        "fractal_crop_colourise_accents": [F, I, I, U8, I, I, D, I, D, D, D, D, U8, I, I, I, I],
+        "fractal_atlas_colourise": [F, I, I, I, F, I, I, I, U8, I, I, D, D, I, D, D, D, D, I],
+        "fractal_atlas_colourise_interior": [F, I, I, I, F, I, I, I, U8, I, I, D, D, I, D, D, D, D, I, I, I, I],
+        "fractal_atlas_colourise_accents": [F, I, I, I, F, I, I, I, U8, I, I, D, D, I, D, D, D, D, U8, I, I, I, I],
+    }.items():
+        getattr(lib, name).argtypes = args
+        getattr(lib, name).restype = ctypes.c_int
     lib.fractal_set_stats_enabled.argtypes = [ctypes.c_int]
     lib.fractal_get_last_stats_ex.argtypes = [_c_uint64_p, ctypes.c_int]
     lib.fractal_get_last_stats_ex.restype = ctypes.c_int

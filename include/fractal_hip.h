@@ -312,6 +312,103 @@ int fractal_atlas_colourise_kfp_raw_precise(
     const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
     int threads);
 
+/* ---- Ordinary (Aurora) palette and per-frame compositors --------------
+ * The palette (renderer.cpp:4782-4923) is built on the host; every pixel is
+ * computed on the device.  Byte and float results equal the reference's
+ * CPU functions. */
+
+/* renderer.h:400-411 / renderer.cpp:15926-15951, 13771-13825. */
+int fractal_colourise(
+    const float *field, uint8_t *output, int width, int height, int max_iter,
+    double phase, double vocal, double instrumental, double pitch, int threads);
+/* renderer.h:426-441 / renderer.cpp:16210-16248: the palette projected
+ * through three RGB accents, with an interior colour. */
+int fractal_colourise_accents(
+    const float *field, uint8_t *output, int width, int height, int max_iter,
+    double phase, double vocal, double instrumental, double pitch,
+    const uint8_t *accents, int interior_red, int interior_green,
+    int interior_blue, int threads);
+/* renderer.h:442-449 / renderer.cpp:16250-16300 (in place). */
+int fractal_apply_aurora_accents(
+    uint8_t *output, int width, int height, const uint8_t *accents,
+    double pitch, int threads);
+/* renderer.h:716-725 / renderer.cpp:18399-18465. */
+int fractal_crop_rgb(
+    const uint8_t *source, int source_width, int source_height,
+    uint8_t *output, int output_width, int output_height, double zoom_factor,
+    int threads);
+/* renderer.h:726-741 / renderer.cpp:18467-18627. */
+int fractal_atlas_composite_rgb(
+    const uint8_t *parent, int parent_width, int parent_height,
+    const uint8_t *child, int child_width, int child_height, uint8_t *output,
+    int output_width, int output_height, double parent_zoom,
+    double child_fraction, double child_zoom, int feather, int threads);
+/* renderer.h:786-795 / renderer.cpp:18631-18689. */
+int fractal_crop_field(
+    const float *source, int source_width, int source_height, float *output,
+    int output_width, int output_height, double zoom_factor, int threads);
+/* renderer.h:797-813 / renderer.cpp:18890-18927. */
+int fractal_atlas_field(
+    const float *parent, int parent_width, int parent_height,
+    int parent_max_iter, const float *child, int child_width,
+    int child_height, int child_max_iter, float *output, int output_width,
+    int output_height, double parent_zoom, double child_fraction,
+    int palette_max_iter, int threads);
+/* renderer.h:815-835 / renderer.cpp:18697-18888. */
+int fractal_atlas_field_ex(
+    const float *parent, int parent_width, int parent_height,
+    int parent_max_iter, const float *child, int child_width,
+    int child_height, int child_max_iter, float *output, int output_width,
+    int output_height, double parent_zoom, double child_fraction,
+    double child_zoom, double parent_field_bias, double child_field_bias,
+    double output_field_bias, int palette_max_iter, int feather, int threads);
+/* renderer.h:943-957 / renderer.cpp:18930-19053. */
+int fractal_crop_colourise(
+    const float *source, int source_width, int source_height,
+    uint8_t *output, int output_width, int output_height, double zoom_factor,
+    int max_iter, double phase, double vocal, double instrumental,
+    double pitch, int threads);
+/* renderer.h:959-976 / renderer.cpp:19055-19094. */
+int fractal_crop_colourise_interior(
+    const float *source, int source_width, int source_height,
+    uint8_t *output, int output_width, int output_height, double zoom_factor,
+    int max_iter, double phase, double vocal, double instrumental,
+    double pitch, int interior_red, int interior_green, int interior_blue,
+    int threads);
+/* renderer.h:978-996 / renderer.cpp:19096-19136. */
+int fractal_crop_colourise_accents(
+    const float *source, int source_width, int source_height,
+    uint8_t *output, int output_width, int output_height, double zoom_factor,
+    int max_iter, double phase, double vocal, double instrumental,
+    double pitch, const uint8_t *accents, int interior_red,
+    int interior_green, int interior_blue, int threads);
+/* renderer.h:997-1017 / renderer.cpp:19137-19455. */
+int fractal_atlas_colourise(
+    const float *parent, int parent_width, int parent_height,
+    int parent_max_iter, const float *child, int child_width,
+    int child_height, int child_max_iter, uint8_t *output, int output_width,
+    int output_height, double parent_zoom, double child_fraction,
+    int palette_max_iter, double phase, double vocal, double instrumental,
+    double pitch, int threads);
+/* renderer.h:1019-1042 / renderer.cpp:19457-19508. */
+int fractal_atlas_colourise_interior(
+    const float *parent, int parent_width, int parent_height,
+    int parent_max_iter, const float *child, int child_width,
+    int child_height, int child_max_iter, uint8_t *output, int output_width,
+    int output_height, double parent_zoom, double child_fraction,
+    int palette_max_iter, double phase, double vocal, double instrumental,
+    double pitch, int interior_red, int interior_green, int interior_blue,
+    int threads);
+/* renderer.h:1044-1068 / renderer.cpp:19510-19566. */
+int fractal_atlas_colourise_accents(
+    const float *parent, int parent_width, int parent_height,
+    int parent_max_iter, const float *child, int child_width,
+    int child_height, int child_max_iter, uint8_t *output, int output_width,
+    int output_height, double parent_zoom, double child_fraction,
+    int palette_max_iter, double phase, double vocal, double instrumental,
+    double pitch, const uint8_t *accents, int interior_red,
+    int interior_green, int interior_blue, int threads);
+
 /* Deep render of caller-supplied dc offsets (mantissas and exponents, one
  * value per point) against a reference handle: the field loop of
  * fractal_render_reference_ex with the points in place of a pixel grid and

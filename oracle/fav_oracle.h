@@ -117,6 +117,61 @@ int fo_atlas_colourise_kfp_raw(
 int fo_kfp_palette_lut(const uint8_t *stops, int stop_count, uint8_t *lut,
                        int lut_size);
 
+/* Ordinary palette and per-frame compositors (fav_oracle_aurora.c): each
+ * fo_X takes the arguments of the renderer.h entry point fractal_X
+ * (renderer.h:400-449, 713-835, 943-1068) and returns 0, or 1 for invalid
+ * arguments. */
+int fo_colourise(const float *field, uint8_t *output, int width, int height, int max_iter, double phase,
+                 double vocal, double instrumental, double pitch, int threads);
+int fo_colourise_accents(const float *field, uint8_t *output, int width, int height, int max_iter, double phase,
+                         double vocal, double instrumental, double pitch, const uint8_t *accents, int interior_red,
+                         int interior_green, int interior_blue, int threads);
+int fo_apply_aurora_accents(uint8_t *output, int width, int height, const uint8_t *accents, double pitch,
+                            int threads);
+int fo_crop_rgb(const uint8_t *source, int source_width, int source_height, uint8_t *output, int output_width,
+                int output_height, double zoom_factor, int threads);
+int fo_atlas_composite_rgb(const uint8_t *parent, int parent_width, int parent_height, const uint8_t *child,
+                           int child_width, int child_height, uint8_t *output, int output_width, int output_height,
+                           double parent_zoom, double child_fraction, double child_zoom, int feather, int threads);
+int fo_crop_field(const float *source, int source_width, int source_height, float *output, int output_width,
+                  int output_height, double zoom_factor, int threads);
+int fo_atlas_field(const float *parent, int parent_width, int parent_height, int parent_max_iter, const float *child,
+                   int child_width, int child_height, int child_max_iter, float *output, int output_width,
+                   int output_height, double parent_zoom, double child_fraction, int palette_max_iter, int threads);
+int fo_atlas_field_ex(const float *parent, int parent_width, int parent_height, int parent_max_iter,
+                      const float *child, int child_width, int child_height, int child_max_iter, float *output,
+                      int output_width, int output_height, double parent_zoom, double child_fraction,
+                      double child_zoom, double parent_field_bias, double child_field_bias, double output_field_bias,
+                      int palette_max_iter, int feather, int threads);
+int fo_crop_colourise(const float *source, int source_width, int source_height, uint8_t *output, int output_width,
+                      int output_height, double zoom_factor, int max_iter, double phase, double vocal,
+                      double instrumental, double pitch, int threads);
+int fo_crop_colourise_interior(const float *source, int source_width, int source_height, uint8_t *output,
+                               int output_width, int output_height, double zoom_factor, int max_iter, double phase,
+                               double vocal, double instrumental, double pitch, int interior_red, int interior_green,
+                               int interior_blue, int threads);
+int fo_crop_colourise_accents(const float *source, int source_width, int source_height, uint8_t *output,
+                              int output_width, int output_height, double zoom_factor, int max_iter, double phase,
+                              double vocal, double instrumental, double pitch, const uint8_t *accents,
+                              int interior_red, int interior_green, int interior_blue, int threads);
+int fo_atlas_colourise(const float *parent, int parent_width, int parent_height, int parent_max_iter,
+                       const float *child, int child_width, int child_height, int child_max_iter, uint8_t *output,
+                       int output_width, int output_height, double parent_zoom, double child_fraction,
+                       int palette_max_iter, double phase, double vocal, double instrumental, double pitch,
+                       int threads);
+int fo_atlas_colourise_interior(const float *parent, int parent_width, int parent_height, int parent_max_iter,
+                                const float *child, int child_width, int child_height, int child_max_iter,
+                                uint8_t *output, int output_width, int output_height, double parent_zoom,
+                                double child_fraction, int palette_max_iter, double phase, double vocal,
+                                double instrumental, double pitch, int interior_red, int interior_green,
+                                int interior_blue, int threads);
+int fo_atlas_colourise_accents(const float *parent, int parent_width, int parent_height, int parent_max_iter,
+                               const float *child, int child_width, int child_height, int child_max_iter,
+                               uint8_t *output, int output_width, int output_height, double parent_zoom,
+                               double child_fraction, int palette_max_iter, double phase, double vocal,
+                               double instrumental, double pitch, const uint8_t *accents, int interior_red,
+                               int interior_green, int interior_blue, int threads);
+
 #ifdef __cplusplus
 }
 #endif
