@@ -406,14 +406,15 @@ int colourise_kfp(const float* field, uint8_t* output, int width, int height, in
     try {
         if (!field || !output || !valid_dimensions(width, height) || !valid_iterations(max_iter)
             || !valid_threads(threads) || !std::isfinite(phase) || !std::isfinite(vocal)
-            || !std::isfinite(instrumental) || !std::isfinite(pitch) || !lut || !fav::kfp::valid_options(options, lut_size))
+            || !std::isfinite(instrumental) || !std::isfinite(pitch) || !lut || !fav::kfp::valid_options(options, lut_size)
+            || !fav::kfp::valid_planes(planes, width, height))
             throw std::runtime_error("invalid native KFP colour dimensions or controls");
-        if (planes != nullptr)
-            throw std::runtime_error(
-                "the HIP KFP colour pass takes the scalar field only; orbit/DE/phase/texture planes are not supported");
-        const fav::kfp::KfpParams k = fav::kfp::make_params(*options, field, output, width, height, max_iter, lut,
-                                                            lut_size);
-        fav::colourise_kfp_on_device(k, device_pointers);
+        if (planes != nullptr && device_pointers)
+            throw std::runtime_error("Kalles planes are taken by the host-pointer KFP entries");
+        fav::kfp::KfpParams k = fav::kfp::make_params(*options, field, output, width, height, max_iter, lut,
+                                                      lut_size);
+        fav::kfp::set_plane_context(k, *options, planes);
+        fav::colourise_kfp_on_device(k, device_pointers, planes);
         set_error("");
         return 0;
     } catch (const std::exception& e) {

@@ -75,7 +75,7 @@ struct LaunchState {
     DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords, fx_counts, tile_order, work,
         mlp_y, mlp_act, mlp_delta, mlp_err2, mlp_partial, mlp_grad, mlp_features, mlp_weights, preview_offsets,
         preview_output, preview_work,
-        atlas_tiles, atlas_axes, atlas_field, points, aurora_in, aurora_out;
+        atlas_tiles, atlas_axes, atlas_field, points, aurora_in, aurora_out, kfp_planes;
     int compute_units = 256;
     double wall_clock_khz = 0.0;
     LastTiming timing{};
@@ -342,7 +342,7 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     if (r.stats) r.stats[22] = static_cast<uint64_t>(static_cast<double>(kernel_ms) * 1e6);  // render_ns
 }
 
-void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers) {
+void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers, const FractalKfpPlanes* planes) {
     LaunchState& s = state();
     std::lock_guard<std::mutex> lock(s.mutex);
     init_locked(s);
@@ -353,6 +353,37 @@ void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers) {
         check(hipMemcpyAsync(f, host.field, pixels * sizeof(float), hipMemcpyHostToDevice, s.stream), "H2D field");
         k.field = static_cast<const float*>(f);
         k.output = static_cast<uint8_t*>(s.kfp_rgb.get(pixels * 3));
+    }
+    if (planes && !device_pointers) {  // the present planes, 8 bytes per pixel each, then the texture
+        const void* sources[9] = {planes->orbit_iteration, planes->iteration, planes->bailout, planes->transition,
+                                  planes->phase, planes->de_x, planes->de_y, planes->test1, planes->test2};
+        const void** targets[9] = {reinterpret_cast<const void**>(&k.orbit_iteration),
+                                   reinterpret_cast<const void**>(&k.iteration),
+                                   reinterpret_cast<const void**>(&k.bailout),
+                                   reinterpret_cast<const void**>(&k.transition),
+                                   reinterpret_cast<const void**>(&k.phase),
+                                   reinterpret_cast<const void**>(&k.de_x),
+                                   reinterpret_cast<const void**>(&k.de_y),
+                                   reinterpret_cast<const void**>(&k.test1),
+                                   reinterpret_cast<const void**>(&k.test2)};
+        const size_t plane_bytes = (pixels * 8 + 255) & ~size_t(255);
+        const size_t texture_bytes = k.texture_active
+            ? static_cast<size_t>(planes->texture_stride) * static_cast<size_t>(planes->texture_height) : 0;
+        char* block = static_cast<char*>(s.kfp_planes.get(plane_bytes * 9 + texture_bytes + 256));
+        for (int i = 0; i < 9; ++i) {
+            *targets[i] = nullptr;
+            if (!sources[i]) continue;
+            check(hipMemcpyAsync(block + plane_bytes * i, sources[i], pixels * 8, hipMemcpyHostToDevice, s.stream),
+                  "H2D KFP plane");
+            *targets[i] = block + plane_bytes * i;
+        }
+        k.texture = nullptr;
+        if (texture_bytes) {
+            check(hipMemcpyAsync(block + plane_bytes * 9, planes->texture_rgb, texture_bytes, hipMemcpyHostToDevice,
+                                 s.stream),
+                  "H2D KFP texture");
+            k.texture = reinterpret_cast<const uint8_t*>(block + plane_bytes * 9);
+        }
     }
     void* l = s.kfp_lut.get(static_cast<size_t>(host.lut_size) * 3);
     check(hipMemcpyAsync(l, host.lut, static_cast<size_t>(host.lut_size) * 3, hipMemcpyHostToDevice, s.stream), "H2D lut");

@@ -135,7 +135,8 @@ void aurora_run(const aurora::Params& params, const AuroraInputs& inputs);
 // device_pointers is set.
 void render_direct_on_device(const direct::DirectParams& params, const double* x, const double* y, bool planes,
                              bool device_pointers);
-void colourise_kfp_on_device(const kfp::KfpParams& params, bool device_pointers);
+void colourise_kfp_on_device(const kfp::KfpParams& params, bool device_pointers,
+                             const FractalKfpPlanes* planes = nullptr);
 LastTiming last_timing();
 bool device_available();
 

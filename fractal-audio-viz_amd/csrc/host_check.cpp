@@ -149,13 +149,27 @@ extern "C" int fav_host_check_render(
 // KFP colour pass of kfp_kernel.hip run on the CPU (test-only): the same
 // option validation and per-pixel code, with the method-4 range reduced
 // serially.  Returns 1 with `err` set when the options are rejected.
-extern "C" int fav_host_check_kfp(const float* field, uint8_t* output, int width, int height, int max_iter,
-                                  const FractalKfpOptions* options, const uint8_t* lut, int lut_size, char* err,
-                                  int cap) {
+extern "C" int fav_host_check_kfp_planes(const float* field, uint8_t* output, int width, int height, int max_iter,
+                                         const FractalKfpOptions* options, const FractalKfpPlanes* planes,
+                                         const uint8_t* lut, int lut_size, char* err, int cap) {
     try {
-        if (!fav::kfp::valid_options(options, lut_size)) throw std::runtime_error("invalid KFP options");
+        if (!fav::kfp::valid_options(options, lut_size) || !fav::kfp::valid_planes(planes, width, height))
+            throw std::runtime_error("invalid KFP options");
         fav::kfp::KfpParams k =
             fav::kfp::make_params(*options, field, output, width, height, max_iter, lut, lut_size);
+        fav::kfp::set_plane_context(k, *options, planes);
+        if (planes) {
+            k.orbit_iteration = planes->orbit_iteration;
+            k.iteration = planes->iteration;
+            k.bailout = planes->bailout;
+            k.transition = planes->transition;
+            k.phase = planes->phase;
+            k.de_x = planes->de_x;
+            k.de_y = planes->de_y;
+            k.test1 = planes->test1;
+            k.test2 = planes->test2;
+            k.texture = k.texture_active ? planes->texture_rgb : nullptr;
+        }
         double bounds[2] = {0.0, 1.0};
         if (k.color_method == 4) {
             long long lo = 0, hi = 0;
@@ -184,6 +198,13 @@ extern "C" int fav_host_check_kfp(const float* field, uint8_t* output, int width
         }
         return 1;
     }
+}
+
+extern "C" int fav_host_check_kfp(const float* field, uint8_t* output, int width, int height, int max_iter,
+                                  const FractalKfpOptions* options, const uint8_t* lut, int lut_size, char* err,
+                                  int cap) {
+    return fav_host_check_kfp_planes(field, output, width, height, max_iter, options, nullptr, lut, lut_size, err,
+                                     cap);
 }
 
 #include "direct_prepare.h"

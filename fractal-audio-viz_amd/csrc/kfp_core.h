@@ -1,5 +1,7 @@
 // kfp_core.h -- per-pixel Kalles SetColor transfer of a scalar iteration
-// field (renderer.cpp:6696-7289 without orbit planes, phase or texture).
+// field and, when given, its Kalles planes: orbit / stored iteration,
+// bailout, transition, phase, analytic DE, |z|^2 tests and a texture
+// (renderer.cpp:5655-5976, 6696-7289).
 //
 // Compiled for the gfx950 colour kernel (kfp_kernel.hip) and, unchanged, for
 // the test-only CPU harness.  Double and float stages follow the reference's
@@ -33,6 +35,25 @@ struct KfpParams {
     int32_t interior_color[3];
     double multi_color_period[256];
     int32_t multi_color_type[256];
+    // FractalKfpPlanes (each null when absent; has_planes when a planes
+    // struct was given at all) and the kfp_pixel_context constants
+    // (renderer.cpp:5655-5763) evaluated on the host
+    int32_t has_planes;
+    const int64_t* orbit_iteration;
+    const int64_t* iteration;
+    const double* bailout;
+    const double* transition;
+    const double* phase;
+    const double* de_x;
+    const double* de_y;
+    const double* test1;
+    const double* test2;
+    const uint8_t* texture;
+    int32_t texture_width, texture_height, texture_stride;
+    int32_t smooth_method, has_constant_bailout, has_smooth_bailout_log, has_smooth_power_log;
+    int32_t has_smooth_norm_exponent, texture_active, show_glitches;
+    double power, selected_bailout_radius, smooth_bailout_radius, smooth_bailout_log, smooth_power_log;
+    double smooth_norm_exponent, phase_color_strength, texture_merge, texture_power, texture_ratio;
 };
 
 FAV_HD double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -72,6 +93,103 @@ FAV_HD double reflected(const KfpParams& k, int x, int y, int ox, int oy, double
     const int px = x - ox, py = y - oy;
     if (px >= 0 && px < k.width && py >= 0 && py < k.height) return 2.0 * centre - colour_sample(k, px, py);
     return centre;
+}
+
+FAV_HD bool has_iteration_planes(const KfpParams& k) { return k.orbit_iteration || k.iteration; }
+
+// kfp_plane_bailout, renderer.cpp:5765-5774.
+FAV_HD double plane_bailout(const KfpParams& k, size_t i) {
+    if (k.bailout) {
+        const double v = k.bailout[i];
+        if (__builtin_isfinite(v) && v > 1.0) return v;
+    }
+    return k.selected_bailout_radius;
+}
+
+// kfp_plane_smooth_part with the pixel context, renderer.cpp:5776-5837.
+FAV_HD double plane_smooth_part(const KfpParams& k, size_t i) {
+    if (!k.test1) return 0.0;
+    const double test1 = k.test1[i];
+    if (!__builtin_isfinite(test1) || test1 <= 0.0) return 0.0;
+    double smooth = 0.0;
+    if (k.smooth_method == 0) {
+        const double radius = k.has_constant_bailout ? k.smooth_bailout_radius : plane_bailout(k, i);
+        if (radius > 1.0 && __builtin_isfinite(k.power) && k.power > 1.0) {
+            const double numerator = ::log(::sqrt(test1));
+            const double denominator = k.has_smooth_bailout_log ? k.smooth_bailout_log : ::log(radius);
+            const double power_log = k.has_smooth_power_log ? k.smooth_power_log : ::log(k.power);
+            if (__builtin_isfinite(numerator) && numerator > 0.0 && __builtin_isfinite(denominator)
+                && denominator != 0.0 && __builtin_isfinite(power_log) && power_log != 0.0)
+                smooth = 1.0 - ::log(numerator / denominator) / power_log;
+        }
+    } else if (k.smooth_method == 1 && k.test2) {
+        const double test2 = k.test2[i];
+        if (__builtin_isfinite(test2) && test2 >= 0.0 && k.has_smooth_norm_exponent) {
+            const double root1 = ::pow(test1, k.smooth_norm_exponent);
+            const double root2 = ::pow(test2, k.smooth_norm_exponent);
+            const double denominator = root1 - root2;
+            const double radius = k.has_constant_bailout ? k.smooth_bailout_radius : plane_bailout(k, i);
+            if (__builtin_isfinite(root1) && __builtin_isfinite(root2) && __builtin_isfinite(radius)
+                && denominator != 0.0)
+                smooth = 1.0 - (root1 - radius) / denominator;
+        }
+    }
+    return __builtin_isfinite(smooth) ? smooth : 0.0;
+}
+
+// kfp_plane_orbit_sample, renderer.cpp:5844-5864: the continuous colour
+// sample of an orbit-plane pixel, max_iter + 1 for interior.
+FAV_HD double orbit_sample(const KfpParams& k, size_t i, bool& inside) {
+    const int64_t raw = k.orbit_iteration[i] > 0 ? k.orbit_iteration[i] : 0;
+    if (raw >= k.max_iter) {
+        inside = true;
+        return static_cast<double>(k.max_iter) + 1.0;
+    }
+    const double combined = static_cast<double>(raw) + plane_smooth_part(k, i);
+    inside = combined >= static_cast<double>(k.max_iter);
+    return inside ? static_cast<double>(k.max_iter) + 1.0 : combined;
+}
+
+// kfp_plane_colour_sample, renderer.cpp:5866-5905 (with the context's
+// smooth offset).
+FAV_HD double plane_colour_sample(const KfpParams& k, int x, int y) {
+    x = clampi(x, 0, k.width - 1);
+    y = clampi(y, 0, k.height - 1);
+    const size_t i = static_cast<size_t>(y) * static_cast<size_t>(k.width) + static_cast<size_t>(x);
+    if (k.orbit_iteration) {
+        bool inside;
+        return orbit_sample(k, i, inside);
+    }
+    if (k.iteration) {
+        const int64_t stored = k.iteration[i];
+        if (stored >= k.max_iter) return static_cast<double>(k.max_iter) + 1.0;
+        const double t = k.transition ? k.transition[i] : 1.0;
+        return static_cast<double>(stored > 0 ? stored : 0) + 1.0 - (__builtin_isfinite(t) ? t : 1.0);
+    }
+    return colour_sample(k, x, y);
+}
+
+// The stencil sample of write_kfp_pixel's `reflected` (renderer.cpp:6834-6889).
+FAV_HD double reflected_sample(const KfpParams& k, int x, int y, int ox, int oy, double centre) {
+    if (!has_iteration_planes(k)) return reflected(k, x, y, ox, oy, centre);
+    const int sx = x + ox, sy = y + oy;
+    if (sx >= 0 && sx < k.width && sy >= 0 && sy < k.height) return plane_colour_sample(k, sx, sy);
+    const int px = x - ox, py = y - oy;
+    if (px >= 0 && px < k.width && py >= 0 && py < k.height) return 2.0 * centre - plane_colour_sample(k, px, py);
+    return centre;
+}
+
+FAV_HD double plane_value(const double* v, size_t i) {  // kfp_plane_value, renderer.cpp:5936-5943
+    if (!v) return 0.0;
+    return __builtin_isfinite(v[i]) ? v[i] : 0.0;
+}
+
+// kfp_texture_index, renderer.cpp:5971-5976.
+FAV_HD int texture_index(double value, int limit) {
+    if (limit <= 0 || !__builtin_isfinite(value)) return 0;
+    if (value <= 0.0) return 0;
+    if (value >= static_cast<double>(limit - 1)) return limit - 1;
+    return static_cast<int>(value);
 }
 
 // kfp_difference_magnitude, renderer.cpp:5497-5589.
@@ -158,59 +276,87 @@ FAV_HD void palette_rgb(const KfpParams& k, double input, float& r, float& g, fl
     }
 }
 
-// write_kfp_pixel without planes, renderer.cpp:6696-7289.
+// write_kfp_pixel, renderer.cpp:6696-7289.
 FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
     const size_t idx = static_cast<size_t>(y) * static_cast<size_t>(k.width) + static_cast<size_t>(x);
     uint8_t* dst = k.output + idx * 3U;
-    const float raw = k.field[idx];
-    const bool inside = inside_value(raw, k.max_iter, k.field_bias);
-    if (inside) {
+    if (k.transition && k.transition[idx] < 0.0 && !k.show_glitches) {  // a hidden glitch pixel
+        dst[0] = 0;
+        dst[1] = 0;
+        dst[2] = 0;
+        return;
+    }
+    bool inside = false;
+    double smooth_iter;
+    if (k.orbit_iteration) {
+        smooth_iter = orbit_sample(k, idx, inside);
+    } else {
+        const float raw = k.field[idx];
+        if (k.iteration) {
+            inside = k.iteration[idx] >= k.max_iter;  // kfp_plane_inside, renderer.cpp:5907-5934
+            smooth_iter = plane_colour_sample(k, x, y);
+        } else {
+            inside = inside_value(raw, k.max_iter, k.field_bias);
+            const double safe = __builtin_isfinite(raw)
+                ? clampd(static_cast<double>(raw) + k.field_bias, 0.0, static_cast<double>(k.max_iter)) : 0.0;
+            smooth_iter = maxd(0.0, safe + k.smooth_offset);
+        }
+    }
+    if (inside && !k.texture_active) {
         dst[0] = static_cast<uint8_t>(k.interior_color[0]);
         dst[1] = static_cast<uint8_t>(k.interior_color[1]);
         dst[2] = static_cast<uint8_t>(k.interior_color[2]);
         return;
     }
-    // the centre sample (materialised or not, the value is the same for an
-    // escaped pixel)
-    const double safe = __builtin_isfinite(raw)
-        ? clampd(static_cast<double>(raw) + k.field_bias, 0.0, static_cast<double>(k.max_iter)) : 0.0;
-    const double smooth_iter = maxd(0.0, safe + k.smooth_offset);
     const double colour_iter = k.flat ? ::floor(smooth_iter) : smooth_iter;
     const double c = smooth_iter;
     double l = c, r = c, u = c, dn = c, tl = c, tr = c, bl = c, br = c;
-    if (k.needs_difference || k.needs_slopes) {
-        l = reflected(k, x, y, -1, 0, c);
-        u = reflected(k, x, y, 0, -1, c);
+    if (k.needs_difference || k.needs_slopes || k.texture_active) {
+        l = reflected_sample(k, x, y, -1, 0, c);
+        u = reflected_sample(k, x, y, 0, -1, c);
         if (k.needs_difference) {
             switch (k.differences) {
                 case 0:
-                    tl = reflected(k, x, y, -1, -1, c);
-                    bl = reflected(k, x, y, -1, 1, c);
+                    tl = reflected_sample(k, x, y, -1, -1, c);
+                    bl = reflected_sample(k, x, y, -1, 1, c);
                     break;
                 case 1: case 2: case 5: case 6:
-                    r = reflected(k, x, y, 1, 0, c);
-                    dn = reflected(k, x, y, 0, 1, c);
-                    tl = reflected(k, x, y, -1, -1, c);
-                    tr = reflected(k, x, y, 1, -1, c);
-                    bl = reflected(k, x, y, -1, 1, c);
-                    br = reflected(k, x, y, 1, 1, c);
+                    r = reflected_sample(k, x, y, 1, 0, c);
+                    dn = reflected_sample(k, x, y, 0, 1, c);
+                    tl = reflected_sample(k, x, y, -1, -1, c);
+                    tr = reflected_sample(k, x, y, 1, -1, c);
+                    bl = reflected_sample(k, x, y, -1, 1, c);
+                    br = reflected_sample(k, x, y, 1, 1, c);
                     break;
                 case 3: case 4:
-                    tl = reflected(k, x, y, -1, -1, c);
+                    tl = reflected_sample(k, x, y, -1, -1, c);
                     break;
                 default:
-                    r = reflected(k, x, y, 1, 0, c);
-                    dn = reflected(k, x, y, 0, 1, c);
+                    r = reflected_sample(k, x, y, 1, 0, c);
+                    dn = reflected_sample(k, x, y, 0, 1, c);
                     break;
             }
         }
-        if (k.needs_slopes) {
-            if (x == 0) r = reflected(k, x, y, 1, 0, c);
-            if (y == 0) dn = reflected(k, x, y, 0, 1, c);
+        if (k.needs_slopes || k.texture_active) {
+            if (x == 0) r = reflected_sample(k, x, y, 1, 0, c);
+            if (y == 0) dn = reflected_sample(k, x, y, 0, 1, c);
         }
     }
-    const double gradient = k.needs_difference
-        ? difference_magnitude(k.differences, c, l, r, u, dn, tl, tr, bl, br) : 0.0;
+    // analytic slopes from the DE planes in float (renderer.cpp:6943-6962)
+    const bool analytic = k.differences == 7 && k.de_x && k.de_y;
+    double slope_x = 0.0, slope_y = 0.0, gradient = 0.0;
+    if ((k.needs_difference || k.needs_slopes) && analytic) {
+        const float dx = static_cast<float>(plane_value(k.de_x, idx));
+        const float dy = static_cast<float>(plane_value(k.de_y, idx));
+        const float denominator = dx * dx + dy * dy;
+        if (denominator > 0.0F && __builtin_isfinite(denominator)) {
+            gradient = static_cast<double>(1.0F / ::sqrtf(denominator));
+            slope_x = static_cast<double>(dx / denominator);
+            slope_y = static_cast<double>(-dy / denominator);
+        }
+    } else if (k.needs_difference) {
+        gradient = difference_magnitude(k.differences, c, l, r, u, dn, tl, tr, bl, br);
+    }
     double distance = __builtin_isnan(gradient) ? 0.0 : gradient * static_cast<double>(k.width) / 640.0;
     if (distance < 0.0) distance = 0.0;
     double transfer = colour_iter;
@@ -236,6 +382,7 @@ FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
     }
     if (__builtin_isnan(transfer) || transfer < 0.0) transfer = 0.0;
     if (k.color_method == 5 || k.color_method == 7 || k.color_method == 8) transfer = clampd(transfer, 0.0, 1024.0);
+    const double phase_shift = k.phase ? k.phase_color_strength / 100.0 * 1024.0 * plane_value(k.phase, idx) : 0.0;
     double palette_value = transfer;
     if (k.iter_div != 1.0) palette_value /= k.iter_div;
     const double base = palette_value + k.color_offset;
@@ -255,7 +402,7 @@ FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
         hsv_to_rgb(static_cast<float>(hc > 0 ? hs / hc : 0.0), static_cast<float>(sc > 0 ? ss / sc : 0.0),
                    static_cast<float>(vc > 0 ? vs / vc : 0.0), mr, mg, mb);
         if (k.blend_multi_color) {
-            palette_rgb(k, base, red, green, blue);
+            palette_rgb(k, base + phase_shift, red, green, blue);
             red = (red + mr) * 0.5F;
             green = (green + mg) * 0.5F;
             blue = (blue + mb) * 0.5F;
@@ -263,11 +410,41 @@ FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
             red = mr; green = mg; blue = mb;
         }
     } else {
-        palette_rgb(k, base, red, green, blue);
+        palette_rgb(k, base + phase_shift, red, green, blue);
+    }
+    if (k.texture_active && inside) {  // textured interior starts from the interior colour
+        red = static_cast<float>(k.interior_color[0]) / 255.0F;
+        green = static_cast<float>(k.interior_color[1]) / 255.0F;
+        blue = static_cast<float>(k.interior_color[2]) / 255.0F;
+    }
+    if (k.texture_active) {  // CFraktalSFT::SetTexture, renderer.cpp:7190-7252
+        const double tdx = x > 0 ? l - c : (x + 1 < k.width ? c - r : 0.0);
+        const double tdy = y > 0 ? u - c : (y + 1 < k.height ? c - dn : 0.0);
+        auto offset = [&](double difference) {
+            const double powered = ::pow(1.0 + difference, k.texture_power);
+            const bool forward = powered > 1.0;
+            const double selected = forward ? powered : 1.0 / powered;
+            const double mapped = !__builtin_isnan(selected)
+                ? (::atan(selected) - kPi / 4.0) / (kPi / 4.0) * (k.texture_ratio / 100.0) : 0.0;
+            return __builtin_isfinite(mapped) ? (forward ? k.texture_power * mapped : -k.texture_power * mapped)
+                                              : 0.0;
+        };
+        int image_offset = 0;
+        if (__builtin_isfinite(k.texture_power))
+            image_offset = static_cast<int>(clampd(k.texture_power / 64.0, -2147483648.0, 2147483647.0));
+        const int sx = texture_index(static_cast<double>(x + image_offset) + offset(tdx), k.texture_width);
+        const int sy = texture_index(static_cast<double>(y + image_offset) + offset(tdy), k.texture_height);
+        const int row = k.texture_height - 1 - sy;
+        const uint8_t* t = k.texture + static_cast<size_t>(row) * static_cast<size_t>(k.texture_stride)
+            + static_cast<size_t>(sx) * 3U;
+        const double m = k.texture_merge;
+        red = static_cast<float>(static_cast<double>(red) * (1.0 - m) + m * static_cast<double>(t[0]) / 255.0);
+        green = static_cast<float>(static_cast<double>(green) * (1.0 - m) + m * static_cast<double>(t[1]) / 255.0);
+        blue = static_cast<float>(static_cast<double>(blue) * (1.0 - m) + m * static_cast<double>(t[2]) / 255.0);
     }
     if (k.needs_slopes) {
-        const double horizontal = x > 0 ? l - c : c - r;
-        const double vertical = y == 0 ? c - dn : u - c;
+        const double horizontal = analytic ? slope_x : (x > 0 ? l - c : c - r);
+        const double vertical = analytic ? slope_y : (y == 0 ? c - dn : u - c);
         const double projected = ((horizontal * k.cosine) + (vertical * k.sine)) * k.slope_power
             * static_cast<double>(k.width) / 640.0;
         const double strength = ::atan(__builtin_fabs(projected)) / (kPi / 2.0) * k.slope_ratio / 100.0;
@@ -290,6 +467,25 @@ FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
 // false when the pixel is excluded.  The value is an integer in
 // [floor(smooth_offset), max_iter - 2].
 FAV_HD bool transfer_bound_value(const KfpParams& k, size_t index, long long& value) {
+    if (k.has_planes) {  // kfp_transfer_bounds_planes, renderer.cpp:7336-7394
+        double v;
+        if (k.orbit_iteration) {
+            bool inside;
+            const double colour = orbit_sample(k, index, inside);
+            if (colour > static_cast<double>(k.max_iter)) return false;
+            v = ::floor(maxd(0.0, colour));
+        } else {
+            const int y = static_cast<int>(index / static_cast<size_t>(k.width));
+            const int x = static_cast<int>(index - static_cast<size_t>(y) * static_cast<size_t>(k.width));
+            if (k.iteration ? k.iteration[index] >= k.max_iter : inside_value(k.field[index], k.max_iter, k.field_bias))
+                return false;
+            const double colour = plane_colour_sample(k, x, y);  // kfp_plane_stored_iteration
+            v = colour > static_cast<double>(k.max_iter) ? static_cast<double>(k.max_iter) : ::floor(maxd(0.0, colour));
+        }
+        if (v >= static_cast<double>(k.max_iter - 1)) return false;
+        value = static_cast<long long>(v);
+        return true;
+    }
     const float raw = k.field[index];
     if (inside_value(raw, k.max_iter, k.field_bias)) return false;
     const double safe = __builtin_isfinite(raw)

@@ -105,5 +105,83 @@ inline KfpParams make_params(const FractalKfpOptions& o, const float* field, uin
     return k;
 }
 
+// kfp_selected_bailout_radius / _norm, renderer.cpp:5672-5716.
+inline double selected_bailout_radius(const FractalKfpOptions& o) {
+    double radius = 10000.0;
+    switch (o.bailout_radius_preset) {
+        case 0: radius = 10000.0; break;
+        case 1: radius = 2.0; break;
+        case 2:
+            if (o.power > 1.0 && std::isfinite(o.power)) radius = std::pow(2.0, 1.0 / (o.power - 1.0));
+            else return 0.0;
+            break;
+        case 3: radius = o.bailout_radius_custom; break;
+        default: return 0.0;
+    }
+    return std::isfinite(radius) && radius > 1.0 ? radius : 0.0;
+}
+inline double selected_bailout_norm(const FractalKfpOptions& o) {
+    switch (o.bailout_norm_preset) {
+        case 0: return 1.0;
+        case 1: return 2.0;
+        case 2: return HUGE_VAL;
+        case 3: return std::isfinite(o.bailout_norm_custom) && o.bailout_norm_custom > 0.0 ? o.bailout_norm_custom : 0.0;
+        default: return 0.0;
+    }
+}
+
+// valid_kfp_planes, renderer.cpp:7467-7491.
+inline bool valid_planes(const FractalKfpPlanes* p, int width, int height) {
+    if (!p) return true;
+    if (p->struct_size < sizeof(FractalKfpPlanes) || p->version != FRACTAL_KFP_PLANES_VERSION) return false;
+    if (!p->texture_rgb) return p->texture_width == 0 && p->texture_height == 0 && p->texture_stride == 0;
+    if (p->texture_width <= 0 || p->texture_height <= 0
+        || static_cast<int64_t>(p->texture_stride) < static_cast<int64_t>(p->texture_width) * 3)
+        return false;
+    const uint64_t bytes = static_cast<uint64_t>(p->texture_stride) * static_cast<uint64_t>(p->texture_height);
+    return bytes <= 100000000ULL * 3U && width > 0 && height > 0
+        && static_cast<uint64_t>(width) <= 100000000ULL / static_cast<uint64_t>(height);
+}
+
+// The plane-dependent launch constants: kfp_pixel_context
+// (renderer.cpp:5723-5763) and kfp_show_glitches (7456-7465).  Plane
+// pointers are left null: the caller binds the device copies.
+inline void set_plane_context(KfpParams& k, const FractalKfpOptions& o, const FractalKfpPlanes* planes) {
+    k.has_planes = planes != nullptr;
+    k.smooth_method = o.smooth_method;
+    k.power = o.power;
+    k.has_constant_bailout = planes != nullptr && planes->bailout == nullptr;
+    k.selected_bailout_radius = selected_bailout_radius(o);
+    k.smooth_bailout_radius = k.selected_bailout_radius;
+    if (std::isfinite(o.power) && o.power > 1.0) {
+        k.smooth_power_log = std::log(o.power);
+        k.has_smooth_power_log = std::isfinite(k.smooth_power_log) && k.smooth_power_log != 0.0;
+    }
+    if (k.has_constant_bailout && k.smooth_bailout_radius > 1.0 && std::isfinite(k.smooth_bailout_radius)) {
+        k.smooth_bailout_log = std::log(k.smooth_bailout_radius);
+        k.has_smooth_bailout_log = std::isfinite(k.smooth_bailout_log) && k.smooth_bailout_log != 0.0;
+    }
+    const double norm = selected_bailout_norm(o);
+    if (norm > 0.0 && std::isfinite(norm)) {
+        k.smooth_norm_exponent = 1.0 / norm;
+        k.has_smooth_norm_exponent = 1;
+    } else if (std::isinf(norm) && norm > 0.0) {
+        k.smooth_norm_exponent = 1.0;
+        k.has_smooth_norm_exponent = 1;
+    }
+    k.texture_active = planes != nullptr && planes->texture_rgb != nullptr && o.texture_enabled != 0;
+    const uint32_t offset = static_cast<uint32_t>(offsetof(FractalKfpOptions, show_glitches));
+    k.show_glitches = o.struct_size < offset + sizeof(o.show_glitches) ? 1 : (o.show_glitches != 0);
+    k.phase_color_strength = o.phase_color_strength;
+    k.texture_merge = o.texture_merge;
+    k.texture_power = o.texture_power;
+    k.texture_ratio = o.texture_ratio;
+    if (planes && k.texture_active) {
+        k.texture_width = planes->texture_width;
+        k.texture_height = planes->texture_height;
+        k.texture_stride = planes->texture_stride;
+    }
+}
+
 }  // namespace kfp
 }  // namespace fav

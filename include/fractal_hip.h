@@ -273,9 +273,10 @@ int fractal_colourise_kfp_opencl(
     double phase, double vocal, double instrumental, double pitch,
     const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
     int threads);
-/* renderer.h:483-498 / renderer.cpp:16729-16748.  planes must be NULL: the
- * orbit/DE/phase/texture plane transfer is not part of the HIP pass and a
- * non-NULL planes pointer fails with an error. */
+/* renderer.h:483-498 / renderer.cpp:16729-16748, 16314-16545: with
+ * Kalles planes (orbit or stored iteration, bailout, transition, phase, DE,
+ * |z|^2 tests, texture; each may be NULL) the colour follows the planes as
+ * the reference's SetColor does (renderer.cpp:5655-5976, 6696-7289). */
 int fractal_colourise_kfp_planes(
     const float *field, uint8_t *output, int width, int height, int max_iter,
     double phase, double vocal, double instrumental, double pitch,

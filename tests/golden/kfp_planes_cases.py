@@ -1,0 +1,44 @@
+"""Golden cases of the plane-aware KFP colour pass
+(fractal_colourise_kfp_planes with Kalles planes, renderer.cpp:16314-16545,
+6696-7289, 5655-5976).  The planes come from the field golden vectors'
+render planes (orbit iteration, phase, DE, |z|^2 tests); stored iteration /
+transition, bailout and texture planes are derived from them with a seed
+(tests/kfp_planes_io.py)."""
+from kfp_cases import KALLES_DEFAULT
+
+FULL = ("orbit_iteration", "phase", "de_x", "de_y", "test1", "test2")
+
+
+def _c(name, source, max_iter, planes, options, stops="kalles", seed=1, **extra):
+    return dict(name=name, source=source, max_iter=max_iter, planes=planes, options=options, stops=stops, seed=seed,
+                **extra)
+
+
+CASES = [
+    # Kalles default from the full orbit planes: smoothing from test1, stencil differences, slopes
+    _c("kfp_planes_default_c1", "c1_series", 20000, FULL, dict(KALLES_DEFAULT)),
+    _c("kfp_planes_default_bla", "bla_1e25", 50000, FULL, dict(KALLES_DEFAULT)),
+    # analytic DE differences with relief
+    _c("kfp_planes_analytic_de", "c1_series", 20000, FULL,
+       dict(color_method=5, differences=7, slopes=1, slope_power=40.0, slope_ratio=30.0, slope_angle=60.0,
+            iter_div=0.05, smooth=1)),
+    # phase colour strength on the plain transfer
+    _c("kfp_planes_phase", "bla_1e25", 50000, FULL,
+       dict(color_method=0, iter_div=2.0, phase_color_strength=40.0, smooth=1), stops="seeded"),
+    # the test2 smoothing (norm preset 1) and the method-4 range over the plane samples
+    _c("kfp_planes_test2_stretch", "c1_kalles", 20000, FULL,
+       dict(color_method=4, smooth_method=1, bailout_norm_preset=1, slopes=1, differences=2), stops="seeded"),
+    # stored iteration + transition planes (no orbit plane), hidden glitches
+    _c("kfp_planes_iteration_hidden_glitches", "c1_series", 20000, ("iteration",),
+       dict(color_method=1, differences=1, slopes=1, show_glitches=0), transition_glitches=40),
+    # per-pixel bailout plane with the test1 smoothing
+    _c("kfp_planes_bailout", "bla_1e27", 50000, FULL + ("bailout",),
+       dict(color_method=3, slopes=0, iter_div=0.3), stops="seeded", seed=2),
+    # texture merged into the colour (interior textured too)
+    _c("kfp_planes_texture", "c1_series", 20000, FULL,
+       dict(color_method=7, differences=3, texture_enabled=1, texture_merge=0.4, texture_power=8.0,
+            texture_ratio=30.0, iter_div=0.01, slopes=1, slope_power=50.0, slope_ratio=20.0), texture=True),
+    # a planes struct carrying only phase: the scalar field path plus phase, method-4 range via the plane path
+    _c("kfp_planes_phase_only", "bla_1e16_series", 30000, ("phase",),
+       dict(color_method=4, phase_color_strength=-25.0, differences=4, slopes=1), stops="seeded"),
+]

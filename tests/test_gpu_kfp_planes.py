@@ -1,0 +1,25 @@
+"""The HIP plane-aware KFP colour pass against the reference's RGB: every
+channel within one 8-bit level (device log / pow / atan may round
+differently from the host libm at a quantisation step), almost all exact."""
+import numpy as np
+import pytest
+
+import golden_io
+import kfp_planes_io
+import native
+from kfp_planes_cases import CASES
+from test_kfp_planes_cpu import load
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_hip_kfp_planes_match_reference(gpu_product, oracle, case):
+    print(f"[gpu] kfp planes {case['name']}", flush=True)
+    field, planes, lut, opts, max_iter = kfp_planes_io.inputs(case, oracle)
+    rgb = kfp_planes_io.colourise(native.bind_kfp(gpu_product), field, planes, lut, opts, max_iter)
+    expected = load(case["name"])
+    golden_io.assert_rgb_within_one_level(rgb, expected, case["name"])
+    exact = float(np.mean(rgb == expected))
+    print(f"[gpu] kfp planes {case['name']}: exact channel fraction {exact:.6f}", flush=True)
+    assert exact > 0.995

@@ -1,0 +1,75 @@
+"""The plane-aware KFP colour pass (kfp_core.h compiled for the CPU by the
+test harness) against the fixtures the reference's renderer.cpp wrote:
+identical RGB, since both use the host libm; plus the HIP library's
+argument checks for planes."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import host_check
+import kfp_planes_io
+import native
+from golden_io import GOLDEN
+from kfp_planes_cases import CASES
+
+
+def load(name):
+    with np.load(GOLDEN / f"{name}.npz", allow_pickle=False) as g:
+        return g["rgb"]
+
+
+@pytest.fixture(scope="module")
+def harness():
+    if not host_check.HOST_CHECK_SO.exists():
+        pytest.fail("libfav_host_check.so is not built: run __graft_entry__.build()")
+    return host_check.load()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_kernel_code_matches_reference(harness, oracle, case):
+    field, planes, lut, opts, max_iter = kfp_planes_io.inputs(case, oracle)
+    rgb = kfp_planes_io.colourise_host_check(harness, field, planes, lut, opts, max_iter)
+    np.testing.assert_array_equal(rgb, load(case["name"]))
+
+
+def test_planes_change_the_colours(harness, oracle):
+    """The orbit planes are read: the same options without planes colour
+    differently (the smoothing comes from test1, not from the field)."""
+    case = CASES[0]
+    field, planes, lut, opts, max_iter = kfp_planes_io.inputs(case, oracle)
+    plain = host_check.colourise(host_check.bind_kfp(harness), field, max_iter, opts, lut)
+    assert (plain != load(case["name"])).any()
+
+
+def test_hidden_glitches_are_black(oracle):
+    case = next(c for c in CASES if "transition_glitches" in c)
+    _, planes, _, _, _ = kfp_planes_io.inputs(case, oracle)
+    hidden = planes.arrays["transition"] < 0
+    rgb = load(case["name"])
+    assert hidden.sum() == case["transition_glitches"]
+    assert (rgb[hidden] == 0).all()
+
+
+@pytest.fixture(scope="module")
+def hip(product):
+    return native.bind_kfp(product)
+
+
+def test_hip_rejects_bad_planes(hip, oracle):
+    field, planes, lut, opts, max_iter = kfp_planes_io.inputs(CASES[0], oracle)
+    planes.struct.version = 2
+    with pytest.raises(RuntimeError, match="invalid native KFP colour"):
+        kfp_planes_io.colourise(hip, field, planes, lut, opts, max_iter)
+    planes.struct.version = 1
+    planes.struct.texture_width = 5  # a texture size without texture bytes
+    with pytest.raises(RuntimeError, match="invalid native KFP colour"):
+        kfp_planes_io.colourise(hip, field, planes, lut, opts, max_iter)
+
+
+def test_hip_planes_without_device_fail_loudly(hip, oracle):
+    if hip.fractal_backend_capabilities() & 8:
+        pytest.skip("a device is present")
+    field, planes, lut, opts, max_iter = kfp_planes_io.inputs(CASES[0], oracle)
+    with pytest.raises(RuntimeError):
+        kfp_planes_io.colourise(hip, field, planes, lut, opts, max_iter)
