@@ -191,11 +191,12 @@ def test_kfp_rejects_field_bias_above_cap(product):
     assert b"field bias exceeds" in product.fractal_last_error()
 
 
-def test_kfp_planes_pointer_rejected(product):
-    """The HIP pass colours the scalar field only: a planes pointer is an error."""
+def test_kfp_malformed_planes_rejected(product):
+    """A planes struct without the FractalKfpPlanes size and version is
+    rejected before any device work (valid_kfp_planes, renderer.cpp:7467-7491)."""
     planes = ctypes.create_string_buffer(256)
     assert _kfp_call(product, native.KfpOptions.make(), planes=ctypes.cast(planes, ctypes.c_void_p)) == 1
-    assert b"scalar field only" in product.fractal_last_error()
+    assert b"invalid native KFP colour dimensions or controls" in product.fractal_last_error()
 
 
 def test_package_kfp_options_mirror_header_layout():
