@@ -344,6 +344,104 @@ int render_points_impl(float* output, int point_count, const char* zoom, const d
     }
 }
 
+// fractal_atlas_colourise_kfp_planes, renderer.cpp:17323-17801: the
+// reference's checks; tile selection, geometry and maps on the host; the
+// compose of the field and planes and the KFP pass on the device.
+int atlas_kfp_planes(const float* parent, int parent_width, int parent_height, int parent_max_iter,
+                     const FractalKfpPlanes* parent_planes, const float* child, int child_width, int child_height,
+                     int child_max_iter, const FractalKfpPlanes* child_planes, uint8_t* output, int output_width,
+                     int output_height, double parent_zoom, double child_fraction, double child_zoom, int max_iter,
+                     int centered_input, double phase, double vocal, double instrumental, double pitch,
+                     const FractalKfpOptions* options, const uint8_t* lut, int lut_size, int threads) {
+    try {
+        if (!parent || !output || !parent_planes || !valid_dimensions(parent_width, parent_height)
+            || !valid_dimensions(output_width, output_height) || !valid_iterations(parent_max_iter)
+            || !valid_iterations(max_iter) || !valid_threads(threads) || centered_input < 0 || centered_input > 1
+            || !std::isfinite(parent_zoom) || parent_zoom <= 0.0 || !std::isfinite(child_fraction)
+            || child_fraction < 0.0 || child_fraction > 1.0 || !std::isfinite(child_zoom) || child_zoom <= 0.0
+            || !std::isfinite(phase) || !std::isfinite(vocal) || !std::isfinite(instrumental) || !std::isfinite(pitch)
+            || !lut || !fav::kfp::valid_options(options, lut_size)
+            || !fav::kfp::valid_planes(parent_planes, parent_width, parent_height))
+            throw std::runtime_error("invalid native plane-aware KFP atlas dimensions or controls");
+        const bool use_child = child != nullptr && child_fraction > 0.0;
+        if (use_child) {
+            if (!valid_dimensions(child_width, child_height) || !valid_iterations(child_max_iter) || !child_planes
+                || !fav::kfp::valid_planes(child_planes, child_width, child_height))
+                throw std::runtime_error("invalid native plane-aware KFP atlas child tile");
+        } else if (child != nullptr || child_planes != nullptr || child_width != 0 || child_height != 0
+                   || child_max_iter != 0) {
+            throw std::runtime_error("invalid native plane-aware KFP atlas child absence");
+        }
+        if (options->field_bias > static_cast<double>(max_iter))
+            throw std::runtime_error("native KFP field bias exceeds iteration cap");
+        parent_zoom = std::max(parent_zoom, 1.0);
+        child_zoom = std::max(child_zoom, 1.0);
+        const bool full_child = use_child && child_fraction >= 0.999999;
+        fav::PlaneTile tiles[2];
+        tiles[0] = fav::PlaneTile{full_child ? child : parent, full_child ? child_width : parent_width,
+                                  full_child ? child_height : parent_height, full_child ? child_planes : parent_planes};
+        if (use_child && !full_child) tiles[1] = fav::PlaneTile{child, child_width, child_height, child_planes};
+        const int primary_max_iter = full_child ? child_max_iter : parent_max_iter;
+        const int secondary_max_iter = use_child && !full_child ? child_max_iter : 0;
+        const int target_max_iter = full_child ? child_max_iter : max_iter;
+        fav::atlas::PlanesCompose a{};
+        a.primary.width = tiles[0].width;
+        a.primary.max_iter = primary_max_iter;
+        a.primary.bias = centered_input ? static_cast<double>(primary_max_iter) : 0.0;
+        a.secondary.width = tiles[1].width;
+        a.secondary.max_iter = secondary_max_iter;
+        a.secondary.bias = centered_input ? static_cast<double>(secondary_max_iter) : 0.0;
+        a.use_secondary = tiles[1].field != nullptr;
+        a.visible_width = a.use_secondary
+            ? std::max(1, static_cast<int>(std::lround(static_cast<double>(output_width) * child_fraction))) : 0;
+        a.visible_height = a.use_secondary
+            ? std::max(1, static_cast<int>(std::lround(static_cast<double>(output_height) * child_fraction))) : 0;
+        a.visible_left = a.use_secondary ? (output_width - a.visible_width) / 2 : 0;
+        a.visible_top = a.use_secondary ? (output_height - a.visible_height) / 2 : 0;
+        a.feather = a.use_secondary ? fav::atlas::seam_feather(a.visible_width, a.visible_height) : 0;
+        a.width = output_width;
+        a.height = output_height;
+        a.target_max_iter = target_max_iter;
+        a.field_bias = options->field_bias;
+        fav::atlas::Axis axes[4];
+        const double primary_zoom = full_child ? child_zoom : parent_zoom;
+        fav::atlas::fill_axis(axes[0], tiles[0].width, output_width, primary_zoom);
+        fav::atlas::fill_axis(axes[1], tiles[0].height, output_height, primary_zoom);
+        if (a.use_secondary) {
+            fav::atlas::fill_axis(axes[2], child_width, a.visible_width, child_zoom);
+            fav::atlas::fill_axis(axes[3], child_height, a.visible_height, child_zoom);
+        }
+        // the composed planes: present where either tile has them; the
+        // texture of the primary tile, else the secondary's (17513-17522)
+        const FractalKfpPlanes* primary_planes = tiles[0].planes;
+        const FractalKfpPlanes* secondary_planes = tiles[1].planes;
+        const FractalKfpPlanes* texture = primary_planes;
+        if (!texture->texture_rgb && secondary_planes && secondary_planes->texture_rgb) texture = secondary_planes;
+        FractalKfpPlanes shape{};
+        shape.struct_size = sizeof(FractalKfpPlanes);
+        shape.version = FRACTAL_KFP_PLANES_VERSION;
+        static const double present = 0.0;
+        const bool has_bailout = primary_planes->bailout || (secondary_planes && secondary_planes->bailout);
+        shape.bailout = has_bailout ? &present : nullptr;
+        shape.texture_rgb = texture->texture_rgb;
+        shape.texture_width = texture->texture_width;
+        shape.texture_height = texture->texture_height;
+        shape.texture_stride = texture->texture_stride;
+        fav::kfp::KfpParams k = fav::kfp::make_params(*options, nullptr, output, output_width, output_height,
+                                                      target_max_iter, lut, lut_size);
+        fav::kfp::set_plane_context(k, *options, &shape);
+        fav::atlas_colourise_kfp_planes(a, tiles, axes, k, texture, output);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native plane-aware KFP atlas colourizer failed with an unknown exception");
+        return 1;
+    }
+}
+
 int one_shot(float* output, int width, int height, const char* zoom, const char* x, const char* y, int max_iter,
              int bits, int use_perturbation, int threads, int formula, double julia_re, double julia_im,
              const FractalRenderOptions* supplied, FractalRenderPlanes* planes) {
@@ -568,6 +666,35 @@ int fractal_atlas_colourise_kfp_raw_precise(const float* parent, int parent_widt
     return atlas_kfp_raw(parent, parent_width, parent_height, parent_max_iter, child, child_width, child_height,
                          child_max_iter, output, output_width, output_height, parent_zoom, child_fraction,
                          child_zoom, max_iter, phase, vocal, instrumental, pitch, options, lut, lut_size, threads);
+}
+
+int fractal_atlas_colourise_kfp_planes(const float* parent, int parent_width, int parent_height, int parent_max_iter,
+                                       const FractalKfpPlanes* parent_planes, const float* child, int child_width,
+                                       int child_height, int child_max_iter, const FractalKfpPlanes* child_planes,
+                                       uint8_t* output, int output_width, int output_height, double parent_zoom,
+                                       double child_fraction, double child_zoom, int max_iter, int centered_input,
+                                       double phase, double vocal, double instrumental, double pitch,
+                                       const FractalKfpOptions* options, const uint8_t* lut, int lut_size,
+                                       int threads) {
+    return atlas_kfp_planes(parent, parent_width, parent_height, parent_max_iter, parent_planes, child, child_width,
+                            child_height, child_max_iter, child_planes, output, output_width, output_height,
+                            parent_zoom, child_fraction, child_zoom, max_iter, centered_input, phase, vocal,
+                            instrumental, pitch, options, lut, lut_size, threads);
+}
+
+// renderer.cpp:18329-18364: the same pass (the KFP transfer here is always
+// the precise one).
+int fractal_atlas_colourise_kfp_planes_precise(
+    const float* parent, int parent_width, int parent_height, int parent_max_iter,
+    const FractalKfpPlanes* parent_planes, const float* child, int child_width, int child_height, int child_max_iter,
+    const FractalKfpPlanes* child_planes, uint8_t* output, int output_width, int output_height, double parent_zoom,
+    double child_fraction, double child_zoom, int max_iter, int centered_input, double phase, double vocal,
+    double instrumental, double pitch, const FractalKfpOptions* options, const uint8_t* lut, int lut_size,
+    int threads) {
+    return atlas_kfp_planes(parent, parent_width, parent_height, parent_max_iter, parent_planes, child, child_width,
+                            child_height, child_max_iter, child_planes, output, output_width, output_height,
+                            parent_zoom, child_fraction, child_zoom, max_iter, centered_input, phase, vocal,
+                            instrumental, pitch, options, lut, lut_size, threads);
 }
 
 int fractal_hip_colourise_kfp_device(const float* device_field, uint8_t* device_rgb, int width, int height,

@@ -164,5 +164,131 @@ FAV_HD float compose_pixel(const ComposeParams& a, int x, int y) {
     return value;
 }
 
+// ---- the plane-aware compose (fractal_atlas_colourise_kfp_planes,
+// renderer.cpp:17323-17793): the scalar field and every Kalles plane that
+// either tile carries, reprojected into the frame in one pass per pixel.
+
+// One source tile of the plane-aware compose (device pointers; absent
+// planes are null).
+struct PlaneSource {
+    const float* field;
+    int32_t width, max_iter;
+    double bias;  // centred storage: the tile's own cap
+    AxisDev x, y;
+    const int64_t* ints[2];      // orbit_iteration, iteration
+    const double* doubles[7];    // bailout, transition, phase, de_x, de_y, test1, test2
+};
+
+struct PlanesCompose {
+    PlaneSource primary, secondary;
+    int32_t use_secondary, visible_left, visible_top, visible_width, visible_height, feather;
+    int32_t width, height, target_max_iter;
+    double field_bias;  // the KFP options' bias of the composed field
+    float* out_field;
+    int64_t* out_ints[2];
+    double* out_doubles[7];
+};
+
+// (preserve_negative, reset_inside) of each double plane (renderer.cpp:17626-17651).
+FAV_HD bool plane_preserves_negative(int k) { return k == 1; }
+FAV_HD bool plane_resets_inside(int k) { return k != 0; }
+
+// sample_nearest_mapped_*, renderer.cpp:5105-5139.
+template <typename T>
+FAV_HD T sample_nearest(const T* source, int32_t width, const AxisDev& ax, const AxisDev& ay, int x, int y) {
+    const int xi = ax.weight[x] < 0.5F ? ax.index0[x] : ax.index1[x];
+    const int yi = ay.weight[y] < 0.5F ? ay.index0[y] : ay.index1[y];
+    return source[static_cast<size_t>(yi) * static_cast<size_t>(width) + static_cast<size_t>(xi)];
+}
+
+// sample_bilinear_mapped_double, renderer.cpp:5076-5103, then the
+// negative-nearest override of a sign-carrying plane (17588-17597).
+FAV_HD double sample_plane(const double* source, int32_t width, const AxisDev& ax, const AxisDev& ay, int x, int y,
+                           bool preserve_negative) {
+    const int x0 = ax.index0[x], x1 = ax.index1[x];
+    const double wx = static_cast<double>(ax.weight[x]);
+    const int y0 = ay.index0[y], y1 = ay.index1[y];
+    const double wy = static_cast<double>(ay.weight[y]);
+    const double* top = source + static_cast<size_t>(y0) * static_cast<size_t>(width);
+    const double* bottom = source + static_cast<size_t>(y1) * static_cast<size_t>(width);
+    auto ordinary = [preserve_negative](double v) { return preserve_negative && v < 0.0 ? 0.0 : v; };
+    const double t = ordinary(top[x0]) * (1.0 - wx) + ordinary(top[x1]) * wx;
+    const double b = ordinary(bottom[x0]) * (1.0 - wx) + ordinary(bottom[x1]) * wx;
+    double value = t * (1.0 - wy) + b * wy;
+    if (preserve_negative) {
+        const double nearest = sample_nearest(source, width, ax, ay, x, y);
+        if (nearest < 0.0) value = nearest;
+    }
+    return value;
+}
+
+FAV_HD void compose_planes_pixel(const PlanesCompose& a, int x, int y) {
+    const size_t i = static_cast<size_t>(y) * static_cast<size_t>(a.width) + static_cast<size_t>(x);
+    const float interior = encode(static_cast<double>(a.target_max_iter), a.field_bias);
+    const PlaneSource& p = a.primary;
+    bool inside = false;
+    const float smooth = sample(p.field, p.width, p.x, p.y, x, y, p.max_iter, inside, p.bias);
+    float value = inside ? interior : encode(static_cast<double>(smooth), a.field_bias);
+    int64_t ints[2];
+    double doubles[7];
+    for (int k = 0; k < 2; ++k) {
+        ints[k] = 0;
+        if (p.ints[k])
+            ints[k] = inside ? static_cast<int64_t>(a.target_max_iter) : sample_nearest(p.ints[k], p.width, p.x, p.y, x, y);
+    }
+    for (int k = 0; k < 7; ++k) {
+        doubles[k] = 0.0;
+        if (p.doubles[k]) {
+            const double v = sample_plane(p.doubles[k], p.width, p.x, p.y, x, y, plane_preserves_negative(k));
+            doubles[k] = inside && plane_resets_inside(k) ? 0.0 : v;
+        }
+    }
+    if (a.use_secondary && x >= a.visible_left && x < a.visible_left + a.visible_width && y >= a.visible_top
+        && y < a.visible_top + a.visible_height) {
+        const PlaneSource& c = a.secondary;
+        const int cx = x - a.visible_left, cy = y - a.visible_top;
+        bool child_inside = false;
+        const float cs = sample(c.field, c.width, c.x, c.y, cx, cy, c.max_iter, child_inside, c.bias);
+        const float child_value = child_inside ? interior : encode(static_cast<double>(cs), a.field_bias);
+        float alpha = 1.0F;
+        if (a.feather >= 2) {  // renderer.cpp:17675-17689
+            int edge = cx < a.visible_width - 1 - cx ? cx : a.visible_width - 1 - cx;
+            const int edge_y = cy < a.visible_height - 1 - cy ? cy : a.visible_height - 1 - cy;
+            edge = edge < edge_y ? edge : edge_y;
+            float linear = static_cast<float>(edge) / static_cast<float>(a.feather);
+            linear = linear < 0.0F ? 0.0F : (linear > 1.0F ? 1.0F : linear);
+            alpha = linear * linear * (3.0F - 2.0F * linear);
+        }
+        value = value * (1.0F - alpha) + child_value * alpha;
+        for (int k = 0; k < 2; ++k)
+            if (c.ints[k] && a.out_ints[k])
+                ints[k] = child_inside ? static_cast<int64_t>(a.target_max_iter)
+                                       : sample_nearest(c.ints[k], c.width, c.x, c.y, cx, cy);
+        for (int k = 0; k < 7; ++k) {
+            if (!c.doubles[k] || !a.out_doubles[k]) continue;
+            const bool negative = plane_preserves_negative(k);
+            double v = sample_plane(c.doubles[k], c.width, c.x, c.y, cx, cy, negative);
+            if (child_inside && plane_resets_inside(k)) v = 0.0;
+            if (!p.doubles[k]) {
+                doubles[k] = v;
+                continue;
+            }
+            const double parent_value = doubles[k];
+            const double blended = parent_value * (1.0 - static_cast<double>(alpha)) + v * static_cast<double>(alpha);
+            if (negative) {
+                const double selected = alpha >= 0.5F ? v : parent_value;
+                doubles[k] = selected < 0.0 ? selected : blended;
+            } else {
+                doubles[k] = blended;
+            }
+        }
+    }
+    a.out_field[i] = value;
+    for (int k = 0; k < 2; ++k)
+        if (a.out_ints[k]) a.out_ints[k][i] = ints[k];
+    for (int k = 0; k < 7; ++k)
+        if (a.out_doubles[k]) a.out_doubles[k][i] = doubles[k];
+}
+
 }  // namespace atlas
 }  // namespace fav

@@ -33,6 +33,7 @@ hipError_t launch_fx_kernel(const RenderParams& p, const fx::FxOut& out, hipStre
 hipError_t launch_batch(const BatchLaunch& q, int fast_blocks, int persistent_blocks, hipStream_t stream);
 hipError_t launch_atlas_compose(const atlas::ComposeParams& a, hipStream_t stream);
 hipError_t launch_aurora(const aurora::Params& p, hipStream_t stream);
+hipError_t launch_atlas_planes(const atlas::PlanesCompose& a, hipStream_t stream);
 
 namespace {
 
@@ -396,6 +397,102 @@ void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers, c
     if (!device_pointers)
         check(hipMemcpyAsync(host.output, k.output, pixels * 3, hipMemcpyDeviceToHost, s.stream), "D2H rgb");
     check(hipStreamSynchronize(s.stream), "KFP colour");
+    float kernel_ms = 0.0f;
+    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    s.timing.kernel_ms = kernel_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
+}
+
+void atlas_colourise_kfp_planes(const atlas::PlanesCompose& host, const PlaneTile* tiles, const atlas::Axis* axes,
+                                const kfp::KfpParams& colour, const FractalKfpPlanes* texture, uint8_t* output) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const size_t pixels = static_cast<size_t>(host.width) * static_cast<size_t>(host.height);
+    atlas::PlanesCompose a = host;
+    auto align = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+    // inputs: per tile the field and its present planes; then the four axes
+    size_t total = 0;
+    for (int t = 0; t < 2; ++t) {
+        if (!tiles[t].field) continue;
+        const size_t n = static_cast<size_t>(tiles[t].width) * static_cast<size_t>(tiles[t].height);
+        total += align(n * 4) + 9 * align(n * 8);
+    }
+    for (int i = 0; i < 4; ++i) total += 3 * align(axes[i].weight.size() * 4);
+    char* in = static_cast<char*>(s.atlas_tiles.get(total + 256));
+    size_t at = 0;
+    auto put = [&](const void* src, size_t bytes) -> const void* {
+        if (!src || !bytes) return nullptr;
+        check(hipMemcpyAsync(in + at, src, bytes, hipMemcpyHostToDevice, s.stream), "H2D atlas planes input");
+        const void* d = in + at;
+        at += align(bytes);
+        return d;
+    };
+    atlas::PlaneSource* sources[2] = {&a.primary, &a.secondary};
+    for (int t = 0; t < 2; ++t) {
+        atlas::PlaneSource& src = *sources[t];
+        src.field = nullptr;
+        for (auto& q : src.ints) q = nullptr;
+        for (auto& q : src.doubles) q = nullptr;
+        if (!tiles[t].field) continue;
+        const size_t n = static_cast<size_t>(tiles[t].width) * static_cast<size_t>(tiles[t].height);
+        src.field = static_cast<const float*>(put(tiles[t].field, n * 4));
+        const FractalKfpPlanes* p = tiles[t].planes;
+        const int64_t* ints[2] = {p->orbit_iteration, p->iteration};
+        const double* doubles[7] = {p->bailout, p->transition, p->phase, p->de_x, p->de_y, p->test1, p->test2};
+        for (int k = 0; k < 2; ++k) src.ints[k] = static_cast<const int64_t*>(put(ints[k], ints[k] ? n * 8 : 0));
+        for (int k = 0; k < 7; ++k)
+            src.doubles[k] = static_cast<const double*>(put(doubles[k], doubles[k] ? n * 8 : 0));
+    }
+    atlas::AxisDev* views[4] = {&a.primary.x, &a.primary.y, &a.secondary.x, &a.secondary.y};
+    for (int i = 0; i < 4; ++i) {
+        const size_t n = axes[i].weight.size();
+        *views[i] = atlas::AxisDev{static_cast<const int32_t*>(put(axes[i].index0.data(), n * 4)),
+                                   static_cast<const int32_t*>(put(axes[i].index1.data(), n * 4)),
+                                   static_cast<const float*>(put(axes[i].weight.data(), n * 4))};
+    }
+    // outputs: the field and each plane that either tile carries
+    char* out = static_cast<char*>(s.atlas_field.get(align(pixels * 4) + 9 * align(pixels * 8) + 256));
+    a.out_field = reinterpret_cast<float*>(out);
+    for (int k = 0; k < 2; ++k)
+        a.out_ints[k] = (a.primary.ints[k] || a.secondary.ints[k])
+            ? reinterpret_cast<int64_t*>(out + align(pixels * 4) + static_cast<size_t>(k) * align(pixels * 8)) : nullptr;
+    for (int k = 0; k < 7; ++k)
+        a.out_doubles[k] = (a.primary.doubles[k] || a.secondary.doubles[k])
+            ? reinterpret_cast<double*>(out + align(pixels * 4) + static_cast<size_t>(k + 2) * align(pixels * 8))
+            : nullptr;
+    kfp::KfpParams k = colour;
+    k.field = a.out_field;
+    k.orbit_iteration = a.out_ints[0];
+    k.iteration = a.out_ints[1];
+    k.bailout = a.out_doubles[0];
+    k.transition = a.out_doubles[1];
+    k.phase = a.out_doubles[2];
+    k.de_x = a.out_doubles[3];
+    k.de_y = a.out_doubles[4];
+    k.test1 = a.out_doubles[5];
+    k.test2 = a.out_doubles[6];
+    k.texture = nullptr;
+    if (k.texture_active && texture) {
+        const size_t bytes = static_cast<size_t>(texture->texture_stride) * static_cast<size_t>(texture->texture_height);
+        void* t = s.kfp_planes.get(bytes + 256);
+        check(hipMemcpyAsync(t, texture->texture_rgb, bytes, hipMemcpyHostToDevice, s.stream), "H2D texture");
+        k.texture = static_cast<const uint8_t*>(t);
+    }
+    k.output = static_cast<uint8_t*>(s.kfp_rgb.get(pixels * 3));
+    void* l = s.kfp_lut.get(static_cast<size_t>(colour.lut_size) * 3);
+    check(hipMemcpyAsync(l, colour.lut, static_cast<size_t>(colour.lut_size) * 3, hipMemcpyHostToDevice, s.stream),
+          "H2D lut");
+    k.lut = static_cast<const uint8_t*>(l);
+    void* scratch = s.kfp_scratch.get(64);
+    const uint64_t wall0 = now_ns();
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_atlas_planes(a, s.stream), "atlas planes compose launch");
+    check(launch_kfp_kernel(k, scratch, s.compute_units, s.stream), "KFP colour launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    check(hipMemcpyAsync(output, k.output, pixels * 3, hipMemcpyDeviceToHost, s.stream), "D2H rgb");
+    check(hipStreamSynchronize(s.stream), "atlas planes colour");
     float kernel_ms = 0.0f;
     check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
     s.timing.kernel_ms = kernel_ms;

@@ -106,6 +106,21 @@ void atlas_colourise_kfp_raw(const atlas::ComposeParams& compose, size_t parent_
                              const atlas::Axis* axes, const kfp::KfpParams& colour, uint8_t* output);
 
 
+// Plane-aware KFP atlas colour pass (atlas_core.h compose_planes_pixel,
+// then the KFP colour kernel).  `compose` carries the geometry (its
+// pointers are filled here); tiles[0] is the primary tile, tiles[1] the
+// secondary one (field null when absent); axes are primary x / y and
+// secondary x / y; `colour` the KFP constants for the composed planes;
+// `texture` the texture image the composed planes carry (or null).
+namespace atlas { struct PlanesCompose; }
+struct PlaneTile {
+    const float* field = nullptr;
+    int width = 0, height = 0;
+    const FractalKfpPlanes* planes = nullptr;
+};
+void atlas_colourise_kfp_planes(const atlas::PlanesCompose& compose, const PlaneTile* tiles, const atlas::Axis* axes,
+                                const kfp::KfpParams& colour, const FractalKfpPlanes* texture, uint8_t* output);
+
 // Ordinary-palette colouriser and crop / atlas compositors (aurora_core.h).
 // `params` carries the mode, geometry and palette controls; the sources,
 // axes (parent x / y, child x / y; empty when unused), seam weights and

@@ -313,6 +313,33 @@ int fractal_atlas_colourise_kfp_raw_precise(
     const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
     int threads);
 
+/* renderer.h:604-631 / renderer.cpp:17323-17801: the parent and centred
+ * child scalar tiles and their Kalles planes reprojected into one frame
+ * (field interior-preserving, integer planes nearest, double planes
+ * bilinear with the sign-carrying transition kept), then one KFP pass over
+ * the composed planes; compose and colour run on the device. */
+int fractal_atlas_colourise_kfp_planes(
+    const float *parent, int parent_width, int parent_height,
+    int parent_max_iter, const FractalKfpPlanes *parent_planes,
+    const float *child, int child_width, int child_height, int child_max_iter,
+    const FractalKfpPlanes *child_planes, uint8_t *output, int output_width,
+    int output_height, double parent_zoom, double child_fraction,
+    double child_zoom, int max_iter, int centered_input, double phase,
+    double vocal, double instrumental, double pitch,
+    const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
+    int threads);
+/* renderer.h:632-661 / renderer.cpp:18329-18364: the same pass. */
+int fractal_atlas_colourise_kfp_planes_precise(
+    const float *parent, int parent_width, int parent_height,
+    int parent_max_iter, const FractalKfpPlanes *parent_planes,
+    const float *child, int child_width, int child_height, int child_max_iter,
+    const FractalKfpPlanes *child_planes, uint8_t *output, int output_width,
+    int output_height, double parent_zoom, double child_fraction,
+    double child_zoom, int max_iter, int centered_input, double phase,
+    double vocal, double instrumental, double pitch,
+    const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
+    int threads);
+
 /* ---- Ordinary (Aurora) palette and per-frame compositors --------------
  * The palette (renderer.cpp:4782-4923) is built on the host; every pixel is
  * computed on the device.  Byte and float results equal the reference's

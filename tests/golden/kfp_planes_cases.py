@@ -42,3 +42,32 @@ CASES = [
     _c("kfp_planes_phase_only", "bla_1e16_series", 30000, ("phase",),
        dict(color_method=4, phase_color_strength=-25.0, differences=4, slopes=1), stops="seeded"),
 ]
+
+
+def _a(name, parent, child, ow, oh, parent_zoom, fraction, child_zoom, max_iter, options, stops="kalles", seed=1,
+       **extra):
+    return dict(name=name, parent=parent, child=child, ow=ow, oh=oh, parent_zoom=parent_zoom, fraction=fraction,
+                child_zoom=child_zoom, max_iter=max_iter, options=options, stops=stops, seed=seed, **extra)
+
+
+# fractal_atlas_colourise_kfp_planes (renderer.cpp:17323-17801): tiles are
+# field golden vectors with their render planes
+ATLAS_CASES = [
+    _a("kfp_atlas_planes_child_third", "c1_series", "c1_noseries", 120, 68, 1.5, 0.33, 1.2, 20000,
+       dict(KALLES_DEFAULT)),
+    _a("kfp_atlas_planes_centred", "c1_series", "c1_noseries", 120, 68, 1.3, 0.5, 1.1, 20000,
+       dict(KALLES_DEFAULT, field_bias=20000.0), centred=1),
+    _a("kfp_atlas_planes_full_child", "c1_series", "c1_kalles", 100, 56, 1.2, 1.0, 1.4, 20000,
+       dict(color_method=4, smooth_method=1, bailout_norm_preset=1, slopes=1, differences=2), stops="seeded"),
+    _a("kfp_atlas_planes_no_child", "bla_1e25", None, 80, 45, 1.6, 0.0, 1.0, 50000,
+       dict(color_method=5, differences=7, slopes=1, iter_div=0.05)),
+    # the child lacks phase, carries hidden glitches; the parent carries phase
+    _a("kfp_atlas_planes_mixed", "c1_series", "c1_noseries", 128, 72, 1.4, 0.45, 1.3, 20000,
+       dict(color_method=0, iter_div=2.0, phase_color_strength=40.0, show_glitches=0, slopes=1, differences=3),
+       stops="seeded", child_planes=("orbit_iteration", "de_x", "de_y", "test1", "test2"),
+       child_transition_glitches=30),
+    # a texture on the child tile only
+    _a("kfp_atlas_planes_child_texture", "c1_series", "c1_noseries", 96, 54, 1.2, 0.6, 1.0, 20000,
+       dict(KALLES_DEFAULT, texture_enabled=1, texture_merge=0.5, texture_power=4.0, texture_ratio=50.0),
+       child_texture=True, parent_planes=("orbit_iteration", "test1", "test2")),
+]

@@ -110,3 +110,76 @@ def colourise_host_check(lib, field, planes, lut, opts, max_iter):
     if status != 0:
         raise RuntimeError(err.value.decode())
     return out
+
+
+ATLAS_SIGNATURE = None
+
+
+def render_tile(source, planes=("orbit_iteration", "phase", "de_x", "de_y", "test1", "test2"), centred=False,
+                cap=None, extra=None):
+    """(field, KfpPlanes) of a field golden vector used as an atlas tile."""
+    with np.load(GOLDEN / f"{source}.npz", allow_pickle=False) as g:
+        field = g["field"].astype(np.float32)
+        arrays = {k: g[k].reshape(field.shape) for k in planes}
+        cap = cap or json_max_iter(g)
+    if centred:
+        field = (field.astype(np.float64) - cap).astype(np.float32)
+    arrays.update(extra or {})
+    return np.ascontiguousarray(field), arrays, cap
+
+
+def json_max_iter(g):
+    import json
+    return int(json.loads(str(g["meta"]))["max_iter"])
+
+
+def atlas_inputs(case, oracle):
+    """Parent / child tiles with their planes, LUT and options of an atlas case."""
+    rng = np.random.default_rng(case["seed"])
+    centred = bool(case.get("centred", 0))
+    pf, pa, pcap = render_tile(case["parent"], case.get("parent_planes", FULL_PLANES), centred)
+    parent = (pf, KfpPlanes(pa, texture_image() if case.get("parent_texture") else None), pcap)
+    child = None
+    if case.get("child"):
+        extra = {}
+        cf0, _, _ = render_tile(case["child"], (), False)
+        if case.get("child_transition_glitches"):
+            t = np.ones(cf0.shape)
+            t.reshape(-1)[rng.choice(t.size, size=case["child_transition_glitches"], replace=False)] = -1.0
+            extra["transition"] = t
+        cf, ca, ccap = render_tile(case["child"], case.get("child_planes", FULL_PLANES), centred, extra=extra)
+        child = (cf, KfpPlanes(ca, texture_image(seed=9) if case.get("child_texture") else None), ccap)
+    stops = np.asarray(KALLES_DEFAULT_STOPS if case["stops"] == "kalles" else seeded_stops(), dtype=np.uint8)
+    lut = native.palette_lut_oracle(oracle, stops, 1024)
+    opts = native.KfpOptions.make(**case["options"])
+    return parent, child, lut, opts
+
+
+FULL_PLANES = ("orbit_iteration", "phase", "de_x", "de_y", "test1", "test2")
+
+
+def atlas_colourise(lib, case, parent, child, lut, opts, entry="fractal_atlas_colourise_kfp_planes"):
+    fn = getattr(lib, entry)
+    fn.argtypes = [native.c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, native.c_float_p,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, native.c_uint8_p, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                   ctypes.POINTER(native.KfpOptions), native.c_uint8_p, ctypes.c_int, ctypes.c_int]
+    fn.restype = ctypes.c_int
+    ow, oh = case["ow"], case["oh"]
+    out = np.zeros((oh, ow, 3), dtype=np.uint8)
+    pf, pp, pcap = parent
+    if child is not None:
+        cf, cp, ccap = child
+        child_args = (cf.ctypes.data_as(native.c_float_p), cf.shape[1], cf.shape[0], ccap,
+                      ctypes.cast(ctypes.byref(cp.struct), ctypes.c_void_p))
+    else:
+        child_args = (None, 0, 0, 0, None)
+    status = fn(pf.ctypes.data_as(native.c_float_p), pf.shape[1], pf.shape[0], pcap,
+                ctypes.cast(ctypes.byref(pp.struct), ctypes.c_void_p), *child_args,
+                out.ctypes.data_as(native.c_uint8_p), ow, oh, case["parent_zoom"], case["fraction"], case["child_zoom"],
+                case["max_iter"], int(case.get("centred", 0)), 0.0, 0.0, 0.0, 0.0, ctypes.byref(opts),
+                lut.ctypes.data_as(native.c_uint8_p), lut.shape[0], 0)
+    if status != 0:
+        raise RuntimeError(lib.fractal_last_error().decode())
+    return out

@@ -7,7 +7,7 @@ import pytest
 import golden_io
 import kfp_planes_io
 import native
-from kfp_planes_cases import CASES
+from kfp_planes_cases import ATLAS_CASES, CASES
 from test_kfp_planes_cpu import load
 
 pytestmark = pytest.mark.gpu
@@ -22,4 +22,18 @@ def test_hip_kfp_planes_match_reference(gpu_product, oracle, case):
     golden_io.assert_rgb_within_one_level(rgb, expected, case["name"])
     exact = float(np.mean(rgb == expected))
     print(f"[gpu] kfp planes {case['name']}: exact channel fraction {exact:.6f}", flush=True)
+    assert exact > 0.995
+
+
+@pytest.mark.parametrize("case", ATLAS_CASES, ids=[c["name"] for c in ATLAS_CASES])
+@pytest.mark.parametrize("entry", ["fractal_atlas_colourise_kfp_planes", "fractal_atlas_colourise_kfp_planes_precise"])
+def test_hip_atlas_kfp_planes_match_reference(gpu_product, oracle, case, entry):
+    """Parent / child tiles and planes composed and coloured on the device."""
+    print(f"[gpu] kfp atlas planes {case['name']} {entry}", flush=True)
+    parent, child, lut, opts = kfp_planes_io.atlas_inputs(case, oracle)
+    rgb = kfp_planes_io.atlas_colourise(gpu_product, case, parent, child, lut, opts, entry)
+    expected = load(case["name"])
+    golden_io.assert_rgb_within_one_level(rgb, expected, case["name"])
+    exact = float(np.mean(rgb == expected))
+    print(f"[gpu] kfp atlas planes {case['name']}: exact channel fraction {exact:.6f}", flush=True)
     assert exact > 0.995

@@ -73,3 +73,17 @@ def test_hip_planes_without_device_fail_loudly(hip, oracle):
     field, planes, lut, opts, max_iter = kfp_planes_io.inputs(CASES[0], oracle)
     with pytest.raises(RuntimeError):
         kfp_planes_io.colourise(hip, field, planes, lut, opts, max_iter)
+
+
+def test_hip_atlas_planes_argument_checks(hip, oracle):
+    """The plane-aware atlas entry's checks (renderer.cpp:17351-17388) run
+    before any device work."""
+    from kfp_planes_cases import ATLAS_CASES
+    case = ATLAS_CASES[0]
+    parent, child, lut, opts = kfp_planes_io.atlas_inputs(case, oracle)
+    with pytest.raises(RuntimeError, match="invalid native plane-aware KFP atlas dimensions or controls"):
+        kfp_planes_io.atlas_colourise(hip, dict(case, fraction=1.5), parent, child, lut, opts)
+    with pytest.raises(RuntimeError, match="child absence"):
+        kfp_planes_io.atlas_colourise(hip, dict(case, fraction=0.0), parent, child, lut, opts)
+    with pytest.raises(RuntimeError, match="field bias exceeds"):
+        kfp_planes_io.atlas_colourise(hip, case, parent, child, lut, native.KfpOptions.make(field_bias=30000.0))
