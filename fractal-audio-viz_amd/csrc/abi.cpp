@@ -344,6 +344,48 @@ int render_points_impl(float* output, int point_count, const char* zoom, const d
     }
 }
 
+// fractal_crop_colourise_kfp, renderer.cpp:17803-17968: the centred crop of
+// one scalar tile, interior-preserving with the KFP field bias as the tile's
+// own, then the KFP pass -- the raw atlas pass without a child.
+int crop_kfp(const float* source, int source_width, int source_height, uint8_t* output, int output_width,
+             int output_height, double zoom_factor, int max_iter, double phase, double vocal, double instrumental,
+             double pitch, const FractalKfpOptions* options, const uint8_t* lut, int lut_size, int threads) {
+    try {
+        if (!source || !output || !valid_dimensions(source_width, source_height)
+            || !valid_dimensions(output_width, output_height) || !valid_iterations(max_iter) || !valid_threads(threads)
+            || !std::isfinite(zoom_factor) || zoom_factor <= 0.0 || !std::isfinite(phase) || !std::isfinite(vocal)
+            || !std::isfinite(instrumental) || !std::isfinite(pitch) || !lut || !fav::kfp::valid_options(options, lut_size))
+            throw std::runtime_error("invalid native KFP crop dimensions or controls");
+        zoom_factor = std::max(zoom_factor, 1.0);
+        if (options->field_bias > static_cast<double>(max_iter))
+            throw std::runtime_error("native KFP field bias exceeds iteration cap");
+        fav::atlas::ComposeParams a{};
+        a.parent = source;
+        a.parent_bias = options->field_bias;
+        a.field_bias = options->field_bias;
+        a.parent_width = source_width;
+        a.parent_max_iter = max_iter;
+        a.width = output_width;
+        a.height = output_height;
+        a.max_iter = max_iter;
+        fav::atlas::Axis axes[4];
+        fav::atlas::fill_axis(axes[0], source_width, output_width, zoom_factor);
+        fav::atlas::fill_axis(axes[1], source_height, output_height, zoom_factor);
+        const fav::kfp::KfpParams k = fav::kfp::make_params(*options, nullptr, output, output_width, output_height,
+                                                            max_iter, lut, lut_size);
+        fav::atlas_colourise_kfp_raw(a, static_cast<size_t>(source_width) * static_cast<size_t>(source_height), 0,
+                                     axes, k, output);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native KFP crop colourizer failed with an unknown exception");
+        return 1;
+    }
+}
+
 // fractal_atlas_colourise_kfp_planes, renderer.cpp:17323-17801: the
 // reference's checks; tile selection, geometry and maps on the host; the
 // compose of the field and planes and the KFP pass on the device.
@@ -666,6 +708,24 @@ int fractal_atlas_colourise_kfp_raw_precise(const float* parent, int parent_widt
     return atlas_kfp_raw(parent, parent_width, parent_height, parent_max_iter, child, child_width, child_height,
                          child_max_iter, output, output_width, output_height, parent_zoom, child_fraction,
                          child_zoom, max_iter, phase, vocal, instrumental, pitch, options, lut, lut_size, threads);
+}
+
+int fractal_crop_colourise_kfp(const float* source, int source_width, int source_height, uint8_t* output,
+                               int output_width, int output_height, double zoom_factor, int max_iter, double phase,
+                               double vocal, double instrumental, double pitch, const FractalKfpOptions* options,
+                               const uint8_t* lut, int lut_size, int threads) {
+    return crop_kfp(source, source_width, source_height, output, output_width, output_height, zoom_factor, max_iter,
+                    phase, vocal, instrumental, pitch, options, lut, lut_size, threads);
+}
+
+// renderer.cpp:18273-18296: the same pass.
+int fractal_crop_colourise_kfp_precise(const float* source, int source_width, int source_height, uint8_t* output,
+                                       int output_width, int output_height, double zoom_factor, int max_iter,
+                                       double phase, double vocal, double instrumental, double pitch,
+                                       const FractalKfpOptions* options, const uint8_t* lut, int lut_size,
+                                       int threads) {
+    return crop_kfp(source, source_width, source_height, output, output_width, output_height, zoom_factor, max_iter,
+                    phase, vocal, instrumental, pitch, options, lut, lut_size, threads);
 }
 
 int fractal_atlas_colourise_kfp_planes(const float* parent, int parent_width, int parent_height, int parent_max_iter,

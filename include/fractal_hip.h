@@ -313,6 +313,22 @@ int fractal_atlas_colourise_kfp_raw_precise(
     const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
     int threads);
 
+/* renderer.h:518-535 / renderer.cpp:17803-17968: the centred crop of one
+ * scalar tile (interior-preserving) coloured by the KFP pass, on the
+ * device. */
+int fractal_crop_colourise_kfp(
+    const float *source, int source_width, int source_height,
+    uint8_t *output, int output_width, int output_height, double zoom_factor,
+    int max_iter, double phase, double vocal, double instrumental,
+    double pitch, const FractalKfpOptions *options, const uint8_t *lut,
+    int lut_size, int threads);
+/* renderer.h:536-553 / renderer.cpp:18273-18296: the same pass. */
+int fractal_crop_colourise_kfp_precise(
+    const float *source, int source_width, int source_height,
+    uint8_t *output, int output_width, int output_height, double zoom_factor,
+    int max_iter, double phase, double vocal, double instrumental,
+    double pitch, const FractalKfpOptions *options, const uint8_t *lut,
+    int lut_size, int threads);
 /* renderer.h:604-631 / renderer.cpp:17323-17801: the parent and centred
  * child scalar tiles and their Kalles planes reprojected into one frame
  * (field interior-preserving, integer planes nearest, double planes

@@ -71,3 +71,15 @@ ATLAS_CASES = [
        dict(KALLES_DEFAULT, texture_enabled=1, texture_merge=0.5, texture_power=4.0, texture_ratio=50.0),
        child_texture=True, parent_planes=("orbit_iteration", "test1", "test2")),
 ]
+
+
+# fractal_crop_colourise_kfp (renderer.cpp:17803-17968): one scalar tile,
+# optionally centred on its cap (then field_bias is the cap)
+CROP_CASES = [
+    dict(name="kfp_crop_default", source="c1_series", ow=80, oh=45, zoom=1.6, max_iter=20000,
+         options=dict(KALLES_DEFAULT), stops="kalles", centred=False),
+    dict(name="kfp_crop_stretch", source="bla_1e25", ow=96, oh=54, zoom=2.0, max_iter=50000,
+         options=dict(color_method=4, differences=1, slopes=1), stops="seeded", centred=False),
+    dict(name="kfp_crop_centred", source="c1_kalles", ow=70, oh=40, zoom=1.3, max_iter=20000,
+         options=dict(KALLES_DEFAULT, field_bias=20000.0), stops="kalles", centred=True),
+]

@@ -183,3 +183,28 @@ def atlas_colourise(lib, case, parent, child, lut, opts, entry="fractal_atlas_co
     if status != 0:
         raise RuntimeError(lib.fractal_last_error().decode())
     return out
+
+
+def crop_inputs(case, oracle):
+    with np.load(GOLDEN / f"{case['source']}.npz", allow_pickle=False) as g:
+        field = g["field"].astype(np.float64)
+    if case["centred"]:
+        field = field - case["max_iter"]
+    stops = np.asarray(KALLES_DEFAULT_STOPS if case["stops"] == "kalles" else seeded_stops(), dtype=np.uint8)
+    lut = native.palette_lut_oracle(oracle, stops, 1024)
+    return np.ascontiguousarray(field, dtype=np.float32), lut, native.KfpOptions.make(**case["options"])
+
+
+def crop_colourise(lib, case, field, lut, opts, entry="fractal_crop_colourise_kfp"):
+    fn = getattr(lib, entry)
+    fn.argtypes = [native.c_float_p, ctypes.c_int, ctypes.c_int, native.c_uint8_p, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                   ctypes.POINTER(native.KfpOptions), native.c_uint8_p, ctypes.c_int, ctypes.c_int]
+    fn.restype = ctypes.c_int
+    out = np.zeros((case["oh"], case["ow"], 3), dtype=np.uint8)
+    status = fn(field.ctypes.data_as(native.c_float_p), field.shape[1], field.shape[0],
+                out.ctypes.data_as(native.c_uint8_p), case["ow"], case["oh"], case["zoom"], case["max_iter"], 0.0, 0.0,
+                0.0, 0.0, ctypes.byref(opts), lut.ctypes.data_as(native.c_uint8_p), lut.shape[0], 0)
+    if status != 0:
+        raise RuntimeError(lib.fractal_last_error().decode())
+    return out

@@ -7,7 +7,7 @@ import pytest
 import golden_io
 import kfp_planes_io
 import native
-from kfp_planes_cases import ATLAS_CASES, CASES
+from kfp_planes_cases import ATLAS_CASES, CASES, CROP_CASES
 from test_kfp_planes_cpu import load
 
 pytestmark = pytest.mark.gpu
@@ -37,3 +37,14 @@ def test_hip_atlas_kfp_planes_match_reference(gpu_product, oracle, case, entry):
     exact = float(np.mean(rgb == expected))
     print(f"[gpu] kfp atlas planes {case['name']}: exact channel fraction {exact:.6f}", flush=True)
     assert exact > 0.995
+
+
+@pytest.mark.parametrize("case", CROP_CASES, ids=[c["name"] for c in CROP_CASES])
+@pytest.mark.parametrize("entry", ["fractal_crop_colourise_kfp", "fractal_crop_colourise_kfp_precise"])
+def test_hip_crop_kfp_matches_reference(gpu_product, oracle, case, entry):
+    print(f"[gpu] kfp crop {case['name']} {entry}", flush=True)
+    field, lut, opts = kfp_planes_io.crop_inputs(case, oracle)
+    rgb = kfp_planes_io.crop_colourise(gpu_product, case, field, lut, opts, entry)
+    expected = load(case["name"])
+    golden_io.assert_rgb_within_one_level(rgb, expected, case["name"])
+    assert float(np.mean(rgb == expected)) > 0.995

@@ -87,3 +87,24 @@ def test_hip_atlas_planes_argument_checks(hip, oracle):
         kfp_planes_io.atlas_colourise(hip, dict(case, fraction=0.0), parent, child, lut, opts)
     with pytest.raises(RuntimeError, match="field bias exceeds"):
         kfp_planes_io.atlas_colourise(hip, case, parent, child, lut, native.KfpOptions.make(field_bias=30000.0))
+
+
+@pytest.mark.parametrize("name", ["kfp_crop_default", "kfp_crop_stretch", "kfp_crop_centred"])
+def test_crop_equals_the_childless_raw_atlas_oracle(oracle, name):
+    """The KFP crop is the raw atlas pass without a child when the KFP field
+    bias is 0 or the tile's cap (renderer.cpp:17803-17968 against
+    17969-18265): the C oracle's raw atlas reproduces the reference's crop."""
+    from kfp_planes_cases import CROP_CASES
+    case = next(c for c in CROP_CASES if c["name"] == name)
+    field, lut, opts = kfp_planes_io.crop_inputs(case, oracle)
+    fn = oracle.fo_atlas_colourise_kfp_raw
+    fn.argtypes = [native.c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, native.c_float_p, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, native.c_uint8_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(native.KfpOptions),
+                   native.c_uint8_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    out = np.zeros((case["oh"], case["ow"], 3), dtype=np.uint8)
+    err = ctypes.create_string_buffer(256)
+    assert fn(field.ctypes.data_as(native.c_float_p), field.shape[1], field.shape[0], case["max_iter"], None, 0, 0, 0,
+              out.ctypes.data_as(native.c_uint8_p), case["ow"], case["oh"], case["zoom"], 0.0, 1.0, case["max_iter"],
+              ctypes.byref(opts), lut.ctypes.data_as(native.c_uint8_p), lut.shape[0], 0, err, 256) == 0, err.value
+    np.testing.assert_array_equal(out, load(name))
