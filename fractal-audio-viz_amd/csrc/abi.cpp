@@ -344,6 +344,66 @@ int render_points_impl(float* output, int point_count, const char* zoom, const d
     }
 }
 
+// fractal_atlas_colourise_kfp, renderer.cpp:16750-17321: an output-sized
+// parent and a child tile at an explicit rectangle, each coloured from its
+// own field and transfer range, blended across the feather.
+int atlas_kfp_rect(const float* parent, int parent_width, int parent_height, const float* child, int child_width,
+                   int child_height, uint8_t* output, int output_width, int output_height, int max_iter,
+                   int child_left, int child_top, int feather, double phase, double vocal, double instrumental,
+                   double pitch, const FractalKfpOptions* options, const uint8_t* lut, int lut_size, int threads) {
+    try {
+        if (!parent || !output || !valid_dimensions(parent_width, parent_height)
+            || !valid_dimensions(output_width, output_height) || parent_width != output_width
+            || parent_height != output_height || !valid_iterations(max_iter) || !valid_threads(threads)
+            || !std::isfinite(phase) || !std::isfinite(vocal) || !std::isfinite(instrumental) || !std::isfinite(pitch)
+            || !lut || !fav::kfp::valid_options(options, lut_size))
+            throw std::runtime_error("invalid native KFP atlas dimensions or controls");
+        const bool use_child = child != nullptr;
+        if (use_child) {
+            if (!valid_dimensions(child_width, child_height) || child_left < 0 || child_top < 0
+                || child_width > output_width - child_left || child_height > output_height - child_top || feather < 0
+                || feather > std::min(child_width, child_height))
+                throw std::runtime_error("invalid native KFP atlas child rectangle");
+        } else if (child_width != 0 || child_height != 0 || child_left != 0 || child_top != 0 || feather != 0) {
+            throw std::runtime_error("invalid native KFP atlas child absence");
+        }
+        if (options->field_bias > static_cast<double>(max_iter))
+            throw std::runtime_error("native KFP field bias exceeds iteration cap");
+        const fav::kfp::KfpParams p = fav::kfp::make_params(*options, parent, output, output_width, output_height,
+                                                            max_iter, lut, lut_size);
+        fav::kfp::KfpParams c{};
+        if (use_child) c = fav::kfp::make_params(*options, child, output, child_width, child_height, max_iter, lut,
+                                                 lut_size);
+        fav::kfp::RectAtlas g{};
+        g.use_child = use_child;
+        g.child_left = child_left;
+        g.child_top = child_top;
+        g.feather = feather;
+        std::vector<float> ex, ey;
+        if (use_child && feather >= 2) {  // renderer.cpp:16887-16905
+            auto fill = [feather](std::vector<float>& v, int n) {
+                v.resize(static_cast<size_t>(n));
+                for (int i = 0; i < n; ++i) {
+                    const int edge = std::min(i, n - 1 - i);
+                    const float linear = std::min(1.0F, static_cast<float>(edge) / static_cast<float>(feather));
+                    v[static_cast<size_t>(i)] = linear * linear * (3.0F - 2.0F * linear);
+                }
+            };
+            fill(ex, child_width);
+            fill(ey, child_height);
+        }
+        fav::atlas_colourise_kfp_rect(p, c, g, ex, ey);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native KFP atlas colourizer failed with an unknown exception");
+        return 1;
+    }
+}
+
 // fractal_crop_colourise_kfp, renderer.cpp:17803-17968: the centred crop of
 // one scalar tile, interior-preserving with the KFP field bias as the tile's
 // own, then the KFP pass -- the raw atlas pass without a child.
@@ -708,6 +768,28 @@ int fractal_atlas_colourise_kfp_raw_precise(const float* parent, int parent_widt
     return atlas_kfp_raw(parent, parent_width, parent_height, parent_max_iter, child, child_width, child_height,
                          child_max_iter, output, output_width, output_height, parent_zoom, child_fraction,
                          child_zoom, max_iter, phase, vocal, instrumental, pitch, options, lut, lut_size, threads);
+}
+
+int fractal_atlas_colourise_kfp(const float* parent, int parent_width, int parent_height, const float* child,
+                                int child_width, int child_height, uint8_t* output, int output_width,
+                                int output_height, int max_iter, int child_left, int child_top, int feather,
+                                double phase, double vocal, double instrumental, double pitch,
+                                const FractalKfpOptions* options, const uint8_t* lut, int lut_size, int threads) {
+    return atlas_kfp_rect(parent, parent_width, parent_height, child, child_width, child_height, output, output_width,
+                          output_height, max_iter, child_left, child_top, feather, phase, vocal, instrumental, pitch,
+                          options, lut, lut_size, threads);
+}
+
+// renderer.cpp:18298-18327: the same pass.
+int fractal_atlas_colourise_kfp_precise(const float* parent, int parent_width, int parent_height, const float* child,
+                                        int child_width, int child_height, uint8_t* output, int output_width,
+                                        int output_height, int max_iter, int child_left, int child_top, int feather,
+                                        double phase, double vocal, double instrumental, double pitch,
+                                        const FractalKfpOptions* options, const uint8_t* lut, int lut_size,
+                                        int threads) {
+    return atlas_kfp_rect(parent, parent_width, parent_height, child, child_width, child_height, output, output_width,
+                          output_height, max_iter, child_left, child_top, feather, phase, vocal, instrumental, pitch,
+                          options, lut, lut_size, threads);
 }
 
 int fractal_crop_colourise_kfp(const float* source, int source_width, int source_height, uint8_t* output,

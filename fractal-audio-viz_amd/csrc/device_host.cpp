@@ -34,6 +34,8 @@ hipError_t launch_batch(const BatchLaunch& q, int fast_blocks, int persistent_bl
 hipError_t launch_atlas_compose(const atlas::ComposeParams& a, hipStream_t stream);
 hipError_t launch_aurora(const aurora::Params& p, hipStream_t stream);
 hipError_t launch_atlas_planes(const atlas::PlanesCompose& a, hipStream_t stream);
+hipError_t launch_kfp_rect_atlas(kfp::KfpParams* host_params, kfp::KfpParams* params, const kfp::RectAtlas& g,
+                                 void* scratch, int compute_units, hipStream_t stream);
 
 namespace {
 
@@ -498,6 +500,50 @@ void atlas_colourise_kfp_planes(const atlas::PlanesCompose& host, const PlaneTil
     s.timing.kernel_ms = kernel_ms;
     s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
     s.timing.pixels = static_cast<double>(pixels);
+}
+
+void atlas_colourise_kfp_rect(const kfp::KfpParams& parent, const kfp::KfpParams& child, const kfp::RectAtlas& g,
+                              const std::vector<float>& edge_x, const std::vector<float>& edge_y) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    auto align = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+    const size_t pn = static_cast<size_t>(parent.width) * static_cast<size_t>(parent.height);
+    const size_t cn = child.field ? static_cast<size_t>(child.width) * static_cast<size_t>(child.height) : 0;
+    const size_t lut_bytes = static_cast<size_t>(parent.lut_size) * 3;
+    char* in = static_cast<char*>(s.atlas_tiles.get(align(pn * 4) + align(cn * 4) + align(edge_x.size() * 4)
+                                                    + align(edge_y.size() * 4) + align(lut_bytes) + 256));
+    size_t at = 0;
+    auto put = [&](const void* src, size_t bytes) -> const void* {
+        if (!src || !bytes) return nullptr;
+        check(hipMemcpyAsync(in + at, src, bytes, hipMemcpyHostToDevice, s.stream), "H2D KFP atlas input");
+        const void* d = in + at;
+        at += align(bytes);
+        return d;
+    };
+    kfp::KfpParams params[2] = {parent, child};
+    params[0].field = static_cast<const float*>(put(parent.field, pn * 4));
+    params[1].field = static_cast<const float*>(put(child.field, cn * 4));
+    const uint8_t* lut = static_cast<const uint8_t*>(put(parent.lut, lut_bytes));
+    params[0].lut = params[1].lut = lut;
+    kfp::RectAtlas geometry = g;
+    geometry.edge_x = static_cast<const float*>(put(edge_x.data(), edge_x.size() * 4));
+    geometry.edge_y = static_cast<const float*>(put(edge_y.data(), edge_y.size() * 4));
+    params[0].output = params[1].output = static_cast<uint8_t*>(s.kfp_rgb.get(pn * 3));
+    auto* device_params = static_cast<kfp::KfpParams*>(s.kfp_planes.get(2 * sizeof(kfp::KfpParams) + 256));
+    void* scratch = s.kfp_scratch.get(128);
+    const uint64_t wall0 = now_ns();
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_kfp_rect_atlas(params, device_params, geometry, scratch, s.compute_units, s.stream),
+          "KFP atlas launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    check(hipMemcpyAsync(parent.output, params[0].output, pn * 3, hipMemcpyDeviceToHost, s.stream), "D2H rgb");
+    check(hipStreamSynchronize(s.stream), "KFP atlas");
+    float kernel_ms = 0.0f;
+    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    s.timing.kernel_ms = kernel_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pn);
 }
 
 void aurora_run(const aurora::Params& host, const AuroraInputs& in) {

@@ -121,6 +121,14 @@ struct PlaneTile {
 void atlas_colourise_kfp_planes(const atlas::PlanesCompose& compose, const PlaneTile* tiles, const atlas::Axis* axes,
                                 const kfp::KfpParams& colour, const FractalKfpPlanes* texture, uint8_t* output);
 
+// Explicit-rectangle KFP atlas (kfp_core.h rect_atlas_pixel): `parent` and
+// `child` carry host field / LUT / output pointers (child.field null when
+// there is no child); edge_x / edge_y the child's seam weights (empty
+// without a feather).
+namespace kfp { struct RectAtlas; }
+void atlas_colourise_kfp_rect(const kfp::KfpParams& parent, const kfp::KfpParams& child, const kfp::RectAtlas& g,
+                              const std::vector<float>& edge_x, const std::vector<float>& edge_y);
+
 // Ordinary-palette colouriser and crop / atlas compositors (aurora_core.h).
 // `params` carries the mode, geometry and palette controls; the sources,
 // axes (parent x / y, child x / y; empty when unused), seam weights and

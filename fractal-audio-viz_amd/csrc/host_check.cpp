@@ -424,3 +424,56 @@ extern "C" int fav_host_check_fx(const char* x, const char* y, const char* ref_z
         return 1;
     }
 }
+
+// The explicit-rectangle KFP atlas of kfp_kernel.hip run on the CPU
+// (test-only), with the caller's checks done by the test.
+extern "C" int fav_host_check_kfp_rect(const float* parent, const float* child, int child_width, int child_height,
+                                       uint8_t* output, int width, int height, int max_iter, int child_left,
+                                       int child_top, int feather, const FractalKfpOptions* options,
+                                       const uint8_t* lut, int lut_size) {
+    try {
+        fav::kfp::KfpParams k[2];
+        k[0] = fav::kfp::make_params(*options, parent, output, width, height, max_iter, lut, lut_size);
+        k[1] = child ? fav::kfp::make_params(*options, child, output, child_width, child_height, max_iter, lut,
+                                             lut_size)
+                     : fav::kfp::KfpParams{};
+        double bounds[4] = {0.0, 1.0, 0.0, 1.0};
+        for (int t = 0; t < 2; ++t) {
+            k[t].bounds = bounds + 2 * t;
+            if (k[t].color_method != 4 || !k[t].field) continue;
+            long long lo = 0, hi = 0;
+            bool any = false;
+            for (size_t i = 0; i < static_cast<size_t>(k[t].width) * k[t].height; ++i) {
+                long long v;
+                if (!fav::kfp::transfer_bound_value(k[t], i, v)) continue;
+                lo = any ? std::min(lo, v) : v;
+                hi = any ? std::max(hi, v) : v;
+                any = true;
+            }
+            if (any) {
+                bounds[2 * t] = static_cast<double>(lo);
+                bounds[2 * t + 1] = static_cast<double>(hi);
+            }
+        }
+        std::vector<float> ex, ey;
+        if (child && feather >= 2) {
+            auto fill = [feather](std::vector<float>& v, int n) {
+                v.resize(static_cast<size_t>(n));
+                for (int i = 0; i < n; ++i) {
+                    const int edge = std::min(i, n - 1 - i);
+                    const float linear = std::min(1.0F, static_cast<float>(edge) / static_cast<float>(feather));
+                    v[static_cast<size_t>(i)] = linear * linear * (3.0F - 2.0F * linear);
+                }
+            };
+            fill(ex, child_width);
+            fill(ey, child_height);
+        }
+        fav::kfp::RectAtlas g{child != nullptr, child_left, child_top, feather, ex.data(), ey.data()};
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < height; ++y)
+            for (int x = 0; x < width; ++x) fav::kfp::rect_atlas_pixel(k[0], k[1], g, x, y);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}

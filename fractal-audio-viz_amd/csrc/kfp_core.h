@@ -276,10 +276,11 @@ FAV_HD void palette_rgb(const KfpParams& k, double input, float& r, float& g, fl
     }
 }
 
-// write_kfp_pixel, renderer.cpp:6696-7289.
-FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
+// write_kfp_pixel, renderer.cpp:6696-7289: the pixel (x, y) of the field,
+// dithered at the output position (dither_x, dither_y), with distances and
+// slopes scaled by the output width `spatial_width`.
+FAV_HD void colour_at(const KfpParams& k, int x, int y, int dither_x, int dither_y, int spatial_width, uint8_t* dst) {
     const size_t idx = static_cast<size_t>(y) * static_cast<size_t>(k.width) + static_cast<size_t>(x);
-    uint8_t* dst = k.output + idx * 3U;
     if (k.transition && k.transition[idx] < 0.0 && !k.show_glitches) {  // a hidden glitch pixel
         dst[0] = 0;
         dst[1] = 0;
@@ -357,7 +358,7 @@ FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
     } else if (k.needs_difference) {
         gradient = difference_magnitude(k.differences, c, l, r, u, dn, tl, tr, bl, br);
     }
-    double distance = __builtin_isnan(gradient) ? 0.0 : gradient * static_cast<double>(k.width) / 640.0;
+    double distance = __builtin_isnan(gradient) ? 0.0 : gradient * static_cast<double>(spatial_width) / 640.0;
     if (distance < 0.0) distance = 0.0;
     double transfer = colour_iter;
     switch (k.color_method) {
@@ -446,7 +447,7 @@ FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
         const double horizontal = analytic ? slope_x : (x > 0 ? l - c : c - r);
         const double vertical = analytic ? slope_y : (y == 0 ? c - dn : u - c);
         const double projected = ((horizontal * k.cosine) + (vertical * k.sine)) * k.slope_power
-            * static_cast<double>(k.width) / 640.0;
+            * static_cast<double>(spatial_width) / 640.0;
         const double strength = ::atan(__builtin_fabs(projected)) / (kPi / 2.0) * k.slope_ratio / 100.0;
         if (projected >= 0.0) {
             red = static_cast<float>(static_cast<double>(red) * (1.0 - strength));
@@ -458,9 +459,65 @@ FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
             blue = static_cast<float>(static_cast<double>(blue) * (1.0 - strength) + strength);
         }
     }
-    dst[0] = dithered_srgb_byte(red, x, y, 0);
-    dst[1] = dithered_srgb_byte(green, x, y, 1);
-    dst[2] = dithered_srgb_byte(blue, x, y, 2);
+    dst[0] = dithered_srgb_byte(red, dither_x, dither_y, 0);
+    dst[1] = dithered_srgb_byte(green, dither_x, dither_y, 1);
+    dst[2] = dithered_srgb_byte(blue, dither_x, dither_y, 2);
+}
+
+// A frame pixel: field, dither and output at the same position.
+FAV_HD void colour_pixel(const KfpParams& k, int x, int y) {
+    const size_t idx = static_cast<size_t>(y) * static_cast<size_t>(k.width) + static_cast<size_t>(x);
+    colour_at(k, x, y, x, y, k.width, k.output + idx * 3U);
+}
+
+// fractal_atlas_colourise_kfp's geometry (renderer.cpp:16750-17321): an
+// output-sized parent and a child tile at (child_left, child_top), each
+// coloured from its own field and transfer range.
+struct RectAtlas {
+    int32_t use_child, child_left, child_top, feather;
+    const float* edge_x;  // separable seam weights of the child columns / rows
+    const float* edge_y;
+};
+
+FAV_HD uint8_t rounded_byte(double v) {  // rounded_colour_byte, renderer.cpp:5237-5242
+    v = v < 0.0 ? 0.0 : (v > 255.0 ? 255.0 : v);
+    const int i = static_cast<int>(__builtin_rint(v));
+    return static_cast<uint8_t>(i < 0 ? 0 : (i > 255 ? 255 : i));
+}
+
+// One output pixel of fractal_atlas_colourise_kfp (renderer.cpp:17080-17305).
+FAV_HD void rect_atlas_pixel(const KfpParams& parent, const KfpParams& child, const RectAtlas& g, int x, int y) {
+    uint8_t* dst = parent.output + (static_cast<size_t>(y) * static_cast<size_t>(parent.width) + static_cast<size_t>(x)) * 3U;
+    const int ow = parent.width;
+    if (!g.use_child || x < g.child_left || x >= g.child_left + child.width || y < g.child_top
+        || y >= g.child_top + child.height) {
+        colour_at(parent, x, y, x, y, ow, dst);
+        return;
+    }
+    const int cx = x - g.child_left, cy = y - g.child_top;
+    const float child_raw = child.field[static_cast<size_t>(cy) * static_cast<size_t>(child.width) + static_cast<size_t>(cx)];
+    const bool child_inside = inside_value(child_raw, child.max_iter, child.field_bias);
+    const float alpha = g.feather >= 2 ? (g.edge_x[cx] < g.edge_y[cy] ? g.edge_x[cx] : g.edge_y[cy]) : 1.0F;
+    if (child_inside && alpha >= 0.999999F) {
+        colour_at(child, cx, cy, x, y, ow, dst);
+        return;
+    }
+    if (g.feather < 2) {
+        const float parent_raw = parent.field[static_cast<size_t>(y) * static_cast<size_t>(ow) + static_cast<size_t>(x)];
+        if (!inside_value(parent_raw, parent.max_iter, parent.field_bias)) colour_at(parent, x, y, x, y, ow, dst);
+        else colour_at(child, cx, cy, x, y, ow, dst);
+        return;
+    }
+    if (alpha >= 0.999999F) {
+        colour_at(child, cx, cy, x, y, ow, dst);
+        return;
+    }
+    uint8_t p[3], c[3];
+    colour_at(parent, x, y, x, y, ow, p);
+    colour_at(child, cx, cy, x, y, ow, c);
+    const double a = static_cast<double>(alpha);
+    for (int ch = 0; ch < 3; ++ch)
+        dst[ch] = rounded_byte(static_cast<double>(p[ch]) * (1.0 - a) + static_cast<double>(c[ch]) * a);
 }
 
 // The value whose range kfp_transfer_bounds takes (renderer.cpp:7296-7334):

@@ -60,7 +60,43 @@ __global__ void kfp_bounds_finish(const long long* bounds, double* out) {
     out[1] = empty ? 1.0 : static_cast<double>(bounds[1]);
 }
 
+__global__ void __launch_bounds__(256) kfp_rect_atlas_kernel(const kfp::KfpParams* params, kfp::RectAtlas g) {
+    const kfp::KfpParams& parent = params[0];
+    const int x = blockIdx.x * 64 + static_cast<int>(threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + static_cast<int>(threadIdx.x >> 6);
+    if (x >= parent.width || y >= parent.height) return;
+    kfp::rect_atlas_pixel(parent, params[1], g, x, y);
+}
+
+void launch_bounds(const kfp::KfpParams& k, long long* bounds_i, double* bounds_d, int compute_units,
+                   hipStream_t stream) {
+    kfp_bounds_init<<<1, 1, 0, stream>>>(bounds_i);
+    kfp_bounds_kernel<<<dim3(compute_units * 4), dim3(256), 0, stream>>>(k, bounds_i);
+    kfp_bounds_finish<<<1, 1, 0, stream>>>(bounds_i, bounds_d);
+}
+
 }  // namespace
+
+// The explicit-rectangle KFP atlas: `params` is device memory for two
+// KfpParams (parent, child) that the host fills after this call sets their
+// bounds; `scratch` holds 4 int64 + 4 doubles.  The caller copies
+// `host_params` (bounds pointers set here) to `params` on `stream` before
+// the colour launch, which this function issues after the copy.
+hipError_t launch_kfp_rect_atlas(kfp::KfpParams* host_params, kfp::KfpParams* params, const kfp::RectAtlas& g,
+                                 void* scratch, int compute_units, hipStream_t stream) {
+    long long* bounds_i = static_cast<long long*>(scratch);
+    double* bounds_d = reinterpret_cast<double*>(bounds_i + 4);
+    for (int t = 0; t < 2; ++t) {
+        host_params[t].bounds = bounds_d + 2 * t;
+        if (host_params[t].color_method == 4 && (t == 0 || g.use_child))
+            launch_bounds(host_params[t], bounds_i + 2 * t, bounds_d + 2 * t, compute_units, stream);
+    }
+    hipError_t e = hipMemcpyAsync(params, host_params, 2 * sizeof(kfp::KfpParams), hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return e;
+    const kfp::KfpParams& parent = host_params[0];
+    kfp_rect_atlas_kernel<<<dim3((parent.width + 63) / 64, (parent.height + 3) / 4), dim3(256), 0, stream>>>(params, g);
+    return hipGetLastError();
+}
 
 // `scratch` holds 2 int64 + 2 doubles.  Shapes validated by the caller.
 hipError_t launch_kfp_kernel(const kfp::KfpParams& params, void* scratch, int compute_units, hipStream_t stream) {

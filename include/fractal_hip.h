@@ -313,6 +313,25 @@ int fractal_atlas_colourise_kfp_raw_precise(
     const FractalKfpOptions *options, const uint8_t *lut, int lut_size,
     int threads);
 
+/* renderer.h:554-576 / renderer.cpp:16750-17321: an output-sized parent
+ * field and a child field placed at (child_left, child_top), each coloured
+ * by the KFP pass from its own samples and transfer range, blended across
+ * the feather; on the device. */
+int fractal_atlas_colourise_kfp(
+    const float *parent, int parent_width, int parent_height,
+    const float *child, int child_width, int child_height, uint8_t *output,
+    int output_width, int output_height, int max_iter, int child_left,
+    int child_top, int feather, double phase, double vocal,
+    double instrumental, double pitch, const FractalKfpOptions *options,
+    const uint8_t *lut, int lut_size, int threads);
+/* renderer.h:577-599 / renderer.cpp:18298-18327: the same pass. */
+int fractal_atlas_colourise_kfp_precise(
+    const float *parent, int parent_width, int parent_height,
+    const float *child, int child_width, int child_height, uint8_t *output,
+    int output_width, int output_height, int max_iter, int child_left,
+    int child_top, int feather, double phase, double vocal,
+    double instrumental, double pitch, const FractalKfpOptions *options,
+    const uint8_t *lut, int lut_size, int threads);
 /* renderer.h:518-535 / renderer.cpp:17803-17968: the centred crop of one
  * scalar tile (interior-preserving) coloured by the KFP pass, on the
  * device. */

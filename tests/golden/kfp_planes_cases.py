@@ -83,3 +83,17 @@ CROP_CASES = [
     dict(name="kfp_crop_centred", source="c1_kalles", ow=70, oh=40, zoom=1.3, max_iter=20000,
          options=dict(KALLES_DEFAULT, field_bias=20000.0), stops="kalles", centred=True),
 ]
+
+
+# fractal_atlas_colourise_kfp (renderer.cpp:16750-17321): the output-sized
+# parent and a child field at an explicit rectangle
+RECT_CASES = [
+    dict(name="kfp_rect_feather", parent="c1_series", child="bla_1e27", left=20, top=12, feather=8, max_iter=50000,
+         options=dict(KALLES_DEFAULT), stops="kalles"),
+    dict(name="kfp_rect_stretch", parent="c1_series", child="c2_kalles", left=40, top=20, feather=12, max_iter=100000,
+         options=dict(color_method=4, differences=2, slopes=1), stops="seeded"),
+    dict(name="kfp_rect_hard_edge", parent="bla_1e25", child="bla_1e27", left=8, top=5, feather=0, max_iter=50000,
+         options=dict(color_method=7, differences=3, slopes=1, iter_div=0.02), stops="kalles"),
+    dict(name="kfp_rect_no_child", parent="c1_kalles", child=None, left=0, top=0, feather=0, max_iter=20000,
+         options=dict(KALLES_DEFAULT), stops="kalles"),
+]

@@ -18,7 +18,7 @@ sys.path.insert(0, str(HERE))
 
 import kfp_planes_io  # noqa: E402
 import native  # noqa: E402
-from kfp_planes_cases import ATLAS_CASES, CASES, CROP_CASES  # noqa: E402
+from kfp_planes_cases import ATLAS_CASES, CASES, CROP_CASES, RECT_CASES  # noqa: E402
 
 
 def main():
@@ -32,6 +32,11 @@ def main():
         rgb = kfp_planes_io.colourise(ref, field, planes, lut, opts, max_iter)
         np.savez_compressed(HERE / f"{case['name']}.npz", meta=np.array(json.dumps(case)), rgb=rgb)
         print(case["name"], field.shape, "mean rgb", rgb.reshape(-1, 3).mean(axis=0).round(1), flush=True)
+    for case in RECT_CASES:
+        parent, child, lut, opts = kfp_planes_io.rect_inputs(case, oracle)
+        rgb = kfp_planes_io.rect_colourise(ref, case, parent, child, lut, opts)
+        np.savez_compressed(HERE / f"{case['name']}.npz", meta=np.array(json.dumps(case)), rgb=rgb)
+        print(case["name"], rgb.shape, "mean rgb", rgb.reshape(-1, 3).mean(axis=0).round(1), flush=True)
     for case in CROP_CASES:
         field, lut, opts = kfp_planes_io.crop_inputs(case, oracle)
         rgb = kfp_planes_io.crop_colourise(ref, case, field, lut, opts)

@@ -208,3 +208,49 @@ def crop_colourise(lib, case, field, lut, opts, entry="fractal_crop_colourise_kf
     if status != 0:
         raise RuntimeError(lib.fractal_last_error().decode())
     return out
+
+
+def rect_inputs(case, oracle):
+    def field(name):
+        with np.load(GOLDEN / f"{name}.npz", allow_pickle=False) as g:
+            return np.ascontiguousarray(g["field"], dtype=np.float32)
+    parent = field(case["parent"])
+    child = field(case["child"]) if case["child"] else None
+    stops = np.asarray(KALLES_DEFAULT_STOPS if case["stops"] == "kalles" else seeded_stops(), dtype=np.uint8)
+    lut = native.palette_lut_oracle(oracle, stops, 1024)
+    return parent, child, lut, native.KfpOptions.make(**case["options"])
+
+
+def rect_colourise(lib, case, parent, child, lut, opts, entry="fractal_atlas_colourise_kfp"):
+    fn = getattr(lib, entry)
+    fn.argtypes = [native.c_float_p, ctypes.c_int, ctypes.c_int, native.c_float_p, ctypes.c_int, ctypes.c_int,
+                   native.c_uint8_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                   ctypes.POINTER(native.KfpOptions), native.c_uint8_p, ctypes.c_int, ctypes.c_int]
+    fn.restype = ctypes.c_int
+    h, w = parent.shape
+    out = np.zeros((h, w, 3), dtype=np.uint8)
+    ch, cw = child.shape if child is not None else (0, 0)
+    status = fn(parent.ctypes.data_as(native.c_float_p), w, h,
+                child.ctypes.data_as(native.c_float_p) if child is not None else None, cw, ch,
+                out.ctypes.data_as(native.c_uint8_p), w, h, case["max_iter"], case["left"], case["top"],
+                case["feather"], 0.0, 0.0, 0.0, 0.0, ctypes.byref(opts), lut.ctypes.data_as(native.c_uint8_p),
+                lut.shape[0], 0)
+    if status != 0:
+        raise RuntimeError(lib.fractal_last_error().decode())
+    return out
+
+
+def rect_host_check(lib, case, parent, child, lut, opts):
+    fn = lib.fav_host_check_kfp_rect
+    fn.argtypes = [native.c_float_p, native.c_float_p, ctypes.c_int, ctypes.c_int, native.c_uint8_p, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                   ctypes.POINTER(native.KfpOptions), native.c_uint8_p, ctypes.c_int]
+    fn.restype = ctypes.c_int
+    h, w = parent.shape
+    out = np.zeros((h, w, 3), dtype=np.uint8)
+    ch, cw = child.shape if child is not None else (0, 0)
+    assert fn(parent.ctypes.data_as(native.c_float_p), child.ctypes.data_as(native.c_float_p) if child is not None
+              else None, cw, ch, out.ctypes.data_as(native.c_uint8_p), w, h, case["max_iter"], case["left"],
+              case["top"], case["feather"], ctypes.byref(opts), lut.ctypes.data_as(native.c_uint8_p), lut.shape[0]) == 0
+    return out

@@ -48,3 +48,17 @@ def test_hip_crop_kfp_matches_reference(gpu_product, oracle, case, entry):
     expected = load(case["name"])
     golden_io.assert_rgb_within_one_level(rgb, expected, case["name"])
     assert float(np.mean(rgb == expected)) > 0.995
+
+
+from kfp_planes_cases import RECT_CASES  # noqa: E402
+
+
+@pytest.mark.parametrize("case", RECT_CASES, ids=[c["name"] for c in RECT_CASES])
+@pytest.mark.parametrize("entry", ["fractal_atlas_colourise_kfp", "fractal_atlas_colourise_kfp_precise"])
+def test_hip_rect_atlas_matches_reference(gpu_product, oracle, case, entry):
+    print(f"[gpu] kfp rect atlas {case['name']} {entry}", flush=True)
+    parent, child, lut, opts = kfp_planes_io.rect_inputs(case, oracle)
+    rgb = kfp_planes_io.rect_colourise(gpu_product, case, parent, child, lut, opts, entry)
+    expected = load(case["name"])
+    golden_io.assert_rgb_within_one_level(rgb, expected, case["name"])
+    assert float(np.mean(rgb == expected)) > 0.995

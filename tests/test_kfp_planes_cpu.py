@@ -108,3 +108,24 @@ def test_crop_equals_the_childless_raw_atlas_oracle(oracle, name):
               out.ctypes.data_as(native.c_uint8_p), case["ow"], case["oh"], case["zoom"], 0.0, 1.0, case["max_iter"],
               ctypes.byref(opts), lut.ctypes.data_as(native.c_uint8_p), lut.shape[0], 0, err, 256) == 0, err.value
     np.testing.assert_array_equal(out, load(name))
+
+
+from kfp_planes_cases import RECT_CASES  # noqa: E402
+
+
+@pytest.mark.parametrize("case", RECT_CASES, ids=[c["name"] for c in RECT_CASES])
+def test_rect_atlas_kernel_code_matches_reference(harness, oracle, case):
+    """fractal_atlas_colourise_kfp's per-pixel code (kfp_core.h
+    rect_atlas_pixel) on the CPU: the reference's bytes."""
+    parent, child, lut, opts = kfp_planes_io.rect_inputs(case, oracle)
+    rgb = kfp_planes_io.rect_host_check(harness, case, parent, child, lut, opts)
+    np.testing.assert_array_equal(rgb, load(case["name"]))
+
+
+def test_hip_rect_atlas_argument_checks(hip, oracle):
+    case = RECT_CASES[0]
+    parent, child, lut, opts = kfp_planes_io.rect_inputs(case, oracle)
+    with pytest.raises(RuntimeError, match="child rectangle"):
+        kfp_planes_io.rect_colourise(hip, dict(case, left=60), parent, child, lut, opts)
+    with pytest.raises(RuntimeError, match="child absence"):
+        kfp_planes_io.rect_colourise(hip, dict(case, feather=3), parent, None, lut, opts)
