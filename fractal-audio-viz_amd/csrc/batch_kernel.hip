@@ -77,7 +77,7 @@ batch_fast_kernel(BatchLaunch q) {
             core::Counters<false> st;
             const SC dc = core::pixel_dc(p, px, py);
             core::PixelState s = core::init_pixel<false, false>(p, index, dc, st);
-            if (core::run_linear<false, false>(p, index, s, dc, st) == core::kHandoff) {
+            if (core::run_linear<false, false>(p, index, s, dc, st, nullptr) == core::kHandoff) {
                 core::save_state(p.state, index, s);
                 handoff = true;
             }

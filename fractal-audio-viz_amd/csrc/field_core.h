@@ -689,6 +689,58 @@ FAV_HD PixelState init_pixel(const RenderParams& p, size_t index, SC dc, Counter
 
 enum Outcome : int { kDone = 0, kHandoff = 1, kTail = 2 };
 
+// Capacity, in blocks, of one wave's derivative log (one byte per block).
+// A test build sets a tiny capacity so that replays on a full log run often.
+#ifndef FAV_DERIVATIVE_LOG
+#define FAV_DERIVATIVE_LOG 4096
+#endif
+constexpr int kDerivativeLog = FAV_DERIVATIVE_LOG;
+
+// The derivative after `count` logged blocks from reference index `start`:
+// g <- A g + B per block (renderer.cpp:13499-13511), block k of level l at
+// index r being linear[dir.offset[l] + ((r - 1) >> l)] as the block loop
+// chose it.  The *_nn forms give the general forms' results while the
+// operand and the result are normal; otherwise the general forms run.
+FAV_HD SC replay_derivative(const RenderParams& p, const uint8_t* dlog, int count, int start, SC g) {
+    const LevelDir& dir = p.linear_dir;
+    count = wave_first(count);  // equal on every lane that shares the log
+    int r = wave_first(start);
+    for (int i = 0; i < count; ++i) {
+        const int level = wave_first(static_cast<int>(dlog[i]));
+        const LinearStepDev step = load_uniform(p.linear, dir.offset[level] + ((r - 1) >> level));
+        const SC a{step.a_re, step.a_im, step.a_e};
+        const SC b{step.b_re, step.b_im, step.b_e};
+        SC g1;
+        bool general = !wave_all(sc_normal(g));
+        if (!general) {
+            g1 = sc_add_nn(sc_mul_nn(a, g), b);
+            general = !wave_all(!sc_is_zero(g1) && g1.e <= (1 << 28) && g1.e >= -(1 << 28));
+        }
+        if (general) g1 = sc_add(sc_mul(a, g), b);
+        g = g1;
+        r += step.length;
+    }
+    return g;
+}
+
+// Logged blocks whose derivative update is still owed: the pixel's
+// derivative is its state's derivative advanced through `count` logged
+// blocks from reference index `start`.
+struct DerivativeDebt {
+    int count = 0;
+    int start = 0;
+};
+
+template <bool Planes>
+FAV_HD void settle_derivative(const RenderParams& p, const uint8_t* dlog, PixelState& s, DerivativeDebt& debt) {
+    if constexpr (Planes) {
+        if (debt.count > 0) {
+            s.derivative = replay_derivative(p, dlog, debt.count, debt.start, s.derivative);
+            debt.count = 0;
+        }
+    }
+}
+
 // The common iterations of run_linear below, with the same results: while
 // the loop top is in the negligible-perturbation regime (Z + delta, the
 // margin and the norm are the orbit's own values, so the glitch / escape /
@@ -699,13 +751,25 @@ enum Outcome : int { kDone = 0, kHandoff = 1, kTail = 2 };
 // and choose the same BLA level, the table reads are wave-uniform (scalar)
 // loads; any other case returns to run_linear's general loop top, where
 // each lane continues from the state of its last committed block.
+//
+// With planes, the derivative update of each committed block (A g + B,
+// renderer.cpp:13499-13511) is deferred: the block's BLA level goes into the
+// wave's derivative log, and replay_derivative applies the logged blocks when
+// the log is full or when the loop ends with a lane short of the cap.  The
+// derivative feeds no decision of the loop, so the replayed value equals the
+// one computed block by block, and a wave whose lanes all reach the cap
+// (interior pixels) never computes it.  When the loop ends with every lane
+// at most one iteration short of the cap, no lane can come back to this loop
+// (one more step or block reaches the cap), so the log stays intact and the
+// replay is owed in `debt` instead: run_linear settles it only where the
+// derivative is read, and not at all for a last step that reaches the cap.
 template <bool Planes, bool Stats>
-FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Counters<Stats>& st) {
+FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Counters<Stats>& st, uint8_t* dlog,
+                              DerivativeDebt& debt) {
     const OrbitDev& o = p.orbit;
     const int max_iter = p.max_iter;
     const LevelDir& dir = p.linear_dir;
     bool ok = s.have_total && s.iteration < max_iter && sc_normal(s.delta);
-    if constexpr (Planes) ok = ok && sc_normal(s.derivative);
     int r = wave_first(s.ref);
     if (!wave_all(ok && s.ref == r)) return;
     if (r < 0 || r >= o.size) return;
@@ -713,8 +777,9 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
     // commit, the orbit-only loop-top values (total, its norm, the margin)
     // are those of this record; they are written to the pixel state on exit.
     OrbitFastDev top = load_uniform(o.fast, r);
-    SC delta = s.delta, derivative = s.derivative;
+    SC delta = s.delta, derivative = s.derivative;  // the derivative at log_start
     int iteration = s.iteration;
+    int log_start = r, logged = 0;
     bool committed = false;
     for (;;) {
         if (o.map_end > 1 && r >= o.map_end) break;
@@ -736,7 +801,7 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
         // levels above the alignment of offset are skipped by the lookup
         if (offset != 0 && __builtin_ctz(static_cast<unsigned>(offset)) < h)
             h = __builtin_ctz(static_cast<unsigned>(offset));
-        int k = -1;
+        int k = -1, chosen = 0;
         bool mixed = false;
         for (int level = h; level >= 0; --level) {
             const int index = offset >> level;
@@ -747,6 +812,7 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
             const bool fits = dn.e != cand.r2_e ? dn.e < cand.r2_e : dn.m < cand.r2_m;
             if (wave_all(fits)) {
                 k = static_cast<int>(kk);
+                chosen = level;
                 break;
             }
             if (wave_any(fits)) {  // lanes disagree on the level
@@ -764,16 +830,20 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
         const SC a{step.a_re, step.a_im, step.a_e};
         const SC b{step.b_re, step.b_im, step.b_e};
         const SC d1 = sc_add_nn(sc_mul_nn(a, delta), sc_mul_nn(b, dc));
-        SC g1 = derivative;
-        if constexpr (Planes) g1 = sc_add_nn(sc_mul_nn(a, derivative), b);
-        // the results stay normal unless a sum cancelled to zero or an
+        // the result stays normal unless the sum cancelled to zero or the
         // exponent left the normal range
-        bool normal = !sc_is_zero(d1) && d1.e <= next.small_exp && d1.e >= -(1 << 28);
-        if constexpr (Planes) normal = normal && !sc_is_zero(g1) && g1.e <= (1 << 28) && g1.e >= -(1 << 28);
+        const bool normal = !sc_is_zero(d1) && d1.e <= next.small_exp && d1.e >= -(1 << 28);
         if (!wave_all(normal)) break;
         // commit, renderer.cpp:13499-13511
         delta = d1;
-        derivative = g1;
+        if constexpr (Planes) {
+            if (logged == kDerivativeLog) {
+                derivative = replay_derivative(p, dlog, logged, log_start, derivative);
+                logged = 0;
+                log_start = r;
+            }
+            dlog[logged++] = static_cast<uint8_t>(chosen);
+        }
         iteration += length;
         r = end;
         top = next;
@@ -792,6 +862,16 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
     }
     if (committed) {
         const SN norm{top.zn_m, top.zn_e};
+        if constexpr (Planes) {
+            if (logged > 0) {
+                if (wave_all(iteration >= max_iter - 1)) {
+                    debt.count = logged;
+                    debt.start = log_start;
+                } else {
+                    derivative = replay_derivative(p, dlog, logged, log_start, derivative);
+                }
+            }
+        }
         s.delta = delta;
         s.derivative = derivative;
         s.iteration = iteration;
@@ -808,9 +888,11 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
 // Any other branch (glitch, escape, rebase, reference escape, cubic order,
 // plain-double tail, replay, retry) is left to the general kernel: the pixel
 // is handed on with its state at that loop top.  Interior pixels that reach
-// the cap finish here.
+// the cap finish here.  With planes, `dlog` is the wave's derivative log of
+// kDerivativeLog bytes, private to the wave (unused without planes).
 template <bool Planes, bool Stats>
-FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st) {
+FAV_HD Outcome linear_loop(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st,
+                           uint8_t* dlog, DerivativeDebt& debt) {
     const OrbitDev& o = p.orbit;
     const int max_iter = p.max_iter;
     const int mode = p.escape_radius_mode;
@@ -818,7 +900,8 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
     const bool normal_blocks = sc_normal(dc) && !p.disable_bla && !s.pixel_disable_bla;
     while (s.iteration < max_iter) {
         if (normal_blocks) {
-            run_normal_blocks<Planes, Stats>(p, s, dc, st);
+            settle_derivative<Planes>(p, dlog, s, debt);
+            run_normal_blocks<Planes, Stats>(p, s, dc, st, dlog, debt);
             if (s.iteration >= max_iter) break;
         }
         if (s.ref < 0 || s.ref >= o.size) return kHandoff;
@@ -867,6 +950,7 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
                 b = coef(p.cubic[k], 1);
             }
             const SC delta = sc_add(sc_mul(a, s.delta), sc_mul(b, dc));
+            settle_derivative<Planes>(p, dlog, s, debt);
             SC derivative = s.derivative;
             if constexpr (Planes) derivative = sc_add(sc_mul(a, s.derivative), b);
             const int ref = s.ref + length;
@@ -898,9 +982,15 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
             st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
             continue;
         }
-        // exact perturbation step, renderer.cpp:13516-13535
+        // exact perturbation step, renderer.cpp:13516-13535; a step that
+        // reaches the cap leaves an interior pixel, which reads no derivative
         const SC z = load_z(o, s.ref);
-        if constexpr (Planes) s.derivative = sc_add(sc_double(sc_mul(s.total, s.derivative)), kOne);
+        if constexpr (Planes) {
+            if (s.iteration + 1 < max_iter) {
+                settle_derivative<Planes>(p, dlog, s, debt);
+                s.derivative = sc_add(sc_double(sc_mul(s.total, s.derivative)), kOne);
+            }
+        }
         s.delta = sc_add(sc_double(sc_mul(z, s.delta)), sc_add(sc_mul(s.delta, s.delta), dc));
         ++s.ref;
         ++s.iteration;
@@ -911,6 +1001,18 @@ FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC
     p.output[index] = encode_iteration(max_iter, p.output_bias);
     st.max(ST_MAX_PIXEL_ITER, static_cast<unsigned long long>(s.iteration > 0 ? s.iteration : 0));
     return kDone;
+}
+
+// linear_loop with the derivative settled for a pixel handed on (the state
+// the general kernel resumes from carries it); an interior pixel ends with
+// kDone and reads none.
+template <bool Planes, bool Stats>
+FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st,
+                          uint8_t* dlog) {
+    DerivativeDebt debt;
+    const Outcome out = linear_loop<Planes, Stats>(p, index, s, dc, st, dlog, debt);
+    if (out == kHandoff) settle_derivative<Planes>(p, dlog, s, debt);
+    return out;
 }
 
 // The general loop of render_bla_impl, renderer.cpp:12889-13544, resumed

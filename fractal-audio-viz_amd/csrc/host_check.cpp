@@ -100,7 +100,8 @@ extern "C" int fav_host_check_render(
                 st.clear();
                 const SC dc = core::pixel_dc(p, px, py);
                 core::PixelState s = core::init_pixel<Planes, Stats>(p, i, dc, st);
-                if (core::run_linear<Planes, Stats>(p, i, s, dc, st) == core::kHandoff) {
+                uint8_t dlog[core::kDerivativeLog];
+                if (core::run_linear<Planes, Stats>(p, i, s, dc, st, dlog) == core::kHandoff) {
                     core::save_state(p.state, i, s);
                     outcome[i] = 1;
                 }

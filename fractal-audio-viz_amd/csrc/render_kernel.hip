@@ -59,12 +59,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
     const bool valid = px < p.width && py < p.height && (!p.point_re || index < static_cast<size_t>(p.point_count));
     bool handoff = false;
+    uint8_t* dlog = nullptr;
+    if constexpr (Planes) {
+        // one derivative log per wave, 16 KiB for the block's four waves
+        __shared__ uint8_t derivative_log[4][core::kDerivativeLog];
+        dlog = derivative_log[wave];
+    }
     if (valid) {
         core::Counters<Stats> st;
         st.clear();
         const SC dc = core::pixel_dc(p, px, py);
         core::PixelState s = core::init_pixel<Planes, Stats>(p, index, dc, st);
-        if (core::run_linear<Planes, Stats>(p, index, s, dc, st) == core::kHandoff) {
+        if (core::run_linear<Planes, Stats>(p, index, s, dc, st, dlog) == core::kHandoff) {
             core::save_state(p.state, index, s);
             handoff = true;
         }

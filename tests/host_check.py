@@ -7,10 +7,12 @@ import numpy as np
 import native
 
 HOST_CHECK_SO = native.ROOT / "fractal-audio-viz_amd" / "_build" / "libfav_host_check.so"
+# The same code built with a five-block derivative log (replays on a full log).
+HOST_CHECK_LOG5_SO = native.ROOT / "fractal-audio-viz_amd" / "_build" / "libfav_host_check_log5.so"
 
 
-def load():
-    lib = ctypes.CDLL(str(HOST_CHECK_SO))
+def load(path=HOST_CHECK_SO):
+    lib = ctypes.CDLL(str(path))
     lib.fav_host_check_render.argtypes = [
         ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         ctypes.c_char_p, ctypes.POINTER(native.RenderOptions), native.c_float_p, ctypes.c_int,

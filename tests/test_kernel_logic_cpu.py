@@ -24,6 +24,28 @@ def test_kernel_code_on_cpu_matches_golden(harness, name):
 
 
 @pytest.fixture(scope="module")
+def log5_harness():
+    if not host_check.HOST_CHECK_LOG5_SO.exists():
+        pytest.fail("libfav_host_check_log5.so is not built: run __graft_entry__.build()")
+    return host_check.load(host_check.HOST_CHECK_LOG5_SO)
+
+
+@pytest.mark.parametrize("name", golden_io.case_names())
+def test_deferred_derivative_replays_on_full_log(harness, log5_harness, name):
+    """The block loop defers each block's derivative update to a per-wave
+    log; with a five-block log the full-log replays run many times per
+    pixel.  The planes match the goldens, and the distance-estimate planes
+    are bit-identical to those of the 4096-block log."""
+    golden = golden_io.load(name)
+    field, planes, stats = host_check.render(log5_harness, golden["meta"])
+    golden_io.assert_hip_parity(golden, field, planes, stats)
+    _, wide, _ = host_check.render(harness, golden["meta"])
+    for plane in ("de_x", "de_y", "phase"):
+        np.testing.assert_array_equal(getattr(planes, plane).view(np.uint64), getattr(wide, plane).view(np.uint64),
+                                      err_msg=plane)
+
+
+@pytest.fixture(scope="module")
 def kfp_harness(harness):
     return host_check.bind_kfp(harness)
 
