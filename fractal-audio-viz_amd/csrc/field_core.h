@@ -790,8 +790,10 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
         const int limit = p.max_linear_length < remaining ? p.max_linear_length : remaining;
         const int lim = wave_first(limit);
         // the norm compare uses |Z|^2 of the record: equal to the state's
-        // total_norm whenever delta is negligible, which the first clause checks
-        ok = delta.e <= top.small_exp && sn_at_least(SN{top.zn_m, top.zn_e}, dn) && order <= 1 && limit == lim;
+        // total_norm whenever delta is negligible, which the first clause
+        // checks; a lane at the cap ends the loop before any further work
+        ok = iteration < max_iter && delta.e <= top.small_exp && sn_at_least(SN{top.zn_m, top.zn_e}, dn)
+             && order <= 1 && limit == lim;
         if (!wave_all(ok)) break;
         // BLA level: every lane must take the same (highest valid) level
         if (r <= 0 || lim <= 0 || dir.count == 0 || r > dir.size[0]) break;
@@ -858,7 +860,6 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
             st.add(ST_HISTOGRAM + bucket);
         }
         st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
-        if (!wave_all(iteration < max_iter)) break;
     }
     if (committed) {
         const SN norm{top.zn_m, top.zn_e};
