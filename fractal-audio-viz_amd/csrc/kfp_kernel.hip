@@ -16,7 +16,12 @@
 namespace fav {
 namespace {
 
-__global__ void __launch_bounds__(256) kfp_colour_kernel(kfp::KfpParams k) {
+// The parameters are read in place from the kernel-argument segment, where
+// the only explicit argument sits at offset 0: a reference to the by-value
+// argument would make the compiler copy all of it (3.4 KB with the period
+// tables) into every lane's private memory.
+__global__ void __launch_bounds__(256) kfp_colour_kernel(kfp::KfpParams) {
+    const kfp::KfpParams& k = *(const kfp::KfpParams*)__builtin_amdgcn_kernarg_segment_ptr();
     const int x = blockIdx.x * 64 + static_cast<int>(threadIdx.x & 63);
     const int y = blockIdx.y * 4 + static_cast<int>(threadIdx.x >> 6);
     if (x >= k.width || y >= k.height) return;
