@@ -56,6 +56,27 @@ def escape_count_sum(orbit_iteration, field, cap):
     return int(counts.sum())
 
 
+def cpu_model():
+    """The host CPU's model name from /proc/cpuinfo (platform.processor()
+    only names the architecture on Linux)."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.machine()
+
+
+def reference_build_flags():
+    """Compiler flags oracle/_ref/mandelbrot.so was built with (recorded by
+    oracle/Makefile next to the library)."""
+    try:
+        return (ROOT / "oracle" / "_ref" / "build_flags.txt").read_text().strip()
+    except OSError:
+        return "unknown"
+
+
 def cpu_baseline(work, threads):
     """The reference's renderer.cpp (oracle/_ref, built from source) on the
     host cores, over a quarter-area sample (half width, half height) of the
@@ -63,7 +84,7 @@ def cpu_baseline(work, threads):
     sys.path.insert(0, str(ROOT / "tests"))
     import native  # checker-side ctypes mirrors
 
-    lib = native.load_reference()
+    lib = native.load_reference(release=True)
     kind = "reference"
     sample_w, sample_h = work["width"] // 2, work["height"] // 2
     opts = native.RenderOptions.defaults(backend=0, **work["options"])
@@ -88,7 +109,9 @@ def cpu_baseline(work, threads):
     total = escape_count_sum(planes.orbit_iteration, field.reshape(-1), work["max_iter"])
     return {"value": total / seconds / 1e9, "unit": "Gpixel-iter/s", "cores": threads, "kind": kind,
             "sample": f"{sample_w}x{sample_h} of the same field (1/4 of the pixels), "
-                      f"{seconds:.1f} s on {threads} threads ({platform.processor() or platform.machine()})"}, build_s
+                      f"{seconds:.1f} s on {threads} threads",
+            "cpu_model": cpu_model(),
+            "build_flags": reference_build_flags() if kind == "reference" else "oracle/Makefile"}, build_s
 
 
 def copy_calibration(mib, copies):

@@ -1245,16 +1245,14 @@ int fractal_hip_render_reference_floatexp(float* output, int32_t* counts, int wi
 
 // ---- learned tile-cost model (no counterpart in the reference) ----
 
-namespace {
+// static: inside the extern "C" block an anonymous namespace would still
+// give the helper C linkage and export it.
+static constexpr int kMaxMlpSamples = 1 << 22;
 
-constexpr int kMaxMlpSamples = 1 << 22;
-
-void check_samples(int count) {
+static void check_samples(int count) {
     if (count <= 0 || count > kMaxMlpSamples)
         throw std::runtime_error("cost model sample count must be in [1, 4194304]");
 }
-
-}  // namespace
 
 int fractal_hip_cost_model_parameter_count(void) { return fav::mlp::kParams; }
 

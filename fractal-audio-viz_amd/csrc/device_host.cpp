@@ -126,6 +126,8 @@ DeviceTables::~DeviceTables() {
     if (series_block) (void)hipFree(series_block);
     if (alt_block) (void)hipFree(alt_block);
     if (fast_block) (void)hipFree(fast_block);
+    for (auto& tier : rung_block) for (void* p : tier) if (p) (void)hipFree(p);
+    for (auto& tier : plain_block) for (void* p : tier) if (p) (void)hipFree(p);
 }
 
 namespace {
@@ -210,6 +212,29 @@ void ensure_device_tables(ReferenceContext& c) {
     c.device = std::move(t);
 }
 
+// The plain-double block loop tables of linear tier `tier` for level cap
+// `cap` (table_image.cpp make_plain_tables), built and copied to HBM on first
+// use; null pointers when the reference has none.
+void ensure_plain_tables(ReferenceContext& c, int tier, int cap, const RungDev*& rungs,
+                         const PlainStepDev*& steps) {
+    std::lock_guard<std::mutex> lock(c.device_mutex);
+    DeviceTables& t = *c.device;
+    if (!t.plain_built[tier][cap]) {
+        std::vector<RungDev> r;
+        std::vector<PlainStepDev> st;
+        make_plain_tables(c, tier, cap, r, st);
+        if (!r.empty()) {
+            LaunchState& s = state();
+            upload_vector(r, t.rung_block[tier][cap], t.rungs[tier][cap], s.stream, t.bytes);
+            upload_vector(st, t.plain_block[tier][cap], t.plain[tier][cap], s.stream, t.bytes);
+            check(hipStreamSynchronize(s.stream), "plain table upload");
+        }
+        t.plain_built[tier][cap] = true;
+    }
+    rungs = t.rungs[tier][cap];
+    steps = t.plain[tier][cap];
+}
+
 // One deep field render on the device, host part of renderer.cpp:12672-13583.
 // A point render's offsets in HBM: re then im (doubles), then exponents.
 // The host copies stay alive in `staging` until the caller synchronises.
@@ -254,6 +279,13 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     p.series.radius2_e = scalar_series.radius2_e;
     p.linear = t.linear[prep.linear_tier];
     p.linear_dir = t.linear_dir[prep.linear_tier];
+    {
+        // floor(log2) of the block length limit (at least 2)
+        const int cap = 31 - __builtin_clz(static_cast<unsigned>(p.max_linear_length));
+        ensure_plain_tables(c, prep.linear_tier, cap, p.rungs, p.plain_steps);
+        p.plain_top = cap;
+        p.plain_pass = p.rungs != nullptr && p.fast_linear ? 1 : 0;
+    }
     p.cubic = t.cubic;
     p.cubic_dir = t.cubic_dir;
 

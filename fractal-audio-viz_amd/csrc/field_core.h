@@ -884,6 +884,401 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
     }
 }
 
+// ---- plain-double block loop ---------------------------------------------
+// run_normal_blocks in plain doubles.  Inside the loop every lane holds
+// delta = m * 2^E and dc = c * 2^E with one wave-uniform E, and the tables
+// hold A, B, the validity radii and |Z|^2 as plain doubles (table_image.cpp
+// make_plain_tables).  A block is then m' = A*m + B*c: the reference's
+// sc_mul / sc_mul / sc_add (renderer.cpp:7676-7722) without their
+// renormalisations, since scaling by a power of two commutes with rounding
+// as long as no value is subnormal and the sum is not cut off.  The bounds
+// that guarantee this, checked per lane:
+//  * m: larger component in [2^-100, 2^100], smaller one 0 or >= 2^-320;
+//  * c, A, B: nonzero components in [2^-450, 2^500], the smaller nonzero
+//    component at least 2^-420 times the larger;
+//  * the two products differ by at most 2^60 in their larger components, so
+//    the reference's exponent gap (renderer.cpp:7688) is at most 60 and
+//    both are added.
+// Then every product, sum and cancellation is a normal double both here and
+// in the reference's normalised operands, and m' * 2^E equals the
+// reference's delta.  |delta|^2 = |m|^2 * 2^2E is kept a normal double, so
+// the loop-top tests and the BLA lookup compare plain doubles with the same
+// outcome as the reference's ScaledNorm / FloatExp comparisons.  Any failed
+// bound leaves the loop before the block commits; run_normal_blocks then
+// continues from the same state.
+#ifndef FAV_PLAIN_BLOCK_PROBE
+#define FAV_PLAIN_BLOCK_PROBE() ((void)0)
+#endif
+#ifndef FAV_PLAIN_EXIT_PROBE
+#define FAV_PLAIN_EXIT_PROBE(reason) ((void)0)
+#endif
+constexpr double kPlainLow = 0x1p-100;
+constexpr double kPlainHigh = 0x1p100;
+constexpr double kPlainTiny = 0x1p-320;
+constexpr double kOperandLow = 0x1p-450;
+constexpr double kOperandHigh = 0x1p500;
+constexpr double kOperandSpan = 0x1p-420;
+
+FAV_HD double min_abs(double x, double y) { return __builtin_fmin(fabs_(x), fabs_(y)); }
+
+// the scaled delta's bounds
+FAV_HD bool plain_delta_ok(double re, double im) {
+    const double hi = max_abs(re, im), lo = min_abs(re, im);
+    return hi >= kPlainLow && hi <= kPlainHigh && (lo == 0.0 || lo >= kPlainTiny);
+}
+
+// dc's bounds, from its normalised mantissas (the span test runs on them,
+// so a component that scaling would flush to zero is caught) and its
+// scaled value
+FAV_HD bool plain_operand_ok(SC v, double re, double im) {
+    const double span_hi = max_abs(v.re, v.im), span_lo = min_abs(v.re, v.im);
+    const double hi = max_abs(re, im), lo = min_abs(re, im);
+    return (span_lo == 0.0 || span_lo >= span_hi * kOperandSpan) && hi >= kOperandLow && hi <= kOperandHigh
+        && (lo == 0.0 || lo >= kOperandLow);
+}
+
+// Ballot of `v` over the active lanes, and the active lanes themselves (a
+// single pixel is its own one-lane wave on the host).
+FAV_HD uint64_t wave_mask(bool v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(v);
+#else
+    return v ? 1ULL : 0ULL;
+#endif
+}
+
+FAV_HD uint64_t wave_active() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_read_exec();
+#else
+    return 1ULL;
+#endif
+}
+
+enum PlainExit : int {
+    kPlainNotEntered = 0,  // the state is unchanged
+    kPlainLoopTop = 1,     // the state is a loop top at s.ref with a negligible delta
+    kPlainInterior = 2,    // ... whose step reaches the cap: the pixel is interior
+};
+
+template <bool Planes, bool Stats>
+FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, Counters<Stats>& st, uint8_t* dlog,
+                                  DerivativeDebt& debt) {
+    FAV_PLAIN_EXIT_PROBE(14);
+    const int max_iter = p.max_iter;
+    const int levels = p.linear_dir.count;
+    const RungDev* rungs = p.rungs;
+    if (rungs == nullptr || levels == 0 || p.approximation_order == 2) return kPlainNotEntered;
+    const int r0 = wave_first(s.ref);
+    const int it0 = wave_first(s.iteration);
+    const bool entry = s.iteration < max_iter && sc_normal(s.delta) && s.ref == r0 && s.iteration == it0;
+    if (!wave_all(entry)) { FAV_PLAIN_EXIT_PROBE(0); return kPlainNotEntered; }
+    if (r0 <= 0 || r0 >= p.orbit.size) return kPlainNotEntered;
+    int E = wave_first(s.delta.e);
+    // |delta|^2 = |m|^2 * 2^2E with |m| >= 2^-100 stays a normal double
+    // while E >= -410; |delta| < |Z| keeps it finite
+    if (E < -410 || E > (1 << 20)) return kPlainNotEntered;
+    double mr = ldexp_(s.delta.re, s.delta.e - E), mi = ldexp_(s.delta.im, s.delta.e - E);
+    double cr = ldexp_(dc.re, dc.e - E), ci = ldexp_(dc.im, dc.e - E);
+    double mx = max_abs(mr, mi);
+    RungDev rung = load_uniform(rungs, r0);
+    // delta's span from its mantissas, as for dc: 2^-220 keeps the smaller
+    // component of m at or above 2^-320.  Delta must be negligible at r0
+    // (delta.e <= small_exp: the larger component below 2^small_exp;
+    // small_d = 0 also stands for the glitch / escape flags and the
+    // reference's escape), as it is at every index a block ends on, so that
+    // the loop-top values at the exit index are the orbit's own.
+    const double dspan_hi = max_abs(s.delta.re, s.delta.im), dspan_lo = min_abs(s.delta.re, s.delta.im);
+    if (!wave_all(plain_delta_ok(mr, mi) && (dspan_lo == 0.0 || dspan_lo >= dspan_hi * 0x1p-220)
+                  && plain_operand_ok(dc, cr, ci) && ldexp_(mx, E) < rung.small_d)) {
+        FAV_PLAIN_EXIT_PROBE(1);
+        return kPlainNotEntered;
+    }
+    // From here every exit writes the whole loop-top state (delta, the
+    // derivative, the indices and the orbit's loop-top values at r), so
+    // nothing of the entry state stays live across the loop.
+    // Order 1 (renderer.cpp:13252-13273): |delta|^2 < 2^-161 when the
+    // approximation order is 3; order 2 never takes a linear block.
+    const double order_bound = p.approximation_order >= 3 ? 0x1p-161 : __builtin_inf();
+    const int max_length = p.max_linear_length;
+    // the rungs start at the tables' level cap, which must bound the
+    // lookup's highest level (floor(log2) of the length limit)
+    const int top_cap = p.plain_top;
+    if (highest_level(max_length) > top_cap) return kPlainNotEntered;
+    const PlainStepDev* steps = p.plain_steps;
+    int r = r0, iteration = it0;
+    double sq = mr * mr + mi * mi;
+    // slot 0's step at r, loaded with the rung: the top level is the common
+    // choice, so its coefficients are usually in hand when the level is
+    PlainStepDev step0 = load_uniform(steps, static_cast<int64_t>(r) * kRungSlots);
+    SC derivative = s.derivative;  // the derivative at log_start
+    int log_start = r, logged = 0;
+    bool committed = false;
+    for (;;) {
+        // small_d > 0 at r (checked at entry and at every commit) means a
+        // block may start here; the loop-top tests of run_normal_blocks that
+        // depend on |delta|^2: |Z|^2 >= |delta|^2 and order 1
+        if (iteration >= max_iter) { FAV_PLAIN_EXIT_PROBE(2); break; }
+        const double dd = ldexp_(sq, 2 * E);
+        const uint64_t active = wave_active();
+        if ((wave_mask(dd <= rung.zn_d) & wave_mask(dd < order_bound)) != active) {
+            FAV_PLAIN_EXIT_PROBE(4);
+            break;
+        }
+        // BLA level: the highest aligned level at or below log2 of the
+        // limit whose radius every lane's |delta|^2 is below
+        const int remaining = max_iter - iteration;
+        const int limit_level = highest_level(max_length < remaining ? max_length : remaining);
+        const int offset = r - 1;
+        int top_level = levels - 1 < top_cap ? levels - 1 : top_cap;
+        if (offset != 0 && __builtin_ctz(static_cast<unsigned>(offset)) < top_level)
+            top_level = __builtin_ctz(static_cast<unsigned>(offset));
+        int slot = -1;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+        for (int k = 0; k < kRungSlots; ++k) {
+            const int l = top_level - k;
+            if (l < 1) break;
+            if (l > limit_level) continue;
+            const double t = rung.r2[k];
+            const uint64_t fits = wave_mask(dd < fabs_(t));
+            if (fits == 0) continue;
+            if (fits == active && !__builtin_signbit(t)) slot = k;  // else: lanes disagree, or not plain
+            break;
+        }
+        if (slot < 0) { FAV_PLAIN_EXIT_PROBE(9); break; }
+        const int level = top_level - slot;
+        const int length = 1 << level;
+        const int end = r + length;
+        if (end > max_iter) { FAV_PLAIN_EXIT_PROBE(10); break; }
+        const PlainStepDev step = slot == 0 ? step0 : load_uniform(steps, static_cast<int64_t>(r) * kRungSlots + slot);
+        const RungDev next = load_uniform(rungs, end);
+        const PlainStepDev next0 = load_uniform(steps, static_cast<int64_t>(end) * kRungSlots);
+        // A*delta + B*dc, each product in sc_mul's operation order
+        const double pr = step.a_re * mr - step.a_im * mi;
+        const double pi = step.a_re * mi + step.a_im * mr;
+        const double qr = step.b_re * cr - step.b_im * ci;
+        const double qi = step.b_re * ci + step.b_im * cr;
+        double nr = pr + qr, ni = pi + qi;
+        const double pm = max_abs(pr, pi), qm = max_abs(qr, qi);
+        if (wave_mask(__builtin_fmax(pm, qm) <= __builtin_fmin(pm, qm) * 0x1p60) != active) {
+            FAV_PLAIN_EXIT_PROBE(12);
+            break;
+        }
+        int shift = 0;
+        double nx = max_abs(nr, ni);
+        const double nn = min_abs(nr, ni);
+        if ((wave_mask(nx >= kPlainLow) & wave_mask(nx <= kPlainHigh) & wave_mask(nn == 0.0 || nn >= kPlainTiny))
+            != active) {
+            // rescale the wave by its first lane's magnitude
+            shift = wave_first(frexp_exp_nz(nx));
+            nr = ldexp_(nr, -shift);
+            ni = ldexp_(ni, -shift);
+            nx = max_abs(nr, ni);
+            const double tr = ldexp_(cr, -shift), ti = ldexp_(ci, -shift);
+            if (!wave_all(plain_delta_ok(nr, ni) && plain_operand_ok(dc, tr, ti)) || E + shift < -410) {
+                FAV_PLAIN_EXIT_PROBE(13);
+                break;
+            }
+            cr = tr;
+            ci = ti;
+        }
+        // negligible at the end index, the condition of the reference's
+        // loop-top values there being the orbit's
+        if (wave_mask(ldexp_(nx, E + shift) < next.small_d) != active) {
+            FAV_PLAIN_EXIT_PROBE(11);
+            break;
+        }
+        // commit, renderer.cpp:13499-13511
+        E += shift;
+        mr = nr;
+        mi = ni;
+        mx = nx;
+        sq = mr * mr + mi * mi;
+        if constexpr (Planes) {
+            if (logged == kDerivativeLog) {
+                derivative = replay_derivative(p, dlog, logged, log_start, derivative);
+                logged = 0;
+                log_start = r;
+            }
+            dlog[logged++] = static_cast<uint8_t>(level);
+        }
+        iteration += length;
+        r = end;
+        rung = next;
+        step0 = next0;
+        committed = true;
+        FAV_PLAIN_BLOCK_PROBE();
+        st.add(ST_BLA_BLOCKS);
+        st.add(ST_LINEAR_BLOCKS);
+        st.add(ST_SERIES_JUMPS);
+        if constexpr (Stats) st.add(ST_HISTOGRAM + (level < 15 ? level : 15));
+        st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
+    }
+    // At one iteration below the cap no block fits (lengths are at least 2),
+    // so the loop top takes an exact step to the cap, and the pixel is
+    // interior, when none of its tests fires: no glitch / escape / map end
+    // (small_d > 0, as checked for r), no rebase (|Z|^2 >= |delta|^2), order
+    // 1 and no plain-double tail (|delta|^2 < 2^-90; renderer.cpp:13252-
+    // 13290).  That pixel reads no derivative.
+    const double dd_end = ldexp_(sq, 2 * E);
+    const bool interior = iteration == max_iter - 1
+        && wave_all(dd_end <= rung.zn_d && dd_end < order_bound && dd_end < 0x1p-90);
+    if constexpr (Planes) {
+        if (logged > 0 && !interior) {
+            if (wave_all(iteration >= max_iter - 1)) {
+                debt.count = logged;
+                debt.start = log_start;
+            } else {
+                derivative = replay_derivative(p, dlog, logged, log_start, derivative);
+            }
+        }
+    }
+    // the loop-top state at r: delta is negligible there, so Z + delta, its
+    // norm and the margin are the orbit's own values (the state a committed
+    // block leaves, renderer.cpp:13499-13511, with the previous norm equal
+    // to the new one); without a commit they are the values the loop top
+    // at r0 computes, and the previous norm stays
+    const OrbitFastDev top = load_uniform(p.orbit.fast, r);
+    const SN norm{top.zn_m, top.zn_e};
+    const int k = frexp_exp_nz(mx);
+    s.delta = SC{ldexp_(mr, -k), ldexp_(mi, -k), E + k};
+    s.derivative = derivative;
+    s.iteration = iteration;
+    s.ref = r;
+    s.total = SC{top.z_re, top.z_im, top.z_e};
+    s.total_norm = norm;
+    s.margin = FE{top.margin_m, top.margin_e};
+    if (committed) s.previous_total_norm = norm;
+    s.have_total = true;
+    return interior ? kPlainInterior : kPlainLoopTop;
+}
+
+// ---- the pixel on the reference --------------------------------------------
+// lookup_level (renderer.cpp:8816-8877) with |delta|^2 = 0, through scalar
+// loads of the fields it reads: the highest aligned level whose step is
+// longer than one step and has a finite, nonzero radius.
+template <typename Step>
+FAV_HD int lookup_level_zero(const Step* steps, const LevelDir& d, int start, int max_length) {
+    if (start <= 0 || max_length <= 0 || d.count == 0 || start > d.size[0]) return -1;
+    int h = highest_level(max_length);
+    if (h > d.count - 1) h = d.count - 1;
+    const int offset = start - 1;
+    for (int level = h; level >= 0; --level) {
+        if ((offset & ((1 << level) - 1)) != 0) continue;
+        const int index = offset >> level;
+        if (index >= d.size[level]) continue;
+        const int64_t k = d.offset[level] + index;
+        const int length = load_uniform(&steps[k].length, 0);
+        const double r2 = load_uniform(&steps[k].r2_m, 0);
+        if (length > 1 && isfinite_(r2) && r2 != 0.0) return static_cast<int>(k);
+    }
+    return -1;
+}
+
+// The loop of linear_loop for a pixel sitting on the reference (delta = dc
+// = 0): Z + delta is Z, so every loop top takes the orbit's own values (the
+// glitch and escape tests are the orbit's flags, |delta|^2 = 0 never
+// rebases), the lookup runs with |delta|^2 = 0 in the table of the
+// approximation order (order 1: linear; otherwise the cubic table, whose
+// block is A*0 + B*0 = 0 as in linear_loop), and delta stays zero.  Table
+// reads are scalar; any other branch returns to linear_loop at the same loop
+// top.
+template <bool Planes, bool Stats>
+FAV_HD void run_reference_blocks(const RenderParams& p, PixelState& s, Counters<Stats>& st) {
+    const OrbitDev& o = p.orbit;
+    const int max_iter = p.max_iter;
+    int r = wave_first(s.ref);
+    int iteration = wave_first(s.iteration);
+    if (!wave_all(s.ref == r && s.iteration == iteration && sc_is_zero(s.delta))) return;
+    const bool linear = p.approximation_order <= 1;
+    SC derivative = s.derivative;
+    bool committed = false;
+    for (;;) {
+        if (iteration >= max_iter || r < 0 || r >= o.size) break;
+        if ((load_uniform(&o.fast[r].flags, 0) & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) break;
+        if (o.map_end > 1 && r >= o.map_end) break;
+        const int remaining = max_iter - iteration;
+        int k, length;
+        SC a, b;
+        if (linear) {
+            k = lookup_level_zero(p.linear, p.linear_dir, r,
+                                  p.max_linear_length < remaining ? p.max_linear_length : remaining);
+            if (k < 0) break;
+            const LinearStepDev step = load_uniform(p.linear, k);
+            length = step.length;
+            a = SC{step.a_re, step.a_im, step.a_e};
+            b = SC{step.b_re, step.b_im, step.b_e};
+        } else {
+            k = lookup_level_zero(p.cubic, p.cubic_dir, r, p.max_bla_length < remaining ? p.max_bla_length : remaining);
+            if (k < 0) break;
+            const CubicStepDev* step = p.cubic + k;
+            length = load_uniform(&step->length, 0);
+            if constexpr (Planes) {
+                a = SC{load_uniform(&step->re[0], 0), load_uniform(&step->im[0], 0), load_uniform(&step->e[0], 0)};
+                b = SC{load_uniform(&step->re[1], 0), load_uniform(&step->im[1], 0), load_uniform(&step->e[1], 0)};
+            }
+        }
+        const int end = r + length;
+        if (end > max_iter) break;
+        // the endpoint's loop-top norm is |Z|^2 there (renderer.cpp:13422)
+        if ((load_uniform(&o.fast[end].flags, 0) & kOrbitEndpointFlag) != 0) break;
+        if constexpr (Planes) derivative = sc_add(sc_mul(a, derivative), b);
+        // commit, renderer.cpp:13499-13511
+        r = end;
+        iteration += length;
+        committed = true;
+        st.add(ST_BLA_BLOCKS);
+        st.add(linear ? ST_LINEAR_BLOCKS : ST_CUBIC_BLOCKS);
+        st.add(ST_SERIES_JUMPS);
+        if constexpr (Stats) {
+            int bucket = 0;
+            unsigned v = static_cast<unsigned>(length);
+            while (v > 1U && bucket < 15) { v >>= 1U; ++bucket; }
+            st.add(ST_HISTOGRAM + bucket);
+        }
+        st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
+    }
+    if (committed) {
+        const OrbitFastDev top = load_uniform(o.fast, r);
+        const SN norm{top.zn_m, top.zn_e};
+        s.derivative = derivative;
+        s.iteration = iteration;
+        s.ref = r;
+        s.total = SC{top.z_re, top.z_im, top.z_e};
+        s.total_norm = norm;
+        s.margin = FE{top.margin_m, top.margin_e};
+        s.previous_total_norm = norm;
+        s.have_total = true;
+    }
+}
+
+// The plain pass over a freshly set-up pixel (init_pixel): the plain-double
+// block loop from the pixel's first loop top.  Returns true when the pixel is
+// finished (interior at the cap: output and statistics written); otherwise
+// `s` is a loop-top state to resume with run_linear, its derivative settled.
+template <bool Planes, bool Stats>
+FAV_HD bool plain_pixel(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st,
+                        uint8_t* dlog) {
+    const int max_iter = p.max_iter;
+    if (!p.fast_linear || !sc_normal(dc) || p.disable_bla || s.pixel_disable_bla) return false;
+    DerivativeDebt debt;
+    const PlainExit exit = run_plain_blocks<Planes, Stats>(p, s, dc, st, dlog, debt);
+    if (exit == kPlainInterior) {  // the exact step to the cap, renderer.cpp:13516-13539
+        s.iteration = max_iter;
+        st.add(ST_LOGICAL);
+        st.add(ST_EXACT);
+    }
+    if (s.iteration >= max_iter) {
+        p.output[index] = encode_iteration(max_iter, p.output_bias);
+        st.max(ST_MAX_PIXEL_ITER, static_cast<unsigned long long>(s.iteration));
+        return true;
+    }
+    settle_derivative<Planes>(p, dlog, s, debt);
+    return false;
+}
+
 // Fast path: the loop of renderer.cpp:12889-13536 restricted to iterations
 // that end in a committed linear BLA block or in an exact perturbation step.
 // Any other branch (glitch, escape, rebase, reference escape, cubic order,
@@ -903,6 +1298,9 @@ FAV_HD Outcome linear_loop(const RenderParams& p, size_t index, PixelState& s, S
         if (normal_blocks) {
             settle_derivative<Planes>(p, dlog, s, debt);
             run_normal_blocks<Planes, Stats>(p, s, dc, st, dlog, debt);
+            if (s.iteration >= max_iter) break;
+        } else if (sc_is_zero(dc) && !p.disable_bla && !s.pixel_disable_bla) {
+            run_reference_blocks<Planes, Stats>(p, s, st);
             if (s.iteration >= max_iter) break;
         }
         if (s.ref < 0 || s.ref >= o.size) return kHandoff;

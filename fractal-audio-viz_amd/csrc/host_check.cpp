@@ -11,8 +11,34 @@
 #include <type_traits>
 #include <vector>
 
+// Commits of the plain-double block loop, counted by the CPU build only
+// (tests read how much of a field the loop covers).
+static unsigned long long g_plain_blocks = 0;
+#define FAV_PLAIN_BLOCK_PROBE() __atomic_fetch_add(&g_plain_blocks, 1ULL, __ATOMIC_RELAXED)
+static unsigned long long g_plain_exits[16] = {};
+#define FAV_PLAIN_EXIT_PROBE(reason) __atomic_fetch_add(&g_plain_exits[(reason) & 15], 1ULL, __ATOMIC_RELAXED)
+
 #include "field_core.h"
 #include "table_image.h"
+
+// Tests switch the plain-double block loop off to compare both block loops
+// against the same goldens.
+static int g_plain_enabled = 1;
+static bool plain_blocks_enabled() { return g_plain_enabled != 0; }
+
+extern "C" void fav_host_check_set_plain(int enabled) { g_plain_enabled = enabled; }
+// exits of the plain-double block loop by reason (field_core.h
+// FAV_PLAIN_EXIT_PROBE codes); reset clears them
+extern "C" unsigned long long fav_host_check_plain_exits(int reason, int reset) {
+    const unsigned long long v = g_plain_exits[reason & 15];
+    if (reset) g_plain_exits[reason & 15] = 0;
+    return v;
+}
+extern "C" unsigned long long fav_host_check_plain_blocks(int reset) {
+    const unsigned long long v = g_plain_blocks;
+    if (reset) g_plain_blocks = 0;
+    return v;
+}
 
 extern "C" int fav_host_check_render(
     const char* x, const char* y, const char* ref_zoom, int max_iter, int bits, int image_series_order,
@@ -57,6 +83,16 @@ extern "C" int fav_host_check_render(
         p.series.e = img.series_e.data();
         p.linear = img.linear[prep.linear_tier].data();
         p.linear_dir = img.linear_dir[prep.linear_tier];
+        std::vector<RungDev> rungs;
+        std::vector<PlainStepDev> plain_steps;
+        if (plain_blocks_enabled()) {
+            p.plain_top = 31 - __builtin_clz(static_cast<unsigned>(p.max_linear_length));
+            make_plain_tables(c, prep.linear_tier, p.plain_top, rungs, plain_steps);
+            if (!rungs.empty()) {
+                p.rungs = rungs.data();
+                p.plain_steps = plain_steps.data();
+            }
+        }
         p.cubic = img.cubic.data();
         p.cubic_dir = img.cubic_dir;
         p.x_off_m = prep.offset_m.data();
@@ -82,9 +118,11 @@ extern "C" int fav_host_check_render(
         const long long n = static_cast<long long>(width) * height;
         std::vector<unsigned char> state_block(static_cast<size_t>(n) * kStateBytesPerPixel + 20 * 256);
         bind_state_store(p, state_block.data(), static_cast<size_t>(n));
-        // The three kernels of render_kernel.hip, run one after another on
-        // the CPU: fast pass over every pixel, then the general queue, then
-        // the tail queue.
+        // The kernels of render_kernel.hip, run one after another on the
+        // CPU: the plain pass (when the tables exist), the fast pass over
+        // every pixel it did not finish, then the general queue, then the
+        // tail queue.
+        p.plain_pass = p.rungs != nullptr && p.fast_linear ? 1 : 0;
         std::vector<char> outcome(static_cast<size_t>(n), 0);
         auto pass = [&](auto&& body) {
 #pragma omp parallel for schedule(dynamic, 64)
@@ -93,13 +131,29 @@ extern "C" int fav_host_check_render(
         auto run = [&](auto planes_tag, auto stats_tag) {
             constexpr bool Planes = decltype(planes_tag)::value;
             constexpr bool Stats = decltype(stats_tag)::value;
+            if (p.plain_pass) {
+                pass([&](size_t i) {
+                    const int py = static_cast<int>(i / static_cast<size_t>(width));
+                    const int px = static_cast<int>(i % static_cast<size_t>(width));
+                    core::Counters<Stats> st;
+                    st.clear();
+                    const SC dc = core::pixel_dc(p, px, py);
+                    core::PixelState s = core::init_pixel<Planes, Stats>(p, i, dc, st);
+                    uint8_t dlog[core::kDerivativeLog];
+                    if (core::plain_pixel<Planes, Stats>(p, i, s, dc, st, dlog)) p.state.iter[i] = p.max_iter;
+                    else core::save_state(p.state, i, s);
+                    if constexpr (Stats) st.flush(p.stats);
+                });
+            }
             pass([&](size_t i) {
+                if (p.plain_pass && p.state.iter[i] >= p.max_iter) return;
                 const int py = static_cast<int>(i / static_cast<size_t>(width));
                 const int px = static_cast<int>(i % static_cast<size_t>(width));
                 core::Counters<Stats> st;
                 st.clear();
                 const SC dc = core::pixel_dc(p, px, py);
-                core::PixelState s = core::init_pixel<Planes, Stats>(p, i, dc, st);
+                core::PixelState s = p.plain_pass ? core::load_state(p.state, i)
+                                                  : core::init_pixel<Planes, Stats>(p, i, dc, st);
                 uint8_t dlog[core::kDerivativeLog];
                 if (core::run_linear<Planes, Stats>(p, i, s, dc, st, dlog) == core::kHandoff) {
                     core::save_state(p.state, i, s);

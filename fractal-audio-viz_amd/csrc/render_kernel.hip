@@ -3,8 +3,11 @@
 // A field is rendered by three kernels on one stream (field_core.h holds the
 // per-pixel code):
 //
-//   fast_kernel     every pixel: set-up, image-series jump, then the
-//                   linear-BLA loop.  A 256-thread block covers a 16x16 tile
+//   plain_kernel    every pixel: set-up, image-series jump, then the
+//                   plain-double block loop; interior pixels finish here
+//                   (run only when the reference has plain-double tables).
+//   fast_kernel     every pixel the plain pass did not finish (or every
+//                   pixel, set up here, without it): the linear-BLA loop.  A 256-thread block covers a 16x16 tile
 //                   as four 8x8 wavefronts, so a wave's lanes share their
 //                   reference index and BLA level and the table loads are
 //                   wave-uniform.  Interior pixels finish here; any other
@@ -50,6 +53,35 @@ __device__ unsigned wave_grab(unsigned* cursor, unsigned count) {
     return __shfl(base, 0);
 }
 
+// The plain pass: pixel set-up, image-series jump and the plain-double block
+// loop (field_core.h run_plain_blocks) on the same 16x16 tiles as
+// fast_kernel.  Interior pixels finish here; every other pixel's loop-top
+// state goes to the state store for fast_kernel.  This kernel holds only the
+// plain loop's registers, not those of the general per-lane loop.
+template <bool Planes, bool Stats>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) plain_kernel(RenderParams p) {
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
+    const bool valid = px < p.width && py < p.height && (!p.point_re || index < static_cast<size_t>(p.point_count));
+    uint8_t* dlog = nullptr;
+    if constexpr (Planes) {
+        __shared__ uint8_t derivative_log[4][core::kDerivativeLog];
+        dlog = derivative_log[wave];
+    }
+    if (valid) {
+        core::Counters<Stats> st;
+        st.clear();
+        const SC dc = core::pixel_dc(p, px, py);
+        core::PixelState s = core::init_pixel<Planes, Stats>(p, index, dc, st);
+        if (core::plain_pixel<Planes, Stats>(p, index, s, dc, st, dlog)) p.state.iter[index] = p.max_iter;
+        else core::save_state(p.state, index, s);
+        if constexpr (Stats) st.flush(p.stats);
+    }
+}
+
 // Occupancy targets measured on the 1920x1080 zoom-1e100 field: 6 waves per
 // SIMD (80 VGPRs) with planes, 7 (72 VGPRs) without; the spills they cost sit
 // in the per-pixel outer loop, not in the block loop.
@@ -60,7 +92,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Planes
     const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
     const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
-    const bool valid = px < p.width && py < p.height && (!p.point_re || index < static_cast<size_t>(p.point_count));
+    bool valid = px < p.width && py < p.height && (!p.point_re || index < static_cast<size_t>(p.point_count));
+    // pixels the plain pass finished
+    if (valid && p.plain_pass && p.state.iter[index] >= p.max_iter) valid = false;
     bool handoff = false;
     uint8_t* dlog = nullptr;
     if constexpr (Planes) {
@@ -72,7 +106,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Planes
         core::Counters<Stats> st;
         st.clear();
         const SC dc = core::pixel_dc(p, px, py);
-        core::PixelState s = core::init_pixel<Planes, Stats>(p, index, dc, st);
+        core::PixelState s = p.plain_pass ? core::load_state(p.state, index)
+                                          : core::init_pixel<Planes, Stats>(p, index, dc, st);
         if (core::run_linear<Planes, Stats>(p, index, s, dc, st, dlog) == core::kHandoff) {
             core::save_state(p.state, index, s);
             handoff = true;
@@ -144,7 +179,9 @@ template <bool Planes, bool Stats>
 void launch_all(const RenderParams& p, int persistent_blocks, hipStream_t stream) {
     prologue_kernel<<<1, 64, 0, stream>>>(p.state.counters, const_cast<unsigned long long*>(p.budget_start),
                                           p.budget_flag);
-    fast_kernel<Planes, Stats><<<dim3((p.width + 15) / 16, (p.height + 15) / 16), dim3(256), 0, stream>>>(p);
+    const dim3 tiles((p.width + 15) / 16, (p.height + 15) / 16);
+    if (p.plain_pass) plain_kernel<Planes, Stats><<<tiles, dim3(256), 0, stream>>>(p);
+    fast_kernel<Planes, Stats><<<tiles, dim3(256), 0, stream>>>(p);
     general_kernel<Planes, Stats><<<dim3(persistent_blocks), dim3(256), 0, stream>>>(p);
     tail_kernel<Planes, Stats><<<dim3(persistent_blocks), dim3(256), 0, stream>>>(p);
 }

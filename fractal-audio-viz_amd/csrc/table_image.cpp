@@ -24,6 +24,40 @@ void flatten(const std::vector<std::vector<Step>>& levels, std::vector<Step>& fl
     }
 }
 
+// Operand bounds of the plain-double block loop: every nonzero component of
+// A, B (and of the loop's scaled delta and dc, checked on the device) lies in
+// [2^-450, 2^500], and the smaller nonzero component of a value is at least
+// 2^-420 times the larger.  Under these bounds every product, sum and
+// cancellation of the block step A*delta + B*dc is a normal double both in
+// plain form and in the reference's per-operand normalised form, so scaling
+// by powers of two commutes with every rounding.
+bool plain_operand(double re, double im, int e) {
+    const double r = std::ldexp(re, e), i = std::ldexp(im, e);
+    const double hi = std::max(std::fabs(r), std::fabs(i)), lo = std::min(std::fabs(r), std::fabs(i));
+    if (!std::isfinite(hi) || hi > 0x1p500 || hi < 0x1p-450) return false;
+    return lo == 0.0 || (lo >= 0x1p-450 && lo >= hi * 0x1p-420);
+}
+
+// x * 2^e as a plain double, 0 below the normal range and +inf above it (a
+// validity radius, |Z|^2 or the negligible-delta bound of the block loop,
+// compared with |delta|^2 values that the loop keeps in the normal range).
+double plain_threshold(double m, int e) {
+    if (!std::isfinite(m)) return HUGE_VAL;
+    if (m == 0.0) return 0.0;
+    const double v = std::ldexp(m, e);
+    return std::fabs(v) < 0x1p-1022 ? 0.0 : v;
+}
+
+// Lookup and fast-path flags of flattened linear steps (LinearStepDev::pad[0]).
+void mark_linear_steps(std::vector<LinearStepDev>& flat) {
+    for (LinearStepDev& s : flat) {
+        int32_t f = 0;
+        if (s.length > 1 && std::isfinite(s.r2_m) && s.r2_m != 0.0) f |= kStepUsable;
+        if (sc_normal(SC{s.a_re, s.a_im, s.a_e}) && sc_normal(SC{s.b_re, s.b_im, s.b_e})) f |= kStepNormalCoefficients;
+        s.pad[0] = f;
+    }
+}
+
 }  // namespace
 
 void make_table_image(const ReferenceContext& c, TableImage& img) {
@@ -55,12 +89,7 @@ void make_table_image(const ReferenceContext& c, TableImage& img) {
     flatten(c.cubic.levels, img.cubic, img.cubic_dir);
     for (int t = 0; t < 2; ++t) {
         flatten(c.linear[t].levels, img.linear[t], img.linear_dir[t]);
-        for (LinearStepDev& s : img.linear[t]) {  // lookup and fast-path flags
-            int32_t f = 0;
-            if (s.length > 1 && std::isfinite(s.r2_m) && s.r2_m != 0.0) f |= kStepUsable;
-            if (sc_normal(SC{s.a_re, s.a_im, s.a_e}) && sc_normal(SC{s.b_re, s.b_im, s.b_e})) f |= kStepNormalCoefficients;
-            s.pad[0] = f;
-        }
+        mark_linear_steps(img.linear[t]);
     }
     img.series_re.clear();
     img.series_im.clear();
@@ -71,6 +100,59 @@ void make_table_image(const ReferenceContext& c, TableImage& img) {
         img.series_e.push_back(k.e);
     }
     flatten(c.alt.levels, img.alt, img.alt_dir);
+}
+
+// The rung and plain-step tables of one linear tier for the level cap
+// top_cap (fav_device.h).
+void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, std::vector<RungDev>& rungs,
+                       std::vector<PlainStepDev>& steps) {
+    const OrbitData& o = *c.orbit;
+    const size_t n = o.scaled.size();
+    rungs.clear();
+    steps.clear();
+    if (o.zn_m.size() != n || o.flags.size() != n || o.small_exp.size() != n) return;
+    std::vector<LinearStepDev> flat;
+    LevelDir dir{};
+    flatten(c.linear[tier].levels, flat, dir);
+    mark_linear_steps(flat);
+    const int map_end = c.map_end;
+    steps.assign(n * kRungSlots, PlainStepDev{});
+    rungs.assign(n, RungDev{});
+    for (size_t r = 0; r < n; ++r) {
+        RungDev& g = rungs[r];
+        // past the level-0 steps (lookup start > size[0], renderer.cpp:
+        // 8816-8877) no block starts: stop there too
+        const bool stop = (o.flags[r] & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0
+            || (map_end > 1 && r >= static_cast<size_t>(map_end)) || r == 0 || dir.count == 0
+            || r > static_cast<size_t>(dir.size[0]);
+        g.small_d = stop ? 0.0 : std::ldexp(1.0, o.small_exp[r]);
+        g.zn_d = plain_threshold(o.zn_m[r], o.zn_e[r]);
+        if (r == 0 || dir.count == 0) continue;
+        const unsigned offset = static_cast<unsigned>(r - 1);
+        int top = std::min(dir.count - 1, top_cap);
+        if (offset != 0) top = std::min(top, __builtin_ctz(offset));
+        for (int k = 0; k < kRungSlots; ++k) {
+            const int level = top - k;
+            if (level < 1) break;
+            const int64_t index = offset >> level;
+            if (index >= dir.size[level]) continue;  // slot stays 0: no lane fits
+            const LinearStepDev& s = flat[static_cast<size_t>(dir.offset[level] + index)];
+            if ((s.pad[0] & kStepUsable) == 0) continue;
+            double r2 = plain_threshold(s.r2_m, s.r2_e);
+            const size_t end = r + static_cast<size_t>(s.length);
+            const bool plain = s.length == (1 << level) && plain_operand(s.a_re, s.a_im, s.a_e)
+                && plain_operand(s.b_re, s.b_im, s.b_e) && end < n && (o.flags[end] & kOrbitEndpointFlag) == 0;
+            if (plain) {
+                steps[r * kRungSlots + static_cast<size_t>(k)] = PlainStepDev{
+                    std::ldexp(s.a_re, s.a_e), std::ldexp(s.a_im, s.a_e), std::ldexp(s.b_re, s.b_e),
+                    std::ldexp(s.b_im, s.b_e)};
+            } else {
+                r2 = -r2;  // sign bit: fitting this step ends the plain loop
+                if (r2 == 0.0) r2 = -0.0;
+            }
+            g.r2[k] = r2;
+        }
+    }
 }
 
 void prepare_alternate(const ReferenceContext& c, const FieldRequest& r, PreparedAlternate& out) {

@@ -41,13 +41,17 @@ def field_1e12_640x360():
 
 # Algorithmic FP64 operations per event (the roofline numerator).  Complex
 # multiply 6, complex add 2, squared norm 3; normalisation, exponent and
-# comparison work is not counted.
+# comparison work is not counted.  A linear block is A*d + B*dc (14) plus
+# the |d|^2 that the BLA level lookup compares with the validity radii
+# before every block (renderer.cpp:8816-8877), 3: the loop-top glitch,
+# escape and endpoint tests of a block are properties of the reference
+# index alone while d is negligible, so they are not per-pixel work.
 FLOPS = {
     "exact_step": 16 + 16,    # d' = 2Zd + d^2 + dc, plus |Z+d|^2 margin and |d|^2
     "replay_step": 16 + 16,
     "tail_step": 19,          # plain-double tail step (renderer.cpp:12527-12554)
-    "linear_block": 14 + 16,  # A*d + B*dc, plus endpoint margin and |d|^2
-    "cubic_block": 70 + 16,   # order-3 Horner (renderer.cpp:8710-8730)
+    "linear_block": 14 + 3,   # A*d + B*dc, plus |d|^2 for the level lookup
+    "cubic_block": 70 + 16,   # order-3 Horner (renderer.cpp:8710-8730) with its margin
     "derivative": 10,         # dz' = 2 z dz + 1, per step when planes are written
 }
 

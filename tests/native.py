@@ -20,6 +20,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent.parent
 PRODUCT_SO = ROOT / "fractal-audio-viz_amd" / "libfractal_hip.so"
 ORACLE_SO = ROOT / "oracle" / "_build" / "libfav_oracle.so"
+REFERENCE_RELEASE_SO = ROOT / "oracle" / "_ref" / "mandelbrot_release.so"
 REFERENCE_SO = ROOT / "oracle" / "_ref" / "mandelbrot.so"
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
@@ -228,11 +229,14 @@ def render_direct_oracle(lib, case, threads=0):
     return out.reshape(h, w), planes
 
 
-def load_reference():
-    """The reference renderer.cpp built from source, or None if absent."""
-    if not REFERENCE_SO.exists():
+def load_reference(release=False):
+    """The reference renderer.cpp built from source, or None if absent.
+    release=True: the build with the reference Makefile's own flags (FMA
+    contraction on), used only as the CPU timing baseline."""
+    path = REFERENCE_RELEASE_SO if release else REFERENCE_SO
+    if not path.exists():
         return None
-    return _bind_render_abi(ctypes.CDLL(str(REFERENCE_SO)))
+    return _bind_render_abi(ctypes.CDLL(str(path)))
 
 
 def load_oracle():
