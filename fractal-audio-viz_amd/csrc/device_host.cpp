@@ -25,7 +25,7 @@
 namespace fav {
 
 hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, int persistent_blocks,
-                               hipStream_t stream);
+                               hipStream_t stream, hipStream_t side, hipEvent_t fork, hipEvent_t join);
 hipError_t launch_kfp_kernel(const kfp::KfpParams& params, void* scratch, int compute_units, hipStream_t stream);
 hipError_t launch_direct_kernel(const direct::DirectParams& p, bool planes, hipStream_t stream);
 hipError_t launch_alt_kernel(const AltRenderParams& p, hipStream_t stream);
@@ -73,6 +73,8 @@ struct LaunchState {
     bool ready = false;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // default priority, like `stream`
+    hipEvent_t fork = nullptr, join = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
     hipEvent_t mark[3] = {nullptr, nullptr, nullptr};
     DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords, fx_counts, tile_order, work,
@@ -96,6 +98,9 @@ void init_locked(LaunchState& s) {
     if (count <= 0) throw std::runtime_error("no HIP device is visible");
     check(hipGetDevice(&s.device), "hipGetDevice");
     check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipEventCreateWithFlags(&s.fork, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&s.join, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreate(&s.start), "hipEventCreate");
     check(hipEventCreate(&s.stop), "hipEventCreate");
     for (hipEvent_t& e : s.mark) check(hipEventCreate(&e), "hipEventCreate");
@@ -306,10 +311,18 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     p.y_off_e = p.x_off_e + w;
     PointStaging points;
     size_t out_count = pixels;
+    p.reference_pixel = -1;
     if (r.points) {
         upload_points(s, r, points, p.point_re, p.point_im, p.point_e);
         p.point_count = r.point_count;
         out_count = static_cast<size_t>(r.point_count);
+        // a point with dc = 0 stays on fast_kernel's path
+    } else if (p.plain_pass) {
+        // the pixel whose column and row offsets are both zero (dc = 0)
+        int zx = -1, zy = -1;
+        for (int x = 0; x < w && zx < 0; ++x) if (off_m[static_cast<size_t>(x)] == 0.0) zx = x;
+        for (int y = 0; y < h && zy < 0; ++y) if (off_m[static_cast<size_t>(w + y)] == 0.0) zy = y;
+        if (zx >= 0 && zy >= 0) p.reference_pixel = static_cast<int64_t>(zy) * w + zx;
     }
 
     const bool want_planes = r.planes != nullptr;
@@ -350,7 +363,8 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
 
     const uint64_t wall0 = now_ns();
     check(hipEventRecord(s.start, s.stream), "event");
-    check(launch_field_kernel(p, want_planes, r.stats != nullptr, s.compute_units * 8, s.stream),
+    check(launch_field_kernel(p, want_planes, r.stats != nullptr, s.compute_units * 8, s.stream, s.side, s.fork,
+                              s.join),
           "field kernel launch");
     check(hipEventRecord(s.stop, s.stream), "event");
     if (!r.device_pointers) {

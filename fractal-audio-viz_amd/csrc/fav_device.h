@@ -258,6 +258,10 @@ struct RenderParams {
     // state store (state.iter >= max_iter marks a pixel it finished)
     int32_t plain_pass;
     int32_t plain_top;  // level cap the rung tables were built for
+    // with the plain pass: the one grid pixel whose dc is zero (it sits on
+    // the reference), rendered by reference_pixel_kernel on a side stream
+    // while the plain pass runs; -1 when there is none
+    int64_t reference_pixel;
 };
 
 // One batched launch over several fields (batch_kernel.hip).

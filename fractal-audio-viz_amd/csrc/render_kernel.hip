@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
         __shared__ uint8_t derivative_log[4][core::kDerivativeLog];
         dlog = derivative_log[wave];
     }
-    if (valid) {
+    if (valid && static_cast<int64_t>(index) != p.reference_pixel) {
         core::Counters<Stats> st;
         st.clear();
         const SC dc = core::pixel_dc(p, px, py);
@@ -80,6 +80,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
         else core::save_state(p.state, index, s);
         if constexpr (Stats) st.flush(p.stats);
     }
+}
+
+// The pixel on the reference (dc = 0) when the plain pass runs: set-up and
+// the linear loop (its wave-uniform on-reference block loop) for that one
+// pixel, on a side stream alongside plain_kernel, so that its long chain of
+// single-lane blocks does not trail the plain pass.  A pixel it does not
+// finish goes to the general queue like fast_kernel's.  One wave.
+template <bool Planes, bool Stats>
+__global__ void __launch_bounds__(64) reference_pixel_kernel(RenderParams p) {
+    const size_t index = static_cast<size_t>(p.reference_pixel);
+    const bool valid = lane_id() == 0;
+    bool handoff = false;
+    if (valid) {
+        const int py = static_cast<int>(index / static_cast<size_t>(p.width));
+        const int px = static_cast<int>(index - static_cast<size_t>(py) * static_cast<size_t>(p.width));
+        core::Counters<Stats> st;
+        st.clear();
+        const SC dc = core::pixel_dc(p, px, py);
+        core::PixelState s = core::init_pixel<Planes, Stats>(p, index, dc, st);
+        if (core::run_linear<Planes, Stats>(p, index, s, dc, st, nullptr) == core::kHandoff) {
+            core::save_state(p.state, index, s);
+            handoff = true;
+        }
+        if constexpr (Stats) st.flush(p.stats);
+    }
+    wave_push(p.state.general_queue, p.state.counters + 0, static_cast<uint32_t>(index), handoff);
 }
 
 // Occupancy targets measured on the 1920x1080 zoom-1e100 field: 6 waves per
@@ -93,8 +119,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Planes
     const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
     const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
     bool valid = px < p.width && py < p.height && (!p.point_re || index < static_cast<size_t>(p.point_count));
-    // pixels the plain pass finished
-    if (valid && p.plain_pass && p.state.iter[index] >= p.max_iter) valid = false;
+    // pixels the plain pass finished, and the one reference_pixel_kernel renders
+    if (valid && p.plain_pass && (static_cast<int64_t>(index) == p.reference_pixel || p.state.iter[index] >= p.max_iter))
+        valid = false;
     bool handoff = false;
     uint8_t* dlog = nullptr;
     if constexpr (Planes) {
@@ -176,32 +203,47 @@ __global__ void prologue_kernel(unsigned* counters, unsigned long long* budget_s
 }
 
 template <bool Planes, bool Stats>
-void launch_all(const RenderParams& p, int persistent_blocks, hipStream_t stream) {
+hipError_t launch_all(const RenderParams& p, int persistent_blocks, hipStream_t stream, hipStream_t side,
+                      hipEvent_t fork, hipEvent_t join) {
     prologue_kernel<<<1, 64, 0, stream>>>(p.state.counters, const_cast<unsigned long long*>(p.budget_start),
                                           p.budget_flag);
     const dim3 tiles((p.width + 15) / 16, (p.height + 15) / 16);
+    const bool side_pixel = p.plain_pass && p.reference_pixel >= 0;
+    hipError_t e = hipSuccess;
+    if (side_pixel) {
+        // after the prologue (queue counters cleared), beside the plain pass
+        if ((e = hipEventRecord(fork, stream)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(side, fork, 0)) != hipSuccess) return e;
+        reference_pixel_kernel<Planes, Stats><<<1, 64, 0, side>>>(p);
+        if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
+    }
     if (p.plain_pass) plain_kernel<Planes, Stats><<<tiles, dim3(256), 0, stream>>>(p);
     fast_kernel<Planes, Stats><<<tiles, dim3(256), 0, stream>>>(p);
+    if (side_pixel && (e = hipStreamWaitEvent(stream, join, 0)) != hipSuccess) return e;
     general_kernel<Planes, Stats><<<dim3(persistent_blocks), dim3(256), 0, stream>>>(p);
     tail_kernel<Planes, Stats><<<dim3(persistent_blocks), dim3(256), 0, stream>>>(p);
+    return hipSuccess;
 }
 
 }  // namespace
 
-// Launches the field kernels on `stream`.  The caller validated the shapes:
-// width*height <= 1e8 and every pointer in `p` (outputs, planes, state
-// store) sized for width x height.  `persistent_blocks` is the grid of the
-// queue-driven kernels.
+// Launches the field kernels on `stream` (and, for the pixel on the
+// reference during a plain pass, on `side`, joined back before the general
+// kernel through the events `fork` and `join`).  The caller validated the
+// shapes: width*height <= 1e8 and every pointer in `p` (outputs, planes,
+// state store) sized for width x height.  `persistent_blocks` is the grid of
+// the queue-driven kernels.
 hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, int persistent_blocks,
-                               hipStream_t stream) {
+                               hipStream_t stream, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+    hipError_t e;
     if (planes) {
-        if (stats) launch_all<true, true>(p, persistent_blocks, stream);
-        else launch_all<true, false>(p, persistent_blocks, stream);
+        e = stats ? launch_all<true, true>(p, persistent_blocks, stream, side, fork, join)
+                  : launch_all<true, false>(p, persistent_blocks, stream, side, fork, join);
     } else {
-        if (stats) launch_all<false, true>(p, persistent_blocks, stream);
-        else launch_all<false, false>(p, persistent_blocks, stream);
+        e = stats ? launch_all<false, true>(p, persistent_blocks, stream, side, fork, join)
+                  : launch_all<false, false>(p, persistent_blocks, stream, side, fork, join);
     }
-    return hipGetLastError();
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 }  // namespace fav

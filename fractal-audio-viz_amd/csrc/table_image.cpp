@@ -270,6 +270,7 @@ void prepare_render(const ReferenceContext& c, const FieldRequest& r, PreparedRe
     p.approximation_order = std::clamp(r.series_order, 1, 3);
     // the linear fast kernel skips the per-step budget clock and cycle checks
     p.fast_linear = !(p.cycle_enabled || opt.time_budget_ms > 0);
+    p.reference_pixel = -1;
 }
 
 }  // namespace fav
