@@ -927,6 +927,13 @@ FAV_HD bool plain_delta_ok(double re, double im) {
     return hi >= kPlainLow && hi <= kPlainHigh && (lo == 0.0 || lo >= kPlainTiny);
 }
 
+// the operand bounds of a value whose span is already known to be within
+// them (a rescaled delta or dc)
+FAV_HD bool plain_value_ok(double re, double im) {
+    const double hi = max_abs(re, im), lo = min_abs(re, im);
+    return hi >= kOperandLow && hi <= kOperandHigh && (lo == 0.0 || lo >= kOperandLow);
+}
+
 // dc's bounds, from its normalised mantissas (the span test runs on them,
 // so a component that scaling would flush to zero is caught) and its
 // scaled value
@@ -949,7 +956,9 @@ FAV_HD uint64_t wave_mask(bool v) {
 
 FAV_HD uint64_t wave_active() {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_read_exec();
+    // a ballot (uniform to the compiler's divergence analysis) rather than a
+    // read of the exec mask, so that tests against it are uniform branches
+    return __builtin_amdgcn_ballot_w64(true);
 #else
     return 1ULL;
 #endif
@@ -960,6 +969,32 @@ enum PlainExit : int {
     kPlainLoopTop = 1,     // the state is a loop top at s.ref with a negligible delta
     kPlainInterior = 2,    // ... whose step reaches the cap: the pixel is interior
 };
+
+// A wave-uniform double kept in scalar registers: readfirstlane of both
+// halves tells the compiler the value is uniform (a no-op on a value that
+// is already scalar), so loop-carried table records stay out of VGPRs.
+FAV_HD double uniform_double(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(b & 0xffffffffULL)));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32)));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+#else
+    return v;
+#endif
+}
+
+FAV_HD RungDev uniform_rung(RungDev g) {
+    for (int k = 0; k < kRungSlots; ++k) g.r2[k] = uniform_double(g.r2[k]);
+    g.small_d = uniform_double(g.small_d);
+    g.zn_d = uniform_double(g.zn_d);
+    return g;
+}
+
+FAV_HD PlainStepDev uniform_step(PlainStepDev t) {
+    return PlainStepDev{uniform_double(t.a_re), uniform_double(t.a_im), uniform_double(t.b_re),
+                        uniform_double(t.b_im)};
+}
 
 template <bool Planes, bool Stats>
 FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, Counters<Stats>& st, uint8_t* dlog,
@@ -981,7 +1016,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     double mr = ldexp_(s.delta.re, s.delta.e - E), mi = ldexp_(s.delta.im, s.delta.e - E);
     double cr = ldexp_(dc.re, dc.e - E), ci = ldexp_(dc.im, dc.e - E);
     double mx = max_abs(mr, mi);
-    RungDev rung = load_uniform(rungs, r0);
+    const double entry_small = load_uniform(&rungs[r0].small_d, 0);
     // delta's span from its mantissas, as for dc: 2^-220 keeps the smaller
     // component of m at or above 2^-320.  Delta must be negligible at r0
     // (delta.e <= small_exp: the larger component below 2^small_exp;
@@ -990,7 +1025,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     // the loop-top values at the exit index are the orbit's own.
     const double dspan_hi = max_abs(s.delta.re, s.delta.im), dspan_lo = min_abs(s.delta.re, s.delta.im);
     if (!wave_all(plain_delta_ok(mr, mi) && (dspan_lo == 0.0 || dspan_lo >= dspan_hi * 0x1p-220)
-                  && plain_operand_ok(dc, cr, ci) && ldexp_(mx, E) < rung.small_d)) {
+                  && plain_operand_ok(dc, cr, ci) && ldexp_(mx, E) < entry_small)) {
         FAV_PLAIN_EXIT_PROBE(1);
         return kPlainNotEntered;
     }
@@ -1008,113 +1043,129 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     const PlainStepDev* steps = p.plain_steps;
     int r = r0, iteration = it0;
     double sq = mr * mr + mi * mi;
-    // slot 0's step at r, loaded with the rung: the top level is the common
-    // choice, so its coefficients are usually in hand when the level is
-    PlainStepDev step0 = load_uniform(steps, static_cast<int64_t>(r) * kRungSlots);
     SC derivative = s.derivative;  // the derivative at log_start
     int log_start = r, logged = 0;
     bool committed = false;
+    // The block loop runs at a fixed scale E.  A result outside m's bounds
+    // ends it without a commit; the committed delta and dc are then rescaled
+    // by the result's magnitude (first lane) and the block is tried again.
+    // A rescaled m only needs the operand bounds (its span is unchanged),
+    // and |delta|^2 is unchanged, so every bound above still holds.
+    bool retried = false;
     for (;;) {
-        // small_d > 0 at r (checked at entry and at every commit) means a
-        // block may start here; the loop-top tests of run_normal_blocks that
-        // depend on |delta|^2: |Z|^2 >= |delta|^2 and order 1
-        if (iteration >= max_iter) { FAV_PLAIN_EXIT_PROBE(2); break; }
-        const double dd = ldexp_(sq, 2 * E);
-        const uint64_t active = wave_active();
-        if ((wave_mask(dd <= rung.zn_d) & wave_mask(dd < order_bound)) != active) {
-            FAV_PLAIN_EXIT_PROBE(4);
-            break;
-        }
-        // BLA level: the highest aligned level at or below log2 of the
-        // limit whose radius every lane's |delta|^2 is below
-        const int remaining = max_iter - iteration;
-        const int limit_level = highest_level(max_length < remaining ? max_length : remaining);
-        const int offset = r - 1;
-        int top_level = levels - 1 < top_cap ? levels - 1 : top_cap;
-        if (offset != 0 && __builtin_ctz(static_cast<unsigned>(offset)) < top_level)
-            top_level = __builtin_ctz(static_cast<unsigned>(offset));
-        int slot = -1;
+        int shift = 0;
+        bool rescale = false;
+        for (;;) {
+            // small_d > 0 at r (checked at entry and at every commit) means a
+            // block may start here; the loop-top tests of run_normal_blocks
+            // that depend on |delta|^2: |Z|^2 >= |delta|^2 and order 1
+            if (iteration >= max_iter) { FAV_PLAIN_EXIT_PROBE(2); break; }
+            // slot 0's step (the top level is the common choice) and the rung
+            // at r, read afresh each block: the other waves of the CU walk the
+            // same indices, so these are scalar-cache hits
+            PlainStepDev step = uniform_step(load_uniform(steps, static_cast<int64_t>(r) * kRungSlots));
+            const RungDev rung = uniform_rung(load_uniform(rungs, r));
+            const double dd = ldexp_(sq, 2 * E);
+            const uint64_t active = wave_active();
+            if ((wave_mask(dd <= rung.zn_d) & wave_mask(dd < order_bound)) != active) {
+                FAV_PLAIN_EXIT_PROBE(4);
+                break;
+            }
+            // BLA level: the highest aligned level at or below log2 of the
+            // limit whose radius every lane's |delta|^2 is below
+            const int remaining = max_iter - iteration;
+            const int limit_level = highest_level(max_length < remaining ? max_length : remaining);
+            const int offset = r - 1;
+            int top_level = levels - 1 < top_cap ? levels - 1 : top_cap;
+            if (offset != 0 && __builtin_ctz(static_cast<unsigned>(offset)) < top_level)
+                top_level = __builtin_ctz(static_cast<unsigned>(offset));
+            int slot = -1;
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
-        for (int k = 0; k < kRungSlots; ++k) {
-            const int l = top_level - k;
-            if (l < 1) break;
-            if (l > limit_level) continue;
-            const double t = rung.r2[k];
-            const uint64_t fits = wave_mask(dd < fabs_(t));
-            if (fits == 0) continue;
-            if (fits == active && !__builtin_signbit(t)) slot = k;  // else: lanes disagree, or not plain
-            break;
-        }
-        if (slot < 0) { FAV_PLAIN_EXIT_PROBE(9); break; }
-        const int level = top_level - slot;
-        const int length = 1 << level;
-        const int end = r + length;
-        if (end > max_iter) { FAV_PLAIN_EXIT_PROBE(10); break; }
-        const PlainStepDev step = slot == 0 ? step0 : load_uniform(steps, static_cast<int64_t>(r) * kRungSlots + slot);
-        const RungDev next = load_uniform(rungs, end);
-        const PlainStepDev next0 = load_uniform(steps, static_cast<int64_t>(end) * kRungSlots);
-        // A*delta + B*dc, each product in sc_mul's operation order
-        const double pr = step.a_re * mr - step.a_im * mi;
-        const double pi = step.a_re * mi + step.a_im * mr;
-        const double qr = step.b_re * cr - step.b_im * ci;
-        const double qi = step.b_re * ci + step.b_im * cr;
-        double nr = pr + qr, ni = pi + qi;
-        const double pm = max_abs(pr, pi), qm = max_abs(qr, qi);
-        if (wave_mask(__builtin_fmax(pm, qm) <= __builtin_fmin(pm, qm) * 0x1p60) != active) {
-            FAV_PLAIN_EXIT_PROBE(12);
-            break;
-        }
-        int shift = 0;
-        double nx = max_abs(nr, ni);
-        const double nn = min_abs(nr, ni);
-        if ((wave_mask(nx >= kPlainLow) & wave_mask(nx <= kPlainHigh) & wave_mask(nn == 0.0 || nn >= kPlainTiny))
-            != active) {
-            // rescale the wave by its first lane's magnitude
-            shift = wave_first(frexp_exp_nz(nx));
-            nr = ldexp_(nr, -shift);
-            ni = ldexp_(ni, -shift);
-            nx = max_abs(nr, ni);
-            const double tr = ldexp_(cr, -shift), ti = ldexp_(ci, -shift);
-            if (!wave_all(plain_delta_ok(nr, ni) && plain_operand_ok(dc, tr, ti)) || E + shift < -410) {
-                FAV_PLAIN_EXIT_PROBE(13);
+            for (int k = 0; k < kRungSlots; ++k) {
+                const int l = top_level - k;
+                if (l < 1) break;
+                if (l > limit_level) continue;
+                const double t = rung.r2[k];
+                const uint64_t fits = wave_mask(dd < fabs_(t));
+                if (fits == 0) continue;
+                if (fits == active && !__builtin_signbit(t)) slot = k;  // else: lanes disagree, or not plain
                 break;
             }
-            cr = tr;
-            ci = ti;
+            if (slot < 0) { FAV_PLAIN_EXIT_PROBE(9); break; }
+            const int level = top_level - slot;
+            const int length = 1 << level;
+            const int end = r + length;
+            if (end > max_iter) { FAV_PLAIN_EXIT_PROBE(10); break; }
+            if (slot != 0) step = uniform_step(load_uniform(steps, static_cast<int64_t>(r) * kRungSlots + slot));
+            const double next_small = uniform_double(load_uniform(&rungs[end].small_d, 0));
+            // A*delta + B*dc, each product in sc_mul's operation order
+            const double pr = step.a_re * mr - step.a_im * mi;
+            const double pi = step.a_re * mi + step.a_im * mr;
+            const double qr = step.b_re * cr - step.b_im * ci;
+            const double qi = step.b_re * ci + step.b_im * cr;
+            const double nr = pr + qr, ni = pi + qi;
+            const double pm = max_abs(pr, pi), qm = max_abs(qr, qi);
+            if (wave_mask(__builtin_fmax(pm, qm) <= __builtin_fmin(pm, qm) * 0x1p60) != active) {
+                FAV_PLAIN_EXIT_PROBE(12);
+                break;
+            }
+            const double nx = max_abs(nr, ni), nn = min_abs(nr, ni);
+            if ((wave_mask(nx >= kPlainLow) & wave_mask(nx <= kPlainHigh)
+                 & (wave_mask(nn == 0.0) | wave_mask(nn >= kPlainTiny))) != active) {
+                shift = wave_first(frexp_exp_nz(nx));
+                rescale = true;
+                break;
+            }
+            // negligible at the end index, the condition of the reference's
+            // loop-top values there being the orbit's
+            if (wave_mask(ldexp_(nx, E) < next_small) != active) {
+                FAV_PLAIN_EXIT_PROBE(11);
+                break;
+            }
+            // commit, renderer.cpp:13499-13511
+            mr = nr;
+            mi = ni;
+            mx = nx;
+            sq = mr * mr + mi * mi;
+            if constexpr (Planes) {
+                if (logged == kDerivativeLog) {
+                    derivative = replay_derivative(p, dlog, logged, log_start, derivative);
+                    logged = 0;
+                    log_start = r;
+                }
+                dlog[logged++] = static_cast<uint8_t>(level);
+            }
+            iteration += length;
+            r = end;
+            committed = true;
+            retried = false;
+            FAV_PLAIN_BLOCK_PROBE();
+            st.add(ST_BLA_BLOCKS);
+            st.add(ST_LINEAR_BLOCKS);
+            st.add(ST_SERIES_JUMPS);
+            if constexpr (Stats) st.add(ST_HISTOGRAM + (level < 15 ? level : 15));
+            st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
         }
-        // negligible at the end index, the condition of the reference's
-        // loop-top values there being the orbit's
-        if (wave_mask(ldexp_(nx, E + shift) < next.small_d) != active) {
-            FAV_PLAIN_EXIT_PROBE(11);
+        if (!rescale || retried || E + shift < -410) {
+            if (rescale) FAV_PLAIN_EXIT_PROBE(13);
             break;
         }
-        // commit, renderer.cpp:13499-13511
-        E += shift;
-        mr = nr;
-        mi = ni;
-        mx = nx;
-        sq = mr * mr + mi * mi;
-        if constexpr (Planes) {
-            if (logged == kDerivativeLog) {
-                derivative = replay_derivative(p, dlog, logged, log_start, derivative);
-                logged = 0;
-                log_start = r;
-            }
-            dlog[logged++] = static_cast<uint8_t>(level);
+        const double sr = ldexp_(mr, -shift), si = ldexp_(mi, -shift);
+        const double tr = ldexp_(cr, -shift), ti = ldexp_(ci, -shift);
+        if (!wave_all(plain_value_ok(sr, si) && plain_value_ok(tr, ti))) {
+            FAV_PLAIN_EXIT_PROBE(13);
+            break;
         }
-        iteration += length;
-        r = end;
-        rung = next;
-        step0 = next0;
-        committed = true;
-        FAV_PLAIN_BLOCK_PROBE();
-        st.add(ST_BLA_BLOCKS);
-        st.add(ST_LINEAR_BLOCKS);
-        st.add(ST_SERIES_JUMPS);
-        if constexpr (Stats) st.add(ST_HISTOGRAM + (level < 15 ? level : 15));
-        st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
+        mr = sr;
+        mi = si;
+        cr = tr;
+        ci = ti;
+        E += shift;
+        mx = max_abs(mr, mi);
+        sq = mr * mr + mi * mi;
+        retried = true;
     }
     // At one iteration below the cap no block fits (lengths are at least 2),
     // so the loop top takes an exact step to the cap, and the pixel is
@@ -1124,7 +1175,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     // 13290).  That pixel reads no derivative.
     const double dd_end = ldexp_(sq, 2 * E);
     const bool interior = iteration == max_iter - 1
-        && wave_all(dd_end <= rung.zn_d && dd_end < order_bound && dd_end < 0x1p-90);
+        && wave_all(dd_end <= load_uniform(&rungs[r].zn_d, 0) && dd_end < order_bound && dd_end < 0x1p-90);
     if constexpr (Planes) {
         if (logged > 0 && !interior) {
             if (wave_all(iteration >= max_iter - 1)) {
