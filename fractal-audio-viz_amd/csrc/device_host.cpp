@@ -132,7 +132,6 @@ DeviceTables::~DeviceTables() {
     if (alt_block) (void)hipFree(alt_block);
     if (fast_block) (void)hipFree(fast_block);
     for (auto& tier : rung_block) for (void* p : tier) if (p) (void)hipFree(p);
-    for (auto& tier : plain_block) for (void* p : tier) if (p) (void)hipFree(p);
 }
 
 namespace {
@@ -217,27 +216,23 @@ void ensure_device_tables(ReferenceContext& c) {
     c.device = std::move(t);
 }
 
-// The plain-double block loop tables of linear tier `tier` for level cap
-// `cap` (table_image.cpp make_plain_tables), built and copied to HBM on first
-// use; null pointers when the reference has none.
-void ensure_plain_tables(ReferenceContext& c, int tier, int cap, const RungDev*& rungs,
-                         const PlainStepDev*& steps) {
+// The plain-double block loop's rung table of linear tier `tier` for level
+// cap `cap` (table_image.cpp make_plain_tables), built and copied to HBM on
+// first use; null when the reference has none.
+void ensure_plain_tables(ReferenceContext& c, int tier, int cap, const RungDev*& rungs) {
     std::lock_guard<std::mutex> lock(c.device_mutex);
     DeviceTables& t = *c.device;
     if (!t.plain_built[tier][cap]) {
         std::vector<RungDev> r;
-        std::vector<PlainStepDev> st;
-        make_plain_tables(c, tier, cap, r, st);
+        make_plain_tables(c, tier, cap, r);
         if (!r.empty()) {
             LaunchState& s = state();
             upload_vector(r, t.rung_block[tier][cap], t.rungs[tier][cap], s.stream, t.bytes);
-            upload_vector(st, t.plain_block[tier][cap], t.plain[tier][cap], s.stream, t.bytes);
             check(hipStreamSynchronize(s.stream), "plain table upload");
         }
         t.plain_built[tier][cap] = true;
     }
     rungs = t.rungs[tier][cap];
-    steps = t.plain[tier][cap];
 }
 
 // One deep field render on the device, host part of renderer.cpp:12672-13583.
@@ -287,7 +282,7 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     {
         // floor(log2) of the block length limit (at least 2)
         const int cap = 31 - __builtin_clz(static_cast<unsigned>(p.max_linear_length));
-        ensure_plain_tables(c, prep.linear_tier, cap, p.rungs, p.plain_steps);
+        ensure_plain_tables(c, prep.linear_tier, cap, p.rungs);
         p.plain_top = cap;
         p.plain_pass = p.rungs != nullptr && p.fast_linear ? 1 : 0;
     }

@@ -53,37 +53,43 @@ struct alignas(64) OrbitFastDev {
 static_assert(sizeof(OrbitFastDev) == 64, "orbit record is one cache line");
 
 // The plain-double block loop (field_core.h run_plain_blocks) reads one
-// linear BLA tier through two tables built from it, both indexed by the
-// block start r:
+// linear BLA tier through one table built from it (table_image.cpp
+// make_plain_tables): a 256-byte RungDev per block start r, so that every
+// read of a block comes from one base address.
 //
-// * RungDev, one per r, everything the loop reads at r:
-//   - r2[k]: the validity radius of the step of level top - k that starts
-//     at r, as a plain double; top is the highest level aligned at r, at
-//     most the tables' level cap (RenderParams::plain_top).  A slot the
-//     reference's lookup skips (length 1, non-finite or zero r2, past the
-//     level's end) holds 0, so no lane fits it.  A step the loop may not
-//     apply carries the sign bit, so that fitting it ends the loop: A or B
-//     outside the operand bounds, a block shorter than 2^level, or an end
-//     index where |Z|^2 >= 3 (renderer.cpp:13422);
-//   - small_d: 2^small_exp[r], the negligible-delta bound, or 0 where the
-//     loop must stop at r (the glitch or escape test fires for a
-//     negligible delta, r is at or past the reference's escape, or no
-//     block starts at r: r past the level-0 steps);
-//   - zn_d: |Z_r|^2 (0 below, +inf above the normal range).
-// * PlainStepDev, kRungSlots per r: A and B of slot k's step as plain
-//   doubles (their exact values) at r * kRungSlots + k.
-constexpr int kRungSlots = 6;
-struct alignas(64) RungDev {
-    double r2[kRungSlots];
-    double small_d;
-    double zn_d;
-};
-static_assert(sizeof(RungDev) == 64, "rung is one cache line");
-
+// * r2[k]: the validity radius of the step of level top - k that starts at
+//   r, as a plain double; top is the highest level aligned at r, at most the
+//   table's level cap (RenderParams::plain_top).  A slot the reference's
+//   lookup skips (length 1, non-finite or zero r2, past the level's end)
+//   holds 0, so no lane fits it.  A step the loop may not apply carries the
+//   sign bit, so that fitting it ends the loop: A or B outside the operand
+//   bounds, a block shorter than 2^level, or an end index where |Z|^2 >= 3
+//   (renderer.cpp:13422).
+// * small_d: 2^small_exp[r], the negligible-delta bound at r, or 0 where the
+//   loop must stop at r (the glitch or escape test fires for a negligible
+//   delta, r is at or past the reference's escape, or no block starts at r:
+//   r past the level-0 steps); small_top: small_d of slot 0's end index
+//   r + 2^top (the common next index), so that a block reads no other record.
+// * zn_d: |Z_r|^2 (0 below, +inf above the normal range).
+// * step[k]: A and B of slot k's step as plain doubles (their exact values).
 struct alignas(32) PlainStepDev {
     double a_re, a_im, b_re, b_im;
 };
 static_assert(sizeof(PlainStepDev) == 32, "plain step is half a cache line");
+
+constexpr int kRungSlots = 5;
+struct alignas(256) RungDev {
+    double r2[kRungSlots];
+    double small_top;
+    double zn_d;
+    int32_t top;
+    int32_t pad0;
+    PlainStepDev step[kRungSlots];
+    double small_d;
+    double pad1[3];
+};
+static_assert(sizeof(RungDev) == 256, "rung is four cache lines");
+static_assert(offsetof(RungDev, step) == 64, "the block's header is one cache line");
 
 // Device copy of one reference (orbit + tables), structure of arrays.  The
 // per-index FloatExp parts of Z, the reference escape margin |Z|^2-R^2 and
@@ -253,7 +259,6 @@ struct RenderParams {
     // plain-double block loop tables of the selected linear tier (rungs null:
     // the loop is off and the scaled-operand block loop runs instead)
     const RungDev* rungs;
-    const PlainStepDev* plain_steps;
     // 1: plain_kernel ran first; fast_kernel resumes each pixel from the
     // state store (state.iter >= max_iter marks a pixel it finished)
     int32_t plain_pass;

@@ -67,9 +67,7 @@ struct DeviceTables {
     // plain-double block loop tables per linear tier and level cap, built on
     // first use (ensure_plain_tables); null until then
     void* rung_block[2][kMaxLevels] = {};
-    void* plain_block[2][kMaxLevels] = {};
     const RungDev* rungs[2][kMaxLevels] = {};
-    const PlainStepDev* plain[2][kMaxLevels] = {};
     bool plain_built[2][kMaxLevels] = {};
     const AltStepDev* alt = nullptr;
     const double* alt_margin_m = nullptr;  // inside orbit_block

@@ -972,7 +972,7 @@ enum PlainExit : int {
 
 // A wave-uniform double kept in scalar registers: readfirstlane of both
 // halves tells the compiler the value is uniform (a no-op on a value that
-// is already scalar), so loop-carried table records stay out of VGPRs.
+// is already scalar).
 FAV_HD double uniform_double(double v) {
 #if defined(__HIP_DEVICE_COMPILE__)
     const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
@@ -984,16 +984,33 @@ FAV_HD double uniform_double(double v) {
 #endif
 }
 
-FAV_HD RungDev uniform_rung(RungDev g) {
-    for (int k = 0; k < kRungSlots; ++k) g.r2[k] = uniform_double(g.r2[k]);
-    g.small_d = uniform_double(g.small_d);
-    g.zn_d = uniform_double(g.zn_d);
-    return g;
+// A step read field by field through scalar loads.
+FAV_HD PlainStepDev load_step(const PlainStepDev* t) {
+    return PlainStepDev{uniform_double(load_uniform(&t->a_re, 0)), uniform_double(load_uniform(&t->a_im, 0)),
+                        uniform_double(load_uniform(&t->b_re, 0)), uniform_double(load_uniform(&t->b_im, 0))};
 }
 
-FAV_HD PlainStepDev uniform_step(PlainStepDev t) {
-    return PlainStepDev{uniform_double(t.a_re), uniform_double(t.a_im), uniform_double(t.b_re),
-                        uniform_double(t.b_im)};
+// A uniform radius slot: fits (no lane at or above it) and applicable (sign
+// bit clear), read as an integer test of the high word.
+FAV_HD bool slot_plain(double t) {
+    return static_cast<int>(__builtin_bit_cast(unsigned long long, t) >> 32) >= 0;
+}
+
+// The first cache line of a rung: what a block reads besides its step.
+struct RungHead {
+    double r2[kRungSlots];
+    double small_top;
+    double zn_d;
+    int32_t top;
+};
+
+FAV_HD RungHead load_head(const RungDev* g) {
+    RungHead h;
+    for (int k = 0; k < kRungSlots; ++k) h.r2[k] = uniform_double(load_uniform(&g->r2[k], 0));
+    h.small_top = uniform_double(load_uniform(&g->small_top, 0));
+    h.zn_d = uniform_double(load_uniform(&g->zn_d, 0));
+    h.top = load_uniform(&g->top, 0);
+    return h;
 }
 
 template <bool Planes, bool Stats>
@@ -1001,9 +1018,8 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
                                   DerivativeDebt& debt) {
     FAV_PLAIN_EXIT_PROBE(14);
     const int max_iter = p.max_iter;
-    const int levels = p.linear_dir.count;
     const RungDev* rungs = p.rungs;
-    if (rungs == nullptr || levels == 0 || p.approximation_order == 2) return kPlainNotEntered;
+    if (rungs == nullptr || p.linear_dir.count == 0 || p.approximation_order == 2) return kPlainNotEntered;
     const int r0 = wave_first(s.ref);
     const int it0 = wave_first(s.iteration);
     const bool entry = s.iteration < max_iter && sc_normal(s.delta) && s.ref == r0 && s.iteration == it0;
@@ -1013,10 +1029,14 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     // |delta|^2 = |m|^2 * 2^2E with |m| >= 2^-100 stays a normal double
     // while E >= -410; |delta| < |Z| keeps it finite
     if (E < -410 || E > (1 << 20)) return kPlainNotEntered;
+    // the rungs start at the table's level cap, which must bound the
+    // lookup's highest level (floor(log2) of the length limit)
+    const int max_length = p.max_linear_length;
+    const int full_level = highest_level(max_length);
+    if (full_level > p.plain_top) return kPlainNotEntered;
     double mr = ldexp_(s.delta.re, s.delta.e - E), mi = ldexp_(s.delta.im, s.delta.e - E);
     double cr = ldexp_(dc.re, dc.e - E), ci = ldexp_(dc.im, dc.e - E);
     double mx = max_abs(mr, mi);
-    const double entry_small = load_uniform(&rungs[r0].small_d, 0);
     // delta's span from its mantissas, as for dc: 2^-220 keeps the smaller
     // component of m at or above 2^-320.  Delta must be negligible at r0
     // (delta.e <= small_exp: the larger component below 2^small_exp;
@@ -1025,7 +1045,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     // the loop-top values at the exit index are the orbit's own.
     const double dspan_hi = max_abs(s.delta.re, s.delta.im), dspan_lo = min_abs(s.delta.re, s.delta.im);
     if (!wave_all(plain_delta_ok(mr, mi) && (dspan_lo == 0.0 || dspan_lo >= dspan_hi * 0x1p-220)
-                  && plain_operand_ok(dc, cr, ci) && ldexp_(mx, E) < entry_small)) {
+                  && plain_operand_ok(dc, cr, ci) && ldexp_(mx, E) < load_uniform(&rungs[r0].small_d, 0))) {
         FAV_PLAIN_EXIT_PROBE(1);
         return kPlainNotEntered;
     }
@@ -1035,12 +1055,8 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     // Order 1 (renderer.cpp:13252-13273): |delta|^2 < 2^-161 when the
     // approximation order is 3; order 2 never takes a linear block.
     const double order_bound = p.approximation_order >= 3 ? 0x1p-161 : __builtin_inf();
-    const int max_length = p.max_linear_length;
-    // the rungs start at the tables' level cap, which must bound the
-    // lookup's highest level (floor(log2) of the length limit)
-    const int top_cap = p.plain_top;
-    if (highest_level(max_length) > top_cap) return kPlainNotEntered;
-    const PlainStepDev* steps = p.plain_steps;
+    // below this iteration the length limit is max_length itself
+    const int last_full = max_iter - max_length;
     int r = r0, iteration = it0;
     double sq = mr * mr + mi * mi;
     SC derivative = s.derivative;  // the derivative at log_start
@@ -1057,71 +1073,82 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
         bool rescale = false;
         for (;;) {
             // small_d > 0 at r (checked at entry and at every commit) means a
-            // block may start here; the loop-top tests of run_normal_blocks
-            // that depend on |delta|^2: |Z|^2 >= |delta|^2 and order 1
-            if (iteration >= max_iter) { FAV_PLAIN_EXIT_PROBE(2); break; }
-            // slot 0's step (the top level is the common choice) and the rung
-            // at r, read afresh each block: the other waves of the CU walk the
-            // same indices, so these are scalar-cache hits
-            PlainStepDev step = uniform_step(load_uniform(steps, static_cast<int64_t>(r) * kRungSlots));
-            const RungDev rung = uniform_rung(load_uniform(rungs, r));
+            // block may start here.  The rung's first line and slot 0's step
+            // (the top level is the common choice); the other waves of the CU
+            // walk the same indices.
+            const RungDev* g = rungs + r;
+            const RungHead h = load_head(g);
+            PlainStepDev step = load_step(&g->step[0]);
+            // the loop-top tests of run_normal_blocks that depend on
+            // |delta|^2: |Z|^2 >= |delta|^2 and order 1
             const double dd = ldexp_(sq, 2 * E);
             const uint64_t active = wave_active();
-            if ((wave_mask(dd <= rung.zn_d) & wave_mask(dd < order_bound)) != active) {
-                FAV_PLAIN_EXIT_PROBE(4);
-                break;
+            const uint64_t top_ok = wave_mask(dd <= h.zn_d) & wave_mask(dd < order_bound);
+            const uint64_t fits0 = wave_mask(dd < fabs_(h.r2[0]));
+            int level = h.top;
+            double next_small = h.small_top;
+            // The common case, as one integer test: away from the cap (the
+            // length limit is max_length, so the lookup starts at the top
+            // level, and r <= iteration keeps the block below the cap), the
+            // top level at least 1, every lane within slot 0's radius and
+            // the step applicable.
+            const int common = static_cast<int>((top_ok & fits0) == active) & static_cast<int>(iteration <= last_full)
+                & static_cast<int>(level >= 1) & static_cast<int>(slot_plain(h.r2[0]));
+            if (!common) {
+                if (iteration >= max_iter) { FAV_PLAIN_EXIT_PROBE(2); break; }
+                if (top_ok != active) { FAV_PLAIN_EXIT_PROBE(4); break; }
+                // BLA level: the highest aligned level at or below log2 of
+                // the limit whose radius every lane's |delta|^2 is below;
+                // slots above the limit are skipped, and the slots end at
+                // level 1
+                const int top_level = h.top;
+                int limit_level = full_level;
+                if (iteration > last_full) limit_level = highest_level(max_iter - iteration);
+                const int first = top_level > limit_level ? top_level - limit_level : 0;
+                const int slots = top_level < kRungSlots ? top_level : kRungSlots;
+                int slot = -1;
+                for (int k = first; k < slots; ++k) {
+                    const uint64_t fits = wave_mask(dd < fabs_(load_uniform(&g->r2[k], 0)));
+                    if (fits == 0) continue;
+                    // else: lanes disagree, or the step is not plain
+                    if (fits == active && slot_plain(load_uniform(&g->r2[k], 0))) slot = k;
+                    break;
+                }
+                if (slot < 0) { FAV_PLAIN_EXIT_PROBE(9); break; }
+                level = top_level - slot;
+                if (r + (1 << level) > max_iter) { FAV_PLAIN_EXIT_PROBE(10); break; }
+                if (slot != 0) {
+                    step = load_step(&g->step[slot]);
+                    next_small = uniform_double(load_uniform(&rungs[r + (1 << level)].small_d, 0));
+                }
             }
-            // BLA level: the highest aligned level at or below log2 of the
-            // limit whose radius every lane's |delta|^2 is below
-            const int remaining = max_iter - iteration;
-            const int limit_level = highest_level(max_length < remaining ? max_length : remaining);
-            const int offset = r - 1;
-            int top_level = levels - 1 < top_cap ? levels - 1 : top_cap;
-            if (offset != 0 && __builtin_ctz(static_cast<unsigned>(offset)) < top_level)
-                top_level = __builtin_ctz(static_cast<unsigned>(offset));
-            int slot = -1;
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-#endif
-            for (int k = 0; k < kRungSlots; ++k) {
-                const int l = top_level - k;
-                if (l < 1) break;
-                if (l > limit_level) continue;
-                const double t = rung.r2[k];
-                const uint64_t fits = wave_mask(dd < fabs_(t));
-                if (fits == 0) continue;
-                if (fits == active && !__builtin_signbit(t)) slot = k;  // else: lanes disagree, or not plain
-                break;
-            }
-            if (slot < 0) { FAV_PLAIN_EXIT_PROBE(9); break; }
-            const int level = top_level - slot;
             const int length = 1 << level;
             const int end = r + length;
-            if (end > max_iter) { FAV_PLAIN_EXIT_PROBE(10); break; }
-            if (slot != 0) step = uniform_step(load_uniform(steps, static_cast<int64_t>(r) * kRungSlots + slot));
-            const double next_small = uniform_double(load_uniform(&rungs[end].small_d, 0));
             // A*delta + B*dc, each product in sc_mul's operation order
             const double pr = step.a_re * mr - step.a_im * mi;
             const double pi = step.a_re * mi + step.a_im * mr;
             const double qr = step.b_re * cr - step.b_im * ci;
             const double qi = step.b_re * ci + step.b_im * cr;
             const double nr = pr + qr, ni = pi + qi;
+            // the tests of the result, taken together: both products added
+            // (exponent gap at most 60), m's bounds, and negligible at the
+            // end index (the condition of the reference's loop-top values
+            // there being the orbit's)
             const double pm = max_abs(pr, pi), qm = max_abs(qr, qi);
-            if (wave_mask(__builtin_fmax(pm, qm) <= __builtin_fmin(pm, qm) * 0x1p60) != active) {
-                FAV_PLAIN_EXIT_PROBE(12);
-                break;
-            }
+            const uint64_t added = wave_mask(__builtin_fmax(pm, qm) <= __builtin_fmin(pm, qm) * 0x1p60);
             const double nx = max_abs(nr, ni), nn = min_abs(nr, ni);
-            if ((wave_mask(nx >= kPlainLow) & wave_mask(nx <= kPlainHigh)
-                 & (wave_mask(nn == 0.0) | wave_mask(nn >= kPlainTiny))) != active) {
-                shift = wave_first(frexp_exp_nz(nx));
-                rescale = true;
-                break;
-            }
-            // negligible at the end index, the condition of the reference's
-            // loop-top values there being the orbit's
-            if (wave_mask(ldexp_(nx, E) < next_small) != active) {
-                FAV_PLAIN_EXIT_PROBE(11);
+            const uint64_t bounded = wave_mask(nx >= kPlainLow) & wave_mask(nx <= kPlainHigh)
+                & (wave_mask(nn == 0.0) | wave_mask(nn >= kPlainTiny));
+            const uint64_t negligible = wave_mask(ldexp_(nx, E) < next_small);
+            if ((added & bounded & negligible) != active) {
+                if (added != active) {
+                    FAV_PLAIN_EXIT_PROBE(12);
+                } else if (bounded != active) {
+                    shift = wave_first(frexp_exp_nz(nx));
+                    rescale = true;
+                } else {
+                    FAV_PLAIN_EXIT_PROBE(11);
+                }
                 break;
             }
             // commit, renderer.cpp:13499-13511

@@ -84,14 +84,10 @@ extern "C" int fav_host_check_render(
         p.linear = img.linear[prep.linear_tier].data();
         p.linear_dir = img.linear_dir[prep.linear_tier];
         std::vector<RungDev> rungs;
-        std::vector<PlainStepDev> plain_steps;
         if (plain_blocks_enabled()) {
             p.plain_top = 31 - __builtin_clz(static_cast<unsigned>(p.max_linear_length));
-            make_plain_tables(c, prep.linear_tier, p.plain_top, rungs, plain_steps);
-            if (!rungs.empty()) {
-                p.rungs = rungs.data();
-                p.plain_steps = plain_steps.data();
-            }
+            make_plain_tables(c, prep.linear_tier, p.plain_top, rungs);
+            if (!rungs.empty()) p.rungs = rungs.data();
         }
         p.cubic = img.cubic.data();
         p.cubic_dir = img.cubic_dir;

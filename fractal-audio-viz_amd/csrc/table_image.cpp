@@ -102,21 +102,18 @@ void make_table_image(const ReferenceContext& c, TableImage& img) {
     flatten(c.alt.levels, img.alt, img.alt_dir);
 }
 
-// The rung and plain-step tables of one linear tier for the level cap
-// top_cap (fav_device.h).
-void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, std::vector<RungDev>& rungs,
-                       std::vector<PlainStepDev>& steps) {
+// The rung table of one linear tier for the level cap top_cap
+// (fav_device.h).
+void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, std::vector<RungDev>& rungs) {
     const OrbitData& o = *c.orbit;
     const size_t n = o.scaled.size();
     rungs.clear();
-    steps.clear();
     if (o.zn_m.size() != n || o.flags.size() != n || o.small_exp.size() != n) return;
     std::vector<LinearStepDev> flat;
     LevelDir dir{};
     flatten(c.linear[tier].levels, flat, dir);
     mark_linear_steps(flat);
     const int map_end = c.map_end;
-    steps.assign(n * kRungSlots, PlainStepDev{});
     rungs.assign(n, RungDev{});
     for (size_t r = 0; r < n; ++r) {
         RungDev& g = rungs[r];
@@ -131,6 +128,7 @@ void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, std::ve
         const unsigned offset = static_cast<unsigned>(r - 1);
         int top = std::min(dir.count - 1, top_cap);
         if (offset != 0) top = std::min(top, __builtin_ctz(offset));
+        g.top = top;
         for (int k = 0; k < kRungSlots; ++k) {
             const int level = top - k;
             if (level < 1) break;
@@ -143,15 +141,19 @@ void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, std::ve
             const bool plain = s.length == (1 << level) && plain_operand(s.a_re, s.a_im, s.a_e)
                 && plain_operand(s.b_re, s.b_im, s.b_e) && end < n && (o.flags[end] & kOrbitEndpointFlag) == 0;
             if (plain) {
-                steps[r * kRungSlots + static_cast<size_t>(k)] = PlainStepDev{
-                    std::ldexp(s.a_re, s.a_e), std::ldexp(s.a_im, s.a_e), std::ldexp(s.b_re, s.b_e),
-                    std::ldexp(s.b_im, s.b_e)};
+                g.step[k] = PlainStepDev{std::ldexp(s.a_re, s.a_e), std::ldexp(s.a_im, s.a_e),
+                                         std::ldexp(s.b_re, s.b_e), std::ldexp(s.b_im, s.b_e)};
             } else {
                 r2 = -r2;  // sign bit: fitting this step ends the plain loop
                 if (r2 == 0.0) r2 = -0.0;
             }
             g.r2[k] = r2;
         }
+    }
+    // slot 0's end index, r + 2^top, once every small_d is set
+    for (size_t r = 1; r < n; ++r) {
+        const size_t end = r + (size_t(1) << rungs[r].top);
+        rungs[r].small_top = end < n ? rungs[end].small_d : 0.0;
     }
 }
 
