@@ -846,6 +846,28 @@ int fractal_hip_colourise_kfp_device(const float* device_field, uint8_t* device_
                          lut, lut_size, 0, true);
 }
 
+int fractal_hip_colourise_kfp_device_noalloc(const float* device_field, uint8_t* device_rgb, int width, int height,
+                                             int max_iter, const FractalKfpOptions* options, const uint8_t* lut,
+                                             int lut_size) {
+    try {
+        if (!device_field || !device_rgb || !valid_dimensions(width, height) || !valid_iterations(max_iter) || !lut
+            || !fav::kfp::valid_options(options, lut_size))
+            throw std::runtime_error("invalid native KFP colour dimensions or controls");
+        fav::kfp::KfpParams k = fav::kfp::make_params(*options, device_field, device_rgb, width, height, max_iter, lut,
+                                                      lut_size);
+        fav::kfp::set_plane_context(k, *options, nullptr);
+        fav::colourise_kfp_device_noalloc(k);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native allocation-free KFP colouriser failed with an unknown exception");
+        return 1;
+    }
+}
+
 
 int fractal_abi_version(void) { return FRACTAL_ABI_VERSION; }
 

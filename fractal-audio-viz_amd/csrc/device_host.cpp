@@ -27,6 +27,7 @@ namespace fav {
 hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, int persistent_blocks,
                                hipStream_t stream, hipStream_t side, hipEvent_t fork, hipEvent_t join);
 hipError_t launch_kfp_kernel(const kfp::KfpParams& params, void* scratch, int compute_units, hipStream_t stream);
+hipError_t kfp_noalloc_scratch(int device, unsigned char** out, size_t* lut_offset, size_t* bytes);
 hipError_t launch_direct_kernel(const direct::DirectParams& p, bool planes, hipStream_t stream);
 hipError_t launch_alt_kernel(const AltRenderParams& p, hipStream_t stream);
 hipError_t launch_fx_kernel(const RenderParams& p, const fx::FxOut& out, hipStream_t stream);
@@ -439,6 +440,37 @@ void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers, c
     check(hipEventRecord(s.stop, s.stream), "event");
     if (!device_pointers)
         check(hipMemcpyAsync(host.output, k.output, pixels * 3, hipMemcpyDeviceToHost, s.stream), "D2H rgb");
+    check(hipStreamSynchronize(s.stream), "KFP colour");
+    float kernel_ms = 0.0f;
+    check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");
+    s.timing.kernel_ms = kernel_ms;
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
+}
+
+// The KFP colour pass of fractal_hip_colourise_kfp_device without any
+// device allocation: the LUT is staged into, and the range reduction works
+// in, the file-scope scratch array of kfp_kernel.hip (its address looked up
+// once per device).  Field and RGB are the caller's device pointers.  Calls
+// on one device share that scratch, so overlapping calls race.
+void colourise_kfp_device_noalloc(const kfp::KfpParams& host) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    unsigned char* scratch = nullptr;
+    size_t lut_offset = 0, bytes = 0;
+    check(kfp_noalloc_scratch(s.device, &scratch, &lut_offset, &bytes), "hipGetSymbolAddress(scratch)");
+    const size_t lut_bytes = static_cast<size_t>(host.lut_size) * 3;
+    if (lut_offset + lut_bytes > bytes)
+        throw std::runtime_error("allocation-free KFP colour: the LUT exceeds the 65536-entry scratch");
+    kfp::KfpParams k = host;
+    check(hipMemcpyAsync(scratch + lut_offset, host.lut, lut_bytes, hipMemcpyHostToDevice, s.stream), "H2D lut");
+    k.lut = scratch + lut_offset;
+    const size_t pixels = static_cast<size_t>(host.width) * static_cast<size_t>(host.height);
+    const uint64_t wall0 = now_ns();
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_kfp_kernel(k, scratch, s.compute_units, s.stream), "KFP colour launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
     check(hipStreamSynchronize(s.stream), "KFP colour");
     float kernel_ms = 0.0f;
     check(hipEventElapsedTime(&kernel_ms, s.start, s.stop), "event time");

@@ -160,6 +160,9 @@ void render_direct_on_device(const direct::DirectParams& params, const double* x
                              bool device_pointers);
 void colourise_kfp_on_device(const kfp::KfpParams& params, bool device_pointers,
                              const FractalKfpPlanes* planes = nullptr);
+// colourise_kfp_on_device with device pointers and no device allocation
+// (scratch in a file-scope array; the LUT is a host pointer staged there).
+void colourise_kfp_device_noalloc(const kfp::KfpParams& params);
 LastTiming last_timing();
 bool device_available();
 

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <mutex>
 
 #include "kfp_core.h"
 
@@ -101,6 +102,37 @@ hipError_t launch_kfp_rect_atlas(kfp::KfpParams* host_params, kfp::KfpParams* pa
     const kfp::KfpParams& parent = host_params[0];
     kfp_rect_atlas_kernel<<<dim3((parent.width + 63) / 64, (parent.height + 3) / 4), dim3(256), 0, stream>>>(params, g);
     return hipGetLastError();
+}
+
+// ---- scratch of the allocation-free entry --------------------------------
+// fractal_hip_colourise_kfp_device_noalloc keeps its scratch in this
+// file-scope array instead of library-allocated buffers: bytes [0, 256) are
+// the colour-method-4 range reduction (2 int64 + 2 doubles), then the
+// palette LUT staged from the host (at most 65536 entries of 3 bytes).
+// Every device the library runs on holds one copy until the process ends;
+// every call on a device shares it.
+constexpr size_t kNoAllocLutOffset = 256;
+constexpr size_t kNoAllocScratchBytes = kNoAllocLutOffset + 65536 * 3;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// The device address of `scratch` on `device` (which must be current):
+// looked up once per device with hipGetSymbolAddress and kept; host code
+// uses only this address, never the array's name.
+hipError_t kfp_noalloc_scratch(int device, unsigned char** out, size_t* lut_offset, size_t* bytes) {
+    static std::mutex mutex;
+    static unsigned char* addresses[64] = {};
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mutex);
+    if (!addresses[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addresses[device] = static_cast<unsigned char*>(ptr);
+    }
+    *out = addresses[device];
+    *lut_offset = kNoAllocLutOffset;
+    *bytes = kNoAllocScratchBytes;
+    return hipSuccess;
 }
 
 // `scratch` holds 2 int64 + 2 doubles.  Shapes validated by the caller.

@@ -617,6 +617,19 @@ int fractal_hip_colourise_kfp_device(
     int max_iter, const FractalKfpOptions *options, const uint8_t *lut,
     int lut_size);
 
+/* fractal_hip_colourise_kfp_device with no device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results; the LUT (a host pointer,
+ * at most 65536 entries: more is an error) is staged into, and the colour
+ * method 4 range reduction works in, a fixed 196,864-byte scratch array of
+ * the library that each device holds until the process ends.  Every call on
+ * a device shares that scratch: calls that overlap on one device race. */
+int fractal_hip_colourise_kfp_device_noalloc(
+    const float *device_field, uint8_t *device_rgb, int width, int height,
+    int max_iter, const FractalKfpOptions *options, const uint8_t *lut,
+    int lut_size);
+
 /* Same contract as fractal_render_reference_ex_planes, but `device_output`
  * (width*height floats) and the plane arrays of `device_planes` (may be
  * NULL) are device pointers already allocated in HBM.  Used to time the

@@ -479,6 +479,9 @@ def bind_kfp(lib):
             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(KfpOptions),
             c_uint8_p, ctypes.c_int]
         lib.fractal_hip_colourise_kfp_device.restype = ctypes.c_int
+    if hasattr(lib, "fractal_hip_colourise_kfp_device_noalloc"):
+        lib.fractal_hip_colourise_kfp_device_noalloc.argtypes = lib.fractal_hip_colourise_kfp_device.argtypes
+        lib.fractal_hip_colourise_kfp_device_noalloc.restype = ctypes.c_int
     return lib
 
 
