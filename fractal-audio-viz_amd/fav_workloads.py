@@ -154,3 +154,214 @@ def atlas_tile(width, height, max_iter, seed, centered=False):
     if centered:
         field = field - max_iter
     return field.astype(np.float32)
+
+
+# ---- near-boundary centres: Newton refinement to a minibrot nucleus ------
+#
+# A centre is found in three seeded steps, at `digits` + 30 decimal digits
+# of working precision (mpmath):
+#   1. a start point c0 at a seeded distance 10^-k and angle from a seeded
+#      anchor, a boundary point where the map is expanding (a Misiurewicz
+#      point), so that the minibrots of period p near it shrink
+#      geometrically with p;
+#   2. the "partials" of c0: the iterations n <= 500 at which |z_n| reaches
+#      a new minimum.  Each is the period of a nucleus whose atom domain
+#      holds c0, so Newton's method started at c0 converges to it (the atom
+#      domain method of C. Heiland-Allen);
+#   3. for the partials p in [50, 500] in increasing order, Newton's method
+#      on z_p(c) = 0 from c0, kept when it converges to a nucleus of exact
+#      period p whose estimated minibrot size lies between 1/480 and 1/8 of
+#      the view height (2.8 / zoom): the minibrot shows in the view,
+#      surrounded by its decorations.
+# The centre is that nucleus plus a seeded 10^-(zoom exponent + 2) offset.
+# The anchors are chosen for slow shrinking (about 1.3x to 2.4x per period
+# step), so that both zoom 1e30 and zoom 1e100 have sizes in range at
+# periods in [50, 500]; c = -2, whose orbit lands on the escape circle
+# |z| = 2, is left out.
+# The burning ship (x' = x^2 - y^2 + a, y' = 2|x||y| + b, renderer.cpp:
+# 268-279 and 10837-10846) is not holomorphic, so its Newton step solves
+# the real 2x2 system of d(x_p, y_p)/d(a, b), whose chain rule carries the
+# sign s = sign(x) sign(y) of each fold.  On the real axis it is the real
+# Mandelbrot map, so its real Misiurewicz points serve as anchors too.
+
+# (x, y, preperiod k, period q): z_{k+q} = z_k, refined by _anchor to the
+# working precision (a start point's offset from an anchor that is exact
+# only to 17 digits would be swamped by the anchor's own error).  The
+# principal Misiurewicz point of the 1/3 limb and its mirror image, the
+# real point M_{3,1} (root of c^3 + 2c^2 + 2c + 2) and c = i.
+MANDELBROT_ANCHORS = (("-0.10109636384562216", "0.9562865108091415", 4, 1),
+                      ("-0.10109636384562216", "-0.9562865108091415", 4, 1),
+                      ("-1.5436890126920764", "0", 3, 1), ("0", "1", 2, 2))
+SHIP_ANCHORS = (("-1.5436890126920764", "0", 3, 1),)
+
+
+def _anchor(x, y, preperiod, period):
+    """The Misiurewicz point z_{k+q}(c) = z_k(c) nearest (x, y), by Newton's
+    method on the Mandelbrot map at the current precision."""
+    import mpmath as mp
+    c = mp.mpc(mp.mpf(x), mp.mpf(y))
+    for _ in range(200):
+        z, dz, zk, dzk = mp.mpc(0), mp.mpc(0), None, None
+        for n in range(1, preperiod + period + 1):
+            z, dz = z * z + c, 2 * z * dz + 1
+            if n == preperiod:
+                zk, dzk = z, dz
+        step = (z - zk) / (dz - dzk)
+        c -= step
+        if abs(step) < mp.mpf(10) ** -(mp.mp.dps - 5):
+            break
+    return c.real, c.imag
+
+
+def _ship_step(x, y, a, b):
+    return x * x - y * y + a, 2 * abs(x) * abs(y) + b
+
+
+def _partials(formula, a, b, limit=500):
+    """[(n, size estimate)] for the partials n of c0 = (a, b): the
+    nucleus_properties size formula evaluated along c0's own orbit, which
+    shadows the nucleus' orbit up to n, so that Newton's method runs only
+    on the partials whose minibrot can be in range."""
+    import mpmath as mp
+    x = y = xa = xb = ya = yb = mp.mpf(0)
+    best, gain, out = mp.inf, mp.mpf(1), []
+    for n in range(1, limit + 1):
+        s = (-1 if (x < 0) != (y < 0) else 1) if formula == 2 else 1
+        xa, xb, ya, yb = (2 * x * xa - 2 * y * ya + 1, 2 * x * xb - 2 * y * yb,
+                          2 * s * (y * xa + x * ya), 2 * s * (y * xb + x * yb) + 1)
+        x, y = _ship_step(x, y, a, b) if formula == 2 else (x * x - y * y + a, 2 * x * y + b)
+        r = x * x + y * y
+        if r < best:
+            best = r
+            dzdc = mp.sqrt(abs(xa * yb - xb * ya))
+            out.append((n, 1 / (dzdc * gain) if dzdc > 0 else mp.inf))
+        if r > 4:
+            break
+        gain *= 2 * mp.sqrt(r)
+    return out
+
+
+def _newton(formula, a, b, period, tolerance, steps=80):
+    """Newton's method on z_period(c) = 0; (a, b) of the converged root or
+    None.  The Jacobian d(x, y)/d(a, b) is carried as four reals (for the
+    Mandelbrot map it is the Cauchy-Riemann matrix of dz/dc)."""
+    import mpmath as mp
+    for _ in range(steps):
+        x = y = xa = xb = ya = yb = mp.mpf(0)
+        for _ in range(period):
+            if formula == 2:
+                s = -1 if (x < 0) != (y < 0) else 1
+                xa, xb, ya, yb = (2 * x * xa - 2 * y * ya + 1, 2 * x * xb - 2 * y * yb,
+                                  2 * s * (y * xa + x * ya), 2 * s * (y * xb + x * yb) + 1)
+                x, y = _ship_step(x, y, a, b)
+            else:
+                xa, xb, ya, yb = (2 * x * xa - 2 * y * ya + 1, 2 * x * xb - 2 * y * yb,
+                                  2 * (y * xa + x * ya), 2 * (y * xb + x * yb) + 1)
+                x, y = x * x - y * y + a, 2 * x * y + b
+            if abs(x) + abs(y) > 1e50:
+                return None
+        det = xa * yb - xb * ya
+        if det == 0:
+            return None
+        da, db = (yb * x - xb * y) / det, (xa * y - ya * x) / det
+        a, b = a - da, b - db
+        if max(abs(da), abs(db)) < tolerance:
+            return a, b
+    return None
+
+
+def nucleus_properties(formula, a, b, period):
+    """(min |z_n| over 0 < n < period, |z_period|, size estimate) of a
+    candidate nucleus.  The size is 1 / (|dz_p/dc| prod_{n<p} 2|z_n|): for
+    the Mandelbrot map this is the minibrot size estimate 1 / |beta lambda^2|
+    (lambda = prod 2 z_n, dz_p/dc = beta lambda); for the burning ship
+    |dz_p/dc| is the square root of the Jacobian determinant."""
+    import mpmath as mp
+    x = y = xa = xb = ya = yb = mp.mpf(0)
+    smallest, gain = mp.inf, mp.mpf(1)
+    for n in range(1, period + 1):
+        s = (-1 if (x < 0) != (y < 0) else 1) if formula == 2 else 1
+        xa, xb, ya, yb = (2 * x * xa - 2 * y * ya + 1, 2 * x * xb - 2 * y * yb,
+                          2 * s * (y * xa + x * ya), 2 * s * (y * xb + x * yb) + 1)
+        x, y = _ship_step(x, y, a, b) if formula == 2 else (x * x - y * y + a, 2 * x * y + b)
+        if n < period:
+            modulus = mp.sqrt(x * x + y * y)
+            smallest = min(smallest, modulus)
+            gain *= 2 * modulus
+    scale = mp.sqrt(abs(xa * yb - xb * ya)) * gain
+    return smallest, mp.sqrt(x * x + y * y), 1 / scale if scale != 0 else mp.inf
+
+
+def nucleus_centre(formula: int, seed: int, zoom_exponent: int, digits: int = 120, attempts: int = 200):
+    """A seeded near-boundary centre for a view at zoom 10^zoom_exponent:
+    the Newton-refined nucleus of a period-p minibrot (p in [50, 500],
+    estimated size 1/480 to 1/8 of the view height) plus a seeded
+    10^-(zoom_exponent+2) offset, as `digits`-digit decimal text.  formula
+    0 is the Mandelbrot map, 2 the burning ship.  Raises if no start point
+    within `attempts` converges."""
+    import mpmath as mp
+    if formula not in (0, 2):
+        raise ValueError("nucleus centres are defined for the Mandelbrot map (0) and the burning ship (2)")
+    rng = random.Random(f"nucleus-{formula}-{seed}-{zoom_exponent}")
+    anchors = SHIP_ANCHORS if formula == 2 else MANDELBROT_ANCHORS
+    with mp.workdps(digits + 30):
+        tolerance = mp.mpf(10) ** -(digits + 15)
+        view = mp.mpf("2.8") * mp.mpf(10) ** -zoom_exponent
+        smallest_size, largest_size = view / 480, view / 8
+        exact = {(a[0], a[1]): _anchor(*a) for a in anchors}
+        for attempt in range(attempts):
+            ax, ay, preperiod, period = anchors[rng.randrange(len(anchors))]
+            a_exact, b_exact = exact[(ax, ay)]
+            depth = rng.uniform(0.6 * zoom_exponent, 1.1 * zoom_exponent)
+            angle = 2 * mp.pi * rng.random()
+            a0 = a_exact + mp.mpf(10) ** -depth * mp.cos(angle)
+            b0 = b_exact + mp.mpf(10) ** -depth * mp.sin(angle)
+            for period, estimate in _partials(formula, a0, b0):
+                if period < 50 or estimate > 100 * largest_size:
+                    continue
+                if estimate < smallest_size / 100:
+                    break
+                root = _newton(formula, a0, b0, period, tolerance)
+                if root is None:
+                    continue
+                smallest, residual, size = nucleus_properties(formula, root[0], root[1], period)
+                if smallest < mp.mpf(10) ** -40:
+                    continue   # the nucleus of a smaller period
+                if size > largest_size:
+                    continue   # too large: try the next partial
+                if size < smallest_size:
+                    break      # later partials are smaller still: next start point
+                offset_angle = 2 * mp.pi * rng.random()
+                step = mp.mpf(10) ** -(zoom_exponent + 2)
+                x = root[0] + step * mp.cos(offset_angle)
+                y = root[1] + step * mp.sin(offset_angle)
+                return dict(x=mp.nstr(x, digits, strip_zeros=False), y=mp.nstr(y, digits, strip_zeros=False),
+                            nucleus_x=mp.nstr(root[0], digits, strip_zeros=False),
+                            nucleus_y=mp.nstr(root[1], digits, strip_zeros=False), period=period,
+                            size=mp.nstr(size, 6), attempt=attempt, anchor=[ax, ay])
+    raise RuntimeError(f"no nucleus found for formula {formula}, seed {seed}, zoom 1e{zoom_exponent}")
+
+
+def c3_bits(zoom_exponent: int) -> int:
+    """Orbit precision of a near-boundary field: 64 bits above the zoom's,
+    rounded up to a multiple of 32."""
+    import math
+    return 32 * math.ceil((64 + zoom_exponent * math.log2(10)) / 32)
+
+
+def near_boundary_field(formula: int, seed: int, zoom_exponent: int, width: int = 2560, height: int = 1440,
+                        max_iter: int = 100000, centre=None):
+    """A 2560x1440, cap-100000 field at zoom 10^zoom_exponent around a
+    nucleus_centre: orbit precision c3_bits, glitch rebase on
+    (allow_recovery), the reference's default image series, BLA order 3 with
+    block 256 for the Mandelbrot map (the alternate formulas take no BLA
+    order).  The orbit near a nucleus returns close to 0 every period, which
+    drives the reference's glitch handling: with strict rendering and no
+    rebase most of such a view is NaN."""
+    c = centre if centre is not None else nucleus_centre(formula, seed, zoom_exponent)
+    name = "mandelbrot" if formula == 0 else "burning_ship"
+    return dict(name=f"{name}_{width}x{height}_zoom1e{zoom_exponent}_cap{max_iter}_seed{seed}", formula=formula,
+                julia=("0", "0"), x=c["x"], y=c["y"], zoom=f"1e{zoom_exponent}", width=width, height=height,
+                max_iter=max_iter, bits=c3_bits(zoom_exponent), image_series_order=8,
+                series_order=3 if formula == 0 else 8, series_block=256 if formula == 0 else 32,
+                options=dict(allow_recovery=1), period=c["period"])

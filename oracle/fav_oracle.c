@@ -1782,17 +1782,23 @@ static void render_pixel(const render_ctx *c, float *output, size_t index, sc_t 
         st[ST_MAX_PIXEL_ITER] = (uint64_t)(iteration > 0 ? iteration : 0);
 }
 
+typedef struct { int x, y, width, height; } fo_rect;
+
 static int render_alternate(float *output, int width, int height, fe_t zoom, const fo_reference *r, int max_iter,
                             const FractalRenderOptions *o, uint64_t *stats, int threads, struct timespec started,
-                            const sc_t *points, const fe_t *point_radius);
+                            const sc_t *points, const fe_t *point_radius, const fo_rect *crop);
 
 /* render_bla_impl's field loop (renderer.cpp:12672-13583) over a grid, or
  * over caller-supplied normalised dc offsets (width = count, height = 1)
- * when points is set. */
+ * when points is set.  With crop set, only the pixels of that rectangle of
+ * the width x height grid are rendered, into a crop->width x crop->height
+ * output: every per-pixel offset and every field-wide decision is the full
+ * grid's. */
 static int render_mandel_field(float *output, int width, int height, fe_t zoom, const fo_reference *r, int max_iter,
                              int series_order, int series_block, const FractalRenderOptions *options,
                              FractalRenderPlanes *planes, uint64_t *stats, int threads, const sc_t *points,
-                             const fe_t *point_radius, struct timespec started, char *error, int cap) {
+                             const fe_t *point_radius, struct timespec started, char *error, int cap,
+                             const fo_rect *crop) {
     render_ctx c;
     memset(&c, 0, sizeof(c));
     c.r = r;
@@ -1856,13 +1862,14 @@ static int render_mandel_field(float *output, int width, int height, fe_t zoom, 
     const int workers = 1;
 #endif
     uint64_t *local = stats ? (uint64_t *)calloc((size_t)workers * FO_STATS_FIELDS, sizeof(uint64_t)) : NULL;
-    const long long count = (long long)width * (long long)height;
+    const fo_rect area = crop ? *crop : (fo_rect){0, 0, width, height};
+    const long long count = (long long)area.width * (long long)area.height;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 256)
 #endif
     for (long long lp = 0; lp < count; ++lp) {
-        const int py = (int)(lp / width);
-        const int px = (int)(lp - (long long)py * width);
+        const int py = area.y + (int)(lp / area.width);
+        const int px = area.x + (int)(lp % area.width);
 #ifdef _OPENMP
         uint64_t *st = local ? local + (size_t)omp_get_thread_num() * FO_STATS_FIELDS : NULL;
 #else
@@ -1890,13 +1897,19 @@ static int render_mandel_field(float *output, int width, int height, fe_t zoom, 
     return 0;
 }
 
-int fo_render(float *output, int width, int height, const char *zoom_text, const fo_reference *r,
-              int max_iter, int series_order, int series_block, const FractalRenderOptions *options,
-              FractalRenderPlanes *planes, uint64_t *stats, int threads, char *error, int cap) {
+static int render_grid(float *output, int width, int height, const char *zoom_text, const fo_reference *r,
+                       int max_iter, int series_order, int series_block, const FractalRenderOptions *options,
+                       FractalRenderPlanes *planes, uint64_t *stats, int threads, char *error, int cap,
+                       const fo_rect *crop) {
     struct timespec started;
     clock_gettime(CLOCK_MONOTONIC, &started);
     if (!output || !r || !options || width <= 0 || height <= 0) {
         fo_set_error(error, cap, "invalid render arguments");
+        return 1;
+    }
+    if (crop && (crop->x < 0 || crop->y < 0 || crop->width <= 0 || crop->height <= 0
+                 || crop->width > width - crop->x || crop->height > height - crop->y)) {
+        fo_set_error(error, cap, "crop rectangle outside the field");
         return 1;
     }
     fe_t zoom;
@@ -1910,14 +1923,30 @@ int fo_render(float *output, int width, int height, const char *zoom_text, const
             return 1;
         }
         if (render_alternate(output, width, height, zoom, r, max_iter, options, stats, threads, started, NULL,
-                             NULL)) {
+                             NULL, crop)) {
             fo_set_error(error, cap, "out of memory");
             return 1;
         }
         return 0;
     }
     return render_mandel_field(output, width, height, zoom, r, max_iter, series_order, series_block, options, planes,
-                             stats, threads, NULL, NULL, started, error, cap);
+                             stats, threads, NULL, NULL, started, error, cap, crop);
+}
+
+int fo_render(float *output, int width, int height, const char *zoom_text, const fo_reference *r,
+              int max_iter, int series_order, int series_block, const FractalRenderOptions *options,
+              FractalRenderPlanes *planes, uint64_t *stats, int threads, char *error, int cap) {
+    return render_grid(output, width, height, zoom_text, r, max_iter, series_order, series_block, options, planes,
+                       stats, threads, error, cap, NULL);
+}
+
+int fo_render_crop(float *output, int width, int height, int crop_x, int crop_y, int crop_width, int crop_height,
+                   const char *zoom_text, const fo_reference *r, int max_iter, int series_order, int series_block,
+                   const FractalRenderOptions *options, FractalRenderPlanes *planes, uint64_t *stats, int threads,
+                   char *error, int cap) {
+    const fo_rect crop = {crop_x, crop_y, crop_width, crop_height};
+    return render_grid(output, width, height, zoom_text, r, max_iter, series_order, series_block, options, planes,
+                       stats, threads, error, cap, &crop);
 }
 
 /* fractal_render_points_impl, renderer.cpp:20019-20186, for every formula:
@@ -1960,13 +1989,14 @@ int fo_render_points(float *output, int point_count, const char *zoom_text, cons
     const fe_t point_radius = fe_sqrt((fe_t){largest.m, largest.e});
     if (r->formula != 0) { /* renderer.cpp:20105-20136 */
         const int failed = render_alternate(output, point_count, 1, zoom, r, max_iter, options, stats, threads,
-                                            started, points, &point_radius);
+                                            started, points, &point_radius, NULL);
         free(points);
         if (failed) fo_set_error(error, cap, "out of memory");
         return failed;
     }
     const int status = render_mandel_field(output, point_count, 1, zoom, r, max_iter, series_order, series_block,
-                                         options, NULL, stats, threads, points, &point_radius, started, error, cap);
+                                         options, NULL, stats, threads, points, &point_radius, started, error, cap,
+                                         NULL);
     free(points);
     return status;
 }
@@ -2349,6 +2379,7 @@ typedef struct {
     double bias;
     const fe_t *xo, *yo;
     const sc_t *points; /* point render: one normalised dc per pixel index, or NULL */
+    int crop_x, crop_y, out_width; /* output holds the out_width-wide rectangle at (crop_x, crop_y) */
     int budget_enabled;
     struct timespec deadline;
     atomic_int *budget_exceeded;
@@ -2356,7 +2387,7 @@ typedef struct {
 
 static void alt_pixel(const alt_ctx *c, int px, int py) {
     const fo_reference *r = c->r;
-    const size_t index = (size_t)py * (size_t)c->width + (size_t)px;
+    const size_t index = (size_t)(py - c->crop_y) * (size_t)c->out_width + (size_t)(px - c->crop_x);
     const int julia = r->formula == 1;
     const int mode = c->mode;
     const sc_t dc = c->points ? c->points[(size_t)py * (size_t)c->width + (size_t)px]
@@ -2498,7 +2529,8 @@ static void alt_pixel(const alt_ctx *c, int px, int py) {
  * height = 1). */
 static int render_alternate(float *output, int width, int height, fe_t zoom, const fo_reference *r, int max_iter,
                             const FractalRenderOptions *o, uint64_t *stats, int threads, struct timespec started,
-                            const sc_t *points, const fe_t *point_radius) {
+                            const sc_t *points, const fe_t *point_radius, const fo_rect *crop) {
+    const fo_rect area = crop ? *crop : (fo_rect){0, 0, width, height};
     const double factor = viewport_height_factor(o->coordinate_mode);
     const fe_t view_height = fe_muld(fe_div(fe_from_parts(1.0, 0), zoom), factor);
     const fe_t view_width = fe_muld(view_height, (double)width / (double)height);
@@ -2513,6 +2545,7 @@ static int render_alternate(float *output, int width, int height, fe_t zoom, con
     memset(&c, 0, sizeof(c));
     c.r = r; c.output = output; c.width = width; c.max_iter = max_iter; c.mode = o->escape_radius_mode;
     c.strict = o->strict != 0; c.bias = o->output_bias; c.xo = xo; c.yo = yo; c.points = points;
+    c.crop_x = area.x; c.crop_y = area.y; c.out_width = area.width;
     c.cycle_enabled = o->disable_cycle == 0 && (o->strict == 0 || o->strict_cycle != 0);
     /* renderer.cpp:12121-12128 */
     c.series_available = r->formula == 1 && r->series_enabled && r->series_iteration < max_iter
@@ -2534,15 +2567,13 @@ static int render_alternate(float *output, int width, int height, fe_t zoom, con
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(dynamic, 256)
 #endif
-    for (long long lp = 0; lp < (long long)width * height; ++lp) {
-        const int py = (int)(lp / width);
-        alt_pixel(&c, (int)(lp - (long long)py * width), py);
-    }
+    for (long long lp = 0; lp < (long long)area.width * area.height; ++lp)
+        alt_pixel(&c, area.x + (int)(lp % area.width), area.y + (int)(lp / area.width));
     if (stats) {
         struct timespec ended;
         clock_gettime(CLOCK_MONOTONIC, &ended);
         memset(stats, 0, sizeof(uint64_t) * FO_STATS_FIELDS);
-        stats[0] = (uint64_t)width * (uint64_t)height;
+        stats[0] = (uint64_t)area.width * (uint64_t)area.height;
         stats[ST_RENDER_NS] = (uint64_t)((ended.tv_sec - started.tv_sec) * 1000000000LL
                                          + (ended.tv_nsec - started.tv_nsec));
     }

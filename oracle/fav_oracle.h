@@ -73,6 +73,19 @@ int fo_render(
     FractalRenderPlanes *planes, uint64_t *stats, int threads,
     char *error, int error_capacity);
 
+/* fo_render restricted to the crop_width x crop_height rectangle at
+ * (crop_x, crop_y) of the width x height field: output (and planes) hold
+ * the rectangle row-major; each pixel's offset and every field-wide
+ * decision are the full field's, so the values equal the matching pixels of
+ * the full render.  stats count the rectangle's pixels. */
+int fo_render_crop(
+    float *output, int width, int height, int crop_x, int crop_y,
+    int crop_width, int crop_height, const char *zoom_text,
+    const fo_reference *reference, int max_iter, int series_order,
+    int series_block, const FractalRenderOptions *options,
+    FractalRenderPlanes *planes, uint64_t *stats, int threads,
+    char *error, int error_capacity);
+
 /* Deep render (Mandelbrot or alternate formula) of caller-supplied dc offsets (mantissa pairs and
  * exponents, one value per point; fractal_render_points, renderer.cpp:
  * 20019-20186).  Glitched points stay NaN in strict mode (no MPFR repair).

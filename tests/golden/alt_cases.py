@@ -23,6 +23,14 @@ def _case(name, formula, x, y, zoom, max_iter, bits=200, width=96, height=54, ju
                 bits=bits, width=width, height=height, julia=list(julia), clone_zoom=clone_zoom, options=options)
 
 
+def _c3_centre(zoom_exponent):
+    """A burning-ship near-boundary centre of c3_centres.json."""
+    import json
+    from pathlib import Path
+    entry = json.loads((Path(__file__).resolve().parent / "c3_centres.json").read_text())[f"2_{zoom_exponent}"]
+    return entry["x"], entry["y"]
+
+
 CASES = [
     _case("alt_burning_ship_1e7", 2, BS_X, BS_Y, "1e7", 4000),
     _case("alt_burning_ship_nobla", 2, BS_X, BS_Y, "1e7", 4000, width=64, height=36, disable_bla=1),
@@ -35,4 +43,11 @@ CASES = [
     _case("alt_julia_kalles_1e8", 1, JULIA_X, JULIA_Y, "1e8", 4000, julia=JULIA_C, escape_radius_mode=1,
           coordinate_mode=1),
     _case("alt_julia_clone", 1, JULIA_X, JULIA_Y, "1e8", 4000, julia=JULIA_C, clone_zoom="1e11"),
+    # near-boundary precision centres (burning-ship period-68 / period-226
+    # nucleus plus a 1e-32 / 1e-102 offset), the 2560x1440 fields' views at
+    # 64x36
+    _case("c3_burning_ship_1e30", 2, *_c3_centre(30), "1e30", 100000, bits=192, width=64, height=36,
+          allow_recovery=1),
+    _case("c3_burning_ship_1e100", 2, *_c3_centre(100), "1e100", 100000, bits=416, width=64, height=36,
+          allow_recovery=1),
 ]

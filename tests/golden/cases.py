@@ -25,6 +25,19 @@ def extended_centre(seed: int = 12345, digits: int = 100):
 X100, Y100 = extended_centre()
 
 
+def c3_centre(formula, zoom_exponent):
+    """A near-boundary centre of c3_centres.json (make_c3_centres.py)."""
+    import json
+    from pathlib import Path
+    table = json.loads((Path(__file__).resolve().parent / "c3_centres.json").read_text())
+    entry = table[f"{formula}_{zoom_exponent}"]
+    return entry["x"], entry["y"]
+
+
+C3_30 = c3_centre(0, 30)
+C3_100 = c3_centre(0, 100)
+
+
 def _case(name, x, y, ref_zoom, zoom, max_iter, bits, width, height, series_order=8,
           series_block=32, image_series_order=8, clone_zoom=None, **options):
     return dict(name=name, x=x, y=y, ref_zoom=ref_zoom, zoom=zoom, max_iter=max_iter,
@@ -67,6 +80,18 @@ CASES = [
     # radius tier clone of a reusable root (renderer.cpp:13704-13768)
     _case("clone_tier", X100, Y100, "1e90", "1e100", 100000, 360, 32, 18, series_order=3,
           series_block=256, series_min_terms=32, clone_zoom="1e100"),
+    # near-boundary precision centres (a period-121 / period-406 nucleus plus
+    # a 1e-32 / 1e-102 offset), the 2560x1440 fields' views at 64x36: the
+    # orbit returns near 0 every period, so the glitch test fires; strict
+    # rendering leaves those pixels NaN, glitch rebase re-anchors them
+    _case("c3_mandelbrot_1e30", *C3_30, "1e30", "1e30", 100000, 192, 64, 36, series_order=3, series_block=256,
+          allow_recovery=1),
+    _case("c3_mandelbrot_1e30_strict", *C3_30, "1e30", "1e30", 100000, 192, 64, 36, series_order=3,
+          series_block=256),
+    _case("c3_mandelbrot_1e100", *C3_100, "1e100", "1e100", 100000, 416, 64, 36, series_order=3,
+          series_block=256, allow_recovery=1),
+    _case("c3_mandelbrot_1e100_strict", *C3_100, "1e100", "1e100", 100000, 416, 32, 18, series_order=3,
+          series_block=256),
 ]
 
 STAT_NAMES = [

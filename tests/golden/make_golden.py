@@ -5,7 +5,7 @@ Run in a container where /root/reference exists, after
 sources (with -ffp-contract=off).  The fixtures hold inputs and outputs only:
 the float32 field, the six render planes, and the RenderStats counters.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case names, default all]
 """
 import ctypes
 import json
@@ -48,7 +48,8 @@ def main():
     lib = native.load_reference()
     if lib is None:
         raise SystemExit("oracle/_ref/mandelbrot.so is missing: run make -C oracle ref")
-    for case in CASES:
+    only = set(sys.argv[1:])
+    for case in (c for c in CASES if not only or c["name"] in only):
         field, planes, stats = render_case(lib, case)
         meta = {k: v for k, v in case.items()}
         np.savez_compressed(

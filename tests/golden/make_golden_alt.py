@@ -23,7 +23,8 @@ def main():
     lib = native.load_reference()
     if lib is None:
         raise SystemExit("oracle/_ref/mandelbrot.so is missing: run make -C oracle ref")
-    for case in CASES:
+    only = set(sys.argv[1:])
+    for case in (c for c in CASES if not only or c["name"] in only):
         field, stats = golden_io.render_alt_abi(lib, case, backend=0)
         np.savez_compressed(HERE / f"{case['name']}.npz", meta=np.array(json.dumps(case)),
                             field=field.astype(np.float32), stats=stats.astype(np.uint64))

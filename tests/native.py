@@ -254,6 +254,11 @@ def load_oracle():
         ctypes.c_int, ctypes.c_int, ctypes.POINTER(RenderOptions), ctypes.POINTER(RenderPlanes),
         c_uint64_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
     lib.fo_render.restype = ctypes.c_int
+    lib.fo_render_crop.argtypes = [
+        c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(RenderOptions),
+        ctypes.POINTER(RenderPlanes), c_uint64_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.fo_render_crop.restype = ctypes.c_int
     lib.fo_render_points.argtypes = [
         c_float_p, ctypes.c_int, ctypes.c_char_p, c_double_p, c_double_p, c_int32_p, ctypes.c_void_p,
         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(RenderOptions), c_uint64_p, ctypes.c_int,
@@ -322,6 +327,25 @@ class OracleReference:
         if status != 0:
             raise RuntimeError(err.value.decode())
         return out.reshape(height, width), plane_arrays, stat_values
+
+    def render_crop(self, width, height, crop, zoom, max_iter, series_order=8, series_block=32,
+                    options=None, planes=False, stats=False, threads=0):
+        """The pixels of crop = (x, y, w, h) of the width x height field, as
+        an h x w array equal to that block of the full render."""
+        x, y, w, h = crop
+        out = np.empty(w * h, dtype=np.float32)
+        opts = options if options is not None else RenderOptions.defaults()
+        plane_arrays = PlaneArrays(w, h) if planes else None
+        stat_values = np.zeros(39, dtype=np.uint64) if stats else None
+        err = ctypes.create_string_buffer(512)
+        status = self.lib.fo_render_crop(
+            out.ctypes.data_as(c_float_p), width, height, x, y, w, h, _b(zoom), self.handle, max_iter,
+            series_order, series_block, ctypes.byref(opts),
+            ctypes.byref(plane_arrays.struct) if plane_arrays else None,
+            stat_values.ctypes.data_as(c_uint64_p) if stats else None, threads, err, 512)
+        if status != 0:
+            raise RuntimeError(err.value.decode())
+        return out.reshape(h, w), plane_arrays, stat_values
 
     def __del__(self):
         if getattr(self, "handle", None):
