@@ -1163,10 +1163,14 @@ void* fractal_hip_batch_create(const FractalHipBatchField* fields, int count, co
             const long double log10_zoom =
                 (std::log(static_cast<long double>(z.m)) + static_cast<long double>(z.e) * 0.693147180559945309417232121458176568L)
                 / std::log(10.0L);
-            if (log10_zoom < 6.0L) throw std::runtime_error("batched fields are perturbation fields (zoom >= 1e6)");
             fav::FieldRequest request{f.device_output, f.width, f.height, f.zoom_text, f.max_iter, f.series_order,
                                       f.series_block, o, nullptr, nullptr, true};
-            list.push_back(fav::BatchFieldRequest{std::move(context), request});
+            fav::BatchFieldRequest entry{std::move(context), request};
+            if (log10_zoom < 6.0L) {  // the handle render's shallow branch (fractal_render_reference_ex above)
+                entry.direct = true;
+                entry.direct_zoom = fav::direct::parse_zoom_long(f.zoom_text);
+            }
+            list.push_back(std::move(entry));
         }
         void* handle = register_batch(fav::prepare_batch(std::move(list)));
         set_error("");

@@ -7,16 +7,21 @@
 //                         field's launch parameters in LDS when the field
 //                         changes, and runs the same per-pixel fast path as
 //                         fast_kernel.  Hand-offs go to one queue shared by
-//                         all fields as (field, pixel) pairs.
+//                         all fields as (field, pixel) pairs.  A tile of a
+//                         shallow field (zoom below 1e6) runs the direct
+//                         renderer's per-pixel code instead, as a
+//                         single-field render of that zoom does
+//                         (renderer.cpp:19920-19960).
 //   batch_general_kernel  the general loop over that queue.
 //   batch_tail_kernel     the plain-double tail.
 //
-// The per-pixel code is field_core.h, unchanged, so every field's result is
-// bit-identical to its single-field render.  The tile order (field-major,
+// The per-pixel code is field_core.h (direct_core.h for shallow fields),
+// unchanged, so every field's result is bit-identical to its single-field
+// render.  The tile order (field-major,
 // interleaved, or by predicted cost) only changes the schedule.
 #include <hip/hip_runtime.h>
 
-#include "field_core.h"
+#include "direct_core.h"
 
 namespace fav {
 namespace {
@@ -59,6 +64,15 @@ batch_fast_kernel(BatchLaunch q) {
         const uint32_t code = q.tiles[t];
         const int field = static_cast<int>(code >> kTileBits);
         const int tile = static_cast<int>(code & ((1U << kTileBits) - 1U));
+        if (q.direct_fields != nullptr && q.direct_fields[field].width > 0) {
+            const direct::DirectParams& d = q.direct_fields[field];
+            const int tiles_x = (d.width + 15) / 16;
+            const int px = (tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+            const int py = (tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+            if (px < d.width && py < d.height) direct::pixel<0, false, false>(d, px, py);
+            __syncthreads();  // every wave has read slot before the next tile
+            continue;
+        }
         if (field != staged) {
             const uint32_t* src = reinterpret_cast<const uint32_t*>(q.fields + field);
             uint32_t* dst = reinterpret_cast<uint32_t*>(&sp);
@@ -143,7 +157,8 @@ __global__ void batch_prologue(unsigned* counters) {
 }  // namespace
 
 // One batch: q.fields holds field_count RenderParams with tables, offsets,
-// outputs and state stores bound; q.tiles the tile_count tile codes
+// outputs and state stores bound (q.direct_fields, when set, the direct
+// parameters of the shallow fields); q.tiles the tile_count tile codes
 // (field << 20 | tile); the queues hold one slot per pixel of the batch.
 hipError_t launch_batch(const BatchLaunch& q, int fast_blocks, int persistent_blocks, hipStream_t stream) {
     batch_prologue<<<1, 64, 0, stream>>>(q.counters);

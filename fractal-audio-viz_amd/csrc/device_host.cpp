@@ -17,6 +17,7 @@
 #include "table_image.h"
 #include "kfp_core.h"
 #include "direct_core.h"
+#include "direct_prepare.h"
 #include "fx_core.h"
 #include "sched_mlp.h"
 #include "atlas_core.h"
@@ -1098,7 +1099,7 @@ public:
     std::vector<RenderParams> params;      // host image of q.fields
     std::vector<uint32_t> field_major;     // tile codes in field-major order
     std::vector<uint32_t> order;           // tile codes in launch order
-    DeviceBuffer params_d, tiles_d, offsets_d, state_d, queues_d, counters_d;
+    DeviceBuffer params_d, tiles_d, offsets_d, state_d, queues_d, counters_d, direct_d, coords_d;
     BatchLaunch q{};
     int fast_blocks_per_cu = 5;
     hipGraph_t graph = nullptr;
@@ -1107,7 +1108,7 @@ public:
     ~PreparedBatch() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (DeviceBuffer* b : {&params_d, &tiles_d, &offsets_d, &state_d, &queues_d, &counters_d})
+        for (DeviceBuffer* b : {&params_d, &tiles_d, &offsets_d, &state_d, &queues_d, &counters_d, &direct_d, &coords_d})
             if (b->ptr) (void)hipFree(b->ptr);
     }
     void drop_graph() {
@@ -1123,19 +1124,31 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
     if (fields.size() >= (1U << 12)) throw std::runtime_error("too many fields in one batch");
     auto b = std::make_shared<PreparedBatch>();
     std::vector<PreparedRender> preps(fields.size());
-    size_t pixels_total = 0, offsets_total = 0, state_total = 0;
+    std::vector<direct::DirectSetup> directs(fields.size());
+    size_t pixels_total = 0, offsets_total = 0, state_total = 0, coords_total = 0;
+    bool any_direct = false;
     for (size_t f = 0; f < fields.size(); ++f) {
         const FieldRequest& r = fields[f].request;
         if (!r.device_pointers || r.planes || r.stats) throw std::runtime_error("batch fields render to device pointers only");
         if (fields[f].context->formula != 0) throw std::runtime_error("batched fields use the Mandelbrot formula");
         const size_t tiles = static_cast<size_t>((r.width + 15) / 16) * static_cast<size_t>((r.height + 15) / 16);
         if (tiles >= (1U << 20)) throw std::runtime_error("batch field too large");
-        prepare_render(*fields[f].context, r, preps[f]);
-        ensure_device_tables(*fields[f].context);
         const size_t pixels = static_cast<size_t>(r.width) * static_cast<size_t>(r.height);
         pixels_total += pixels;
-        offsets_total += (preps[f].offset_m.size() * 12 + 512 + 255) & ~size_t(255);
-        state_total += (pixels * kStateBytesPerPixel + 20 * 256 + 255) & ~size_t(255);
+        if (fields[f].direct) {
+            const ReferenceContext& c = *fields[f].context;
+            const FractalRenderOptions& o = r.options;
+            directs[f] = direct::make_setup_values(r.width, r.height, fields[f].direct_zoom, c.x_center, c.y_center,
+                                                   r.max_iter, 0, 0.0, 0.0, o.output_bias, o.escape_radius_mode,
+                                                   o.coordinate_mode, static_cast<uint32_t>(o.reserved[0]));
+            coords_total += (static_cast<size_t>(r.width + r.height) * 8 + 255) & ~size_t(255);
+            any_direct = true;
+        } else {
+            prepare_render(*fields[f].context, r, preps[f]);
+            ensure_device_tables(*fields[f].context);
+            offsets_total += (preps[f].offset_m.size() * 12 + 512 + 255) & ~size_t(255);
+            state_total += (pixels * kStateBytesPerPixel + 20 * 256 + 255) & ~size_t(255);
+        }
         for (uint32_t t = 0; t < tiles; ++t) b->field_major.push_back((static_cast<uint32_t>(f) << 20) | t);
     }
     LaunchState& s = state();
@@ -1143,10 +1156,28 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
     init_locked(s);
     char* off_base = static_cast<char*>(b->offsets_d.get(offsets_total));
     char* state_base = static_cast<char*>(b->state_d.get(state_total));
-    size_t off_at = 0, state_at = 0;
-    b->params.resize(fields.size());
+    size_t off_at = 0, state_at = 0, coords_at = 0;
+    b->params.assign(fields.size(), RenderParams{});
+    std::vector<direct::DirectParams> direct_params(any_direct ? fields.size() : 0);  // width 0: not direct
+    char* coords_base = any_direct ? static_cast<char*>(b->coords_d.get(coords_total)) : nullptr;
     for (size_t f = 0; f < fields.size(); ++f) {
         const FieldRequest& r = fields[f].request;
+        if (fields[f].direct) {
+            direct::DirectParams d = directs[f].params;
+            double* coords = reinterpret_cast<double*>(coords_base + coords_at);
+            coords_at += (static_cast<size_t>(r.width + r.height) * 8 + 255) & ~size_t(255);
+            check(hipMemcpyAsync(coords, directs[f].x.data(), static_cast<size_t>(r.width) * 8, hipMemcpyHostToDevice,
+                                 s.stream), "batch direct x");
+            check(hipMemcpyAsync(coords + r.width, directs[f].y.data(), static_cast<size_t>(r.height) * 8,
+                                 hipMemcpyHostToDevice, s.stream), "batch direct y");
+            d.x = coords;
+            d.y = coords + r.width;
+            d.output = r.output;
+            direct_params[f] = d;
+            b->params[f].width = r.width;
+            b->params[f].height = r.height;
+            continue;
+        }
         const DeviceTables& t = *fields[f].context->device;
         const PreparedRender& prep = preps[f];
         RenderParams p = prep.params;
@@ -1185,6 +1216,13 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
                          s.stream), "batch params");
     void* queues = b->queues_d.get(pixels_total * 16);
     void* counters = b->counters_d.get(256);
+    b->q.direct_fields = nullptr;
+    if (any_direct) {
+        void* direct_d = b->direct_d.get(direct_params.size() * sizeof(direct::DirectParams));
+        check(hipMemcpyAsync(direct_d, direct_params.data(), direct_params.size() * sizeof(direct::DirectParams),
+                             hipMemcpyHostToDevice, s.stream), "batch direct params");
+        b->q.direct_fields = static_cast<const direct::DirectParams*>(direct_d);
+    }
     b->q.fields = static_cast<const RenderParams*>(params_d);
     b->q.field_count = static_cast<unsigned>(fields.size());
     b->q.general_queue = static_cast<unsigned long long*>(queues);

@@ -269,9 +269,15 @@ struct RenderParams {
     int64_t reference_pixel;
 };
 
+namespace direct { struct DirectParams; }
+
 // One batched launch over several fields (batch_kernel.hip).
 struct BatchLaunch {
     const RenderParams* fields;  // field_count parameter sets in HBM
+    // field_count direct-renderer parameter sets (zoom below 1e6: plain
+    // doubles, no reference orbit), width 0 for a perturbation field; null
+    // when the batch has no such field
+    const direct::DirectParams* direct_fields;
     const uint32_t* tiles;       // tile_count codes: field << 20 | tile index (row-major 16x16 tiles)
     unsigned tile_count;
     unsigned field_count;

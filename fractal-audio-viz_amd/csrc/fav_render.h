@@ -55,6 +55,10 @@ void render_field_floatexp(ReferenceContext& context, const FieldRequest& reques
 struct BatchFieldRequest {
     std::shared_ptr<ReferenceContext> context;
     FieldRequest request;  // device_pointers output, no planes, no stats
+    // zoom below 1e6: rendered by the direct double renderer at the
+    // reference's centre (renderer.cpp:19920-19960), at this zoom
+    bool direct = false;
+    long double direct_zoom = 0.0L;
 };
 
 class PreparedBatch;

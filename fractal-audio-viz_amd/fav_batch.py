@@ -35,11 +35,12 @@ def precision_bits(zoom_log10: float) -> int:
     return max(128, int(math.ceil((zoom_log10 + 20) * 3.33)) + 32)
 
 
-def segment_fields(count: int = 48, seed: int = 7, decades: float = 100.0, min_log10: float = 6.0,
+def segment_fields(count: int = 48, seed: int = 7, decades: float = 100.0, min_log10: float = 0.0,
                    sizes=SIZES, cap_scale: float = 1.0):
     """`count` keyframes at zooms 10^(decades*k/(count-1)) around the 100-digit
     extended deep point, sizes drawn uniformly (seeded) from `sizes`.  Fields
-    below `min_log10` are dropped (they belong to the direct renderer)."""
+    below zoom 1e6 are direct-renderer fields (the batch renders them from
+    the same tile queue); fields below `min_log10` are dropped."""
     rng = random.Random(seed)
     x, y = fav_workloads.extended_centre()
     fields = []

@@ -581,8 +581,10 @@ int fractal_hip_cost_model_train(float *weights, float *adam_m, float *adam_v,
  * Several Mandelbrot fields with their own reference handles, sizes, zooms
  * and caps rendered by one launch sequence over a shared device tile queue
  * (the keyframe fields of one atlas segment).  Each field's output is
- * bit-identical to its single-field render; outputs are device pointers;
- * zooms must be >= 1e6 (perturbation fields). */
+ * bit-identical to its single-field render; outputs are device pointers.
+ * A field below zoom 1e6 is rendered, as its single-field render is, by the
+ * direct double renderer at its handle's centre (renderer.cpp:19920-19960),
+ * from the same tile queue. */
 typedef struct FractalHipBatchField {
     void *handle;
     const char *zoom_text;

@@ -269,12 +269,13 @@ def test_alternate_rejects_bad_julia_constant(product):
 
 def test_atlas_segment_workload_rules():
     """Keyframe zooms 10^(100k/47), caps 2000 + 1000/decade clamped to
-    [64, 100000], sizes from the three frame sizes, shallow fields dropped."""
+    [64, 100000], sizes from the three frame sizes, all 48 fields (k = 0, 1,
+    2 below 1e6 are direct-renderer fields)."""
     import fav_batch
     fields = fav_batch.segment_fields()
-    assert len(fields) == 45  # k = 0, 1, 2 are below 1e6
+    assert len(fields) == 48
+    assert sum(1 for f in fields if f["zoom_log10"] < 6.0) == 3
     for f in fields:
-        assert f["zoom_log10"] >= 6.0
         assert f["max_iter"] == min(100000, max(64, int(2000 + 1000 * f["zoom_log10"])))
         assert (f["width"], f["height"]) in fav_batch.SIZES
         assert float(f["zoom"]) == pytest.approx(10.0 ** f["zoom_log10"], rel=1e-9)

@@ -1,6 +1,7 @@
 """Batched fields: one launch sequence over several references, sizes,
-zooms and caps gives every field bit-for-bit its single-field render, in
-every tile order and launch mode."""
+zooms and caps (shallow direct-renderer fields among them) gives every
+field bit-for-bit its single-field render, in every tile order and launch
+mode."""
 import numpy as np
 import pytest
 
@@ -12,8 +13,9 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def small_batch(gpu_product):
-    fields = fav_batch.segment_fields(count=12, seed=3, decades=40.0, min_log10=6.0,
-                                      sizes=((96, 54), (160, 90), (200, 120)), cap_scale=0.25)
+    # zooms 10^(40k/11): k = 0, 1 are direct-renderer fields in the batch
+    fields = fav_batch.segment_fields(count=12, seed=3, decades=40.0, sizes=((96, 54), (160, 90), (200, 120)),
+                                      cap_scale=0.25)
     batch = fav_batch.Batch(fields, threads=4)
     yield batch
     batch.close()
