@@ -123,4 +123,8 @@ def test_c3_floatexp_count_mismatch_against_fp64(gpu_product, key):
     print(f"[gpu] floatexp {f['name']}: {differ.mean() * 100:.4f}% of pixels differ from FP64; "
           f"{resolved.mean() * 100:.2f}% resolved without a glitch, of which {on_resolved * 100:.4f}% differ; "
           f"max smooth difference {smooth:.3e} where counts agree", flush=True)
-    assert on_resolved <= 0.5
+    # floatexp keeps FP64 only for the orbit, the BLA tables, the rebase test
+    # and the final smoothing: on the resolved pixels of the 1e30 field
+    # 0.16% of the counts differ.  The 1e100 field has no resolved pixel
+    # (every pixel glitches at the reference's first near-zero return).
+    assert on_resolved <= 0.0025
