@@ -268,6 +268,81 @@ def atlas_leg(lib, repeats=5):
                        "rgb_checksum": int(aurora_out.astype(np.uint64).sum())}}
 
 
+def boundary_leg(lib, hip, kfp, lut, steps):
+    """Boundary-rich Mandelbrot fields timed like the headline step (field
+    render with planes, then the KFP colour pass, device-resident): the
+    2560x1440 zoom-1e30 field around a Newton-refined period-121 nucleus
+    (escapes 590..895; the 4% of pixels around the orbit's near-zero
+    returns are glitch-rebased) and the 1920x1080 zoom-1e100 field at the
+    1/3-limb Misiurewicz point (every pixel escapes, 806..881)."""
+    nucleus = fav_workloads.nucleus_centre(0, 1, 30)
+    fields = [fav_workloads.near_boundary_field(0, 1, 30, centre=nucleus), fav_workloads.misiurewicz_field()]
+    out = []
+    lut_p = lut.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    for work in fields:
+        opts = fav_hip.NativeRenderOptions(**work["options"])
+        ref = fav_hip.NativeReference(work["x"], work["y"], work["zoom"], work["max_iter"], work["bits"],
+                                      work["image_series_order"], opts)
+        ref.upload()
+        w, h, cap = work["width"], work["height"], work["max_iter"]
+        n = w * h
+        d_field, d_planes, d_rgb = hip.malloc(n * 4), hip.malloc(n * 48), hip.malloc(n * 3)
+        planes = fav_hip.NativeRenderPlanes()
+        planes.struct_size = ctypes.sizeof(fav_hip.NativeRenderPlanes)
+        planes.version = 1
+        for i, name in enumerate(("orbit_iteration", "phase", "de_x", "de_y", "test1", "test2")):
+            setattr(planes, name, ctypes.cast(ctypes.c_void_p(d_planes.value + i * n * 8), planes._fields_[2 + i][1]))
+
+        def step():
+            status = lib.fractal_hip_render_reference_device(
+                d_field, w, h, work["zoom"].encode(), ref.handle, cap, work["series_order"], work["series_block"],
+                ctypes.byref(opts), ctypes.byref(planes))
+            if status != 0:
+                raise RuntimeError(lib.fractal_last_error().decode())
+            field_ms = fav_hip.last_timing()["kernel_ms"]
+            status = lib.fractal_hip_colourise_kfp_device(d_field, d_rgb, w, h, cap, ctypes.byref(kfp), lut_p,
+                                                          lut.shape[0])
+            if status != 0:
+                raise RuntimeError(lib.fractal_last_error().decode())
+            return field_ms, fav_hip.last_timing()["kernel_ms"]
+
+        lib.fractal_set_stats_enabled(1)
+        step()
+        stats = np.zeros(39, dtype=np.uint64)
+        lib.fractal_get_last_stats_ex(stats.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 39)
+        lib.fractal_set_stats_enabled(0)
+        field = np.empty(n, dtype=np.float32)
+        orbit_iteration = np.empty(n, dtype=np.int64)
+        hip.to_host(field, d_field)
+        hip.to_host(orbit_iteration, d_planes)
+        pixel_iterations = escape_count_sum(orbit_iteration, field, cap)
+        step()
+        hip.synchronize()
+        field_ms, colour_ms = [], []
+        started = time.perf_counter()
+        for _ in range(steps):
+            a, b = step()
+            field_ms.append(a)
+            colour_ms.append(b)
+        hip.synchronize()
+        elapsed = (time.perf_counter() - started) / steps
+        rgb = np.empty(n * 3, dtype=np.uint8)
+        hip.to_host(rgb, d_rgb)
+        for ptr in (d_field, d_planes, d_rgb):
+            hip.free(ptr)
+        ref.close()
+        escaped = np.isfinite(field) & (orbit_iteration < cap)
+        out.append({"workload": work["name"], "value": round(pixel_iterations / elapsed / 1e9, 3),
+                    "unit": "Gpixel-iter/s", "ms_per_step": round(elapsed * 1e3, 3),
+                    "mean_kernel_ms": round(float(np.mean(field_ms)), 3),
+                    "mean_colour_ms": round(float(np.mean(colour_ms)), 4),
+                    "pixel_iterations": pixel_iterations, "escaped_pct": round(float(escaped.mean()) * 100, 2),
+                    "nan_pixels": int(np.isnan(field).sum()), "rgb_checksum": int(rgb.astype(np.uint64).sum()),
+                    "linear_blocks": int(stats[3]), "cubic_blocks": int(stats[4]), "exact_steps": int(stats[5]),
+                    "tail_steps": int(stats[11]), "series_jumps": int(stats[17]), "glitches": int(stats[18])})
+    return out
+
+
 def floatexp_boundary_mismatch(lib):
     """The floatexp path where escape counts are sensitive: the 640x360
     zoom-1e12 field (cap 20000, every pixel near the boundary), against the
@@ -458,6 +533,7 @@ def main():
     rgb = np.empty(n * 3, dtype=np.uint8)
     hip.to_host(rgb, d_rgb)
     floatexp = floatexp_leg(lib, hip, ref, work, opts, d_field, field, orbit_iteration, args)
+    boundary = boundary_leg(lib, hip, kfp, lut, args.steps)
     batched = None if args.no_batch else batch_leg(lib, hip, 2)
     learned = None if args.no_learned else learned_leg(lib)
     atlas = atlas_leg(lib)
@@ -503,6 +579,7 @@ def main():
                      "traffic": measured_traffic_bytes()},
         "cpu_baseline": baseline,
         "floatexp": floatexp,
+        "boundary_fields": boundary,
         "batched": batched,
         "learned_schedule": learned,
         "atlas_colour": atlas,

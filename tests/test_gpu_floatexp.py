@@ -40,8 +40,11 @@ def fp64_counts(golden):
     return np.where(np.isnan(f), -1, np.where(oi >= cap, cap, oi + 1))
 
 
+# c1_series and bla_1e25 measure 9.09% and 9.81% (escapes after long
+# plain-tail runs, where the FP32 mantissa drifts): the bar sits just above
+# them, so a regression fails; the 0.1% target is not met on these fields.
 @pytest.mark.parametrize("name,max_mismatch", [("c2_kalles", 0.0), ("deep_tier", 0.0), ("ultra_tier", 0.0),
-                                               ("cycle_1e100", 0.0), ("c1_series", 0.25), ("bla_1e25", 0.25)])
+                                               ("cycle_1e100", 0.0), ("c1_series", 0.10), ("bla_1e25", 0.10)])
 def test_floatexp_count_mismatch_against_fp64(gpu_product, name, max_mismatch):
     print(f"[gpu] floatexp {name}", flush=True)
     g = golden_io.load(name)
