@@ -27,7 +27,6 @@ parent-tile fallback of draft renders (spatial recovery is kept for
 from __future__ import annotations
 
 import math
-from collections import deque
 from decimal import Decimal, localcontext
 from typing import Callable, Optional
 
@@ -71,98 +70,105 @@ def zoom_text(log10_zoom: float) -> str:
 
 
 def decimal_to_scaled(value: Decimal) -> tuple[float, int]:
-    """(mantissa in [0.5, 1), binary exponent) of a Decimal (visualizer.py:6642-6671)."""
+    """(mantissa in [0.5, 1), binary exponent) of a Decimal, formed as the
+    reference's point offsets are (visualizer.py:6642-6671): log2 of the
+    magnitude in double from the Decimal's own mantissa and exponent, its
+    floor as the exponent, 2 to the fractional rest as the mantissa.  The
+    double logarithm leaves the mantissa exact only to about 1e-14, and
+    the offsets must carry that same rounding to equal the reference's."""
     if not value.is_finite():
         raise ValueError("scaled Decimal value must be finite")
     if value.is_zero():
         return 0.0, 0
-    sign = -1.0 if value.is_signed() else 1.0
-    magnitude = abs(value)
-    decimal_exponent = magnitude.adjusted()
-    decimal_mantissa = float(magnitude.scaleb(-decimal_exponent))
-    log2_value = math.log2(decimal_mantissa) + float(decimal_exponent) * math.log2(10.0)
-    binary_exponent = math.floor(log2_value)
-    fractional = float(decimal_exponent) * math.log2(10.0) + math.log2(decimal_mantissa) - binary_exponent
-    mantissa, shift = math.frexp(2.0 ** fractional)
-    return sign * float(mantissa), int(binary_exponent + shift)
+    digits_exponent = abs(value).adjusted()                   # |value| = d * 10^k, 1 <= d < 10
+    leading = float(abs(value).scaleb(-digits_exponent))
+    log_magnitude = math.log2(leading) + digits_exponent * math.log2(10.0)
+    exponent = math.floor(log_magnitude)
+    rest = digits_exponent * math.log2(10.0) + math.log2(leading) - exponent
+    mantissa, carry = math.frexp(2.0 ** rest)
+    if not 0.5 <= mantissa < 1.0:
+        raise ValueError("scaled Decimal mantissa is not representable")
+    return math.copysign(mantissa, -1.0 if value.is_signed() else 1.0), int(exponent + carry)
 
 
 def unresolved_regions(mask) -> list[tuple[int, int, int, int, int, int, int]]:
-    """8-connected regions of a mask as (x0, x1, y0, y1, count, probe_x,
-    probe_y), largest first (visualizer.py:7966-8023).  The box carries a
-    one-pixel halo clipped to the image; the probe is the rounded mean
-    position; equal counts keep the raster order of their first pixel."""
-    unresolved = np.asarray(mask, dtype=bool)
-    if unresolved.ndim != 2:
+    """8-connected components of `mask` as (x0, x1, y0, y1, count, probe_x,
+    probe_y), the order and fields of the reference's region search
+    (visualizer.py:7966-8023): the bounding box widened by one pixel and
+    clipped to the image (x1, y1 exclusive), the pixel count, and the
+    component's mean position rounded half to even; largest first, equal
+    counts in the raster order of their first pixel.  scipy's labelling
+    numbers components in that raster order, so a stable sort by count
+    gives the reference's order."""
+    from scipy import ndimage
+    flags = np.asarray(mask, dtype=bool)
+    if flags.ndim != 2:
         raise ValueError("unresolved mask must be two-dimensional")
-    height, width = unresolved.shape
-    visited = np.zeros_like(unresolved, dtype=bool)
-    regions = []
-    for seed_y, seed_x in zip(*np.nonzero(unresolved)):
-        if visited[seed_y, seed_x]:
-            continue
-        stack = deque([(int(seed_y), int(seed_x))])
-        visited[seed_y, seed_x] = True
-        x0 = x1 = int(seed_x)
-        y0 = y1 = int(seed_y)
-        count = sum_x = sum_y = 0
-        while stack:
-            y, x = stack.pop()
-            count += 1
-            sum_x += x
-            sum_y += y
-            x0, x1, y0, y1 = min(x0, x), max(x1, x), min(y0, y), max(y1, y)
-            for dy in (-1, 0, 1):
-                for dx in (-1, 0, 1):
-                    ny, nx = y + dy, x + dx
-                    if (dy or dx) and 0 <= ny < height and 0 <= nx < width and unresolved[ny, nx] \
-                            and not visited[ny, nx]:
-                        visited[ny, nx] = True
-                        stack.append((ny, nx))
-        regions.append((max(0, x0 - 1), min(width, x1 + 2), max(0, y0 - 1), min(height, y1 + 2), count,
-                        int(round(sum_x / max(count, 1))), int(round(sum_y / max(count, 1)))))
-    regions.sort(key=lambda region: region[4], reverse=True)
-    return regions
+    labels, components = ndimage.label(flags, structure=np.ones((3, 3), dtype=bool))
+    if components == 0:
+        return []
+    height, width = flags.shape
+    ys, xs = np.nonzero(labels)
+    ids = labels[ys, xs]
+    counts = np.bincount(ids, minlength=components + 1)[1:]
+    mean_x = np.rint(np.bincount(ids, weights=xs, minlength=components + 1)[1:] / counts)
+    mean_y = np.rint(np.bincount(ids, weights=ys, minlength=components + 1)[1:] / counts)
+    out = []
+    for k, (rows, cols) in enumerate(ndimage.find_objects(labels)):
+        out.append((max(0, cols.start - 1), min(width, cols.stop + 1), max(0, rows.start - 1),
+                    min(height, rows.stop + 1), int(counts[k]), int(mean_x[k]), int(mean_y[k])))
+    order = np.argsort(-counts, kind="stable")
+    return [out[k] for k in order]
 
 
-def _view(log10_zoom: float, width: int, height: int, coordinate_mode: int):
-    """(view_width, view_height) as Decimals in the current context."""
-    exponent = math.floor(log10_zoom)
-    zoom_mantissa = Decimal(str(10.0 ** (log10_zoom - exponent)))
-    inverse_zoom = Decimal(1).scaleb(-exponent) / zoom_mantissa
-    view_height = (Decimal("4.0") if coordinate_mode == 1 else Decimal("2.8")) * inverse_zoom
-    return view_height * Decimal(width) / Decimal(height), view_height
+class _View:
+    """A frame's view in Decimal arithmetic: width x height pixels around
+    (x_center, y_center) at 10^log10_zoom, view height 2.8 / zoom (project
+    grid, pixel centres at (p - (n-1)/2) / n) or 4 / zoom (Kalles grid,
+    (p - n/2) / n) (renderer.cpp:142-158).  Positions are given doubled,
+    as integers: 2p for pixel p's centre, x0 + x1 - 1 for the middle of
+    the pixel range [x0, x1)."""
+
+    def __init__(self, width: int, height: int, log10_zoom: float, x_center: str, y_center: str,
+                 coordinate_mode: int):
+        if coordinate_mode not in (0, 1):
+            raise ValueError("coordinate mode must be project (0) or Kalles (1)")
+        self.width, self.height, self.mode = width, height, coordinate_mode
+        self.x_center, self.y_center = Decimal(x_center), Decimal(y_center)
+        whole = math.floor(log10_zoom)
+        zoom = Decimal(str(10.0 ** (log10_zoom - whole))).scaleb(whole)
+        self.span_y = (Decimal("4.0") if coordinate_mode == 1 else Decimal("2.8")) / zoom
+        self.span_x = self.span_y * width / height
+
+    def offset(self, doubled: int, size: int) -> Decimal:
+        """Axis offset of a doubled position, as a fraction of the span."""
+        origin = size if self.mode == 1 else size - 1
+        return Decimal(doubled - origin) / Decimal(2 * size)
+
+    def point(self, doubled_x: int, doubled_y: int) -> tuple[Decimal, Decimal]:
+        return (self.x_center + self.span_x * self.offset(doubled_x, self.width),
+                self.y_center - self.span_y * self.offset(doubled_y, self.height))
 
 
 def local_reference_cell(width: int, height: int, log10_zoom: float, x_center: str, y_center: str,
                          cell: tuple[int, int, int, int], probe: Optional[tuple[int, int]] = None,
                          coordinate_mode: int = 0) -> tuple[int, int, int, int, str, str, float]:
     """Centre (decimal text) and zoom of a secondary reference for a cell
-    (visualizer.py:7906-7963): the probe pixel's centre, or the cell's
-    centre without a probe, and the zoom that fits the cell to the frame."""
+    (visualizer.py:7906-7963): the probe pixel's centre, or the middle of
+    the cell without a probe, and the zoom at which the cell's longer side
+    relative to the frame fills the frame."""
     x0, x1, y0, y1 = cell
     if not (0 <= x0 < x1 <= width and 0 <= y0 < y1 <= height):
         raise ValueError("invalid local-reference cell")
-    if coordinate_mode not in (0, 1):
-        raise ValueError("coordinate mode must be project (0) or Kalles (1)")
+    if probe is not None and not (0 <= probe[0] < width and 0 <= probe[1] < height):
+        raise ValueError("glitch probe is outside the render")
     with localcontext() as context:
         context.prec = max(96, decimal_precision(x_center, y_center, log10_zoom) + 48)
-        view_width, view_height = _view(log10_zoom, width, height, coordinate_mode)
-        origin_x = Decimal(width if coordinate_mode else width - 1)
-        origin_y = Decimal(height if coordinate_mode else height - 1)
-        if probe is None:
-            x_numerator = Decimal(x0 + x1 - 1) - origin_x
-            y_numerator = origin_y - Decimal(y0 + y1 - 1)
-        else:
-            probe_x, probe_y = probe
-            if not (0 <= probe_x < width and 0 <= probe_y < height):
-                raise ValueError("glitch probe is outside the render")
-            x_numerator = Decimal(2 * probe_x) - origin_x
-            y_numerator = origin_y - Decimal(2 * probe_y)
-        local_x = Decimal(x_center) + view_width * (x_numerator / (Decimal(2) * Decimal(width)))
-        local_y = Decimal(y_center) + view_height * (y_numerator / (Decimal(2) * Decimal(height)))
-    scale = max(width / float(x1 - x0), height / float(y1 - y0))
-    return x0, x1, y0, y1, str(local_x), str(local_y), float(log10_zoom) + math.log10(scale)
+        view = _View(width, height, log10_zoom, x_center, y_center, coordinate_mode)
+        doubled = (x0 + x1 - 1, y0 + y1 - 1) if probe is None else (2 * probe[0], 2 * probe[1])
+        local_x, local_y = view.point(*doubled)
+    fill = max(width / float(x1 - x0), height / float(y1 - y0))
+    return x0, x1, y0, y1, str(local_x), str(local_y), float(log10_zoom) + math.log10(fill)
 
 
 def point_offsets(width: int, height: int, log10_zoom: float, cell: tuple[int, int, int, int],
@@ -179,9 +185,9 @@ def point_offsets(width: int, height: int, log10_zoom: float, cell: tuple[int, i
     with localcontext() as context:
         context.prec = max(96, decimal_precision(x_center, y_center, log10_zoom) + 48,
                            decimal_precision(reference_x, reference_y, log10_zoom) + 48)
-        view_width, view_height = _view(log10_zoom, width, height, coordinate_mode)
-        step_x_mantissa, step_x_exponent = decimal_to_scaled(view_width / Decimal(width))
-        step_y_mantissa, step_y_exponent = decimal_to_scaled(view_height / Decimal(height))
+        view = _View(width, height, log10_zoom, x_center, y_center, coordinate_mode)
+        step_x_mantissa, step_x_exponent = decimal_to_scaled(view.span_x / Decimal(width))
+        step_y_mantissa, step_y_exponent = decimal_to_scaled(view.span_y / Decimal(height))
     w, h = x1 - x0, y1 - y0
     x_delta = np.arange(x0, x1, dtype=np.float64) - float(probe_x)
     y_delta = float(probe_y) - np.arange(y0, y1, dtype=np.float64)
@@ -200,39 +206,35 @@ def point_offsets(width: int, height: int, log10_zoom: float, cell: tuple[int, i
 
 
 def spatial_recover_field(local_field, fallback_field=None):
-    """Fill NaN pixels from a fallback, then by averaging finite
-    8-neighbours (visualizer.py:8026-8087); draft renders only."""
+    """Draft-render recovery of a cell that stays unresolved: NaN pixels
+    take the fallback's finite value where there is one, then grow inward
+    from the finite pixels, each newly reached pixel the mean of its finite
+    8-neighbours, ring by ring (the recovery of visualizer.py:8026-8087,
+    as an estimate: the neighbour sums here are float64 convolutions, so
+    values are not bit-identical to the reference's).  Returns the field and
+    the number of pixels filled."""
+    from scipy import ndimage
     result = np.asarray(local_field, dtype=np.float32).copy()
-    missing_before = ~np.isfinite(result)
-    if not missing_before.any():
+    missing = ~np.isfinite(result)
+    if not missing.any():
         return result, 0
     if fallback_field is not None:
         fallback = np.asarray(fallback_field, dtype=np.float32)
-        use = missing_before & np.isfinite(fallback)
-        result[use] = fallback[use]
-    shape = result.shape
-    for _ in range(max(shape)):
-        finite = np.isfinite(result)
-        if finite.all():
+        take = missing & np.isfinite(fallback)
+        result[take] = fallback[take]
+    ring = np.ones((3, 3))
+    ring[1, 1] = 0.0
+    while True:
+        known = np.isfinite(result)
+        if known.all():
             break
-        padded = np.pad(result, 1, mode="constant", constant_values=np.nan)
-        total = np.zeros(shape, dtype=np.float32)
-        count = np.zeros(shape, dtype=np.float32)
-        for dy in range(3):
-            for dx in range(3):
-                if dx == 1 and dy == 1:
-                    continue
-                neighbour = padded[dy:dy + shape[0], dx:dx + shape[1]]
-                valid = np.isfinite(neighbour)
-                total[valid] += neighbour[valid]
-                count[valid] += 1.0
-        fillable = ~finite & (count > 0.0)
-        if not fillable.any():
-            break
-        result[fillable] = total[fillable] / count[fillable]
-    if not np.isfinite(result).all():
-        raise RuntimeError("adaptive local reference has no spatially valid recovery")
-    return result, int(missing_before.sum())
+        sums = ndimage.convolve(np.where(known, result, 0.0).astype(np.float64), ring, mode="constant")
+        counts = ndimage.convolve(known.astype(np.float64), ring, mode="constant")
+        reached = ~known & (counts > 0.0)
+        if not reached.any():
+            raise RuntimeError("adaptive local reference has no spatially valid recovery")
+        result[reached] = (sums[reached] / counts[reached]).astype(np.float32)
+    return result, int(missing.sum())
 
 
 def hip_reference_factory(formula: int = 0, julia: tuple[str, str] = ("0", "0"), library=None) -> Callable:

@@ -110,3 +110,26 @@ def test_repair_over_oracle_matches_reference(oracle, case):
     np.testing.assert_array_equal(repaired.view(np.uint32), golden["repaired"].view(np.uint32))
     assert diagnostics == golden["meta"]["diagnostics"]
     assert np.isfinite(repaired).all() and np.isnan(golden["initial"]).any()
+
+
+def test_spatial_recovery_grows_means_inward():
+    """A 3x3 hole in a linear ramp: the ring next to the finite pixels is
+    the mean of its finite neighbours, the centre the mean of the ring;
+    fallback values are taken first where finite."""
+    field = np.add.outer(np.arange(7.0), 10.0 * np.arange(7.0)).astype(np.float32)
+    hole = field.copy()
+    hole[2:5, 2:5] = np.nan
+    out, filled = fav_repair.spatial_recover_field(hole)
+    assert filled == 9 and np.isfinite(out).all()
+    np.testing.assert_array_equal(out[~np.isnan(hole)], field[~np.isnan(hole)])
+    # corner of the hole: neighbours (1,1) (1,2) (1,3) (2,1) (3,1) are finite
+    expect = np.mean([field[1, 1], field[1, 2], field[1, 3], field[2, 1], field[3, 1]])
+    assert out[2, 2] == pytest.approx(expect, rel=1e-6)
+    # the centre (the ramp is linear, so the symmetric ring averages to the exact value)
+    assert out[3, 3] == pytest.approx(field[3, 3], rel=1e-6)
+    fallback = np.full_like(field, np.nan)
+    fallback[3, 3] = -5.0
+    out, filled = fav_repair.spatial_recover_field(hole, fallback)
+    assert out[3, 3] == -5.0 and filled == 9
+    with pytest.raises(RuntimeError):
+        fav_repair.spatial_recover_field(np.full((3, 3), np.nan, dtype=np.float32))
