@@ -1125,6 +1125,7 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
     auto b = std::make_shared<PreparedBatch>();
     std::vector<PreparedRender> preps(fields.size());
     std::vector<direct::DirectSetup> directs(fields.size());
+    std::vector<const RungDev*> rungs(fields.size(), nullptr);
     size_t pixels_total = 0, offsets_total = 0, state_total = 0, coords_total = 0;
     bool any_direct = false;
     for (size_t f = 0; f < fields.size(); ++f) {
@@ -1146,6 +1147,8 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
         } else {
             prepare_render(*fields[f].context, r, preps[f]);
             ensure_device_tables(*fields[f].context);
+            const int cap = 31 - __builtin_clz(static_cast<unsigned>(preps[f].params.max_linear_length));
+            ensure_plain_tables(*fields[f].context, preps[f].linear_tier, cap, rungs[f]);
             offsets_total += (preps[f].offset_m.size() * 12 + 512 + 255) & ~size_t(255);
             state_total += (pixels * kStateBytesPerPixel + 20 * 256 + 255) & ~size_t(255);
         }
@@ -1193,6 +1196,12 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
         p.linear_dir = t.linear_dir[prep.linear_tier];
         p.cubic = t.cubic;
         p.cubic_dir = t.cubic_dir;
+        // the plain pass as in render_field; the pixel on the reference
+        // (reference_pixel -1) is left to batch_fast_kernel
+        p.rungs = rungs[f];
+        p.plain_top = 31 - __builtin_clz(static_cast<unsigned>(p.max_linear_length));
+        p.plain_pass = p.rungs != nullptr && p.fast_linear ? 1 : 0;
+        p.reference_pixel = -1;
         char* doff = off_base + off_at;
         const size_t exp_offset = (prep.offset_m.size() * 8 + 255) & ~size_t(255);
         check(hipMemcpyAsync(doff, prep.offset_m.data(), prep.offset_m.size() * 8, hipMemcpyHostToDevice, s.stream),

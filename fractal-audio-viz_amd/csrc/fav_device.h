@@ -283,7 +283,8 @@ struct BatchLaunch {
     unsigned field_count;
     unsigned long long* general_queue;  // (field << 32 | pixel) hand-offs, one slot per batch pixel
     unsigned long long* tail_queue;
-    unsigned* counters;  // [0] tile cursor, [1]/[2] general count/cursor, [3]/[4] tail count/cursor
+    unsigned* counters;  // [0] fast tile cursor, [1]/[2] general count/cursor, [3]/[4] tail count/cursor,
+                         // [5] plain tile cursor
 };
 
 }  // namespace fav
