@@ -37,6 +37,15 @@ def c3_centre(formula, zoom_exponent):
 C3_30 = c3_centre(0, 30)
 C3_100 = c3_centre(0, 100)
 
+# The principal Misiurewicz point of the 1/3 limb plus a seeded 1e-101
+# offset (fav_workloads.misiurewicz_field): every pixel escapes near
+# iteration 810, and with the Kalles bailout some linear blocks end with
+# |Z+d|^2 >= 3, so the endpoint replay runs.
+MISIUREWICZ_X = ("-0.101096363845622161025785445738622565463805442826253483876931177660780840740470584274821219810"
+                 "516779032980624190497440698")
+MISIUREWICZ_Y = ("0.9562865108091415007710960577299774358098333365105291700343143215005246590657167325269784107873"
+                 "39807194353461420359511709")
+
 
 def _case(name, x, y, ref_zoom, zoom, max_iter, bits, width, height, series_order=8,
           series_block=32, image_series_order=8, clone_zoom=None, **options):
@@ -92,6 +101,10 @@ CASES = [
           series_block=256, allow_recovery=1),
     _case("c3_mandelbrot_1e100_strict", *C3_100, "1e100", "1e100", 100000, 416, 32, 18, series_order=3,
           series_block=256),
+    # BLA endpoint replay (renderer.cpp:13363-13514): linear blocks up to
+    # 4096 steps, series off, Kalles bailout and grid
+    _case("replay_misiurewicz", MISIUREWICZ_X, MISIUREWICZ_Y, "1e100", "1e100", 100000, 416, 64, 36, series_order=1,
+          series_block=4096, series_min_terms=32, escape_radius_mode=1, coordinate_mode=1, allow_recovery=1),
 ]
 
 STAT_NAMES = [

@@ -27,9 +27,10 @@ def test_golden_cases_cover_the_paths():
         totals["series_pixels"] = totals.get("series_pixels", 0) + int(s[16])
         totals["glitch"] = totals.get("glitch", 0) + int(s[18])
         totals["unresolved"] = totals.get("unresolved", 0) + int(s[19])
-    for key in ("linear", "cubic", "exact", "tail_pixels", "tail_rebases", "series_pixels",
+    for key in ("linear", "cubic", "exact", "replay", "tail_pixels", "tail_rebases", "series_pixels",
                 "glitch", "unresolved"):
         assert totals[key] > 0, key
+    assert int(golden_io.load("replay_misiurewicz")["stats"][6]) > 0
 
 
 @pytest.mark.skipif(not native.REFERENCE_SO.exists(), reason="reference build absent")
