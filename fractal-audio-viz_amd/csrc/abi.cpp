@@ -1184,6 +1184,46 @@ void* fractal_hip_batch_create(const FractalHipBatchField* fields, int count, co
     }
 }
 
+int fractal_hip_render_segment(const FractalHipSegmentField* fields, int count, const FractalRenderOptions* options,
+                               int build_threads, int pipelined, double* timings) {
+    try {
+        if (!fields || count <= 0 || count > 4095 || build_threads < 1 || build_threads > 256)
+            throw std::runtime_error("invalid segment field list");
+        const FractalRenderOptions o = checked_options(options);
+        require_hip_backend(o);
+        std::vector<fav::SegmentField> list;
+        list.reserve(static_cast<size_t>(count));
+        for (int i = 0; i < count; ++i) {
+            const FractalHipSegmentField& f = fields[i];
+            if (!f.x_center || !f.y_center || !f.zoom_text || !f.device_output || !valid_dimensions(f.width, f.height)
+                || !valid_iterations(f.max_iter) || !valid_precision(f.precision_bits)
+                || !valid_series(f.series_order, f.series_block) || !valid_text(f.x_center) || !valid_text(f.y_center)
+                || !valid_text(f.zoom_text))
+                throw std::runtime_error("invalid native render dimensions or reference configuration");
+            parse_coordinate(f.x_center, "real");
+            parse_coordinate(f.y_center, "imaginary");
+            list.push_back(fav::SegmentField{f.x_center, f.y_center, f.zoom_text, f.device_output, f.width, f.height,
+                                             f.max_iter, f.precision_bits, f.series_order, f.series_block});
+        }
+        fav::SegmentTiming t;
+        fav::render_segment(list, o, build_threads, pipelined != 0, t);
+        if (timings) {
+            timings[0] = t.wall_ms;
+            timings[1] = t.build_ms;
+            timings[2] = t.render_ms;
+            timings[3] = t.built_ms;
+        }
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("native segment render failed with an unknown exception");
+        return 1;
+    }
+}
+
 int fractal_hip_batch_tile_count(void* batch) {
     const auto b = acquire_batch(batch);
     if (!b) {

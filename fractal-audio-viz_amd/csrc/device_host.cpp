@@ -3,6 +3,11 @@
 // scratch buffers, the kernel launch on the library stream, HIP-event
 // timing and the copy of results back to the caller.
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <exception>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -147,8 +152,149 @@ void upload_vector(const std::vector<T>& v, void*& block, const T*& view, hipStr
     bytes += n;
 }
 
+// A reference's device tables as host images: the orbit arrays packed into
+// one blob (each at a 256-byte boundary), the series coefficients into
+// another, and the step tables.  Host work only, safe on any thread.
+struct HostTables {
+    TableImage img;
+    std::vector<unsigned char> orbit_blob, series_blob;
+    size_t off_re = 0, off_im = 0, off_e = 0, off_zr_m = 0, off_zi_m = 0, off_zr_e = 0, off_zi_e = 0;
+    size_t off_mm = 0, off_me = 0, off_gm = 0, off_ge = 0, off_rd = 0, off_id = 0, off_znm = 0, off_zne = 0;
+    size_t off_small = 0, off_flags = 0, off_am = 0, off_ae = 0;
+    size_t s_re = 0, s_im = 0, s_e = 0;
+};
+
+void build_host_tables(const ReferenceContext& c, HostTables& h) {
+    make_table_image(c, h.img);
+    const OrbitData& o = *c.orbit;
+    std::vector<unsigned char>& blob = h.orbit_blob;
+    h.off_re = append(blob, h.img.re);
+    h.off_im = append(blob, h.img.im);
+    h.off_e = append(blob, h.img.e);
+    h.off_zr_m = append(blob, o.zr_m);
+    h.off_zi_m = append(blob, o.zi_m);
+    h.off_zr_e = append(blob, o.zr_e);
+    h.off_zi_e = append(blob, o.zi_e);
+    h.off_mm = append(blob, o.margin_m);
+    h.off_me = append(blob, o.margin_e);
+    h.off_gm = append(blob, o.glitch_m);
+    h.off_ge = append(blob, o.glitch_e);
+    h.off_rd = append(blob, o.real_double);
+    h.off_id = append(blob, o.imag_double);
+    h.off_znm = append(blob, o.zn_m);
+    h.off_zne = append(blob, o.zn_e);
+    h.off_small = append(blob, o.small_exp);
+    h.off_flags = append(blob, o.flags);
+    h.off_am = append(blob, o.alt_margin_m);
+    h.off_ae = append(blob, o.alt_margin_e);
+    h.s_re = append(h.series_blob, h.img.series_re);
+    h.s_im = append(h.series_blob, h.img.series_im);
+    h.s_e = append(h.series_blob, h.img.series_e);
+}
+
+// The parts of HostTables in upload order.
+enum TablePart { kPartOrbit, kPartAlt, kPartFast, kPartSeries, kPartCubic, kPartLinear0, kPartLinear1, kTableParts };
+
+// The host images copied into one pinned (page-locked) buffer, each part at
+// a 256-byte boundary, so that the copies to HBM are asynchronous DMA
+// transfers.  `done` marks the end of the copies once they are issued.
+struct StagedTables {
+    unsigned char* pinned = nullptr;
+    size_t offset[kTableParts] = {}, bytes[kTableParts] = {};
+    hipEvent_t done = nullptr;
+    StagedTables() = default;
+    StagedTables(const StagedTables&) = delete;
+    StagedTables& operator=(const StagedTables&) = delete;
+    ~StagedTables() { release(); }
+    void release() {
+        if (done) (void)hipEventDestroy(done);
+        if (pinned) (void)hipHostFree(pinned);
+        done = nullptr;
+        pinned = nullptr;
+    }
+};
+
+template <typename T>
+std::pair<const void*, size_t> table_part(const std::vector<T>& v) {
+    return {v.data(), v.size() * sizeof(T)};
+}
+
+void stage_tables(const HostTables& h, StagedTables& s) {
+    const std::pair<const void*, size_t> parts[kTableParts] = {
+        table_part(h.orbit_blob), table_part(h.img.alt), table_part(h.img.fast), table_part(h.series_blob),
+        table_part(h.img.cubic), table_part(h.img.linear[0]), table_part(h.img.linear[1])};
+    size_t total = 0;
+    for (int i = 0; i < kTableParts; ++i) {
+        s.offset[i] = total;
+        s.bytes[i] = parts[i].second;
+        total = (total + parts[i].second + 255) & ~size_t(255);
+    }
+    check(hipHostMalloc(reinterpret_cast<void**>(&s.pinned), std::max<size_t>(total, 256), hipHostMallocPortable),
+          "hipHostMalloc table staging");
+    for (int i = 0; i < kTableParts; ++i)
+        if (s.bytes[i]) std::memcpy(s.pinned + s.offset[i], parts[i].first, s.bytes[i]);
+}
+
+// HBM blocks for the staged parts and their copies on `stream`; the views
+// of the returned tables point into those blocks.  Records `s.done` after
+// the copies.
+std::unique_ptr<DeviceTables> upload_staged(const ReferenceContext& c, const HostTables& h, StagedTables& s,
+                                            int device, hipStream_t stream) {
+    auto t = std::make_unique<DeviceTables>();
+    t->device = device;
+    auto copy = [&](int part, void*& block) {
+        const size_t n = std::max<size_t>(s.bytes[part], 1);
+        check(hipMalloc(&block, n), "hipMalloc tables");
+        if (s.bytes[part])
+            check(hipMemcpyAsync(block, s.pinned + s.offset[part], s.bytes[part], hipMemcpyHostToDevice, stream),
+                  "table upload");
+        t->bytes += n;
+        return static_cast<const unsigned char*>(block);
+    };
+    const unsigned char* base = copy(kPartOrbit, t->orbit_block);
+    t->orbit.re = reinterpret_cast<const double*>(base + h.off_re);
+    t->orbit.im = reinterpret_cast<const double*>(base + h.off_im);
+    t->orbit.e = reinterpret_cast<const int32_t*>(base + h.off_e);
+    t->orbit.zr_m = reinterpret_cast<const double*>(base + h.off_zr_m);
+    t->orbit.zi_m = reinterpret_cast<const double*>(base + h.off_zi_m);
+    t->orbit.zr_e = reinterpret_cast<const int32_t*>(base + h.off_zr_e);
+    t->orbit.zi_e = reinterpret_cast<const int32_t*>(base + h.off_zi_e);
+    t->orbit.margin_m = reinterpret_cast<const double*>(base + h.off_mm);
+    t->orbit.margin_e = reinterpret_cast<const int32_t*>(base + h.off_me);
+    t->orbit.glitch_m = reinterpret_cast<const double*>(base + h.off_gm);
+    t->orbit.glitch_e = reinterpret_cast<const int32_t*>(base + h.off_ge);
+    t->orbit.real_double = reinterpret_cast<const double*>(base + h.off_rd);
+    t->orbit.imag_double = reinterpret_cast<const double*>(base + h.off_id);
+    t->orbit.zn_m = reinterpret_cast<const double*>(base + h.off_znm);
+    t->orbit.zn_e = reinterpret_cast<const int32_t*>(base + h.off_zne);
+    t->orbit.small_exp = reinterpret_cast<const int32_t*>(base + h.off_small);
+    t->orbit.flags = reinterpret_cast<const int32_t*>(base + h.off_flags);
+    t->orbit.size = static_cast<int32_t>(c.orbit->scaled.size());
+    t->orbit.map_end = c.map_end;
+    t->alt_margin_m = reinterpret_cast<const double*>(base + h.off_am);
+    t->alt_margin_e = reinterpret_cast<const int32_t*>(base + h.off_ae);
+    t->alt = reinterpret_cast<const AltStepDev*>(copy(kPartAlt, t->alt_block));
+    t->orbit.fast = reinterpret_cast<const OrbitFastDev*>(copy(kPartFast, t->fast_block));
+    t->alt_dir = h.img.alt_dir;
+    const unsigned char* sbase = copy(kPartSeries, t->series_block);
+    t->series.re = reinterpret_cast<const double*>(sbase + h.s_re);
+    t->series.im = reinterpret_cast<const double*>(sbase + h.s_im);
+    t->series.e = reinterpret_cast<const int32_t*>(sbase + h.s_e);
+    t->cubic = reinterpret_cast<const CubicStepDev*>(copy(kPartCubic, t->cubic_block));
+    t->cubic_dir = h.img.cubic_dir;
+    for (int tier = 0; tier < 2; ++tier) {
+        t->linear[tier] = reinterpret_cast<const LinearStepDev*>(copy(kPartLinear0 + tier, t->linear_block[tier]));
+        t->linear_dir[tier] = h.img.linear_dir[tier];
+    }
+    check(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventRecord(s.done, stream), "hipEventRecord");
+    return t;
+}
+
 }  // namespace
 
+// Host images, pinned staging and the copies on the side stream (default
+// priority, like the compute stream), waited for before returning.
 void ensure_device_tables(ReferenceContext& c) {
     std::lock_guard<std::mutex> lock(c.device_mutex);
     if (c.device) return;
@@ -158,62 +304,12 @@ void ensure_device_tables(ReferenceContext& c) {
         init_locked(s);
     }
     const uint64_t t0 = now_ns();
-    TableImage img;
-    make_table_image(c, img);
-    auto t = std::make_unique<DeviceTables>();
-    t->device = s.device;
-    const OrbitData& o = *c.orbit;
-    std::vector<unsigned char> blob;
-    const size_t off_re = append(blob, img.re), off_im = append(blob, img.im), off_e = append(blob, img.e);
-    const size_t off_zr_m = append(blob, o.zr_m), off_zi_m = append(blob, o.zi_m);
-    const size_t off_zr_e = append(blob, o.zr_e), off_zi_e = append(blob, o.zi_e);
-    const size_t off_mm = append(blob, o.margin_m), off_me = append(blob, o.margin_e);
-    const size_t off_gm = append(blob, o.glitch_m), off_ge = append(blob, o.glitch_e);
-    const size_t off_rd = append(blob, o.real_double), off_id = append(blob, o.imag_double);
-    const size_t off_znm = append(blob, o.zn_m), off_zne = append(blob, o.zn_e);
-    const size_t off_small = append(blob, o.small_exp), off_flags = append(blob, o.flags);
-    const size_t off_am = append(blob, o.alt_margin_m), off_ae = append(blob, o.alt_margin_e);
-    const unsigned char* base = nullptr;
-    upload_vector(blob, t->orbit_block, base, s.stream, t->bytes);
-    t->orbit.re = reinterpret_cast<const double*>(base + off_re);
-    t->orbit.im = reinterpret_cast<const double*>(base + off_im);
-    t->orbit.e = reinterpret_cast<const int32_t*>(base + off_e);
-    t->orbit.zr_m = reinterpret_cast<const double*>(base + off_zr_m);
-    t->orbit.zi_m = reinterpret_cast<const double*>(base + off_zi_m);
-    t->orbit.zr_e = reinterpret_cast<const int32_t*>(base + off_zr_e);
-    t->orbit.zi_e = reinterpret_cast<const int32_t*>(base + off_zi_e);
-    t->orbit.margin_m = reinterpret_cast<const double*>(base + off_mm);
-    t->orbit.margin_e = reinterpret_cast<const int32_t*>(base + off_me);
-    t->orbit.glitch_m = reinterpret_cast<const double*>(base + off_gm);
-    t->orbit.glitch_e = reinterpret_cast<const int32_t*>(base + off_ge);
-    t->orbit.real_double = reinterpret_cast<const double*>(base + off_rd);
-    t->orbit.imag_double = reinterpret_cast<const double*>(base + off_id);
-    t->orbit.zn_m = reinterpret_cast<const double*>(base + off_znm);
-    t->orbit.zn_e = reinterpret_cast<const int32_t*>(base + off_zne);
-    t->orbit.small_exp = reinterpret_cast<const int32_t*>(base + off_small);
-    t->orbit.flags = reinterpret_cast<const int32_t*>(base + off_flags);
-    t->orbit.size = static_cast<int32_t>(o.scaled.size());
-    t->orbit.map_end = c.map_end;
-    t->alt_margin_m = reinterpret_cast<const double*>(base + off_am);
-    t->alt_margin_e = reinterpret_cast<const int32_t*>(base + off_ae);
-    upload_vector(img.alt, t->alt_block, t->alt, s.stream, t->bytes);
-    upload_vector(img.fast, t->fast_block, t->orbit.fast, s.stream, t->bytes);
-    t->alt_dir = img.alt_dir;
-    std::vector<unsigned char> sblob;
-    const size_t s_re = append(sblob, img.series_re), s_im = append(sblob, img.series_im);
-    const size_t s_e = append(sblob, img.series_e);
-    const unsigned char* sbase = nullptr;
-    upload_vector(sblob, t->series_block, sbase, s.stream, t->bytes);
-    t->series.re = reinterpret_cast<const double*>(sbase + s_re);
-    t->series.im = reinterpret_cast<const double*>(sbase + s_im);
-    t->series.e = reinterpret_cast<const int32_t*>(sbase + s_e);
-    upload_vector(img.cubic, t->cubic_block, t->cubic, s.stream, t->bytes);
-    t->cubic_dir = img.cubic_dir;
-    for (int tier = 0; tier < 2; ++tier) {
-        upload_vector(img.linear[tier], t->linear_block[tier], t->linear[tier], s.stream, t->bytes);
-        t->linear_dir[tier] = img.linear_dir[tier];
-    }
-    check(hipStreamSynchronize(s.stream), "table upload");
+    HostTables h;
+    build_host_tables(c, h);
+    StagedTables staged;
+    stage_tables(h, staged);
+    auto t = upload_staged(c, h, staged, s.device, s.side);
+    check(hipEventSynchronize(staged.done), "table upload");
     c.upload_ns = now_ns() - t0;
     c.device = std::move(t);
 }
@@ -1367,6 +1463,135 @@ void render_direct_on_device(const direct::DirectParams& host, const double* x, 
     s.timing.kernel_ms = kernel_ms;
     s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
     s.timing.pixels = static_cast<double>(pixels);
+}
+
+// ---- pipelined segment --------------------------------------------------------
+
+namespace {
+
+// One field's reference and staged tables, built by a worker and rendered
+// by the calling thread.
+struct SegmentSlot {
+    std::unique_ptr<ReferenceContext> context;
+    HostTables host;
+    StagedTables staged;
+    bool direct = false;
+    long double direct_zoom = 0.0L;
+};
+
+// The handle render's shallow test (renderer.cpp:19920-19960).
+bool shallow_zoom(const char* zoom, long precision_bits) {
+    const FE z = parse_zoom(zoom, precision_bits);
+    const long double log10_zoom =
+        (std::log(static_cast<long double>(z.m)) + static_cast<long double>(z.e) * 0.693147180559945309417232121458176568L)
+        / std::log(10.0L);
+    return log10_zoom < 6.0L;
+}
+
+// Tables staged by a worker: copies on the side stream, and the compute
+// stream waits for them before the render's launches (no host wait).
+void upload_prepared_tables(ReferenceContext& c, const HostTables& h, StagedTables& staged) {
+    std::lock_guard<std::mutex> lock(c.device_mutex);
+    LaunchState& s = state();
+    {
+        std::lock_guard<std::mutex> slock(s.mutex);
+        init_locked(s);
+    }
+    auto t = upload_staged(c, h, staged, s.device, s.side);
+    check(hipStreamWaitEvent(s.stream, staged.done, 0), "wait for table upload");
+    c.device = std::move(t);
+}
+
+}  // namespace
+
+void render_segment(const std::vector<SegmentField>& fields, const FractalRenderOptions& o, int threads,
+                    bool pipelined, SegmentTiming& timing) {
+    const size_t n = fields.size();
+    const uint64_t wall0 = now_ns();
+    std::vector<SegmentSlot> slots(n);
+    std::mutex m;
+    std::condition_variable cv;
+    std::deque<size_t> ready;
+    std::exception_ptr failure;
+    std::atomic<size_t> next{0};
+    std::atomic<bool> stop{false};
+    uint64_t build_ns = 0, built_ns = 0;
+    auto work = [&]() {
+        for (;;) {
+            const size_t i = next.fetch_add(1);
+            if (i >= n || stop.load()) return;
+            try {
+                const uint64_t t0 = now_ns();
+                const SegmentField& f = fields[i];
+                SegmentSlot& slot = slots[i];
+                slot.context = build_reference(f.x.c_str(), f.y.c_str(), f.zoom.c_str(), f.max_iter, f.precision_bits,
+                                               8, false, 0, o.escape_radius_mode, o.coordinate_mode);
+                slot.direct = shallow_zoom(f.zoom.c_str(), slot.context->precision_bits);
+                if (slot.direct) {
+                    slot.direct_zoom = direct::parse_zoom_long(f.zoom.c_str());
+                } else {
+                    build_host_tables(*slot.context, slot.host);
+                    stage_tables(slot.host, slot.staged);
+                }
+                const uint64_t t1 = now_ns();
+                std::lock_guard<std::mutex> lock(m);
+                build_ns += t1 - t0;
+                built_ns = t1 - wall0;
+                ready.push_back(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lock(m);
+                if (!failure) failure = std::current_exception();
+                stop = true;
+            }
+            cv.notify_all();
+        }
+    };
+    std::vector<std::thread> pool;
+    const int workers = static_cast<int>(std::max<size_t>(1, std::min<size_t>(static_cast<size_t>(std::max(threads, 1)), n)));
+    for (int w = 0; w < workers; ++w) pool.emplace_back(work);
+    std::exception_ptr render_failure;
+    try {
+        for (size_t done = 0; done < n; ++done) {
+            size_t i = done;
+            {
+                std::unique_lock<std::mutex> lock(m);
+                cv.wait(lock, [&] { return failure != nullptr || (pipelined ? !ready.empty() : ready.size() == n); });
+                if (failure) break;
+                if (pipelined) {
+                    i = ready.front();
+                    ready.pop_front();
+                }
+            }
+            const SegmentField& f = fields[i];
+            SegmentSlot& slot = slots[i];
+            if (slot.direct) {
+                direct::DirectSetup d = direct::make_setup_values(
+                    f.width, f.height, slot.direct_zoom, slot.context->x_center, slot.context->y_center, f.max_iter, 0,
+                    0.0, 0.0, o.output_bias, o.escape_radius_mode, o.coordinate_mode, static_cast<uint32_t>(o.reserved[0]));
+                d.params.output = f.output;
+                render_direct_on_device(d.params, d.x.data(), d.y.data(), false, true);
+            } else {
+                upload_prepared_tables(*slot.context, slot.host, slot.staged);
+                const FieldRequest request{f.output, f.width, f.height, f.zoom.c_str(), f.max_iter, f.series_order,
+                                           f.series_block, o, nullptr, nullptr, true};
+                render_field(*slot.context, request);
+            }
+            timing.render_ms += last_timing().kernel_ms;
+            // the render has synchronised: the reference, its tables and the staging go
+            slot.staged.release();
+            slot.host = HostTables{};
+            slot.context.reset();
+        }
+    } catch (...) {
+        render_failure = std::current_exception();
+        stop = true;
+    }
+    for (std::thread& t : pool) t.join();
+    if (render_failure) std::rethrow_exception(render_failure);
+    if (failure) std::rethrow_exception(failure);
+    timing.build_ms = static_cast<double>(build_ns) * 1e-6;
+    timing.built_ms = static_cast<double>(built_ns) * 1e-6;
+    timing.wall_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
 }
 
 LastTiming last_timing() {

@@ -70,6 +70,28 @@ size_t batch_tile_count(const PreparedBatch& batch);
 // one render; fast_blocks_per_cu <= 0 keeps the current setting
 void batch_render(PreparedBatch& batch, int fast_blocks_per_cu, bool use_graph);
 
+// A segment of Mandelbrot fields whose references are built here: worker
+// threads build each field's MPFR reference and stage its tables in pinned
+// memory; the calling thread copies a staged reference to HBM on the side
+// stream (hipMemcpyAsync) and renders it on the compute stream while the
+// workers build the next ones.  With pipelined false every reference is
+// built before the first render (the unpipelined baseline).  Outputs are
+// device pointers; a field below zoom 1e6 renders through the direct
+// renderer at its centre, as a handle render does.
+struct SegmentField {
+    std::string x, y, zoom;
+    float* output;  // device memory, width * height floats
+    int width, height, max_iter, precision_bits, series_order, series_block;
+};
+struct SegmentTiming {
+    double wall_ms = 0.0;        // the whole call
+    double build_ms = 0.0;       // summed host time of the reference builds and staging
+    double render_ms = 0.0;      // summed HIP-event time of the field launches
+    double built_ms = 0.0;       // wall time until the last reference was staged
+};
+void render_segment(const std::vector<SegmentField>& fields, const FractalRenderOptions& options, int threads,
+                    bool pipelined, SegmentTiming& timing);
+
 // Alternate-formula field (Julia, burning ship, tricorn) of a reference
 // handle; request.planes must be null.
 void render_alternate(ReferenceContext& context, const FieldRequest& request, const uint32_t* tile_order = nullptr,

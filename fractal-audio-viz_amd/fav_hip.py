@@ -169,6 +169,9 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.fractal_hip_batch_render.restype = ctypes.c_int
     lib.fractal_hip_batch_destroy.argtypes = [ctypes.c_void_p]
     lib.fractal_hip_batch_destroy.restype = None
+    lib.fractal_hip_render_segment.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(NativeRenderOptions),
+                                               ctypes.c_int, ctypes.c_int, _c_double_p]
+    lib.fractal_hip_render_segment.restype = ctypes.c_int
     lib.fractal_hip_last_timing.argtypes = [_c_double_p, ctypes.c_int]
     lib.fractal_hip_last_timing.restype = ctypes.c_int
     lib.fractal_hip_reference_upload.argtypes = [ctypes.c_void_p]
@@ -387,6 +390,39 @@ class NativeBatchField(ctypes.Structure):
         ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("max_iter", ctypes.c_int32),
         ("series_order", ctypes.c_int32), ("series_block", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
+
+
+class NativeSegmentField(ctypes.Structure):
+    """FractalHipSegmentField (include/fractal_hip.h)."""
+
+    _fields_ = [
+        ("x_center", ctypes.c_char_p), ("y_center", ctypes.c_char_p), ("zoom_text", ctypes.c_char_p),
+        ("device_output", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+        ("max_iter", ctypes.c_int32), ("precision_bits", ctypes.c_int32), ("series_order", ctypes.c_int32),
+        ("series_block", ctypes.c_int32),
+    ]
+
+
+def render_segment(fields, outputs, options: "NativeRenderOptions", threads: int = 16, pipelined: bool = True,
+                   series_order: int = 3, series_block: int = 256, library=None) -> dict:
+    """Builds each field's reference on `threads` host threads and renders
+    the fields into the device buffers `outputs` (fields: dicts with x, y,
+    zoom, width, height, max_iter, bits), pipelined or build-then-render.
+    Returns the call's timings in ms."""
+    lib = library or load_library()
+    entries = (NativeSegmentField * len(fields))()
+    keep = []
+    for i, f in enumerate(fields):
+        texts = (f["x"].encode(), f["y"].encode(), f["zoom"].encode())
+        keep.append(texts)
+        entries[i] = NativeSegmentField(texts[0], texts[1], texts[2], outputs[i], f["width"], f["height"],
+                                        f["max_iter"], f["bits"], series_order, series_block)
+    timings = (ctypes.c_double * 4)()
+    status = lib.fractal_hip_render_segment(entries, len(fields), ctypes.byref(options), threads, int(pipelined),
+                                            timings)
+    if status != 0:
+        raise RuntimeError(lib.fractal_last_error().decode())
+    return dict(zip(("wall_ms", "build_ms", "render_ms", "built_ms"), list(timings)))
 
 
 class HipRuntime:

@@ -612,6 +612,39 @@ int fractal_hip_batch_set_order(void *batch, int order, const float *tile_costs,
 int fractal_hip_batch_render(void *batch, int fast_blocks_per_cu, int use_graph);
 void fractal_hip_batch_destroy(void *batch);
 
+/* ---- Pipelined segment (no counterpart in the reference) ---------------
+ * The keyframe fields of a segment with their references built here:
+ * build_threads host threads build each field's MPFR reference orbit, image
+ * series and BLA tables (fractal_create_reference_ex_options with image
+ * series order 8, Mandelbrot) and stage the tables in pinned memory; the
+ * calling thread copies each staged reference to HBM with
+ * hipMemcpyAsync on a side stream and renders the field on the compute
+ * stream while later references are still being built (the reference's
+ * pipeline builds every tier on a pool first, visualizer.py:16834-16990,
+ * then uploads and renders synchronously, renderer.cpp:9079-9252).
+ * pipelined 0 builds every reference before the first render.  Each field
+ * equals its single-field handle render (a field below zoom 1e6 goes
+ * through the direct renderer at its centre).  Outputs are device
+ * pointers.  timings, when set, receives 4 doubles: wall ms of the call,
+ * summed host build ms, summed field-launch ms, wall ms until the last
+ * reference was staged.  Returns 0 or 1 (fractal_last_error). */
+typedef struct FractalHipSegmentField {
+    const char *x_center;
+    const char *y_center;
+    const char *zoom_text;
+    float *device_output;
+    int32_t width;
+    int32_t height;
+    int32_t max_iter;
+    int32_t precision_bits;
+    int32_t series_order;
+    int32_t series_block;
+} FractalHipSegmentField;
+
+int fractal_hip_render_segment(const FractalHipSegmentField *fields, int count,
+                               const FractalRenderOptions *options, int build_threads,
+                               int pipelined, double *timings);
+
 /* fractal_colourise_kfp on device pointers: device_field (W*H floats) and
  * device_rgb (W*H*3 bytes) already live in HBM; lut is a host pointer. */
 int fractal_hip_colourise_kfp_device(
