@@ -132,7 +132,7 @@ def copy_calibration(mib, copies):
     return 0
 
 
-def batch_leg(lib, hip, steps):
+def batch_leg(lib, hip, steps, model=None):
     """One atlas segment (48 keyframe fields from zoom 1 to 1e100, each with
     its own 100-digit-centre reference, sizes drawn from 720p/1080p/4K, caps
     2000 + 1000/decade up to 100000; the three below 1e6 are direct-renderer
@@ -165,7 +165,25 @@ def batch_leg(lib, hip, steps):
             pixel_iterations += int(stats[1]) + int(stats[0])
         lib.fractal_set_stats_enabled(0)
         per_field_ms = min(batch.render_per_field() for _ in range(max(1, steps)))
-        trials, best = fav_batch.autotune(batch, repeats=1)
+        learned = None
+        costs = None
+        if model is not None:
+            # the cost model's order over the batch queue, and the makespan
+            # model of the batch's blocks (measured work maps) under it
+            costs, times = fav_batch.learned_tile_costs(batch, model)
+            measured = fav_batch.measured_block_costs(batch)
+            slots = fav_sched.resident_blocks(blocks_per_cu=5)
+            order = np.argsort(-costs.astype(np.float64), kind="stable")
+            learned = {"preview_ms": round(times["preview_ms"], 3),
+                       "makespan_field_major": fav_sched.makespan(measured, np.arange(measured.size), slots),
+                       "makespan_learned": fav_sched.makespan(measured, order, slots),
+                       "makespan_measured_order": fav_sched.makespan(
+                           measured, np.argsort(-measured, kind="stable"), slots),
+                       "makespan_bound": round(max(float(measured.sum()) / slots, float(measured.max())), 1)}
+        trials, best = fav_batch.autotune(batch, costs=costs, repeats=1)
+        if learned is not None:
+            learned["learned_order_ms"] = round(min(t["ms"] for t in trials if t["order"] == 2), 3)
+            learned["field_major_ms"] = round(min(t["ms"] for t in trials if t["order"] == 0), 3)
         batched_ms = min(batch.render(0, graph=True) for _ in range(max(1, steps)))
         # end to end with the references built here: every reference first,
         # then the renders, against references built on host threads while
@@ -177,7 +195,7 @@ def batch_leg(lib, hip, steps):
                 "batched_ms": round(batched_ms, 3), "per_field_launches_ms": round(per_field_ms, 3),
                 "speedup_vs_per_field": round(per_field_ms / batched_ms, 3),
                 "pixel_iterations": pixel_iterations, "reference_build_s": round(batch.reference_build_s, 3),
-                "autotuned": best, "trials": len(trials),
+                "autotuned": best, "trials": len(trials), "learned_order": learned,
                 "end_to_end": {"build_then_render_s": round(serial["wall_ms"] * 1e-3, 3),
                                "pipelined_s": round(overlapped["wall_ms"] * 1e-3, 3),
                                "speedup": round(serial["wall_ms"] / overlapped["wall_ms"], 3),
@@ -190,45 +208,67 @@ def batch_leg(lib, hip, steps):
 
 
 def learned_leg(lib, repeats=2):
-    """The learned tile scheduler: a cost model trained on the GPU over the
-    seeded training set (12 Julia and 6 tricorn fields, 1920x1080, cap
-    20000), then held-out fields at 3840x2160, zoom 1e100, cap 200000 (the
-    tricorn branching junction and Julia seeds 1, 2, 4) rendered in the
-    plain grid and in the learned heaviest-first order.  Net gain counts the
-    preview, features and model time; prediction error is against each
-    field's measured tile costs."""
-    model, training = fav_sched.train_model(fav_sched.training_fields(), library=lib)
+    """The learned tile scheduler: one cost model trained on the GPU over the
+    seeded training fields of both families (Mandelbrot map: the 640x360
+    zoom-1e12 field, the 1920x1080 zoom-1e100 field, 2560x1440 near-nucleus
+    fields and atlas keyframes; Julia and tricorn fields at 1920x1080, cap
+    20000), then held-out fields rendered in the plain grid and in the
+    learned heaviest-first order: the tricorn branching junction and Julia
+    seeds 1, 2, 4 at 3840x2160, Julia seed 1 at 7680x4320 (zoom 1e100, cap
+    200000), a near-nucleus 2560x1440 zoom-1e30 Mandelbrot field, the
+    Misiurewicz 1920x1080 zoom-1e100 field and the 3840x2160 zoom-1e100
+    interior field.  Net gain counts the preview, features and model time;
+    prediction error and the makespan model (fav_sched.makespan_report) are
+    against each field's measured work map.  Returns the leg's JSON and the
+    model."""
+    model, training = fav_sched.train_model(fav_sched.mandelbrot_training_fields() + fav_sched.training_fields(),
+                                            library=lib)
     fields = []
     cases = [fav_workloads.tricorn_field("branching-junction", 3840, 2160, "1e100", 200000)] + [
         fav_workloads.julia_field(seed=seed, width=3840, height=2160, zoom="1e100", max_iter=200000)
-        for seed in (1, 2, 4)]
+        for seed in (1, 2, 4)] + [
+        fav_workloads.julia_field(seed=1, width=7680, height=4320, zoom="1e100", max_iter=200000),
+        fav_workloads.near_boundary_field(0, 1, 30), dict(fav_workloads.misiurewicz_field(), formula=0),
+        dict(fav_workloads.field_1e100_1080p(), formula=0, width=3840, height=2160,
+             name="field_3840x2160_zoom1e100_cap100000")]
     for case in cases:
         w, h, z, n = case["width"], case["height"], case["zoom"], case["max_iter"]
+        opts, series = fav_sched.case_options(case), fav_sched._series(case)
         ref = fav_sched.reference_for(case, lib)
         try:
-            _, work = ref.render_scheduled(w, h, z, n, work=True)
+            work = fav_sched.field_work(ref, case)
             costs = fav_sched.tile_costs(work, w, h)
-            features, _ = fav_sched.preview_features(ref, w, h, z, n)
-            report = fav_sched.prediction_report(model.predict(features), costs)
+            features, _ = fav_sched.field_features(ref, case)
+            predicted = model.predict(features)
+            report = fav_sched.prediction_report(predicted, costs)
+            schedule = fav_sched.makespan_report(work, predicted, w, h, fav_sched.kernel_slots(case.get("formula", 0)))
             plain = []
             for _ in range(repeats):
-                ref.render_scheduled(w, h, z, n)
+                ref.render_scheduled(w, h, z, n, series_order=series[0], series_block=series[1], options=opts)
                 plain.append(fav_hip.last_timing()["kernel_ms"])
-            learned = [fav_sched.render_learned(ref, model, w, h, z, n)[2] for _ in range(repeats)]
+            learned = [fav_sched.render_learned(ref, model, w, h, z, n, options=opts, series=series)[2]
+                       for _ in range(repeats)]
             best = min(learned, key=lambda t: t["preview_ms"] + t["model_ms"] + t["render_ms"])
             total = best["preview_ms"] + best["model_ms"] + best["render_ms"]
             fields.append({"field": case["name"], "plain_ms": round(min(plain), 3),
                            "learned_total_ms": round(total, 3), "preview_ms": round(best["preview_ms"], 3),
                            "model_ms": round(best["model_ms"], 3), "ordered_render_ms": round(best["render_ms"], 3),
                            "net_gain_pct": round(100.0 * (min(plain) - total) / min(plain), 2),
+                           "makespan_model": {k: round(v, 2) if isinstance(v, float) else v
+                                              for k, v in schedule.items()},
                            "prediction": {k: round(v, 4) if isinstance(v, float) else v for k, v in report.items()}})
         finally:
             ref.close()
-    return {"workload": "3840x2160, zoom 1e100, cap 200000: tricorn branching junction; Julia seeds 1 2 4 "
-                        "at their repelling fixed point",
-            "preview": f"quarter size, {fav_sched.PREVIEW_WORK_LIMIT} passes per pixel",
+    alternate = [f["net_gain_pct"] for f in fields if not f["field"].startswith(("mandelbrot", "misiurewicz", "field"))]
+    return {"workload": "held-out fields: tricorn branching junction and Julia seeds 1 2 4 at 3840x2160, Julia "
+                        "seed 1 at 7680x4320 (zoom 1e100, cap 200000); Mandelbrot near-nucleus 2560x1440 zoom "
+                        "1e30, Misiurewicz 1920x1080 zoom 1e100, interior 3840x2160 zoom 1e100",
+            "preview": f"quarter size, {fav_sched.PREVIEW_WORK_LIMIT} passes per pixel (alternate formulas), "
+                       f"cap {fav_sched.MANDELBROT_PREVIEW_CAP} iterations (Mandelbrot map)",
             "training": {k: round(v, 4) if isinstance(v, float) else v for k, v in training.items()},
-            "mean_net_gain_pct": round(float(np.mean([f["net_gain_pct"] for f in fields])), 2), "fields": fields}
+            "mean_net_gain_pct_alternate": round(float(np.mean(alternate)), 2),
+            "mean_makespan_gain_pct": round(float(np.mean([f["makespan_model"]["gain_pct"] for f in fields])), 2),
+            "fields": fields}, model
 
 
 def atlas_leg(lib, repeats=5):
@@ -474,7 +514,7 @@ def main():
         raise SystemExit(f"no gfx950 device: {lib.fractal_last_error()!r}")
     hip = fav_hip.HipRuntime()
     if args.learned_only:
-        print(json.dumps({"learned_schedule": learned_leg(lib)}))
+        print(json.dumps({"learned_schedule": learned_leg(lib)[0]}))
         return 0
     if args.batch_only:
         print(json.dumps({"batched": batch_leg(lib, hip, 2)}))
@@ -546,8 +586,8 @@ def main():
     hip.to_host(rgb, d_rgb)
     floatexp = floatexp_leg(lib, hip, ref, work, opts, d_field, field, orbit_iteration, args)
     boundary = boundary_leg(lib, hip, kfp, lut, args.steps)
-    batched = None if args.no_batch else batch_leg(lib, hip, 2)
-    learned = None if args.no_learned else learned_leg(lib)
+    learned, model = (None, None) if args.no_learned else learned_leg(lib)
+    batched = None if args.no_batch else batch_leg(lib, hip, 2, model)
     atlas = atlas_leg(lib)
     hip.free(d_field)
     hip.free(d_planes_block)

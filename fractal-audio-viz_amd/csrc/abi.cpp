@@ -1049,10 +1049,10 @@ namespace {
 
 constexpr int kMaxPreviewWork = 1 << 24;
 
-// Checks shared by the scheduled, preview and learned entries: an
-// alternate-formula handle, zoom >= 1e6 (perturbation fields), a prepared
-// cap and matching escape mode.
-std::shared_ptr<ReferenceContext> alternate_context(void* handle, const char* zoom, int width, int height,
+// Checks shared by the scheduled, preview and learned entries: a handle of
+// any formula, zoom >= 1e6 (perturbation fields), a prepared cap and
+// matching escape mode.
+std::shared_ptr<ReferenceContext> scheduled_context(void* handle, const char* zoom, int width, int height,
                                                     int max_iter, int series_order, int series_block,
                                                     const FractalRenderOptions& o) {
     if (!zoom || !handle || !valid_dimensions(width, height) || !valid_iterations(max_iter)
@@ -1061,9 +1061,6 @@ std::shared_ptr<ReferenceContext> alternate_context(void* handle, const char* zo
     require_hip_backend(o);
     auto context = acquire(handle);
     if (!context) throw std::runtime_error("invalid or already-destroyed reference handle");
-    if (context->formula == 0)
-        throw std::runtime_error("scheduled single-field renders are for the alternate formulas; "
-                                 "Mandelbrot fields take a tile order through a batch");
     if (context->escape_radius_mode != o.escape_radius_mode)
         throw std::runtime_error("reference escape-radius mode does not match render options");
     if (max_iter > context->requested_max_iter)
@@ -1103,10 +1100,21 @@ int fractal_hip_render_reference_scheduled(float* output, int width, int height,
         if (!output || tile_count < 0 || (tile_order && tile_count == 0))
             throw std::runtime_error("invalid native render dimensions or handle");
         const FractalRenderOptions o = checked_options(options);
-        const auto context = alternate_context(handle, zoom, width, height, max_iter, series_order, series_block, o);
+        const auto context = scheduled_context(handle, zoom, width, height, max_iter, series_order, series_block, o);
+        if (tile_order) fav::check_tile_order(tile_order, static_cast<size_t>(tile_count), width, height);
+        uint64_t stats[fav::kStatsFields] = {};
+        const bool collect = stats_enabled.load(std::memory_order_relaxed);
         fav::FieldRequest request{output, width, height, zoom, max_iter, series_order, series_block, o,
-                                  nullptr, nullptr, false};
-        fav::render_alternate(*context, request, tile_order, static_cast<size_t>(tile_count), work);
+                                  nullptr, collect ? stats : nullptr, false};
+        if (context->formula != 0) {
+            fav::render_alternate(*context, request, tile_order, static_cast<size_t>(tile_count), work);
+        } else {
+            request.tile_order = tile_order;
+            request.tile_count = static_cast<size_t>(tile_count);
+            request.work = work;
+            fav::render_field(*context, request);
+        }
+        if (collect) publish_stats(stats);
     });
 }
 
@@ -1118,10 +1126,11 @@ int fractal_hip_preview_features(float* features, int width, int height, const c
         if (work_limit <= 0 || work_limit > kMaxPreviewWork)
             throw std::runtime_error("preview work limit must be in [1, 16777216]");
         const FractalRenderOptions o = checked_options(options);
-        const auto context = alternate_context(handle, zoom, width, height, max_iter, series_order, series_block, o);
+        const auto context = scheduled_context(handle, zoom, width, height, max_iter, series_order, series_block, o);
         fav::FieldRequest request{nullptr, width, height, zoom, max_iter, series_order, series_block, o,
                                   nullptr, nullptr, false};
-        fav::alternate_preview_features(*context, request, work_limit, features);
+        if (context->formula != 0) fav::alternate_preview_features(*context, request, work_limit, features);
+        else fav::field_preview_features(*context, request, work_limit, features);
     });
 }
 
@@ -1134,10 +1143,13 @@ int fractal_hip_render_reference_learned(float* output, int width, int height, c
         if (work_limit <= 0 || work_limit > kMaxPreviewWork)
             throw std::runtime_error("preview work limit must be in [1, 16777216]");
         const FractalRenderOptions o = checked_options(options);
-        const auto context = alternate_context(handle, zoom, width, height, max_iter, series_order, series_block, o);
+        const auto context = scheduled_context(handle, zoom, width, height, max_iter, series_order, series_block, o);
         fav::FieldRequest request{output, width, height, zoom, max_iter, series_order, series_block, o,
                                   nullptr, nullptr, false};
-        fav::render_alternate_learned(*context, request, weights, work_limit, tile_order, timings);
+        if (context->formula != 0)
+            fav::render_alternate_learned(*context, request, weights, work_limit, tile_order, timings);
+        else
+            fav::render_field_learned(*context, request, weights, work_limit, tile_order, timings);
     });
 }
 

@@ -86,7 +86,7 @@ struct LaunchState {
     hipEvent_t mark[3] = {nullptr, nullptr, nullptr};
     DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords, fx_counts, tile_order, work,
         mlp_y, mlp_act, mlp_delta, mlp_err2, mlp_partial, mlp_grad, mlp_features, mlp_weights, preview_offsets,
-        preview_output, preview_work,
+        preview_output, preview_work, preview_state, preview_stats,
         atlas_tiles, atlas_axes, atlas_field, points, aurora_in, aurora_out, kfp_planes;
     int compute_units = 256;
     double wall_clock_khz = 0.0;
@@ -360,10 +360,12 @@ void upload_points(LaunchState& s, const FieldRequest& r, PointStaging& staging,
     e = reinterpret_cast<const int32_t*>(dp + e_at);
 }
 
-void render_field(ReferenceContext& c, const FieldRequest& r) {
-    const int w = r.width, h = r.height;
-    PreparedRender prep;
-    prepare_render(c, r, prep);
+namespace {
+
+// Launch parameters of one Mandelbrot field: the per-frame values of `prep`
+// with the reference's device tables bound (uploaded first, with the
+// plain-double rung tables of the selected tier on first use).
+RenderParams bind_field_tables(ReferenceContext& c, const PreparedRender& prep) {
     ensure_device_tables(c);
     const DeviceTables& t = *c.device;
     RenderParams p = prep.params;
@@ -386,15 +388,21 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     }
     p.cubic = t.cubic;
     p.cubic_dir = t.cubic_dir;
+    return p;
+}
 
-    LaunchState& s = state();
-    std::lock_guard<std::mutex> lock(s.mutex);
-    init_locked(s);
+// The per-frame uploads of one field on the library stream (lock held):
+// pixel offsets into `offsets`, the points of a point render, the pixel on
+// the reference, the time budget and the state store `state`.  `prep` and
+// `points` must stay alive until the stream has run the copies.
+void bind_field_frame(RenderParams& p, const PreparedRender& prep, const FieldRequest& r, LaunchState& s,
+                      DeviceBuffer& offsets, DeviceBuffer& state_buffer, PointStaging& points) {
+    const int w = r.width, h = r.height;
     const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
     const std::vector<double>& off_m = prep.offset_m;
     const std::vector<int32_t>& off_e = prep.offset_e;
     // offsets: doubles first, then the int32 exponents
-    char* doff = static_cast<char*>(s.offsets.get(off_m.size() * 12 + 512));
+    char* doff = static_cast<char*>(offsets.get(off_m.size() * 12 + 512));
     const size_t exp_offset = (off_m.size() * 8 + 255) & ~size_t(255);
     check(hipMemcpyAsync(doff, off_m.data(), off_m.size() * 8, hipMemcpyHostToDevice, s.stream), "offsets");
     check(hipMemcpyAsync(doff + exp_offset, off_e.data(), off_e.size() * 4, hipMemcpyHostToDevice, s.stream), "offsets");
@@ -402,13 +410,10 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     p.y_off_m = p.x_off_m + w;
     p.x_off_e = reinterpret_cast<const int32_t*>(doff + exp_offset);
     p.y_off_e = p.x_off_e + w;
-    PointStaging points;
-    size_t out_count = pixels;
     p.reference_pixel = -1;
     if (r.points) {
         upload_points(s, r, points, p.point_re, p.point_im, p.point_e);
         p.point_count = r.point_count;
-        out_count = static_cast<size_t>(r.point_count);
         // a point with dc = 0 stays on fast_kernel's path
     } else if (p.plain_pass) {
         // the pixel whose column and row offsets are both zero (dc = 0)
@@ -417,6 +422,36 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
         for (int y = 0; y < h && zy < 0; ++y) if (off_m[static_cast<size_t>(w + y)] == 0.0) zy = y;
         if (zx >= 0 && zy >= 0) p.reference_pixel = static_cast<int64_t>(zy) * w + zx;
     }
+    void* budget = s.budget.get(256);
+    p.budget_start = static_cast<const unsigned long long*>(budget);
+    p.budget_flag = reinterpret_cast<int*>(static_cast<char*>(budget) + 8);
+    p.budget_ticks = r.options.time_budget_ms > 0
+        ? static_cast<unsigned long long>(r.options.time_budget_ms * s.wall_clock_khz)
+        : 0ULL;
+    bind_state_store(p, state_buffer.get(pixels * kStateBytesPerPixel + 20 * 256), pixels);
+}
+
+// Uploads a 32x32 tile order (checked by the caller) on the library stream.
+void bind_tile_order(RenderParams& p, LaunchState& s, const uint32_t* order, size_t count) {
+    void* d = s.tile_order.get(count * 4);
+    check(hipMemcpyAsync(d, order, count * 4, hipMemcpyHostToDevice, s.stream), "H2D tile order");
+    p.tile_order = static_cast<const uint32_t*>(d);
+    p.tile_count = static_cast<int32_t>(count);
+}
+
+}  // namespace
+
+namespace {
+
+// One Mandelbrot field render with `p` bound to the reference's tables
+// (bind_field_tables over `prep`); lock held.
+void render_field_locked(const FieldRequest& r, const PreparedRender& prep, RenderParams p, LaunchState& s) {
+    const int w = r.width, h = r.height;
+    const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
+    PointStaging points;
+    bind_field_frame(p, prep, r, s, s.offsets, s.state, points);
+    const size_t out_count = r.points ? static_cast<size_t>(r.point_count) : pixels;
+    if (r.tile_order) bind_tile_order(p, s, r.tile_order, r.tile_count);
 
     const bool want_planes = r.planes != nullptr;
     if (r.device_pointers) {
@@ -441,23 +476,20 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
             p.plane_test2 = p.plane_test1 + pixels;
         }
     }
-    if (r.stats) {
+    // the work map needs the statistics kernels (their per-pixel counters)
+    const bool counters = r.stats != nullptr || r.work != nullptr;
+    if (counters) {
         p.stats = static_cast<unsigned long long*>(s.stats.get(kStatsFields * 8));
         check(hipMemsetAsync(p.stats, 0, kStatsFields * 8, s.stream), "stats clear");
     }
-    void* budget = s.budget.get(256);
-    p.budget_start = static_cast<const unsigned long long*>(budget);
-    p.budget_flag = reinterpret_cast<int*>(static_cast<char*>(budget) + 8);
-    p.budget_ticks = r.options.time_budget_ms > 0
-        ? static_cast<unsigned long long>(r.options.time_budget_ms * s.wall_clock_khz)
-        : 0ULL;
-
-    bind_state_store(p, s.state.get(pixels * kStateBytesPerPixel + 20 * 256), pixels);
+    if (r.work) {
+        p.work = r.device_pointers ? r.work : static_cast<int32_t*>(s.work.get(pixels * 4));
+        check(hipMemsetAsync(p.work, 0, pixels * 4, s.stream), "work clear");
+    }
 
     const uint64_t wall0 = now_ns();
     check(hipEventRecord(s.start, s.stream), "event");
-    check(launch_field_kernel(p, want_planes, r.stats != nullptr, s.compute_units * 8, s.stream, s.side, s.fork,
-                              s.join),
+    check(launch_field_kernel(p, want_planes, counters, s.compute_units * 8, s.stream, s.side, s.fork, s.join),
           "field kernel launch");
     check(hipEventRecord(s.stop, s.stream), "event");
     if (!r.device_pointers) {
@@ -471,6 +503,7 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
             check(hipMemcpyAsync(r.planes->test1, p.plane_test1, pixels * 8, hipMemcpyDeviceToHost, s.stream), "D2H");
             check(hipMemcpyAsync(r.planes->test2, p.plane_test2, pixels * 8, hipMemcpyDeviceToHost, s.stream), "D2H");
         }
+        if (r.work) check(hipMemcpyAsync(r.work, p.work, pixels * 4, hipMemcpyDeviceToHost, s.stream), "D2H work");
     }
     if (r.stats) {
         check(hipMemcpyAsync(r.stats, p.stats, kStatsFields * 8, hipMemcpyDeviceToHost, s.stream), "D2H stats");
@@ -482,6 +515,18 @@ void render_field(ReferenceContext& c, const FieldRequest& r) {
     s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
     s.timing.pixels = static_cast<double>(pixels);
     if (r.stats) r.stats[22] = static_cast<uint64_t>(static_cast<double>(kernel_ms) * 1e6);  // render_ns
+}
+
+}  // namespace
+
+void render_field(ReferenceContext& c, const FieldRequest& r) {
+    PreparedRender prep;
+    prepare_render(c, r, prep);
+    const RenderParams p = bind_field_tables(c, prep);
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    render_field_locked(r, prep, p, s);
 }
 
 void colourise_kfp_on_device(const kfp::KfpParams& host, bool device_pointers, const FractalKfpPlanes* planes) {
@@ -906,8 +951,6 @@ void render_field_floatexp(ReferenceContext& c, const FieldRequest& r, int32_t* 
     s.timing.pixels = static_cast<double>(pixels);
 }
 
-namespace {
-
 void check_tile_order(const uint32_t* tile_order, size_t tile_count, int width, int height) {
     const size_t tiles = static_cast<size_t>((width + 31) / 32) * static_cast<size_t>((height + 31) / 32);
     if (tile_count != tiles) throw std::runtime_error("tile order must list every 32x32 tile once");
@@ -916,6 +959,8 @@ void check_tile_order(const uint32_t* tile_order, size_t tile_count, int width, 
     for (size_t i = 0; i < tiles; ++i)
         if (sorted[i] != i) throw std::runtime_error("tile order must list every 32x32 tile once");
 }
+
+namespace {
 
 // Launch parameters of one alternate-formula render with the reference's
 // device tables bound and the per-column / per-row offsets uploaded to
@@ -1005,6 +1050,69 @@ float* launch_preview_features(const ReferenceContext& c, const FieldRequest& r,
     return features;
 }
 
+// The cost model over the device features of `tiles` tiles with the host
+// `weights`, and the tiles sorted by decreasing prediction (equal or NaN
+// predictions keep row-major order); lock held.  s.mark[2] is recorded
+// after the model.
+std::vector<uint32_t> heaviest_first(LaunchState& s, const float* features, int tiles, const float* weights) {
+    float* w = static_cast<float*>(s.mlp_weights.get(mlp::kParams * sizeof(float)));
+    float* y = static_cast<float*>(s.mlp_y.get(static_cast<size_t>(tiles) * sizeof(float)));
+    check(hipMemcpyAsync(w, weights, mlp::kParams * sizeof(float), hipMemcpyHostToDevice, s.stream), "H2D weights");
+    check(mlp::forward(w, features, tiles, y, nullptr, s.stream), "cost model forward");
+    check(hipEventRecord(s.mark[2], s.stream), "event");
+    std::vector<float> predicted(static_cast<size_t>(tiles));
+    check(hipMemcpyAsync(predicted.data(), y, predicted.size() * sizeof(float), hipMemcpyDeviceToHost, s.stream),
+          "D2H costs");
+    check(hipStreamSynchronize(s.stream), "cost model");
+    std::vector<uint32_t> order(static_cast<size_t>(tiles));
+    for (int i = 0; i < tiles; ++i) order[static_cast<size_t>(i)] = static_cast<uint32_t>(i);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const float ca = predicted[a], cb = predicted[b];
+        return (ca == ca ? ca : -INFINITY) > (cb == cb ? cb : -INFINITY);
+    });
+    return order;
+}
+
+// Quarter-size preview of Mandelbrot field `r` and the tile features of the
+// full field into s.mlp_features (lock held).  The preview `q` runs with cap
+// min(max_iter, work_limit): every loop pass advances at least one
+// iteration, so no pixel takes more than work_limit passes.  Its work map
+// (loop passes per pixel, from the statistics kernels) feeds the features
+// exactly as an alternate-formula preview's does.  `p` is bound to the
+// reference's tables for `prep` (the preview's per-frame values); `points`
+// must outlive the stream work.
+float* launch_field_preview(const FieldRequest& r, const FieldRequest& q, const PreparedRender& prep, RenderParams p,
+                            double log10z, LaunchState& s, PointStaging& points, hipEvent_t before,
+                            hipEvent_t after) {
+    bind_field_frame(p, prep, q, s, s.preview_offsets, s.preview_state, points);
+    const size_t pixels = static_cast<size_t>(q.width) * static_cast<size_t>(q.height);
+    p.output = static_cast<float*>(s.preview_output.get(pixels * sizeof(float)));
+    p.stats = static_cast<unsigned long long*>(s.preview_stats.get(kStatsFields * 8));
+    p.work = static_cast<int32_t*>(s.preview_work.get(pixels * 4));
+    const int tiles_x = (r.width + mlp::kTile - 1) / mlp::kTile, tiles_y = (r.height + mlp::kTile - 1) / mlp::kTile;
+    float* features = static_cast<float*>(
+        s.mlp_features.get(static_cast<size_t>(tiles_x) * tiles_y * mlp::kInputs * sizeof(float)));
+    check(hipEventRecord(before, s.stream), "event");
+    check(hipMemsetAsync(p.stats, 0, kStatsFields * 8, s.stream), "stats clear");
+    check(hipMemsetAsync(p.work, 0, pixels * 4, s.stream), "work clear");
+    check(launch_field_kernel(p, false, true, s.compute_units * 8, s.stream, s.side, s.fork, s.join),
+          "preview kernel launch");
+    check(mlp::features(p.work, q.width, q.height, tiles_x, tiles_y, static_cast<float>(log10z / 100.0), features,
+                        s.stream),
+          "tile features");
+    check(hipEventRecord(after, s.stream), "event");
+    return features;
+}
+
+FieldRequest field_preview_request(const FieldRequest& r, int work_limit) {
+    FieldRequest q = preview_request(r);
+    q.max_iter = std::min(r.max_iter, work_limit);
+    q.work = nullptr;
+    q.tile_order = nullptr;
+    q.tile_count = 0;
+    return q;
+}
+
 }  // namespace
 
 void render_alternate(ReferenceContext& c, const FieldRequest& r, const uint32_t* tile_order, size_t tile_count,
@@ -1082,22 +1190,7 @@ void render_alternate_learned(ReferenceContext& c, const FieldRequest& r, const 
     const int tiles = ((r.width + 31) / 32) * ((r.height + 31) / 32);
     // preview, features, cost model
     const float* features = launch_preview_features(c, r, preview_prep, work_limit, s, s.mark[0], s.mark[1]);
-    float* w = static_cast<float*>(s.mlp_weights.get(mlp::kParams * sizeof(float)));
-    float* y = static_cast<float*>(s.mlp_y.get(static_cast<size_t>(tiles) * sizeof(float)));
-    check(hipMemcpyAsync(w, weights, mlp::kParams * sizeof(float), hipMemcpyHostToDevice, s.stream), "H2D weights");
-    check(mlp::forward(w, features, tiles, y, nullptr, s.stream), "cost model forward");
-    check(hipEventRecord(s.mark[2], s.stream), "event");
-    std::vector<float> predicted(static_cast<size_t>(tiles));
-    check(hipMemcpyAsync(predicted.data(), y, predicted.size() * sizeof(float), hipMemcpyDeviceToHost, s.stream),
-          "D2H costs");
-    check(hipStreamSynchronize(s.stream), "cost model");
-    // heaviest first; equal (or NaN) predictions keep row-major order
-    std::vector<uint32_t> order(static_cast<size_t>(tiles));
-    for (int i = 0; i < tiles; ++i) order[static_cast<size_t>(i)] = static_cast<uint32_t>(i);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        const float ca = predicted[a], cb = predicted[b];
-        return (ca == ca ? ca : -INFINITY) > (cb == cb ? cb : -INFINITY);
-    });
+    const std::vector<uint32_t> order = heaviest_first(s, features, tiles, weights);
     if (order_out) std::copy(order.begin(), order.end(), order_out);
     const uint64_t sorted_ns = now_ns();
     // the field in that order
@@ -1126,6 +1219,56 @@ void render_alternate_learned(ReferenceContext& c, const FieldRequest& r, const 
         timings[3] = static_cast<double>(sorted_ns - wall0) * 1e-6;  // wall time up to the ordered launch
         timings[4] = s.timing.total_ms;
     }
+}
+
+void field_preview_features(ReferenceContext& c, const FieldRequest& r, int work_limit, float* features) {
+    const FieldRequest q = field_preview_request(r, work_limit);
+    PreparedRender prep;
+    prepare_render(c, q, prep);
+    const RenderParams p = bind_field_tables(c, prep);
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const size_t count = static_cast<size_t>((r.width + 31) / 32) * ((r.height + 31) / 32) * mlp::kInputs;
+    PointStaging points;
+    const float* d = launch_field_preview(r, q, prep, p, log10_zoom(c, r.zoom_text), s, points, s.start, s.stop);
+    check(hipMemcpyAsync(features, d, count * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H features");
+    check(hipStreamSynchronize(s.stream), "preview");
+    const float ms = elapsed(s.start, s.stop);
+    s.timing = LastTiming{ms, ms, static_cast<double>(count / mlp::kInputs)};
+}
+
+void render_field_learned(ReferenceContext& c, const FieldRequest& r, const float* weights, int work_limit,
+                          uint32_t* order_out, double* timings) {
+    const FieldRequest q = field_preview_request(r, work_limit);
+    PreparedRender preview_prep, prep;
+    prepare_render(c, q, preview_prep);
+    prepare_render(c, r, prep);
+    const RenderParams preview_params = bind_field_tables(c, preview_prep);
+    const RenderParams params = bind_field_tables(c, prep);
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const uint64_t wall0 = now_ns();
+    const int tiles = ((r.width + 31) / 32) * ((r.height + 31) / 32);
+    PointStaging points;
+    const float* features = launch_field_preview(r, q, preview_prep, preview_params, log10_zoom(c, r.zoom_text), s,
+                                                 points, s.mark[0], s.mark[1]);
+    const std::vector<uint32_t> order = heaviest_first(s, features, tiles, weights);
+    if (order_out) std::copy(order.begin(), order.end(), order_out);
+    const uint64_t sorted_ns = now_ns();
+    FieldRequest ordered = r;
+    ordered.tile_order = order.data();
+    ordered.tile_count = order.size();
+    render_field_locked(ordered, prep, params, s);
+    if (timings) {
+        timings[0] = elapsed(s.mark[0], s.mark[1]);  // preview render + features
+        timings[1] = elapsed(s.mark[1], s.mark[2]);  // weights upload + cost model
+        timings[2] = s.timing.kernel_ms;             // ordered render
+        timings[3] = static_cast<double>(sorted_ns - wall0) * 1e-6;
+        timings[4] = static_cast<double>(now_ns() - wall0) * 1e-6;
+    }
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
 }
 
 // ---- learned tile-cost model -------------------------------------------------

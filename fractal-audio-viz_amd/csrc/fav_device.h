@@ -267,6 +267,15 @@ struct RenderParams {
     // the reference), rendered by reference_pixel_kernel on a side stream
     // while the plain pass runs; -1 when there is none
     int64_t reference_pixel;
+    // schedule: when tile_order is set, plain_kernel and fast_kernel run a
+    // 1-D grid of four 16x16 blocks per 32x32 tile, tiles in tile_order
+    // (tile_count row-major 32x32 tile indices); null = the 2-D grid
+    const uint32_t* tile_order;
+    int32_t tile_count;
+    // with statistics on: each pixel's loop passes (BLA blocks, exact,
+    // replayed and tail steps) added to work[pixel] by every kernel that
+    // runs it (the caller zeroes it); null = no work map
+    int32_t* work;
 };
 
 namespace direct { struct DirectParams; }

@@ -33,6 +33,12 @@ struct FieldRequest {
     const SC* points = nullptr;
     int point_count = 0;
     FE point_radius{};             // sqrt of the largest |dc|^2
+    // Mandelbrot schedule hooks: 32x32 tiles launched in tile_order (checked
+    // by the caller; null = the 2-D grid) and each pixel's loop passes
+    // written to `work` (host or device like `output`; null = none)
+    const uint32_t* tile_order = nullptr;
+    size_t tile_count = 0;
+    int32_t* work = nullptr;
 };
 
 struct LastTiming {
@@ -92,6 +98,10 @@ struct SegmentTiming {
 void render_segment(const std::vector<SegmentField>& fields, const FractalRenderOptions& options, int threads,
                     bool pipelined, SegmentTiming& timing);
 
+// Throws unless tile_order lists each of the ceil(w/32)*ceil(h/32)
+// row-major 32x32 tiles exactly once.
+void check_tile_order(const uint32_t* tile_order, size_t tile_count, int width, int height);
+
 // Alternate-formula field (Julia, burning ship, tricorn) of a reference
 // handle; request.planes must be null.
 void render_alternate(ReferenceContext& context, const FieldRequest& request, const uint32_t* tile_order = nullptr,
@@ -107,6 +117,14 @@ void alternate_preview_features(ReferenceContext& context, const FieldRequest& r
 // ordered render ms, wall ms up to the ordered launch, total wall ms.
 void render_alternate_learned(ReferenceContext& context, const FieldRequest& request, const float* weights,
                               int work_limit, uint32_t* order_out, double* timings);
+// The same two for a Mandelbrot perturbation field (zoom >= 1e6): the
+// preview runs the field kernels at quarter size with cap
+// min(max_iter, work_limit) and its per-pixel loop passes as the work map;
+// the ordered render is render_field with the order as request.tile_order.
+void field_preview_features(ReferenceContext& context, const FieldRequest& request, int work_limit,
+                            float* features);
+void render_field_learned(ReferenceContext& context, const FieldRequest& request, const float* weights,
+                          int work_limit, uint32_t* order_out, double* timings);
 
 // Learned tile-cost model (sched_mlp.h shapes); every pointer is device
 // memory and the caller checks n and steps.  loss_history gets one mean

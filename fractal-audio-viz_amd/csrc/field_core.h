@@ -87,6 +87,15 @@ struct Counters {
     }
     FAV_HD void add(int field, unsigned long long n = 1) { v[field] += n; }
     FAV_HD void max(int field, unsigned long long n) { v[field] = n > v[field] ? n : v[field]; }
+    // loop passes of this pixel so far: BLA blocks (linear and cubic), exact
+    // steps (the plain-double tail's included) and replayed steps
+    FAV_HD unsigned long long passes() const {
+        return v[ST_LINEAR_BLOCKS] + v[ST_CUBIC_BLOCKS] + v[ST_EXACT] + v[ST_REPLAY];
+    }
+    // adds the passes to the pixel's work-map entry (none when work is null)
+    FAV_HD void flush_work(int32_t* work, size_t index) const {
+        if (work) work[index] += static_cast<int32_t>(passes());
+    }
     FAV_HD void flush(unsigned long long* out) {
         for (int i = 0; i < kStatsFields; ++i) {
             if (v[i] == 0) continue;
@@ -101,6 +110,7 @@ struct Counters<false> {
     FAV_HD void clear() {}
     FAV_HD void add(int, unsigned long long = 1) {}
     FAV_HD void max(int, unsigned long long) {}
+    FAV_HD void flush_work(int32_t*, size_t) const {}
     FAV_HD void flush(unsigned long long*) {}
 };
 

@@ -30,6 +30,26 @@ namespace {
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63); }
 
+// This thread's pixel of the block's 16x16 tile (four 8x8 waves): the 2-D
+// grid's tile (blockIdx.x, blockIdx.y), or with a tile order the quarter
+// blockIdx.x % 4 of 32x32 tile tile_order[blockIdx.x / 4].
+__device__ __forceinline__ void block_pixel(const RenderParams& p, int& px, int& py) {
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    int x0, y0;
+    if (p.tile_order) {
+        const uint32_t t = p.tile_order[blockIdx.x >> 2];
+        const uint32_t tiles_x = static_cast<uint32_t>((p.width + 31) / 32);
+        x0 = static_cast<int>(t % tiles_x) * 32 + static_cast<int>(blockIdx.x & 1) * 16;
+        y0 = static_cast<int>(t / tiles_x) * 32 + static_cast<int>((blockIdx.x >> 1) & 1) * 16;
+    } else {
+        x0 = blockIdx.x * 16;
+        y0 = blockIdx.y * 16;
+    }
+    px = x0 + (wave & 1) * 8 + (lane & 7);
+    py = y0 + (wave >> 1) * 8 + (lane >> 3);
+}
+
 // Appends `value` for every lane with `want`, one atomic per wave.  Called
 // by all lanes of the wave.
 __device__ void wave_push(uint32_t* queue, unsigned* count, uint32_t value, bool want) {
@@ -61,9 +81,8 @@ __device__ unsigned wave_grab(unsigned* cursor, unsigned count) {
 template <bool Planes, bool Stats>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) plain_kernel(RenderParams p) {
     const int wave = threadIdx.x >> 6;
-    const int lane = lane_id();
-    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    int px, py;
+    block_pixel(p, px, py);
     const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
     const bool valid = px < p.width && py < p.height && (!p.point_re || index < static_cast<size_t>(p.point_count));
     uint8_t* dlog = nullptr;
@@ -78,7 +97,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
         core::PixelState s = core::init_pixel<Planes, Stats>(p, index, dc, st);
         if (core::plain_pixel<Planes, Stats>(p, index, s, dc, st, dlog)) p.state.iter[index] = p.max_iter;
         else core::save_state(p.state, index, s);
-        if constexpr (Stats) st.flush(p.stats);
+        if constexpr (Stats) {
+            st.flush_work(p.work, index);
+            st.flush(p.stats);
+        }
     }
 }
 
@@ -103,7 +125,10 @@ __global__ void __launch_bounds__(64) reference_pixel_kernel(RenderParams p) {
             core::save_state(p.state, index, s);
             handoff = true;
         }
-        if constexpr (Stats) st.flush(p.stats);
+        if constexpr (Stats) {
+            st.flush_work(p.work, index);
+            st.flush(p.stats);
+        }
     }
     wave_push(p.state.general_queue, p.state.counters + 0, static_cast<uint32_t>(index), handoff);
 }
@@ -114,9 +139,8 @@ __global__ void __launch_bounds__(64) reference_pixel_kernel(RenderParams p) {
 template <bool Planes, bool Stats>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Planes ? 6 : 7, 8))) fast_kernel(RenderParams p) {
     const int wave = threadIdx.x >> 6;
-    const int lane = lane_id();
-    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    int px, py;
+    block_pixel(p, px, py);
     const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
     bool valid = px < p.width && py < p.height && (!p.point_re || index < static_cast<size_t>(p.point_count));
     // pixels the plain pass finished, and the one reference_pixel_kernel renders
@@ -139,7 +163,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Planes
             core::save_state(p.state, index, s);
             handoff = true;
         }
-        if constexpr (Stats) st.flush(p.stats);
+        if constexpr (Stats) {
+            st.flush_work(p.work, index);
+            st.flush(p.stats);
+        }
     }
     wave_push(p.state.general_queue, p.state.counters + 0, static_cast<uint32_t>(index), handoff);
 }
@@ -165,7 +192,10 @@ __global__ void __launch_bounds__(256) general_kernel(RenderParams p) {
                 core::save_state(p.state, index, s);
                 to_tail = true;
             }
-            if constexpr (Stats) st.flush(p.stats);
+            if constexpr (Stats) {
+                st.flush_work(p.work, index);
+                st.flush(p.stats);
+            }
         }
         wave_push(p.state.tail_queue, p.state.counters + 2, index, to_tail);
     }
@@ -187,7 +217,10 @@ __global__ void __launch_bounds__(256) tail_kernel(RenderParams p) {
             const SC dc = core::pixel_dc(p, px, py);
             core::PixelState s = core::load_state(p.state, index);
             core::run_tail<Planes, Stats>(p, index, s, dc, st);
-            if constexpr (Stats) st.flush(p.stats);
+            if constexpr (Stats) {
+                st.flush_work(p.work, index);
+                st.flush(p.stats);
+            }
         }
     }
 }
@@ -217,8 +250,9 @@ hipError_t launch_all(const RenderParams& p, int persistent_blocks, hipStream_t 
         reference_pixel_kernel<Planes, Stats><<<1, 64, 0, side>>>(p);
         if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
     }
-    if (p.plain_pass) plain_kernel<Planes, Stats><<<tiles, dim3(256), 0, stream>>>(p);
-    fast_kernel<Planes, Stats><<<tiles, dim3(256), 0, stream>>>(p);
+    const dim3 grid = p.tile_order ? dim3(4 * static_cast<unsigned>(p.tile_count)) : tiles;
+    if (p.plain_pass) plain_kernel<Planes, Stats><<<grid, dim3(256), 0, stream>>>(p);
+    fast_kernel<Planes, Stats><<<grid, dim3(256), 0, stream>>>(p);
     if (side_pixel && (e = hipStreamWaitEvent(stream, join, 0)) != hipSuccess) return e;
     general_kernel<Planes, Stats><<<dim3(persistent_blocks), dim3(256), 0, stream>>>(p);
     tail_kernel<Planes, Stats><<<dim3(persistent_blocks), dim3(256), 0, stream>>>(p);

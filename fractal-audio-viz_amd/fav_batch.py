@@ -140,6 +140,46 @@ class Batch:
             ref.close()
 
 
+def expand_to_blocks(tile_values: np.ndarray, width: int, height: int) -> np.ndarray:
+    """Per-32x32-tile values (row-major) repeated onto the field's row-major
+    16x16 batch tiles: each batch tile takes the value of the 32x32 tile
+    that contains it."""
+    tx = (int(width) + 31) // 32
+    bx, by = (int(width) + 15) // 16, (int(height) + 15) // 16
+    ys, xs = np.divmod(np.arange(bx * by), bx)
+    return np.asarray(tile_values)[(ys // 2) * tx + xs // 2]
+
+
+def learned_tile_costs(batch: Batch, model) -> tuple[np.ndarray, dict]:
+    """The cost model's prediction for every batch tile, in field-major tile
+    order (the input of Batch.set_order(2, costs)): each field's preview
+    features (fav_sched.field_features), the model on the GPU, and the
+    32x32 predictions spread over the field's 16x16 batch tiles.  Returns
+    the costs and the preview / model times."""
+    import fav_sched
+    costs, preview_ms, model_s = [], 0.0, 0.0
+    for i, f in enumerate(batch.fields):
+        case = fav_sched.segment_case(f, f"field{i}")
+        features, ms = fav_sched.field_features(batch.references[i], case)
+        preview_ms += ms
+        started = time.perf_counter()
+        predicted = model.predict(features)
+        model_s += time.perf_counter() - started
+        costs.append(expand_to_blocks(predicted, f["width"], f["height"]))
+    return np.concatenate(costs).astype(np.float32), {"preview_ms": preview_ms, "model_wall_ms": model_s * 1e3}
+
+
+def measured_block_costs(batch: Batch) -> np.ndarray:
+    """Every batch tile's measured cost (fav_sched.block_costs of the
+    field's per-pixel loop passes), in field-major tile order."""
+    import fav_sched
+    costs = []
+    for i, f in enumerate(batch.fields):
+        work = fav_sched.field_work(batch.references[i], fav_sched.segment_case(f, f"field{i}"))
+        costs.append(fav_sched.block_costs(work, f["width"], f["height"]))
+    return np.concatenate(costs)
+
+
 def autotune(batch: Batch, costs=None, repeats: int = 2):
     """Times every (tile order, fast-kernel blocks per CU) candidate on the
     batch (graph launches, best of `repeats`) and keeps the fastest.

@@ -322,9 +322,9 @@ class NativeReference:
     def render_scheduled(self, width: int, height: int, zoom: str, max_iter: int, tile_order=None,
                          work: bool = False, series_order: int = 3, series_block: int = 256,
                          options: NativeRenderOptions | None = None):
-        """Alternate-formula field (H, W) with its 32x32 tiles launched in
-        `tile_order` (None = the plain grid); with work=True also the (H, W)
-        int32 map of per-pixel loop passes."""
+        """Field (H, W) of any formula (zoom >= 1e6, no planes) with its
+        32x32 tiles launched in `tile_order` (None = the plain grid); with
+        work=True also the (H, W) int32 map of per-pixel loop passes."""
         opts = options or self.options
         out = np.empty((int(height), int(width)), dtype=np.float32)
         work_map = np.empty((int(height), int(width)), dtype=np.int32) if work else None
@@ -341,8 +341,10 @@ class NativeReference:
     def preview_features(self, width: int, height: int, zoom: str, max_iter: int, work_limit: int,
                          series_order: int = 3, series_block: int = 256,
                          options: NativeRenderOptions | None = None) -> np.ndarray:
-        """(tiles, 65) float32 cost-model inputs of an alternate-formula field
-        from its quarter-size preview with `work_limit` passes per pixel."""
+        """(tiles, 65) float32 cost-model inputs of a field from its
+        quarter-size preview: at most `work_limit` loop passes per pixel for
+        the alternate formulas, cap min(max_iter, work_limit) for the
+        Mandelbrot map."""
         opts = options or self.options
         tiles = ((int(width) + 31) // 32) * ((int(height) + 31) // 32)
         out = np.empty((tiles, 65), dtype=np.float32)
@@ -356,7 +358,7 @@ class NativeReference:
     def render_learned(self, width: int, height: int, zoom: str, max_iter: int, weights: np.ndarray,
                        work_limit: int, series_order: int = 3, series_block: int = 256,
                        options: NativeRenderOptions | None = None):
-        """Alternate-formula field (H, W) rendered in the cost model's
+        """Field (H, W) of any formula rendered in the cost model's
         heaviest-first tile order; returns (field, order, timings dict)."""
         opts = options or self.options
         w = np.ascontiguousarray(weights, dtype=np.float32)

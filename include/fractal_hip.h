@@ -507,21 +507,26 @@ int fractal_hip_render_reference_floatexp(
     int series_block, const FractalRenderOptions *options,
     int device_pointers);
 
-/* An alternate-formula field (Julia, burning ship, tricorn; zoom >= 1e6)
- * with its 32x32 tiles launched in tile_order (a permutation of the
- * ceil(w/32)*ceil(h/32) row-major tiles; NULL = the plain grid), and, when
- * work is not NULL, each pixel's loop passes written to it.  Host pointers;
- * the field is identical for every order.  No counterpart in the reference:
- * the schedule hook of the learned tile-cost model. */
+/* A perturbation field of any formula (Mandelbrot, Julia, burning ship,
+ * tricorn; zoom >= 1e6; no planes) with its 32x32 tiles launched in
+ * tile_order (a permutation of the ceil(w/32)*ceil(h/32) row-major tiles;
+ * NULL = the plain grid), and, when work is not NULL, each pixel's loop
+ * passes (BLA blocks, exact, replayed and tail steps) written to it.  Host
+ * pointers; the field is identical for every order.  RenderStats are
+ * published for Mandelbrot fields when statistics are enabled.  No
+ * counterpart in the reference: the schedule hook of the learned tile-cost
+ * model. */
 int fractal_hip_render_reference_scheduled(
     float *output, int width, int height, const char *zoom_text, void *handle,
     int max_iter, int series_order, int series_block,
     const FractalRenderOptions *options, const uint32_t *tile_order,
     int tile_count, int32_t *work);
 
-/* Tile features of an alternate-formula field for the cost model below:
- * renders the quarter-size preview ((w+3)/4 x (h+3)/4, same view) with at
- * most work_limit loop passes per pixel and writes, per 32x32 tile of the
+/* Tile features of a field for the cost model below: renders the
+ * quarter-size preview ((w+3)/4 x (h+3)/4, same view) with at most
+ * work_limit loop passes per pixel (alternate formulas) or with cap
+ * min(max_iter, work_limit) (Mandelbrot map; every pass advances at least
+ * one iteration, so passes stay within the limit too) and writes, per 32x32 tile of the
  * w x h field (row-major), its 8x8 patch of log(1 + preview passes) / 8
  * (edge pixels repeated past the preview's border) and log10(zoom) / 100:
  * ceil(w/32) * ceil(h/32) * 65 floats to the host buffer `features`.

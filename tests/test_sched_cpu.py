@@ -156,13 +156,50 @@ def test_cost_model_abi_rejects_bad_arguments_without_device(product):
     assert b"learning rate" in lib.fractal_last_error()
 
 
-def test_scheduled_render_rejects_mandelbrot_and_bad_orders_on_host(product):
-    """The order is validated before any launch; the Mandelbrot formula
-    takes its order through a batch."""
+def test_scheduled_render_rejects_bad_orders_on_host(product):
+    """A Mandelbrot handle takes a tile order too; orders that are not
+    permutations of the 32x32 tiles, and views below zoom 1e6, are refused
+    on the host before any launch."""
     import fav_hip
     lib = fav_hip.load_library()
     opts = fav_hip.NativeRenderOptions()
     ref = fav_hip.NativeReference("-0.75", "0.1", "1e7", 2000, 128, options=opts, library=lib)
-    with pytest.raises(RuntimeError, match="alternate formulas"):
-        ref.render_scheduled(64, 64, "1e7", 2000, tile_order=np.arange(4))
+    for bad in (np.arange(3), np.zeros(4), np.arange(1, 5)):
+        with pytest.raises(RuntimeError, match="every 32x32 tile once"):
+            ref.render_scheduled(64, 64, "1e7", 2000, tile_order=bad)
+    with pytest.raises(RuntimeError, match="zoom >= 1e6"):
+        ref.render_scheduled(64, 64, "1e5", 2000, tile_order=np.arange(4))
+    with pytest.raises(RuntimeError, match="zoom >= 1e6"):
+        ref.preview_features(64, 64, "1e5", 2000, 4096)
     ref.close()
+
+
+def test_block_order_and_makespan_model():
+    """The launch order of a 32x32 tile order over 16x16 blocks, the block
+    costs, and the list-schedule makespan on small closed cases."""
+    import fav_sched
+    # 70x40: 3x2 tiles of 32, 5x3 blocks of 16 (the last column and row partial)
+    order = fav_sched.blocks_of_tiles(np.array([5, 0, 1, 2, 3, 4]), 70, 40)
+    # tile 5 = (2, 1): blocks (4, 2) only; tile 0 = (0, 0): blocks 0, 1, 5, 6
+    assert order.tolist()[:5] == [14, 0, 1, 5, 6]
+    assert sorted(order.tolist()) == list(range(15))
+    work = np.zeros((40, 70), dtype=np.int32)
+    work[3, 2] = 7        # block 0, wave (0, 0)
+    work[20, 40] = 11     # block 1 * 5 + 2 = 7
+    work[39, 69] = 5      # block 14
+    costs = fav_sched.block_costs(work, 70, 40)
+    assert costs.shape == (15,) and costs[0] == 7 and costs[7] == 11 and costs[14] == 5 and costs.sum() == 23
+    # two machines: row order 3,3,3,4,5 -> {3,3,5} {3,4} = 11; heaviest first
+    # 5,4,3,3,3 -> {5,3} {4,3,3} = 10
+    jobs = np.array([3.0, 3.0, 3.0, 4.0, 5.0])
+    assert fav_sched.makespan(jobs, np.arange(5), 2) == 11.0
+    assert fav_sched.makespan(jobs, np.array([4, 3, 0, 1, 2]), 2) == 10.0
+    assert fav_sched.makespan(jobs, np.arange(5), 5) == 5.0
+    assert fav_sched.makespan(jobs, np.arange(5), 1) == 18.0
+
+
+def test_batch_costs_spread_over_batch_tiles():
+    import fav_batch
+    # 70x40: 3x2 tiles of 32, 5x3 batch tiles of 16
+    spread = fav_batch.expand_to_blocks(np.array([0.0, 1.0, 2.0, 3.0, 4.0, 5.0]), 70, 40)
+    assert spread.tolist() == [0, 0, 1, 1, 2, 0, 0, 1, 1, 2, 3, 3, 4, 4, 5]
