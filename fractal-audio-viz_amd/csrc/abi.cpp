@@ -1074,6 +1074,16 @@ std::shared_ptr<ReferenceContext> scheduled_context(void* handle, const char* zo
     return context;
 }
 
+// Alternate-formula perturbation renders need zoom >= 1e6 (shallower views
+// of a handle render through the direct renderer).
+void require_perturbation_zoom(const ReferenceContext& context, const char* zoom) {
+    const fav::FE z = fav::parse_zoom(zoom, context.precision_bits);
+    const long double log10_zoom =
+        (std::log(static_cast<long double>(z.m)) + static_cast<long double>(z.e) * 0.693147180559945309417232121458176568L)
+        / std::log(10.0L);
+    if (log10_zoom < 6.0L) throw std::runtime_error("this render needs a perturbation field (zoom >= 1e6)");
+}
+
 int guarded(const char* what, const std::function<void()>& body) {
     try {
         body();
@@ -1115,6 +1125,41 @@ int fractal_hip_render_reference_scheduled(float* output, int width, int height,
             fav::render_field(*context, request);
         }
         if (collect) publish_stats(stats);
+    });
+}
+
+int fractal_hip_render_reference_counts(float* output, int32_t* counts, int width, int height, const char* zoom,
+                                        void* handle, int max_iter, int series_order, int series_block,
+                                        const FractalRenderOptions* options) {
+    return guarded("native counts render", [&] {
+        if (!output || !counts) throw std::runtime_error("output and counts must not be null");
+        const FractalRenderOptions o = checked_options(options);
+        const auto context = scheduled_context(handle, zoom, width, height, max_iter, series_order, series_block, o);
+        fav::FieldRequest request{output, width, height, zoom, max_iter, series_order, series_block, o,
+                                  nullptr, nullptr, false};
+        if (context->formula != 0) {
+            fav::render_alternate(*context, request, nullptr, 0, nullptr, counts);
+            return;
+        }
+        // the Mandelbrot map: from the iteration plane (the escape iteration is
+        // orbit_iteration + 1, the cap inside), -1 where the field is NaN
+        const size_t n = static_cast<size_t>(width) * static_cast<size_t>(height);
+        std::vector<int64_t> iteration(n);
+        std::vector<double> scratch(5 * n);
+        FractalRenderPlanes planes{};
+        planes.struct_size = sizeof(FractalRenderPlanes);
+        planes.version = FRACTAL_RENDER_PLANES_VERSION;
+        planes.orbit_iteration = iteration.data();
+        planes.phase = scratch.data();
+        planes.de_x = scratch.data() + n;
+        planes.de_y = scratch.data() + 2 * n;
+        planes.test1 = scratch.data() + 3 * n;
+        planes.test2 = scratch.data() + 4 * n;
+        request.planes = &planes;
+        fav::render_field(*context, request);
+        for (size_t i = 0; i < n; ++i)
+            counts[i] = output[i] != output[i] ? -1
+                : static_cast<int32_t>(iteration[i] < max_iter ? iteration[i] + 1 : max_iter);
     });
 }
 
@@ -1301,12 +1346,12 @@ int fractal_hip_render_reference_floatexp(float* output, int32_t* counts, int wi
         require_hip_backend(o);
         const auto context = acquire(handle);
         if (!context) throw std::runtime_error("invalid or already-destroyed reference handle");
-        if (context->formula != 0) throw std::runtime_error("the floatexp path renders the Mandelbrot formula only");
         if (context->escape_radius_mode != o.escape_radius_mode)
             throw std::runtime_error("reference escape-radius mode does not match render options");
         if (max_iter > context->requested_max_iter)
             throw std::runtime_error("render iteration count exceeds prepared reference");
         if (!valid_text(zoom)) throw std::runtime_error("native zoom text is too long or null");
+        if (context->formula != 0) require_perturbation_zoom(*context, zoom);
         fav::FieldRequest request{output, width, height, zoom, max_iter, series_order, series_block, o,
                                   nullptr, nullptr, device_pointers != 0};
         fav::render_field_floatexp(*context, request, counts);

@@ -138,44 +138,34 @@ FAV_HD SC pixel_dc(const AltRenderParams& p, int px, int py) {  // renderer.cpp:
     return sc_from_fe(FE{p.x_off_m[px], p.x_off_e[px]}, FE{p.y_off_m[py], p.y_off_e[py]});
 }
 
-// One pixel of render_alternate_reference_impl; writes p.output[index] and
-// counts the loop passes (blocks, exact and replayed steps) in `work`.
-FAV_HD void pixel_body(const AltRenderParams& p, int px, int py, int& work) {
-    const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + static_cast<size_t>(px);
+// The field value and escape count of one pixel (max_iter inside, -1 for a
+// NaN pixel).
+struct Result {
+    float value;
+    int count;
+};
+
+// The loop of render_alternate_reference_impl from a loop top: the
+// perturbation `delta` against reference index `ref` at `iteration`, with
+// Z + delta and its norm; dc, the parameter perturbation and its squared
+// norm.  Counts the loop passes (blocks, exact and replayed steps) in `work`.
+FAV_HD Result run_loop(const AltRenderParams& p, SC dc, SC pdelta, FE pn, SC delta, int ref, int iteration, SC total,
+                       SN total_norm, int& work) {
     const bool julia = p.formula == 1;
     const int mode = p.escape_radius_mode;
     const double bias = p.output_bias;
-    const SC dc = pixel_dc(p, px, py);
-    const SC pdelta = julia ? sc_zero() : dc;
-    const SN pnorm = sc_norm_squared(pdelta);
-    const FE pn{pnorm.m, pnorm.e};
-    SC delta = dc;
-    int ref = julia ? 0 : 1, iteration = julia ? 0 : 1;
-    if (p.series_available && sc_compare_norm(sc_norm_squared(dc), SN{p.series.radius2_m, p.series.radius2_e}) <= 0) {
-        delta = series_value(p.series, dc);
-        ref = p.series.iteration;
-        iteration = p.series.iteration;
-    }
     AltBudget budget{&p, 0U};
-    SC total = sc_add(load_z(p, ref), delta);
-    SN total_norm = sn_from_margin(margin(p, ref, delta), mode);
-    if (sc_outside_escape(total_norm, mode)) {
-        p.output[index] = core::smooth_escape(iteration, total_norm, bias);
-        return;
-    }
     const SN limit{0.75, 2};
     SC tortoise = sc_zero();
     int cycle_power = 1, cycle_length = 0, cycle_hits = 0;
     bool cycle_ready = false;
     while (iteration < p.max_iter) {
         if (p.work_limit > 0 && work >= p.work_limit) {
-            p.output[index] = __builtin_nanf("");
-            return;
+            return Result{__builtin_nanf(""), -1};
         }
         ++work;
         if (budget.expired()) {
-            p.output[index] = __builtin_nanf("");
-            return;
+            return Result{__builtin_nanf(""), -1};
         }
         const int next = ref + 1;
         if (next >= p.orbit_size) {  // past the reference's escape: exact recurrence
@@ -184,8 +174,7 @@ FAV_HD void pixel_body(const AltRenderParams& p, int px, int py, int& work) {
             ++iteration;
             total_norm = sc_norm_squared(total);
             if (sc_outside_escape(total_norm, mode)) {
-                p.output[index] = core::smooth_escape(iteration, total_norm, bias);
-                return;
+                return Result{core::smooth_escape(iteration, total_norm, bias), iteration};
             }
             continue;
         }
@@ -213,23 +202,20 @@ FAV_HD void pixel_body(const AltRenderParams& p, int px, int py, int& work) {
             // replay the block one exact step at a time
             for (int replay = 0; replay < step.length && iteration < p.max_iter; ++replay) {
                 if (p.work_limit > 0 && work >= p.work_limit) {
-                    p.output[index] = __builtin_nanf("");
-                    return;
+                    return Result{__builtin_nanf(""), -1};
                 }
                 ++work;
                 delta = delta_step(p, ref, delta, pdelta);
                 ++ref;
                 ++iteration;
                 if (!sc_finite(delta)) {
-                    p.output[index] = p.strict ? __builtin_nanf("") : core::encode_iteration(iteration, bias);
-                    return;
+                    return Result{p.strict ? __builtin_nanf("") : core::encode_iteration(iteration, bias), p.strict ? -1 : iteration};
                 }
                 const FE m = margin(p, ref, delta);
                 total_norm = sn_from_margin(m, mode);
                 total = sc_add(load_z(p, ref), delta);
                 if (fe_compare(m, fe_zero()) > 0 || sc_outside_escape(total_norm, mode)) {
-                    p.output[index] = core::smooth_escape(iteration, total_norm, bias);
-                    return;
+                    return Result{core::smooth_escape(iteration, total_norm, bias), iteration};
                 }
             }
             continue;
@@ -238,15 +224,13 @@ FAV_HD void pixel_body(const AltRenderParams& p, int px, int py, int& work) {
         ref = next;
         ++iteration;
         if (!sc_finite(delta)) {
-            p.output[index] = p.strict ? __builtin_nanf("") : core::encode_iteration(iteration, bias);
-            return;
+            return Result{p.strict ? __builtin_nanf("") : core::encode_iteration(iteration, bias), p.strict ? -1 : iteration};
         }
         const FE m = margin(p, ref, delta);
         total_norm = sn_from_margin(m, mode);
         total = sc_add(load_z(p, ref), delta);
         if (fe_compare(m, fe_zero()) > 0 || sc_outside_escape(total_norm, mode)) {
-            p.output[index] = core::smooth_escape(iteration, total_norm, bias);
-            return;
+            return Result{core::smooth_escape(iteration, total_norm, bias), iteration};
         }
         if (p.cycle_enabled && iteration >= 2048 && (iteration & 31) == 0 && sc_compare_norm(total_norm, limit) < 0) {
             if (!cycle_ready) {
@@ -260,8 +244,7 @@ FAV_HD void pixel_body(const AltRenderParams& p, int px, int py, int& work) {
                 const int scale = total_norm.e > 0 ? total_norm.e : 0;
                 if (distance.m == 0.0 || distance.e <= scale - 78) {
                     if (++cycle_hits >= 3) {
-                        p.output[index] = core::encode_iteration(p.max_iter, bias);
-                        return;
+                        return Result{core::encode_iteration(p.max_iter, bias), p.max_iter};
                     }
                 } else {
                     cycle_hits = 0;
@@ -274,13 +257,49 @@ FAV_HD void pixel_body(const AltRenderParams& p, int px, int py, int& work) {
             }
         }
     }
-    p.output[index] = core::encode_iteration(p.max_iter, bias);
+    return Result{core::encode_iteration(p.max_iter, bias), p.max_iter};
+}
+
+
+// One pixel of render_alternate_reference_impl; writes p.output[index] and
+// its escape count, and counts the loop passes in `work`.
+FAV_HD void pixel_body(const AltRenderParams& p, int px, int py, int& work, int& count) {
+    const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + static_cast<size_t>(px);
+    const bool julia = p.formula == 1;
+    const int mode = p.escape_radius_mode;
+    const double bias = p.output_bias;
+    // the field value and the escape count (max_iter inside, -1 when NaN)
+    auto finish = [&](float value, int n) {
+        p.output[index] = value;
+        count = n;
+    };
+    const SC dc = pixel_dc(p, px, py);
+    const SC pdelta = julia ? sc_zero() : dc;
+    const SN pnorm = sc_norm_squared(pdelta);
+    const FE pn{pnorm.m, pnorm.e};
+    SC delta = dc;
+    int ref = julia ? 0 : 1, iteration = julia ? 0 : 1;
+    if (p.series_available && sc_compare_norm(sc_norm_squared(dc), SN{p.series.radius2_m, p.series.radius2_e}) <= 0) {
+        delta = series_value(p.series, dc);
+        ref = p.series.iteration;
+        iteration = p.series.iteration;
+    }
+    SC total = sc_add(load_z(p, ref), delta);
+    SN total_norm = sn_from_margin(margin(p, ref, delta), mode);
+    if (sc_outside_escape(total_norm, mode)) {
+        finish(core::smooth_escape(iteration, total_norm, bias), iteration);
+        return;
+    }
+    const Result r = run_loop(p, dc, pdelta, pn, delta, ref, iteration, total, total_norm, work);
+    finish(r.value, r.count);
 }
 
 FAV_HD void pixel(const AltRenderParams& p, int px, int py) {
-    int work = 0;
-    pixel_body(p, px, py, work);
-    if (p.work) p.work[static_cast<size_t>(py) * static_cast<size_t>(p.width) + static_cast<size_t>(px)] = work;
+    int work = 0, count = 0;
+    pixel_body(p, px, py, work, count);
+    const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + static_cast<size_t>(px);
+    if (p.work) p.work[index] = work;
+    if (p.counts) p.counts[index] = count;
 }
 
 }  // namespace alt

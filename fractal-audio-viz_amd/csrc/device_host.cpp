@@ -37,6 +37,7 @@ hipError_t kfp_noalloc_scratch(int device, unsigned char** out, size_t* lut_offs
 hipError_t launch_direct_kernel(const direct::DirectParams& p, bool planes, hipStream_t stream);
 hipError_t launch_alt_kernel(const AltRenderParams& p, hipStream_t stream);
 hipError_t launch_fx_kernel(const RenderParams& p, const fx::FxOut& out, hipStream_t stream);
+hipError_t launch_fx_alt_kernel(const AltRenderParams& p, const fx::FxOut& out, hipStream_t stream);
 hipError_t launch_batch(const BatchLaunch& q, int fast_blocks, int persistent_blocks, hipStream_t stream);
 hipError_t launch_atlas_compose(const atlas::ComposeParams& a, hipStream_t stream);
 hipError_t launch_aurora(const aurora::Params& p, hipStream_t stream);
@@ -899,7 +900,11 @@ void atlas_colourise_kfp_raw(const atlas::ComposeParams& host, size_t parent_cou
     s.timing.pixels = static_cast<double>(pixels);
 }
 
-void render_field_floatexp(ReferenceContext& c, const FieldRequest& r, int32_t* counts) {
+void render_field_floatexp(ReferenceContext& c, const FieldRequest& r, int32_t* counts, bool double_tail) {
+    if (c.formula != 0) {
+        render_alternate_floatexp(c, r, counts);
+        return;
+    }
     const int w = r.width, h = r.height;
     PreparedRender prep;
     prepare_render(c, r, prep);
@@ -916,6 +921,7 @@ void render_field_floatexp(ReferenceContext& c, const FieldRequest& r, int32_t* 
     p.series.radius2_e = scalar_series.radius2_e;
     p.linear = t.linear[prep.linear_tier];
     p.linear_dir = t.linear_dir[prep.linear_tier];
+    p.budget_ticks = 0;  // no time budget on this path
 
     LaunchState& s = state();
     std::lock_guard<std::mutex> lock(s.mutex);
@@ -935,6 +941,7 @@ void render_field_floatexp(ReferenceContext& c, const FieldRequest& r, int32_t* 
     fx::FxOut out{};
     out.field = r.device_pointers ? r.output : static_cast<float*>(s.output.get(pixels * sizeof(float)));
     out.counts = r.device_pointers ? counts : static_cast<int32_t*>(s.fx_counts.get(pixels * sizeof(int32_t)));
+    out.double_tail = double_tail ? 1 : 0;
     const uint64_t wall0 = now_ns();
     check(hipEventRecord(s.start, s.stream), "event");
     check(launch_fx_kernel(p, out, s.stream), "floatexp kernel launch");
@@ -1116,7 +1123,7 @@ FieldRequest field_preview_request(const FieldRequest& r, int work_limit) {
 }  // namespace
 
 void render_alternate(ReferenceContext& c, const FieldRequest& r, const uint32_t* tile_order, size_t tile_count,
-                      int32_t* work) {
+                      int32_t* work, int32_t* counts) {
     PreparedAlternate prep;
     prepare_alternate(c, r, prep);
     if (tile_order) check_tile_order(tile_order, tile_count, r.width, r.height);
@@ -1135,6 +1142,7 @@ void render_alternate(ReferenceContext& c, const FieldRequest& r, const uint32_t
         p.tile_count = static_cast<int32_t>(tile_count);
     }
     if (work) p.work = static_cast<int32_t*>(s.work.get(pixels * 4));
+    if (counts) p.counts = static_cast<int32_t*>(s.fx_counts.get(pixels * 4));
     PointStaging points;
     size_t out_count = pixels;
     if (r.points) {
@@ -1150,6 +1158,7 @@ void render_alternate(ReferenceContext& c, const FieldRequest& r, const uint32_t
         check(hipMemcpyAsync(r.output, p.output, out_count * sizeof(float), hipMemcpyDeviceToHost, s.stream),
               "D2H");
     if (work) check(hipMemcpyAsync(work, p.work, pixels * 4, hipMemcpyDeviceToHost, s.stream), "D2H work");
+    if (counts) check(hipMemcpyAsync(counts, p.counts, pixels * 4, hipMemcpyDeviceToHost, s.stream), "D2H counts");
     check(hipStreamSynchronize(s.stream), "alternate render");
     const float kernel_ms = elapsed(s.start, s.stop);
     s.timing.kernel_ms = kernel_ms;
@@ -1160,6 +1169,33 @@ void render_alternate(ReferenceContext& c, const FieldRequest& r, const uint32_t
         r.stats[0] = static_cast<uint64_t>(out_count);
         r.stats[22] = static_cast<uint64_t>(static_cast<double>(kernel_ms) * 1e6);
     }
+}
+
+void render_alternate_floatexp(ReferenceContext& c, const FieldRequest& r, int32_t* counts) {
+    PreparedAlternate prep;
+    prepare_alternate(c, r, prep);
+    ensure_device_tables(c);
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    init_locked(s);
+    const size_t pixels = static_cast<size_t>(r.width) * static_cast<size_t>(r.height);
+    AltRenderParams p = bind_alternate(c, prep, s, s.offsets);
+    p.budget_ticks = 0;  // no time budget on this path
+    fx::FxOut out{};
+    out.field = r.device_pointers ? r.output : static_cast<float*>(s.output.get(pixels * sizeof(float)));
+    out.counts = r.device_pointers ? counts : static_cast<int32_t*>(s.fx_counts.get(pixels * sizeof(int32_t)));
+    const uint64_t wall0 = now_ns();
+    check(hipEventRecord(s.start, s.stream), "event");
+    check(launch_fx_alt_kernel(p, out, s.stream), "floatexp kernel launch");
+    check(hipEventRecord(s.stop, s.stream), "event");
+    if (!r.device_pointers) {
+        check(hipMemcpyAsync(r.output, out.field, pixels * sizeof(float), hipMemcpyDeviceToHost, s.stream), "D2H");
+        check(hipMemcpyAsync(counts, out.counts, pixels * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream), "D2H");
+    }
+    check(hipStreamSynchronize(s.stream), "floatexp render");
+    s.timing.kernel_ms = elapsed(s.start, s.stop);
+    s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
+    s.timing.pixels = static_cast<double>(pixels);
 }
 
 void alternate_preview_features(ReferenceContext& c, const FieldRequest& r, int work_limit, float* features) {

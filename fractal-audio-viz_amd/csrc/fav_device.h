@@ -202,6 +202,9 @@ struct AltRenderParams {
     const uint32_t* tile_order;
     int32_t tile_count;
     int32_t* work;
+    // when not null: each pixel's escape iteration (max_iter inside, -1 for
+    // a NaN pixel)
+    int32_t* counts;
     // preview budget: when > 0 a pixel stops after this many loop passes
     // (output NaN, work = the budget)
     int32_t work_limit;

@@ -49,10 +49,14 @@ struct LastTiming {
 
 void render_field(ReferenceContext& context, const FieldRequest& request);
 
-// FP32 floatexp field of a Mandelbrot handle: request.output (field) and
-// `counts` (escape iteration per pixel, max_iter inside, -1 unresolved) are
-// host or device pointers as request.device_pointers says; no planes.
-void render_field_floatexp(ReferenceContext& context, const FieldRequest& request, int32_t* counts);
+// FP32 floatexp field of a handle of any formula: request.output (field)
+// and `counts` (escape iteration per pixel, max_iter inside, -1 unresolved)
+// are host or device pointers as request.device_pointers says; no planes.
+// Mandelbrot map: with double_tail the pixels finish in the FP64
+// plain-double tail where the FP64 path enters it.
+void render_field_floatexp(ReferenceContext& context, const FieldRequest& request, int32_t* counts,
+                           bool double_tail = true);
+void render_alternate_floatexp(ReferenceContext& context, const FieldRequest& request, int32_t* counts);
 
 // A prepared batch of Mandelbrot fields rendered by one launch sequence
 // (batch_kernel.hip).  Owns the device copies of every field's launch
@@ -104,8 +108,10 @@ void check_tile_order(const uint32_t* tile_order, size_t tile_count, int width, 
 
 // Alternate-formula field (Julia, burning ship, tricorn) of a reference
 // handle; request.planes must be null.
+// counts (host, when set): each pixel's escape iteration, max_iter inside,
+// -1 for a NaN pixel.
 void render_alternate(ReferenceContext& context, const FieldRequest& request, const uint32_t* tile_order = nullptr,
-                      size_t tile_count = 0, int32_t* work = nullptr);
+                      size_t tile_count = 0, int32_t* work = nullptr, int32_t* counts = nullptr);
 // Learned schedule of an alternate-formula field: the quarter-size preview
 // with `work_limit` passes per pixel, its tile features (host `features`,
 // tiles * 65 floats) ...

@@ -7,11 +7,17 @@
 // small fraction of boundary pixels may escape at a different iteration; the
 // caller measures that fraction against the FP64 kernel.
 //
-// Mandelbrot only.  Linear BLA blocks at every delta size (the FP64 path
+// Mandelbrot map: linear BLA blocks at every delta size (the FP64 path
 // switches to the cubic table for large deltas), rebasing to the start of
-// the orbit when |Z+delta| < |delta| or the reference escapes.
+// the orbit when |Z+delta| < |delta| or the reference escapes, and the FP64
+// plain-double tail where the FP64 path enters it.  Julia, burning ship and
+// tricorn (alt_pixel): the exact perturbation step of each map (the burning
+// ship's sign branches with the axis-crossing reconstruction) and the real
+// 2x2 linear BLA blocks of the FP64 alternate tables, in the same mantissa
+// arithmetic.
 #pragma once
 
+#include "alt_core.h"
 #include "field_core.h"
 
 namespace fav {
@@ -150,7 +156,31 @@ FAV_HD void run_blocks(const RenderParams& p, FX& delta, FX dc, int& ref, int& i
 struct FxOut {
     float* field;
     int32_t* counts;  // escape iteration, max_iter for interior, -1 for an unresolved (NaN) pixel
+    // 1: where the FP64 path enters its plain-double tail (no BLA block and
+    // |d|^2 > 2^-90), the pixel continues in that tail in FP64
+    int32_t double_tail;
 };
+
+// The FP64 plain-double tail of field_core.h from the floatexp state at a
+// loop top (delta converted exactly to a ScaledComplex, `norm` = |Z+d|^2).
+// Returns false when the tail gave up (the FP64 path's restart with BLA
+// off); otherwise the pixel is finished: field and count written.
+FAV_HD bool finish_in_double_tail(const RenderParams& p, const FxOut& out, size_t index, SC dc, FX delta, int& ref,
+                                  int& iteration, SN norm) {
+    core::Counters<false> st;
+    core::Budget budget{&p, 0U};
+    bool deadline = false, unresolved = false;
+    SC derivative{1.0, 0.0, 0};
+    float value = 0.0F;
+    const SC d{static_cast<double>(delta.re), static_cast<double>(delta.im), delta.e};
+    const bool gave_up = core::double_tail<false, false>(p, core::plane_out(p), index, dc, iteration, ref, d,
+                                                         derivative, budget, deadline, unresolved, st, norm, value);
+    if (gave_up && !deadline && !unresolved) return false;
+    const float interior = core::encode_iteration(p.max_iter, p.output_bias);
+    out.field[index] = value;
+    out.counts[index] = value != value ? -1 : (value == interior || iteration >= p.max_iter ? p.max_iter : iteration);
+    return true;
+}
 
 // One pixel of the floatexp path.  `p` carries the FP64 path's tables and
 // launch parameters (orbit, linear BLA tier, series, offsets).
@@ -181,14 +211,28 @@ FAV_HD void pixel(const RenderParams& p, const FxOut& out, int px, int py) {
     // last usable reference index: the reference's escape (map_end) or the
     // end of the stored orbit
     const int last = o.map_end > 0 && o.map_end < o.size ? o.map_end : o.size - 1;
+    bool bla_off = p.disable_bla != 0;  // also after a double tail gave up
     while (iteration < max_iter) {
-        if (!p.disable_bla) {
+        if (!bla_off) {
             run_blocks(p, delta, dc, ref, iteration, last);
             if (iteration >= max_iter) break;
         }
-        if (ref >= last) {  // continue against the orbit's start, Z_0 = 0
-            delta = add(from_sc(core::load_z(o, ref)), delta);
-            ref = 0;
+        if (ref >= last) {
+            // the reference escaped: the direct recurrence on the full value in
+            // FP64, as the FP64 path continues (renderer.cpp:13155-13250)
+            const SC full = sc_add(core::load_z(o, ref), SC{delta.re, delta.im, delta.e});
+            core::Counters<false> st;
+            core::Budget budget{&p, 0U};
+            bool deadline = false, unresolved = false;
+            SC derivative{1.0, 0.0, 0};
+            float value = 0.0F;
+            core::exact_tail<false, false>(p, core::plane_out(p), index, dc_sc, iteration, full,
+                                           sc_norm_squared(full), derivative, budget, deadline, unresolved, st,
+                                           value);
+            const float interior = core::encode_iteration(max_iter, p.output_bias);
+            out.field[index] = value;
+            out.counts[index] = value != value ? -1 : (value == interior || iteration >= max_iter ? max_iter : iteration);
+            return;
         }
         const FX z = from_sc(core::load_z(o, ref));
         const FX total = add(z, delta);
@@ -216,16 +260,27 @@ FAV_HD void pixel(const RenderParams& p, const FxOut& out, int px, int py) {
             continue;
         }
         const int remaining = max_iter - iteration;
-        const int k = p.disable_bla ? -1
+        const int k = bla_off ? -1
             : core::lookup_level(p.linear, p.linear_dir, ref, FE{dn.m, dn.e},
                                  p.max_linear_length < remaining ? p.max_linear_length : remaining);
-        if (k >= 0 && ref + p.linear[k].length <= max_iter && ref + p.linear[k].length <= last) {
+        const bool block = k >= 0 && ref + p.linear[k].length <= max_iter && ref + p.linear[k].length <= last;
+        if (block) {
             const LinearStepDev& step = p.linear[k];
             const FX a = from_sc(SC{step.a_re, step.a_im, step.a_e});
             const FX b = from_sc(SC{step.b_re, step.b_im, step.b_e});
             delta = add(mul(a, delta), mul(b, dc));
             ref += step.length;
             iteration += step.length;
+            continue;
+        }
+        // no block: where the FP64 path starts its plain-double tail
+        // (renderer.cpp:13283-13290), the pixel finishes in that tail in FP64
+        if (out.double_tail && !bla_off && dn.e > -90) {
+            if (finish_in_double_tail(p, out, index, dc_sc, delta, ref, iteration, norm)) return;
+            delta = dc;  // the tail gave up: restart without BLA, as the FP64 path does
+            ref = 1;
+            iteration = 1;
+            bla_off = true;
             continue;
         }
         // exact step d' = 2 Z d + d^2 + dc
@@ -235,6 +290,184 @@ FAV_HD void pixel(const RenderParams& p, const FxOut& out, int px, int py) {
     }
     out.field[index] = core::encode_iteration(max_iter, p.output_bias);
     out.counts[index] = max_iter;
+}
+
+// ---- alternate formulas ------------------------------------------------------
+
+// A real value as a float mantissa and an int32 exponent (one part of an FX).
+struct FS {
+    float m;
+    int e;
+};
+
+FAV_HD FS fs_norm(float m, int e) {
+    if (m == 0.0F) return FS{0.0F, 0};
+    const int shift = frexp_exp_f(m);
+    return FS{ldexp_f(m, -shift), e + shift};
+}
+
+FAV_HD FS fs_from_fe(FE v) { return fs_norm(static_cast<float>(v.m), v.e); }
+FAV_HD FS fs_mul(FS a, FS b) { return fs_norm(a.m * b.m, a.e + b.e); }
+
+FAV_HD FS fs_add(FS a, FS b) {
+    if (a.m == 0.0F) return b;
+    if (b.m == 0.0F) return a;
+    const bool swap = b.e > a.e;
+    const FS larger = swap ? b : a;
+    const FS smaller = swap ? a : b;
+    const int d = larger.e - smaller.e;
+    if (d > 30) return larger;
+    return fs_norm(larger.m + ldexp_f(smaller.m, -d), larger.e);
+}
+
+// The complex value (re, im), its parts aligned to the larger exponent.
+FAV_HD FX from_parts(FS re, FS im) {
+    if (re.m == 0.0F && im.m == 0.0F) return FX{0.0F, 0.0F, 0};
+    const int e = re.m == 0.0F ? im.e : (im.m == 0.0F ? re.e : (re.e > im.e ? re.e : im.e));
+    const float r = re.m == 0.0F ? 0.0F : (e - re.e > 30 ? 0.0F : ldexp_f(re.m, re.e - e));
+    const float i = im.m == 0.0F ? 0.0F : (e - im.e > 30 ? 0.0F : ldexp_f(im.m, im.e - e));
+    return normalize(r, i, e);
+}
+
+FAV_HD FX conj(FX v) { return FX{v.re, -v.im, v.e}; }
+FAV_HD FX abs_parts(FX v) { return FX{__builtin_fabsf(v.re), __builtin_fabsf(v.im), v.e}; }
+
+// apply_alternate_linear_bla (renderer.cpp:11399-11416) in mantissa
+// arithmetic: d' = M d + P q with the eight FP64 coefficients rounded to
+// float mantissas.
+FAV_HD FX alt_apply(const AltStepDev& s, FX d, FX q) {
+    const FS dr{d.re, d.e}, di{d.im, d.e}, qr{q.re, q.e}, qi{q.im, q.e};
+    FS c[8];
+    for (int i = 0; i < 8; ++i) c[i] = fs_from_fe(FE{s.m[i], s.e[i]});
+    const FS rr = fs_add(fs_add(fs_mul(c[0], dr), fs_mul(c[1], di)), fs_add(fs_mul(c[4], qr), fs_mul(c[5], qi)));
+    const FS ri = fs_add(fs_add(fs_mul(c[2], dr), fs_mul(c[3], di)), fs_add(fs_mul(c[6], qr), fs_mul(c[7], qi)));
+    return from_parts(rr, ri);
+}
+
+// One exact perturbation step of the alternate map at reference index i
+// (the steps of alternate_delta_step, renderer.cpp:11933-12034, and of the
+// reference's mixed-precision device kernel, renderer.cpp:1537-1701).
+FAV_HD FX alt_step(const AltRenderParams& p, int i, FX d, FX pd) {
+    const SC z_sc = alt::load_z(p, i);
+    const FX z = from_sc(z_sc);
+    if (p.formula == 3) {
+        const FX cz = conj(z), cd = conj(d);
+        return add(twice(mul(cz, cd)), add(mul(cd, cd), pd));
+    }
+    if (p.formula == 2) {
+        // the sign tests per component on FloatExp sums, as the FP64 path
+        // takes them: the shared exponent of a float pair cannot hold a part
+        // of Z far below the other (Im Z near 0 on the real axis) against a
+        // delta part above it
+        const FEC zp = sc_parts(z_sc);
+        const FE tr = fe_add(zp.re, fe_from_parts(static_cast<double>(d.re), d.e));
+        const FE ti = fe_add(zp.im, fe_from_parts(static_cast<double>(d.im), d.e));
+        const bool crosses_real = (fe_compare(zp.re, fe_zero()) >= 0) != (fe_compare(tr, fe_zero()) >= 0);
+        const bool crosses_imag = (fe_compare(zp.im, fe_zero()) >= 0) != (fe_compare(ti, fe_zero()) >= 0);
+        if (crosses_real || crosses_imag) {
+            // the crossing perturbation is rebuilt from |Z+d|^2 minus the next
+            // reference value: a difference of values of the reference's
+            // size, taken in FP64 as the FP64 path takes it (a float mantissa
+            // would round away the perturbation)
+            return from_sc(alt::delta_step(p, i, SC{d.re, d.im, d.e}, SC{pd.re, pd.im, pd.e}));
+        }
+        const FX sd{zp.re.m < 0.0 ? -d.re : d.re, zp.im.m < 0.0 ? -d.im : d.im, d.e};
+        return add(twice(mul(abs_parts(z), sd)), add(mul(sd, sd), pd));
+    }
+    return add(twice(mul(z, d)), add(mul(d, d), pd));
+}
+
+// |d|^2 above 2^kDoubleDelta hands an alternate-formula pixel to the FP64
+// loop: the bound at which the Mandelbrot path's FP64 plain-double tail
+// starts (renderer.cpp:13283-13290).
+constexpr int kDoubleDelta = -90;
+
+FAV_HD void alt_finish(const FxOut& out, size_t index, float value, int count) {
+    out.field[index] = value;
+    out.counts[index] = count;
+}
+
+// One pixel of an alternate-formula field in floatexp: the control flow of
+// render_alternate_reference_impl (alt_core.h pixel_body) without the cycle
+// test and time budget; escape and block-validity decisions on FP64 norms of
+// the floatexp totals.
+FAV_HD void alt_pixel(const AltRenderParams& p, const FxOut& out, int px, int py) {
+    const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + static_cast<size_t>(px);
+    const bool julia = p.formula == 1;
+    const int mode = p.escape_radius_mode;
+    const double bias = p.output_bias;
+    const int max_iter = p.max_iter;
+    const SC dc_sc = alt::pixel_dc(p, px, py);
+    const SC pd_sc = julia ? sc_zero() : dc_sc;
+    const SN pnorm = sc_norm_squared(pd_sc);
+    const FE pn{pnorm.m, pnorm.e};
+    const FX pd = from_sc(pd_sc);
+    FX delta = from_sc(dc_sc);
+    int ref = julia ? 0 : 1, iteration = julia ? 0 : 1;
+    if (p.series_available && sc_compare_norm(sc_norm_squared(dc_sc), SN{p.series.radius2_m, p.series.radius2_e}) <= 0) {
+        delta = from_sc(alt::series_value(p.series, dc_sc));
+        ref = p.series.iteration;
+        iteration = p.series.iteration;
+    }
+    const SN start_norm = norm2(add(from_sc(alt::load_z(p, ref)), delta));
+    if (sc_outside_escape(start_norm, mode)) {
+        alt_finish(out, index, core::smooth_escape(iteration, start_norm, bias), iteration);
+        return;
+    }
+    const SN limit{0.75, 2};
+    while (iteration < max_iter) {
+        const int next = ref + 1;
+        const SN dn = norm2(delta);
+        if (next >= p.orbit_size || dn.e > kDoubleDelta) {
+            // past the reference's escape, or |d|^2 > 2^-90: the pixel
+            // continues in the FP64 loop from this loop top (a float mantissa
+            // cannot hold Z + d once d is no longer small against Z, nor the
+            // pixel's offset from the parameter on the full value)
+            const SC d{static_cast<double>(delta.re), static_cast<double>(delta.im), delta.e};
+            int work = 0;
+            const alt::Result r = alt::run_loop(p, dc_sc, pd_sc, pn, d, ref, iteration, sc_add(alt::load_z(p, ref), d),
+                                                sn_from_margin(alt::margin(p, ref, d), mode), work);
+            alt_finish(out, index, r.value, r.count);
+            return;
+        }
+        const int remaining = max_iter - iteration;
+        const int k = p.bla_enabled
+            ? alt::lookup(p, ref, FE{dn.m, dn.e}, pn, p.max_block < remaining ? p.max_block : remaining) : -1;
+        if (k >= 0 && ref + p.steps[k].length < p.orbit_size) {
+            const AltStepDev& step = p.steps[k];
+            const int end = ref + step.length;
+            const FX cand = alt_apply(step, delta, pd);
+            const FX ct = add(from_sc(alt::load_z(p, end)), cand);
+            const SN cn = norm2(ct);
+            if (!sc_outside_escape(cn, mode) && sc_compare_norm(cn, limit) < 0) {
+                delta = cand;
+                ref = end;
+                iteration += step.length;
+                continue;
+            }
+            // replay the block one exact step at a time
+            for (int replay = 0; replay < step.length && iteration < max_iter; ++replay) {
+                delta = alt_step(p, ref, delta, pd);
+                ++ref;
+                ++iteration;
+                const SN n = norm2(add(from_sc(alt::load_z(p, ref)), delta));
+                if (sc_outside_escape(n, mode)) {
+                    alt_finish(out, index, core::smooth_escape(iteration, n, bias), iteration);
+                    return;
+                }
+            }
+            continue;
+        }
+        delta = alt_step(p, ref, delta, pd);
+        ref = next;
+        ++iteration;
+        const SN n = norm2(add(from_sc(alt::load_z(p, ref)), delta));
+        if (sc_outside_escape(n, mode)) {
+            alt_finish(out, index, core::smooth_escape(iteration, n, bias), iteration);
+            return;
+        }
+    }
+    alt_finish(out, index, core::encode_iteration(max_iter, bias), max_iter);
 }
 
 }  // namespace fx

@@ -461,11 +461,14 @@ extern "C" int fav_host_check_fx(const char* x, const char* y, const char* ref_z
         p.series.e = img.series_e.data();
         p.linear = img.linear[prep.linear_tier].data();
         p.linear_dir = img.linear_dir[prep.linear_tier];
+        p.orbit.real_double = o.real_double.data();
+        p.orbit.imag_double = o.imag_double.data();
         p.x_off_m = prep.offset_m.data();
         p.y_off_m = prep.offset_m.data() + width;
         p.x_off_e = prep.offset_e.data();
         p.y_off_e = prep.offset_e.data() + width;
-        const fx::FxOut out{field, counts};
+        p.budget_ticks = 0;
+        const fx::FxOut out{field, counts, 1};
 #pragma omp parallel for schedule(dynamic, 16)
         for (int py = 0; py < height; ++py)
             for (int px = 0; px < width; ++px) fx::pixel(p, out, px, py);
@@ -525,6 +528,61 @@ extern "C" int fav_host_check_kfp_rect(const float* parent, const float* child, 
             for (int x = 0; x < width; ++x) fav::kfp::rect_atlas_pixel(k[0], k[1], g, x, y);
         return 0;
     } catch (...) {
+        return 1;
+    }
+}
+
+// The alternate-formula floatexp pixel (fx_core.h alt_pixel) on the CPU
+// (test-only), on the tables of a reference built exactly as for the FP64
+// alternate path, and the FP64 pixel (alt_core.h) of the same tables with
+// its escape counts.
+extern "C" int fav_host_check_fx_alt(const char* x, const char* y, const char* ref_zoom, int max_iter, int bits,
+                                     int formula, const char* julia_re, const char* julia_im,
+                                     const FractalRenderOptions* options, float* field, int32_t* counts,
+                                     float* fp64_field, int32_t* fp64_counts, int width, int height,
+                                     const char* zoom, char* error, int error_capacity) {
+    using namespace fav;
+    try {
+        auto c = build_reference(x, y, ref_zoom, max_iter, bits, 8, true, formula, options->escape_radius_mode,
+                                 options->coordinate_mode, julia_re, julia_im);
+        TableImage img;
+        make_table_image(*c, img);
+        FieldRequest request{field, width, height, zoom, max_iter, 8, 32, *options, nullptr, nullptr, false};
+        PreparedAlternate prep;
+        prepare_alternate(*c, request, prep);
+        AltRenderParams p = prep.params;
+        const OrbitData& o = *c->orbit;
+        p.re = img.re.data();
+        p.im = img.im.data();
+        p.e = img.e.data();
+        p.margin_m = o.alt_margin_m.data();
+        p.margin_e = o.alt_margin_e.data();
+        p.steps = img.alt.data();
+        p.dir = img.alt_dir;
+        p.series.re = img.series_re.data();
+        p.series.im = img.series_im.data();
+        p.series.e = img.series_e.data();
+        p.x_off_m = prep.offset_m.data();
+        p.y_off_m = prep.offset_m.data() + width;
+        p.x_off_e = prep.offset_e.data();
+        p.y_off_e = prep.offset_e.data() + width;
+        const fx::FxOut out{field, counts, 0};
+        int flag = 0;
+        unsigned long long start = 0;
+        p.budget_flag = &flag;
+        p.budget_start = &start;
+        p.budget_ticks = 0;
+        p.output = fp64_field;
+        p.counts = fp64_counts;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int py = 0; py < height; ++py)
+            for (int px = 0; px < width; ++px) {
+                fx::alt_pixel(p, out, px, py);
+                alt::pixel(p, px, py);
+            }
+        return 0;
+    } catch (const std::exception& e) {
+        std::snprintf(error, static_cast<size_t>(error_capacity), "%s", e.what());
         return 1;
     }
 }

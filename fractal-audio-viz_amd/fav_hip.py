@@ -159,6 +159,10 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p,
         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(NativeRenderOptions), ctypes.c_int]
     lib.fractal_hip_render_reference_floatexp.restype = ctypes.c_int
+    lib.fractal_hip_render_reference_counts.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(NativeRenderOptions)]
+    lib.fractal_hip_render_reference_counts.restype = ctypes.c_int
     lib.fractal_hip_batch_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(NativeRenderOptions)]
     lib.fractal_hip_batch_create.restype = ctypes.c_void_p
     lib.fractal_hip_batch_tile_count.argtypes = [ctypes.c_void_p]
@@ -337,6 +341,30 @@ class NativeReference:
         if status != 0:
             raise RuntimeError(self.library.fractal_last_error().decode())
         return (out, work_map) if work else out
+
+    def render_floatexp(self, width: int, height: int, zoom: str, max_iter: int, series_order: int = 3,
+                        series_block: int = 256, options: NativeRenderOptions | None = None):
+        """The FP32 floatexp field (H, W) and its per-pixel escape counts
+        (max_iter inside, -1 unresolved), any formula."""
+        return self._field_and_counts(self.library.fractal_hip_render_reference_floatexp, width, height, zoom,
+                                      max_iter, series_order, series_block, options, 0)
+
+    def render_counts(self, width: int, height: int, zoom: str, max_iter: int, series_order: int = 3,
+                      series_block: int = 256, options: NativeRenderOptions | None = None):
+        """The FP64 field (H, W) and its per-pixel escape counts, the
+        convention of render_floatexp."""
+        return self._field_and_counts(self.library.fractal_hip_render_reference_counts, width, height, zoom,
+                                      max_iter, series_order, series_block, options)
+
+    def _field_and_counts(self, entry, width, height, zoom, max_iter, series_order, series_block, options, *extra):
+        opts = options or self.options
+        out = np.empty((int(height), int(width)), dtype=np.float32)
+        counts = np.empty((int(height), int(width)), dtype=np.int32)
+        status = entry(out.ctypes.data, counts.ctypes.data, int(width), int(height), zoom.encode(), self.handle,
+                       int(max_iter), int(series_order), int(series_block), ctypes.byref(opts), *extra)
+        if status != 0:
+            raise RuntimeError(self.library.fractal_last_error().decode())
+        return out, counts
 
     def preview_features(self, width: int, height: int, zoom: str, max_iter: int, work_limit: int,
                          series_order: int = 3, series_block: int = 256,

@@ -496,11 +496,15 @@ int fractal_render_points_ex_planes(
 
 /* ---- Extensions of the HIP backend (no counterpart in renderer.h) ------ */
 
-/* The FP32 floatexp perturbation path of a Mandelbrot handle (no
- * counterpart in the reference): deltas in float mantissas with an int32
- * exponent, FP64 orbit/BLA tables, FP64 escape/rebase tests and smoothing.
- * Writes the float field and per-pixel escape counts (max_iter inside, -1
- * unresolved); device_pointers = 1 when both already live in HBM. */
+/* The FP32 floatexp perturbation path of a handle (no counterpart in the
+ * reference): deltas in float mantissas with an int32 exponent, FP64
+ * orbit/BLA tables, FP64 escape/rebase tests and smoothing.  Mandelbrot map:
+ * pixels finish in the FP64 plain-double tail where the FP64 path enters it.
+ * Julia, burning ship, tricorn (zoom >= 1e6): the map's exact perturbation
+ * step (the burning ship's sign branches and axis-crossing reconstruction)
+ * and the real 2x2 BLA blocks in float mantissas.  Writes the float field
+ * and per-pixel escape counts (max_iter inside, -1 unresolved);
+ * device_pointers = 1 when both already live in HBM. */
 int fractal_hip_render_reference_floatexp(
     float *output, int32_t *counts, int width, int height,
     const char *zoom_text, void *handle, int max_iter, int series_order,
@@ -521,6 +525,16 @@ int fractal_hip_render_reference_scheduled(
     int max_iter, int series_order, int series_block,
     const FractalRenderOptions *options, const uint32_t *tile_order,
     int tile_count, int32_t *work);
+
+/* The FP64 field of a handle (zoom >= 1e6, no planes) and each pixel's
+ * escape iteration (max_iter inside, -1 where the field is NaN), host
+ * pointers: the FP64 counts the floatexp path is measured against.  No
+ * counterpart in the reference (the reference reports counts only through
+ * the Mandelbrot iteration plane). */
+int fractal_hip_render_reference_counts(
+    float *output, int32_t *counts, int width, int height,
+    const char *zoom_text, void *handle, int max_iter, int series_order,
+    int series_block, const FractalRenderOptions *options);
 
 /* Tile features of a field for the cost model below: renders the
  * quarter-size preview ((w+3)/4 x (h+3)/4, same view) with at most

@@ -112,3 +112,53 @@ def render_alt(lib, case):
     if status != 0:
         raise RuntimeError(err.value.decode())
     return out.reshape(h, w)
+
+
+def bind_fx(lib):
+    lib.fav_host_check_fx.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+        ctypes.POINTER(native.RenderOptions), native.c_float_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+        ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.fav_host_check_fx.restype = ctypes.c_int
+    lib.fav_host_check_fx_alt.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(native.RenderOptions), native.c_float_p, ctypes.c_void_p,
+        native.c_float_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
+        ctypes.c_int]
+    lib.fav_host_check_fx_alt.restype = ctypes.c_int
+    return lib
+
+
+def render_fx(lib, meta):
+    """The Mandelbrot floatexp path's per-pixel code on the CPU: (field,
+    counts), the counts as the GPU entry writes them."""
+    opts = native.RenderOptions.defaults(backend=2, **meta["options"])
+    w, h = meta["width"], meta["height"]
+    field = np.empty(w * h, dtype=np.float32)
+    counts = np.empty(w * h, dtype=np.int32)
+    err = ctypes.create_string_buffer(512)
+    status = lib.fav_host_check_fx(meta["x"].encode(), meta["y"].encode(), meta["ref_zoom"].encode(),
+                                   meta["max_iter"], meta["bits"], ctypes.byref(opts),
+                                   field.ctypes.data_as(native.c_float_p), counts.ctypes.data, w, h,
+                                   meta["zoom"].encode(), meta["series_order"], meta["series_block"], err, 512)
+    if status != 0:
+        raise RuntimeError(err.value.decode())
+    return field, counts
+
+
+def render_fx_alt(lib, case):
+    """The alternate-formula floatexp pixel and the FP64 pixel on the CPU:
+    (fx field, fx counts, FP64 field, FP64 counts)."""
+    opts = native.RenderOptions.defaults(backend=2, **case.get("options", {}))
+    w, h = case["width"], case["height"]
+    out = [np.empty(w * h, dtype=np.float32), np.empty(w * h, dtype=np.int32),
+           np.empty(w * h, dtype=np.float32), np.empty(w * h, dtype=np.int32)]
+    err = ctypes.create_string_buffer(512)
+    status = lib.fav_host_check_fx_alt(
+        case["x"].encode(), case["y"].encode(), case.get("ref_zoom", case["zoom"]).encode(), case["max_iter"],
+        case["bits"], case["formula"], str(case["julia"][0]).encode(), str(case["julia"][1]).encode(),
+        ctypes.byref(opts), out[0].ctypes.data_as(native.c_float_p), out[1].ctypes.data,
+        out[2].ctypes.data_as(native.c_float_p), out[3].ctypes.data, w, h, case["zoom"].encode(), err, 512)
+    if status != 0:
+        raise RuntimeError(err.value.decode())
+    return tuple(out)

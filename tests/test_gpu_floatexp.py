@@ -10,6 +10,8 @@ import ctypes
 import numpy as np
 import pytest
 
+import alt_cases
+import fav_hip
 import golden_io
 import native
 
@@ -40,11 +42,12 @@ def fp64_counts(golden):
     return np.where(np.isnan(f), -1, np.where(oi >= cap, cap, oi + 1))
 
 
-# c1_series and bla_1e25 measure 9.09% and 9.81% (escapes after long
-# plain-tail runs, where the FP32 mantissa drifts): the bar sits just above
-# them, so a regression fails; the 0.1% target is not met on these fields.
+# With the FP64 plain-double tail, c1_series and bla_1e25 measure 3.74% and
+# 7.34% (escapes after long chaotic runs that start from the 24-bit
+# mantissa's rounding of the pre-tail perturbation): the bar sits just
+# above them, so a regression fails; the 0.1% target is not met there.
 @pytest.mark.parametrize("name,max_mismatch", [("c2_kalles", 0.0), ("deep_tier", 0.0), ("ultra_tier", 0.0),
-                                               ("cycle_1e100", 0.0), ("c1_series", 0.10), ("bla_1e25", 0.10)])
+                                               ("cycle_1e100", 0.0), ("c1_series", 0.045), ("bla_1e25", 0.08)])
 def test_floatexp_count_mismatch_against_fp64(gpu_product, name, max_mismatch):
     print(f"[gpu] floatexp {name}", flush=True)
     g = golden_io.load(name)
@@ -54,3 +57,72 @@ def test_floatexp_count_mismatch_against_fp64(gpu_product, name, max_mismatch):
     assert mismatch <= max_mismatch
     if max_mismatch == 0.0:
         np.testing.assert_allclose(field, g["field"].reshape(-1), rtol=0, atol=1e-3)
+
+
+@pytest.mark.parametrize("name", ["c1_series", "bla_1e25", "replay_misiurewicz"])
+def test_floatexp_device_equals_its_cpu_build(gpu_product, name):
+    """The gfx950 floatexp kernel and the same code built for the CPU agree
+    bit for bit (IEEE float and double operations, exact frexp / ldexp)."""
+    import host_check
+    print(f"[gpu] floatexp device = CPU build, {name}", flush=True)
+    g = golden_io.load(name)
+    field, counts = render_fx(gpu_product, g["meta"])
+    cpu_field, cpu_counts = host_check.render_fx(host_check.bind_fx(host_check.load()), g["meta"])
+    np.testing.assert_array_equal(counts, cpu_counts)
+    np.testing.assert_array_equal(field.view(np.uint32), cpu_field.view(np.uint32))
+
+
+ALT = {c["name"]: c for c in alt_cases.CASES}
+
+
+def alt_reference(case):
+    opts = fav_hip.NativeRenderOptions(**case["options"])
+    ref = fav_hip.NativeReference(case["x"], case["y"], case["ref_zoom"], case["max_iter"], case["bits"], 8, opts,
+                                  formula=case["formula"], julia=tuple(case["julia"]))
+    return ref, opts
+
+
+@pytest.mark.parametrize("name,bar", [("alt_burning_ship_1e7", 0.0), ("alt_burning_ship_1e15", 0.0),
+                                      ("c3_burning_ship_1e30", 0.0), ("c3_burning_ship_1e100", 0.03),
+                                      ("alt_julia_1e8", 0.02), ("alt_tricorn_1e7", 0.12)])
+def test_alternate_floatexp_against_fp64_counts(gpu_product, name, bar):
+    """Julia, burning ship and tricorn in floatexp against the FP64 counts
+    of the same handle (fractal_hip_render_reference_counts, whose field is
+    the golden one), and the kernel against its CPU build bit for bit."""
+    import host_check
+    print(f"[gpu] alternate floatexp {name}", flush=True)
+    case = ALT[name]
+    ref, opts = alt_reference(case)
+    try:
+        w, h, z, n = case["width"], case["height"], case["zoom"], case["max_iter"]
+        fp64_field, fp64 = ref.render_counts(w, h, z, n, 8, 32, options=opts)
+        golden = golden_io.load(name)
+        golden_io.assert_alt_parity(golden, fp64_field)
+        assert np.array_equal(fp64.reshape(-1) == -1, np.isnan(golden["field"].reshape(-1)))
+        field, counts = ref.render_floatexp(w, h, z, n, 8, 32, options=opts)
+    finally:
+        ref.close()
+    mismatch = float(np.mean(counts != fp64))
+    print(f"[gpu] alternate floatexp {name}: {mismatch * 100:.3f}% of pixels differ from FP64", flush=True)
+    assert mismatch <= bar
+    cpu_field, cpu_counts, _, _ = host_check.render_fx_alt(host_check.bind_fx(host_check.load()), case)
+    np.testing.assert_array_equal(counts.reshape(-1), cpu_counts)
+    np.testing.assert_array_equal(field.reshape(-1).view(np.uint32), cpu_field.view(np.uint32))
+
+
+def test_mandelbrot_counts_entry_follows_the_iteration_plane(gpu_product):
+    """fractal_hip_render_reference_counts on a Mandelbrot handle: the field
+    of the ordinary render and the counts of its iteration plane."""
+    print("[gpu] counts entry, Mandelbrot", flush=True)
+    g = golden_io.load("c1_series")
+    meta = g["meta"]
+    opts = fav_hip.NativeRenderOptions(**meta["options"])
+    ref = fav_hip.NativeReference(meta["x"], meta["y"], meta["ref_zoom"], meta["max_iter"], meta["bits"],
+                                  meta["image_series_order"], opts)
+    try:
+        field, counts = ref.render_counts(meta["width"], meta["height"], meta["zoom"], meta["max_iter"],
+                                          meta["series_order"], meta["series_block"], options=opts)
+    finally:
+        ref.close()
+    np.testing.assert_array_equal(counts.reshape(-1), fp64_counts(g))
+    np.testing.assert_array_equal(np.isnan(field.reshape(-1)), np.isnan(g["field"].reshape(-1)))
