@@ -424,6 +424,116 @@ def floatexp_boundary_mismatch(lib):
                 float(np.max(np.abs(field[agree].astype(np.float64) - f64[agree]))) if agree.any() else 0.0}
 
 
+def floatexp_precision_fields(lib, repeats=2):
+    """The floatexp path on the 2560x1440 near-nucleus precision fields
+    (Mandelbrot map and burning ship, zooms 1e30 and 1e100, cap 100000,
+    glitch rebase on): the fraction of pixels whose escape count differs
+    from the FP64 counts of the same handle (for the Mandelbrot map also
+    over the pixels a strict FP64 render resolves without a glitch), and
+    kernel time against the FP64 path."""
+    out = []
+    for formula in (0, 2):
+        for exponent in (30, 100):
+            f = fav_workloads.near_boundary_field(formula, 1, exponent)
+            w, h, z, n = f["width"], f["height"], f["zoom"], f["max_iter"]
+            series = (f["series_order"], f["series_block"])
+            opts = fav_hip.NativeRenderOptions(**f["options"])
+            ref = fav_hip.NativeReference(f["x"], f["y"], z, n, f["bits"], f["image_series_order"], opts,
+                                          formula=formula, julia=tuple(f["julia"]))
+            try:
+                fp64_field, fp64 = ref.render_counts(w, h, z, n, *series, options=opts)
+                fp64_ms = []
+                for _ in range(repeats):
+                    if formula == 0:
+                        ref.render(w, h, z, n, *series, options=opts)
+                    else:
+                        ref.render_scheduled(w, h, z, n, series_order=series[0], series_block=series[1], options=opts)
+                    fp64_ms.append(fav_hip.last_timing()["kernel_ms"])
+                fx_ms = []
+                for _ in range(repeats):
+                    field, counts = ref.render_floatexp(w, h, z, n, *series, options=opts)
+                    fx_ms.append(fav_hip.last_timing()["kernel_ms"])
+                entry = {"workload": f["name"], "fp64_kernel_ms": round(min(fp64_ms), 3),
+                         "floatexp_kernel_ms": round(min(fx_ms), 3),
+                         "pixels_differing_from_fp64_pct": round(float(np.mean(counts != fp64)) * 100.0, 4)}
+                if formula == 0:
+                    strict_opts = fav_hip.NativeRenderOptions(**dict(f["options"], allow_recovery=0))
+                    strict = ref.render(w, h, z, n, *series, options=strict_opts).reshape(h, w)
+                    resolved = ~np.isnan(strict)
+                    entry["resolved_without_glitch_pct"] = round(float(resolved.mean()) * 100.0, 2)
+                    entry["resolved_pixels_differing_pct"] = (
+                        round(float((counts != fp64)[resolved].mean()) * 100.0, 4) if resolved.any() else None)
+                out.append(entry)
+            finally:
+                ref.close()
+    return out
+
+
+def combined_leg(lib, hip, model, steps=3):
+    """The three additions together on the 3840x2160 zoom-1e100 cap-100000
+    Mandelbrot field (the deep point's 100-digit extension): a batch of
+    that field rendered by the floatexp pixel in the cost model's order,
+    against the plain FP64 kernel (no planes) on the same field, and the
+    fraction of pixels whose count differs from FP64.  The same for the
+    48-field atlas segment: floatexp batch in learned order against the
+    FP64 batch."""
+    work = dict(fav_workloads.field_1e100_1080p(), width=3840, height=2160)
+    w, h, z, n = work["width"], work["height"], work["zoom"], work["max_iter"]
+    opts = fav_hip.NativeRenderOptions(**work["options"])
+    field = dict(x=work["x"], y=work["y"], zoom=z, zoom_log10=100.0, width=w, height=h, max_iter=n, bits=work["bits"])
+    batch = fav_batch.Batch([field], options=opts, library=lib, hip=hip)
+    try:
+        ref = batch.references[0]
+        fp64_ms = []
+        for _ in range(steps + 1):
+            status = lib.fractal_hip_render_reference_device(batch.outputs[0], w, h, z.encode(), ref.handle, n,
+                                                             work["series_order"], work["series_block"],
+                                                             ctypes.byref(opts), None)
+            if status != 0:
+                raise RuntimeError(lib.fractal_last_error().decode())
+            fp64_ms.append(fav_hip.last_timing()["kernel_ms"])
+        fp64_field = batch.field(0).copy()
+        costs, times = fav_batch.learned_tile_costs(batch, model) if model is not None else (None, {"preview_ms": 0})
+        result = {"workload": "field_3840x2160_zoom1e100_cap100000", "plain_fp64_kernel_ms": round(min(fp64_ms[1:]), 3)}
+        for name, floatexp, order in (("batched_fp64_learned", False, 2), ("batched_floatexp_plain_order", True, 0),
+                                      ("batched_floatexp_learned", True, 2)):
+            batch.set_precision(floatexp)
+            batch.set_order(order, costs if order == 2 else None)
+            batch.render(5)
+            ms = min(batch.render(5) for _ in range(steps))
+            result[name + "_ms"] = round(ms, 3)
+        fx_field = batch.field(0)
+        batch.set_precision(False)
+        result["preview_and_model_ms"] = round(times["preview_ms"], 3)
+        combined = result["batched_floatexp_learned_ms"] + times["preview_ms"]
+        result["combined_vs_plain_fp64"] = round(result["plain_fp64_kernel_ms"] / combined, 3)
+        result["pixels_differing_from_fp64_pct"] = round(
+            float(np.mean(np.floor(fx_field) != np.floor(fp64_field))) * 100.0, 4)
+    finally:
+        batch.close()
+    fields = fav_batch.segment_fields()
+    seg = fav_batch.Batch(fields, library=lib, hip=hip, threads=16)
+    try:
+        seg.set_order(0)
+        seg.render(5)
+        fp64_ms = min(seg.render(5) for _ in range(2))
+        fp64 = [seg.field(i).copy() for i in range(len(fields))]
+        seg_costs, seg_times = fav_batch.learned_tile_costs(seg, model) if model is not None else (None, {})
+        seg.set_precision(True)
+        seg.set_order(2 if seg_costs is not None else 0, seg_costs)
+        seg.render(5)
+        fx_ms = min(seg.render(5) for _ in range(2))
+        moved = sum(int(np.sum(np.floor(seg.field(i)) != np.floor(fp64[i]))) for i in range(len(fields)))
+        result["segment"] = {"workload": "atlas segment: 48 fields, zoom 1..1e100",
+                             "batched_fp64_ms": round(fp64_ms, 3), "batched_floatexp_learned_ms": round(fx_ms, 3),
+                             "preview_and_model_ms": round(seg_times.get("preview_ms", 0.0), 3),
+                             "ratio": round(fp64_ms / (fx_ms + seg_times.get("preview_ms", 0.0)), 3),
+                             "pixels_whose_integer_part_differs_pct": round(100.0 * moved / seg.pixels(), 4)}
+    finally:
+        seg.close()
+    return result
+
+
 def floatexp_leg(lib, hip, ref, work, opts, d_field, fp64_field, fp64_orbit_iteration, args):
     """The FP32 floatexp path on the same field: throughput over the same
     step count, and the fraction of pixels whose escape count differs from
@@ -479,6 +589,7 @@ def floatexp_leg(lib, hip, ref, work, opts, d_field, fp64_field, fp64_orbit_iter
     return {"value": round(pixel_iterations / elapsed / 1e9, 3), "unit": "Gpixel-iter/s",
             "planes": False,
             "boundary_field": boundary,
+            "precision_fields": floatexp_precision_fields(lib),
             "fp64_without_planes_value": round(fp64_pixel_iterations / fp64_elapsed / 1e9, 3),
             "ms_per_step": round(elapsed * 1e3, 3), "mean_kernel_ms": round(float(np.mean(kernel_ms)), 3),
             "pixels_differing_from_fp64_pct": round(float(np.mean(counts != reference)) * 100.0, 4),
@@ -497,6 +608,9 @@ def main():
                         help="run only the learned tile-scheduler leg and print its JSON")
     parser.add_argument("--batch-only", action="store_true",
                         help="run only the batched atlas-segment leg and print its JSON")
+    parser.add_argument("--combined-only", action="store_true",
+                        help="run only the floatexp precision fields and the combined leg (with a trained "
+                             "cost model) and print their JSON")
     parser.add_argument("--copy-calibration-mib", type=int, default=0,
                         help="only run N device-to-device copies of this many MiB (HBM counter calibration)")
     args = parser.parse_args()
@@ -518,6 +632,12 @@ def main():
         return 0
     if args.batch_only:
         print(json.dumps({"batched": batch_leg(lib, hip, 2)}))
+        return 0
+    if args.combined_only:
+        model, _ = fav_sched.train_model(fav_sched.mandelbrot_training_fields() + fav_sched.training_fields(),
+                                         library=lib)
+        print(json.dumps({"precision_fields": floatexp_precision_fields(lib),
+                          "combined": combined_leg(lib, hip, model)}))
         return 0
     opts = fav_hip.NativeRenderOptions(**work["options"])
     t0 = time.perf_counter()
@@ -588,6 +708,7 @@ def main():
     boundary = boundary_leg(lib, hip, kfp, lut, args.steps)
     learned, model = (None, None) if args.no_learned else learned_leg(lib)
     batched = None if args.no_batch else batch_leg(lib, hip, 2, model)
+    combined = None if args.no_batch else combined_leg(lib, hip, model)
     atlas = atlas_leg(lib)
     hip.free(d_field)
     hip.free(d_planes_block)
@@ -634,6 +755,7 @@ def main():
         "boundary_fields": boundary,
         "batched": batched,
         "learned_schedule": learned,
+        "combined": combined,
         "atlas_colour": atlas,
         "host_setup": host_setup,
     }

@@ -1304,6 +1304,20 @@ int fractal_hip_batch_set_order(void* batch, int order, const float* tile_costs,
     }
 }
 
+int fractal_hip_batch_set_precision(void* batch, int floatexp) {
+    try {
+        const auto b = acquire_batch(batch);
+        if (!b) throw std::runtime_error("invalid or already-destroyed batch handle");
+        if (floatexp != 0 && floatexp != 1) throw std::runtime_error("batch precision must be 0 (FP64) or 1 (floatexp)");
+        fav::batch_set_floatexp(*b, floatexp != 0);
+        set_error("");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+
 int fractal_hip_batch_render(void* batch, int fast_blocks_per_cu, int use_graph) {
     try {
         const auto b = acquire_batch(batch);

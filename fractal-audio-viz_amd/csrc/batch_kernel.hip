@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include "direct_core.h"
+#include "fx_core.h"
 
 namespace fav {
 namespace {
@@ -209,6 +210,33 @@ __global__ void __launch_bounds__(256) batch_tail_kernel(BatchLaunch q) {
     }
 }
 
+// The floatexp batch: one persistent kernel over the tile queue, each
+// perturbation tile through the FP32 floatexp pixel (fx_core.h, with the
+// FP64 plain-double tail where the FP64 path enters it), each shallow tile
+// through the direct renderer.  No hand-off queues: the floatexp pixel runs
+// its whole loop in one kernel.
+__global__ void __launch_bounds__(256) batch_fx_kernel(BatchLaunch q) {
+    __shared__ RenderParams sp;
+    __shared__ unsigned slot;
+    int staged = -1;
+    for (;;) {
+        const TileClaim c = claim_tile(q, q.counters + 5, slot);
+        if (c.done) break;
+        int px, py;
+        if (is_direct(q, c.field)) {
+            const direct::DirectParams& d = q.direct_fields[c.field];
+            tile_pixel(d.width, c.tile, px, py);
+            if (px < d.width && py < d.height) direct::pixel<0, false, false>(d, px, py);
+        } else {
+            stage_field(q, sp, staged, c.field);
+            const RenderParams& p = sp;
+            tile_pixel(p.width, c.tile, px, py);
+            if (px < p.width && py < p.height) fx::pixel(p, fx::FxOut{p.output, nullptr, 1}, px, py);
+        }
+        __syncthreads();  // every wave is done with sp and slot before the next tile
+    }
+}
+
 // Clears the tile cursors ([0] fast, [5] plain) and the queue counters (a
 // kernel, so the launch sequence can be captured in a HIP graph).
 __global__ void batch_prologue(unsigned* counters) {
@@ -218,11 +246,16 @@ __global__ void batch_prologue(unsigned* counters) {
 }  // namespace
 
 // One batch: q.fields holds field_count RenderParams with tables, offsets,
-// outputs and state stores bound (q.direct_fields, when set, the direct
+// outputs and state stores bound (with q.floatexp, one persistent kernel of
+// fast_blocks blocks renders every tile in floatexp) (q.direct_fields, when set, the direct
 // parameters of the shallow fields); q.tiles the tile_count tile codes
 // (field << 20 | tile); the queues hold one slot per pixel of the batch.
 hipError_t launch_batch(const BatchLaunch& q, int fast_blocks, int persistent_blocks, hipStream_t stream) {
     batch_prologue<<<1, 64, 0, stream>>>(q.counters);
+    if (q.floatexp) {
+        batch_fx_kernel<<<dim3(fast_blocks), dim3(256), 0, stream>>>(q);
+        return hipGetLastError();
+    }
     batch_plain_kernel<<<dim3(persistent_blocks), dim3(256), 0, stream>>>(q);
     batch_fast_kernel<<<dim3(fast_blocks), dim3(256), 0, stream>>>(q);
     batch_general_kernel<<<dim3(persistent_blocks), dim3(256), 0, stream>>>(q);

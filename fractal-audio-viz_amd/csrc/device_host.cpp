@@ -1559,6 +1559,14 @@ void batch_set_order(PreparedBatch& b, int order, const float* costs, size_t cos
 
 size_t batch_tile_count(const PreparedBatch& b) { return b.field_major.size(); }
 
+void batch_set_floatexp(PreparedBatch& b, bool floatexp) {
+    LaunchState& s = state();
+    std::lock_guard<std::mutex> lock(s.mutex);
+    if ((b.q.floatexp != 0) == floatexp) return;
+    b.q.floatexp = floatexp ? 1 : 0;
+    b.drop_graph();  // the captured launch sequence differs
+}
+
 void batch_render(PreparedBatch& b, int fast_blocks_per_cu, bool use_graph) {
     LaunchState& s = state();
     std::lock_guard<std::mutex> lock(s.mutex);

@@ -297,6 +297,9 @@ struct BatchLaunch {
     unsigned long long* tail_queue;
     unsigned* counters;  // [0] fast tile cursor, [1]/[2] general count/cursor, [3]/[4] tail count/cursor,
                          // [5] plain tile cursor
+    // 1: the perturbation fields render through the FP32 floatexp pixel
+    // (fx_core.h) in one persistent kernel over the tile queue
+    int32_t floatexp;
 };
 
 }  // namespace fav

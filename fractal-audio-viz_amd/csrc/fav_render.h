@@ -77,6 +77,10 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
 // given per-tile cost, heaviest first (costs in field-major tile order)
 void batch_set_order(PreparedBatch& batch, int order, const float* costs, size_t cost_count);
 size_t batch_tile_count(const PreparedBatch& batch);
+// floatexp: the perturbation fields render through the FP32 floatexp pixel
+// in one persistent kernel (fast_blocks_per_cu blocks per CU); false: the
+// FP64 launch sequence
+void batch_set_floatexp(PreparedBatch& batch, bool floatexp);
 // one render; fast_blocks_per_cu <= 0 keeps the current setting
 void batch_render(PreparedBatch& batch, int fast_blocks_per_cu, bool use_graph);
 

@@ -155,7 +155,7 @@ FAV_HD void run_blocks(const RenderParams& p, FX& delta, FX dc, int& ref, int& i
 
 struct FxOut {
     float* field;
-    int32_t* counts;  // escape iteration, max_iter for interior, -1 for an unresolved (NaN) pixel
+    int32_t* counts;  // escape iteration, max_iter for interior, -1 for an unresolved (NaN) pixel; may be null
     // 1: where the FP64 path enters its plain-double tail (no BLA block and
     // |d|^2 > 2^-90), the pixel continues in that tail in FP64
     int32_t double_tail;
@@ -178,7 +178,7 @@ FAV_HD bool finish_in_double_tail(const RenderParams& p, const FxOut& out, size_
     if (gave_up && !deadline && !unresolved) return false;
     const float interior = core::encode_iteration(p.max_iter, p.output_bias);
     out.field[index] = value;
-    out.counts[index] = value != value ? -1 : (value == interior || iteration >= p.max_iter ? p.max_iter : iteration);
+    if (out.counts) out.counts[index] = value != value ? -1 : (value == interior || iteration >= p.max_iter ? p.max_iter : iteration);
     return true;
 }
 
@@ -191,7 +191,7 @@ FAV_HD void pixel(const RenderParams& p, const FxOut& out, int px, int py) {
     const int mode = p.escape_radius_mode;
     if (o.size < 2) {  // no orbit to perturb against
         out.field[index] = __builtin_nanf("");
-        out.counts[index] = -1;
+        if (out.counts) out.counts[index] = -1;
         return;
     }
     const SC dc_sc = core::pixel_dc(p, px, py);
@@ -231,7 +231,7 @@ FAV_HD void pixel(const RenderParams& p, const FxOut& out, int px, int py) {
                                            value);
             const float interior = core::encode_iteration(max_iter, p.output_bias);
             out.field[index] = value;
-            out.counts[index] = value != value ? -1 : (value == interior || iteration >= max_iter ? max_iter : iteration);
+            if (out.counts) out.counts[index] = value != value ? -1 : (value == interior || iteration >= max_iter ? max_iter : iteration);
             return;
         }
         const FX z = from_sc(core::load_z(o, ref));
@@ -239,14 +239,14 @@ FAV_HD void pixel(const RenderParams& p, const FxOut& out, int px, int py) {
         const SN norm = norm2(total);
         if (sc_outside_escape(norm, mode)) {
             out.field[index] = core::smooth_escape(iteration, norm, p.output_bias);
-            out.counts[index] = iteration;
+            if (out.counts) out.counts[index] = iteration;
             return;
         }
         const FE glitch{o.glitch_m[ref], o.glitch_e[ref]};
         if (glitch.m != 0.0 && fe_less_nonneg(FE{norm.m, norm.e}, glitch)) {
             if (!p.allow_recovery || ++rebases > 64) {
                 out.field[index] = __builtin_nanf("");
-                out.counts[index] = -1;
+                if (out.counts) out.counts[index] = -1;
                 return;
             }
             delta = total;
@@ -289,7 +289,7 @@ FAV_HD void pixel(const RenderParams& p, const FxOut& out, int px, int py) {
         ++iteration;
     }
     out.field[index] = core::encode_iteration(max_iter, p.output_bias);
-    out.counts[index] = max_iter;
+    if (out.counts) out.counts[index] = max_iter;
 }
 
 // ---- alternate formulas ------------------------------------------------------
@@ -384,7 +384,7 @@ constexpr int kDoubleDelta = -90;
 
 FAV_HD void alt_finish(const FxOut& out, size_t index, float value, int count) {
     out.field[index] = value;
-    out.counts[index] = count;
+    if (out.counts) out.counts[index] = count;
 }
 
 // One pixel of an alternate-formula field in floatexp: the control flow of

@@ -103,6 +103,11 @@ class Batch:
         if status != 0:
             raise RuntimeError(self.lib.fractal_last_error().decode())
 
+    def set_precision(self, floatexp: bool) -> None:
+        """FP64 launch sequence (False) or the floatexp pixel (True)."""
+        if self.lib.fractal_hip_batch_set_precision(self.handle, 1 if floatexp else 0) != 0:
+            raise RuntimeError(self.lib.fractal_last_error().decode())
+
     def render(self, blocks_per_cu: int = 0, graph: bool = True) -> float:
         """One batched render; returns the HIP-event time in ms."""
         status = self.lib.fractal_hip_batch_render(self.handle, blocks_per_cu, 1 if graph else 0)

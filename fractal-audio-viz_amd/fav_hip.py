@@ -169,6 +169,8 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.fractal_hip_batch_tile_count.restype = ctypes.c_int
     lib.fractal_hip_batch_set_order.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
     lib.fractal_hip_batch_set_order.restype = ctypes.c_int
+    lib.fractal_hip_batch_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.fractal_hip_batch_set_precision.restype = ctypes.c_int
     lib.fractal_hip_batch_render.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.fractal_hip_batch_render.restype = ctypes.c_int
     lib.fractal_hip_batch_destroy.argtypes = [ctypes.c_void_p]

@@ -628,6 +628,13 @@ int fractal_hip_batch_set_order(void *batch, int order, const float *tile_costs,
                                 int cost_count);
 /* One render of the batch; fast_blocks_per_cu 0 keeps the current grid,
  * use_graph 1 replays a captured HIP graph of the launch sequence. */
+/* Precision of a batch's perturbation fields: 0 = the FP64 launch sequence
+ * (default, each field bit-identical to its single-field render), 1 = the
+ * FP32 floatexp pixel of fractal_hip_render_reference_floatexp in one
+ * persistent kernel over the tile queue (fast_blocks_per_cu blocks per CU);
+ * shallow fields stay on the direct renderer.  No counterpart in the
+ * reference. */
+int fractal_hip_batch_set_precision(void *batch, int floatexp);
 int fractal_hip_batch_render(void *batch, int fast_blocks_per_cu, int use_graph);
 void fractal_hip_batch_destroy(void *batch);
 

@@ -57,3 +57,37 @@ def test_batch_graph_replays_repeatedly(small_batch):
         for i, expected in enumerate(eager):
             np.testing.assert_array_equal(small_batch.field(i).view(np.uint32), expected.view(np.uint32))
         small_batch.render_per_field()
+
+
+def test_floatexp_batch_fields_equal_single_floatexp_renders(small_batch):
+    """The batch in floatexp (one persistent kernel over the tile queue):
+    every perturbation field is its single-field floatexp render bit for
+    bit, every direct field its FP64 render, in the field-major and cost
+    orders and as a graph; switching back restores the FP64 fields."""
+    print("[gpu] floatexp batch", flush=True)
+    small_batch.set_order(0)
+    small_batch.set_precision(False)
+    small_batch.render(5, graph=False)
+    fp64 = [small_batch.field(i).copy() for i in range(len(small_batch.fields))]
+    expected = []
+    for i, f in enumerate(small_batch.fields):
+        if f["zoom_log10"] < 6.0:
+            expected.append(fp64[i])
+            continue
+        field, _ = small_batch.references[i].render_floatexp(f["width"], f["height"], f["zoom"], f["max_iter"],
+                                                             options=small_batch.options)
+        expected.append(field)
+    costs = np.random.default_rng(2).random(small_batch.tile_count).astype(np.float32)
+    small_batch.set_precision(True)
+    for order in (0, 2):
+        small_batch.set_order(order, costs if order == 2 else None)
+        for graph in (False, True):
+            small_batch.render(5, graph=graph)
+            for i, e in enumerate(expected):
+                np.testing.assert_array_equal(small_batch.field(i).view(np.uint32), e.view(np.uint32),
+                                              err_msg=f"floatexp order {order} graph {graph} field {i}")
+    small_batch.set_precision(False)
+    small_batch.set_order(0)
+    small_batch.render(5, graph=True)
+    for i, e in enumerate(fp64):
+        np.testing.assert_array_equal(small_batch.field(i).view(np.uint32), e.view(np.uint32))
