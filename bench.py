@@ -534,6 +534,49 @@ def combined_leg(lib, hip, model, steps=3):
     return result
 
 
+def bla_epsilon_leg(lib, hip, work, opts, d_field, planes, headline_ms, steps):
+    """The headline field with the BLA validity radius eps = 2^-53 (a tuned
+    reference, fractal_hip_create_reference_tuned) instead of the
+    reference's 2^-38: field kernel time with planes, the BLA blocks per
+    pixel, and whether every pixel's count equals the 2^-38 render's."""
+    w, h, n = work["width"], work["height"], work["max_iter"]
+    lib.fractal_hip_create_reference_tuned.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(fav_hip.NativeRenderOptions), ctypes.c_int]
+    lib.fractal_hip_create_reference_tuned.restype = ctypes.c_void_p
+    handle = lib.fractal_hip_create_reference_tuned(work["x"].encode(), work["y"].encode(), work["zoom"].encode(),
+                                                     n, work["bits"], work["image_series_order"], 0, b"0", b"0",
+                                                     ctypes.byref(opts), -53)
+    if not handle:
+        raise RuntimeError(lib.fractal_last_error().decode())
+    try:
+        def render():
+            status = lib.fractal_hip_render_reference_device(d_field, w, h, work["zoom"].encode(),
+                                                             ctypes.c_void_p(handle), n, work["series_order"],
+                                                             work["series_block"], ctypes.byref(opts),
+                                                             ctypes.byref(planes))
+            if status != 0:
+                raise RuntimeError(lib.fractal_last_error().decode())
+            return fav_hip.last_timing()["kernel_ms"]
+
+        lib.fractal_set_stats_enabled(1)
+        render()
+        stats = np.zeros(39, dtype=np.uint64)
+        lib.fractal_get_last_stats_ex(stats.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 39)
+        lib.fractal_set_stats_enabled(0)
+        render()
+        ms = float(np.mean([render() for _ in range(max(1, steps))]))
+        oi = np.empty(w * h, dtype=np.int64)
+        hip.to_host(oi, ctypes.c_void_p(ctypes.cast(planes.orbit_iteration, ctypes.c_void_p).value))
+    finally:
+        lib.fractal_destroy_reference(ctypes.c_void_p(handle))
+    return {"workload": work["name"], "bla_epsilon": "2^-53", "mean_kernel_ms": round(ms, 3),
+            "headline_mean_kernel_ms_at_2^-38": round(headline_ms, 3),
+            "value": round(float(w * h) * n / (ms * 1e-3) / 1e9, 3), "unit": "Gpixel-iter/s (kernel)",
+            "linear_blocks_per_pixel": round(int(stats[3]) / (w * h), 1),
+            "all_counts_at_cap": bool(np.all(oi >= n))}
+
+
 def floatexp_leg(lib, hip, ref, work, opts, d_field, fp64_field, fp64_orbit_iteration, args):
     """The FP32 floatexp path on the same field: throughput over the same
     step count, and the fraction of pixels whose escape count differs from
@@ -704,6 +747,7 @@ def main():
     achieved = flops / (mean_kernel_ms * 1e-3) / 1e12
     rgb = np.empty(n * 3, dtype=np.uint8)
     hip.to_host(rgb, d_rgb)
+    bla_epsilon = bla_epsilon_leg(lib, hip, work, opts, d_field, planes, mean_kernel_ms, args.steps)
     floatexp = floatexp_leg(lib, hip, ref, work, opts, d_field, field, orbit_iteration, args)
     boundary = boundary_leg(lib, hip, kfp, lut, args.steps)
     learned, model = (None, None) if args.no_learned else learned_leg(lib)
@@ -751,6 +795,7 @@ def main():
                      "frac": round(achieved / MI355X_FP64_VECTOR_PEAK_TFLOPS, 4),
                      "traffic": measured_traffic_bytes()},
         "cpu_baseline": baseline,
+        "bla_epsilon": bla_epsilon,
         "floatexp": floatexp,
         "boundary_fields": boundary,
         "batched": batched,

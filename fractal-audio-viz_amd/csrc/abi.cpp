@@ -194,7 +194,8 @@ int copy_stats(uint64_t* values, int capacity, int fields) {
 
 void* create(const char* x, const char* y, const char* zoom, int max_iter, int bits, int series_order,
              bool reusable, int formula, const char* julia_re, const char* julia_im,
-             const FractalRenderOptions* supplied, bool use_options, const char* what) {
+             const FractalRenderOptions* supplied, bool use_options, const char* what,
+             int bla_epsilon_exponent = -38) {
     try {
         if (!x || !y || !zoom || !julia_re || !julia_im || !valid_iterations(max_iter) || !valid_precision(bits)
             || !valid_series(series_order, 2) || formula < 0 || formula > 3)
@@ -206,7 +207,8 @@ void* create(const char* x, const char* y, const char* zoom, int max_iter, int b
         parse_coordinate(y, "imaginary");
         auto context = fav::build_reference(x, y, zoom, max_iter, bits, series_order, reusable, formula,
                                             use_options ? o.escape_radius_mode : 0,
-                                            use_options ? o.coordinate_mode : 0, julia_re, julia_im);
+                                            use_options ? o.coordinate_mode : 0, julia_re, julia_im,
+                                            bla_epsilon_exponent);
         void* handle = register_reference(std::move(context));
         set_error("");
         return handle;
@@ -923,6 +925,14 @@ void* fractal_create_reference_ex_options(const char* x, const char* y, const ch
                                           const char* julia_im, const FractalRenderOptions* options) {
     return create(x, y, zoom, max_iter, bits, series_order, formula != FRACTAL_FORMULA_MANDELBROT, formula,
                   julia_re, julia_im, options, true, "invalid formula-aware reference configuration");
+}
+
+void* fractal_hip_create_reference_tuned(const char* x, const char* y, const char* zoom, int max_iter, int bits,
+                                         int series_order, int formula, const char* julia_re, const char* julia_im,
+                                         const FractalRenderOptions* options, int bla_epsilon_exponent) {
+    return create(x, y, zoom, max_iter, bits, series_order, formula != FRACTAL_FORMULA_MANDELBROT, formula,
+                  julia_re, julia_im, options, true, "invalid formula-aware reference configuration",
+                  bla_epsilon_exponent);
 }
 
 void* fractal_clone_reference(void* source_handle, const char* zoom) {

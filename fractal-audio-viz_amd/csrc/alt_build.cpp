@@ -78,7 +78,7 @@ void build_alternate_bla(ReferenceContext& c) {
     const int last = std::min(c.map_end, orbit - 1);
     const int base_count = last - first;
     if (base_count <= 0) return;
-    const FE tolerance = fe_from_parts(std::ldexp(1.0, -38), 0);
+    const FE tolerance = fe_from_parts(std::ldexp(1.0, c.bla_epsilon_exponent), 0);
     const FE two = fe_from_parts(2.0, 0), quarter = fe_from_parts(0.25, 0), one = fe_from_parts(1.0, 0);
     std::vector<AltBuilder> base(static_cast<size_t>(base_count));
     for (int k = 0; k < base_count; ++k) {
@@ -151,7 +151,7 @@ void stabilize_alternate_cycle(ReferenceContext& c, OrbitData& o) {
     const size_t scan_limit = std::min<size_t>(last_finite - 1, 1024);
     const int max_period = std::min<int>(64, static_cast<int>((scan_limit + 1) / 4));
     const double tolerance = 1.0e-12;
-    const double r2 = c.escape_radius_mode == 1 ? 1.0e8 : 4.0;
+    const double r2 = escape_r2_double(c.escape_radius_mode);
     for (int period = 1; period <= max_period; ++period) {
         for (size_t end = static_cast<size_t>(4 * period - 1); end <= scan_limit; ++end) {
             const size_t start = end - static_cast<size_t>(3 * period) + 1U;

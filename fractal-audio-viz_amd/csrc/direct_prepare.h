@@ -92,7 +92,7 @@ inline DirectSetup make_setup_values(int width, int height, long double zoom, lo
     p.julia_re = julia_re;
     p.julia_im = julia_im;
     p.output_bias = output_bias;
-    p.escape2 = escape_radius_mode == 1 ? 1.0e8 : 4.0;
+    p.escape2 = escape_r2_double(escape_radius_mode);
     p.safe_escape2 = p.escape2 + 1.0e-7;
     p.log_power = std::log(2.0);
     const FE spacing = fe_from_parts(height_span / static_cast<double>(height), 0);

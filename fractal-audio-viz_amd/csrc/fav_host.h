@@ -81,6 +81,9 @@ struct ReferenceContext {
     int formula = 0;
     int escape_radius_mode = 0;
     int coordinate_mode = 0;
+    // BLA validity radius eps = 2^bla_epsilon_exponent * |Z| (the
+    // reference's -38, renderer.cpp:11644-11645)
+    int bla_epsilon_exponent = -38;
     long precision_bits = 0;
     int requested_max_iter = 0;
     int requested_series_order = 8;
@@ -118,7 +121,7 @@ struct ReferenceContext {
 std::unique_ptr<ReferenceContext> build_reference(
     const char* x, const char* y, const char* zoom, int max_iter, int precision_bits,
     int series_order, bool reusable, int formula, int escape_radius_mode, int coordinate_mode,
-    const char* julia_re = "0", const char* julia_im = "0");
+    const char* julia_re = "0", const char* julia_im = "0", int bla_epsilon_exponent = -38);
 
 // Alternate formulas (alt_build.cpp): the real 2x2 linear BLA hierarchy and
 // the settled-cycle repair of the projected orbit.

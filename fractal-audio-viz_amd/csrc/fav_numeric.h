@@ -128,6 +128,9 @@ FAV_HD FE fe_escape_r2(int mode) {
     return mode == 1 ? FE{0.7450580596923828125, 27} : FE{0.5, 3};
 }
 
+// R^2 of an escape-radius mode as a plain double (4 or 1e8).
+FAV_HD double escape_r2_double(int mode) { return mode == 1 ? 1.0e8 : 4.0; }
+
 FAV_HD FE fe_neg(FE v) { return FE{-v.m, v.e}; }
 
 // fe_add, renderer.cpp:8198-8235.  The reference renormalises the aligned

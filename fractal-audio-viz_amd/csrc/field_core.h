@@ -423,7 +423,7 @@ template <bool Planes, bool Stats>
 FAV_HD TailStatus tail_step(const TailParams& q, const PlaneOut& planes, size_t index, int& iteration, int& ref,
                             SC& derivative, Budget& budget, bool& deadline, bool& unresolved, Counters<Stats>& st,
                             TailState& t, float& out) {
-    const double escape_squared = q.escape_radius_mode == 1 ? 1.0e8 : 4.0;
+    const double escape_squared = escape_r2_double(q.escape_radius_mode);
     const double safe_escape_squared = escape_squared + 1.0e-7;
     // Budget::expired returns false without counting a tick when no budget
     // is set, so skipping the call then changes nothing

@@ -213,7 +213,7 @@ void make_orbit(ReferenceContext& c, const char* x, const char* y, const char* z
     c.fast_orbit.clear();
     c.fast_orbit.reserve(static_cast<size_t>(max_iter) + 1);
     alt_margins.clear();
-    const double r2_double = c.escape_radius_mode == 1 ? 1.0e8 : 4.0;
+    const double r2_double = escape_r2_double(c.escape_radius_mode);
     FE ore = fe_from_mpfr(zr.v), oim = fe_from_mpfr(zi.v);
     for (int i = 0; i <= max_iter; ++i) {
         if (!fe_finite(ore) || !fe_finite(oim)) break;
@@ -610,7 +610,7 @@ LinearStepDev compact_linear(const LinearBuilder& b) {
 
 // build_linear_levels, renderer.cpp:11712-11757.
 LinearLevels build_linear_levels(const ReferenceContext& c, int base_count, FE composition_radius) {
-    const FE tolerance = fe_from_parts(std::ldexp(1.0, -38), 0);
+    const FE tolerance = fe_from_parts(std::ldexp(1.0, c.bla_epsilon_exponent), 0);
     const FE two = fe_from_parts(2.0, 0);
     std::vector<std::vector<LinearBuilder>> builder(1);
     builder[0].resize(static_cast<size_t>(base_count));
@@ -653,7 +653,7 @@ void build_bla(ReferenceContext& c) {
     c.linear[0].levels.clear();
     c.linear[1].levels.clear();
     if (base_count == 0) return;
-    const FE tolerance = fe_from_parts(std::ldexp(1.0, -38), 0);
+    const FE tolerance = fe_from_parts(std::ldexp(1.0, c.bla_epsilon_exponent), 0);
     const FE two = fe_from_parts(2.0, 0);
     const FEC zero{fe_zero(), fe_zero()};
     const FEC one{fe_from_parts(1.0, 0), fe_zero()};
@@ -749,12 +749,15 @@ FE parse_zoom(const char* text, long precision_bits) {  // renderer.cpp:11892-11
 std::unique_ptr<ReferenceContext> build_reference(
     const char* x, const char* y, const char* zoom, int max_iter, int precision_bits, int series_order,
     bool reusable, int formula, int escape_radius_mode, int coordinate_mode, const char* julia_re,
-    const char* julia_im) {
+    const char* julia_im, int bla_epsilon_exponent) {
     if (formula < 0 || formula > 3) throw std::runtime_error("invalid native reference formula or Julia constant");
+    if (bla_epsilon_exponent < -60 || bla_epsilon_exponent > -20)
+        throw std::runtime_error("BLA epsilon exponent must be in [-60, -20]");
     (void)parse_zoom(zoom, precision_bits);  // renderer.cpp:13638-13645
     auto c = std::make_unique<ReferenceContext>();
     c->formula = formula;
     c->escape_radius_mode = escape_radius_mode;
+    c->bla_epsilon_exponent = bla_epsilon_exponent;
     c->coordinate_mode = coordinate_mode;
     c->precision_bits = std::max(128, precision_bits);
     c->requested_max_iter = max_iter;
@@ -807,6 +810,7 @@ std::unique_ptr<ReferenceContext> clone_reference(const ReferenceContext& s, con
     c->y_text = s.y_text;
     c->formula = s.formula;
     c->escape_radius_mode = s.escape_radius_mode;
+    c->bla_epsilon_exponent = s.bla_epsilon_exponent;
     c->coordinate_mode = s.coordinate_mode;
     c->precision_bits = s.precision_bits;
     c->reference_build_ns = now_ns() - t0;

@@ -496,6 +496,17 @@ int fractal_render_points_ex_planes(
 
 /* ---- Extensions of the HIP backend (no counterpart in renderer.h) ------ */
 
+/* fractal_create_reference_ex_options with the BLA validity radius
+ * eps = 2^bla_epsilon_exponent * |Z| instead of the reference's fixed
+ * 2^-38 (renderer.cpp:11644-11645; linear, cubic and alternate tables;
+ * -60..-20).  A smaller eps gives shorter BLA blocks whose truncation
+ * error is smaller.  No counterpart in the reference. */
+void *fractal_hip_create_reference_tuned(
+    const char *x_center, const char *y_center, const char *viewport_zoom,
+    int max_iter, int precision_bits, int image_series_order, int formula,
+    const char *julia_real, const char *julia_imag,
+    const FractalRenderOptions *options, int bla_epsilon_exponent);
+
 /* The FP32 floatexp perturbation path of a handle (no counterpart in the
  * reference): deltas in float mantissas with an int32 exponent, FP64
  * orbit/BLA tables, FP64 escape/rebase tests and smoothing.  Mandelbrot map:

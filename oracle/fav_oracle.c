@@ -79,7 +79,9 @@ static inline int fe_zero(fe_t v) { return v.m == 0.0; }
 
 /* escape_radius_squared_float_exp, renderer.cpp:7594-7598. */
 static inline fe_t fe_escape_r2(int mode) {
-    return mode == 1 ? fe_from_parts(1.0e8, 0) : fe_from_parts(4.0, 0);
+    /* mode 2: radius 1e6 (R^2 = 1e12), a mode of neither the reference nor
+       the HIP library, kept to show what the reference's norm does there */
+    return mode == 1 ? fe_from_parts(1.0e8, 0) : (mode == 2 ? fe_from_parts(1.0e12, 0) : fe_from_parts(4.0, 0));
 }
 
 /* ---- FloatExp arithmetic (renderer.cpp:8190-8349) ---------------------- */
@@ -445,6 +447,7 @@ struct fo_reference {
     int series_count;
     int requested_max_iter, requested_series_order;
     int escape_radius_mode, coordinate_mode;
+    int bla_epsilon_exponent;    /* BLA validity radius eps = 2^this * |Z| (the reference's -38) */
     long precision_bits;
     /* alternate formulas (renderer.cpp:8507-8565, 8881-8905) */
     int formula;                 /* 0 Mandelbrot, 1 Julia, 2 burning ship, 3 tricorn */
@@ -485,7 +488,7 @@ static int parse_zoom_fe(const char *text, long precision, fe_t *out) {
 static inline double viewport_height_factor(int mode) { return mode == 1 ? 4.0 : 2.8; } /* 153-158 */
 
 /* escape_radius_squared_double, renderer.cpp:160-168 (R = 2 or 10000). */
-static inline double mode_r2_double(int mode) { return mode == 1 ? 1.0e8 : 4.0; }
+static inline double mode_r2_double(int mode) { return mode == 1 ? 1.0e8 : (mode == 2 ? 1.0e12 : 4.0); }
 
 static inline double pixel_axis_offset(int index, int dimension, int mode) { /* 142-151 */
     const double origin = mode == 1 ? (double)(dimension / 2) : (double)(dimension - 1) / 2.0;
@@ -937,7 +940,7 @@ static linear_step_t compact_linear(const linear_builder_t *b) {
 
 /* build_linear_levels lambda, renderer.cpp:11712-11757. */
 static int build_linear_levels(fo_reference *r, int base_count, fe_t composition_radius, int tier) {
-    const fe_t tolerance = fe_from_parts(ldexp(1.0, -38), 0);
+    const fe_t tolerance = fe_from_parts(ldexp(1.0, r->bla_epsilon_exponent), 0);
     const fe_t two = fe_from_parts(2.0, 0);
     int level_count = 1;
     size_t sizes[64];
@@ -999,7 +1002,7 @@ static int build_bla(fo_reference *r) {
     r->cubic_levels = 0;
     r->linear_levels[0] = r->linear_levels[1] = 0;
     if (base_count == 0) return 1;
-    const fe_t tolerance = fe_from_parts(ldexp(1.0, -38), 0);
+    const fe_t tolerance = fe_from_parts(ldexp(1.0, r->bla_epsilon_exponent), 0);
     const fe_t two = fe_from_parts(2.0, 0);
     size_t sizes[64];
     int level_count = 1;
@@ -1064,8 +1067,16 @@ fo_reference *fo_reference_create_formula(const char *x, const char *y, const ch
                                           int precision_bits, int series_order, int escape_radius_mode,
                                           int coordinate_mode, int formula, const char *julia_re,
                                           const char *julia_im, char *error, int cap) {
+    return fo_reference_create_tuned(x, y, zoom, max_iter, precision_bits, series_order, escape_radius_mode,
+                                     coordinate_mode, formula, julia_re, julia_im, -38, error, cap);
+}
+
+fo_reference *fo_reference_create_tuned(const char *x, const char *y, const char *zoom, int max_iter,
+                                        int precision_bits, int series_order, int escape_radius_mode,
+                                        int coordinate_mode, int formula, const char *julia_re,
+                                        const char *julia_im, int bla_epsilon_exponent, char *error, int cap) {
     if (!x || !y || !zoom || max_iter <= 0 || precision_bits < 128 || formula < 0 || formula > 3 || !julia_re
-        || !julia_im) {
+        || !julia_im || bla_epsilon_exponent < -60 || bla_epsilon_exponent > -20) {
         fo_set_error(error, cap, "invalid reference configuration");
         return NULL;
     }
@@ -1078,6 +1089,7 @@ fo_reference *fo_reference_create_formula(const char *x, const char *y, const ch
     if (!r) return NULL;
     r->escape_radius_mode = escape_radius_mode;
     r->coordinate_mode = coordinate_mode;
+    r->bla_epsilon_exponent = bla_epsilon_exponent;
     r->requested_series_order = series_order < 8 ? 8 : (series_order > 32 ? 32 : series_order);
     r->requested_max_iter = max_iter;
     r->formula = formula;
@@ -1157,6 +1169,7 @@ fo_reference *fo_reference_clone(const fo_reference *s, const char *zoom, char *
     r->requested_series_order = s->requested_series_order;
     r->escape_radius_mode = s->escape_radius_mode;
     r->coordinate_mode = s->coordinate_mode;
+    r->bla_epsilon_exponent = s->bla_epsilon_exponent;
     r->precision_bits = s->precision_bits;
     r->input_radius = fe_div(fe_from_parts(viewport_height_factor(r->coordinate_mode), 0), parsed);
     if (s->formula != 0) {
@@ -2235,7 +2248,7 @@ static int build_alternate_bla(fo_reference *r) {
     size_t offset = 0;
     for (int l = 0; l < levels; ++l) { r->alt_offset[l] = offset; r->alt_size[l] = sizes[l]; offset += sizes[l]; }
     r->alt_levels = levels;
-    const fe_t tolerance = fe_from_parts(ldexp(1.0, -38), 0);
+    const fe_t tolerance = fe_from_parts(ldexp(1.0, r->bla_epsilon_exponent), 0);
     const fe_t two = fe_from_parts(2.0, 0), quarter = fe_from_parts(0.25, 0), one = fe_from_parts(1.0, 0);
     for (int k = 0; k < base_count; ++k) {
         const fec_t ref = r->fast_orbit[first + k];

@@ -48,6 +48,17 @@ fo_reference *fo_reference_create_formula(
     int series_order, int escape_radius_mode, int coordinate_mode, int formula,
     const char *julia_re, const char *julia_im, char *error, int error_capacity);
 
+/* The same with the BLA validity radius eps = 2^bla_epsilon_exponent
+ * (the reference's -38, renderer.cpp:11644-11645; -60..-20).
+ * escape_radius_mode 2 (every function here) is radius 1e6 (R^2 = 1e12):
+ * not a mode of the reference or of the HIP library; the oracle carries it
+ * to show what the reference's arithmetic does with that radius. */
+fo_reference *fo_reference_create_tuned(
+    const char *x, const char *y, const char *zoom, int max_iter, int precision_bits,
+    int series_order, int escape_radius_mode, int coordinate_mode, int formula,
+    const char *julia_re, const char *julia_im, int bla_epsilon_exponent,
+    char *error, int error_capacity);
+
 fo_reference *fo_reference_clone(
     const fo_reference *source, const char *viewport_zoom,
     char *error, int error_capacity);

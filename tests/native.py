@@ -282,18 +282,20 @@ class OracleReference:
     """A fo_reference owned by Python."""
 
     def __init__(self, lib, x, y, zoom, max_iter, precision_bits, series_order=8,
-                 escape_radius_mode=0, coordinate_mode=0, handle=None, formula=0, julia=("0", "0")):
+                 escape_radius_mode=0, coordinate_mode=0, handle=None, formula=0, julia=("0", "0"),
+                 bla_epsilon_exponent=-38):
         self.lib = lib
         if handle is None:
             err = ctypes.create_string_buffer(512)
-            lib.fo_reference_create_formula.argtypes = [
+            lib.fo_reference_create_tuned.argtypes = [
                 ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
-                ctypes.c_int]
-            lib.fo_reference_create_formula.restype = ctypes.c_void_p
-            handle = lib.fo_reference_create_formula(_b(x), _b(y), _b(zoom), max_iter, precision_bits,
-                                                      series_order, escape_radius_mode, coordinate_mode,
-                                                      formula, _b(julia[0]), _b(julia[1]), err, 512)
+                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                ctypes.c_char_p, ctypes.c_int]
+            lib.fo_reference_create_tuned.restype = ctypes.c_void_p
+            handle = lib.fo_reference_create_tuned(_b(x), _b(y), _b(zoom), max_iter, precision_bits,
+                                                    series_order, escape_radius_mode, coordinate_mode,
+                                                    formula, _b(julia[0]), _b(julia[1]), bla_epsilon_exponent,
+                                                    err, 512)
             if not handle:
                 raise RuntimeError(err.value.decode())
         self.handle = handle
@@ -357,12 +359,21 @@ class AbiReference:
     """A reference handle created through a renderer.h-speaking library."""
 
     def __init__(self, lib, x, y, zoom, max_iter, precision_bits, series_order=8,
-                 options=None, formula=0, julia=("0", "0")):
+                 options=None, formula=0, julia=("0", "0"), bla_epsilon_exponent=None):
         self.lib = lib
         opts = options if options is not None else RenderOptions.defaults()
-        handle = lib.fractal_create_reference_ex_options(
-            _b(x), _b(y), _b(zoom), max_iter, precision_bits, series_order, formula,
-            _b(julia[0]), _b(julia[1]), ctypes.byref(opts))
+        if bla_epsilon_exponent is None:
+            handle = lib.fractal_create_reference_ex_options(
+                _b(x), _b(y), _b(zoom), max_iter, precision_bits, series_order, formula,
+                _b(julia[0]), _b(julia[1]), ctypes.byref(opts))
+        else:  # the HIP library's extension entry
+            lib.fractal_hip_create_reference_tuned.argtypes = [
+                ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(RenderOptions), ctypes.c_int]
+            lib.fractal_hip_create_reference_tuned.restype = ctypes.c_void_p
+            handle = lib.fractal_hip_create_reference_tuned(
+                _b(x), _b(y), _b(zoom), max_iter, precision_bits, series_order, formula,
+                _b(julia[0]), _b(julia[1]), ctypes.byref(opts), int(bla_epsilon_exponent))
         if not handle:
             raise RuntimeError(lib.fractal_last_error().decode())
         self.handle = handle
