@@ -112,6 +112,51 @@ def test_repair_over_oracle_matches_reference(oracle, case):
     assert np.isfinite(repaired).all() and np.isnan(golden["initial"]).any()
 
 
+def load_visualizer_fixture(case):
+    import json
+    from pathlib import Path
+    path = Path(__file__).resolve().parent / "golden" / f"repair_visualizer_{case['name'][len('repair_'):]}.npz"
+    data = np.load(path, allow_pickle=False)
+    return json.loads(str(data["meta"])), data["field"]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_driver_against_the_reference_repair_function(case):
+    """The reference's own `_atlas_glitch_reference_field` (run over its
+    renderer.cpp by tests/golden/make_golden_repair_visualizer.py, from the
+    same strict initial field) against this driver's fixture, which the
+    driver reproduces over the oracle and the HIP path (tests above and
+    tests/test_gpu_repair.py):
+
+    * a region large enough for the region queue (the 9216-pixel all-NaN
+      BLA-off field at 1e100, split into four secondary references): the
+      whole repaired field bit for bit;
+    * 14 scattered glitch pixels (Old Wood Dish at 1e60): the reference
+      takes its exact arbitrary-precision pixel retry for masks this small
+      (visualizer.py:8472-8526, 8640-8650), which this driver leaves out.
+      Its values equal the driver's secondary-reference values bit for bit
+      on the 14 pixels; around them the driver, as the reference's region
+      path does (visualizer.py:8905-8906), also takes the secondary render's
+      finite halo pixels, which the exact retry leaves at their initial
+      values."""
+    meta, reference_field = load_visualizer_fixture(case)
+    golden = repair_io.load(case["name"])
+    initial, repaired = golden["initial"], golden["repaired"]
+    glitched = np.isnan(initial)
+    if meta["diagnostics"]["exact_pixel_recoveries"] == 0:
+        np.testing.assert_array_equal(reference_field.view(np.uint32), repaired.view(np.uint32))
+        assert meta["diagnostics"]["secondary_references"] == golden["meta"]["diagnostics"]["secondary_references"]
+        return
+    assert meta["diagnostics"]["exact_pixel_recoveries"] == int(glitched.sum())
+    np.testing.assert_array_equal(reference_field[glitched].view(np.uint32), repaired[glitched].view(np.uint32))
+    np.testing.assert_array_equal(reference_field[~glitched].view(np.uint32), initial[~glitched].view(np.uint32))
+    changed = repaired != initial
+    boxes = np.zeros_like(glitched)
+    for x0, x1, y0, y1, *_ in fav_repair.unresolved_regions(glitched):
+        boxes[y0:y1, x0:x1] = True
+    assert not (changed & ~glitched & ~boxes).any()  # the driver only touches the regions' boxes
+
+
 def test_spatial_recovery_grows_means_inward():
     """A 3x3 hole in a linear ramp: the ring next to the finite pixels is
     the mean of its finite neighbours, the centre the mean of the ring;
