@@ -139,7 +139,10 @@ DeviceTables::~DeviceTables() {
     if (series_block) (void)hipFree(series_block);
     if (alt_block) (void)hipFree(alt_block);
     if (fast_block) (void)hipFree(fast_block);
-    for (auto& tier : rung_block) for (void* p : tier) if (p) (void)hipFree(p);
+    for (auto& tier : rung_block)
+        for (auto& cap : tier)
+            for (void* p : cap)
+                if (p) (void)hipFree(p);
 }
 
 namespace {
@@ -316,22 +319,24 @@ void ensure_device_tables(ReferenceContext& c) {
 }
 
 // The plain-double block loop's rung table of linear tier `tier` for level
-// cap `cap` (table_image.cpp make_plain_tables), built and copied to HBM on
-// first use; null when the reference has none.
-void ensure_plain_tables(ReferenceContext& c, int tier, int cap, const RungDev*& rungs) {
+// cap `cap` and approximation order 3 (order3) or 1 (table_image.cpp
+// make_plain_tables), built and copied to HBM on first use; null when the
+// reference has none.
+void ensure_plain_tables(ReferenceContext& c, int tier, int cap, bool order3, const RungDev*& rungs) {
     std::lock_guard<std::mutex> lock(c.device_mutex);
     DeviceTables& t = *c.device;
-    if (!t.plain_built[tier][cap]) {
+    const int o = order3 ? 1 : 0;
+    if (!t.plain_built[tier][cap][o]) {
         std::vector<RungDev> r;
-        make_plain_tables(c, tier, cap, r);
+        make_plain_tables(c, tier, cap, order3, r);
         if (!r.empty()) {
             LaunchState& s = state();
-            upload_vector(r, t.rung_block[tier][cap], t.rungs[tier][cap], s.stream, t.bytes);
+            upload_vector(r, t.rung_block[tier][cap][o], t.rungs[tier][cap][o], s.stream, t.bytes);
             check(hipStreamSynchronize(s.stream), "plain table upload");
         }
-        t.plain_built[tier][cap] = true;
+        t.plain_built[tier][cap][o] = true;
     }
-    rungs = t.rungs[tier][cap];
+    rungs = t.rungs[tier][cap][o];
 }
 
 // One deep field render on the device, host part of renderer.cpp:12672-13583.
@@ -383,7 +388,7 @@ RenderParams bind_field_tables(ReferenceContext& c, const PreparedRender& prep) 
     {
         // floor(log2) of the block length limit (at least 2)
         const int cap = 31 - __builtin_clz(static_cast<unsigned>(p.max_linear_length));
-        ensure_plain_tables(c, prep.linear_tier, cap, p.rungs);
+        ensure_plain_tables(c, prep.linear_tier, cap, p.approximation_order >= 3, p.rungs);
         p.plain_top = cap;
         p.plain_pass = p.rungs != nullptr && p.fast_linear ? 1 : 0;
     }
@@ -1423,7 +1428,8 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
             prepare_render(*fields[f].context, r, preps[f]);
             ensure_device_tables(*fields[f].context);
             const int cap = 31 - __builtin_clz(static_cast<unsigned>(preps[f].params.max_linear_length));
-            ensure_plain_tables(*fields[f].context, preps[f].linear_tier, cap, rungs[f]);
+            ensure_plain_tables(*fields[f].context, preps[f].linear_tier, cap, preps[f].params.approximation_order >= 3,
+                                rungs[f]);
             offsets_total += (preps[f].offset_m.size() * 12 + 512 + 255) & ~size_t(255);
             state_total += (pixels * kStateBytesPerPixel + 20 * 256 + 255) & ~size_t(255);
         }

@@ -55,7 +55,8 @@ static_assert(sizeof(OrbitFastDev) == 64, "orbit record is one cache line");
 // The plain-double block loop (field_core.h run_plain_blocks) reads one
 // linear BLA tier through one table built from it (table_image.cpp
 // make_plain_tables): a 256-byte RungDev per block start r, so that every
-// read of a block comes from one base address.
+// read of a block comes from one base address, and the block of the common
+// case (slot 0, below) from the rung's first cache line alone.
 //
 // * r2[k]: the validity radius of the step of level top - k that starts at
 //   r, as a plain double; top is the highest level aligned at r, at most the
@@ -71,7 +72,14 @@ static_assert(sizeof(OrbitFastDev) == 64, "orbit record is one cache line");
 //   r past the level-0 steps); small_top: small_d of slot 0's end index
 //   r + 2^top (the common next index), so that a block reads no other record.
 // * zn_d: |Z_r|^2 (0 below, +inf above the normal range).
-// * step[k]: A and B of slot k's step as plain doubles (their exact values).
+// * a_re .. b_im: slot 0's A and B as plain doubles (their exact values);
+//   step_low[k - 1]: those of slot k >= 1.
+// * stride: 256 << top, the byte distance from this rung to slot 0's end.
+// * fast_r2: one bound that |delta|^2 < fast_r2 implies every loop-top test
+//   of a slot-0 block at r: |Z|^2 >= |delta|^2 (fast_r2 <= the double after
+//   zn_d), |delta|^2 below slot 0's radius, the step applicable and top >= 1
+//   (else fast_r2 = 0), and, in tables built for approximation order 3,
+//   |delta|^2 < 2^-161 (renderer.cpp:13252-13273).
 struct alignas(32) PlainStepDev {
     double a_re, a_im, b_re, b_im;
 };
@@ -79,17 +87,20 @@ static_assert(sizeof(PlainStepDev) == 32, "plain step is half a cache line");
 
 constexpr int kRungSlots = 5;
 struct alignas(256) RungDev {
-    double r2[kRungSlots];
+    double fast_r2;
     double small_top;
-    double zn_d;
+    double a_re, a_im, b_re, b_im;
+    uint32_t stride;
     int32_t top;
-    int32_t pad0;
-    PlainStepDev step[kRungSlots];
+    double zn_d;
+    double r2[kRungSlots];
     double small_d;
-    double pad1[3];
+    double pad[2];
+    PlainStepDev step_low[kRungSlots - 1];
 };
 static_assert(sizeof(RungDev) == 256, "rung is four cache lines");
-static_assert(offsetof(RungDev, step) == 64, "the block's header is one cache line");
+static_assert(offsetof(RungDev, r2) == 64, "a slot-0 block reads the first cache line only");
+static_assert(offsetof(RungDev, step_low) == 128, "steps of slots 1.. are aligned");
 
 // Device copy of one reference (orbit + tables), structure of arrays.  The
 // per-index FloatExp parts of Z, the reference escape margin |Z|^2-R^2 and

@@ -66,9 +66,9 @@ struct DeviceTables {
     void* fast_block = nullptr;
     // plain-double block loop tables per linear tier and level cap, built on
     // first use (ensure_plain_tables); null until then
-    void* rung_block[2][kMaxLevels] = {};
-    const RungDev* rungs[2][kMaxLevels] = {};
-    bool plain_built[2][kMaxLevels] = {};
+    void* rung_block[2][kMaxLevels][2] = {};
+    const RungDev* rungs[2][kMaxLevels][2] = {};
+    bool plain_built[2][kMaxLevels][2] = {};
     const AltStepDev* alt = nullptr;
     const double* alt_margin_m = nullptr;  // inside orbit_block
     const int32_t* alt_margin_e = nullptr;

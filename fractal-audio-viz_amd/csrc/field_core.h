@@ -699,54 +699,82 @@ FAV_HD PixelState init_pixel(const RenderParams& p, size_t index, SC dc, Counter
 
 enum Outcome : int { kDone = 0, kHandoff = 1, kTail = 2 };
 
-// Capacity, in blocks, of one wave's derivative log (one byte per block).
-// A test build sets a tiny capacity so that replays on a full log run often.
+// One wave's derivative log: an entry per block that a loop chose by the
+// general rules, (r << 5) | level with r the block's start index (r < 2^24:
+// the iteration cap is at most 10^7, renderer.cpp:88-93; level < 32).  The
+// plain block loop's common blocks (slot 0 of their rung, run_plain_blocks)
+// take no entry: a block that starts at r between two entries is slot 0 of
+// rung r, level rungs[r].top, so the replay walks those rungs from one entry
+// to the next.  The capacity counts entries; a test build sets a tiny one so
+// that replays on a full log run often.
 #ifndef FAV_DERIVATIVE_LOG
-#define FAV_DERIVATIVE_LOG 4096
+#define FAV_DERIVATIVE_LOG 1024
 #endif
 constexpr int kDerivativeLog = FAV_DERIVATIVE_LOG;
 
-// The derivative after `count` logged blocks from reference index `start`:
-// g <- A g + B per block (renderer.cpp:13499-13511), block k of level l at
-// index r being linear[dir.offset[l] + ((r - 1) >> l)] as the block loop
-// chose it.  The *_nn forms give the general forms' results while the
-// operand and the result are normal; otherwise the general forms run.
-FAV_HD SC replay_derivative(const RenderParams& p, const uint8_t* dlog, int count, int start, SC g) {
-    const LevelDir& dir = p.linear_dir;
-    count = wave_first(count);  // equal on every lane that shares the log
-    int r = wave_first(start);
-    for (int i = 0; i < count; ++i) {
-        const int level = wave_first(static_cast<int>(dlog[i]));
-        const LinearStepDev step = load_uniform(p.linear, dir.offset[level] + ((r - 1) >> level));
-        const SC a{step.a_re, step.a_im, step.a_e};
-        const SC b{step.b_re, step.b_im, step.b_e};
-        SC g1;
-        bool general = !wave_all(sc_normal(g));
-        if (!general) {
-            g1 = sc_add_nn(sc_mul_nn(a, g), b);
-            general = !wave_all(!sc_is_zero(g1) && g1.e <= (1 << 28) && g1.e >= -(1 << 28));
-        }
-        if (general) g1 = sc_add(sc_mul(a, g), b);
-        g = g1;
-        r += step.length;
+FAV_HD int32_t log_entry(int r, int level) { return (r << 5) | level; }
+
+// One block of the derivative update: g <- A g + B (renderer.cpp:13499-13511)
+// with the linear step of `level` at index r, linear[dir.offset[level] +
+// ((r - 1) >> level)].  The *_nn forms give the general forms' results while
+// the operand and the result are normal; otherwise the general forms run.
+// Returns the step's length.
+FAV_HD int replay_block(const RenderParams& p, int level, int r, SC& g) {
+    const LinearStepDev step = load_uniform(p.linear, p.linear_dir.offset[level] + ((r - 1) >> level));
+    const SC a{step.a_re, step.a_im, step.a_e};
+    const SC b{step.b_re, step.b_im, step.b_e};
+    SC g1;
+    bool general = !wave_all(sc_normal(g));
+    if (!general) {
+        g1 = sc_add_nn(sc_mul_nn(a, g), b);
+        general = !wave_all(!sc_is_zero(g1) && g1.e <= (1 << 28) && g1.e >= -(1 << 28));
     }
+    if (general) g1 = sc_add(sc_mul(a, g), b);
+    g = g1;
+    return step.length;
+}
+
+// The common blocks from r up to index `to`: slot 0 of each rung on the way.
+FAV_HD int replay_common(const RenderParams& p, int r, int to, SC& g) {
+    while (r < to) {
+        const int level = load_uniform(&p.rungs[r].top, 0);
+        r += replay_block(p, level, r, g);
+    }
+    return r;
+}
+
+// The derivative after the blocks from reference index `start` to `end`:
+// the `count` logged blocks in order, each preceded by the common blocks
+// that lead to its start, then the common blocks up to `end`.  Every value
+// is equal across the lanes that share the log.
+FAV_HD SC replay_derivative(const RenderParams& p, const int32_t* dlog, int count, int start, int end, SC g) {
+    count = wave_first(count);
+    int r = wave_first(start);
+    end = wave_first(end);
+    for (int i = 0; i < count; ++i) {
+        const int entry = wave_first(dlog[i]);
+        r = replay_common(p, r, entry >> 5, g);
+        r += replay_block(p, entry & 31, r, g);
+    }
+    replay_common(p, r, end, g);
     return g;
 }
 
 // Logged blocks whose derivative update is still owed: the pixel's
-// derivative is its state's derivative advanced through `count` logged
-// blocks from reference index `start`.
+// derivative is its state's derivative advanced through the blocks from
+// reference index `start` to `end`, `count` of them logged.
 struct DerivativeDebt {
     int count = 0;
     int start = 0;
+    int end = 0;
 };
 
 template <bool Planes>
-FAV_HD void settle_derivative(const RenderParams& p, const uint8_t* dlog, PixelState& s, DerivativeDebt& debt) {
+FAV_HD void settle_derivative(const RenderParams& p, const int32_t* dlog, PixelState& s, DerivativeDebt& debt) {
     if constexpr (Planes) {
-        if (debt.count > 0) {
-            s.derivative = replay_derivative(p, dlog, debt.count, debt.start, s.derivative);
-            debt.count = 0;
+        if (debt.end != debt.start) {
+            s.derivative = replay_derivative(p, dlog, debt.count, debt.start, debt.end, s.derivative);
+            debt = DerivativeDebt{};
         }
     }
 }
@@ -774,7 +802,7 @@ FAV_HD void settle_derivative(const RenderParams& p, const uint8_t* dlog, PixelS
 // replay is owed in `debt` instead: run_linear settles it only where the
 // derivative is read, and not at all for a last step that reaches the cap.
 template <bool Planes, bool Stats>
-FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Counters<Stats>& st, uint8_t* dlog,
+FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Counters<Stats>& st, int32_t* dlog,
                               DerivativeDebt& debt) {
     const OrbitDev& o = p.orbit;
     const int max_iter = p.max_iter;
@@ -850,11 +878,11 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
         delta = d1;
         if constexpr (Planes) {
             if (logged == kDerivativeLog) {
-                derivative = replay_derivative(p, dlog, logged, log_start, derivative);
+                derivative = replay_derivative(p, dlog, logged, log_start, r, derivative);
                 logged = 0;
                 log_start = r;
             }
-            dlog[logged++] = static_cast<uint8_t>(chosen);
+            dlog[logged++] = log_entry(r, chosen);
         }
         iteration += length;
         r = end;
@@ -878,8 +906,9 @@ FAV_HD void run_normal_blocks(const RenderParams& p, PixelState& s, SC dc, Count
                 if (wave_all(iteration >= max_iter - 1)) {
                     debt.count = logged;
                     debt.start = log_start;
+                    debt.end = r;
                 } else {
-                    derivative = replay_derivative(p, dlog, logged, log_start, derivative);
+                    derivative = replay_derivative(p, dlog, logged, log_start, r, derivative);
                 }
             }
         }
@@ -1001,30 +1030,71 @@ FAV_HD PlainStepDev load_step(const PlainStepDev* t) {
 }
 
 // A uniform radius slot: fits (no lane at or above it) and applicable (sign
-// bit clear), read as an integer test of the high word.
+// bit clear), read as an integer test of the high word.  On the device the
+// word goes through readfirstlane, so that the test stays a 32-bit scalar
+// compare instead of being folded into a 64-bit vector one.
 FAV_HD bool slot_plain(double t) {
-    return static_cast<int>(__builtin_bit_cast(unsigned long long, t) >> 32) >= 0;
+    const int hi = static_cast<int>(__builtin_bit_cast(unsigned long long, t) >> 32);
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_readfirstlane(hi) >= 0;
+#else
+    return hi >= 0;
+#endif
 }
 
-// The first cache line of a rung: what a block reads besides its step.
-struct RungHead {
-    double r2[kRungSlots];
-    double small_top;
-    double zn_d;
-    int32_t top;
+// What a slot-0 block reads: the first cache line of its rung.
+struct RungLine {
+    double fast_r2, small_top;
+    PlainStepDev step;
+    uint32_t stride;
 };
 
-FAV_HD RungHead load_head(const RungDev* g) {
-    RungHead h;
-    for (int k = 0; k < kRungSlots; ++k) h.r2[k] = uniform_double(load_uniform(&g->r2[k], 0));
-    h.small_top = uniform_double(load_uniform(&g->small_top, 0));
-    h.zn_d = uniform_double(load_uniform(&g->zn_d, 0));
-    h.top = load_uniform(&g->top, 0);
-    return h;
+FAV_HD RungLine load_line(const RungDev* g) {
+    return RungLine{uniform_double(load_uniform(&g->fast_r2, 0)), uniform_double(load_uniform(&g->small_top, 0)),
+                    PlainStepDev{uniform_double(load_uniform(&g->a_re, 0)), uniform_double(load_uniform(&g->a_im, 0)),
+                                 uniform_double(load_uniform(&g->b_re, 0)), uniform_double(load_uniform(&g->b_im, 0))},
+                    load_uniform(&g->stride, 0)};
+}
+
+// The step of slot k of a rung.
+FAV_HD PlainStepDev slot_step(const RungDev* g, int k) {
+    if (k == 0)
+        return PlainStepDev{uniform_double(load_uniform(&g->a_re, 0)), uniform_double(load_uniform(&g->a_im, 0)),
+                            uniform_double(load_uniform(&g->b_re, 0)), uniform_double(load_uniform(&g->b_im, 0))};
+    return load_step(&g->step_low[k - 1]);
+}
+
+// One block m' = A*m + B*c and the tests of its result, taken together:
+// both products added (exponent gap at most 60), m's bounds, and negligible
+// at the end index (the condition of the reference's loop-top values there
+// being the orbit's).
+struct PlainBlock {
+    double nr, ni, nx;
+    uint64_t added, bounded, negligible;
+};
+
+FAV_HD PlainBlock plain_block(const PlainStepDev& step, double mr, double mi, double cr, double ci, int E,
+                              double next_small) {
+    // A*delta + B*dc, each product in sc_mul's operation order
+    const double pr = step.a_re * mr - step.a_im * mi;
+    const double pi = step.a_re * mi + step.a_im * mr;
+    const double qr = step.b_re * cr - step.b_im * ci;
+    const double qi = step.b_re * ci + step.b_im * cr;
+    PlainBlock b;
+    b.nr = pr + qr;
+    b.ni = pi + qi;
+    const double pm = max_abs(pr, pi), qm = max_abs(qr, qi);
+    b.added = wave_mask(__builtin_fmax(pm, qm) <= __builtin_fmin(pm, qm) * 0x1p60);
+    b.nx = max_abs(b.nr, b.ni);
+    const double nn = min_abs(b.nr, b.ni);
+    b.bounded = wave_mask(b.nx >= kPlainLow) & wave_mask(b.nx <= kPlainHigh)
+        & (wave_mask(nn == 0.0) | wave_mask(nn >= kPlainTiny));
+    b.negligible = wave_mask(ldexp_(b.nx, E) < next_small);
+    return b;
 }
 
 template <bool Planes, bool Stats>
-FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, Counters<Stats>& st, uint8_t* dlog,
+FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, Counters<Stats>& st, int32_t* dlog,
                                   DerivativeDebt& debt) {
     FAV_PLAIN_EXIT_PROBE(14);
     const int max_iter = p.max_iter;
@@ -1046,7 +1116,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     if (full_level > p.plain_top) return kPlainNotEntered;
     double mr = ldexp_(s.delta.re, s.delta.e - E), mi = ldexp_(s.delta.im, s.delta.e - E);
     double cr = ldexp_(dc.re, dc.e - E), ci = ldexp_(dc.im, dc.e - E);
-    double mx = max_abs(mr, mi);
+    const double mx = max_abs(mr, mi);
     // delta's span from its mantissas, as for dc: 2^-220 keeps the smaller
     // component of m at or above 2^-320.  Delta must be negligible at r0
     // (delta.e <= small_exp: the larger component below 2^small_exp;
@@ -1065,12 +1135,21 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     // Order 1 (renderer.cpp:13252-13273): |delta|^2 < 2^-161 when the
     // approximation order is 3; order 2 never takes a linear block.
     const double order_bound = p.approximation_order >= 3 ? 0x1p-161 : __builtin_inf();
+    // iteration - r stays `lag`: both advance by each block's length
+    const int lag = it0 - r0;
     // below this iteration the length limit is max_length itself
     const int last_full = max_iter - max_length;
-    int r = r0, iteration = it0;
+    // r is kept as the byte offset of its rung (r < 2^24, so it fits)
+    const char* const base = reinterpret_cast<const char*>(rungs);
+    uint32_t off = static_cast<uint32_t>(r0) << 8;
+    // A common block starts at r <= r_full (iteration <= last_full: the
+    // length limit is max_length, so the lookup starts at the top level,
+    // and r <= iteration keeps the block below the cap); r >= 1 here
+    const int r_full = last_full - lag;
+    const uint32_t off_full = r_full > 0 ? static_cast<uint32_t>(r_full) << 8 : 0u;
     double sq = mr * mr + mi * mi;
     SC derivative = s.derivative;  // the derivative at log_start
-    int log_start = r, logged = 0;
+    int log_start = r0, logged = 0;
     bool committed = false;
     // The block loop runs at a fixed scale E.  A result outside m's bounds
     // ends it without a commit; the committed delta and dc are then rescaled
@@ -1082,100 +1161,104 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
         int shift = 0;
         bool rescale = false;
         for (;;) {
-            // small_d > 0 at r (checked at entry and at every commit) means a
-            // block may start here.  The rung's first line and slot 0's step
-            // (the top level is the common choice); the other waves of the CU
-            // walk the same indices.
-            const RungDev* g = rungs + r;
-            const RungHead h = load_head(g);
-            PlainStepDev step = load_step(&g->step[0]);
-            // the loop-top tests of run_normal_blocks that depend on
-            // |delta|^2: |Z|^2 >= |delta|^2 and order 1
-            const double dd = ldexp_(sq, 2 * E);
             const uint64_t active = wave_active();
-            const uint64_t top_ok = wave_mask(dd <= h.zn_d) & wave_mask(dd < order_bound);
-            const uint64_t fits0 = wave_mask(dd < fabs_(h.r2[0]));
-            int level = h.top;
-            double next_small = h.small_top;
-            // The common case, as one integer test: away from the cap (the
-            // length limit is max_length, so the lookup starts at the top
-            // level, and r <= iteration keeps the block below the cap), the
-            // top level at least 1, every lane within slot 0's radius and
-            // the step applicable.
-            const int common = static_cast<int>((top_ok & fits0) == active) & static_cast<int>(iteration <= last_full)
-                & static_cast<int>(level >= 1) & static_cast<int>(slot_plain(h.r2[0]));
-            if (!common) {
-                if (iteration >= max_iter) { FAV_PLAIN_EXIT_PROBE(2); break; }
-                if (top_ok != active) { FAV_PLAIN_EXIT_PROBE(4); break; }
-                // BLA level: the highest aligned level at or below log2 of
-                // the limit whose radius every lane's |delta|^2 is below;
-                // slots above the limit are skipped, and the slots end at
-                // level 1
-                const int top_level = h.top;
-                int limit_level = full_level;
-                if (iteration > last_full) limit_level = highest_level(max_iter - iteration);
-                const int first = top_level > limit_level ? top_level - limit_level : 0;
-                const int slots = top_level < kRungSlots ? top_level : kRungSlots;
-                int slot = -1;
-                for (int k = first; k < slots; ++k) {
-                    const uint64_t fits = wave_mask(dd < fabs_(load_uniform(&g->r2[k], 0)));
-                    if (fits == 0) continue;
-                    // else: lanes disagree, or the step is not plain
-                    if (fits == active && slot_plain(load_uniform(&g->r2[k], 0))) slot = k;
-                    break;
+            // Common blocks: slot 0 of the rung while every lane's |delta|^2
+            // is below its fast bound (every loop-top test of the block
+            // passes, fav_device.h) and r <= r_full.  They take no log entry.
+            // small_d > 0 at r (checked at entry and at every commit) means a
+            // block may start here; the other waves of the CU walk the same
+            // rungs.
+            // The next rung's line is read while this block runs (a line with
+            // fast_r2 > 0 has stride > 0 and slot 0's end index inside the
+            // table; otherwise stride = 0 and it is this line again).  Each
+            // block is formed before its tests, which are one ballot.
+            const uint32_t off_common = off;
+            RungLine line = load_line(reinterpret_cast<const RungDev*>(base + off));
+            for (;;) {
+                const uint32_t next = off + line.stride;
+                const RungLine next_line = load_line(reinterpret_cast<const RungDev*>(base + next));
+                const double dd = ldexp_(sq, 2 * E);
+                const PlainBlock b = plain_block(line.step, mr, mi, cr, ci, E, line.small_top);
+                const uint64_t ok = wave_mask(dd < line.fast_r2) & b.added & b.bounded & b.negligible
+                    & (off <= off_full ? ~0ULL : 0ULL);
+                if (ok != active) break;
+                // commit, renderer.cpp:13499-13511
+                mr = b.nr;
+                mi = b.ni;
+                sq = mr * mr + mi * mi;
+                FAV_PLAIN_BLOCK_PROBE();
+                st.add(ST_BLA_BLOCKS);
+                st.add(ST_LINEAR_BLOCKS);
+                st.add(ST_SERIES_JUMPS);
+                if constexpr (Stats) {
+                    const int level = __builtin_ctz(line.stride) - 8;
+                    st.add(ST_HISTOGRAM + (level < 15 ? level : 15));
                 }
-                if (slot < 0) { FAV_PLAIN_EXIT_PROBE(9); break; }
-                level = top_level - slot;
-                if (r + (1 << level) > max_iter) { FAV_PLAIN_EXIT_PROBE(10); break; }
-                if (slot != 0) {
-                    step = load_step(&g->step[slot]);
-                    next_small = uniform_double(load_uniform(&rungs[r + (1 << level)].small_d, 0));
-                }
+                st.add(ST_LOGICAL, static_cast<unsigned long long>(line.stride >> 8));
+                off = next;
+                line = next_line;
             }
-            const int length = 1 << level;
-            const int end = r + length;
-            // A*delta + B*dc, each product in sc_mul's operation order
-            const double pr = step.a_re * mr - step.a_im * mi;
-            const double pi = step.a_re * mi + step.a_im * mr;
-            const double qr = step.b_re * cr - step.b_im * ci;
-            const double qi = step.b_re * ci + step.b_im * cr;
-            const double nr = pr + qr, ni = pi + qi;
-            // the tests of the result, taken together: both products added
-            // (exponent gap at most 60), m's bounds, and negligible at the
-            // end index (the condition of the reference's loop-top values
-            // there being the orbit's)
-            const double pm = max_abs(pr, pi), qm = max_abs(qr, qi);
-            const uint64_t added = wave_mask(__builtin_fmax(pm, qm) <= __builtin_fmin(pm, qm) * 0x1p60);
-            const double nx = max_abs(nr, ni), nn = min_abs(nr, ni);
-            const uint64_t bounded = wave_mask(nx >= kPlainLow) & wave_mask(nx <= kPlainHigh)
-                & (wave_mask(nn == 0.0) | wave_mask(nn >= kPlainTiny));
-            const uint64_t negligible = wave_mask(ldexp_(nx, E) < next_small);
-            if ((added & bounded & negligible) != active) {
-                if (added != active) {
+            if (off != off_common) {
+                committed = true;
+                retried = false;
+            }
+            // Any other block, by the lookup of run_normal_blocks: the
+            // loop-top tests that depend on |delta|^2 (|Z|^2 >= |delta|^2 and
+            // order 1), then the highest aligned level at or below log2 of
+            // the limit whose radius every lane's |delta|^2 is below; slots
+            // above the limit are skipped, and the slots end at level 1.
+            const int r = static_cast<int>(off >> 8);
+            const int iteration = r + lag;
+            if (iteration >= max_iter) { FAV_PLAIN_EXIT_PROBE(2); break; }
+            const RungDev* g = rungs + r;
+            const double dd = ldexp_(sq, 2 * E);
+            const double zn_d = uniform_double(load_uniform(&g->zn_d, 0));
+            if ((wave_mask(dd <= zn_d) & wave_mask(dd < order_bound)) != active) { FAV_PLAIN_EXIT_PROBE(4); break; }
+            const int top_level = load_uniform(&g->top, 0);
+            int limit_level = full_level;
+            if (iteration > last_full) limit_level = highest_level(max_iter - iteration);
+            const int first = top_level > limit_level ? top_level - limit_level : 0;
+            const int slots = top_level < kRungSlots ? top_level : kRungSlots;
+            int slot = -1;
+            for (int k = first; k < slots; ++k) {
+                const double r2 = uniform_double(load_uniform(&g->r2[k], 0));
+                const uint64_t fits = wave_mask(dd < fabs_(r2));
+                if (fits == 0) continue;
+                // else: lanes disagree, or the step is not plain
+                if (fits == active && slot_plain(r2)) slot = k;
+                break;
+            }
+            if (slot < 0) { FAV_PLAIN_EXIT_PROBE(9); break; }
+            const int level = top_level - slot;
+            const int end = r + (1 << level);
+            if (end > max_iter) { FAV_PLAIN_EXIT_PROBE(10); break; }
+            const double next_small = slot == 0 ? uniform_double(load_uniform(&g->small_top, 0))
+                                                : uniform_double(load_uniform(&rungs[end].small_d, 0));
+            const PlainBlock b = plain_block(slot_step(g, slot), mr, mi, cr, ci, E, next_small);
+            if ((b.added & b.bounded & b.negligible) != active) {
+                if (b.added != active) {
                     FAV_PLAIN_EXIT_PROBE(12);
-                } else if (bounded != active) {
-                    shift = wave_first(frexp_exp_nz(nx));
+                } else if (b.bounded != active) {
+                    shift = wave_first(frexp_exp_nz(b.nx));
                     rescale = true;
                 } else {
                     FAV_PLAIN_EXIT_PROBE(11);
                 }
                 break;
             }
-            // commit, renderer.cpp:13499-13511
-            mr = nr;
-            mi = ni;
-            mx = nx;
+            // commit, renderer.cpp:13499-13511, with a log entry
+            mr = b.nr;
+            mi = b.ni;
             sq = mr * mr + mi * mi;
             if constexpr (Planes) {
                 if (logged == kDerivativeLog) {
-                    derivative = replay_derivative(p, dlog, logged, log_start, derivative);
+                    derivative = replay_derivative(p, dlog, logged, log_start, r, derivative);
                     logged = 0;
                     log_start = r;
                 }
-                dlog[logged++] = static_cast<uint8_t>(level);
+                dlog[logged++] = log_entry(r, level);
             }
-            iteration += length;
-            r = end;
+            off = static_cast<uint32_t>(end) << 8;
             committed = true;
             retried = false;
             FAV_PLAIN_BLOCK_PROBE();
@@ -1183,7 +1266,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
             st.add(ST_LINEAR_BLOCKS);
             st.add(ST_SERIES_JUMPS);
             if constexpr (Stats) st.add(ST_HISTOGRAM + (level < 15 ? level : 15));
-            st.add(ST_LOGICAL, static_cast<unsigned long long>(length));
+            st.add(ST_LOGICAL, static_cast<unsigned long long>(1 << level));
         }
         if (!rescale || retried || E + shift < -410) {
             if (rescale) FAV_PLAIN_EXIT_PROBE(13);
@@ -1200,7 +1283,6 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
         cr = tr;
         ci = ti;
         E += shift;
-        mx = max_abs(mr, mi);
         sq = mr * mr + mi * mi;
         retried = true;
     }
@@ -1210,16 +1292,19 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     // (small_d > 0, as checked for r), no rebase (|Z|^2 >= |delta|^2), order
     // 1 and no plain-double tail (|delta|^2 < 2^-90; renderer.cpp:13252-
     // 13290).  That pixel reads no derivative.
+    const int r = static_cast<int>(off >> 8);
+    const int iteration = r + lag;
     const double dd_end = ldexp_(sq, 2 * E);
     const bool interior = iteration == max_iter - 1
         && wave_all(dd_end <= load_uniform(&rungs[r].zn_d, 0) && dd_end < order_bound && dd_end < 0x1p-90);
     if constexpr (Planes) {
-        if (logged > 0 && !interior) {
+        if (r != log_start && !interior) {
             if (wave_all(iteration >= max_iter - 1)) {
                 debt.count = logged;
                 debt.start = log_start;
+                debt.end = r;
             } else {
-                derivative = replay_derivative(p, dlog, logged, log_start, derivative);
+                derivative = replay_derivative(p, dlog, logged, log_start, r, derivative);
             }
         }
     }
@@ -1230,7 +1315,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     // at r0 computes, and the previous norm stays
     const OrbitFastDev top = load_uniform(p.orbit.fast, r);
     const SN norm{top.zn_m, top.zn_e};
-    const int k = frexp_exp_nz(mx);
+    const int k = frexp_exp_nz(max_abs(mr, mi));
     s.delta = SC{ldexp_(mr, -k), ldexp_(mi, -k), E + k};
     s.derivative = derivative;
     s.iteration = iteration;
@@ -1348,7 +1433,7 @@ FAV_HD void run_reference_blocks(const RenderParams& p, PixelState& s, Counters<
 // `s` is a loop-top state to resume with run_linear, its derivative settled.
 template <bool Planes, bool Stats>
 FAV_HD bool plain_pixel(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st,
-                        uint8_t* dlog) {
+                        int32_t* dlog) {
     const int max_iter = p.max_iter;
     if (!p.fast_linear || !sc_normal(dc) || p.disable_bla || s.pixel_disable_bla) return false;
     DerivativeDebt debt;
@@ -1376,7 +1461,7 @@ FAV_HD bool plain_pixel(const RenderParams& p, size_t index, PixelState& s, SC d
 // kDerivativeLog bytes, private to the wave (unused without planes).
 template <bool Planes, bool Stats>
 FAV_HD Outcome linear_loop(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st,
-                           uint8_t* dlog, DerivativeDebt& debt) {
+                           int32_t* dlog, DerivativeDebt& debt) {
     const OrbitDev& o = p.orbit;
     const int max_iter = p.max_iter;
     const int mode = p.escape_radius_mode;
@@ -1495,7 +1580,7 @@ FAV_HD Outcome linear_loop(const RenderParams& p, size_t index, PixelState& s, S
 // kDone and reads none.
 template <bool Planes, bool Stats>
 FAV_HD Outcome run_linear(const RenderParams& p, size_t index, PixelState& s, SC dc, Counters<Stats>& st,
-                          uint8_t* dlog) {
+                          int32_t* dlog) {
     DerivativeDebt debt;
     const Outcome out = linear_loop<Planes, Stats>(p, index, s, dc, st, dlog, debt);
     if (out == kHandoff) settle_derivative<Planes>(p, dlog, s, debt);

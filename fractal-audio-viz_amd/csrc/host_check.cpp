@@ -86,7 +86,7 @@ extern "C" int fav_host_check_render(
         std::vector<RungDev> rungs;
         if (plain_blocks_enabled()) {
             p.plain_top = 31 - __builtin_clz(static_cast<unsigned>(p.max_linear_length));
-            make_plain_tables(c, prep.linear_tier, p.plain_top, rungs);
+            make_plain_tables(c, prep.linear_tier, p.plain_top, p.approximation_order >= 3, rungs);
             if (!rungs.empty()) p.rungs = rungs.data();
         }
         p.cubic = img.cubic.data();
@@ -135,7 +135,7 @@ extern "C" int fav_host_check_render(
                     st.clear();
                     const SC dc = core::pixel_dc(p, px, py);
                     core::PixelState s = core::init_pixel<Planes, Stats>(p, i, dc, st);
-                    uint8_t dlog[core::kDerivativeLog];
+                    int32_t dlog[core::kDerivativeLog];
                     if (core::plain_pixel<Planes, Stats>(p, i, s, dc, st, dlog)) p.state.iter[i] = p.max_iter;
                     else core::save_state(p.state, i, s);
                     if constexpr (Stats) st.flush(p.stats);
@@ -150,7 +150,7 @@ extern "C" int fav_host_check_render(
                 const SC dc = core::pixel_dc(p, px, py);
                 core::PixelState s = p.plain_pass ? core::load_state(p.state, i)
                                                   : core::init_pixel<Planes, Stats>(p, i, dc, st);
-                uint8_t dlog[core::kDerivativeLog];
+                int32_t dlog[core::kDerivativeLog];
                 if (core::run_linear<Planes, Stats>(p, i, s, dc, st, dlog) == core::kHandoff) {
                     core::save_state(p.state, i, s);
                     outcome[i] = 1;

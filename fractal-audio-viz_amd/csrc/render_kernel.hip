@@ -85,9 +85,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     block_pixel(p, px, py);
     const size_t index = static_cast<size_t>(py) * static_cast<size_t>(p.width) + px;
     const bool valid = px < p.width && py < p.height && (!p.point_re || index < static_cast<size_t>(p.point_count));
-    uint8_t* dlog = nullptr;
+    int32_t* dlog = nullptr;
     if constexpr (Planes) {
-        __shared__ uint8_t derivative_log[4][core::kDerivativeLog];
+        __shared__ int32_t derivative_log[4][core::kDerivativeLog];
         dlog = derivative_log[wave];
     }
     if (valid && static_cast<int64_t>(index) != p.reference_pixel) {
@@ -147,10 +147,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Planes
     if (valid && p.plain_pass && (static_cast<int64_t>(index) == p.reference_pixel || p.state.iter[index] >= p.max_iter))
         valid = false;
     bool handoff = false;
-    uint8_t* dlog = nullptr;
+    int32_t* dlog = nullptr;
     if constexpr (Planes) {
         // one derivative log per wave, 16 KiB for the block's four waves
-        __shared__ uint8_t derivative_log[4][core::kDerivativeLog];
+        __shared__ int32_t derivative_log[4][core::kDerivativeLog];
         dlog = derivative_log[wave];
     }
     if (valid) {

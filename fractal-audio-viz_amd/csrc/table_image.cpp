@@ -104,7 +104,7 @@ void make_table_image(const ReferenceContext& c, TableImage& img) {
 
 // The rung table of one linear tier for the level cap top_cap
 // (fav_device.h).
-void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, std::vector<RungDev>& rungs) {
+void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, bool order3, std::vector<RungDev>& rungs) {
     const OrbitData& o = *c.orbit;
     const size_t n = o.scaled.size();
     rungs.clear();
@@ -141,13 +141,30 @@ void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, std::ve
             const bool plain = s.length == (1 << level) && plain_operand(s.a_re, s.a_im, s.a_e)
                 && plain_operand(s.b_re, s.b_im, s.b_e) && end < n && (o.flags[end] & kOrbitEndpointFlag) == 0;
             if (plain) {
-                g.step[k] = PlainStepDev{std::ldexp(s.a_re, s.a_e), std::ldexp(s.a_im, s.a_e),
-                                         std::ldexp(s.b_re, s.b_e), std::ldexp(s.b_im, s.b_e)};
+                const PlainStepDev v{std::ldexp(s.a_re, s.a_e), std::ldexp(s.a_im, s.a_e),
+                                     std::ldexp(s.b_re, s.b_e), std::ldexp(s.b_im, s.b_e)};
+                if (k == 0) {
+                    g.a_re = v.a_re;
+                    g.a_im = v.a_im;
+                    g.b_re = v.b_re;
+                    g.b_im = v.b_im;
+                } else {
+                    g.step_low[k - 1] = v;
+                }
             } else {
                 r2 = -r2;  // sign bit: fitting this step ends the plain loop
                 if (r2 == 0.0) r2 = -0.0;
             }
             g.r2[k] = r2;
+        }
+        // the common block's bound (fav_device.h); 256 << top must fit the
+        // 32-bit stride
+        if (top >= 1 && top <= 23 && !std::signbit(g.r2[0]) && g.r2[0] > 0.0) {
+            double fast = g.r2[0];
+            if (g.zn_d < HUGE_VAL) fast = std::min(fast, std::nextafter(g.zn_d, HUGE_VAL));
+            if (order3) fast = std::min(fast, 0x1p-161);
+            g.fast_r2 = fast;
+            g.stride = 256u << top;
         }
     }
     // slot 0's end index, r + 2^top, once every small_d is set

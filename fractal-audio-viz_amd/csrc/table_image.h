@@ -28,9 +28,11 @@ struct TableImage {
 void make_table_image(const ReferenceContext& context, TableImage& image);
 
 // The plain-double block loop's rung table (fav_device.h) of linear tier
-// `tier`, with rungs starting at level min(aligned level, top_cap); empty
-// when the orbit has no per-index records.
-void make_plain_tables(const ReferenceContext& context, int tier, int top_cap, std::vector<RungDev>& rungs);
+// `tier`, with rungs starting at level min(aligned level, top_cap), for
+// approximation order 3 (order3) or 1; empty when the orbit has no
+// per-index records.
+void make_plain_tables(const ReferenceContext& context, int tier, int top_cap, bool order3,
+                       std::vector<RungDev>& rungs);
 
 // Per-render parameters: everything in RenderParams except the table, output
 // and scratch pointers, plus the per-column / per-row FloatExp offsets
