@@ -56,7 +56,7 @@ static_assert(sizeof(OrbitFastDev) == 64, "orbit record is one cache line");
 // linear BLA tier through one table built from it (table_image.cpp
 // make_plain_tables): a 256-byte RungDev per block start r, so that every
 // read of a block comes from one base address, and the block of the common
-// case (slot 0, below) from the rung's first cache line alone.
+// case (slot 0, below) from the rung's first two cache lines.
 //
 // * r2[k]: the validity radius of the step of level top - k that starts at
 //   r, as a plain double; top is the highest level aligned at r, at most the
@@ -69,17 +69,23 @@ static_assert(sizeof(OrbitFastDev) == 64, "orbit record is one cache line");
 // * small_d: 2^small_exp[r], the negligible-delta bound at r, or 0 where the
 //   loop must stop at r (the glitch or escape test fires for a negligible
 //   delta, r is at or past the reference's escape, or no block starts at r:
-//   r past the level-0 steps); small_top: small_d of slot 0's end index
-//   r + 2^top (the common next index), so that a block reads no other record.
+//   r past the level-0 steps).
 // * zn_d: |Z_r|^2 (0 below, +inf above the normal range).
 // * a_re .. b_im: slot 0's A and B as plain doubles (their exact values);
 //   step_low[k - 1]: those of slot k >= 1.
 // * stride: 256 << top, the byte distance from this rung to slot 0's end.
-// * fast_r2: one bound that |delta|^2 < fast_r2 implies every loop-top test
-//   of a slot-0 block at r: |Z|^2 >= |delta|^2 (fast_r2 <= the double after
-//   zn_d), |delta|^2 below slot 0's radius, the step applicable and top >= 1
-//   (else fast_r2 = 0), and, in tables built for approximation order 3,
+//
+// The slot-0 loop tests a block through bounds that imply the general
+// tests:
+// * fast_r2: |delta|^2 < fast_r2 implies every loop-top test of a slot-0
+//   block at r: |Z|^2 >= |delta|^2 (fast_r2 <= the double after zn_d),
+//   |delta|^2 below slot 0's radius, the step applicable and top >= 1 (else
+//   fast_r2 = 0), and, in tables built for approximation order 3,
 //   |delta|^2 < 2^-161 (renderer.cpp:13252-13273).
+// * small2: (small_d of slot 0's end index)^2 * (1 - 2^-48): |delta'|^2
+//   below it puts the larger component of delta' below that small_d.
+// * gap_lo, gap_hi: |m|^2 / |c|^2 within them puts |A m| / |B c| within
+//   2^-58 .. 2^58 (0 and 0 when slot 0 is not applicable).
 struct alignas(32) PlainStepDev {
     double a_re, a_im, b_re, b_im;
 };
@@ -88,18 +94,18 @@ static_assert(sizeof(PlainStepDev) == 32, "plain step is half a cache line");
 constexpr int kRungSlots = 5;
 struct alignas(256) RungDev {
     double fast_r2;
-    double small_top;
+    double small2;
     double a_re, a_im, b_re, b_im;
     uint32_t stride;
     int32_t top;
     double zn_d;
+    double gap_lo, gap_hi;
     double r2[kRungSlots];
     double small_d;
-    double pad[2];
     PlainStepDev step_low[kRungSlots - 1];
 };
 static_assert(sizeof(RungDev) == 256, "rung is four cache lines");
-static_assert(offsetof(RungDev, r2) == 64, "a slot-0 block reads the first cache line only");
+static_assert(offsetof(RungDev, gap_lo) == 64, "a slot-0 block reads the first two cache lines");
 static_assert(offsetof(RungDev, step_low) == 128, "steps of slots 1.. are aligned");
 
 // Device copy of one reference (orbit + tables), structure of arrays.  The

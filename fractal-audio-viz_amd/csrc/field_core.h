@@ -1044,16 +1044,29 @@ FAV_HD bool slot_plain(double t) {
 
 // What a slot-0 block reads: the first cache line of its rung.
 struct RungLine {
-    double fast_r2, small_top;
+    double fast_r2, small2;
     PlainStepDev step;
     uint32_t stride;
+    double gap_lo, gap_hi;
 };
 
 FAV_HD RungLine load_line(const RungDev* g) {
-    return RungLine{uniform_double(load_uniform(&g->fast_r2, 0)), uniform_double(load_uniform(&g->small_top, 0)),
+    return RungLine{uniform_double(load_uniform(&g->fast_r2, 0)), uniform_double(load_uniform(&g->small2, 0)),
                     PlainStepDev{uniform_double(load_uniform(&g->a_re, 0)), uniform_double(load_uniform(&g->a_im, 0)),
                                  uniform_double(load_uniform(&g->b_re, 0)), uniform_double(load_uniform(&g->b_im, 0))},
-                    load_uniform(&g->stride, 0)};
+                    load_uniform(&g->stride, 0), uniform_double(load_uniform(&g->gap_lo, 0)),
+                    uniform_double(load_uniform(&g->gap_hi, 0))};
+}
+
+// Marks a scalar value as needed here, so that the load that produces it is
+// issued with the loads before it rather than sunk to its first use (an
+// empty asm statement: no instruction is emitted).
+FAV_HD void issue_here(uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __asm__ volatile("" : : "s"(v));
+#else
+    (void)v;
+#endif
 }
 
 // The step of slot k of a rung.
@@ -1091,6 +1104,58 @@ FAV_HD PlainBlock plain_block(const PlainStepDev& step, double mr, double mi, do
         & (wave_mask(nn == 0.0) | wave_mask(nn >= kPlainTiny));
     b.negligible = wave_mask(ldexp_(b.nx, E) < next_small);
     return b;
+}
+
+// A slot-0 block and its tests as one ballot, through the rung's bounds
+// (fav_device.h), each of which implies the corresponding general test of
+// plain_block or of the loop top (so a block this accepts is the block the
+// general rules commit, and one it rejects is decided by them):
+//  * |delta|^2 < fast_r2: the loop-top tests and slot 0 chosen;
+//  * gap_lo <= |m|^2 / |c|^2 <= gap_hi, computed as |delta|^2 * kappa with
+//    kappa = 2^-2E / |c|^2: |A m| / |B c| within 2^+-58 up to rounding, so
+//    the larger components are within 2^59 < 2^60;
+//  * both components of m' at least 2^-99 (so the larger one is at least
+//    2^-100 and the smaller one at least 2^-320; a component closer to
+//    zero is left to the general test) and |m'|^2 <= 2^199 (the larger
+//    component at most 2^100);
+//  * |delta'|^2 < small2 = small^2 (1 - 2^-48): the larger component of
+//    delta' below small (|m'|^2 carries a relative error below 2^-50).
+// |delta'|^2 = |m'|^2 * 2^2E is the next block's |delta|^2.
+struct CommonBlock {
+    double nr, ni, dd;
+    uint64_t ok;
+};
+
+FAV_HD CommonBlock common_block(const RungLine& line, double mr, double mi, double dd, double cr, double ci,
+                                double kappa, int E) {
+    const double h = dd * kappa;
+    const double pr = line.step.a_re * mr - line.step.a_im * mi;
+    const double pi = line.step.a_re * mi + line.step.a_im * mr;
+    const double qr = line.step.b_re * cr - line.step.b_im * ci;
+    const double qi = line.step.b_re * ci + line.step.b_im * cr;
+    CommonBlock b;
+    b.nr = pr + qr;
+    b.ni = pi + qi;
+    const double sq = b.nr * b.nr + b.ni * b.ni;
+    b.dd = ldexp_(sq, 2 * E);
+    const double nn = min_abs(b.nr, b.ni);
+    b.ok = wave_mask(dd < line.fast_r2) & wave_mask(h >= line.gap_lo) & wave_mask(h <= line.gap_hi)
+        & wave_mask(nn >= 0x1p-99) & wave_mask(sq <= 0x1p199) & wave_mask(b.dd < line.small2);
+    return b;
+}
+
+// The counters of a committed slot-0 block of byte stride `stride`.
+template <bool Stats>
+FAV_HD void plain_common_stats(Counters<Stats>& st, uint32_t stride) {
+    FAV_PLAIN_BLOCK_PROBE();
+    st.add(ST_BLA_BLOCKS);
+    st.add(ST_LINEAR_BLOCKS);
+    st.add(ST_SERIES_JUMPS);
+    if constexpr (Stats) {
+        const int level = __builtin_ctz(stride) - 8;
+        st.add(ST_HISTOGRAM + (level < 15 ? level : 15));
+    }
+    st.add(ST_LOGICAL, static_cast<unsigned long long>(stride >> 8));
 }
 
 template <bool Planes, bool Stats>
@@ -1135,6 +1200,8 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     // Order 1 (renderer.cpp:13252-13273): |delta|^2 < 2^-161 when the
     // approximation order is 3; order 2 never takes a linear block.
     const double order_bound = p.approximation_order >= 3 ? 0x1p-161 : __builtin_inf();
+    // 1 / |c|^2, for the slot-0 blocks' product-gap test
+    double inv_cc = 1.0 / (cr * cr + ci * ci);
     // iteration - r stays `lag`: both advance by each block's length
     const int lag = it0 - r0;
     // below this iteration the length limit is max_length itself
@@ -1168,39 +1235,28 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
             // small_d > 0 at r (checked at entry and at every commit) means a
             // block may start here; the other waves of the CU walk the same
             // rungs.
-            // The next rung's line is read while this block runs (a line with
-            // fast_r2 > 0 has stride > 0 and slot 0's end index inside the
-            // table; otherwise stride = 0 and it is this line again).  Each
-            // block is formed before its tests, which are one ballot.
+            // Slot-0 blocks (common_block), one rung read per block.
             const uint32_t off_common = off;
-            RungLine line = load_line(reinterpret_cast<const RungDev*>(base + off));
-            for (;;) {
-                const uint32_t next = off + line.stride;
-                const RungLine next_line = load_line(reinterpret_cast<const RungDev*>(base + next));
-                const double dd = ldexp_(sq, 2 * E);
-                const PlainBlock b = plain_block(line.step, mr, mi, cr, ci, E, line.small_top);
-                const uint64_t ok = wave_mask(dd < line.fast_r2) & b.added & b.bounded & b.negligible
-                    & (off <= off_full ? ~0ULL : 0ULL);
-                if (ok != active) break;
-                // commit, renderer.cpp:13499-13511
-                mr = b.nr;
-                mi = b.ni;
-                sq = mr * mr + mi * mi;
-                FAV_PLAIN_BLOCK_PROBE();
-                st.add(ST_BLA_BLOCKS);
-                st.add(ST_LINEAR_BLOCKS);
-                st.add(ST_SERIES_JUMPS);
-                if constexpr (Stats) {
-                    const int level = __builtin_ctz(line.stride) - 8;
-                    st.add(ST_HISTOGRAM + (level < 15 ? level : 15));
+            {
+                const double kappa = ldexp_(inv_cc, -2 * E);
+                double dd_c = ldexp_(sq, 2 * E);
+                for (;;) {
+                    const RungLine line = load_line(reinterpret_cast<const RungDev*>(base + off));
+                    issue_here(line.stride);
+                    const CommonBlock b = common_block(line, mr, mi, dd_c, cr, ci, kappa, E);
+                    if ((b.ok & (off <= off_full ? ~0ULL : 0ULL)) != active) break;
+                    // commit, renderer.cpp:13499-13511
+                    mr = b.nr;
+                    mi = b.ni;
+                    dd_c = b.dd;
+                    plain_common_stats(st, line.stride);
+                    off += line.stride;
                 }
-                st.add(ST_LOGICAL, static_cast<unsigned long long>(line.stride >> 8));
-                off = next;
-                line = next_line;
             }
             if (off != off_common) {
                 committed = true;
                 retried = false;
+                sq = mr * mr + mi * mi;
             }
             // Any other block, by the lookup of run_normal_blocks: the
             // loop-top tests that depend on |delta|^2 (|Z|^2 >= |delta|^2 and
@@ -1232,8 +1288,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
             const int level = top_level - slot;
             const int end = r + (1 << level);
             if (end > max_iter) { FAV_PLAIN_EXIT_PROBE(10); break; }
-            const double next_small = slot == 0 ? uniform_double(load_uniform(&g->small_top, 0))
-                                                : uniform_double(load_uniform(&rungs[end].small_d, 0));
+            const double next_small = uniform_double(load_uniform(&rungs[end].small_d, 0));
             const PlainBlock b = plain_block(slot_step(g, slot), mr, mi, cr, ci, E, next_small);
             if ((b.added & b.bounded & b.negligible) != active) {
                 if (b.added != active) {
@@ -1284,6 +1339,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
         ci = ti;
         E += shift;
         sq = mr * mr + mi * mi;
+        inv_cc = 1.0 / (cr * cr + ci * ci);
         retried = true;
     }
     // At one iteration below the cap no block fits (lengths are at least 2),

@@ -157,7 +157,7 @@ void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, bool or
             }
             g.r2[k] = r2;
         }
-        // the common block's bound (fav_device.h); 256 << top must fit the
+        // the slot-0 loop's bounds (fav_device.h); 256 << top must fit the
         // 32-bit stride
         if (top >= 1 && top <= 23 && !std::signbit(g.r2[0]) && g.r2[0] > 0.0) {
             double fast = g.r2[0];
@@ -165,12 +165,23 @@ void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, bool or
             if (order3) fast = std::min(fast, 0x1p-161);
             g.fast_r2 = fast;
             g.stride = 256u << top;
+            // |A m|^2 / |B c|^2 = |A|^2/|B|^2 * |m|^2/|c|^2 within 2^-116 ..
+            // 2^116 (long double: no overflow); the bounds are kept inside
+            // 2^-1000 .. 2^1000, which only narrows them
+            const long double a2 = static_cast<long double>(g.a_re) * g.a_re + static_cast<long double>(g.a_im) * g.a_im;
+            const long double b2 = static_cast<long double>(g.b_re) * g.b_re + static_cast<long double>(g.b_im) * g.b_im;
+            if (a2 > 0 && b2 > 0) {
+                const long double ratio = b2 / a2;
+                g.gap_lo = static_cast<double>(std::max(ratio * 0x1p-116L, 0x1p-1000L));
+                g.gap_hi = static_cast<double>(std::min(ratio * 0x1p116L, 0x1p1000L));
+            }
         }
     }
-    // slot 0's end index, r + 2^top, once every small_d is set
+    // the square of slot 0's end bound, once every small_d is set
     for (size_t r = 1; r < n; ++r) {
         const size_t end = r + (size_t(1) << rungs[r].top);
-        rungs[r].small_top = end < n ? rungs[end].small_d : 0.0;
+        const double small = end < n ? rungs[end].small_d : 0.0;
+        rungs[r].small2 = small * small * (1.0 - 0x1p-48);
     }
 }
 
