@@ -1240,17 +1240,29 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
             {
                 const double kappa = ldexp_(inv_cc, -2 * E);
                 double dd_c = ldexp_(sq, 2 * E);
+                // unrolled twice, so that m alternates between two sets of
+                // registers instead of being copied at each commit
                 for (;;) {
                     const RungLine line = load_line(reinterpret_cast<const RungDev*>(base + off));
                     issue_here(line.stride);
                     const CommonBlock b = common_block(line, mr, mi, dd_c, cr, ci, kappa, E);
                     if ((b.ok & (off <= off_full ? ~0ULL : 0ULL)) != active) break;
                     // commit, renderer.cpp:13499-13511
-                    mr = b.nr;
-                    mi = b.ni;
-                    dd_c = b.dd;
                     plain_common_stats(st, line.stride);
                     off += line.stride;
+                    const RungLine line2 = load_line(reinterpret_cast<const RungDev*>(base + off));
+                    issue_here(line2.stride);
+                    const CommonBlock b2 = common_block(line2, b.nr, b.ni, b.dd, cr, ci, kappa, E);
+                    if ((b2.ok & (off <= off_full ? ~0ULL : 0ULL)) != active) {
+                        mr = b.nr;
+                        mi = b.ni;
+                        break;
+                    }
+                    mr = b2.nr;
+                    mi = b2.ni;
+                    dd_c = b2.dd;
+                    plain_common_stats(st, line2.stride);
+                    off += line2.stride;
                 }
             }
             if (off != off_common) {
