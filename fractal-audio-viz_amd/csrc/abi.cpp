@@ -1468,8 +1468,9 @@ int fractal_hip_last_timing(double* values, int capacity) {
     values[0] = t.kernel_ms;
     values[1] = t.total_ms;
     values[2] = t.pixels;
+    if (capacity >= 4) values[3] = t.plain_ms;
     set_error("");
-    return 3;
+    return capacity >= 4 ? 4 : 3;
 }
 
 int fractal_hip_reference_upload(void* handle) {

@@ -31,7 +31,8 @@
 namespace fav {
 
 hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, int persistent_blocks,
-                               hipStream_t stream, hipStream_t side, hipEvent_t fork, hipEvent_t join);
+                               hipStream_t stream, hipStream_t side, hipEvent_t fork, hipEvent_t join,
+                               hipEvent_t plain_start = nullptr, hipEvent_t plain_stop = nullptr);
 hipError_t launch_kfp_kernel(const kfp::KfpParams& params, void* scratch, int compute_units, hipStream_t stream);
 hipError_t kfp_noalloc_scratch(int device, unsigned char** out, size_t* lut_offset, size_t* bytes);
 hipError_t launch_direct_kernel(const direct::DirectParams& p, bool planes, hipStream_t stream);
@@ -84,6 +85,7 @@ struct LaunchState {
     hipStream_t side = nullptr;  // default priority, like `stream`
     hipEvent_t fork = nullptr, join = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
+    hipEvent_t plain_start = nullptr, plain_stop = nullptr;  // around the plain pass
     hipEvent_t mark[3] = {nullptr, nullptr, nullptr};
     DeviceBuffer offsets, output, planes, stats, budget, state, kfp_field, kfp_lut, kfp_rgb, kfp_scratch, direct_coords, fx_counts, tile_order, work,
         mlp_y, mlp_act, mlp_delta, mlp_err2, mlp_partial, mlp_grad, mlp_features, mlp_weights, preview_offsets,
@@ -111,6 +113,8 @@ void init_locked(LaunchState& s) {
     check(hipEventCreateWithFlags(&s.join, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreate(&s.start), "hipEventCreate");
     check(hipEventCreate(&s.stop), "hipEventCreate");
+    check(hipEventCreate(&s.plain_start), "hipEventCreate");
+    check(hipEventCreate(&s.plain_stop), "hipEventCreate");
     for (hipEvent_t& e : s.mark) check(hipEventCreate(&e), "hipEventCreate");
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s.device) != hipSuccess || khz <= 0)
@@ -495,7 +499,8 @@ void render_field_locked(const FieldRequest& r, const PreparedRender& prep, Rend
 
     const uint64_t wall0 = now_ns();
     check(hipEventRecord(s.start, s.stream), "event");
-    check(launch_field_kernel(p, want_planes, counters, s.compute_units * 8, s.stream, s.side, s.fork, s.join),
+    check(launch_field_kernel(p, want_planes, counters, s.compute_units * 8, s.stream, s.side, s.fork, s.join,
+                              s.plain_start, s.plain_stop),
           "field kernel launch");
     check(hipEventRecord(s.stop, s.stream), "event");
     if (!r.device_pointers) {
@@ -520,6 +525,12 @@ void render_field_locked(const FieldRequest& r, const PreparedRender& prep, Rend
     s.timing.kernel_ms = kernel_ms;
     s.timing.total_ms = static_cast<double>(now_ns() - wall0) * 1e-6;
     s.timing.pixels = static_cast<double>(pixels);
+    s.timing.plain_ms = 0.0;
+    if (p.plain_pass) {
+        float plain_ms = 0.0f;
+        check(hipEventElapsedTime(&plain_ms, s.plain_start, s.plain_stop), "event time");
+        s.timing.plain_ms = plain_ms;
+    }
     if (r.stats) r.stats[22] = static_cast<uint64_t>(static_cast<double>(kernel_ms) * 1e6);  // render_ns
 }
 

@@ -45,6 +45,7 @@ struct LastTiming {
     double kernel_ms;   // HIP-event time of the field kernel on the library stream
     double total_ms;    // wall time from launch to results on the host
     double pixels;
+    double plain_ms;    // HIP-event time of the plain pass alone (0 when it did not run)
 };
 
 void render_field(ReferenceContext& context, const FieldRequest& request);

@@ -237,7 +237,7 @@ __global__ void prologue_kernel(unsigned* counters, unsigned long long* budget_s
 
 template <bool Planes, bool Stats>
 hipError_t launch_all(const RenderParams& p, int persistent_blocks, hipStream_t stream, hipStream_t side,
-                      hipEvent_t fork, hipEvent_t join) {
+                      hipEvent_t fork, hipEvent_t join, hipEvent_t plain_start, hipEvent_t plain_stop) {
     prologue_kernel<<<1, 64, 0, stream>>>(p.state.counters, const_cast<unsigned long long*>(p.budget_start),
                                           p.budget_flag);
     const dim3 tiles((p.width + 15) / 16, (p.height + 15) / 16);
@@ -251,7 +251,11 @@ hipError_t launch_all(const RenderParams& p, int persistent_blocks, hipStream_t 
         if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
     }
     const dim3 grid = p.tile_order ? dim3(4 * static_cast<unsigned>(p.tile_count)) : tiles;
-    if (p.plain_pass) plain_kernel<Planes, Stats><<<grid, dim3(256), 0, stream>>>(p);
+    if (p.plain_pass) {
+        if (plain_start && (e = hipEventRecord(plain_start, stream)) != hipSuccess) return e;
+        plain_kernel<Planes, Stats><<<grid, dim3(256), 0, stream>>>(p);
+        if (plain_stop && (e = hipEventRecord(plain_stop, stream)) != hipSuccess) return e;
+    }
     fast_kernel<Planes, Stats><<<grid, dim3(256), 0, stream>>>(p);
     if (side_pixel && (e = hipStreamWaitEvent(stream, join, 0)) != hipSuccess) return e;
     general_kernel<Planes, Stats><<<dim3(persistent_blocks), dim3(256), 0, stream>>>(p);
@@ -266,16 +270,18 @@ hipError_t launch_all(const RenderParams& p, int persistent_blocks, hipStream_t 
 // kernel through the events `fork` and `join`).  The caller validated the
 // shapes: width*height <= 1e8 and every pointer in `p` (outputs, planes,
 // state store) sized for width x height.  `persistent_blocks` is the grid of
-// the queue-driven kernels.
+// the queue-driven kernels.  With `plain_start` and `plain_stop` (may be
+// null) the plain pass is bracketed by those events on `stream`.
 hipError_t launch_field_kernel(const RenderParams& p, bool planes, bool stats, int persistent_blocks,
-                               hipStream_t stream, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+                               hipStream_t stream, hipStream_t side, hipEvent_t fork, hipEvent_t join,
+                               hipEvent_t plain_start, hipEvent_t plain_stop) {
     hipError_t e;
     if (planes) {
-        e = stats ? launch_all<true, true>(p, persistent_blocks, stream, side, fork, join)
-                  : launch_all<true, false>(p, persistent_blocks, stream, side, fork, join);
+        e = stats ? launch_all<true, true>(p, persistent_blocks, stream, side, fork, join, plain_start, plain_stop)
+                  : launch_all<true, false>(p, persistent_blocks, stream, side, fork, join, plain_start, plain_stop);
     } else {
-        e = stats ? launch_all<false, true>(p, persistent_blocks, stream, side, fork, join)
-                  : launch_all<false, false>(p, persistent_blocks, stream, side, fork, join);
+        e = stats ? launch_all<false, true>(p, persistent_blocks, stream, side, fork, join, plain_start, plain_stop)
+                  : launch_all<false, false>(p, persistent_blocks, stream, side, fork, join, plain_start, plain_stop);
     }
     return e != hipSuccess ? e : hipGetLastError();
 }

@@ -511,7 +511,8 @@ def colourise_kfp(field: np.ndarray, max_iter: int, options: NativeKfpOptions, l
 
 
 def last_timing() -> dict:
-    """HIP-event kernel time and wall time of the most recent render."""
-    values = (ctypes.c_double * 3)()
-    load_library().fractal_hip_last_timing(values, 3)
-    return {"kernel_ms": values[0], "total_ms": values[1], "pixels": values[2]}
+    """HIP-event kernel time and wall time of the most recent render, and the
+    plain pass's own HIP-event time (0 when it did not run)."""
+    values = (ctypes.c_double * 4)()
+    load_library().fractal_hip_last_timing(values, 4)
+    return {"kernel_ms": values[0], "total_ms": values[1], "pixels": values[2], "plain_ms": values[3]}

@@ -713,8 +713,10 @@ int fractal_hip_render_reference_device(
 
 /* Timing of the most recent field launch: values[0] = field kernel time in
  * ms from HIP events on the library stream, values[1] = wall time of the
- * call in ms (launch to results on the host), values[2] = pixels.
- * Returns 3, or -1 if capacity < 3. */
+ * call in ms (launch to results on the host), values[2] = pixels, and, when
+ * capacity >= 4, values[3] = the plain pass's own kernel time in ms from
+ * HIP events around it on the same stream (0 when it did not run).
+ * Returns the number of values written (3 or 4), or -1 if capacity < 3. */
 int fractal_hip_last_timing(double *values, int capacity);
 
 /* Copies a reference's orbit and BLA tables into HBM now instead of at its
