@@ -81,7 +81,9 @@ static_assert(sizeof(OrbitFastDev) == 64, "orbit record is one cache line");
 //   block at r: |Z|^2 >= |delta|^2 (fast_r2 <= the double after zn_d),
 //   |delta|^2 below slot 0's radius, the step applicable and top >= 1 (else
 //   fast_r2 = 0), and, in tables built for approximation order 3,
-//   |delta|^2 < 2^-161 (renderer.cpp:13252-13273).
+//   |delta|^2 < 2^-161 (renderer.cpp:13252-13273); all times (1 - 2^-48),
+//   so that a |delta|^2 within 2^-50 of the reference's rounding of it
+//   (the slot-0 loop forms it with one FMA) passes only when that does.
 // * small2: (small_d of slot 0's end index)^2 * (1 - 2^-48): |delta'|^2
 //   below it puts the larger component of delta' below that small_d.
 // * gap_lo, gap_hi: |m|^2 / |c|^2 within them puts |A m| / |B c| within

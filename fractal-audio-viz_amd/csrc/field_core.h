@@ -1120,7 +1120,10 @@ FAV_HD PlainBlock plain_block(const PlainStepDev& step, double mr, double mi, do
 //    component at most 2^100);
 //  * |delta'|^2 < small2 = small^2 (1 - 2^-48): the larger component of
 //    delta' below small (|m'|^2 carries a relative error below 2^-50).
-// |delta'|^2 = |m'|^2 * 2^2E is the next block's |delta|^2.
+// |delta'|^2 = |m'|^2 * 2^2E is the next block's |delta|^2.  |m'|^2 is
+// formed as fma(nr, nr, ni * ni): within 2^-50 of the reference's
+// nr * nr + ni * ni, which the (1 - 2^-48) factors of fast_r2 and small2
+// absorb; the committed m' itself is the reference's value.
 struct CommonBlock {
     double nr, ni, dd;
     uint64_t ok;
@@ -1136,7 +1139,7 @@ FAV_HD CommonBlock common_block(const RungLine& line, double mr, double mi, doub
     CommonBlock b;
     b.nr = pr + qr;
     b.ni = pi + qi;
-    const double sq = b.nr * b.nr + b.ni * b.ni;
+    const double sq = __builtin_fma(b.nr, b.nr, b.ni * b.ni);
     b.dd = ldexp_(sq, 2 * E);
     const double nn = min_abs(b.nr, b.ni);
     b.ok = wave_mask(dd < line.fast_r2) & wave_mask(h >= line.gap_lo) & wave_mask(h <= line.gap_hi)

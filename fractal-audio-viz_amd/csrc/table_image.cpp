@@ -163,7 +163,8 @@ void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, bool or
             double fast = g.r2[0];
             if (g.zn_d < HUGE_VAL) fast = std::min(fast, std::nextafter(g.zn_d, HUGE_VAL));
             if (order3) fast = std::min(fast, 0x1p-161);
-            g.fast_r2 = fast;
+            // room for the slot-0 loop's |delta|^2, formed with one FMA
+            g.fast_r2 = fast * (1.0 - 0x1p-48);
             g.stride = 256u << top;
             // |A m|^2 / |B c|^2 = |A|^2/|B|^2 * |m|^2/|c|^2 within 2^-116 ..
             // 2^116 (long double: no overflow); the bounds are kept inside
