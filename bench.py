@@ -654,6 +654,8 @@ def main():
     parser.add_argument("--combined-only", action="store_true",
                         help="run only the floatexp precision fields and the combined leg (with a trained "
                              "cost model) and print their JSON")
+    parser.add_argument("--boundary-only", action="store_true",
+                        help="run only the boundary-rich Mandelbrot fields and print their JSON")
     parser.add_argument("--copy-calibration-mib", type=int, default=0,
                         help="only run N device-to-device copies of this many MiB (HBM counter calibration)")
     args = parser.parse_args()
@@ -675,6 +677,11 @@ def main():
         return 0
     if args.batch_only:
         print(json.dumps({"batched": batch_leg(lib, hip, 2)}))
+        return 0
+    if args.boundary_only:
+        lut = fav_palette.kfp_palette_lut(fav_palette.seeded_stops(12, 2024), 1024)
+        print(json.dumps({"boundary_fields": boundary_leg(lib, hip, fav_hip.NativeKfpOptions.kalles_default(),
+                                                          lut, args.steps)}))
         return 0
     if args.combined_only:
         model, _ = fav_sched.train_model(fav_sched.mandelbrot_training_fields() + fav_sched.training_fields(),

@@ -220,21 +220,6 @@ FAV_HD void store_escape(const PlaneOut& p, size_t i, int iteration, SC total, S
     p.test2[i] = sn_to_double(test2);
 }
 
-// store_escape kept out of line in the tail loop (its atan2 / log code would
-// otherwise raise the register count of the whole loop).
-#if defined(__HIP_DEVICE_COMPILE__)
-__device__ __attribute__((noinline)) void store_escape_outlined(const PlaneOut& p, size_t i, int iteration,
-                                                                SC total, SN test1, SN test2, SC derivative,
-                                                                FE spacing) {
-    store_escape(p, i, iteration, total, test1, test2, derivative, spacing);
-}
-#else
-inline void store_escape_outlined(const PlaneOut& p, size_t i, int iteration, SC total, SN test1, SN test2,
-                                  SC derivative, FE spacing) {
-    store_escape(p, i, iteration, total, test1, test2, derivative, spacing);
-}
-#endif
-
 struct Budget {  // renderer.cpp:11872-11890
     const RenderParams* p;
     unsigned ticks;
@@ -385,6 +370,9 @@ FAV_HD TailParams tail_params(const RenderParams& p) {
 // double_tail's loop variables.
 struct TailState {
     double dc_re, dc_im, d_re, d_im;
+    // Z at ref and at ref + 1, carried from step to step: each step loads
+    // Z at ref + 2, so that the load has the whole step to arrive
+    double z0_re, z0_im, z1_re, z1_im;
     double tortoise_re, tortoise_im;
     SN previous_norm;
     int cycle_power, cycle_length, tail_steps, tail_iterations, tail_rebases;
@@ -414,7 +402,20 @@ FAV_HD bool tail_continues(const TailParams& q, int iteration, int ref) {
     return iteration < q.max_iter && ref >= 0 && ref + 1 < q.orbit_size;
 }
 
-enum TailStatus { kTailRunning = 0, kTailEnded = 1, kTailGaveUp = 2 };
+// Z at ref and ref + 1 into the carried values (ref + 1 clamped to the
+// orbit: a step only runs while ref + 1 < orbit_size).
+FAV_HD void tail_load_z(const TailParams& q, int ref, TailState& t) {
+    const int r = ref >= 0 && ref < q.orbit_size ? ref : 0;
+    const int r1 = r + 1 < q.orbit_size ? r + 1 : r;
+    t.z0_re = q.z_re[r]; t.z0_im = q.z_im[r];
+    t.z1_re = q.z_re[r1]; t.z1_im = q.z_im[r1];
+}
+
+// kTailEscaped: the step escaped; tail_escape writes the result after the
+// loop, from the state the step left (iteration, ref, delta, previous norm,
+// derivative), so that the lanes of a wave that escape at different steps
+// take the smoothing and the plane stores together, once.
+enum TailStatus { kTailRunning = 0, kTailEnded = 1, kTailGaveUp = 2, kTailEscaped = 3 };
 
 // One iteration of the tail loop (the caller checked tail_continues).
 // kTailEnded / kTailGaveUp end the pixel with `out` (and planes) written as
@@ -424,7 +425,6 @@ FAV_HD TailStatus tail_step(const TailParams& q, const PlaneOut& planes, size_t 
                             SC& derivative, Budget& budget, bool& deadline, bool& unresolved, Counters<Stats>& st,
                             TailState& t, float& out) {
     const double escape_squared = escape_r2_double(q.escape_radius_mode);
-    const double safe_escape_squared = escape_squared + 1.0e-7;
     // Budget::expired returns false without counting a tick when no budget
     // is set, so skipping the call then changes nothing
     if (q.timed && budget.expired()) {
@@ -432,7 +432,10 @@ FAV_HD TailStatus tail_step(const TailParams& q, const PlaneOut& planes, size_t 
         st.add(ST_DEADLINE); st.add(ST_UNRESOLVED);
         return kTailEnded;
     }
-    const double z_re = q.z_re[ref], z_im = q.z_im[ref];
+    const double z_re = t.z0_re, z_im = t.z0_im;
+    // Z at ref + 2 for the next step (clamped: ref + 1 < orbit_size here)
+    const int ahead = ref + 2 < q.orbit_size ? ref + 2 : ref + 1;
+    const double z2_re = q.z_re[ahead], z2_im = q.z_im[ahead];
     if constexpr (Planes) {
         SC prior_total{z_re + t.d_re, z_im + t.d_im, 0};
         sc_normalize(prior_total);
@@ -453,8 +456,8 @@ FAV_HD TailStatus tail_step(const TailParams& q, const PlaneOut& planes, size_t 
         return kTailGaveUp;
     }
     st.add(ST_LOGICAL); st.add(ST_EXACT);
-    const double t_re = q.z_re[ref] + t.d_re;
-    const double t_im = q.z_im[ref] + t.d_im;
+    const double t_re = t.z1_re + t.d_re;
+    const double t_im = t.z1_im + t.d_im;
     const double m2 = t_re * t_re + t_im * t_im;
     if (!isfinite_(m2)) {
         unresolved = true;
@@ -462,18 +465,7 @@ FAV_HD TailStatus tail_step(const TailParams& q, const PlaneOut& planes, size_t 
         out = q.strict ? __builtin_nan("") : encode_iteration(iteration, q.output_bias);
         return kTailEnded;
     }
-    if (m2 > escape_squared) {
-        const double magnitude = ::sqrt(m2 > safe_escape_squared ? m2 : safe_escape_squared);
-        const double qlog = ::log(::log(magnitude)) / kLn2;
-        out = static_cast<float>((static_cast<double>(iteration) - qlog) - q.output_bias);
-        if constexpr (Planes) {
-            SC escaped_total{t_re, t_im, 0};
-            sc_normalize(escaped_total);
-            store_escape_outlined(planes, index, iteration, escaped_total, sn_from_double(m2), t.previous_norm,
-                                  derivative, q.pixel_spacing);
-        }
-        return kTailEnded;
-    }
+    if (m2 > escape_squared) return kTailEscaped;
     t.previous_norm = sn_from_double(m2);
     const double dm2 = t.d_re * t.d_re + t.d_im * t.d_im;
     ++t.tail_steps;
@@ -511,8 +503,34 @@ FAV_HD TailStatus tail_step(const TailParams& q, const PlaneOut& planes, size_t 
             if constexpr (Planes) clear_planes(planes, index, q.max_iter);
             return kTailGaveUp;
         }
+        tail_load_z(q, 0, t);
+        return kTailRunning;
     }
+    t.z0_re = t.z1_re; t.z0_im = t.z1_im;
+    t.z1_re = z2_re; t.z1_im = z2_im;
     return kTailRunning;
+}
+
+// The escape of a step that returned kTailEscaped: |Z+delta|^2 formed again
+// from the same operands (the same value), then the smooth value and the
+// planes (renderer.cpp:12570-12588).
+template <bool Planes>
+FAV_HD void tail_escape(const TailParams& q, const PlaneOut& planes, size_t index, int iteration, int ref,
+                        const SC& derivative, const TailState& t, float& out) {
+    const double escape_squared = escape_r2_double(q.escape_radius_mode);
+    const double safe_escape_squared = escape_squared + 1.0e-7;
+    const double t_re = q.z_re[ref] + t.d_re;
+    const double t_im = q.z_im[ref] + t.d_im;
+    const double m2 = t_re * t_re + t_im * t_im;
+    const double magnitude = ::sqrt(m2 > safe_escape_squared ? m2 : safe_escape_squared);
+    const double qlog = ::log(::log(magnitude)) / kLn2;
+    out = static_cast<float>((static_cast<double>(iteration) - qlog) - q.output_bias);
+    if constexpr (Planes) {
+        SC escaped_total{t_re, t_im, 0};
+        sc_normalize(escaped_total);
+        store_escape(planes, index, iteration, escaped_total, sn_from_double(m2), t.previous_norm, derivative,
+                     q.pixel_spacing);
+    }
 }
 
 // After the loop condition failed: the reference ran out before the cap.
@@ -535,12 +553,19 @@ FAV_HD bool double_tail(const RenderParams& p, const PlaneOut& planes, size_t in
                         Counters<Stats>& st, SN initial_norm, float& out) {
     const TailParams q = tail_params(p);
     TailState t = tail_begin(dc, delta, initial_norm);
+    tail_load_z(q, ref, t);
     out = encode_iteration(q.max_iter, q.output_bias);
+    TailStatus r = kTailRunning;
     while (tail_continues(q, iteration, ref)) {
-        const TailStatus r = tail_step<Planes, Stats>(q, planes, index, iteration, ref, derivative, budget, deadline,
-                                                      unresolved, st, t, out);
-        if (r != kTailRunning) return r == kTailGaveUp;
+        r = tail_step<Planes, Stats>(q, planes, index, iteration, ref, derivative, budget, deadline, unresolved, st,
+                                     t, out);
+        if (r != kTailRunning) break;
     }
+    if (r == kTailEscaped) {
+        tail_escape<Planes>(q, planes, index, iteration, ref, derivative, t, out);
+        return false;
+    }
+    if (r != kTailRunning) return r == kTailGaveUp;
     tail_exhausted<Planes, Stats>(q, planes, index, iteration, unresolved, st, out);
     return false;
 }
@@ -1678,6 +1703,12 @@ FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, S
     float out = 0.0f;
     const int cycle_min = p.strict_cycle ? 2048 : 512;
     const int cycle_margin = p.strict_cycle ? 78 : 60;
+    // How the loop ended, when the end's work runs after it (once for all
+    // the lanes of a wave that end at different iterations): the escape
+    // (kEndEncoded: reference index out of range, the count as is;
+    // kEndSmooth: the smooth value of s.total_norm), or the direct scaled
+    // recurrence (kEndExactTail).
+    enum { kEndNone, kEndEncoded, kEndSmooth, kEndExactTail } end = kEndNone;
 
     while (s.iteration < max_iter) {
         if (budget.expired()) {
@@ -1686,11 +1717,7 @@ FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, S
             break;
         }
         if (s.ref < 0 || s.ref >= o.size) {
-            out = encode_iteration(s.iteration, p.output_bias);
-            if constexpr (Planes)
-                store_escape(planes, index, s.iteration, s.total, s.total_norm, s.previous_total_norm, s.derivative,
-                             spacing);
-            escaped = true;
+            end = kEndEncoded;
             break;
         }
         if (!s.have_total) loop_top_values(o, s.ref, s.delta, mode, s.total, s.margin, s.total_norm);
@@ -1712,11 +1739,7 @@ FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, S
         }
         // escape, renderer.cpp:12989-13006
         if (sc_outside_escape(s.total_norm, mode) || fe_compare(s.margin, fe_zero()) > 0) {
-            out = smooth_escape(s.iteration, s.total_norm, p.output_bias);
-            if constexpr (Planes)
-                store_escape(planes, index, s.iteration, s.total, s.total_norm, s.previous_total_norm, s.derivative,
-                             spacing);
-            escaped = true;
+            end = kEndSmooth;
             break;
         }
         // Brent cycle check, renderer.cpp:13016-13042
@@ -1745,9 +1768,7 @@ FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, S
         const SN delta_norm = sc_norm_squared(s.delta);
         if (sc_compare_norm(s.total_norm, delta_norm) < 0) {
             if (++s.rebases > 64) {
-                exact_tail<Planes, Stats>(p, planes, index, dc, s.iteration, s.total, s.total_norm, s.derivative,
-                                          budget, deadline, unresolved, st, out);
-                if (!deadline) escaped = true;
+                end = kEndExactTail;
                 break;
             }
             s.delta = s.total;
@@ -1756,9 +1777,7 @@ FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, S
         }
         // the reference orbit escaped: direct recurrence, renderer.cpp:13155-13250
         if (o.map_end > 1 && s.ref >= o.map_end) {
-            exact_tail<Planes, Stats>(p, planes, index, dc, s.iteration, s.total, s.total_norm, s.derivative,
-                                      budget, deadline, unresolved, st, out);
-            if (!deadline) escaped = true;
+            end = kEndExactTail;
             break;
         }
         // BLA selection, renderer.cpp:13252-13273
@@ -1872,15 +1891,15 @@ FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, S
                     replay_norm = sn_from_margin(replay_margin, mode);
                     if (!sc_finite(s.delta) || !sc_finite(replay_total)) { retry = true; break; }
                     if (sc_outside_escape(replay_norm, mode) || fe_compare(replay_margin, fe_zero()) > 0) {
-                        out = smooth_escape(s.iteration, replay_norm, p.output_bias);
-                        if constexpr (Planes)
-                            store_escape(planes, index, s.iteration, replay_total, replay_norm, s.total_norm,
-                                         s.derivative, spacing);
-                        escaped = true;
+                        // the escape's values, as the loop-top escape finds them
+                        s.previous_total_norm = s.total_norm;
+                        s.total = replay_total;
+                        s.total_norm = replay_norm;
+                        end = kEndSmooth;
                         break;
                     }
                 }
-                if (deadline || escaped) break;
+                if (deadline || end != kEndNone) break;
                 if (retry) {
                     s.delta = previous_delta; s.derivative = previous_derivative;
                     s.ref = previous_ref; s.iteration = previous_iteration;
@@ -1914,6 +1933,18 @@ FAV_HD Outcome run_general(const RenderParams& p, size_t index, PixelState& s, S
         ++s.iteration;
         s.previous_total_norm = prior_norm;
         st.add(ST_LOGICAL); st.add(ST_EXACT);
+    }
+    if (end == kEndEncoded || end == kEndSmooth) {
+        out = end == kEndSmooth ? smooth_escape(s.iteration, s.total_norm, p.output_bias)
+                                : encode_iteration(s.iteration, p.output_bias);
+        if constexpr (Planes)
+            store_escape(planes, index, s.iteration, s.total, s.total_norm, s.previous_total_norm, s.derivative,
+                         spacing);
+        escaped = true;
+    } else if (end == kEndExactTail) {
+        exact_tail<Planes, Stats>(p, planes, index, dc, s.iteration, s.total, s.total_norm, s.derivative, budget,
+                                  deadline, unresolved, st, out);
+        if (!deadline) escaped = true;
     }
     if (!escaped && !deadline && !unresolved) out = encode_iteration(max_iter, p.output_bias);
     p.output[index] = out;

@@ -25,6 +25,18 @@
 
 #include "field_core.h"
 
+// Occupancy targets (waves per SIMD) of the queue-driven kernels.  The
+// general kernel takes the register file of one wave per SIMD (about 270
+// registers): limited to 2-5 waves it spills, and on boundary-rich fields
+// it measured 5-24% slower at every such target.  The tail kernel fits 4
+// waves without spilling; 5 spills and measured no faster.
+#ifndef FAV_GENERAL_WAVES
+#define FAV_GENERAL_WAVES 1
+#endif
+#ifndef FAV_TAIL_WAVES
+#define FAV_TAIL_WAVES 4
+#endif
+
 namespace fav {
 namespace {
 
@@ -172,7 +184,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Planes
 }
 
 template <bool Planes, bool Stats>
-__global__ void __launch_bounds__(256) general_kernel(RenderParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAV_GENERAL_WAVES, 8)))
+general_kernel(RenderParams p) {
     const unsigned count = p.state.counters[0];
     for (;;) {
         const unsigned base = wave_grab(p.state.counters + 1, count);
@@ -202,7 +215,8 @@ __global__ void __launch_bounds__(256) general_kernel(RenderParams p) {
 }
 
 template <bool Planes, bool Stats>
-__global__ void __launch_bounds__(256) tail_kernel(RenderParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAV_TAIL_WAVES, 8)))
+tail_kernel(RenderParams p) {
     const unsigned count = p.state.counters[2];
     for (;;) {
         const unsigned base = wave_grab(p.state.counters + 3, count);
