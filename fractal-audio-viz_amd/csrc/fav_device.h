@@ -84,8 +84,12 @@ static_assert(sizeof(OrbitFastDev) == 64, "orbit record is one cache line");
 //   |delta|^2 < 2^-161 (renderer.cpp:13252-13273); all times (1 - 2^-48),
 //   so that a |delta|^2 within 2^-50 of the reference's rounding of it
 //   (the slot-0 loop forms it with one FMA) passes only when that does.
-// * small2: (small_d of slot 0's end index)^2 * (1 - 2^-48): |delta'|^2
-//   below it puts the larger component of delta' below that small_d.
+// * end_r2: min(small^2 (1 - 2^-48), fast_r2 of the end index's rung),
+//   where small is small_d of slot 0's end index: |delta'|^2 below it puts
+//   the larger component of delta' below that small_d and passes the next
+//   block's fast_r2 test.  -0.0 (no |delta'|^2 is below it) where that
+//   minimum is outside [2^-900, 2^200] (so that scaling it by up to 2^820
+//   stays exact and finite).
 // * gap_lo, gap_hi: |m|^2 / |c|^2 within them puts |A m| / |B c| within
 //   2^-58 .. 2^58 (0 and 0 when slot 0 is not applicable).
 struct alignas(32) PlainStepDev {
@@ -96,7 +100,7 @@ static_assert(sizeof(PlainStepDev) == 32, "plain step is half a cache line");
 constexpr int kRungSlots = 5;
 struct alignas(256) RungDev {
     double fast_r2;
-    double small2;
+    double end_r2;
     double a_re, a_im, b_re, b_im;
     uint32_t stride;
     int32_t top;

@@ -1069,14 +1069,14 @@ FAV_HD bool slot_plain(double t) {
 
 // What a slot-0 block reads: the first cache line of its rung.
 struct RungLine {
-    double fast_r2, small2;
+    double fast_r2, end_r2;
     PlainStepDev step;
     uint32_t stride;
     double gap_lo, gap_hi;
 };
 
 FAV_HD RungLine load_line(const RungDev* g) {
-    return RungLine{uniform_double(load_uniform(&g->fast_r2, 0)), uniform_double(load_uniform(&g->small2, 0)),
+    return RungLine{uniform_double(load_uniform(&g->fast_r2, 0)), uniform_double(load_uniform(&g->end_r2, 0)),
                     PlainStepDev{uniform_double(load_uniform(&g->a_re, 0)), uniform_double(load_uniform(&g->a_im, 0)),
                                  uniform_double(load_uniform(&g->b_re, 0)), uniform_double(load_uniform(&g->b_im, 0))},
                     load_uniform(&g->stride, 0), uniform_double(load_uniform(&g->gap_lo, 0)),
@@ -1100,6 +1100,28 @@ FAV_HD PlainStepDev slot_step(const RungDev* g, int k) {
         return PlainStepDev{uniform_double(load_uniform(&g->a_re, 0)), uniform_double(load_uniform(&g->a_im, 0)),
                             uniform_double(load_uniform(&g->b_re, 0)), uniform_double(load_uniform(&g->b_im, 0))};
     return load_step(&g->step_low[k - 1]);
+}
+
+// The bound on |m'|^2 of a slot-0 block: end_r2 * 2^-2E (end_r2 bounds
+// |delta'|^2 = |m'|^2 2^2E), capped near 2^199.  end_r2 is -0.0 or a
+// normal double in [2^-900, 2^200] and -2E is in [0, 820], so the scaling
+// is exact as an addition to the exponent field: the high word of the bit
+// pattern minus shift_hi = 2E << 20, a 32-bit integer operation that stays
+// on the scalar unit (both operands are wave-uniform) and cannot overflow;
+// -0.0 stays negative, so that no |m'|^2 is below it.  The cap is the
+// minimum of the high words, which keeps the low word: a capped bound is
+// below 2^199 (1 + 2^-20), so |m'|^2 under it still puts the larger
+// component of m' below 2^100.
+struct DoubleWords {
+    uint32_t lo, hi;
+};
+
+FAV_HD double end_bound(double end_r2, int32_t shift_hi) {
+    DoubleWords w = __builtin_bit_cast(DoubleWords, end_r2);
+    const int32_t hi = static_cast<int32_t>(w.hi) - shift_hi;
+    constexpr int32_t kCapHigh = 0x4c600000;  // high word of 2^199
+    w.hi = static_cast<uint32_t>(hi < kCapHigh ? hi : kCapHigh);
+    return __builtin_bit_cast(double, w);
 }
 
 // One block m' = A*m + B*c and the tests of its result, taken together:
@@ -1135,28 +1157,32 @@ FAV_HD PlainBlock plain_block(const PlainStepDev& step, double mr, double mi, do
 // (fav_device.h), each of which implies the corresponding general test of
 // plain_block or of the loop top (so a block this accepts is the block the
 // general rules commit, and one it rejects is decided by them):
-//  * |delta|^2 < fast_r2: the loop-top tests and slot 0 chosen;
-//  * gap_lo <= |m|^2 / |c|^2 <= gap_hi, computed as |delta|^2 * kappa with
-//    kappa = 2^-2E / |c|^2: |A m| / |B c| within 2^+-58 up to rounding, so
-//    the larger components are within 2^59 < 2^60;
+//  * |delta|^2 < fast_r2 at the block's rung: the loop-top tests and slot 0
+//    chosen.  It is tested by the block before (end_r2 below), or, for the
+//    first block of a run, by the caller;
+//  * gap_lo <= |m|^2 / |c|^2 <= gap_hi, computed as |m|^2 * inv_cc with
+//    inv_cc = 1 / |c|^2 (the ratio of the scaled values): |A m| / |B c|
+//    within 2^+-58 up to rounding, so the larger components are within
+//    2^59 < 2^60;
 //  * both components of m' at least 2^-99 (so the larger one is at least
 //    2^-100 and the smaller one at least 2^-320; a component closer to
-//    zero is left to the general test) and |m'|^2 <= 2^199 (the larger
-//    component at most 2^100);
-//  * |delta'|^2 < small2 = small^2 (1 - 2^-48): the larger component of
-//    delta' below small (|m'|^2 carries a relative error below 2^-50).
-// |delta'|^2 = |m'|^2 * 2^2E is the next block's |delta|^2.  |m'|^2 is
-// formed as fma(nr, nr, ni * ni): within 2^-50 of the reference's
-// nr * nr + ni * ni, which the (1 - 2^-48) factors of fast_r2 and small2
-// absorb; the committed m' itself is the reference's value.
+//    zero is left to the general test) and |m'|^2 < 2^199 (1 + 2^-20)
+//    (the larger component at most 2^100);
+//  * |delta'|^2 < end_r2 = min(small^2, fast_r2 of the end index's rung),
+//    both with a factor (1 - 2^-48): the larger component of delta' below
+//    small (|m'|^2 carries a relative error below 2^-50), and the next
+//    block's loop-top bound.  Tested as |m'|^2 < end_bound(end_r2).
+// |m'|^2 is the next block's |m|^2, formed as fma(nr, nr, ni * ni): within
+// 2^-50 of the reference's nr * nr + ni * ni, which the (1 - 2^-48)
+// factors absorb; the committed m' itself is the reference's value.
 struct CommonBlock {
-    double nr, ni, dd;
+    double nr, ni, sq;
     uint64_t ok;
 };
 
-FAV_HD CommonBlock common_block(const RungLine& line, double mr, double mi, double dd, double cr, double ci,
-                                double kappa, int E) {
-    const double h = dd * kappa;
+FAV_HD CommonBlock common_block(const RungLine& line, double mr, double mi, double sq, double cr, double ci,
+                                double inv_cc, int32_t shift_hi) {
+    const double h = sq * inv_cc;
     const double pr = line.step.a_re * mr - line.step.a_im * mi;
     const double pi = line.step.a_re * mi + line.step.a_im * mr;
     const double qr = line.step.b_re * cr - line.step.b_im * ci;
@@ -1164,11 +1190,10 @@ FAV_HD CommonBlock common_block(const RungLine& line, double mr, double mi, doub
     CommonBlock b;
     b.nr = pr + qr;
     b.ni = pi + qi;
-    const double sq = __builtin_fma(b.nr, b.nr, b.ni * b.ni);
-    b.dd = ldexp_(sq, 2 * E);
+    b.sq = __builtin_fma(b.nr, b.nr, b.ni * b.ni);
     const double nn = min_abs(b.nr, b.ni);
-    b.ok = wave_mask(dd < line.fast_r2) & wave_mask(h >= line.gap_lo) & wave_mask(h <= line.gap_hi)
-        & wave_mask(nn >= 0x1p-99) & wave_mask(sq <= 0x1p199) & wave_mask(b.dd < line.small2);
+    b.ok = wave_mask(h >= line.gap_lo) & wave_mask(h <= line.gap_hi) & wave_mask(nn >= 0x1p-99)
+        & wave_mask(b.sq < end_bound(line.end_r2, shift_hi));
     return b;
 }
 
@@ -1200,8 +1225,9 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     if (r0 <= 0 || r0 >= p.orbit.size) return kPlainNotEntered;
     int E = wave_first(s.delta.e);
     // |delta|^2 = |m|^2 * 2^2E with |m| >= 2^-100 stays a normal double
-    // while E >= -410; |delta| < |Z| keeps it finite
-    if (E < -410 || E > (1 << 20)) return kPlainNotEntered;
+    // while E >= -410; |delta| < |Z| keeps it finite.  E <= 0 (a negligible
+    // delta is far below 1) bounds the scaling of end_bound.
+    if (E < -410 || E > 0) return kPlainNotEntered;
     // the rungs start at the table's level cap, which must bound the
     // lookup's highest level (floor(log2) of the length limit)
     const int max_length = p.max_linear_length;
@@ -1265,22 +1291,26 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
             // rungs.
             // Slot-0 blocks (common_block), one rung read per block.
             const uint32_t off_common = off;
-            {
-                const double kappa = ldexp_(inv_cc, -2 * E);
-                double dd_c = ldexp_(sq, 2 * E);
+            // the first block's loop-top bound (the later ones are tested by
+            // the block before them)
+            const double fast0 =
+                uniform_double(load_uniform(&reinterpret_cast<const RungDev*>(base + off)->fast_r2, 0));
+            if (wave_mask(ldexp_(sq, 2 * E) < fast0) == active) {
+                const int32_t shift_hi = 2 * E * (1 << 20);
+                double sq_c = sq;
                 // unrolled twice, so that m alternates between two sets of
                 // registers instead of being copied at each commit
                 for (;;) {
                     const RungLine line = load_line(reinterpret_cast<const RungDev*>(base + off));
                     issue_here(line.stride);
-                    const CommonBlock b = common_block(line, mr, mi, dd_c, cr, ci, kappa, E);
+                    const CommonBlock b = common_block(line, mr, mi, sq_c, cr, ci, inv_cc, shift_hi);
                     if ((b.ok & (off <= off_full ? ~0ULL : 0ULL)) != active) break;
                     // commit, renderer.cpp:13499-13511
                     plain_common_stats(st, line.stride);
                     off += line.stride;
                     const RungLine line2 = load_line(reinterpret_cast<const RungDev*>(base + off));
                     issue_here(line2.stride);
-                    const CommonBlock b2 = common_block(line2, b.nr, b.ni, b.dd, cr, ci, kappa, E);
+                    const CommonBlock b2 = common_block(line2, b.nr, b.ni, b.sq, cr, ci, inv_cc, shift_hi);
                     if ((b2.ok & (off <= off_full ? ~0ULL : 0ULL)) != active) {
                         mr = b.nr;
                         mi = b.ni;
@@ -1288,7 +1318,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
                     }
                     mr = b2.nr;
                     mi = b2.ni;
-                    dd_c = b2.dd;
+                    sq_c = b2.sq;
                     plain_common_stats(st, line2.stride);
                     off += line2.stride;
                 }
@@ -1363,7 +1393,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
             if constexpr (Stats) st.add(ST_HISTOGRAM + (level < 15 ? level : 15));
             st.add(ST_LOGICAL, static_cast<unsigned long long>(1 << level));
         }
-        if (!rescale || retried || E + shift < -410) {
+        if (!rescale || retried || E + shift < -410 || E + shift > 0) {
             if (rescale) FAV_PLAIN_EXIT_PROBE(13);
             break;
         }

@@ -178,11 +178,13 @@ void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, bool or
             }
         }
     }
-    // the square of slot 0's end bound, once every small_d is set
+    // the square of slot 0's end bound and the end rung's fast bound, once
+    // every small_d and fast_r2 is set
     for (size_t r = 1; r < n; ++r) {
         const size_t end = r + (size_t(1) << rungs[r].top);
         const double small = end < n ? rungs[end].small_d : 0.0;
-        rungs[r].small2 = small * small * (1.0 - 0x1p-48);
+        const double bound = std::min(small * small * (1.0 - 0x1p-48), end < n ? rungs[end].fast_r2 : 0.0);
+        rungs[r].end_r2 = bound >= 0x1p-900 && bound <= 0x1p200 ? bound : -0.0;
     }
 }
 
