@@ -1094,6 +1094,15 @@ FAV_HD void issue_here(uint32_t v) {
 #endif
 }
 
+// Keeps the instructions before it (a rung read issued a block ahead) from
+// being scheduled after the ones that follow it (no instruction is
+// emitted).
+FAV_HD void schedule_fence() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
 // The step of slot k of a rung.
 FAV_HD PlainStepDev slot_step(const RungDev* g, int k) {
     if (k == 0)
@@ -1289,7 +1298,12 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
             // small_d > 0 at r (checked at entry and at every commit) means a
             // block may start here; the other waves of the CU walk the same
             // rungs.
-            // Slot-0 blocks (common_block), one rung read per block.
+            // Slot-0 blocks (common_block), one rung read per block, issued a
+            // block ahead: while a block is formed and tested, the rung its
+            // end index names is already on its way.  That read is in the
+            // table whether or not the block commits: a rung whose slot 0
+            // applies has its end index in the table, and one whose slot 0
+            // does not has stride 0 (it names itself).
             const uint32_t off_common = off;
             // the first block's loop-top bound (the later ones are tested by
             // the block before them)
@@ -1298,19 +1312,27 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
             if (wave_mask(ldexp_(sq, 2 * E) < fast0) == active) {
                 const int32_t shift_hi = 2 * E * (1 << 20);
                 double sq_c = sq;
-                // unrolled twice, so that m alternates between two sets of
-                // registers instead of being copied at each commit
+                RungLine line = load_line(reinterpret_cast<const RungDev*>(base + off));
+                // unrolled twice, so that m and the rung lines alternate
+                // between two sets of registers instead of being copied
                 for (;;) {
-                    const RungLine line = load_line(reinterpret_cast<const RungDev*>(base + off));
-                    issue_here(line.stride);
+                    const uint32_t off2 = off + line.stride;
+                    const RungLine line2 = load_line(reinterpret_cast<const RungDev*>(base + off2));
+                    schedule_fence();
                     const CommonBlock b = common_block(line, mr, mi, sq_c, cr, ci, inv_cc, shift_hi);
+                    // the next address needs line2's stride: read it here,
+                    // after the block, so that it is not left to the commit
+                    schedule_fence();
+                    issue_here(line2.stride);
                     if ((b.ok & (off <= off_full ? ~0ULL : 0ULL)) != active) break;
                     // commit, renderer.cpp:13499-13511
                     plain_common_stats(st, line.stride);
-                    off += line.stride;
-                    const RungLine line2 = load_line(reinterpret_cast<const RungDev*>(base + off));
-                    issue_here(line2.stride);
+                    off = off2;
+                    line = load_line(reinterpret_cast<const RungDev*>(base + off2 + line2.stride));
+                    schedule_fence();
                     const CommonBlock b2 = common_block(line2, b.nr, b.ni, b.sq, cr, ci, inv_cc, shift_hi);
+                    schedule_fence();
+                    issue_here(line.stride);
                     if ((b2.ok & (off <= off_full ? ~0ULL : 0ULL)) != active) {
                         mr = b.nr;
                         mi = b.ni;
