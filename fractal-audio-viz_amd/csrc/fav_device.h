@@ -84,14 +84,24 @@ static_assert(sizeof(OrbitFastDev) == 64, "orbit record is one cache line");
 //   |delta|^2 < 2^-161 (renderer.cpp:13252-13273); all times (1 - 2^-48),
 //   so that a |delta|^2 within 2^-50 of the reference's rounding of it
 //   (the slot-0 loop forms it with one FMA) passes only when that does.
-// * end_r2: min(small^2 (1 - 2^-48), fast_r2 of the end index's rung),
-//   where small is small_d of slot 0's end index: |delta'|^2 below it puts
-//   the larger component of delta' below that small_d and passes the next
-//   block's fast_r2 test.  -0.0 (no |delta'|^2 is below it) where that
-//   minimum is outside [2^-900, 2^200] (so that scaling it by up to 2^820
-//   stays exact and finite).
-// * gap_lo, gap_hi: |m|^2 / |c|^2 within them puts |A m| / |B c| within
-//   2^-58 .. 2^58 (0 and 0 when slot 0 is not applicable).
+// * the product gap: |m|^2 / |c|^2 within [gap_lo, gap_hi] puts |A m| /
+//   |B c| within 2^-58 .. 2^58, where gap_lo = |B|^2/|A|^2 2^-116 and
+//   gap_hi = |B|^2/|A|^2 2^116, narrowed to [2^-600, 2^600] (0 and 0 when
+//   slot 0 is not applicable).
+// Bounds the loop compares with are kept as the high word of a double
+// whose low word is 0 ("hi" below), so that the scalar unit can scale them
+// by a power of two with one integer subtraction and take their minimum
+// with one integer minimum: an upper bound is rounded down to that form,
+// a lower bound up.
+// * end_hi: the upper bound on |delta'|^2 of slot 0's block, min(small^2
+//   (1 - 2^-48), fast_r2 of the end index's rung), where small is small_d
+//   of the end index: |delta'|^2 below it puts the larger component of
+//   delta' below that small_d and passes the next block's fast_r2 test.
+//   The high word of -0.0 (no |delta'|^2 is below it) where that minimum is
+//   outside [2^-900, 2^200].
+// * gap_lo_end, gap_hi_end: gap_lo and gap_hi of the end index's rung, the
+//   next block's product-gap bounds on |m'|^2 / |c|^2; gap_lo_own and
+//   gap_hi_own: this rung's own (for the first block of a run).
 struct alignas(32) PlainStepDev {
     double a_re, a_im, b_re, b_im;
 };
@@ -99,19 +109,25 @@ static_assert(sizeof(PlainStepDev) == 32, "plain step is half a cache line");
 
 constexpr int kRungSlots = 5;
 struct alignas(256) RungDev {
-    double fast_r2;
-    double end_r2;
+    // first cache line: what a slot-0 block reads, and fast_r2
+    int32_t end_hi;
+    int32_t gap_hi_end;
     double a_re, a_im, b_re, b_im;
     uint32_t stride;
+    int32_t gap_lo_end;
+    double fast_r2;
     int32_t top;
+    int32_t gap_lo_own;
+    // second: the slot search
+    int32_t gap_hi_own;
+    int32_t pad;
     double zn_d;
-    double gap_lo, gap_hi;
     double r2[kRungSlots];
     double small_d;
     PlainStepDev step_low[kRungSlots - 1];
 };
 static_assert(sizeof(RungDev) == 256, "rung is four cache lines");
-static_assert(offsetof(RungDev, gap_lo) == 64, "a slot-0 block reads the first two cache lines");
+static_assert(offsetof(RungDev, gap_lo_own) == 60, "a slot-0 block reads the first cache line");
 static_assert(offsetof(RungDev, step_low) == 128, "steps of slots 1.. are aligned");
 
 // Device copy of one reference (orbit + tables), structure of arrays.  The

@@ -1069,18 +1069,16 @@ FAV_HD bool slot_plain(double t) {
 
 // What a slot-0 block reads: the first cache line of its rung.
 struct RungLine {
-    double fast_r2, end_r2;
+    int32_t end_hi, gap_lo_end, gap_hi_end;
     PlainStepDev step;
     uint32_t stride;
-    double gap_lo, gap_hi;
 };
 
 FAV_HD RungLine load_line(const RungDev* g) {
-    return RungLine{uniform_double(load_uniform(&g->fast_r2, 0)), uniform_double(load_uniform(&g->end_r2, 0)),
+    return RungLine{load_uniform(&g->end_hi, 0), load_uniform(&g->gap_lo_end, 0), load_uniform(&g->gap_hi_end, 0),
                     PlainStepDev{uniform_double(load_uniform(&g->a_re, 0)), uniform_double(load_uniform(&g->a_im, 0)),
                                  uniform_double(load_uniform(&g->b_re, 0)), uniform_double(load_uniform(&g->b_im, 0))},
-                    load_uniform(&g->stride, 0), uniform_double(load_uniform(&g->gap_lo, 0)),
-                    uniform_double(load_uniform(&g->gap_hi, 0))};
+                    load_uniform(&g->stride, 0)};
 }
 
 // Marks a scalar value as needed here, so that the load that produces it is
@@ -1092,6 +1090,15 @@ FAV_HD void issue_here(uint32_t v) {
 #else
     (void)v;
 #endif
+}
+
+// The integer fields of a rung line read here rather than where they are
+// used (see issue_here).
+FAV_HD void issue_line(const RungLine& line) {
+    issue_here(line.stride);
+    issue_here(static_cast<uint32_t>(line.end_hi));
+    issue_here(static_cast<uint32_t>(line.gap_lo_end));
+    issue_here(static_cast<uint32_t>(line.gap_hi_end));
 }
 
 // Keeps the instructions before it (a rung read issued a block ahead) from
@@ -1111,26 +1118,90 @@ FAV_HD PlainStepDev slot_step(const RungDev* g, int k) {
     return load_step(&g->step_low[k - 1]);
 }
 
-// The bound on |m'|^2 of a slot-0 block: end_r2 * 2^-2E (end_r2 bounds
-// |delta'|^2 = |m'|^2 2^2E), capped near 2^199.  end_r2 is -0.0 or a
-// normal double in [2^-900, 2^200] and -2E is in [0, 820], so the scaling
-// is exact as an addition to the exponent field: the high word of the bit
-// pattern minus shift_hi = 2E << 20, a 32-bit integer operation that stays
-// on the scalar unit (both operands are wave-uniform) and cannot overflow;
-// -0.0 stays negative, so that no |m'|^2 is below it.  The cap is the
-// minimum of the high words, which keeps the low word: a capped bound is
-// below 2^199 (1 + 2^-20), so |m'|^2 under it still puts the larger
-// component of m' below 2^100.
-struct DoubleWords {
-    uint32_t lo, hi;
+// The scalings a run of slot-0 blocks applies to the rung bounds (fav_device.h
+// "hi" words), fixed while E and c are: as integers subtracted from a high
+// word they multiply the bound by a power of two, exactly.
+//  * shift_hi = 2E << 20: end_hi bounds |delta'|^2 = |m'|^2 2^2E, so
+//    end_hi - shift_hi bounds |m'|^2;
+//  * lo_shift = fl_min << 20 and hi_shift = (fl_max + 1) << 20, with
+//    2^fl_min <= 1/|c|^2 < 2^(fl_max + 1) for every lane (1/|c|^2 being the
+//    lane's inv_cc): |m'|^2 >= gap_lo 2^-fl_min implies |m'|^2 inv_cc >=
+//    gap_lo, and |m'|^2 <= gap_hi 2^-(fl_max + 1) implies |m'|^2 inv_cc <=
+//    gap_hi, which are the product-gap tests on the next block's |m|^2.
+// The ranges (-2E in [0, 820], fl in [-256, 256], hi words of bounds in
+// [2^-900, 2^600] or -0.0) keep every difference a finite double's high
+// word of the same sign, so no subtraction overflows.
+struct RunScale {
+    int32_t shift_hi, lo_shift, hi_shift;
 };
 
-FAV_HD double end_bound(double end_r2, int32_t shift_hi) {
-    DoubleWords w = __builtin_bit_cast(DoubleWords, end_r2);
-    const int32_t hi = static_cast<int32_t>(w.hi) - shift_hi;
+// A double from its high word (low word 0).
+FAV_HD double from_high_word(int32_t hi) {
+    return __builtin_bit_cast(double, static_cast<unsigned long long>(static_cast<uint32_t>(hi)) << 32);
+}
+
+// The largest (want_max) or smallest value of v over the active lanes,
+// every v in [lo, hi]: bisection on ballots, so the result is wave-uniform
+// and lanes outside the exec mask take no part.
+FAV_HD int wave_extreme(int v, int lo, int hi, bool want_max) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int a = lo, b = hi;  // the extreme is in [a, b]
+    while (a < b) {
+        if (want_max) {
+            const int mid = a + (b - a + 1) / 2;
+            if (__ballot(v >= mid) != 0ULL) a = mid; else b = mid - 1;
+        } else {
+            const int mid = a + (b - a) / 2;
+            if (__ballot(v <= mid) != 0ULL) b = mid; else a = mid + 1;
+        }
+    }
+    return a;
+#else
+    (void)lo; (void)hi; (void)want_max;
+    return v;
+#endif
+}
+
+// The gap scalings of RunScale for the lanes' inv_cc (floor(log2 inv_cc),
+// 300 for a value that is not a positive finite double); false when some
+// lane's is outside [-256, 255], where the bounds would leave their range.
+FAV_HD bool gap_scale(double inv_cc, RunScale& k) {
+    int fl = 300;
+    if (inv_cc > 0.0 && isfinite_(inv_cc)) {
+        fl = frexp_exp_(inv_cc) - 1;
+        fl = fl < -300 ? -300 : (fl > 300 ? 300 : fl);
+    }
+    const int fl_min = wave_extreme(fl, -300, 300, false);
+    const int fl_max = wave_extreme(fl, -300, 300, true);
+    k.lo_shift = fl_min * (1 << 20);
+    k.hi_shift = (fl_max + 1) * (1 << 20);
+    return fl_min >= -256 && fl_max <= 255;
+}
+
+// min(a, b) of two wave-uniform values on the scalar unit (left to itself
+// the compiler forms the bound's three-way minimum on the vector unit).
+FAV_HD int32_t scalar_min(int32_t a, int32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int32_t r;
+    __asm__("s_min_i32 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b) : "scc");
+    return r;
+#else
+    return a < b ? a : b;
+#endif
+}
+
+// The upper bound on |m'|^2 of a slot-0 block: the end index's bound on
+// |delta'|^2 and its rung's gap_hi, both scaled, and 2^199 (so that the
+// larger component of m' stays below 2^100), by one integer minimum each;
+// -0.0 (negative high word) stays negative, so that no |m'|^2 is below it.
+FAV_HD double upper_bound(const RungLine& line, const RunScale& k) {
     constexpr int32_t kCapHigh = 0x4c600000;  // high word of 2^199
-    w.hi = static_cast<uint32_t>(hi < kCapHigh ? hi : kCapHigh);
-    return __builtin_bit_cast(double, w);
+    return from_high_word(scalar_min(scalar_min(line.end_hi - k.shift_hi, line.gap_hi_end - k.hi_shift), kCapHigh));
+}
+
+// The lower bound on |m'|^2: the end index's rung's gap_lo, scaled.
+FAV_HD double lower_bound(const RungLine& line, const RunScale& k) {
+    return from_high_word(line.gap_lo_end - k.lo_shift);
 }
 
 // One block m' = A*m + B*c and the tests of its result, taken together:
@@ -1165,33 +1236,36 @@ FAV_HD PlainBlock plain_block(const PlainStepDev& step, double mr, double mi, do
 // A slot-0 block and its tests as one ballot, through the rung's bounds
 // (fav_device.h), each of which implies the corresponding general test of
 // plain_block or of the loop top (so a block this accepts is the block the
-// general rules commit, and one it rejects is decided by them):
+// general rules commit, and one it rejects is decided by them).  The tests
+// on a block's input, m, are made on the result of the block before it
+// (for the first block of a run, by the caller, first_block_fits):
 //  * |delta|^2 < fast_r2 at the block's rung: the loop-top tests and slot 0
-//    chosen.  It is tested by the block before (end_r2 below), or, for the
-//    first block of a run, by the caller;
-//  * gap_lo <= |m|^2 / |c|^2 <= gap_hi, computed as |m|^2 * inv_cc with
-//    inv_cc = 1 / |c|^2 (the ratio of the scaled values): |A m| / |B c|
-//    within 2^+-58 up to rounding, so the larger components are within
-//    2^59 < 2^60;
+//    chosen;
+//  * gap_lo <= |m|^2 / |c|^2 <= gap_hi: |A m| / |B c| within 2^+-58 up to
+//    rounding, so the larger components are within 2^59 < 2^60;
+// and on its result, m':
 //  * both components of m' at least 2^-99 (so the larger one is at least
 //    2^-100 and the smaller one at least 2^-320; a component closer to
-//    zero is left to the general test) and |m'|^2 < 2^199 (1 + 2^-20)
-//    (the larger component at most 2^100);
-//  * |delta'|^2 < end_r2 = min(small^2, fast_r2 of the end index's rung),
-//    both with a factor (1 - 2^-48): the larger component of delta' below
-//    small (|m'|^2 carries a relative error below 2^-50), and the next
-//    block's loop-top bound.  Tested as |m'|^2 < end_bound(end_r2).
-// |m'|^2 is the next block's |m|^2, formed as fma(nr, nr, ni * ni): within
-// 2^-50 of the reference's nr * nr + ni * ni, which the (1 - 2^-48)
-// factors absorb; the committed m' itself is the reference's value.
+//    zero is left to the general test) and |m'|^2 < 2^199 (the larger
+//    component at most 2^100);
+//  * |delta'|^2 below min(small^2, fast_r2 of the end index's rung), both
+//    with a factor (1 - 2^-48): the larger component of delta' below small
+//    (|m'|^2 carries a relative error below 2^-50), and the next block's
+//    loop-top bound;
+//  * the next block's product gap, through the lane-independent bounds of
+//    RunScale.
+// The three bounds on |m'|^2 are two compares with scalar operands
+// (upper_bound, lower_bound).  |m'|^2 is the next block's |m|^2, formed as
+// fma(nr, nr, ni * ni): within 2^-50 of the reference's nr * nr + ni * ni,
+// which the (1 - 2^-48) factors and the gap margins absorb; the committed
+// m' itself is the reference's value.
 struct CommonBlock {
     double nr, ni, sq;
     uint64_t ok;
 };
 
-FAV_HD CommonBlock common_block(const RungLine& line, double mr, double mi, double sq, double cr, double ci,
-                                double inv_cc, int32_t shift_hi) {
-    const double h = sq * inv_cc;
+FAV_HD CommonBlock common_block(const RungLine& line, double mr, double mi, double cr, double ci,
+                                const RunScale& k) {
     const double pr = line.step.a_re * mr - line.step.a_im * mi;
     const double pi = line.step.a_re * mi + line.step.a_im * mr;
     const double qr = line.step.b_re * cr - line.step.b_im * ci;
@@ -1201,8 +1275,8 @@ FAV_HD CommonBlock common_block(const RungLine& line, double mr, double mi, doub
     b.ni = pi + qi;
     b.sq = __builtin_fma(b.nr, b.nr, b.ni * b.ni);
     const double nn = min_abs(b.nr, b.ni);
-    b.ok = wave_mask(h >= line.gap_lo) & wave_mask(h <= line.gap_hi) & wave_mask(nn >= 0x1p-99)
-        & wave_mask(b.sq < end_bound(line.end_r2, shift_hi));
+    b.ok = wave_mask(nn >= 0x1p-99) & wave_mask(b.sq < upper_bound(line, k))
+        & wave_mask(b.sq >= lower_bound(line, k));
     return b;
 }
 
@@ -1263,8 +1337,11 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
     // Order 1 (renderer.cpp:13252-13273): |delta|^2 < 2^-161 when the
     // approximation order is 3; order 2 never takes a linear block.
     const double order_bound = p.approximation_order >= 3 ? 0x1p-161 : __builtin_inf();
-    // 1 / |c|^2, for the slot-0 blocks' product-gap test
+    // 1 / |c|^2, for the slot-0 blocks' product-gap test, and its wave-wide
+    // scalings (gap_ok false: no slot-0 run at this scale)
     double inv_cc = 1.0 / (cr * cr + ci * ci);
+    RunScale scale{};
+    bool gap_ok = gap_scale(inv_cc, scale);
     // iteration - r stays `lag`: both advance by each block's length
     const int lag = it0 - r0;
     // below this iteration the length limit is max_length itself
@@ -1305,34 +1382,37 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
             // applies has its end index in the table, and one whose slot 0
             // does not has stride 0 (it names itself).
             const uint32_t off_common = off;
-            // the first block's loop-top bound (the later ones are tested by
-            // the block before them)
-            const double fast0 =
-                uniform_double(load_uniform(&reinterpret_cast<const RungDev*>(base + off)->fast_r2, 0));
-            if (wave_mask(ldexp_(sq, 2 * E) < fast0) == active) {
-                const int32_t shift_hi = 2 * E * (1 << 20);
-                double sq_c = sq;
-                RungLine line = load_line(reinterpret_cast<const RungDev*>(base + off));
+            // the first block's tests on its input (the loop-top bound and
+            // the product gap; the later blocks' are made by the block
+            // before them)
+            const RungDev* const g0 = reinterpret_cast<const RungDev*>(base + off);
+            scale.shift_hi = 2 * E * (1 << 20);
+            if (gap_ok
+                && (wave_mask(ldexp_(sq, 2 * E) < uniform_double(load_uniform(&g0->fast_r2, 0)))
+                    & wave_mask(sq >= from_high_word(load_uniform(&g0->gap_lo_own, 0) - scale.lo_shift))
+                    & wave_mask(sq <= from_high_word(load_uniform(&g0->gap_hi_own, 0) - scale.hi_shift)))
+                    == active) {
+                RungLine line = load_line(g0);
                 // unrolled twice, so that m and the rung lines alternate
                 // between two sets of registers instead of being copied
                 for (;;) {
                     const uint32_t off2 = off + line.stride;
                     const RungLine line2 = load_line(reinterpret_cast<const RungDev*>(base + off2));
                     schedule_fence();
-                    const CommonBlock b = common_block(line, mr, mi, sq_c, cr, ci, inv_cc, shift_hi);
+                    const CommonBlock b = common_block(line, mr, mi, cr, ci, scale);
                     // the next address needs line2's stride: read it here,
                     // after the block, so that it is not left to the commit
                     schedule_fence();
-                    issue_here(line2.stride);
+                    issue_line(line2);
                     if ((b.ok & (off <= off_full ? ~0ULL : 0ULL)) != active) break;
                     // commit, renderer.cpp:13499-13511
                     plain_common_stats(st, line.stride);
                     off = off2;
                     line = load_line(reinterpret_cast<const RungDev*>(base + off2 + line2.stride));
                     schedule_fence();
-                    const CommonBlock b2 = common_block(line2, b.nr, b.ni, b.sq, cr, ci, inv_cc, shift_hi);
+                    const CommonBlock b2 = common_block(line2, b.nr, b.ni, cr, ci, scale);
                     schedule_fence();
-                    issue_here(line.stride);
+                    issue_line(line);
                     if ((b2.ok & (off <= off_full ? ~0ULL : 0ULL)) != active) {
                         mr = b.nr;
                         mi = b.ni;
@@ -1340,7 +1420,6 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
                     }
                     mr = b2.nr;
                     mi = b2.ni;
-                    sq_c = b2.sq;
                     plain_common_stats(st, line2.stride);
                     off += line2.stride;
                 }
@@ -1432,6 +1511,7 @@ FAV_HD PlainExit run_plain_blocks(const RenderParams& p, PixelState& s, SC dc, C
         E += shift;
         sq = mr * mr + mi * mi;
         inv_cc = 1.0 / (cr * cr + ci * ci);
+        gap_ok = gap_scale(inv_cc, scale);
         retried = true;
     }
     // At one iteration below the cap no block fits (lengths are at least 2),

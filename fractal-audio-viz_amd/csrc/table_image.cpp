@@ -3,6 +3,7 @@
 #include "table_image.h"
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <stdexcept>
 
@@ -102,6 +103,23 @@ void make_table_image(const ReferenceContext& c, TableImage& img) {
     flatten(c.alt.levels, img.alt, img.alt_dir);
 }
 
+// The high word of a double whose low word is 0 and whose value is at most
+// v (v >= 0 or -0.0: the bits are cut), or at least v (v >= 0: rounded up
+// through the carry into the high word, which stays a finite double for the
+// bounds here).
+static int32_t high_word_down(double v) {
+    uint64_t bits;
+    std::memcpy(&bits, &v, sizeof bits);
+    return static_cast<int32_t>(static_cast<uint32_t>(bits >> 32));
+}
+
+static int32_t high_word_up(double v) {
+    uint64_t bits;
+    std::memcpy(&bits, &v, sizeof bits);
+    const uint32_t hi = static_cast<uint32_t>(bits >> 32);
+    return static_cast<int32_t>((bits & 0xffffffffULL) != 0 ? hi + 1 : hi);
+}
+
 // The rung table of one linear tier for the level cap top_cap
 // (fav_device.h).
 void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, bool order3, std::vector<RungDev>& rungs) {
@@ -115,6 +133,9 @@ void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, bool or
     mark_linear_steps(flat);
     const int map_end = c.map_end;
     rungs.assign(n, RungDev{});
+    // the product-gap bounds of each rung (0 and 0 where slot 0 does not
+    // apply), for its own first block and the block that ends at it
+    std::vector<double> gap_lo(n, 0.0), gap_hi(n, 0.0);
     for (size_t r = 0; r < n; ++r) {
         RungDev& g = rungs[r];
         // past the level-0 steps (lookup start > size[0], renderer.cpp:
@@ -173,18 +194,22 @@ void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, bool or
             const long double b2 = static_cast<long double>(g.b_re) * g.b_re + static_cast<long double>(g.b_im) * g.b_im;
             if (a2 > 0 && b2 > 0) {
                 const long double ratio = b2 / a2;
-                g.gap_lo = static_cast<double>(std::max(ratio * 0x1p-116L, 0x1p-1000L));
-                g.gap_hi = static_cast<double>(std::min(ratio * 0x1p116L, 0x1p1000L));
+                gap_lo[r] = static_cast<double>(std::max(ratio * 0x1p-116L, 0x1p-600L));
+                gap_hi[r] = static_cast<double>(std::min(ratio * 0x1p116L, 0x1p600L));
             }
         }
+        g.gap_lo_own = high_word_up(gap_lo[r]);
+        g.gap_hi_own = high_word_down(gap_hi[r]);
     }
-    // the square of slot 0's end bound and the end rung's fast bound, once
-    // every small_d and fast_r2 is set
+    // the bounds at slot 0's end index, once every rung is set: the square
+    // of its negligible bound, its fast bound and its product gap
     for (size_t r = 1; r < n; ++r) {
         const size_t end = r + (size_t(1) << rungs[r].top);
         const double small = end < n ? rungs[end].small_d : 0.0;
         const double bound = std::min(small * small * (1.0 - 0x1p-48), end < n ? rungs[end].fast_r2 : 0.0);
-        rungs[r].end_r2 = bound >= 0x1p-900 && bound <= 0x1p200 ? bound : -0.0;
+        rungs[r].end_hi = high_word_down(bound >= 0x1p-900 && bound <= 0x1p200 ? bound : -0.0);
+        rungs[r].gap_lo_end = high_word_up(end < n ? gap_lo[end] : 0.0);
+        rungs[r].gap_hi_end = high_word_down(end < n ? gap_hi[end] : 0.0);
     }
 }
 
