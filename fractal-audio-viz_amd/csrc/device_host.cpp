@@ -147,6 +147,9 @@ DeviceTables::~DeviceTables() {
         for (auto& cap : tier)
             for (void* p : cap)
                 if (p) (void)hipFree(p);
+    for (auto& table : walk_block)
+        for (void* p : table)
+            if (p) (void)hipFree(p);
 }
 
 namespace {
@@ -343,6 +346,24 @@ void ensure_plain_tables(ReferenceContext& c, int tier, int cap, bool order3, co
     rungs = t.rungs[tier][cap][o];
 }
 
+// The walk of the pixel on the reference through `table` (linear tier 0 or
+// 1, cubic 2) at a block-length limit of floor(log2) `level`.
+void ensure_ref_walk(ReferenceContext& c, int table, int level, const RefWalkDev*& walk) {
+    std::lock_guard<std::mutex> lock(c.device_mutex);
+    DeviceTables& t = *c.device;
+    if (!t.walk_built[table][level]) {
+        std::vector<RefWalkDev> w;
+        make_ref_walk(c, table, 1 << level, w);
+        if (!w.empty()) {
+            LaunchState& s = state();
+            upload_vector(w, t.walk_block[table][level], t.walks[table][level], s.stream, t.bytes);
+            check(hipStreamSynchronize(s.stream), "walk table upload");
+        }
+        t.walk_built[table][level] = true;
+    }
+    walk = t.walks[table][level];
+}
+
 // One deep field render on the device, host part of renderer.cpp:12672-13583.
 // A point render's offsets in HBM: re then im (doubles), then exponents.
 // The host copies stay alive in `staging` until the caller synchronises.
@@ -398,6 +419,16 @@ RenderParams bind_field_tables(ReferenceContext& c, const PreparedRender& prep) 
     }
     p.cubic = t.cubic;
     p.cubic_dir = t.cubic_dir;
+    {
+        // the on-reference walk through the table of the approximation
+        // order; its lookup reads the length limit only through floor(log2)
+        const bool linear = p.approximation_order <= 1;
+        const int level = 31 - __builtin_clz(static_cast<unsigned>(linear ? p.max_linear_length : p.max_bla_length));
+        if (level < kMaxLevels) {
+            ensure_ref_walk(c, linear ? prep.linear_tier : 2, level, p.ref_walk);
+            p.ref_walk_limit = 1 << level;
+        }
+    }
     return p;
 }
 

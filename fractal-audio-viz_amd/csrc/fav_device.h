@@ -182,6 +182,23 @@ struct StateStore {
 };
 
 // Bytes of StateStore storage per pixel (9 doubles, 9 ints, 2 queue slots).
+// The walk of the pixel on the reference (delta = dc = 0, field_core.h
+// run_reference_blocks) through one BLA table at one block-length limit,
+// one record per block start r: the step the lookup picks there with
+// |delta|^2 = 0 when the limit is the full one, and what the walk reads of
+// it, so that a block of that walk costs one scalar read instead of a chain
+// of them.  length > 0 also says that the walk's other tests at r pass: no
+// glitch or escape flag at r, r below the map end, and an end index inside
+// the orbit without the endpoint flag.  length 0: the walk takes the
+// lookup's path at r (table_image.cpp make_ref_walk).
+struct alignas(64) RefWalkDev {
+    double a_re, a_im, b_re, b_im;  // the step's A and B (ScaledComplex mantissas)
+    int32_t a_e, b_e;
+    int32_t length;
+    int32_t pad[5];
+};
+static_assert(sizeof(RefWalkDev) == 64, "walk record is one cache line");
+
 constexpr int32_t kOrbitGlitchFlag = 1;
 constexpr int32_t kOrbitEscapeFlag = 2;
 constexpr int32_t kOrbitEndpointFlag = 4;  // |Z_n|^2 >= 3: a BLA block may not end here
@@ -318,6 +335,11 @@ struct RenderParams {
     // replayed and tail steps) added to work[pixel] by every kernel that
     // runs it (the caller zeroes it); null = no work map
     int32_t* work;
+    // the walk of the pixel on the reference through the table of the
+    // approximation order at the block-length limit ref_walk_limit (null:
+    // every block of that walk takes the lookup)
+    const RefWalkDev* ref_walk;
+    int32_t ref_walk_limit;
 };
 
 namespace direct { struct DirectParams; }

@@ -69,6 +69,12 @@ struct DeviceTables {
     void* rung_block[2][kMaxLevels][2] = {};
     const RungDev* rungs[2][kMaxLevels][2] = {};
     bool plain_built[2][kMaxLevels][2] = {};
+    // the walk of the pixel on the reference per table (linear tiers 0, 1;
+    // cubic 2) and floor(log2) of the block-length limit, built on first use
+    // (ensure_ref_walk); null until then
+    void* walk_block[3][kMaxLevels] = {};
+    const RefWalkDev* walks[3][kMaxLevels] = {};
+    bool walk_built[3][kMaxLevels] = {};
     const AltStepDev* alt = nullptr;
     const double* alt_margin_m = nullptr;  // inside orbit_block
     const int32_t* alt_margin_e = nullptr;

@@ -1596,35 +1596,57 @@ FAV_HD void run_reference_blocks(const RenderParams& p, PixelState& s, Counters<
     const bool linear = p.approximation_order <= 1;
     SC derivative = s.derivative;
     bool committed = false;
+    // The walk's record at r (p.ref_walk), read a block ahead: while the
+    // limit min(max length, remaining) has the walk's floor(log2), a record
+    // with length > 0 is the block the lookup below commits at r.
+    const RefWalkDev* const walk = p.ref_walk;
+    RefWalkDev w{};
+    if (walk != nullptr && r >= 0 && r < o.size) w = load_uniform(walk, r);
     for (;;) {
         if (iteration >= max_iter || r < 0 || r >= o.size) break;
-        if ((load_uniform(&o.fast[r].flags, 0) & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) break;
-        if (o.map_end > 1 && r >= o.map_end) break;
         const int remaining = max_iter - iteration;
-        int k, length;
+        int length;
         SC a, b;
-        if (linear) {
-            k = lookup_level_zero(p.linear, p.linear_dir, r,
-                                  p.max_linear_length < remaining ? p.max_linear_length : remaining);
-            if (k < 0) break;
-            const LinearStepDev step = load_uniform(p.linear, k);
-            length = step.length;
-            a = SC{step.a_re, step.a_im, step.a_e};
-            b = SC{step.b_re, step.b_im, step.b_e};
+        if (walk != nullptr && w.length > 0 && remaining >= p.ref_walk_limit) {
+            length = w.length;
+            a = SC{w.a_re, w.a_im, w.a_e};
+            b = SC{w.b_re, w.b_im, w.b_e};
+            if (r + length > max_iter) break;
         } else {
-            k = lookup_level_zero(p.cubic, p.cubic_dir, r, p.max_bla_length < remaining ? p.max_bla_length : remaining);
-            if (k < 0) break;
-            const CubicStepDev* step = p.cubic + k;
-            length = load_uniform(&step->length, 0);
-            if constexpr (Planes) {
-                a = SC{load_uniform(&step->re[0], 0), load_uniform(&step->im[0], 0), load_uniform(&step->e[0], 0)};
-                b = SC{load_uniform(&step->re[1], 0), load_uniform(&step->im[1], 0), load_uniform(&step->e[1], 0)};
+            if ((load_uniform(&o.fast[r].flags, 0) & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) break;
+            if (o.map_end > 1 && r >= o.map_end) break;
+            int k;
+            if (linear) {
+                k = lookup_level_zero(p.linear, p.linear_dir, r,
+                                      p.max_linear_length < remaining ? p.max_linear_length : remaining);
+                if (k < 0) break;
+                const LinearStepDev step = load_uniform(p.linear, k);
+                length = step.length;
+                a = SC{step.a_re, step.a_im, step.a_e};
+                b = SC{step.b_re, step.b_im, step.b_e};
+            } else {
+                k = lookup_level_zero(p.cubic, p.cubic_dir, r,
+                                      p.max_bla_length < remaining ? p.max_bla_length : remaining);
+                if (k < 0) break;
+                const CubicStepDev* step = p.cubic + k;
+                length = load_uniform(&step->length, 0);
+                if constexpr (Planes) {
+                    a = SC{load_uniform(&step->re[0], 0), load_uniform(&step->im[0], 0),
+                           load_uniform(&step->e[0], 0)};
+                    b = SC{load_uniform(&step->re[1], 0), load_uniform(&step->im[1], 0),
+                           load_uniform(&step->e[1], 0)};
+                }
             }
+            if (r + length > max_iter) break;
+            // the endpoint's loop-top norm is |Z|^2 there (renderer.cpp:13422)
+            if ((load_uniform(&o.fast[r + length].flags, 0) & kOrbitEndpointFlag) != 0) break;
         }
         const int end = r + length;
-        if (end > max_iter) break;
-        // the endpoint's loop-top norm is |Z|^2 there (renderer.cpp:13422)
-        if ((load_uniform(&o.fast[end].flags, 0) & kOrbitEndpointFlag) != 0) break;
+        if (walk != nullptr) {
+            // (an end past the orbit ends the walk at the loop top)
+            w = load_uniform(walk, end < o.size ? end : r);
+            schedule_fence();
+        }
         if constexpr (Planes) derivative = sc_add(sc_mul(a, derivative), b);
         // commit, renderer.cpp:13499-13511
         r = end;

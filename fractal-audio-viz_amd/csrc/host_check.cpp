@@ -91,6 +91,16 @@ extern "C" int fav_host_check_render(
         }
         p.cubic = img.cubic.data();
         p.cubic_dir = img.cubic_dir;
+        std::vector<RefWalkDev> walk;
+        {
+            const bool linear = p.approximation_order <= 1;
+            const int level = 31 - __builtin_clz(static_cast<unsigned>(linear ? p.max_linear_length : p.max_bla_length));
+            make_ref_walk(c, linear ? prep.linear_tier : 2, 1 << level, walk);
+            if (!walk.empty()) {
+                p.ref_walk = walk.data();
+                p.ref_walk_limit = 1 << level;
+            }
+        }
         p.x_off_m = prep.offset_m.data();
         p.y_off_m = prep.offset_m.data() + width;
         p.x_off_e = prep.offset_e.data();

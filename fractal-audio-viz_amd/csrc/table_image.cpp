@@ -213,6 +213,74 @@ void make_plain_tables(const ReferenceContext& c, int tier, int top_cap, bool or
     }
 }
 
+namespace {
+
+// lookup_level with |delta|^2 = 0 (field_core.h lookup_level_zero) over a
+// flattened table.
+template <typename Step>
+int lookup_zero(const std::vector<Step>& steps, const LevelDir& d, int start, int max_length) {
+    if (start <= 0 || max_length <= 0 || d.count == 0 || start > d.size[0]) return -1;
+    int h = 31 - __builtin_clz(static_cast<unsigned>(max_length));
+    if (h > d.count - 1) h = d.count - 1;
+    const int offset = start - 1;
+    for (int level = h; level >= 0; --level) {
+        if ((offset & ((1 << level) - 1)) != 0) continue;
+        const int index = offset >> level;
+        if (index >= d.size[level]) continue;
+        const int64_t k = d.offset[level] + index;
+        const Step& s = steps[static_cast<size_t>(k)];
+        if (s.length > 1 && std::isfinite(s.r2_m) && s.r2_m != 0.0) return static_cast<int>(k);
+    }
+    return -1;
+}
+
+void set_walk(RefWalkDev& w, const LinearStepDev& s) {
+    w.a_re = s.a_re; w.a_im = s.a_im; w.a_e = s.a_e;
+    w.b_re = s.b_re; w.b_im = s.b_im; w.b_e = s.b_e;
+}
+
+void set_walk(RefWalkDev& w, const CubicStepDev& s) {
+    w.a_re = s.re[0]; w.a_im = s.im[0]; w.a_e = s.e[0];
+    w.b_re = s.re[1]; w.b_im = s.im[1]; w.b_e = s.e[1];
+}
+
+template <typename Step>
+void fill_walk(const OrbitData& o, int map_end, const std::vector<Step>& steps, const LevelDir& d, int limit,
+               std::vector<RefWalkDev>& walk) {
+    const size_t n = o.flags.size();
+    walk.assign(n, RefWalkDev{});
+    for (size_t r = 0; r < n; ++r) {
+        if ((o.flags[r] & (kOrbitGlitchFlag | kOrbitEscapeFlag)) != 0) continue;
+        if (map_end > 1 && r >= static_cast<size_t>(map_end)) continue;
+        const int k = lookup_zero(steps, d, static_cast<int>(r), limit);
+        if (k < 0) continue;
+        const Step& s = steps[static_cast<size_t>(k)];
+        const size_t end = r + static_cast<size_t>(s.length);
+        if (end >= n || (o.flags[end] & kOrbitEndpointFlag) != 0) continue;
+        set_walk(walk[r], s);
+        walk[r].length = s.length;
+    }
+}
+
+}  // namespace
+
+void make_ref_walk(const ReferenceContext& c, int table, int limit, std::vector<RefWalkDev>& walk) {
+    walk.clear();
+    const OrbitData& o = *c.orbit;
+    if (o.flags.size() != o.scaled.size() || limit <= 0) return;
+    if (table == 2) {
+        std::vector<CubicStepDev> flat;
+        LevelDir dir{};
+        flatten(c.cubic.levels, flat, dir);
+        fill_walk(o, c.map_end, flat, dir, limit, walk);
+    } else {
+        std::vector<LinearStepDev> flat;
+        LevelDir dir{};
+        flatten(c.linear[table].levels, flat, dir);
+        fill_walk(o, c.map_end, flat, dir, limit, walk);
+    }
+}
+
 void prepare_alternate(const ReferenceContext& c, const FieldRequest& r, PreparedAlternate& out) {
     const int w = r.width, h = r.height;
     const FractalRenderOptions& opt = r.options;

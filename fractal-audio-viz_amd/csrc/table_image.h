@@ -34,6 +34,11 @@ void make_table_image(const ReferenceContext& context, TableImage& image);
 void make_plain_tables(const ReferenceContext& context, int tier, int top_cap, bool order3,
                        std::vector<RungDev>& rungs);
 
+// The walk of the pixel on the reference (fav_device.h RefWalkDev) through
+// linear tier `table` (0 or 1) or the cubic table (2) at block-length limit
+// `limit`; empty when the orbit has no per-index records.
+void make_ref_walk(const ReferenceContext& context, int table, int limit, std::vector<RefWalkDev>& walk);
+
 // Per-render parameters: everything in RenderParams except the table, output
 // and scratch pointers, plus the per-column / per-row FloatExp offsets
 // (width entries, then height entries).
