@@ -1448,6 +1448,8 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
     std::vector<PreparedRender> preps(fields.size());
     std::vector<direct::DirectSetup> directs(fields.size());
     std::vector<const RungDev*> rungs(fields.size(), nullptr);
+    std::vector<const RefWalkDev*> walks(fields.size(), nullptr);
+    std::vector<int> walk_limits(fields.size(), 0);
     size_t pixels_total = 0, offsets_total = 0, state_total = 0, coords_total = 0;
     bool any_direct = false;
     for (size_t f = 0; f < fields.size(); ++f) {
@@ -1472,6 +1474,17 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
             const int cap = 31 - __builtin_clz(static_cast<unsigned>(preps[f].params.max_linear_length));
             ensure_plain_tables(*fields[f].context, preps[f].linear_tier, cap, preps[f].params.approximation_order >= 3,
                                 rungs[f]);
+            {
+                // the on-reference walk, as in render_field
+                const RenderParams& q = preps[f].params;
+                const bool linear = q.approximation_order <= 1;
+                const int level =
+                    31 - __builtin_clz(static_cast<unsigned>(linear ? q.max_linear_length : q.max_bla_length));
+                if (level < kMaxLevels) {
+                    ensure_ref_walk(*fields[f].context, linear ? preps[f].linear_tier : 2, level, walks[f]);
+                    walk_limits[f] = 1 << level;
+                }
+            }
             offsets_total += (preps[f].offset_m.size() * 12 + 512 + 255) & ~size_t(255);
             state_total += (pixels * kStateBytesPerPixel + 20 * 256 + 255) & ~size_t(255);
         }
@@ -1525,6 +1538,8 @@ std::shared_ptr<PreparedBatch> prepare_batch(std::vector<BatchFieldRequest> fiel
         p.plain_top = 31 - __builtin_clz(static_cast<unsigned>(p.max_linear_length));
         p.plain_pass = p.rungs != nullptr && p.fast_linear ? 1 : 0;
         p.reference_pixel = -1;
+        p.ref_walk = walks[f];
+        p.ref_walk_limit = walk_limits[f];
         char* doff = off_base + off_at;
         const size_t exp_offset = (prep.offset_m.size() * 8 + 255) & ~size_t(255);
         check(hipMemcpyAsync(doff, prep.offset_m.data(), prep.offset_m.size() * 8, hipMemcpyHostToDevice, s.stream),
