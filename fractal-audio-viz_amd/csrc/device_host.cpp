@@ -393,6 +393,63 @@ void upload_points(LaunchState& s, const FieldRequest& r, PointStaging& staging,
 
 namespace {
 
+// Which rung table and on-reference walk a field render binds: the linear
+// tier, the level cap (floor(log2) of the linear length limit) and the
+// order-3 flag of the rung table; the walk's table (linear tier, or 2 for
+// the cubic table) and floor(log2) of its length limit.
+struct FieldTableKeys {
+    int tier, cap, order3, walk_table, walk_level;
+};
+
+FieldTableKeys field_table_keys(const PreparedRender& prep) {
+    const RenderParams& q = prep.params;
+    const bool linear = q.approximation_order <= 1;
+    return FieldTableKeys{prep.linear_tier, 31 - __builtin_clz(static_cast<unsigned>(q.max_linear_length)),
+                          q.approximation_order >= 3 ? 1 : 0, linear ? prep.linear_tier : 2,
+                          31 - __builtin_clz(static_cast<unsigned>(linear ? q.max_linear_length : q.max_bla_length))};
+}
+
+// The host images of a field's rung table and walk, built off the render
+// thread (render_segment's workers) and uploaded by install_prebuilt.
+struct PrebuiltTables {
+    FieldTableKeys keys{};
+    std::vector<RungDev> rungs;
+    std::vector<RefWalkDev> walk;
+};
+
+void prebuild_field_tables(const ReferenceContext& c, const PreparedRender& prep, PrebuiltTables& out) {
+    out.keys = field_table_keys(prep);
+    make_plain_tables(c, out.keys.tier, out.keys.cap, out.keys.order3 != 0, out.rungs);
+    if (out.keys.walk_level < kMaxLevels) make_ref_walk(c, out.keys.walk_table, 1 << out.keys.walk_level, out.walk);
+}
+
+// Uploads prebuilt tables the reference does not hold yet (its device tables
+// must exist), so that the render's ensure_* calls find them built.
+void install_prebuilt(ReferenceContext& c, const PrebuiltTables& pre) {
+    std::lock_guard<std::mutex> lock(c.device_mutex);
+    DeviceTables& t = *c.device;
+    LaunchState& s = state();
+    const FieldTableKeys& k = pre.keys;
+    bool uploaded = false;
+    if (!t.plain_built[k.tier][k.cap][k.order3]) {
+        if (!pre.rungs.empty()) {
+            upload_vector(pre.rungs, t.rung_block[k.tier][k.cap][k.order3], t.rungs[k.tier][k.cap][k.order3],
+                          s.stream, t.bytes);
+            uploaded = true;
+        }
+        t.plain_built[k.tier][k.cap][k.order3] = true;
+    }
+    if (k.walk_level < kMaxLevels && !t.walk_built[k.walk_table][k.walk_level]) {
+        if (!pre.walk.empty()) {
+            upload_vector(pre.walk, t.walk_block[k.walk_table][k.walk_level], t.walks[k.walk_table][k.walk_level],
+                          s.stream, t.bytes);
+            uploaded = true;
+        }
+        t.walk_built[k.walk_table][k.walk_level] = true;
+    }
+    if (uploaded) check(hipStreamSynchronize(s.stream), "prebuilt table upload");
+}
+
 // Launch parameters of one Mandelbrot field: the per-frame values of `prep`
 // with the reference's device tables bound (uploaded first, with the
 // plain-double rung tables of the selected tier on first use).
@@ -1725,6 +1782,7 @@ struct SegmentSlot {
     std::unique_ptr<ReferenceContext> context;
     HostTables host;
     StagedTables staged;
+    PrebuiltTables tables;
     bool direct = false;
     long double direct_zoom = 0.0L;
 };
@@ -1766,10 +1824,21 @@ void render_segment(const std::vector<SegmentField>& fields, const FractalRender
     std::atomic<size_t> next{0};
     std::atomic<bool> stop{false};
     uint64_t build_ns = 0, built_ns = 0;
+    // References are built longest first (cap x precision, the MPFR orbit's
+    // cost), so that the deep fields at the end of a segment do not start
+    // last and leave the pool idle but for them; each is rendered as it is
+    // ready, so the order changes no result.
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        return static_cast<double>(fields[a].max_iter) * fields[a].precision_bits
+            > static_cast<double>(fields[b].max_iter) * fields[b].precision_bits;
+    });
     auto work = [&]() {
         for (;;) {
-            const size_t i = next.fetch_add(1);
-            if (i >= n || stop.load()) return;
+            const size_t at = next.fetch_add(1);
+            if (at >= n || stop.load()) return;
+            const size_t i = order[at];
             try {
                 const uint64_t t0 = now_ns();
                 const SegmentField& f = fields[i];
@@ -1782,6 +1851,11 @@ void render_segment(const std::vector<SegmentField>& fields, const FractalRender
                 } else {
                     build_host_tables(*slot.context, slot.host);
                     stage_tables(slot.host, slot.staged);
+                    const FieldRequest request{f.output, f.width, f.height, f.zoom.c_str(), f.max_iter,
+                                               f.series_order, f.series_block, o, nullptr, nullptr, true};
+                    PreparedRender prep;
+                    prepare_render(*slot.context, request, prep);
+                    prebuild_field_tables(*slot.context, prep, slot.tables);
                 }
                 const uint64_t t1 = now_ns();
                 std::lock_guard<std::mutex> lock(m);
@@ -1822,6 +1896,7 @@ void render_segment(const std::vector<SegmentField>& fields, const FractalRender
                 render_direct_on_device(d.params, d.x.data(), d.y.data(), false, true);
             } else {
                 upload_prepared_tables(*slot.context, slot.host, slot.staged);
+                install_prebuilt(*slot.context, slot.tables);
                 const FieldRequest request{f.output, f.width, f.height, f.zoom.c_str(), f.max_iter, f.series_order,
                                            f.series_block, o, nullptr, nullptr, true};
                 render_field(*slot.context, request);
@@ -1830,6 +1905,7 @@ void render_segment(const std::vector<SegmentField>& fields, const FractalRender
             // the render has synchronised: the reference, its tables and the staging go
             slot.staged.release();
             slot.host = HostTables{};
+            slot.tables = PrebuiltTables{};
             slot.context.reset();
         }
     } catch (...) {
