@@ -5,6 +5,18 @@
 
 #include "fx_core.h"
 
+// Occupancy targets (waves per SIMD), measured on the 2560x1440
+// near-nucleus fields: the Mandelbrot kernel is left to the compiler (95
+// registers, 5 waves; 6 and 8 waves spill and run 3-29% slower); the
+// alternate-formula kernel at 4 waves (128 registers, from 166 and 3 waves)
+// renders the burning-ship fields 12-14% faster, at 5 waves 5-15%.
+#ifndef FAV_FX_WAVES
+#define FAV_FX_WAVES 1
+#endif
+#ifndef FAV_FX_ALT_WAVES
+#define FAV_FX_ALT_WAVES 4
+#endif
+
 namespace fav {
 namespace {
 
@@ -15,13 +27,15 @@ __device__ __forceinline__ void tile_pixel(int& px, int& py) {
     py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
 }
 
-__global__ void __launch_bounds__(256) fx_kernel(RenderParams p, fx::FxOut out) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAV_FX_WAVES, 8)))
+fx_kernel(RenderParams p, fx::FxOut out) {
     int px, py;
     tile_pixel(px, py);
     if (px < p.width && py < p.height) fx::pixel(p, out, px, py);
 }
 
-__global__ void __launch_bounds__(256) fx_alt_kernel(AltRenderParams p, fx::FxOut out) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAV_FX_ALT_WAVES, 8)))
+fx_alt_kernel(AltRenderParams p, fx::FxOut out) {
     int px, py;
     tile_pixel(px, py);
     if (px < p.width && py < p.height) fx::alt_pixel(p, out, px, py);
