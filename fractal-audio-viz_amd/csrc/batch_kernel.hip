@@ -215,7 +215,14 @@ __global__ void __launch_bounds__(256) batch_tail_kernel(BatchLaunch q) {
 // FP64 plain-double tail where the FP64 path enters it), each shallow tile
 // through the direct renderer.  No hand-off queues: the floatexp pixel runs
 // its whole loop in one kernel.
-__global__ void __launch_bounds__(256) batch_fx_kernel(BatchLaunch q) {
+// 5 waves per SIMD (102 registers, from the compiler's 138 and 3 waves):
+// the 3840x2160 zoom-1e100 floatexp batch 77.7 -> 55.3 ms; 4 waves 62.8,
+// 6 and 8 waves within 2% of 5.
+#ifndef FAV_BATCH_FX_WAVES
+#define FAV_BATCH_FX_WAVES 5
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAV_BATCH_FX_WAVES, 8)))
+batch_fx_kernel(BatchLaunch q) {
     __shared__ RenderParams sp;
     __shared__ unsigned slot;
     int staged = -1;
